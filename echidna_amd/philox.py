@@ -1,0 +1,105 @@
+"""Philox4x32-10 counter-based generator (host side, numpy).
+
+Algorithm: Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
+1, 2, 3" (SC'11), Philox4x32 with 10 rounds: multipliers 0xD2511F53 and
+0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85.  The device copy
+in ``csrc/philox.h`` implements the same function, so points, weights and
+STDE directions drawn on either side agree bit for bit in their u32 words.
+
+Streams used by this package (key = (seed, stream)):
+  counter = (index_lo, index_hi, 0, 0); one Philox block per scalar draw.
+Uniform in (0, 1): 53-bit mantissa from words 0,1 with a half-ulp offset.
+Gaussian: Box-Muller in binary64 from two such uniforms (words 0,1 and 2,3),
+keeping the cosine branch.  Rademacher: sign from bit 0 of word 0.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+M0 = np.uint64(0xD2511F53)
+M1 = np.uint64(0xCD9E8D57)
+W0 = np.uint32(0x9E3779B9)
+W1 = np.uint32(0xBB67AE85)
+MASK = np.uint64(0xFFFFFFFF)
+
+# Stream tags for the synthetic workloads.
+STREAM_POINTS = 1
+STREAM_W1 = 2
+STREAM_B1 = 3
+STREAM_W2 = 4
+STREAM_B2 = 5
+STREAM_W3 = 6
+STREAM_DIRS = 7
+STREAM_DAG = 8
+STREAM_CV = 9
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Vectorised Philox4x32-10. All inputs broadcastable uint32 arrays."""
+    c0 = np.asarray(c0, dtype=np.uint32).astype(np.uint64)
+    c1 = np.asarray(c1, dtype=np.uint32).astype(np.uint64)
+    c2 = np.asarray(c2, dtype=np.uint32).astype(np.uint64)
+    c3 = np.asarray(c3, dtype=np.uint32).astype(np.uint64)
+    k0 = np.asarray(k0, dtype=np.uint32).astype(np.uint64)
+    k1 = np.asarray(k1, dtype=np.uint32).astype(np.uint64)
+    for r in range(10):
+        p0 = M0 * c0
+        p1 = M1 * c2
+        hi0, lo0 = p0 >> np.uint64(32), p0 & MASK
+        hi1, lo1 = p1 >> np.uint64(32), p1 & MASK
+        n0 = hi1 ^ c1 ^ k0
+        n1 = lo1
+        n2 = hi0 ^ c3 ^ k1
+        n3 = lo0
+        c0, c1, c2, c3 = n0, n1, n2, n3
+        if r < 9:
+            k0 = (k0 + np.uint64(W0)) & MASK
+            k1 = (k1 + np.uint64(W1)) & MASK
+    return (c0.astype(np.uint32), c1.astype(np.uint32),
+            c2.astype(np.uint32), c3.astype(np.uint32))
+
+
+def _words(seed: int, stream: int, idx: np.ndarray):
+    idx = np.asarray(idx, dtype=np.uint64)
+    lo = (idx & MASK).astype(np.uint32)
+    hi = (idx >> np.uint64(32)).astype(np.uint32)
+    z = np.zeros_like(lo)
+    return philox4x32_10(lo, hi, z, z, np.uint32(seed & 0xFFFFFFFF), np.uint32(stream & 0xFFFFFFFF))
+
+
+_TWO_M53 = 1.0 / 9007199254740992.0  # 2^-53
+
+
+def _u53(wa, wb):
+    """Uniform in (0,1) from two u32 words: 53 random bits + half-ulp offset."""
+    hi = wa.astype(np.uint64) >> np.uint64(6)  # 26 bits
+    lo = wb.astype(np.uint64) >> np.uint64(5)  # 27 bits
+    m = (hi << np.uint64(27)) | lo
+    return (m.astype(np.float64) + 0.5) * _TWO_M53
+
+
+def uniform(seed: int, stream: int, idx) -> np.ndarray:
+    w0, w1, _, _ = _words(seed, stream, idx)
+    return _u53(w0, w1)
+
+
+def uniform_range(seed: int, stream: int, idx, lo: float, hi: float) -> np.ndarray:
+    return lo + (hi - lo) * uniform(seed, stream, idx)
+
+
+def normal(seed: int, stream: int, idx) -> np.ndarray:
+    w0, w1, w2, w3 = _words(seed, stream, idx)
+    u1 = _u53(w0, w1)
+    u2 = _u53(w2, w3)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def rademacher(seed: int, stream: int, idx) -> np.ndarray:
+    w0, _, _, _ = _words(seed, stream, idx)
+    return np.where((w0 & np.uint32(1)) == 0, 1.0, -1.0)
+
+
+def normal_matrix(seed: int, stream: int, rows: int, cols: int, std: float = 1.0) -> np.ndarray:
+    idx = np.arange(rows * cols, dtype=np.uint64)
+    return (std * normal(seed, stream, idx)).reshape(rows, cols)
