@@ -1,0 +1,307 @@
+"""Python mirror of the reference's ``GpuBackend`` over the HIP C ABI.
+
+``HipContext`` plays the role of the reference's ``CudaContext``
+(``src/gpu/cuda_backend.rs:477-953``): ``new()`` acquires the device,
+``upload_tape`` validates and uploads a ``GpuTapeData`` (raising like the
+trait's panic on invalid data), and the batch methods take and return numpy
+arrays in the reference's row-major layouts.  f32 and f64 tapes share the
+methods; the ``*_f64`` names exist as aliases for readers of the reference.
+
+Every call goes through ``libechidna_hip.so``; there is no CPU fallback.  If
+the library is missing, importing this module raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .tape import GpuError, GpuTapeData
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libechidna_hip.so")
+
+EHIP_F32 = 0
+EHIP_F64 = 1
+
+_c_u32p = ctypes.POINTER(ctypes.c_uint32)
+_c_u64p = ctypes.POINTER(ctypes.c_uint64)
+_c_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+# Every symbol the C header declares, with its ctypes signature.
+_SIGS = {
+    "ehip_last_error": (ctypes.c_char_p, []),
+    "ehip_version": (ctypes.c_char_p, []),
+    "ehip_ctx_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "ehip_ctx_destroy": (None, [_vp]),
+    "ehip_tape_upload": (ctypes.c_int32, [_vp, ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _vp, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _c_u32p,
+                                          ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "ehip_tape_free": (None, [_vp]),
+    "ehip_num_outputs": (ctypes.c_uint32, [_vp]),
+    "ehip_num_inputs": (ctypes.c_uint32, [_vp]),
+    "ehip_tape_dtype": (ctypes.c_int32, [_vp]),
+    "ehip_forward_batch": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, _vp]),
+    "ehip_gradient_batch": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "ehip_hvp_batch": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "ehip_hvp_batch_points": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "ehip_taylor_kth_batch": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp]),
+    "ehip_forward_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
+    "ehip_gradient_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp]),
+    "ehip_hvp_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _vp, _vp, _vp]),
+    "ehip_taylor_kth_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
+                                                   ctypes.c_uint32, _vp, _vp]),
+    "ehip_program_info": (ctypes.c_int32, [_vp, ctypes.c_int32, _c_u64p]),
+    "ehip_compile_program": (ctypes.c_int32, [ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _c_f64p, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32,
+                                              ctypes.c_int32, ctypes.c_int32, _c_u32p, ctypes.c_uint64, _c_u32p,
+                                              ctypes.c_uint64, _c_u64p]),
+    "ehip_set_launch": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
+    "ehip_set_fusion": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: run echidna_amd/build.py (or __graft_entry__.build()) first")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> List[str]:
+    return list(_SIGS)
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load_library().ehip_last_error().decode()
+        raise GpuError(rc, msg)
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class TaylorBatchResult:
+    """``TaylorBatchResult`` (src/gpu/mod.rs:234-241)."""
+
+    values: np.ndarray
+    c1s: np.ndarray
+    c2s: np.ndarray
+
+
+@dataclass
+class TaylorKthBatchResult:
+    """``TaylorKthBatchResult`` (src/gpu/mod.rs:266-272)."""
+
+    coefficients: List[np.ndarray]
+    order: int
+
+
+def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Optional[bool] = None):
+    """Compile without a device; returns (words[n,8], terms[t,4], info dict)."""
+    lib = load_library()
+    f64 = (np.asarray(data.constants).dtype == np.float64) if f64 is None else f64
+    consts = np.ascontiguousarray(data.constants, dtype=np.float64)
+    outs = _u32(data.output_indices)
+    info = (ctypes.c_uint64 * 4)()
+    args = [EHIP_F64 if f64 else EHIP_F32, _u32(data.opcodes), _u32(data.arg0), _u32(data.arg1), consts]
+    ptrs = [args[0]] + [a.ctypes.data_as(_c_u32p) for a in args[1:4]] + [consts.ctypes.data_as(_c_f64p)]
+    rc = lib.ehip_compile_program(*ptrs, data.num_ops, data.num_inputs, data.output_index,
+                                  outs.ctypes.data_as(_c_u32p), len(outs), mode, int(fuse),
+                                  None, 0, None, 0, info)
+    nw, nt = int(info[2]), int(info[3])
+    words = np.zeros(max(nw, 8), dtype=np.uint32)
+    terms = np.zeros(max(nt * 4, 4), dtype=np.uint32)
+    rc = lib.ehip_compile_program(*ptrs, data.num_ops, data.num_inputs, data.output_index,
+                                  outs.ctypes.data_as(_c_u32p), len(outs), mode, int(fuse),
+                                  words.ctypes.data_as(_c_u32p), len(words), terms.ctypes.data_as(_c_u32p),
+                                  len(terms), info)
+    _check(rc)
+    return (words[:nw].reshape(-1, 8), terms[: nt * 4].reshape(-1, 4),
+            {"instructions": int(info[0]), "registers": int(info[1]), "words": nw, "dot_terms": nt})
+
+
+class HipTapeBuffers:
+    """Uploaded tape handle (``CudaTapeBuffers``, src/gpu/cuda_backend.rs:449-472)."""
+
+    def __init__(self, ctx: "HipContext", handle, f64: bool, num_inputs: int, num_outputs: int):
+        self._ctx = ctx
+        self.handle = handle
+        self.f64 = f64
+        self.num_inputs = num_inputs
+        self.num_outputs = num_outputs
+        self.dtype = np.float64 if f64 else np.float32
+
+    def program_info(self, mode: int = 1) -> dict:
+        out = (ctypes.c_uint64 * 4)()
+        _check(load_library().ehip_program_info(self.handle, mode, out))
+        return {"instructions": int(out[0]), "registers": int(out[1]), "words": int(out[2]),
+                "dot_terms": int(out[3])}
+
+    def free(self) -> None:
+        if self.handle:
+            load_library().ehip_tape_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HipContext:
+    """MI355X backend context (``CudaContext``, src/gpu/cuda_backend.rs:477-953)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @staticmethod
+    def new(device: int = 0) -> Optional["HipContext"]:
+        """``CudaContext::new``: ``None`` when no gfx950 device is present."""
+        lib = load_library()
+        h = ctypes.c_void_p()
+        rc = lib.ehip_ctx_create(device, ctypes.byref(h))
+        if rc == GpuError.NO_DEVICE:
+            return None
+        _check(rc)
+        return HipContext(h)
+
+    def close(self) -> None:
+        if self.handle:
+            load_library().ehip_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_launch(self, wg: int = 0, lds_budget: int = 0) -> None:
+        _check(load_library().ehip_set_launch(self.handle, wg, lds_budget))
+
+    def set_fusion(self, enable: bool) -> None:
+        _check(load_library().ehip_set_fusion(self.handle, int(enable)))
+
+    # -- upload --
+    def upload_tape(self, data: GpuTapeData, f64: Optional[bool] = None) -> HipTapeBuffers:
+        """``upload_tape`` / ``upload_tape_f64``; invalid data raises GpuError."""
+        lib = load_library()
+        f64 = (np.asarray(data.constants).dtype == np.float64) if f64 is None else f64
+        consts = np.ascontiguousarray(data.constants, dtype=np.float64 if f64 else np.float32)
+        op, a0, a1 = _u32(data.opcodes), _u32(data.arg0), _u32(data.arg1)
+        outs = _u32(data.output_indices)
+        h = ctypes.c_void_p()
+        rc = lib.ehip_tape_upload(self.handle, EHIP_F64 if f64 else EHIP_F32, op.ctypes.data_as(_c_u32p),
+                                  a0.ctypes.data_as(_c_u32p), a1.ctypes.data_as(_c_u32p), _ptr(consts),
+                                  data.num_ops, data.num_inputs, data.num_variables, data.output_index,
+                                  outs.ctypes.data_as(_c_u32p), len(outs), ctypes.byref(h))
+        _check(rc)
+        return HipTapeBuffers(self, h, f64, int(lib.ehip_num_inputs(h)), int(lib.ehip_num_outputs(h)))
+
+    upload_tape_f64 = upload_tape
+
+    def num_outputs(self, tape: HipTapeBuffers) -> int:
+        return int(load_library().ehip_num_outputs(tape.handle))
+
+    # -- host-array entry points (reference semantics) --
+    def _arr(self, tape: HipTapeBuffers, a, shape) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=tape.dtype)
+        if a.size != int(np.prod(shape)):
+            raise ValueError(f"expected {int(np.prod(shape))} values, got {a.size}")
+        return a
+
+    def forward_batch(self, tape: HipTapeBuffers, inputs, batch_size: int) -> np.ndarray:
+        x = self._arr(tape, inputs, (batch_size, tape.num_inputs))
+        y = np.zeros(batch_size * tape.num_outputs, dtype=tape.dtype)
+        _check(load_library().ehip_forward_batch(self.handle, tape.handle, _ptr(x), batch_size, _ptr(y)))
+        return y
+
+    def gradient_batch(self, tape: HipTapeBuffers, inputs, batch_size: int):
+        x = self._arr(tape, inputs, (batch_size, tape.num_inputs))
+        y = np.zeros(batch_size * tape.num_outputs, dtype=tape.dtype)
+        g = np.zeros(batch_size * tape.num_inputs, dtype=tape.dtype)
+        _check(load_library().ehip_gradient_batch(self.handle, tape.handle, _ptr(x), batch_size, _ptr(y), _ptr(g)))
+        return y, g
+
+    def hvp_batch(self, tape: HipTapeBuffers, x, tangent_dirs, batch_size: int):
+        xv = self._arr(tape, x, (tape.num_inputs,))
+        v = self._arr(tape, tangent_dirs, (batch_size, tape.num_inputs))
+        g = np.zeros(batch_size * tape.num_inputs, dtype=tape.dtype)
+        h = np.zeros(batch_size * tape.num_inputs, dtype=tape.dtype)
+        _check(load_library().ehip_hvp_batch(self.handle, tape.handle, _ptr(xv), _ptr(v), batch_size, _ptr(g), _ptr(h)))
+        return g, h
+
+    def hvp_batch_points(self, tape: HipTapeBuffers, xs, tangent_dirs, batch_size: int):
+        xv = self._arr(tape, xs, (batch_size, tape.num_inputs))
+        v = self._arr(tape, tangent_dirs, (batch_size, tape.num_inputs))
+        g = np.zeros(batch_size * tape.num_inputs, dtype=tape.dtype)
+        h = np.zeros(batch_size * tape.num_inputs, dtype=tape.dtype)
+        _check(load_library().ehip_hvp_batch_points(self.handle, tape.handle, _ptr(xv), _ptr(v), batch_size,
+                                                     _ptr(g), _ptr(h)))
+        return g, h
+
+    def taylor_forward_kth_batch(self, tape: HipTapeBuffers, primal_inputs, direction_seeds, batch_size: int,
+                                 order: int) -> TaylorKthBatchResult:
+        x = self._arr(tape, primal_inputs, (batch_size, tape.num_inputs))
+        v = self._arr(tape, direction_seeds, (batch_size, tape.num_inputs))
+        total = batch_size * tape.num_outputs
+        c = np.zeros(max(order, 1) * total, dtype=tape.dtype)
+        _check(load_library().ehip_taylor_kth_batch(self.handle, tape.handle, _ptr(x), _ptr(v), batch_size, order,
+                                                     _ptr(c)))
+        return TaylorKthBatchResult([c[k * total:(k + 1) * total] for k in range(order)], order)
+
+    def taylor_forward_2nd_batch(self, tape, primal_inputs, direction_seeds, batch_size: int) -> TaylorBatchResult:
+        """Default trait method: delegates to order 3 (src/gpu/mod.rs:189-204)."""
+        r = self.taylor_forward_kth_batch(tape, primal_inputs, direction_seeds, batch_size, 3)
+        return TaylorBatchResult(*r.coefficients)
+
+    forward_batch_f64 = forward_batch
+    gradient_batch_f64 = gradient_batch
+    hvp_batch_f64 = hvp_batch
+    taylor_forward_kth_batch_f64 = taylor_forward_kth_batch
+    taylor_forward_2nd_batch_f64 = taylor_forward_2nd_batch
+
+    # -- device-pointer entry points (torch tensors already in HBM) --
+    def forward_batch_dev(self, tape, d_inputs, batch: int, d_outputs, stream: int = 0) -> None:
+        _check(load_library().ehip_forward_batch_dev(self.handle, tape.handle, _vp(d_inputs.data_ptr()), batch,
+                                                      _vp(d_outputs.data_ptr()), _vp(stream or None)))
+
+    def gradient_batch_dev(self, tape, d_inputs, batch: int, d_outputs, d_grads, stream: int = 0) -> None:
+        _check(load_library().ehip_gradient_batch_dev(self.handle, tape.handle, _vp(d_inputs.data_ptr()), batch,
+                                                       _vp(d_outputs.data_ptr()), _vp(d_grads.data_ptr()),
+                                                       _vp(stream or None)))
+
+    def hvp_batch_dev(self, tape, d_x, x_stride: int, d_dirs, batch: int, d_grads, d_hvps, stream: int = 0) -> None:
+        _check(load_library().ehip_hvp_batch_dev(self.handle, tape.handle, _vp(d_x.data_ptr()), x_stride,
+                                                  _vp(d_dirs.data_ptr()), batch, _vp(d_grads.data_ptr()),
+                                                  _vp(d_hvps.data_ptr()), _vp(stream or None)))
+
+    def taylor_kth_batch_dev(self, tape, d_x, x_stride: int, d_dirs, batch: int, order: int, d_coeffs,
+                             stream: int = 0) -> None:
+        _check(load_library().ehip_taylor_kth_batch_dev(self.handle, tape.handle, _vp(d_x.data_ptr()), x_stride,
+                                                         _vp(d_dirs.data_ptr()), batch, order,
+                                                         _vp(d_coeffs.data_ptr()), _vp(stream or None)))

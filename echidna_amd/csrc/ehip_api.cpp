@@ -1,0 +1,568 @@
+// Host side of the C ABI in include/echidna_hip.h.
+//
+// Mirrors the reference's CUDA backend host (src/gpu/cuda_backend.rs):
+// context acquisition, tape upload with GpuTapeData::validate
+// (src/gpu/mod.rs:397-564), validate_batch_args (src/gpu/mod.rs:572-582),
+// and the batched entry points.  Unlike the reference, a tape is compiled
+// once at upload into register programs (ehip_program.h) and the per-call
+// scratch lives in a grow-only context workspace instead of per-call
+// allocations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/echidna_hip.h"
+#include "ehip_kernels.h"
+#include "ehip_program.h"
+
+using namespace ehip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int32_t ok() {
+  g_last_error.clear();
+  return EHIP_OK;
+}
+
+int32_t hip_fail(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory) return fail(EHIP_OUT_OF_MEMORY, "GPU out of memory (%s)", what);
+  return fail(EHIP_OTHER, "GPU error: %s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                     \
+  do {                                          \
+    hipError_t _e = (expr);                     \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  hipError_t reserve(size_t bytes) {
+    if (bytes <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) n = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct ehip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;       // guards device state and knobs
+  std::mutex host_mu;  // serializes host-pointer calls (staging buffers)
+  DevBuf regs;                 // HBM register file
+  DevBuf io[6];                // staging for host-pointer entry points
+  uint32_t wg = 64;
+  uint32_t lds_budget = 32768;
+  bool fuse = true;
+  size_t max_lds = 65536;
+};
+
+struct DevProgram {
+  Program host;
+  uint32_t* d_words = nullptr;
+  uint32_t* d_terms = nullptr;
+};
+
+struct ehip_tape {
+  ehip_ctx* ctx = nullptr;
+  bool f64 = false;
+  uint32_t nin = 0, nout = 0;
+  TapeHost host;
+  DevProgram prog[2];
+};
+
+namespace {
+
+int32_t upload_program(DevProgram& dp) {
+  const Program& p = dp.host;
+  size_t wb = std::max<size_t>(p.words.size(), 8) * 4, tb = std::max<size_t>(p.terms.size(), 4) * 4;
+  HIP_TRY(hipMalloc((void**)&dp.d_words, wb), "program upload");
+  HIP_TRY(hipMalloc((void**)&dp.d_terms, tb), "program upload");
+  if (!p.words.empty()) HIP_TRY(hipMemcpy(dp.d_words, p.words.data(), p.words.size() * 4, hipMemcpyHostToDevice), "program upload");
+  if (!p.terms.empty()) HIP_TRY(hipMemcpy(dp.d_terms, p.terms.data(), p.terms.size() * 4, hipMemcpyHostToDevice), "program upload");
+  return EHIP_OK;
+}
+
+void free_program(DevProgram& dp) {
+  if (dp.d_words) (void)hipFree(dp.d_words);
+  if (dp.d_terms) (void)hipFree(dp.d_terms);
+  dp.d_words = dp.d_terms = nullptr;
+}
+
+// validate_batch_args (src/gpu/mod.rs:572-582)
+int32_t check_batch(const ehip_tape* t, uint64_t batch) {
+  if (batch == 0) return fail(EHIP_OTHER, "GPU error: batch_size must be non-zero");
+  if (t->nin == 0)
+    return fail(EHIP_OTHER, "GPU error: tape has no inputs (constant function); evaluate it on the CPU instead");
+  return EHIP_OK;
+}
+
+// Launch one interpreter over `L.B` lanes.  kind 0 scalar, 1 dual, 2 jet(K).
+int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s) {
+  const DevProgram& dp = t->prog[kind == 0 && L.g == nullptr ? PM_FORWARD : (kind == 2 ? PM_FORWARD : PM_GRADIENT)];
+  L.prog = dp.d_words;
+  L.terms = dp.d_terms;
+  L.ninstr = dp.host.ninstr;
+  const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
+  const size_t S = t->f64 ? 8 : 4;
+  const uint32_t wg = c->wg;
+  const size_t per_reg = (size_t)NC * wg * S;
+  size_t budget = std::min<size_t>(c->lds_budget, c->max_lds);
+  uint32_t R = dp.host.nregs;
+  uint32_t R_lds = (uint32_t)std::min<size_t>(R, budget / per_reg);
+  L.R_lds = R_lds;
+  size_t lds = (size_t)R_lds * per_reg;
+  uint64_t grid = (L.B + wg - 1) / wg;
+  uint64_t lanes = grid * wg;
+  if (R > R_lds) {
+    size_t bytes = (size_t)(R - R_lds) * NC * lanes * S;
+    HIP_TRY(c->regs.reserve(bytes), "register file");
+    L.gregs = c->regs.p;
+  } else {
+    L.gregs = nullptr;
+  }
+  hipError_t e;
+  if (kind == 0) e = launch_scalar(t->f64, L, wg, lds, s);
+  else if (kind == 1) e = launch_dual(t->f64, L, wg, lds, s);
+  else e = launch_jet(t->f64, K, L, wg, lds, s);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return EHIP_OK;
+}
+
+hipStream_t pick(ehip_ctx* c, void* stream) { return stream ? (hipStream_t)stream : c->stream; }
+
+}  // namespace
+
+extern "C" {
+
+const char* ehip_last_error(void) { return g_last_error.c_str(); }
+const char* ehip_version(void) { return "echidna-hip 0.1 (gfx950)"; }
+
+int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) {
+  if (!out) return fail(EHIP_OTHER, "GPU error: null output pointer");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(EHIP_NO_DEVICE, "no suitable GPU device found");
+  if (device < 0 || device >= n) return fail(EHIP_NO_DEVICE, "no suitable GPU device found (device %d of %d)", device, n);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(EHIP_NO_DEVICE, "no suitable GPU device found");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(EHIP_NO_DEVICE, "no suitable GPU device found (device %d is %s; this build targets gfx950)", device,
+                prop.gcnArchName);
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  ehip_ctx* c = new ehip_ctx();
+  c->device = device;
+  c->max_lds = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 65536;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "stream create");
+  }
+  *out = c;
+  return ok();
+}
+
+void ehip_ctx_destroy(ehip_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
+  c->regs.release();
+  for (auto& b : c->io) b.release();
+  delete c;
+}
+
+int32_t ehip_set_launch(ehip_ctx* c, uint32_t wg, uint32_t lds_budget) {
+  if (!c) return fail(EHIP_OTHER, "GPU error: null context");
+  if (wg == 0) wg = 64;
+  if (wg != 64 && wg != 128 && wg != 256) return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->wg = wg;
+  c->lds_budget = lds_budget ? lds_budget : 32768;
+  return ok();
+}
+
+int32_t ehip_set_fusion(ehip_ctx* c, int32_t enable) {
+  if (!c) return fail(EHIP_OTHER, "GPU error: null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->fuse = enable != 0;
+  return ok();
+}
+
+int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0,
+                         const uint32_t* arg1, const void* constants, uint32_t num_ops, uint32_t num_inputs,
+                         uint32_t num_variables, uint32_t output_index, const uint32_t* output_indices,
+                         uint32_t num_outputs, ehip_tape** out) {
+  if (!c || !out) return fail(EHIP_OTHER, "GPU error: null argument");
+  *out = nullptr;
+  if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "GPU error: dtype must be EHIP_F32 or EHIP_F64");
+  if (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants))
+    return fail(EHIP_OTHER, "GPU error: null tape array");
+  for (uint32_t i = 0; i < num_ops; ++i)
+    if (opcodes[i] == 43u) return fail(EHIP_CUSTOM_OPS_NOT_SUPPORTED, "tape contains custom ops which cannot run on GPU");
+  std::vector<uint32_t> outs;
+  if (num_outputs == 0) outs.push_back(output_index);
+  else outs.assign(output_indices, output_indices + num_outputs);
+  std::string msg = validate_tape(opcodes, arg0, arg1, num_ops, num_inputs, num_variables, num_variables,
+                                  output_index, outs.data(), (uint32_t)outs.size());
+  if (!msg.empty()) return fail(EHIP_OTHER, "%s", msg.c_str());
+  if (num_ops == 0) return fail(EHIP_OTHER, "invalid GpuTapeData: output_index (%u) >= num_variables (0)", output_index);
+  ehip_tape* t = new ehip_tape();
+  t->ctx = c;
+  t->f64 = dtype == EHIP_F64;
+  t->nin = num_inputs;
+  t->nout = (uint32_t)outs.size();
+  t->host.opcodes.assign(opcodes, opcodes + num_ops);
+  t->host.arg0.assign(arg0, arg0 + num_ops);
+  t->host.arg1.assign(arg1, arg1 + num_ops);
+  t->host.values.resize(num_ops);
+  if (t->f64) {
+    const double* v = (const double*)constants;
+    for (uint32_t i = 0; i < num_ops; ++i) t->host.values[i] = v[i];
+  } else {
+    const float* v = (const float*)constants;
+    for (uint32_t i = 0; i < num_ops; ++i) t->host.values[i] = (double)v[i];
+  }
+  t->host.num_inputs = num_inputs;
+  t->host.outputs = outs;
+  t->host.f64 = t->f64;
+  bool fuse;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    fuse = c->fuse;
+  }
+  t->prog[PM_FORWARD].host = compile_program(t->host, PM_FORWARD, fuse);
+  t->prog[PM_GRADIENT].host = compile_program(t->host, PM_GRADIENT, fuse);
+  (void)hipSetDevice(c->device);
+  for (int m = 0; m < 2; ++m) {
+    int32_t rc = upload_program(t->prog[m]);
+    if (rc != EHIP_OK) {
+      for (auto& p : t->prog) free_program(p);
+      delete t;
+      return rc;
+    }
+  }
+  *out = t;
+  return ok();
+}
+
+void ehip_tape_free(ehip_tape* t) {
+  if (!t) return;
+  for (auto& p : t->prog) free_program(p);
+  delete t;
+}
+
+uint32_t ehip_num_outputs(const ehip_tape* t) { return t ? t->nout : 0; }
+uint32_t ehip_num_inputs(const ehip_tape* t) { return t ? t->nin : 0; }
+int32_t ehip_tape_dtype(const ehip_tape* t) { return t ? (t->f64 ? EHIP_F64 : EHIP_F32) : -1; }
+
+int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) {
+  if (!t || !out4 || (mode != 0 && mode != 1)) return fail(EHIP_OTHER, "GPU error: bad argument");
+  const Program& p = t->prog[mode].host;
+  out4[0] = p.ninstr;
+  out4[1] = p.nregs;
+  out4[2] = p.words.size();
+  out4[3] = p.dot_terms;
+  return ok();
+}
+
+int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                             const double* constants, uint32_t num_ops, uint32_t num_inputs,
+                             uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,
+                             int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
+                             uint64_t terms_cap, uint64_t* out4) {
+  if (!out4 || (mode != 0 && mode != 1) || (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
+    return fail(EHIP_OTHER, "GPU error: bad argument");
+  for (uint32_t i = 0; i < num_ops; ++i)
+    if (opcodes[i] == 43u) return fail(EHIP_CUSTOM_OPS_NOT_SUPPORTED, "tape contains custom ops which cannot run on GPU");
+  std::vector<uint32_t> outs;
+  if (num_outputs == 0) outs.push_back(output_index);
+  else outs.assign(output_indices, output_indices + num_outputs);
+  std::string msg = validate_tape(opcodes, arg0, arg1, num_ops, num_inputs, num_ops, num_ops, output_index,
+                                  outs.data(), (uint32_t)outs.size());
+  if (!msg.empty()) return fail(EHIP_OTHER, "%s", msg.c_str());
+  if (num_ops == 0) return fail(EHIP_OTHER, "invalid GpuTapeData: empty tape");
+  TapeHost th;
+  th.opcodes.assign(opcodes, opcodes + num_ops);
+  th.arg0.assign(arg0, arg0 + num_ops);
+  th.arg1.assign(arg1, arg1 + num_ops);
+  th.values.assign(constants, constants + num_ops);
+  if (dtype == EHIP_F32)
+    for (auto& v : th.values) v = (double)(float)v;
+  th.num_inputs = num_inputs;
+  th.outputs = outs;
+  th.f64 = dtype == EHIP_F64;
+  Program p = compile_program(th, mode == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0);
+  out4[0] = p.ninstr;
+  out4[1] = p.nregs;
+  out4[2] = p.words.size();
+  out4[3] = p.terms.size() / TW;
+  if (p.words.size() > words_cap || p.terms.size() > terms_cap || !words || !terms)
+    return fail(EHIP_OTHER, "GPU error: program buffers too small");
+  std::memcpy(words, p.words.data(), p.words.size() * 4);
+  std::memcpy(terms, p.terms.data(), p.terms.size() * 4);
+  return ok();
+}
+
+// ---------------------------- device-pointer API ----------------------------
+
+int32_t ehip_forward_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_in, uint64_t batch, void* d_out,
+                               void* stream) {
+  if (!c || !t) return fail(EHIP_OTHER, "GPU error: null argument");
+  int32_t rc = check_batch(t, batch);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Launch L{};
+  L.nin = t->nin;
+  L.nout = t->nout;
+  L.B = batch;
+  L.x = d_in;
+  L.xstride = t->nin;
+  L.y = d_out;
+  rc = run(c, t, 0, 1, L, pick(c, stream));
+  return rc ? rc : ok();
+}
+
+int32_t ehip_gradient_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_in, uint64_t batch, void* d_out,
+                                void* d_grads, void* stream) {
+  if (!c || !t || !d_grads) return fail(EHIP_OTHER, "GPU error: null argument");
+  int32_t rc = check_batch(t, batch);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Launch L{};
+  L.nin = t->nin;
+  L.nout = t->nout;
+  L.B = batch;
+  L.x = d_in;
+  L.xstride = t->nin;
+  L.y = d_out;
+  L.g = d_grads;
+  rc = run(c, t, 0, 1, L, pick(c, stream));
+  return rc ? rc : ok();
+}
+
+int32_t ehip_hvp_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t x_stride, const void* d_dirs,
+                           uint64_t batch, void* d_grads, void* d_hvps, void* stream) {
+  if (!c || !t || !d_x || !d_dirs || !d_grads || !d_hvps) return fail(EHIP_OTHER, "GPU error: null argument");
+  int32_t rc = check_batch(t, batch);
+  if (rc) return rc;
+  if (t->nout != 1)
+    return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only; this tape has %u outputs", t->nout);
+  if (x_stride != 0 && x_stride != t->nin) return fail(EHIP_OTHER, "GPU error: x_stride must be 0 or num_inputs");
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Launch L{};
+  L.nin = t->nin;
+  L.nout = t->nout;
+  L.B = batch;
+  L.x = d_x;
+  L.xstride = x_stride;
+  L.dirs = d_dirs;
+  L.g = d_grads;
+  L.h = d_hvps;
+  rc = run(c, t, 1, 2, L, pick(c, stream));
+  return rc ? rc : ok();
+}
+
+int32_t ehip_taylor_kth_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t x_stride,
+                                  const void* d_dirs, uint64_t batch, uint32_t order, void* d_coeffs,
+                                  void* stream) {
+  if (!c || !t || !d_x || !d_dirs || !d_coeffs) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (order < 1 || order > 5)
+    return fail(EHIP_OTHER, "GPU error: taylor_forward_kth_batch supports order 1..=5, got %u", order);
+  int32_t rc = check_batch(t, batch);
+  if (rc) return rc;
+  if (x_stride != 0 && x_stride != t->nin) return fail(EHIP_OTHER, "GPU error: x_stride must be 0 or num_inputs");
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Launch L{};
+  L.nin = t->nin;
+  L.nout = t->nout;
+  L.B = batch;
+  L.x = d_x;
+  L.xstride = x_stride;
+  L.dirs = d_dirs;
+  L.y = d_coeffs;
+  rc = run(c, t, 2, (int)order, L, pick(c, stream));
+  return rc ? rc : ok();
+}
+
+// ----------------------------- host-pointer API -----------------------------
+
+namespace {
+struct HostCall {
+  ehip_ctx* c;
+  size_t S;
+  int32_t stage(int slot, const void* src, size_t bytes, void** dptr) {
+    hipError_t e = c->io[slot].reserve(bytes);
+    if (e != hipSuccess) return hip_fail(e, "staging buffer");
+    *dptr = c->io[slot].p;
+    if (src) {
+      e = hipMemcpyAsync(*dptr, src, bytes, hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) return hip_fail(e, "host to device copy");
+    }
+    return EHIP_OK;
+  }
+  int32_t fetch(void* dst, const void* dsrc, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(dst, dsrc, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return hip_fail(e, "device to host copy");
+    return EHIP_OK;
+  }
+  int32_t sync() {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "synchronize");
+    return EHIP_OK;
+  }
+};
+}  // namespace
+
+#define TRY(x)              \
+  do {                      \
+    int32_t _r = (x);       \
+    if (_r) return _r;      \
+  } while (0)
+
+int32_t ehip_forward_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs, uint32_t batch, void* outputs) {
+  if (!c || !t || !inputs || !outputs) return fail(EHIP_OTHER, "GPU error: null argument");
+  TRY(check_batch(t, batch));
+  std::lock_guard<std::mutex> hlk(c->host_mu);
+  HostCall h{c, t->f64 ? 8u : 4u};
+  void *dx, *dy;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    TRY(h.stage(0, inputs, (size_t)batch * t->nin * h.S, &dx));
+    TRY(h.stage(1, nullptr, (size_t)batch * t->nout * h.S, &dy));
+  }
+  TRY(ehip_forward_batch_dev(c, t, dx, batch, dy, nullptr));
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(h.fetch(outputs, dy, (size_t)batch * t->nout * h.S));
+  TRY(h.sync());
+  return ok();
+}
+
+int32_t ehip_gradient_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs, uint32_t batch, void* outputs,
+                            void* grads) {
+  if (!c || !t || !inputs || !outputs || !grads) return fail(EHIP_OTHER, "GPU error: null argument");
+  TRY(check_batch(t, batch));
+  std::lock_guard<std::mutex> hlk(c->host_mu);
+  HostCall h{c, t->f64 ? 8u : 4u};
+  void *dx, *dy, *dg;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    TRY(h.stage(0, inputs, (size_t)batch * t->nin * h.S, &dx));
+    TRY(h.stage(1, nullptr, (size_t)batch * t->nout * h.S, &dy));
+    TRY(h.stage(2, nullptr, (size_t)batch * t->nin * h.S, &dg));
+  }
+  TRY(ehip_gradient_batch_dev(c, t, dx, batch, dy, dg, nullptr));
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(h.fetch(outputs, dy, (size_t)batch * t->nout * h.S));
+  TRY(h.fetch(grads, dg, (size_t)batch * t->nin * h.S));
+  TRY(h.sync());
+  return ok();
+}
+
+static int32_t hvp_host(ehip_ctx* c, const ehip_tape* t, const void* x, uint64_t xrows, const void* dirs,
+                        uint32_t batch, void* grads, void* hvps) {
+  if (!c || !t || !x || !dirs || !grads || !hvps) return fail(EHIP_OTHER, "GPU error: null argument");
+  TRY(check_batch(t, batch));
+  std::lock_guard<std::mutex> hlk(c->host_mu);
+  HostCall h{c, t->f64 ? 8u : 4u};
+  size_t row = (size_t)t->nin * h.S;
+  void *dx, *dv, *dg, *dh;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    TRY(h.stage(0, x, xrows * row, &dx));
+    TRY(h.stage(1, dirs, (size_t)batch * row, &dv));
+    TRY(h.stage(2, nullptr, (size_t)batch * row, &dg));
+    TRY(h.stage(3, nullptr, (size_t)batch * row, &dh));
+  }
+  TRY(ehip_hvp_batch_dev(c, t, dx, xrows == 1 ? 0 : t->nin, dv, batch, dg, dh, nullptr));
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(h.fetch(grads, dg, (size_t)batch * row));
+  TRY(h.fetch(hvps, dh, (size_t)batch * row));
+  TRY(h.sync());
+  return ok();
+}
+
+int32_t ehip_hvp_batch(ehip_ctx* c, const ehip_tape* t, const void* x, const void* dirs, uint32_t batch,
+                       void* grads, void* hvps) {
+  return hvp_host(c, t, x, 1, dirs, batch, grads, hvps);
+}
+
+int32_t ehip_hvp_batch_points(ehip_ctx* c, const ehip_tape* t, const void* xs, const void* dirs, uint32_t batch,
+                              void* grads, void* hvps) {
+  return hvp_host(c, t, xs, batch, dirs, batch, grads, hvps);
+}
+
+int32_t ehip_taylor_kth_batch(ehip_ctx* c, const ehip_tape* t, const void* primals, const void* seeds,
+                              uint32_t batch, uint32_t order, void* coeffs) {
+  if (!c || !t || !primals || !seeds || !coeffs) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (order < 1 || order > 5)
+    return fail(EHIP_OTHER, "GPU error: taylor_forward_kth_batch supports order 1..=5, got %u", order);
+  TRY(check_batch(t, batch));
+  std::lock_guard<std::mutex> hlk(c->host_mu);
+  HostCall h{c, t->f64 ? 8u : 4u};
+  size_t row = (size_t)t->nin * h.S;
+  size_t total_out = (size_t)batch * t->nout;
+  void *dx, *dv, *dc;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    TRY(h.stage(0, primals, (size_t)batch * row, &dx));
+    TRY(h.stage(1, seeds, (size_t)batch * row, &dv));
+    TRY(h.stage(2, nullptr, total_out * order * h.S, &dc));
+  }
+  TRY(ehip_taylor_kth_batch_dev(c, t, dx, t->nin, dv, batch, order, dc, nullptr));
+  std::vector<unsigned char> raw(total_out * order * h.S);
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    TRY(h.fetch(raw.data(), dc, raw.size()));
+    TRY(h.sync());
+  }
+  // deinterleave_coefficients (src/gpu/mod.rs:247-259)
+  unsigned char* dst = (unsigned char*)coeffs;
+  for (size_t i = 0; i < total_out; ++i)
+    for (uint32_t k = 0; k < order; ++k)
+      std::memcpy(dst + ((size_t)k * total_out + i) * h.S, raw.data() + (i * order + k) * h.S, h.S);
+  return ok();
+}
+
+}  // extern "C"

@@ -1,0 +1,541 @@
+// Tape -> register program compiler (see ehip_program.h).
+#include "ehip_program.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <queue>
+
+namespace ehip {
+
+namespace {
+
+constexpr uint32_t UNUSED = 0xFFFFFFFFu;
+constexpr uint32_t OPC_INPUT = 0, OPC_CONST = 1, OPC_ADD = 2, OPC_SUB = 3, OPC_MUL = 4, OPC_DIV = 5,
+                   OPC_REM = 6, OPC_POWF = 7, OPC_ATAN2 = 8, OPC_HYPOT = 9, OPC_MAX = 10, OPC_MIN = 11,
+                   OPC_NEG = 12, OPC_RECIP = 13, OPC_SQRT = 14, OPC_CBRT = 15, OPC_POWI = 16,
+                   OPC_EXP = 17, OPC_EXP2 = 18, OPC_EXPM1 = 19, OPC_FRACT = 42, OPC_CUSTOM = 43;
+
+inline bool is_binary(uint32_t op) { return op >= OPC_ADD && op <= OPC_MIN; }
+
+// Which operand values the reverse partial of `op` reads when the
+// adjoints of a (accA) and/or b (accB) are consumed.  Mirrors the operands
+// each arm of reverse_partials touches (src/opcode.rs:331-483).
+void rev_reads(uint32_t op, bool accA, bool accB, int32_t exp, bool& ra, bool& rb, bool& rr) {
+  ra = rb = rr = false;
+  switch (op) {
+    case OPC_ADD: case OPC_SUB: case OPC_NEG: case OPC_FRACT:
+    case 37: case 38: case 39: case 40: case 41:  // signum floor ceil round trunc
+      return;
+    case OPC_MUL: rb = accA; ra = accB; return;
+    case OPC_DIV: rb = true; rr = accB; return;
+    case OPC_REM: ra = rb = accB; return;
+    case OPC_POWF: ra = rb = rr = true; return;
+    case OPC_ATAN2: ra = rb = true; return;
+    case OPC_HYPOT: ra = rb = rr = true; return;
+    case OPC_MAX: case OPC_MIN: ra = rb = true; return;
+    case OPC_SQRT: case OPC_CBRT: case OPC_EXP: case OPC_EXP2: case OPC_EXPM1: rr = true; return;
+    case OPC_POWI: ra = true; rr = (exp == INT32_MIN); return;
+    default: ra = true; return;  // recip, logs, trig, hyperbolic, abs
+  }
+}
+
+struct Opnd {
+  bool imm = false;
+  double v = 0.0;
+  int64_t vid = -1;
+};
+
+struct ATerm {
+  uint32_t kind = T_REG;
+  int64_t vid = -1;      // value vid (DOT) / adjoint vid (DOT_REV)
+  double imm = 0.0;
+  bool const_left = false;  // product written c * x on the tape
+  bool first = false;
+  uint32_t depth = 0;
+};
+
+struct AIns {
+  uint32_t op = 0, flags = 0, aux = 0;
+  int64_t dst = -1, g = -1, r = -1, adjA = -1, adjB = -1;
+  Opnd a, b;
+  std::vector<ATerm> terms;
+};
+
+// One chain of `acc + term` links rooted at an Add.
+struct Term {
+  uint32_t kind;
+  uint32_t slot;       // value slot (PROD: the non-constant factor)
+  double imm;
+  bool const_left;
+  uint32_t contrib;    // tape index of the op whose reverse feeds this term
+  uint32_t sub;        // 0 = arg0 side, 1 = arg1 side (tie order)
+  uint32_t add_pos;    // index of the Add link where the term entered
+};
+struct Chain {
+  std::vector<Term> terms;
+  uint32_t lo = 0, hi = 0, count = 0;  // contiguous ordinal range of absorbed ops
+  uint32_t nadds = 0;
+  bool has_prod = false;
+};
+
+}  // namespace
+
+std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                          uint32_t num_ops, uint32_t num_inputs, uint32_t num_variables,
+                          uint32_t num_constants, uint32_t output_index, const uint32_t* outs,
+                          uint32_t num_outputs) {
+  char buf[256];
+  auto err = [&](const char* fmt, auto... xs) {
+    std::snprintf(buf, sizeof buf, fmt, xs...);
+    return std::string("invalid GpuTapeData: ") + buf;
+  };
+  if (num_ops != num_variables) return err("num_ops (%u) != num_variables (%u)", num_ops, num_variables);
+  if (num_constants != num_variables)
+    return err("constants.len() (%u) != num_variables (%u)", num_constants, num_variables);
+  if (num_inputs > num_variables) return err("num_inputs (%u) > num_variables (%u)", num_inputs, num_variables);
+  for (uint32_t i = 0; i < num_ops; ++i) {
+    uint32_t op = opcodes[i];
+    if (op > OPC_FRACT) return err("opcodes[%u] (%u) is not a GPU-executable opcode", i, op);
+    if ((op == OPC_INPUT) != (i < num_inputs)) {
+      if (i < num_inputs) return err("opcodes[%u] (%u) should be Input", i, op);
+      return err("opcodes[%u] is Input, but only the first %u entries may be inputs", i, num_inputs);
+    }
+  }
+  for (uint32_t i = num_inputs; i < num_ops; ++i) {
+    uint32_t op = opcodes[i];
+    if (op == OPC_CONST) continue;
+    if (arg0[i] >= i) return err("arg0[%u] (%u) must reference an earlier slot", i, arg0[i]);
+    if (op == OPC_POWI) continue;
+    if (is_binary(op)) {
+      if (arg1[i] >= i) return err("arg1[%u] (%u) must reference an earlier slot", i, arg1[i]);
+    } else if (arg1[i] != UNUSED) {
+      return err("arg1[%u] (%u) should be the UNUSED sentinel for unary opcode %u", i, arg1[i], op);
+    }
+  }
+  if (output_index >= num_variables) return err("output_index (%u) >= num_variables (%u)", output_index, num_variables);
+  for (uint32_t j = 0; j < num_outputs; ++j)
+    if (outs[j] >= num_variables) return err("output_indices[%u] (%u) >= num_variables (%u)", j, outs[j], num_variables);
+  return "";
+}
+
+Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
+  const uint32_t N = (uint32_t)t.opcodes.size();
+  const uint32_t ni = t.num_inputs;
+  const bool grad = (mode == PM_GRADIENT);
+  auto is_const = [&](uint32_t s) { return t.opcodes[s] == OPC_CONST; };
+  auto binop = [&](uint32_t i) { return is_binary(t.opcodes[i]); };
+
+  // ---- reverse reachability from the seed (output 0) ----
+  std::vector<char> reach(N, 0);
+  const uint32_t seed = t.outputs[0];
+  if (grad && !is_const(seed)) reach[seed] = 1;
+  if (grad) {
+    for (int64_t ii = (int64_t)N - 1; ii >= (int64_t)ni; --ii) {
+      uint32_t i = (uint32_t)ii;
+      if (!reach[i] || is_const(i)) continue;
+      uint32_t a = t.arg0[i];
+      if (!is_const(a)) reach[a] = 1;
+      if (binop(i) && !is_const(t.arg1[i])) reach[t.arg1[i]] = 1;
+    }
+  }
+
+  // ---- forward values needed: outputs, reverse reads, transitive operands ----
+  std::vector<char> need(N, 0);
+  for (uint32_t o : t.outputs)
+    if (!is_const(o)) need[o] = 1;
+  if (grad) {
+    for (uint32_t i = ni; i < N; ++i) {
+      if (!reach[i] || is_const(i)) continue;
+      uint32_t op = t.opcodes[i];
+      bool accA = !is_const(t.arg0[i]);
+      bool accB = binop(i) && !is_const(t.arg1[i]);
+      if (op == OPC_POWI) accB = false;
+      bool ra, rb, rr;
+      rev_reads(op, accA, accB, (int32_t)t.arg1[i], ra, rb, rr);
+      if (ra && !is_const(t.arg0[i])) need[t.arg0[i]] = 1;
+      if (rb && binop(i) && !is_const(t.arg1[i])) need[t.arg1[i]] = 1;
+      if (rr) need[i] = 1;
+      need[i] = 1;  // a reached op is evaluated so its operands exist
+    }
+  }
+  for (int64_t ii = (int64_t)N - 1; ii >= (int64_t)ni; --ii) {
+    uint32_t i = (uint32_t)ii;
+    if (!need[i] || is_const(i)) continue;
+    if (!is_const(t.arg0[i])) need[t.arg0[i]] = 1;
+    if (binop(i) && !is_const(t.arg1[i])) need[t.arg1[i]] = 1;
+  }
+
+  // ---- use counts over needed ops and outputs ----
+  std::vector<uint32_t> uses(N, 0);
+  std::vector<char> is_out(N, 0);
+  for (uint32_t o : t.outputs) is_out[o] = 1;
+  std::vector<uint32_t> ord(N, 0);
+  uint32_t nord = 0;
+  for (uint32_t i = ni; i < N; ++i) {
+    if (!need[i] || is_const(i)) continue;
+    ord[i] = nord++;
+    if (!is_const(t.arg0[i])) uses[t.arg0[i]]++;
+    if (binop(i) && !is_const(t.arg1[i])) uses[t.arg1[i]]++;
+  }
+
+  // ---- DOT fusion ----
+  std::vector<char> absorbed(N, 0);
+  std::vector<int64_t> chain_of(N, -1);
+  std::vector<Chain> chains;
+  auto prod_candidate = [&](uint32_t s, uint32_t consumer) -> bool {
+    if (s < ni || is_const(s) || t.opcodes[s] != OPC_MUL) return false;
+    if (uses[s] != 1 || is_out[s]) return false;
+    bool c0 = is_const(t.arg0[s]), c1 = is_const(t.arg1[s]);
+    if (c0 == c1) return false;
+    (void)consumer;
+    return true;
+  };
+  if (fuse) {
+    for (uint32_t i = ni; i < N; ++i) {
+      if (!need[i] || is_const(i) || t.opcodes[i] != OPC_ADD) continue;
+      uint32_t ops[2] = {t.arg0[i], t.arg1[i]};
+      int take_chain = -1;
+      for (int k = 0; k < 2; ++k) {
+        uint32_t o = ops[k];
+        if (take_chain < 0 && o >= ni && !is_const(o) && t.opcodes[o] == OPC_ADD && chain_of[o] >= 0 &&
+            uses[o] == 1 && !is_out[o]) {
+          const Chain& c = chains[chain_of[o]];
+          // contiguity with op i (and a possible absorbed product on the other side)
+          uint32_t lo = c.lo, hi = c.hi, cnt = c.count;
+          uint32_t other = ops[1 - k];
+          auto extend = [&](uint32_t s) { lo = std::min(lo, ord[s]); hi = std::max(hi, ord[s]); cnt++; };
+          extend(i);
+          if (!is_const(other) && prod_candidate(other, i)) extend(other);
+          if (cnt == hi - lo + 1) take_chain = k;
+        }
+      }
+      Chain ch;
+      if (take_chain >= 0) {
+        uint32_t o = ops[take_chain];
+        ch = chains[chain_of[o]];
+        absorbed[o] = 1;
+      }
+      uint32_t add_pos = ch.nadds;
+      auto add_ord = [&](uint32_t s) {
+        if (ch.count == 0) { ch.lo = ch.hi = ord[s]; ch.count = 1; return; }
+        ch.lo = std::min(ch.lo, ord[s]);
+        ch.hi = std::max(ch.hi, ord[s]);
+        ch.count++;
+      };
+      add_ord(i);
+      for (int k = 0; k < 2; ++k) {
+        if (k == take_chain) continue;
+        uint32_t o = ops[k];
+        Term tm{};
+        tm.sub = (uint32_t)k;
+        tm.add_pos = add_pos;
+        if (is_const(o)) {
+          tm.kind = T_IMM; tm.slot = o; tm.imm = t.values[o]; tm.contrib = i;
+        } else if (prod_candidate(o, i)) {
+          // keep it only if the product stays contiguous with the chain
+          uint32_t lo = std::min(ch.lo, ord[o]), hi = std::max(ch.hi, ord[o]);
+          if (ch.count + 1 == hi - lo + 1) {
+            bool cl = is_const(t.arg0[o]);
+            tm.kind = T_PROD;
+            tm.slot = cl ? t.arg1[o] : t.arg0[o];
+            tm.imm = t.values[cl ? t.arg0[o] : t.arg1[o]];
+            tm.const_left = cl;
+            tm.contrib = o;
+            absorbed[o] = 1;
+            add_ord(o);
+            ch.has_prod = true;
+          } else {
+            tm.kind = T_REG; tm.slot = o; tm.contrib = i;
+          }
+        } else {
+          tm.kind = T_REG; tm.slot = o; tm.contrib = i;
+        }
+        ch.terms.push_back(tm);
+      }
+      ch.nadds++;
+      chain_of[i] = (int64_t)chains.size();
+      chains.push_back(std::move(ch));
+    }
+  }
+  auto dot_root = [&](uint32_t i) -> const Chain* {
+    if (!fuse || chain_of[i] < 0 || absorbed[i]) return nullptr;
+    const Chain& c = chains[chain_of[i]];
+    if (c.terms.size() >= 3 || c.has_prod) return &c;
+    return nullptr;
+  };
+
+  // ---- abstract instruction stream ----
+  std::vector<AIns> prog;
+  auto VAL = [&](uint32_t s) -> int64_t { return (int64_t)s; };
+  auto ADJ = [&](uint32_t s) -> int64_t { return (int64_t)N + s; };
+  auto opnd = [&](uint32_t s) {
+    Opnd o;
+    if (is_const(s)) { o.imm = true; o.v = t.values[s]; }
+    else o.vid = VAL(s);
+    return o;
+  };
+
+  for (uint32_t i = 0; i < ni; ++i) {
+    if (!need[i]) continue;
+    AIns x;
+    x.op = I_LOAD;
+    x.dst = VAL(i);
+    x.aux = i;
+    prog.push_back(x);
+  }
+  uint64_t dot_terms = 0;
+  for (uint32_t i = ni; i < N; ++i) {
+    if (!need[i] || is_const(i) || absorbed[i]) continue;
+    if (const Chain* c = dot_root(i)) {
+      AIns x;
+      x.op = I_DOT;
+      x.dst = VAL(i);
+      for (const Term& tm : c->terms) {
+        ATerm at;
+        at.kind = tm.kind;
+        at.imm = tm.imm;
+        at.const_left = tm.const_left;
+        if (tm.kind != T_IMM) at.vid = VAL(tm.slot);
+        x.terms.push_back(at);
+      }
+      dot_terms += c->terms.size();
+      prog.push_back(std::move(x));
+      continue;
+    }
+    AIns x;
+    uint32_t op = t.opcodes[i];
+    x.op = op;
+    x.dst = VAL(i);
+    x.a = opnd(t.arg0[i]);
+    if (x.a.imm) x.flags |= F_A_IMM;
+    if (op == OPC_POWI) {
+      x.aux = t.arg1[i];
+    } else if (binop(i)) {
+      x.flags |= F_HAS_B;
+      x.b = opnd(t.arg1[i]);
+      if (x.b.imm) x.flags |= F_B_IMM;
+    }
+    prog.push_back(std::move(x));
+  }
+  for (uint32_t j = 0; j < (uint32_t)t.outputs.size(); ++j) {
+    AIns x;
+    x.op = I_STORE_OUT;
+    x.aux = j;
+    x.a = opnd(t.outputs[j]);
+    if (x.a.imm) x.flags |= F_A_IMM;
+    prog.push_back(x);
+  }
+
+  if (grad) {
+    std::vector<char> adj_def(N, 0);
+    if (!is_const(seed)) {
+      AIns x;
+      x.op = I_SEED;
+      x.dst = ADJ(seed);
+      adj_def[seed] = 1;
+      prog.push_back(x);
+    }
+    for (int64_t ii = (int64_t)N - 1; ii >= (int64_t)ni; --ii) {
+      uint32_t i = (uint32_t)ii;
+      if (!reach[i] || is_const(i) || absorbed[i]) continue;
+      if (const Chain* c = dot_root(i)) {
+        AIns x;
+        x.op = I_DOT_REV;
+        x.g = ADJ(i);
+        std::vector<const Term*> order;
+        for (const Term& tm : c->terms) order.push_back(&tm);
+        std::stable_sort(order.begin(), order.end(), [](const Term* p, const Term* q) {
+          if (p->contrib != q->contrib) return p->contrib > q->contrib;
+          return p->sub < q->sub;
+        });
+        for (const Term* tm : order) {
+          if (tm->kind == T_IMM) continue;
+          ATerm at;
+          at.kind = tm->kind;
+          at.imm = tm->imm;
+          at.const_left = tm->const_left;
+          at.vid = ADJ(tm->slot);
+          at.depth = (c->nadds - 1) - tm->add_pos;
+          at.first = !adj_def[tm->slot];
+          adj_def[tm->slot] = 1;
+          x.terms.push_back(at);
+        }
+        prog.push_back(std::move(x));
+        continue;
+      }
+      uint32_t op = t.opcodes[i];
+      uint32_t a = t.arg0[i], b = t.arg1[i];
+      bool accA = !is_const(a);
+      bool accB = binop(i) && !is_const(b);
+      bool ra, rb, rr;
+      rev_reads(op, accA, accB, (int32_t)b, ra, rb, rr);
+      AIns x;
+      x.op = I_REV_BASE + op;
+      x.g = ADJ(i);
+      x.a = opnd(a);
+      if (x.a.imm) x.flags |= F_A_IMM;
+      else if (ra) x.flags |= F_RD_A;
+      if (op == OPC_POWI) {
+        x.aux = b;
+      } else if (binop(i)) {
+        x.flags |= F_HAS_B;
+        x.b = opnd(b);
+        if (x.b.imm) x.flags |= F_B_IMM;
+        else if (rb) x.flags |= F_RD_B;
+      }
+      if (rr) { x.flags |= F_RD_R; x.r = VAL(i); }
+      if (accA) {
+        x.flags |= F_ACC_A;
+        x.adjA = ADJ(a);
+        if (!adj_def[a]) { x.flags |= F_FIRST_A; adj_def[a] = 1; }
+      }
+      if (accB) {
+        x.flags |= F_ACC_B;
+        x.adjB = ADJ(b);
+        if (!adj_def[b]) { x.flags |= F_FIRST_B; adj_def[b] = 1; }
+      }
+      prog.push_back(std::move(x));
+    }
+    for (uint32_t i = 0; i < ni; ++i) {
+      AIns x;
+      x.op = I_STORE_GRAD;
+      x.aux = i;
+      if (adj_def[i]) x.a.vid = ADJ(i);
+      else { x.a.imm = true; x.a.v = 0.0; x.flags |= F_A_IMM; }
+      prog.push_back(x);
+    }
+  }
+
+  // ---- live ranges ----
+  const size_t NV = 2 * (size_t)N;
+  std::vector<int64_t> def(NV, -1), last(NV, -1);
+  auto rd = [&](int64_t v, int64_t p) { if (v >= 0) last[v] = std::max(last[v], p); };
+  auto wr = [&](int64_t v, int64_t p) { if (v >= 0 && def[v] < 0) def[v] = p; };
+  for (int64_t p = 0; p < (int64_t)prog.size(); ++p) {
+    AIns& x = prog[p];
+    if (!x.a.imm) {
+      bool read_a = true;
+      if (x.op >= I_REV_BASE) read_a = (x.flags & F_RD_A) != 0;
+      if (x.op == I_LOAD || x.op == I_SEED) read_a = false;
+      if (read_a) rd(x.a.vid, p);
+    }
+    if (!x.b.imm && (x.flags & F_HAS_B)) {
+      bool read_b = true;
+      if (x.op >= I_REV_BASE) read_b = (x.flags & F_RD_B) != 0;
+      if (read_b) rd(x.b.vid, p);
+    }
+    rd(x.g, p);
+    rd(x.r, p);
+    if (x.op == I_DOT) {
+      for (auto& tm : x.terms) rd(tm.vid, p);
+      wr(x.dst, p);
+    } else if (x.op == I_DOT_REV) {
+      for (auto& tm : x.terms) {
+        if (tm.first) wr(tm.vid, p);
+        else rd(tm.vid, p);
+        rd(tm.vid, p);
+      }
+    } else {
+      wr(x.dst, p);
+      if (x.adjA >= 0) { if (x.flags & F_FIRST_A) wr(x.adjA, p); else rd(x.adjA, p); rd(x.adjA, p); }
+      if (x.adjB >= 0) { if (x.flags & F_FIRST_B) wr(x.adjB, p); else rd(x.adjB, p); rd(x.adjB, p); }
+    }
+  }
+  // A value read before any write would be a compiler bug; treat as defined
+  // at its first read to stay memory-safe.
+  struct Iv { int64_t s, e; size_t v; };
+  std::vector<Iv> ivs;
+  for (size_t v = 0; v < NV; ++v) {
+    if (def[v] < 0 && last[v] < 0) continue;
+    int64_t s = def[v] >= 0 ? def[v] : last[v];
+    int64_t e = std::max(last[v], s);
+    ivs.push_back({s, e, v});
+  }
+  std::sort(ivs.begin(), ivs.end(), [](const Iv& a, const Iv& b) { return a.s < b.s || (a.s == b.s && a.v < b.v); });
+  std::vector<uint32_t> reg(NV, 0);
+  std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> freeq;
+  typedef std::pair<int64_t, uint32_t> EndReg;
+  std::priority_queue<EndReg, std::vector<EndReg>, std::greater<EndReg>> active;
+  uint32_t nregs = 0;
+  for (const Iv& iv : ivs) {
+    while (!active.empty() && active.top().first < iv.s) {
+      freeq.push(active.top().second);
+      active.pop();
+    }
+    uint32_t r;
+    if (!freeq.empty()) { r = freeq.top(); freeq.pop(); }
+    else r = nregs++;
+    reg[iv.v] = r;
+    active.push({iv.e, r});
+  }
+
+  // ---- encode ----
+  Program out;
+  out.nregs = std::max<uint32_t>(nregs, 1);
+  out.dot_terms = dot_terms;
+  auto put_imm = [&](double v, uint32_t& lo, uint32_t& hi) {
+    if (t.f64) {
+      uint64_t bits;
+      std::memcpy(&bits, &v, 8);
+      lo = (uint32_t)bits;
+      hi = (uint32_t)(bits >> 32);
+    } else {
+      float f = (float)v;
+      std::memcpy(&lo, &f, 4);
+      hi = 0;
+    }
+  };
+  auto R = [&](int64_t v) -> uint32_t { return v >= 0 ? reg[v] : 0u; };
+  for (AIns& x : prog) {
+    uint32_t w[NW] = {0, 0, 0, 0, 0, 0, 0, 0};
+    w[0] = x.op | (x.flags << 8);
+    switch (x.op) {
+      case I_LOAD: w[1] = R(x.dst); w[2] = x.aux; break;
+      case I_SEED: w[1] = R(x.dst); break;
+      case I_STORE_OUT:
+      case I_STORE_GRAD:
+        w[1] = x.aux;
+        if (x.a.imm) put_imm(x.a.v, w[2], w[3]);
+        else w[2] = R(x.a.vid);
+        break;
+      case I_DOT:
+      case I_DOT_REV: {
+        w[1] = (x.op == I_DOT) ? R(x.dst) : R(x.g);
+        w[2] = (uint32_t)x.terms.size();
+        w[3] = (uint32_t)(out.terms.size() / TW);
+        for (auto& tm : x.terms) {
+          uint32_t tw[TW] = {0, 0, 0, 0};
+          uint32_t tfl = (tm.first ? 1u : 0u) | (tm.const_left ? 2u : 0u);
+          tw[0] = tm.kind | (tfl << 8) | (std::min<uint32_t>(tm.depth, 0xFFFFu) << 16);
+          tw[1] = (tm.vid >= 0) ? R(tm.vid) : 0u;
+          put_imm(tm.imm, tw[2], tw[3]);
+          out.terms.insert(out.terms.end(), tw, tw + TW);
+        }
+        break;
+      }
+      default:
+        if (x.op < I_REV_BASE) {
+          w[1] = R(x.dst);
+        } else {
+          w[1] = R(x.g);
+          w[6] = R(x.r);
+        }
+        if (x.a.imm) put_imm(x.a.v, w[2], w[3]);
+        else { w[2] = R(x.a.vid); w[3] = R(x.adjA); }
+        if (x.flags & F_HAS_B) {
+          if (x.b.imm) put_imm(x.b.v, w[4], w[5]);
+          else { w[4] = R(x.b.vid); w[5] = R(x.adjB); }
+        }
+        w[7] = x.aux;
+        break;
+    }
+    out.words.insert(out.words.end(), w, w + NW);
+  }
+  out.ninstr = (uint32_t)prog.size();
+  if (out.terms.empty()) out.terms.assign(TW, 0);
+  return out;
+}
+
+}  // namespace ehip

@@ -1,0 +1,158 @@
+/*
+ * echidna_hip.h — C ABI of the MI355X (gfx950) batched BytecodeTape backend.
+ *
+ * Drop-in boundary for the reference's GPU backend trait `GpuBackend`
+ * (src/gpu/mod.rs:92-225) and the CUDA backend's f64 inherent methods
+ * (src/gpu/cuda_backend.rs:566-953).  Plain pointers and sizes only; every
+ * entry point returns an `int32_t` status (EHIP_OK or a `GpuError` variant,
+ * src/gpu/mod.rs:276-287) and leaves a thread-local message readable with
+ * ehip_last_error().  No call panics or throws across the boundary.
+ *
+ * Scalar type: a tape is uploaded as f32 or f64 (EHIP_F32 / EHIP_F64); every
+ * `void*` data argument of a call on that tape points to that scalar type.
+ * f32 tapes serve the trait's f32 methods, f64 tapes the CUDA backend's
+ * `*_f64` inherent methods.
+ *
+ * Layouts at the boundary follow the reference: point-major rows
+ * (`inputs[b * num_inputs + i]`), outputs `[b * num_outputs + j]`, Taylor
+ * coefficients de-interleaved into `coeffs[k * (batch * num_outputs) + ...]`
+ * (TaylorKthBatchResult, src/gpu/mod.rs:266-272).
+ *
+ * Host-pointer entry points (`ehip_*_batch`) copy in, run and copy out
+ * synchronously, like the reference's per-call clone_htod / launch /
+ * synchronize / clone_dtoh.  Device-pointer entry points (`*_dev`) take
+ * buffers already resident in HBM and enqueue on the caller's HIP stream
+ * (NULL = the context's stream) without synchronizing.
+ *
+ * Threading: calls on one context are serialized by the context; the
+ * context's scratch registers are reused across calls, so `*_dev` work on
+ * different streams through ONE context must not overlap — use one context
+ * per stream for concurrent streams.
+ */
+#ifndef ECHIDNA_HIP_H
+#define ECHIDNA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: 0 ok, then the GpuError variants (src/gpu/mod.rs:276-287). */
+enum {
+  EHIP_OK = 0,
+  EHIP_NO_DEVICE = 1,
+  EHIP_SHADER_COMPILATION = 2,
+  EHIP_OUT_OF_MEMORY = 3,
+  EHIP_CUSTOM_OPS_NOT_SUPPORTED = 4,
+  EHIP_OTHER = 5
+};
+
+enum { EHIP_F32 = 0, EHIP_F64 = 1 };
+
+typedef struct ehip_ctx ehip_ctx;
+typedef struct ehip_tape ehip_tape;
+
+/* Message of the last failing call on this thread ("" when none). */
+const char* ehip_last_error(void);
+
+/* Library version string. */
+const char* ehip_version(void);
+
+/* CudaContext::new (src/gpu/cuda_backend.rs:503-552): acquire device
+ * `device`.  Returns EHIP_NO_DEVICE when no gfx950 device is present. */
+int32_t ehip_ctx_create(int32_t device, ehip_ctx** out);
+void ehip_ctx_destroy(ehip_ctx* ctx);
+
+/* GpuBackend::upload_tape (src/gpu/mod.rs:107) / CudaContext::upload_tape_f64
+ * (src/gpu/cuda_backend.rs:566-611).  Runs the GpuTapeData::validate checks
+ * (src/gpu/mod.rs:397-564) before touching the device; invalid data returns
+ * EHIP_OTHER with the reference's message (the Rust shim turns this into the
+ * trait's panic).  An opcode 43 (Custom) returns
+ * EHIP_CUSTOM_OPS_NOT_SUPPORTED.  `constants` has `num_variables` entries of
+ * the tape's scalar type.  `num_outputs == 0` synthesizes
+ * `[output_index]` (src/gpu/cuda_backend.rs:638-647). */
+int32_t ehip_tape_upload(ehip_ctx* ctx, int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0,
+                         const uint32_t* arg1, const void* constants, uint32_t num_ops,
+                         uint32_t num_inputs, uint32_t num_variables, uint32_t output_index,
+                         const uint32_t* output_indices, uint32_t num_outputs, ehip_tape** out);
+void ehip_tape_free(ehip_tape* tape);
+
+/* GpuBackend::num_outputs (src/gpu/mod.rs:115). */
+uint32_t ehip_num_outputs(const ehip_tape* tape);
+uint32_t ehip_num_inputs(const ehip_tape* tape);
+int32_t ehip_tape_dtype(const ehip_tape* tape);
+
+/* GpuBackend::forward_batch (src/gpu/mod.rs:118-124), forward_batch_f64
+ * (src/gpu/cuda_backend.rs:878).  inputs [batch*num_inputs],
+ * outputs [batch*num_outputs]. */
+int32_t ehip_forward_batch(ehip_ctx* ctx, const ehip_tape* tape, const void* inputs, uint32_t batch,
+                           void* outputs);
+
+/* GpuBackend::gradient_batch (src/gpu/mod.rs:126-137), gradient_batch_f64
+ * (src/gpu/cuda_backend.rs:896).  Seeds output_indices[0]. */
+int32_t ehip_gradient_batch(ehip_ctx* ctx, const ehip_tape* tape, const void* inputs, uint32_t batch,
+                            void* outputs, void* grads);
+
+/* GpuBackend::hvp_batch (src/gpu/mod.rs:149-160), hvp_batch_f64
+ * (src/gpu/cuda_backend.rs:935): ONE point x [num_inputs], `batch`
+ * directions [batch*num_inputs] -> grads, hvps [batch*num_inputs]. */
+int32_t ehip_hvp_batch(ehip_ctx* ctx, const ehip_tape* tape, const void* x, const void* tangent_dirs,
+                       uint32_t batch, void* grads, void* hvps);
+
+/* Extension of hvp_batch with one point per direction: xs [batch*num_inputs]
+ * (no reference counterpart; the reference replicates one x on the host,
+ * src/gpu/cuda_backend.rs:244-247). */
+int32_t ehip_hvp_batch_points(ehip_ctx* ctx, const ehip_tape* tape, const void* xs,
+                              const void* tangent_dirs, uint32_t batch, void* grads, void* hvps);
+
+/* GpuBackend::taylor_forward_kth_batch (src/gpu/mod.rs:206-224),
+ * taylor_forward_kth_batch_f64 (src/gpu/cuda_backend.rs:854-873).
+ * order in 1..=5; inputs seeded (x_i, v_i, 0, ...); coeffs receives
+ * order * batch * num_outputs values, coefficient-major. */
+int32_t ehip_taylor_kth_batch(ehip_ctx* ctx, const ehip_tape* tape, const void* primal_inputs,
+                              const void* direction_seeds, uint32_t batch, uint32_t order, void* coeffs);
+
+/* ---- device-pointer variants: buffers already in HBM, enqueued on `stream`
+ * (a hipStream_t; NULL = the context's stream), no synchronization. ---- */
+int32_t ehip_forward_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const void* d_inputs, uint64_t batch,
+                               void* d_outputs, void* stream);
+int32_t ehip_gradient_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const void* d_inputs, uint64_t batch,
+                                void* d_outputs, void* d_grads, void* stream);
+/* x_stride 0: one shared point; x_stride num_inputs: one point per row. */
+int32_t ehip_hvp_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const void* d_x, uint64_t x_stride,
+                           const void* d_dirs, uint64_t batch, void* d_grads, void* d_hvps, void* stream);
+/* coeffs interleaved [batch*num_outputs][order] on the device (the kernel's
+ * native layout; the host entry point de-interleaves). */
+int32_t ehip_taylor_kth_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const void* d_x, uint64_t x_stride,
+                                  const void* d_dirs, uint64_t batch, uint32_t order, void* d_coeffs,
+                                  void* stream);
+
+/* Introspection of the compiled register program (for tests and the
+ * traffic model): mode 0 forward, 1 gradient/HVP. out[0]=instructions,
+ * out[1]=registers, out[2]=program words, out[3]=fused dot terms. */
+int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
+
+/* Compile a tape into its register program without a device (inspection
+ * and tests).  Same validation as ehip_tape_upload.  Writes up to
+ * `words_cap` program words and `terms_cap` DOT side-table words; out4 as in
+ * ehip_program_info.  Returns EHIP_OTHER if a buffer is too small (out4 still
+ * holds the sizes needed: out4[2] words, out4[3] terms*4 side-table words). */
+int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                             const double* constants, uint32_t num_ops, uint32_t num_inputs,
+                             uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,
+                             int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
+                             uint64_t terms_cap, uint64_t* out4);
+
+/* Launch tuning knobs (autotuner): workgroup size (64/128/256) and the
+ * per-workgroup LDS budget for the register file in bytes.  0 = default. */
+int32_t ehip_set_launch(ehip_ctx* ctx, uint32_t wg_size, uint32_t lds_budget_bytes);
+
+/* Dot-product fusion for tapes uploaded after this call (default on).  Off
+ * runs every tape entry as its own instruction; results are identical. */
+int32_t ehip_set_fusion(ehip_ctx* ctx, int32_t enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ECHIDNA_HIP_H */

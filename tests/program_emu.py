@@ -1,0 +1,190 @@
+"""Numpy emulator of the scalar register-program interpreter (test helper).
+
+Executes the words produced by ``ehip_compile_program`` exactly as the HIP
+scalar interpreter does — same register numbers, same FIRST/ACC flags, same
+DOT term order — vectorised over points.  Comparing it with the CPU oracle
+checks the upload-time compiler (pruning, liveness, register reuse, fusion,
+adjoint order) without a GPU.  Arithmetic is numpy's, so transcendentals may
+differ from the oracle by an ulp; structure errors show up as gross
+mismatches.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+I_LOAD, I_STORE_OUT, I_SEED, I_STORE_GRAD, I_DOT, I_DOT_REV, I_REV = 64, 65, 66, 67, 68, 69, 128
+F_A_IMM, F_B_IMM, F_FIRST_A, F_FIRST_B, F_ACC_A, F_ACC_B, F_HAS_B, F_RD_A, F_RD_B, F_RD_R = (
+    1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
+T_PROD, T_REG, T_IMM = 0, 1, 2
+
+
+def _imm(lo, hi, f64):
+    if f64:
+        return np.array([int(lo) | (int(hi) << 32)], dtype=np.uint64).view(np.float64)[0]
+    return np.array([int(lo)], dtype=np.uint32).view(np.float32)[0]
+
+
+def _i32(v):
+    v = int(v) & 0xFFFFFFFF
+    return v - (1 << 32) if v & 0x80000000 else v
+
+
+def _powi(a, n, F):
+    r = np.ones_like(a)
+    recip = n < 0
+    e = n
+    b = a.copy()
+    while True:
+        if e & 1:
+            r = r * b
+        e = int(e / 2)
+        if e == 0:
+            break
+        b = b * b
+    return F(1) / r if recip else r
+
+
+def _eval(op, a, b, F):
+    one = F(1)
+    return {
+        2: lambda: a + b, 3: lambda: a - b, 4: lambda: a * b, 5: lambda: a / b, 6: lambda: np.fmod(a, b),
+        7: lambda: np.power(a, b), 8: lambda: np.arctan2(a, b), 9: lambda: np.hypot(a, b),
+        10: lambda: np.where((a >= b) | np.isnan(b), a, b), 11: lambda: np.where((a <= b) | np.isnan(b), a, b),
+        12: lambda: -a, 13: lambda: one / a, 14: lambda: np.sqrt(a), 15: lambda: np.cbrt(a),
+        17: lambda: np.exp(a), 18: lambda: np.exp2(a), 19: lambda: np.expm1(a), 20: lambda: np.log(a),
+        21: lambda: np.log2(a), 22: lambda: np.log10(a), 23: lambda: np.log1p(a), 24: lambda: np.sin(a),
+        25: lambda: np.cos(a), 26: lambda: np.tan(a), 27: lambda: np.arcsin(a), 28: lambda: np.arccos(a),
+        29: lambda: np.arctan(a), 30: lambda: np.sinh(a), 31: lambda: np.cosh(a), 32: lambda: np.tanh(a),
+        33: lambda: np.arcsinh(a), 34: lambda: np.arccosh(a), 35: lambda: np.arctanh(a), 36: lambda: np.abs(a),
+        37: lambda: np.where(np.isnan(a), a, np.copysign(one, a)), 38: lambda: np.floor(a),
+        39: lambda: np.ceil(a), 40: lambda: np.where(a >= 0, np.floor(a + F(0.5)), np.ceil(a - F(0.5))),
+        41: lambda: np.trunc(a), 42: lambda: a - np.trunc(a),
+    }[op]()
+
+
+def _partials(op, a, b, r, F):
+    z = np.zeros_like(r) if hasattr(r, "shape") else F(0)
+    one = F(1)
+    nan = F(np.nan)
+    if op == 2:
+        return one + z, one + z
+    if op == 3:
+        return one + z, -one + z
+    if op == 4:
+        return b + z, a + z
+    if op == 5:
+        inv = one / b
+        return inv + z, -r * inv
+    if op == 6:
+        return one + z, -np.trunc(a / b)
+    if op == 12:
+        return -one + z, z
+    if op == 13:
+        inv = one / a
+        return -inv * inv, z
+    if op == 14:
+        return one / ((one + one) * r), z
+    if op == 17:
+        return r + z, z
+    if op == 20:
+        return np.where(a >= 0, one / a, nan), z
+    if op == 24:
+        return np.cos(a), z
+    if op == 25:
+        return -np.sin(a), z
+    if op == 32:
+        c = np.cosh(a)
+        return one / (c * c), z
+    if op == 36:
+        return np.where(a == 0, F(0), np.copysign(one, a)), z
+    if op in (10, 11):
+        win = ((a >= b) if op == 10 else (a <= b)) | np.isnan(b)
+        return np.where(win, one, F(0)) + z, np.where(win, F(0), one) + z
+    if op in (37, 38, 39, 40, 41):
+        return z, z
+    if op == 42:
+        return one + z, z
+    raise NotImplementedError(f"emulator partial for opcode {op}")
+
+
+def run(words, terms, x, nin, nout, nregs, f64=True, mode=1):
+    """Run the program over points ``x`` [B, nin]; returns (y [B,nout], g [B,nin])."""
+    F = np.float64 if f64 else np.float32
+    B = x.shape[0]
+    regs = np.full((nregs, B), np.nan, dtype=F)
+    y = np.zeros((B, nout), F)
+    g = np.zeros((B, nin), F)
+    with np.errstate(all="ignore"):
+        for w in words:
+            w = [int(v) for v in w]
+            op, fl = w[0] & 0xFF, w[0] >> 8
+
+            def val(lo, hi, imm_flag):
+                return F(_imm(lo, hi, f64)) if fl & imm_flag else regs[lo].copy()
+
+            if op < 64:
+                a = val(w[2], w[3], F_A_IMM)
+                b = val(w[4], w[5], F_B_IMM) if fl & F_HAS_B else F(0)
+                r = _powi(np.broadcast_to(a, (B,)).astype(F), _i32(w[7]), F) if op == 16 else _eval(op, a, b, F)
+                regs[w[1]] = r
+            elif op >= I_REV:
+                top = op - I_REV
+                gg = regs[w[1]].copy()
+                live = gg != 0
+                a = F(_imm(w[2], w[3], f64)) if fl & F_A_IMM else (regs[w[2]].copy() if fl & F_RD_A else np.zeros(B, F))
+                b = F(_imm(w[4], w[5], f64)) if fl & F_B_IMM else (regs[w[4]].copy() if fl & F_RD_B else np.zeros(B, F))
+                r = regs[w[6]].copy() if fl & F_RD_R else np.zeros(B, F)
+                if top == 16:
+                    e = _i32(w[7])
+                    aa = np.broadcast_to(a, (B,)).astype(F)
+                    da = np.zeros(B, F) if e == 0 else F(e) * _powi(aa, e - 1, F)
+                    db = np.zeros(B, F)
+                else:
+                    da, db = _partials(top, a, b, r, F)
+                    da = np.broadcast_to(da, (B,)).astype(F)
+                    db = np.broadcast_to(db, (B,)).astype(F)
+                if fl & F_ACC_A:
+                    cur = np.zeros(B, F) if fl & F_FIRST_A else regs[w[3]].copy()
+                    m = live & (da != 0)
+                    cur[m] = cur[m] + da[m] * gg[m]
+                    regs[w[3]] = cur
+                if fl & F_ACC_B:
+                    cur = np.zeros(B, F) if fl & F_FIRST_B else regs[w[5]].copy()
+                    m = live & (db != 0)
+                    cur[m] = cur[m] + db[m] * gg[m]
+                    regs[w[5]] = cur
+            elif op == I_DOT:
+                acc = None
+                for k in range(w[2]):
+                    t0, t1, t2, t3 = [int(v) for v in terms[w[3] + k]]
+                    c = F(_imm(t2, t3, f64))
+                    kind = t0 & 0xFF
+                    term = (regs[t1] * c if kind == T_PROD else regs[t1].copy()) if kind != T_IMM else np.full(B, c, F)
+                    acc = term if acc is None else acc + term
+                regs[w[1]] = acc
+            elif op == I_DOT_REV:
+                gg = regs[w[1]].copy()
+                live = gg != 0
+                for k in range(w[2]):
+                    t0, t1, t2, t3 = [int(v) for v in terms[w[3] + k]]
+                    c = F(_imm(t2, t3, f64))
+                    kind = t0 & 0xFF
+                    cur = np.zeros(B, F) if (t0 >> 8) & 1 else regs[t1].copy()
+                    if kind == T_PROD:
+                        if c != 0:
+                            cur[live] = cur[live] + c * gg[live]
+                    else:
+                        cur[live] = cur[live] + F(1) * gg[live]
+                    regs[t1] = cur
+            elif op == I_LOAD:
+                regs[w[1]] = x[:, w[2]]
+            elif op == I_STORE_OUT:
+                y[:, w[1]] = val(w[2], w[3], F_A_IMM)
+            elif op == I_SEED:
+                regs[w[1]] = F(1)
+            elif op == I_STORE_GRAD:
+                g[:, w[1]] = val(w[2], w[3], F_A_IMM)
+            else:
+                raise ValueError(f"unknown instruction {op}")
+    return y, g

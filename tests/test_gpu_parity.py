@@ -1,0 +1,239 @@
+"""GPU parity: the HIP backend against the CPU oracle (f64 reference sweeps).
+
+Bars (stated here, used below):
+  * per-opcode table: the reference's own ULP budgets (tests/gpu_cpu_parity.rs),
+    f64 GPU vs f64 CPU and f32 GPU vs f64 CPU rounded to f32;
+  * composite tapes, f64: |gpu - cpu| <= 1e-12 * max(|cpu_i|, max_j |cpu_j|)
+    per row (transcendental last-ulp differences only);
+  * composite tapes, f32 vs the f64 oracle: same form with 2e-5 (F32_TOL).
+"""
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from echidna_amd import workloads as wl
+from echidna_amd.tape import GpuError, GpuTapeData, record
+from parity_cases import CASES, ulp_diff_f32, ulp_diff_f64
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 2e-5
+
+
+def rowrel(gpu, ref):
+    gpu = np.asarray(gpu, np.float64).reshape(ref.shape)
+    scale = np.maximum(np.abs(ref), np.max(np.abs(ref), axis=-1, keepdims=True))
+    scale = np.where(scale == 0, 1.0, scale)
+    return np.max(np.abs(gpu - ref) / scale)
+
+
+# ---------------------------------------------------------------------------
+# per-opcode parity table
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_parity_table_f64(hip, case):
+    name, build, points, _f32_ulp, f64_ulp = case
+    tape = build()
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    x = np.asarray(points, dtype=np.float64)
+    B = x.shape[0]
+    y, g = hip.gradient_batch(bufs, x, B)
+    y_ref, g_ref = po.gradient(tape, x)
+    yf = hip.forward_batch(bufs, x, B)
+    for b in range(B):
+        assert ulp_diff_f64(y[b], y_ref[b, 0]) <= f64_ulp, (name, x[b], y[b], y_ref[b, 0])
+        assert ulp_diff_f64(yf[b], y_ref[b, 0]) <= f64_ulp, (name, x[b], yf[b], y_ref[b, 0])
+        for i in range(tape.num_inputs):
+            assert ulp_diff_f64(g[b * tape.num_inputs + i], g_ref[b, i]) <= f64_ulp, (name, x[b], i, g, g_ref)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_parity_table_f32(hip, case):
+    name, build, points, f32_ulp, _ = case
+    tape = build()
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64_lossy(tape))
+    x = np.asarray(points, dtype=np.float64)
+    B = x.shape[0]
+    y, g = hip.gradient_batch(bufs, x.astype(np.float32), B)
+    y_ref, g_ref = po.gradient(tape, x)
+    with np.errstate(over="ignore"):
+        for b in range(B):
+            assert ulp_diff_f32(y[b], np.float32(y_ref[b, 0])) <= f32_ulp, (name, x[b], y[b], y_ref[b, 0])
+            for i in range(tape.num_inputs):
+                assert ulp_diff_f32(g[b * tape.num_inputs + i], np.float32(g_ref[b, i])) <= f32_ulp, (name, x[b], i)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+def test_rosenbrock_n16_f64(hip):
+    """Extended Rosenbrock n=16, B=1024, x ~ U[-2,2]^16 from seed 0 (f64)."""
+    tape = wl.rosenbrock(16)
+    x = wl.points_uniform(0, 1024, 16, -2.0, 2.0)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    y, g = hip.gradient_batch(bufs, x, 1024)
+    y_ref, g_ref = po.gradient(tape, x)
+    assert rowrel(y.reshape(-1, 1), y_ref) <= F64_TOL
+    assert rowrel(g.reshape(1024, 16), g_ref) <= F64_TOL
+
+
+def test_rosenbrock_n16_f32(hip):
+    tape = wl.rosenbrock(16)
+    x = wl.points_uniform(0, 1024, 16, -2.0, 2.0)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    y, g = hip.gradient_batch(bufs, x.astype(np.float32), 1024)
+    y_ref, g_ref = po.gradient(tape, x.astype(np.float32).astype(np.float64))
+    assert rowrel(y.reshape(-1, 1), y_ref) <= F32_TOL
+    assert rowrel(g.reshape(1024, 16), g_ref) <= F32_TOL
+
+
+@pytest.fixture(scope="module")
+def mlp32():
+    return wl.tanh_mlp(32, 64, 1)
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+@pytest.mark.parametrize("wg,lds", [(64, 0), (256, 65536), (64, 4096)])
+def test_mlp32_f64(hip, mlp32, fuse, wg, lds):
+    """tanh MLP n=32, H=64 (seed 1); LDS-only, wide workgroup and HBM-spill register files."""
+    hip.set_fusion(fuse)
+    hip.set_launch(wg, lds)
+    try:
+        B = 3000
+        x = wl.points_normal(1, B, 32)
+        bufs = hip.upload_tape(GpuTapeData.from_tape_f64(mlp32))
+        y, g = hip.gradient_batch(bufs, x, B)
+        yf = hip.forward_batch(bufs, x, B)
+    finally:
+        hip.set_fusion(True)
+        hip.set_launch(0, 0)
+    y_ref, g_ref = po.gradient(mlp32, x)
+    assert np.array_equal(y, yf)
+    assert rowrel(y.reshape(-1, 1), y_ref) <= F64_TOL
+    assert rowrel(g.reshape(B, 32), g_ref) <= F64_TOL
+
+
+def test_mlp32_f32(hip, mlp32):
+    B = 4096
+    x = wl.points_normal(1, B, 32).astype(np.float32)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(mlp32))
+    y, g = hip.gradient_batch(bufs, x, B)
+    y_ref, g_ref = po.gradient(mlp32, x.astype(np.float64))
+    assert rowrel(y.reshape(-1, 1), y_ref) <= F32_TOL
+    assert rowrel(g.reshape(B, 32), g_ref) <= F32_TOL
+    # same-precision CPU path
+    y32, g32 = po.gradient(mlp32, x, dtype=np.float32)
+    assert rowrel(g.reshape(B, 32), g32.astype(np.float64)) <= F32_TOL
+
+
+def test_mlp32_full_batch_f32_device(hip, mlp32):
+    """B = 2^20 through the device-pointer entry point; oracle on a strided sample."""
+    torch = pytest.importorskip("torch")
+    B = 1 << 20
+    x = wl.points_normal(1, B, 32).astype(np.float32)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(mlp32))
+    dx = torch.from_numpy(x).cuda()
+    dy = torch.empty(B, dtype=torch.float32, device="cuda")
+    dg = torch.empty(B, 32, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    hip.gradient_batch_dev(bufs, dx, B, dy, dg)
+    hip_sync = torch.cuda.synchronize
+    hip_sync()
+    idx = np.arange(0, B, 997)
+    y_ref, g_ref = po.gradient(mlp32, x[idx].astype(np.float64))
+    g = dg.cpu().numpy()[idx]
+    y = dy.cpu().numpy()[idx]
+    assert rowrel(y.reshape(-1, 1), y_ref) <= F32_TOL
+    assert rowrel(g, g_ref) <= F32_TOL
+    # the last partial workgroup is handled
+    g_last_ref = po.gradient(mlp32, x[-3:].astype(np.float64))[1]
+    assert rowrel(dg.cpu().numpy()[-3:], g_last_ref) <= F32_TOL
+
+
+def test_random_dags(hip):
+    for k in range(4):
+        tape = wl.random_dag(3, k, 400 + 300 * k, 6 + 5 * k)
+        x = wl.points_uniform(10 + k, 257, tape.num_inputs, -1.0, 1.0)
+        bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+        y, g = hip.gradient_batch(bufs, x, 257)
+        y_ref, g_ref = po.gradient(tape, x)
+        assert rowrel(y.reshape(-1, 1), y_ref) <= 1e-11, k
+        assert rowrel(g.reshape(257, -1), g_ref) <= 1e-11, k
+
+
+def test_multi_output_forward(hip):
+    from echidna_amd.tape import record_multi
+    tape, _ = record_multi(lambda v: [v[0] * v[1], v[0].sin() + v[1], v[1]], [1.0, 2.0])
+    x = np.array([[0.5, 2.0], [1.5, -1.0], [3.0, 0.25]])
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    assert hip.num_outputs(bufs) == 3
+    y = hip.forward_batch(bufs, x, 3).reshape(3, 3)
+    y_ref = po.forward(tape, x)
+    assert rowrel(y, y_ref) <= F64_TOL
+    # gradient seeds output 0
+    _, g = hip.gradient_batch(bufs, x, 3)
+    assert rowrel(g.reshape(3, 2), po.gradient(tape, x)[1]) <= F64_TOL
+
+
+# ---------------------------------------------------------------------------
+# validation and error paths
+# ---------------------------------------------------------------------------
+
+def _tiny():
+    return GpuTapeData(opcodes=np.array([0, 1, 4], np.uint32), arg0=np.array([0xFFFFFFFF, 0xFFFFFFFF, 0], np.uint32),
+                       arg1=np.array([0xFFFFFFFF, 0xFFFFFFFF, 1], np.uint32),
+                       constants=np.array([0.0, 3.0, 0.0], np.float32), num_ops=3, num_inputs=1, num_variables=3,
+                       output_index=2, output_indices=np.array([2], np.uint32))
+
+
+def test_upload_rejects_invalid(hip):
+    hip.upload_tape(_tiny())
+    d = _tiny(); d.arg0[2] = 9999
+    with pytest.raises(GpuError, match="invalid GpuTapeData"):
+        hip.upload_tape(d)
+    d = _tiny(); d.opcodes[2] = 24; d.arg1[2] = 9999
+    with pytest.raises(GpuError, match="invalid GpuTapeData"):
+        hip.upload_tape(d)
+    d = _tiny(); d.output_index = 0xFFFFFFFF; d.output_indices = np.zeros(0, np.uint32)
+    with pytest.raises(GpuError, match="invalid GpuTapeData"):
+        hip.upload_tape(d)
+    d = _tiny(); d.output_indices = np.array([3], np.uint32)
+    with pytest.raises(GpuError, match="invalid GpuTapeData"):
+        hip.upload_tape(d)
+    d = _tiny(); d.opcodes[2] = 43
+    with pytest.raises(GpuError) as e:
+        hip.upload_tape(d)
+    assert e.value.code == GpuError.CUSTOM_OPS_NOT_SUPPORTED
+
+
+def test_batch_argument_errors(hip):
+    bufs = hip.upload_tape(_tiny())
+    with pytest.raises(GpuError, match="batch_size must be non-zero"):
+        hip.forward_batch(bufs, np.zeros(0, np.float32), 0)
+    tape, _ = record(lambda v: v[0] * 0.0 + 2.0, [1.0])
+    const_tape = GpuTapeData.from_tape(tape)
+    const_tape.opcodes = const_tape.opcodes[1:]
+    # a zero-input tape: a single Const output
+    z = GpuTapeData(opcodes=np.array([1], np.uint32), arg0=np.array([0xFFFFFFFF], np.uint32),
+                    arg1=np.array([0xFFFFFFFF], np.uint32), constants=np.array([2.0], np.float32), num_ops=1,
+                    num_inputs=0, num_variables=1, output_index=0, output_indices=np.array([0], np.uint32))
+    zb = hip.upload_tape(z)
+    with pytest.raises(GpuError, match="no inputs"):
+        hip.gradient_batch(zb, np.zeros(1, np.float32), 1)
+
+
+def test_gradient_of_input_and_const_outputs(hip):
+    # output is an input slot: gradient is e_0; output Const: gradient 0
+    tape, _ = record(lambda v: v[0], [1.0, 2.0])
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    y, g = hip.gradient_batch(bufs, np.array([[3.0, 4.0]]), 1)
+    assert y[0] == 3.0 and list(g) == [1.0, 0.0]
+    tape, _ = record(lambda v: v[0] * 0.0 + v[1] * 0.0 + 5.0, [1.0, 2.0])
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    y, g = hip.gradient_batch(bufs, np.array([[3.0, 4.0]]), 1)
+    y_ref, g_ref = po.gradient(tape, np.array([[3.0, 4.0]]))
+    assert y[0] == y_ref[0, 0] and list(g) == list(g_ref[0])

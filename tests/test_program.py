@@ -1,0 +1,89 @@
+"""The upload-time compiler, checked on the CPU: the compiled register program
+(emulated in numpy exactly as the scalar HIP interpreter executes it) must
+reproduce the oracle's values and gradients, with and without dot fusion."""
+
+import numpy as np
+import pytest
+
+import program_emu as emu
+import pyoracle as po
+from echidna_amd import workloads as wl
+from echidna_amd.backend import compile_program
+from echidna_amd.tape import GpuTapeData, record, record_multi
+
+TAPES = {
+    "rosenbrock16": lambda: wl.rosenbrock(16),
+    "mlp32": lambda: wl.tanh_mlp(32, 64, 1),
+    "mlp2": lambda: wl.tanh_mlp2(12, 9, 7, 4),
+    "dag0": lambda: wl.random_dag(3, 0, 300, 12),
+    "dag1": lambda: wl.random_dag(3, 1, 600, 30),
+    "square_same_operand": lambda: record(lambda v: v[0] * v[0] + v[0] * v[1], [1.0, 2.0])[0],
+    "output_is_input": lambda: record(lambda v: v[1], [1.0, 2.0])[0],
+}
+
+
+@pytest.mark.parametrize("name", list(TAPES))
+@pytest.mark.parametrize("fuse", [True, False])
+@pytest.mark.parametrize("f64", [True, False])
+def test_program_matches_oracle(name, fuse, f64):
+    tape = TAPES[name]()
+    d = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
+    x = wl.points_normal(5, 9, tape.num_inputs) * 0.7
+    F = np.float64 if f64 else np.float32
+    x = x.astype(F)
+    y_ref, g_ref = po.gradient(tape, x, dtype=F)
+    for mode in (0, 1):
+        words, terms, info = compile_program(d, mode=mode, fuse=fuse, f64=f64)
+        y, g = emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64)
+        tol = 1e-13 if f64 else 2e-5
+        scale = np.maximum(np.abs(y_ref), 1.0)
+        assert np.all(np.abs(y - y_ref) <= tol * scale), (name, mode)
+        if mode == 1:
+            gs = np.maximum(np.max(np.abs(g_ref), axis=1, keepdims=True), 1.0)
+            assert np.all(np.abs(g - g_ref) <= tol * gs), (name, mode)
+
+
+def test_fusion_collapses_mlp_dot_products():
+    tape = wl.tanh_mlp(32, 64, 1)
+    d = GpuTapeData.from_tape(tape)
+    _, _, plain = compile_program(d, mode=1, fuse=False)
+    _, _, fused = compile_program(d, mode=1, fuse=True)
+    assert plain["instructions"] > 8000
+    assert fused["instructions"] < 500
+    assert fused["registers"] <= plain["registers"]
+
+
+def test_constants_become_immediates_and_dead_code_is_dropped():
+    # a dead branch (v[1].sin()) and per-use constants: only live ops remain
+    tape, _ = record(lambda v: (v[1].sin(), v[0] * 2.0 + 3.0)[1], [1.0, 2.0])
+    words, _, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=0, fuse=False)
+    ops = [int(w[0]) & 0xFF for w in words]
+    assert 24 not in ops  # no Sin
+    assert ops.count(1) == 0  # no Const instructions
+    assert info["registers"] <= 3
+
+
+def test_register_reuse_bounds_live_set():
+    # a long chain keeps O(1) values alive
+    def f(v):
+        s = v[0]
+        for _ in range(500):
+            s = (s * 0.999).sin()
+        return s
+    tape, _ = record(f, [0.3])
+    _, _, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=0)
+    assert info["registers"] <= 3
+    x = np.array([[0.3], [1.1]])
+    words, terms, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=1)
+    y, g = emu.run(words, terms, x, 1, 1, info["registers"])
+    y_ref, g_ref = po.gradient(tape, x)
+    assert np.allclose(y, y_ref, rtol=1e-13) and np.allclose(g, g_ref, rtol=1e-12)
+
+
+def test_multi_output_program():
+    tape, _ = record_multi(lambda v: [v[0] * v[1], v[0].sin() + v[1], v[1]], [1.0, 2.0])
+    words, terms, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=1)
+    x = np.array([[0.5, 2.0], [1.5, -1.0]])
+    y, g = emu.run(words, terms, x, 2, 3, info["registers"])
+    y_ref, g_ref = po.gradient(tape, x)
+    assert np.allclose(y, y_ref, rtol=1e-14) and np.allclose(g, g_ref, rtol=1e-14)
