@@ -93,11 +93,15 @@ def main() -> int:
     dx = torch.from_numpy(x.astype(np.float64 if f64 else np.float32)).cuda()
     dy = torch.empty(B, dtype=tdt, device="cuda")
     dg = torch.empty(B, n, dtype=tdt, device="cuda")
-    stream = torch.cuda.current_stream()
+    # A dedicated stream: torch's default stream has handle 0, which the
+    # library reads as "its own stream"; events must sit on the launch stream.
+    stream = torch.cuda.Stream()
     sh = stream.cuda_stream
+    torch.cuda.synchronize()
 
-    for _ in range(args.warmup):
-        ctx.gradient_batch_dev(bufs, dx, B, dy, dg, stream=sh)
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            ctx.gradient_batch_dev(bufs, dx, B, dy, dg, stream=sh)
     torch.cuda.synchronize()
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]

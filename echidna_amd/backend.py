@@ -62,6 +62,13 @@ _SIGS = {
                                               ctypes.c_int32, ctypes.c_int32, _c_u32p, ctypes.c_uint64, _c_u32p,
                                               ctypes.c_uint64, _c_u64p]),
     "ehip_set_launch": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
+    "ehip_laplacian": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint32, _c_f64p]),
+    "ehip_laplacian_with_control": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _c_f64p]),
+    "ehip_hessian_diagonal": (ctypes.c_int32, [_vp, _vp, _vp, _vp, _vp]),
+    "ehip_laplacian_batch": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_uint64, _vp, _c_f64p]),
+    "ehip_laplacian_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint64, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "ehip_set_fusion": (ctypes.c_int32, [_vp, ctypes.c_int32]),
 }
 
@@ -75,6 +82,13 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: run echidna_amd/build.py (or __graft_entry__.build()) first")
+    # torch ships its own libamdhip64.so.7 with the same SONAME as ROCm's; the
+    # first one loaded serves the whole process.  Load torch's first so device
+    # buffers and streams from torch and this library share one HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)

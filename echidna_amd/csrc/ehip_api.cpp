@@ -82,6 +82,7 @@ struct ehip_ctx {
   std::mutex host_mu;  // serializes host-pointer calls (staging buffers)
   DevBuf regs;                 // HBM register file
   DevBuf io[6];                // staging for host-pointer entry points
+  DevBuf jets;                 // STDE per-sample jets
   uint32_t wg = 64;
   uint32_t lds_budget = 32768;
   bool fuse = true;
@@ -201,6 +202,7 @@ void ehip_ctx_destroy(ehip_ctx* c) {
     (void)hipStreamDestroy(c->stream);
   }
   c->regs.release();
+  c->jets.release();
   for (auto& b : c->io) b.release();
   delete c;
 }
@@ -562,6 +564,129 @@ int32_t ehip_taylor_kth_batch(ehip_ctx* c, const ehip_tape* t, const void* prima
   for (size_t i = 0; i < total_out; ++i)
     for (uint32_t k = 0; k < order; ++k)
       std::memcpy(dst + ((size_t)k * total_out + i) * h.S, raw.data() + (i * order + k) * h.S, h.S);
+  return ok();
+}
+
+// ------------------------------- GPU STDE ----------------------------------
+
+int32_t ehip_laplacian_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t npoints, uint32_t S,
+                                 uint32_t dir_kind, uint64_t seed, const void* d_dirs, const void* d_diag,
+                                 uint64_t diag_stride, double* d_results, void* stream) {
+  if (!c || !t || !d_x || !d_results) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (t->nout != 1)
+    return fail(EHIP_OTHER, "GPU error: laplacian_gpu requires a scalar-output tape; got %u outputs", t->nout);
+  if (t->nin == 0)
+    return fail(EHIP_OTHER, "GPU error: laplacian_gpu requires a non-empty x (a zero-input tape has no Laplacian); "
+                            "zero-sized GPU buffers would panic");
+  if (S == 0) return fail(EHIP_OTHER, "GPU error: no directions provided");
+  if (dir_kind > 2) return fail(EHIP_OTHER, "GPU error: dir_kind must be 0 (supplied), 1 (Rademacher) or 2 (Gaussian)");
+  if (dir_kind == 0 && !d_dirs) return fail(EHIP_OTHER, "GPU error: null directions");
+  if (npoints == 0) return ok();
+  if (d_diag && diag_stride != 0 && diag_stride != t->nin)
+    return fail(EHIP_OTHER, "GPU error: diag_stride must be 0 or num_inputs");
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = pick(c, stream);
+  const size_t Sb = t->f64 ? 8 : 4;
+  const uint64_t lanes = npoints * (uint64_t)S;
+  HIP_TRY(c->jets.reserve(lanes * 3 * Sb), "jet scratch");
+  Launch L{};
+  L.nin = t->nin;
+  L.nout = 1;
+  L.B = lanes;
+  L.x = d_x;
+  L.xstride = t->nin;
+  L.S = S;
+  L.rng = dir_kind;
+  L.seed = seed;
+  L.dirs = d_dirs;
+  L.y = c->jets.p;
+  int32_t rc = run(c, t, 2, 3, L, s);
+  if (rc) return rc;
+  StdeReduce R{};
+  R.jets = c->jets.p;
+  R.npoints = npoints;
+  R.S = S;
+  R.nin = t->nin;
+  R.rng = dir_kind;
+  R.seed = seed;
+  R.dirs = d_dirs;
+  R.diag = d_diag;
+  R.diag_stride = diag_stride;
+  R.out = d_results;
+  HIP_TRY(launch_stde_reduce(t->f64, R, s), "stde reduce");
+  return ok();
+}
+
+static int32_t laplacian_host(ehip_ctx* c, const ehip_tape* t, const void* x, uint64_t npoints, const void* dirs,
+                              uint32_t S, uint32_t dir_kind, uint64_t seed, const void* diag, uint64_t diag_rows,
+                              double* results) {
+  if (!c || !t || !x || !results) return fail(EHIP_OTHER, "GPU error: null argument");
+  std::lock_guard<std::mutex> hlk(c->host_mu);
+  const size_t Sb = t->f64 ? 8 : 4;
+  void *dx, *dv = nullptr, *dd = nullptr, *dr;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    HostCall h{c, Sb};
+    TRY(h.stage(0, x, npoints * t->nin * Sb, &dx));
+    if (dir_kind == 0 && dirs) TRY(h.stage(1, dirs, npoints * (uint64_t)S * t->nin * Sb, &dv));
+    if (diag) TRY(h.stage(2, diag, diag_rows * t->nin * Sb, &dd));
+    TRY(h.stage(3, nullptr, npoints * 5 * sizeof(double), &dr));
+  }
+  TRY(ehip_laplacian_batch_dev(c, t, dx, npoints, S, dir_kind, seed, dv, dd, diag_rows > 1 ? t->nin : 0,
+                               (double*)dr, nullptr));
+  std::lock_guard<std::mutex> lk(c->mu);
+  HostCall h{c, Sb};
+  TRY(h.fetch(results, dr, npoints * 5 * sizeof(double)));
+  TRY(h.sync());
+  return ok();
+}
+
+int32_t ehip_laplacian(ehip_ctx* c, const ehip_tape* t, const void* x, const void* dirs, uint32_t S,
+                       double* result5) {
+  if (S > 0 && !dirs) return fail(EHIP_OTHER, "GPU error: null directions");
+  return laplacian_host(c, t, x, 1, dirs, S, 0, 0, nullptr, 0, result5);
+}
+
+int32_t ehip_laplacian_with_control(ehip_ctx* c, const ehip_tape* t, const void* x, const void* dirs, uint32_t S,
+                                    const void* control_diagonal, double* result5) {
+  if (!control_diagonal) return fail(EHIP_OTHER, "GPU error: null control diagonal");
+  if (S > 0 && !dirs) return fail(EHIP_OTHER, "GPU error: null directions");
+  return laplacian_host(c, t, x, 1, dirs, S, 0, 0, control_diagonal, 1, result5);
+}
+
+int32_t ehip_laplacian_batch(ehip_ctx* c, const ehip_tape* t, const void* xs, uint64_t npoints, uint32_t S,
+                             uint32_t dir_kind, uint64_t seed, const void* dirs, double* results) {
+  return laplacian_host(c, t, xs, npoints, dirs, S, dir_kind, seed, nullptr, 0, results);
+}
+
+int32_t ehip_hessian_diagonal(ehip_ctx* c, const ehip_tape* t, const void* x, void* value, void* diag) {
+  if (!c || !t || !x || !value || !diag) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (t->nout != 1)
+    return fail(EHIP_OTHER, "GPU error: hessian_diagonal_gpu requires a scalar-output tape; got %u outputs", t->nout);
+  if (t->nin == 0)
+    return fail(EHIP_OTHER, "GPU error: hessian_diagonal_gpu requires a non-empty x (a zero-input tape has no "
+                            "Hessian diagonal); zero-sized GPU buffers would panic");
+  const uint32_t n = t->nin;
+  const size_t Sb = t->f64 ? 8 : 4;
+  std::vector<unsigned char> xs((size_t)n * n * Sb), seeds((size_t)n * n * Sb, 0), out((size_t)3 * n * Sb);
+  for (uint32_t j = 0; j < n; ++j) {
+    std::memcpy(xs.data() + (size_t)j * n * Sb, x, n * Sb);
+    if (t->f64) ((double*)seeds.data())[(size_t)j * n + j] = 1.0;
+    else ((float*)seeds.data())[(size_t)j * n + j] = 1.0f;
+  }
+  TRY(ehip_taylor_kth_batch(c, t, xs.data(), seeds.data(), n, 3, out.data()));
+  // diag(H)[j] = 2 * c2[j]  (src/gpu/stde_gpu.rs:103-129)
+  if (t->f64) {
+    const double* o = (const double*)out.data();
+    *(double*)value = o[0];
+    for (uint32_t j = 0; j < n; ++j) ((double*)diag)[j] = 2.0 * o[2 * n + j];
+  } else {
+    const float* o = (const float*)out.data();
+    *(float*)value = o[0];
+    for (uint32_t j = 0; j < n; ++j) ((float*)diag)[j] = 2.0f * o[2 * n + j];
+  }
   return ok();
 }
 

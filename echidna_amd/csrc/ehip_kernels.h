@@ -19,7 +19,25 @@ struct Launch {
   void* g;                // gradients
   void* h;                // Hessian-vector products
   void* gregs;            // HBM register file for registers >= R_lds
+  uint32_t S;             // lanes per x row (0: one row per lane)
+  uint32_t rng;           // RNG_NONE: read dirs; else generate (philox.h)
+  uint64_t seed;
 };
+
+// Per-point STDE reduction over S samples (f64 Welford/Chan).
+struct StdeReduce {
+  const void* jets;       // [npoints*S][3] interleaved c0,c1,c2 (scalar output)
+  uint64_t npoints;
+  uint32_t S, nin;
+  uint32_t rng;           // how directions were made (for control variates)
+  uint64_t seed;
+  const void* dirs;       // [npoints*S][nin] when rng == RNG_NONE
+  const void* diag;       // diagonal control variate [.. ][nin] or null
+  uint64_t diag_stride;   // 0: one diagonal shared by all points
+  double* out;            // [npoints][5]: value, estimate, variance, std error, count
+};
+
+hipError_t launch_stde_reduce(bool f64, const StdeReduce& R, hipStream_t s);
 
 hipError_t launch_scalar(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);
 hipError_t launch_dual(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);

@@ -16,6 +16,7 @@
 #include "ehip_kernels.h"
 #include "ehip_math.h"
 #include "ehip_program.h"
+#include "philox.h"
 
 namespace ehip {
 
@@ -253,7 +254,11 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
       }
     } else if (op == I_LOAD) {
       Dual<F> v = dc(F(0));
-      if (act) { v.re = X[p * L.xstride + w2]; v.eps = V[p * L.nin + w2]; }
+      if (act) {
+        const uint64_t row = L.S ? p / L.S : p;
+        v.re = X[row * L.xstride + w2];
+        v.eps = L.rng ? (F)philox_direction(L.rng, L.seed, p * L.nin + w2) : V[p * L.nin + w2];
+      }
       wrd(w1, v);
     } else if (op == I_SEED) {
       wrd(w1, dc(F(1)));
@@ -314,8 +319,10 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
     } else if (op == I_LOAD) {
       Jet<F, K> v = jc<F, K>(F(0));
       if (act) {
-        v.c[0] = X[p * L.xstride + w2];
-        if (K > 1) v.c[K > 1 ? 1 : 0] = V[p * L.nin + w2];
+        const uint64_t row = L.S ? p / L.S : p;
+        v.c[0] = X[row * L.xstride + w2];
+        if (K > 1)
+          v.c[K > 1 ? 1 : 0] = L.rng ? (F)philox_direction(L.rng, L.seed, p * L.nin + w2) : V[p * L.nin + w2];
       }
       R.wr(w1, v.c);
     } else if (op == I_STORE_OUT) {

@@ -1,0 +1,107 @@
+// Per-point STDE statistics on the device, in binary64.
+//
+// The reference reduces the Taylor samples on the host in f32 with
+// WelfordAccumulator (src/stde/types.rs:44-113, src/gpu/stde_gpu.rs:84-97,
+// 135-200).  Here one wavefront owns one point: each lane runs Welford over
+// its strided share of the S samples (skipping non-finite samples, as the
+// reference does), then the 64 partial (count, mean, M2) triples merge with
+// Chan's pairwise update through cross-lane shuffles.  Finalisation follows
+// WelfordAccumulator::finalize: NaN estimate when nothing contributed,
+// variance clamped at zero, standard error sqrt(var / n).
+//
+// Samples: plain STDE uses 2*c2 (= v^T H v); the diagonal control variate
+// uses the residual 2*c2 - sum_j d_j v_j^2 with the exact tr(D) added to the
+// mean afterwards (laplacian_with_control_gpu's form).
+#include <hip/hip_runtime.h>
+
+#include "ehip_kernels.h"
+#include "philox.h"
+
+namespace ehip {
+
+struct Wf {
+  double n, mean, m2;
+};
+
+__device__ __forceinline__ Wf chan_merge(Wf a, Wf b) {
+  if (b.n == 0.0) return a;
+  if (a.n == 0.0) return b;
+  double n = a.n + b.n;
+  double delta = b.mean - a.mean;
+  Wf r;
+  r.n = n;
+  r.mean = a.mean + delta * (b.n / n);
+  r.m2 = a.m2 + b.m2 + delta * delta * (a.n * b.n / n);
+  return r;
+}
+
+template <class F>
+__global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
+  const uint64_t pt = blockIdx.x;
+  if (pt >= R.npoints) return;
+  const uint32_t lane = threadIdx.x;
+  const F* J = reinterpret_cast<const F*>(R.jets);
+  const F* D = reinterpret_cast<const F*>(R.dirs);
+  const F* DG = reinterpret_cast<const F*>(R.diag);
+  Wf w{0.0, 0.0, 0.0};
+  for (uint32_t s = lane; s < R.S; s += 64) {
+    const uint64_t row = pt * R.S + s;
+    double sample = 2.0 * (double)J[row * 3 + 2];
+    if (DG) {
+      const F* d = DG + pt * R.diag_stride;
+      double cv = 0.0;
+      for (uint32_t j = 0; j < R.nin; ++j) {
+        double v = R.rng ? philox_direction(R.rng, R.seed, row * R.nin + j) : (double)D[row * R.nin + j];
+        cv += (double)d[j] * v * v;
+      }
+      sample -= cv;
+    }
+    if (!isfinite(sample)) continue;
+    w.n += 1.0;
+    double delta = sample - w.mean;
+    w.mean += delta / w.n;
+    w.m2 += delta * (sample - w.mean);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    Wf o;
+    o.n = __shfl_xor(w.n, off, 64);
+    o.mean = __shfl_xor(w.mean, off, 64);
+    o.m2 = __shfl_xor(w.m2, off, 64);
+    w = chan_merge(w, o);
+  }
+  if (lane == 0) {
+    double* out = R.out + pt * 5;
+    double tr = 0.0;
+    if (DG) {
+      const F* d = DG + pt * R.diag_stride;
+      for (uint32_t j = 0; j < R.nin; ++j) tr += (double)d[j];
+    }
+    out[0] = (double)J[(pt * R.S) * 3 + 0];
+    if (w.n == 0.0) {
+      out[1] = __builtin_nan("");
+      out[2] = 0.0;
+      out[3] = 0.0;
+    } else if (w.n > 1.0) {
+      double var = w.m2 / (w.n - 1.0);
+      if (!(var > 0.0)) var = 0.0;
+      out[1] = w.mean + tr;
+      out[2] = var;
+      out[3] = sqrt(var / w.n);
+    } else {
+      out[1] = w.mean + tr;
+      out[2] = 0.0;
+      out[3] = 0.0;
+    }
+    out[4] = w.n;
+  }
+}
+
+hipError_t launch_stde_reduce(bool f64, const StdeReduce& R, hipStream_t s) {
+  if (R.npoints == 0) return hipSuccess;
+  void* args[] = {const_cast<StdeReduce*>(&R)};
+  const void* fn = f64 ? (const void*)k_stde_reduce<double> : (const void*)k_stde_reduce<float>;
+  return hipLaunchKernel(fn, dim3((uint32_t)R.npoints), dim3(64), args, 0, s);
+}
+
+}  // namespace ehip

@@ -128,6 +128,38 @@ int32_t ehip_taylor_kth_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const vo
                                   const void* d_dirs, uint64_t batch, uint32_t order, void* d_coeffs,
                                   void* stream);
 
+/* ---- GPU STDE (src/gpu/stde_gpu.rs) ----
+ * Results are binary64, reduced on the device: result5 = {value, estimate,
+ * sample_variance, standard_error, num_samples} (EstimatorResult,
+ * src/stde/types.rs:9-22).  All require a scalar-output tape. */
+
+/* laplacian_gpu (src/gpu/stde_gpu.rs:51-81): one point x [n], S directions
+ * [S*n]; samples 2*c2 through the Welford accumulator. */
+int32_t ehip_laplacian(ehip_ctx* ctx, const ehip_tape* tape, const void* x, const void* dirs, uint32_t S,
+                       double* result5);
+
+/* laplacian_with_control_gpu (src/gpu/stde_gpu.rs:135-200): residual
+ * 2*c2 - sum_j d_j v_j^2, estimate tr(D) + mean. */
+int32_t ehip_laplacian_with_control(ehip_ctx* ctx, const ehip_tape* tape, const void* x, const void* dirs,
+                                    uint32_t S, const void* control_diagonal, double* result5);
+
+/* hessian_diagonal_gpu (src/gpu/stde_gpu.rs:103-129): n basis jets,
+ * value [1] and diag [n] in the tape's scalar type. */
+int32_t ehip_hessian_diagonal(ehip_ctx* ctx, const ehip_tape* tape, const void* x, void* value, void* diag);
+
+/* Batched STDE over many points (no reference counterpart; the reference
+ * handles one x per call): xs [npoints*n], S directions per point either
+ * supplied (dir_kind 0, dirs [npoints*S*n]) or drawn on the device from the
+ * Philox stream `seed` (1 Rademacher, 2 Gaussian; component i of direction s
+ * of point p is draw index (p*S+s)*n+i).  results [npoints*5]. */
+int32_t ehip_laplacian_batch(ehip_ctx* ctx, const ehip_tape* tape, const void* xs, uint64_t npoints, uint32_t S,
+                             uint32_t dir_kind, uint64_t seed, const void* dirs, double* results);
+/* Device-pointer form; d_diag (optional) is a diagonal control variate with
+ * row stride diag_stride (0 = shared). */
+int32_t ehip_laplacian_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const void* d_xs, uint64_t npoints,
+                                 uint32_t S, uint32_t dir_kind, uint64_t seed, const void* d_dirs,
+                                 const void* d_diag, uint64_t diag_stride, double* d_results, void* stream);
+
 /* Introspection of the compiled register program (for tests and the
  * traffic model): mode 0 forward, 1 gradient/HVP. out[0]=instructions,
  * out[1]=registers, out[2]=program words, out[3]=fused dot terms. */

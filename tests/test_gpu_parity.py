@@ -23,8 +23,15 @@ F32_TOL = 2e-5
 
 
 def rowrel(gpu, ref):
+    """Max |gpu - ref| over max(|ref|, row max |ref|, batch RMS of ref).
+
+    The batch-RMS floor makes scalar outputs that cancel to near zero (a sum
+    of 64 terms of size ~0.1) be judged against their typical magnitude.
+    """
     gpu = np.asarray(gpu, np.float64).reshape(ref.shape)
+    rms = np.sqrt(np.mean(ref[np.isfinite(ref)] ** 2)) if np.isfinite(ref).any() else 1.0
     scale = np.maximum(np.abs(ref), np.max(np.abs(ref), axis=-1, keepdims=True))
+    scale = np.maximum(scale, rms)
     scale = np.where(scale == 0, 1.0, scale)
     return np.max(np.abs(gpu - ref) / scale)
 
@@ -214,16 +221,13 @@ def test_batch_argument_errors(hip):
     bufs = hip.upload_tape(_tiny())
     with pytest.raises(GpuError, match="batch_size must be non-zero"):
         hip.forward_batch(bufs, np.zeros(0, np.float32), 0)
-    tape, _ = record(lambda v: v[0] * 0.0 + 2.0, [1.0])
-    const_tape = GpuTapeData.from_tape(tape)
-    const_tape.opcodes = const_tape.opcodes[1:]
     # a zero-input tape: a single Const output
     z = GpuTapeData(opcodes=np.array([1], np.uint32), arg0=np.array([0xFFFFFFFF], np.uint32),
                     arg1=np.array([0xFFFFFFFF], np.uint32), constants=np.array([2.0], np.float32), num_ops=1,
                     num_inputs=0, num_variables=1, output_index=0, output_indices=np.array([0], np.uint32))
     zb = hip.upload_tape(z)
     with pytest.raises(GpuError, match="no inputs"):
-        hip.gradient_batch(zb, np.zeros(1, np.float32), 1)
+        hip.gradient_batch(zb, np.zeros(0, np.float32), 1)
 
 
 def test_gradient_of_input_and_const_outputs(hip):
