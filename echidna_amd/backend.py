@@ -145,16 +145,17 @@ def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Op
     rc = lib.ehip_compile_program(*ptrs, data.num_ops, data.num_inputs, data.output_index,
                                   outs.ctypes.data_as(_c_u32p), len(outs), mode, int(fuse),
                                   None, 0, None, 0, info)
-    nw, nt = int(info[2]), int(info[3])
+    nw, T = int(info[2]), int(info[3])
+    nt = max(4, T * (4 if f64 else 3))
     words = np.zeros(max(nw, 8), dtype=np.uint32)
-    terms = np.zeros(max(nt * 4, 4), dtype=np.uint32)
+    terms = np.zeros(nt, dtype=np.uint32)
     rc = lib.ehip_compile_program(*ptrs, data.num_ops, data.num_inputs, data.output_index,
                                   outs.ctypes.data_as(_c_u32p), len(outs), mode, int(fuse),
                                   words.ctypes.data_as(_c_u32p), len(words), terms.ctypes.data_as(_c_u32p),
                                   len(terms), info)
     _check(rc)
-    return (words[:nw].reshape(-1, 8), terms[: nt * 4].reshape(-1, 4),
-            {"instructions": int(info[0]), "registers": int(info[1]), "words": nw, "dot_terms": nt})
+    info2 = {"instructions": int(info[0]), "registers": int(info[1]), "words": nw, "nterms": T}
+    return words[:nw].reshape(-1, 8), terms[:nt], info2
 
 
 class HipTapeBuffers:

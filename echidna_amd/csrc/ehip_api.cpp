@@ -135,6 +135,7 @@ int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStrea
   L.prog = dp.d_words;
   L.terms = dp.d_terms;
   L.ninstr = dp.host.ninstr;
+  L.nterms = dp.host.nterms;
   const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
   const size_t S = t->f64 ? 8 : 4;
   const uint32_t wg = c->wg;
@@ -331,7 +332,7 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
   out4[0] = p.ninstr;
   out4[1] = p.nregs;
   out4[2] = p.words.size();
-  out4[3] = p.terms.size() / TW;
+  out4[3] = p.nterms;
   if (p.words.size() > words_cap || p.terms.size() > terms_cap || !words || !terms)
     return fail(EHIP_OTHER, "GPU error: program buffers too small");
   std::memcpy(words, p.words.data(), p.words.size() * 4);

@@ -7,8 +7,9 @@ namespace ehip {
 
 struct Launch {
   const uint32_t* prog;   // NW words per instruction (device)
-  const uint32_t* terms;  // DOT side table (device)
+  const uint32_t* terms;  // side table (device): meta[T] | reg[T] | imm[T*IW]
   uint32_t ninstr;
+  uint32_t nterms;        // T
   uint32_t R_lds;         // registers held in LDS
   uint32_t nin, nout;
   uint64_t B;             // points (lanes) in this launch

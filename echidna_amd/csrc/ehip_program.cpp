@@ -277,31 +277,91 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
     return o;
   };
 
-  for (uint32_t i = 0; i < ni; ++i) {
-    if (!need[i]) continue;
+  {
+    // One batched load of every needed input: the loads issue together.
     AIns x;
-    x.op = I_LOAD;
-    x.dst = VAL(i);
-    x.aux = i;
-    prog.push_back(x);
+    x.op = I_LOADN;
+    for (uint32_t i = 0; i < ni; ++i) {
+      if (!need[i]) continue;
+      ATerm at;
+      at.kind = T_REG;
+      at.vid = VAL(i);
+      at.imm = (double)i;  // input index
+      x.terms.push_back(at);
+    }
+    if (!x.terms.empty()) prog.push_back(std::move(x));
+  }
+
+  // ---- DOT piece schedule ----
+  // A fused chain is emitted as one or more DOT pieces.  A term whose operand
+  // is a single-use computed value may be accumulated right after that value
+  // is produced (its register is then freed at once), which matters for
+  // read-out sums over a whole layer; other terms wait for the chain's own
+  // position.  Term order, and so every rounding, is unchanged.
+  std::vector<std::vector<std::pair<uint32_t, std::pair<uint32_t, uint32_t>>>> pieces_at(N + 1);  // pos+1 -> (root, [lo,hi))
+  std::vector<char> split_root(N, 0);
+  if (fuse) {
+    for (uint32_t r = ni; r < N; ++r) {
+      if (!need[r] || is_const(r) || absorbed[r]) continue;
+      const Chain* c = dot_root(r);
+      if (!c) continue;
+      const size_t T = c->terms.size();
+      std::vector<int64_t> q(T, (int64_t)r);
+      std::vector<char> is_imm(T, 0);
+      for (size_t k = 0; k < T; ++k) {
+        const Term& tm = c->terms[k];
+        if (tm.kind == T_IMM) { is_imm[k] = 1; continue; }
+        const uint32_t s_ = tm.slot;
+        bool hoist = s_ >= ni && uses[s_] == 1 && !is_out[s_] && !absorbed[s_];
+        q[k] = hoist ? (int64_t)s_ : (int64_t)r;
+      }
+      // immediates follow the next operand term (or the previous one)
+      for (size_t k = 0; k < T; ++k) {
+        if (!is_imm[k]) continue;
+        int64_t v = -2;
+        for (size_t j = k + 1; j < T; ++j)
+          if (!is_imm[j]) { v = q[j]; break; }
+        if (v == -2) v = (k > 0) ? q[k - 1] : (int64_t)r;
+        q[k] = v;
+      }
+      for (size_t k = 1; k < T; ++k) q[k] = std::max(q[k], q[k - 1]);
+      size_t lo = 0;
+      for (size_t k = 1; k <= T; ++k) {
+        if (k == T || q[k] != q[lo]) {
+          pieces_at[(size_t)(q[lo] + 1)].push_back({r, {(uint32_t)lo, (uint32_t)k}});
+          lo = k;
+        }
+      }
+      split_root[r] = 1;
+    }
   }
   uint64_t dot_terms = 0;
+  auto emit_piece = [&](uint32_t r, uint32_t lo, uint32_t hi) {
+    const Chain& c = chains[chain_of[r]];
+    AIns x;
+    x.op = I_DOT;
+    x.dst = VAL(r);
+    if (lo > 0) x.flags |= F_DOT_CONT;
+    for (uint32_t k = lo; k < hi; ++k) {
+      const Term& tm = c.terms[k];
+      ATerm at;
+      at.kind = tm.kind;
+      at.imm = tm.imm;
+      at.const_left = tm.const_left;
+      if (tm.kind != T_IMM) at.vid = VAL(tm.slot);
+      x.terms.push_back(at);
+    }
+    dot_terms += hi - lo;
+    prog.push_back(std::move(x));
+  };
+  auto flush_pieces = [&](int64_t pos) {
+    for (auto& pc : pieces_at[(size_t)(pos + 1)]) emit_piece(pc.first, pc.second.first, pc.second.second);
+  };
+  flush_pieces(-1);
   for (uint32_t i = ni; i < N; ++i) {
     if (!need[i] || is_const(i) || absorbed[i]) continue;
-    if (const Chain* c = dot_root(i)) {
-      AIns x;
-      x.op = I_DOT;
-      x.dst = VAL(i);
-      for (const Term& tm : c->terms) {
-        ATerm at;
-        at.kind = tm.kind;
-        at.imm = tm.imm;
-        at.const_left = tm.const_left;
-        if (tm.kind != T_IMM) at.vid = VAL(tm.slot);
-        x.terms.push_back(at);
-      }
-      dot_terms += c->terms.size();
-      prog.push_back(std::move(x));
+    if (split_root[i]) {
+      flush_pieces(i);
       continue;
     }
     AIns x;
@@ -318,6 +378,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
       if (x.b.imm) x.flags |= F_B_IMM;
     }
     prog.push_back(std::move(x));
+    flush_pieces(i);
   }
   for (uint32_t j = 0; j < (uint32_t)t.outputs.size(); ++j) {
     AIns x;
@@ -337,32 +398,74 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
       adj_def[seed] = 1;
       prog.push_back(x);
     }
+    // DOT_REV pieces: a term whose operand is a single-use computed value
+    // receives no other adjoint contribution, so its contribution may be
+    // applied just before that operand's own reverse instruction instead of at
+    // the chain root; its adjoint register then lives only briefly.  Terms
+    // keep their relative order.
+    std::vector<std::vector<std::pair<uint32_t, std::vector<const Term*>>>> rpieces(N + 1);
+    std::vector<std::vector<const Term*>> rev_order(N);
+    for (int64_t ii = (int64_t)N - 1; ii >= (int64_t)ni; --ii) {
+      uint32_t r = (uint32_t)ii;
+      if (!reach[r] || is_const(r) || absorbed[r]) continue;
+      const Chain* c = dot_root(r);
+      if (!c) continue;
+      std::vector<const Term*> order;
+      for (const Term& tm : c->terms)
+        if (tm.kind != T_IMM) order.push_back(&tm);
+      std::stable_sort(order.begin(), order.end(), [](const Term* p, const Term* q) {
+        if (p->contrib != q->contrib) return p->contrib > q->contrib;
+        return p->sub < q->sub;
+      });
+      // Only contributions to the same adjoint must keep their order, and a
+      // sunk term's adjoint has no other contribution; every other term stays
+      // at the root.  Terms keep their stored order within each position.
+      std::vector<int64_t> qs;
+      for (const Term* tm : order) {
+        const uint32_t s_ = tm->slot;
+        bool sink = s_ >= ni && uses[s_] == 1 && !is_out[s_] && !absorbed[s_] && reach[s_];
+        qs.push_back(sink ? (int64_t)s_ : (int64_t)r);
+      }
+      std::vector<int64_t> positions(qs);
+      std::sort(positions.begin(), positions.end());
+      positions.erase(std::unique(positions.begin(), positions.end()), positions.end());
+      std::vector<const Term*> group;
+      for (int64_t pos : positions) {
+        group.clear();
+        for (size_t k = 0; k < order.size(); ++k)
+          if (qs[k] == pos) group.push_back(order[k]);
+        rpieces[(size_t)pos].push_back({r, group});
+      }
+    }
+    auto emit_rpiece = [&](uint32_t r, const std::vector<const Term*>& terms_) {
+      const Chain& c = chains[chain_of[r]];
+      AIns x;
+      x.op = I_DOT_REV;
+      x.g = ADJ(r);
+      for (const Term* tm : terms_) {
+        ATerm at;
+        at.kind = tm->kind;
+        at.imm = tm->imm;
+        at.const_left = tm->const_left;
+        at.vid = ADJ(tm->slot);
+        at.depth = (c.nadds - 1) - tm->add_pos;
+        at.first = !adj_def[tm->slot];
+        adj_def[tm->slot] = 1;
+        x.terms.push_back(at);
+      }
+      prog.push_back(std::move(x));
+    };
     for (int64_t ii = (int64_t)N - 1; ii >= (int64_t)ni; --ii) {
       uint32_t i = (uint32_t)ii;
       if (!reach[i] || is_const(i) || absorbed[i]) continue;
-      if (const Chain* c = dot_root(i)) {
-        AIns x;
-        x.op = I_DOT_REV;
-        x.g = ADJ(i);
-        std::vector<const Term*> order;
-        for (const Term& tm : c->terms) order.push_back(&tm);
-        std::stable_sort(order.begin(), order.end(), [](const Term* p, const Term* q) {
-          if (p->contrib != q->contrib) return p->contrib > q->contrib;
-          return p->sub < q->sub;
-        });
-        for (const Term* tm : order) {
-          if (tm->kind == T_IMM) continue;
-          ATerm at;
-          at.kind = tm->kind;
-          at.imm = tm->imm;
-          at.const_left = tm->const_left;
-          at.vid = ADJ(tm->slot);
-          at.depth = (c->nadds - 1) - tm->add_pos;
-          at.first = !adj_def[tm->slot];
-          adj_def[tm->slot] = 1;
-          x.terms.push_back(at);
-        }
-        prog.push_back(std::move(x));
+      // Pieces of other chains sunk onto this entry add to adj(i), so they go
+      // first; then the entry's own reverse (for a chain root: its piece at
+      // the root position).
+      for (auto& pc : rpieces[i])
+        if (pc.first != i) emit_rpiece(pc.first, pc.second);
+      if (dot_root(i)) {
+        for (auto& pc : rpieces[i])
+          if (pc.first == i) emit_rpiece(pc.first, pc.second);
         continue;
       }
       uint32_t op = t.opcodes[i];
@@ -428,8 +531,11 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
     }
     rd(x.g, p);
     rd(x.r, p);
-    if (x.op == I_DOT) {
+    if (x.op == I_LOADN) {
+      for (auto& tm : x.terms) wr(tm.vid, p);
+    } else if (x.op == I_DOT) {
       for (auto& tm : x.terms) rd(tm.vid, p);
+      if (x.flags & F_DOT_CONT) rd(x.dst, p);
       wr(x.dst, p);
     } else if (x.op == I_DOT_REV) {
       for (auto& tm : x.terms) {
@@ -488,6 +594,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
     }
   };
   auto R = [&](int64_t v) -> uint32_t { return v >= 0 ? reg[v] : 0u; };
+  std::vector<uint32_t> t_meta, t_reg, t_imm;
   for (AIns& x : prog) {
     uint32_t w[NW] = {0, 0, 0, 0, 0, 0, 0, 0};
     w[0] = x.op | (x.flags << 8);
@@ -500,18 +607,39 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
         if (x.a.imm) put_imm(x.a.v, w[2], w[3]);
         else w[2] = R(x.a.vid);
         break;
+      case I_LOADN:
       case I_DOT:
       case I_DOT_REV: {
-        w[1] = (x.op == I_DOT) ? R(x.dst) : R(x.g);
+        w[1] = (x.op == I_DOT) ? R(x.dst) : (x.op == I_DOT_REV ? R(x.g) : 0u);
         w[2] = (uint32_t)x.terms.size();
-        w[3] = (uint32_t)(out.terms.size() / TW);
+        w[3] = (uint32_t)t_meta.size();
+        const size_t base = t_meta.size();
         for (auto& tm : x.terms) {
-          uint32_t tw[TW] = {0, 0, 0, 0};
-          uint32_t tfl = (tm.first ? 1u : 0u) | (tm.const_left ? 2u : 0u);
-          tw[0] = tm.kind | (tfl << 8) | (std::min<uint32_t>(tm.depth, 0xFFFFu) << 16);
-          tw[1] = (tm.vid >= 0) ? R(tm.vid) : 0u;
-          put_imm(tm.imm, tw[2], tw[3]);
-          out.terms.insert(out.terms.end(), tw, tw + TW);
+          uint32_t m = tm.kind | (tm.first ? M_FIRST : 0u) | (tm.const_left ? M_CONST_LEFT : 0u) |
+                       (std::min<uint32_t>(tm.depth, 0xFFFFu) << M_DEPTH_SHIFT);
+          t_meta.push_back(m);
+          t_reg.push_back((tm.vid >= 0) ? R(tm.vid) : 0u);
+          uint32_t lo = 0, hi = 0;
+          if (x.op == I_LOADN) lo = (uint32_t)tm.imm;
+          else put_imm(tm.imm, lo, hi);
+          t_imm.push_back(lo);
+          if (t.f64) t_imm.push_back(hi);
+        }
+        // run lengths: independent blocks of up to 8 product terms (reverse
+        // blocks additionally need distinct adjoint registers)
+        const size_t T = x.terms.size();
+        for (size_t k = 0; k < T; ++k) {
+          uint32_t run = 0;
+          for (size_t j = k; j < T && run < 8; ++j) {
+            bool okk = (x.op == I_LOADN) || x.terms[j].kind == T_PROD;
+            if (okk && x.op == I_DOT_REV) {
+              for (size_t q2 = k; q2 < j; ++q2)
+                if (t_reg[base + q2] == t_reg[base + j]) okk = false;
+            }
+            if (!okk) break;
+            ++run;
+          }
+          t_meta[base + k] |= run << M_RUN_SHIFT;
         }
         break;
       }
@@ -534,7 +662,11 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
     out.words.insert(out.words.end(), w, w + NW);
   }
   out.ninstr = (uint32_t)prog.size();
-  if (out.terms.empty()) out.terms.assign(TW, 0);
+  out.nterms = (uint32_t)t_meta.size();
+  out.terms = t_meta;
+  out.terms.insert(out.terms.end(), t_reg.begin(), t_reg.end());
+  out.terms.insert(out.terms.end(), t_imm.begin(), t_imm.end());
+  if (out.terms.empty()) out.terms.assign(4, 0);
   return out;
 }
 

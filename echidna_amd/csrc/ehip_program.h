@@ -34,8 +34,10 @@ namespace ehip {
 //   STORE_OUT       : w1 output index, w2/w3 value
 //   SEED            : w1 dst (adjoint seeded to one)
 //   STORE_GRAD      : w1 input index, w2/w3 adjoint (or immediate zero)
-//   DOT / DOT_REV   : w1 dst (or g), w2 term count, w3 first term word;
-//                     terms live in the side table (see DotTerm)
+//   DOT / DOT_REV   : w1 dst (or g), w2 term count, w3 first term index
+//                     into the side table below
+//   LOADN           : w2 count, w3 first side-table index; reg[] holds the
+//                     destination registers, imm[] the input indices
 constexpr uint32_t NW = 8;
 
 enum : uint32_t {
@@ -45,6 +47,7 @@ enum : uint32_t {
   I_STORE_GRAD = 67,
   I_DOT = 68,
   I_DOT_REV = 69,
+  I_LOADN = 70,
   I_REV_BASE = 128,
 };
 
@@ -61,16 +64,24 @@ enum : uint32_t {
   F_RD_R = 1u << 9,
 };
 
-// DOT side-table entry, 4 x u32:
-//   t0 = kind | flags << 8; kind 0 = reg*imm product, 1 = plain reg, 2 = imm
-//   t1 = reg (value register for DOT, adjoint register for DOT_REV)
-//   t2/t3 = immediate (f32 in t2, f64 in t2/t3)
-// DOT_REV terms are stored in the order their adjoint contributions happen
-// (descending tape index); flag bit 0 = first contribution (adjoint starts
-// at zero).  The DOT_REV chain length (number of Add links between the
-// result and each term) lives in t0 bits 16..31 for the Dual-number replay.
-constexpr uint32_t TW = 4;
+// DOT side table, stored as three arrays of T entries each (T = total terms):
+//   meta[T]  : kind (bits 0-1: 0 reg*imm product, 1 plain reg, 2 imm)
+//              | first (bit 2: adjoint starts at zero, DOT_REV)
+//              | const_left (bit 3: product written c * x on the tape)
+//              | run (bits 4-11: how many consecutive terms from here can be
+//                processed as one independent block, capped at 8)
+//              | depth (bits 16-31: Add links between the DOT_REV root and
+//                the link where the term entered, for the Dual replay)
+//   reg[T]   : value register (DOT) or adjoint register (DOT_REV)
+//   imm[T*IW]: immediate, IW = 1 word (f32) or 2 words (f64)
+// A DOT instruction covers terms [w3, w3 + w2).  DOT_REV terms are stored in
+// the order their adjoint contributions happen (descending tape index).
+// F_DOT_CONT on a DOT continues an accumulator already in register w1 (a
+// chain emitted in several pieces); otherwise term 0 initialises it.
 enum : uint32_t { T_PROD = 0, T_REG = 1, T_IMM = 2 };
+enum : uint32_t { M_FIRST = 1u << 2, M_CONST_LEFT = 1u << 3 };
+constexpr uint32_t M_RUN_SHIFT = 4, M_DEPTH_SHIFT = 16;
+constexpr uint32_t F_DOT_CONT = 1u << 10;
 
 struct TapeHost {
   std::vector<uint32_t> opcodes, arg0, arg1;
@@ -82,10 +93,11 @@ struct TapeHost {
 
 struct Program {
   std::vector<uint32_t> words;  // NW words per instruction
-  std::vector<uint32_t> terms;  // TW words per DOT term
+  std::vector<uint32_t> terms;  // meta[T] | reg[T] | imm[T*IW]
   uint32_t ninstr = 0;
   uint32_t nregs = 0;
   uint64_t dot_terms = 0;
+  uint32_t nterms = 0;          // T
 };
 
 enum ProgMode { PM_FORWARD = 0, PM_GRADIENT = 1 };

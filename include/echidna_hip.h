@@ -167,9 +167,10 @@ int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
 
 /* Compile a tape into its register program without a device (inspection
  * and tests).  Same validation as ehip_tape_upload.  Writes up to
- * `words_cap` program words and `terms_cap` DOT side-table words; out4 as in
- * ehip_program_info.  Returns EHIP_OTHER if a buffer is too small (out4 still
- * holds the sizes needed: out4[2] words, out4[3] terms*4 side-table words). */
+ * `words_cap` program words and `terms_cap` side-table words; out4 =
+ * {instructions, registers, program words, side-table entries T}; the side
+ * table needs max(4, T*(2+IW)) words (IW = 1 f32, 2 f64).  Returns EHIP_OTHER
+ * if a buffer is too small (out4 still holds the sizes). */
 int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                              const double* constants, uint32_t num_ops, uint32_t num_inputs,
                              uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,

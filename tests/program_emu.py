@@ -13,7 +13,8 @@ from __future__ import annotations
 
 import numpy as np
 
-I_LOAD, I_STORE_OUT, I_SEED, I_STORE_GRAD, I_DOT, I_DOT_REV, I_REV = 64, 65, 66, 67, 68, 69, 128
+I_LOAD, I_STORE_OUT, I_SEED, I_STORE_GRAD, I_DOT, I_DOT_REV, I_LOADN, I_REV = 64, 65, 66, 67, 68, 69, 70, 128
+F_DOT_CONT = 1024
 F_A_IMM, F_B_IMM, F_FIRST_A, F_FIRST_B, F_ACC_A, F_ACC_B, F_HAS_B, F_RD_A, F_RD_B, F_RD_R = (
     1, 2, 4, 8, 16, 32, 64, 128, 256, 512)
 T_PROD, T_REG, T_IMM = 0, 1, 2
@@ -108,9 +109,25 @@ def _partials(op, a, b, r, F):
     raise NotImplementedError(f"emulator partial for opcode {op}")
 
 
-def run(words, terms, x, nin, nout, nregs, f64=True, mode=1):
+def split_terms(terms, nterms, f64):
+    """meta[T] | reg[T] | imm[T*IW] -> per-term tuples (meta, reg, lo, hi)."""
+    t = np.asarray(terms, dtype=np.uint32).reshape(-1)
+    meta = t[:nterms]
+    reg = t[nterms:2 * nterms]
+    iw = 2 if f64 else 1
+    imm = t[2 * nterms:2 * nterms + iw * nterms]
+    out = []
+    for k in range(nterms):
+        lo = int(imm[k * iw])
+        hi = int(imm[k * iw + 1]) if f64 else 0
+        out.append((int(meta[k]), int(reg[k]), lo, hi))
+    return out
+
+
+def run(words, terms, x, nin, nout, nregs, f64=True, mode=1, nterms=None):
     """Run the program over points ``x`` [B, nin]; returns (y [B,nout], g [B,nin])."""
     F = np.float64 if f64 else np.float32
+    terms = split_terms(terms, nterms, f64)
     B = x.shape[0]
     regs = np.full((nregs, B), np.nan, dtype=F)
     y = np.zeros((B, nout), F)
@@ -154,12 +171,16 @@ def run(words, terms, x, nin, nout, nregs, f64=True, mode=1):
                     m = live & (db != 0)
                     cur[m] = cur[m] + db[m] * gg[m]
                     regs[w[5]] = cur
-            elif op == I_DOT:
-                acc = None
+            elif op == I_LOADN:
                 for k in range(w[2]):
-                    t0, t1, t2, t3 = [int(v) for v in terms[w[3] + k]]
+                    t0, t1, t2, t3 = terms[w[3] + k]
+                    regs[t1] = x[:, t2]
+            elif op == I_DOT:
+                acc = regs[w[1]].copy() if fl & F_DOT_CONT else None
+                for k in range(w[2]):
+                    t0, t1, t2, t3 = terms[w[3] + k]
                     c = F(_imm(t2, t3, f64))
-                    kind = t0 & 0xFF
+                    kind = t0 & 0x3
                     term = (regs[t1] * c if kind == T_PROD else regs[t1].copy()) if kind != T_IMM else np.full(B, c, F)
                     acc = term if acc is None else acc + term
                 regs[w[1]] = acc
@@ -167,10 +188,10 @@ def run(words, terms, x, nin, nout, nregs, f64=True, mode=1):
                 gg = regs[w[1]].copy()
                 live = gg != 0
                 for k in range(w[2]):
-                    t0, t1, t2, t3 = [int(v) for v in terms[w[3] + k]]
+                    t0, t1, t2, t3 = terms[w[3] + k]
                     c = F(_imm(t2, t3, f64))
-                    kind = t0 & 0xFF
-                    cur = np.zeros(B, F) if (t0 >> 8) & 1 else regs[t1].copy()
+                    kind = t0 & 0x3
+                    cur = np.zeros(B, F) if (t0 >> 2) & 1 else regs[t1].copy()
                     if kind == T_PROD:
                         if c != 0:
                             cur[live] = cur[live] + c * gg[live]

