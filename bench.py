@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-fuse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -99,6 +101,13 @@ def main() -> int:
     sh = stream.cuda_stream
     torch.cuda.synchronize()
 
+    if args.copy_probe:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
     with torch.cuda.stream(stream):
         for _ in range(args.warmup):
             ctx.gradient_batch_dev(bufs, dx, B, dy, dg, stream=sh)
