@@ -108,6 +108,12 @@ def main() -> int:
         dst.copy_(src)  # reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
         del src, dst
+    tuned = None
+    if not (args.wg or args.lds):
+        # launch shape from the autotuner, chosen before the timed region
+        from echidna_amd.backend import TUNE_GRADIENT
+        tuned = ctx.autotune_dev(bufs, TUNE_GRADIENT, B, dx, d_y=dy, d_g=dg)
+        torch.cuda.synchronize()
     with torch.cuda.stream(stream):
         for _ in range(args.warmup):
             ctx.gradient_batch_dev(bufs, dx, B, dy, dg, stream=sh)
@@ -150,7 +156,8 @@ def main() -> int:
         "config": {"workload": f"tanh-MLP tape n={n} H={H} seed 1 (N={counts['N']} computed entries), "
                                f"B=2^{args.batch_log2} points, fwd+grad",
                    "points": B, "inputs": n, "hidden": H, "computed_entries": counts["N"],
-                   "program": bufs.program_info(1), "parallelism": "none (1 GPU)"},
+                   "program": bufs.program_info(1), "parallelism": "none (1 GPU)",
+                   "launch": tuned or {"wg": args.wg, "lds_budget": args.lds}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel_ms": kernel_ms, "algorithmic_bytes_per_point": bytes_per_point},

@@ -70,7 +70,14 @@ _SIGS = {
     "ehip_laplacian_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                                   ctypes.c_uint64, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "ehip_set_fusion": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
+                                           ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
+    "ehip_set_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "ehip_get_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
 }
+
+# Evaluation kinds of the autotuner.
+TUNE_FORWARD, TUNE_GRADIENT, TUNE_HVP, TUNE_TAYLOR = 0, 1, 2, 3
 
 _lib = None
 
@@ -112,6 +119,13 @@ def _u32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
+def _dp(t):
+    """Device pointer of a torch tensor (or a raw address; 0/None = null)."""
+    if t is None or (isinstance(t, int) and t == 0):
+        return _vp(None)
+    return _vp(t.data_ptr() if hasattr(t, "data_ptr") else int(t))
+
+
 def _ptr(a: np.ndarray):
     return ctypes.c_void_p(a.ctypes.data)
 
@@ -134,7 +148,11 @@ class TaylorKthBatchResult:
 
 
 def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Optional[bool] = None):
-    """Compile without a device; returns (words[n,8], terms[t,4], info dict)."""
+    """Compile without a device.
+
+    Modes 0/1 (forward/gradient) return (words[n,8], side table, info);
+    modes 2/3 return (windowed stream words, None, info).
+    """
     lib = load_library()
     f64 = (np.asarray(data.constants).dtype == np.float64) if f64 is None else f64
     consts = np.ascontiguousarray(data.constants, dtype=np.float64)
@@ -155,6 +173,8 @@ def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Op
                                   len(terms), info)
     _check(rc)
     info2 = {"instructions": int(info[0]), "registers": int(info[1]), "words": nw, "nterms": T}
+    if mode >= 2:
+        return words[:nw], None, info2
     return words[:nw].reshape(-1, 8), terms[:nt], info2
 
 
@@ -309,6 +329,25 @@ class HipContext:
         _check(load_library().ehip_gradient_batch_dev(self.handle, tape.handle, _vp(d_inputs.data_ptr()), batch,
                                                        _vp(d_outputs.data_ptr()), _vp(d_grads.data_ptr()),
                                                        _vp(stream or None)))
+
+    def autotune_dev(self, tape: "HipTapeBuffers", what: int, batch: int, d_x, d_y=0, d_g=0, d_h=0, d_dirs=0,
+                     x_stride: Optional[int] = None, order: int = 0) -> dict:
+        """Pick the fastest launch shape for ``what`` (TUNE_*) on these buffers."""
+        out = (ctypes.c_uint32 * 3)()
+        xs = tape.num_inputs if x_stride is None else x_stride
+        _check(load_library().ehip_autotune_dev(self.handle, tape.handle, what, order, _dp(d_x), xs, _dp(d_dirs),
+                                                batch, _dp(d_y), _dp(d_g), _dp(d_h), out))
+        return {"wg": int(out[0]), "lds_budget": int(out[1]), "us": int(out[2])}
+
+    @staticmethod
+    def set_tuned(tape: "HipTapeBuffers", what: int, wg: int, lds_budget: int, order: int = 0) -> None:
+        _check(load_library().ehip_set_tuned(tape.handle, what, order, wg, lds_budget))
+
+    @staticmethod
+    def get_tuned(tape: "HipTapeBuffers", what: int, order: int = 0) -> dict:
+        out = (ctypes.c_uint32 * 2)()
+        _check(load_library().ehip_get_tuned(tape.handle, what, order, out))
+        return {"wg": int(out[0]), "lds_budget": int(out[1])}
 
     def hvp_batch_dev(self, tape, d_x, x_stride: int, d_dirs, batch: int, d_grads, d_hvps, stream: int = 0) -> None:
         _check(load_library().ehip_hvp_batch_dev(self.handle, tape.handle, _vp(d_x.data_ptr()), x_stride,

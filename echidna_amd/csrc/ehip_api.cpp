@@ -91,8 +91,14 @@ struct ehip_ctx {
 
 struct DevProgram {
   Program host;
-  uint32_t* d_words = nullptr;
-  uint32_t* d_terms = nullptr;
+  uint32_t* d_stream = nullptr;  // windowed stream (encode_stream)
+  uint32_t nwin = 0;
+};
+
+// Launch shape chosen by the autotuner for one evaluation kind of one tape.
+struct Tuned {
+  uint32_t wg = 0;          // 0: not tuned, use the context defaults
+  uint32_t lds_budget = 0;  // bytes of LDS for the register file
 };
 
 struct ehip_tape {
@@ -101,24 +107,22 @@ struct ehip_tape {
   uint32_t nin = 0, nout = 0;
   TapeHost host;
   DevProgram prog[2];
+  Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
 };
 
 namespace {
 
-int32_t upload_program(DevProgram& dp) {
-  const Program& p = dp.host;
-  size_t wb = std::max<size_t>(p.words.size(), 8) * 4, tb = std::max<size_t>(p.terms.size(), 4) * 4;
-  HIP_TRY(hipMalloc((void**)&dp.d_words, wb), "program upload");
-  HIP_TRY(hipMalloc((void**)&dp.d_terms, tb), "program upload");
-  if (!p.words.empty()) HIP_TRY(hipMemcpy(dp.d_words, p.words.data(), p.words.size() * 4, hipMemcpyHostToDevice), "program upload");
-  if (!p.terms.empty()) HIP_TRY(hipMemcpy(dp.d_terms, p.terms.data(), p.terms.size() * 4, hipMemcpyHostToDevice), "program upload");
+int32_t upload_program(DevProgram& dp, bool f64) {
+  const std::vector<uint32_t> st = encode_stream(dp.host, f64);
+  dp.nwin = (uint32_t)(st.size() / WIN_WORDS);
+  HIP_TRY(hipMalloc((void**)&dp.d_stream, st.size() * 4), "program upload");
+  HIP_TRY(hipMemcpy(dp.d_stream, st.data(), st.size() * 4, hipMemcpyHostToDevice), "program upload");
   return EHIP_OK;
 }
 
 void free_program(DevProgram& dp) {
-  if (dp.d_words) (void)hipFree(dp.d_words);
-  if (dp.d_terms) (void)hipFree(dp.d_terms);
-  dp.d_words = dp.d_terms = nullptr;
+  if (dp.d_stream) (void)hipFree(dp.d_stream);
+  dp.d_stream = nullptr;
 }
 
 // validate_batch_args (src/gpu/mod.rs:572-582)
@@ -129,18 +133,22 @@ int32_t check_batch(const ehip_tape* t, uint64_t batch) {
   return EHIP_OK;
 }
 
+// Slot of `tuned` for an evaluation: 0 forward, 1 gradient, 2 hvp, 2+K taylor.
+int tune_key(int kind, int K, bool grad) { return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : 2 + K); }
+
 // Launch one interpreter over `L.B` lanes.  kind 0 scalar, 1 dual, 2 jet(K).
-int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s) {
+int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s, const Tuned* force = nullptr) {
   const DevProgram& dp = t->prog[kind == 0 && L.g == nullptr ? PM_FORWARD : (kind == 2 ? PM_FORWARD : PM_GRADIENT)];
-  L.prog = dp.d_words;
-  L.terms = dp.d_terms;
-  L.ninstr = dp.host.ninstr;
-  L.nterms = dp.host.nterms;
+  L.prog = dp.d_stream;
+  L.nwin = dp.nwin;
   const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
   const size_t S = t->f64 ? 8 : 4;
-  const uint32_t wg = c->wg;
+  const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
+  const uint32_t wg = tu.wg ? tu.wg : c->wg;
   const size_t per_reg = (size_t)NC * wg * S;
-  size_t budget = std::min<size_t>(c->lds_budget, c->max_lds);
+  // two instruction windows share the workgroup's LDS with the registers
+  const size_t win_lds = 2 * (size_t)WIN_WORDS * 4;
+  size_t budget = std::min<size_t>(tu.wg ? tu.lds_budget : c->lds_budget, c->max_lds > win_lds ? c->max_lds - win_lds : 0);
   uint32_t R = dp.host.nregs;
   uint32_t R_lds = (uint32_t)std::min<size_t>(R, budget / per_reg);
   L.R_lds = R_lds;
@@ -271,7 +279,7 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
   t->prog[PM_GRADIENT].host = compile_program(t->host, PM_GRADIENT, fuse);
   (void)hipSetDevice(c->device);
   for (int m = 0; m < 2; ++m) {
-    int32_t rc = upload_program(t->prog[m]);
+    int32_t rc = upload_program(t->prog[m], t->f64);
     if (rc != EHIP_OK) {
       for (auto& p : t->prog) free_program(p);
       delete t;
@@ -307,7 +315,7 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
                              uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,
                              int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
                              uint64_t terms_cap, uint64_t* out4) {
-  if (!out4 || (mode != 0 && mode != 1) || (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
+  if (!out4 || mode < 0 || mode > 3 || (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)
     if (opcodes[i] == 43u) return fail(EHIP_CUSTOM_OPS_NOT_SUPPORTED, "tape contains custom ops which cannot run on GPU");
@@ -328,11 +336,19 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
   th.num_inputs = num_inputs;
   th.outputs = outs;
   th.f64 = dtype == EHIP_F64;
-  Program p = compile_program(th, mode == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0);
+  Program p = compile_program(th, (mode & 1) == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0);
   out4[0] = p.ninstr;
   out4[1] = p.nregs;
-  out4[2] = p.words.size();
   out4[3] = p.nterms;
+  if (mode >= 2) {
+    // the windowed stream the kernels execute; `terms` is not used
+    const std::vector<uint32_t> st = encode_stream(p, th.f64);
+    out4[2] = st.size();
+    if (st.size() > words_cap || !words) return fail(EHIP_OTHER, "GPU error: program buffers too small");
+    std::memcpy(words, st.data(), st.size() * 4);
+    return ok();
+  }
+  out4[2] = p.words.size();
   if (p.words.size() > words_cap || p.terms.size() > terms_cap || !words || !terms)
     return fail(EHIP_OTHER, "GPU error: program buffers too small");
   std::memcpy(words, p.words.data(), p.words.size() * 4);
@@ -423,6 +439,125 @@ int32_t ehip_taylor_kth_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d
   L.y = d_coeffs;
   rc = run(c, t, 2, (int)order, L, pick(c, stream));
   return rc ? rc : ok();
+}
+
+// --------------------------------- autotuner ---------------------------------
+
+namespace {
+
+// Launch fields for an evaluation `what` (0 forward, 1 gradient, 2 hvp,
+// 3 taylor of `order`) over caller buffers.
+int32_t tune_launch(const ehip_tape* t, int32_t what, uint32_t order, const void* d_x, uint64_t x_stride,
+                    const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h, Launch& L, int& kind,
+                    int& K) {
+  L = Launch{};
+  L.nin = t->nin;
+  L.nout = t->nout;
+  L.B = batch;
+  L.x = d_x;
+  L.xstride = x_stride;
+  switch (what) {
+    case 0: kind = 0; K = 1; L.y = d_y; break;
+    case 1: kind = 0; K = 1; L.y = d_y; L.g = d_g; if (!d_g) return fail(EHIP_OTHER, "GPU error: null argument"); break;
+    case 2:
+      kind = 1; K = 2; L.dirs = d_dirs; L.g = d_g; L.h = d_h;
+      if (!d_dirs || !d_g || !d_h) return fail(EHIP_OTHER, "GPU error: null argument");
+      if (t->nout != 1) return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only");
+      break;
+    case 3:
+      if (order < 1 || order > 5) return fail(EHIP_OTHER, "GPU error: taylor order must be 1..=5");
+      kind = 2; K = (int)order; L.dirs = d_dirs; L.y = d_y;
+      if (!d_dirs || !d_y) return fail(EHIP_OTHER, "GPU error: null argument");
+      break;
+    default: return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
+  }
+  if (x_stride != 0 && x_stride != t->nin) return fail(EHIP_OTHER, "GPU error: x_stride must be 0 or num_inputs");
+  return EHIP_OK;
+}
+
+}  // namespace
+
+int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t order, const void* d_x,
+                          uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
+                          uint32_t* out3) {
+  if (!c || !t || !d_x || !out3) return fail(EHIP_OTHER, "GPU error: null argument");
+  int32_t rc = check_batch(t, batch);
+  if (rc) return rc;
+  Launch L;
+  int kind = 0, K = 1;
+  rc = tune_launch(t, what, order, d_x, x_stride, d_dirs, batch, d_y, d_g, d_h, L, kind, K);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
+  const size_t S = t->f64 ? 8 : 4;
+  const uint32_t R = t->prog[(kind == 0 && what == 1) || kind == 1 ? PM_GRADIENT : PM_FORWARD].host.nregs;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0), "autotune");
+  HIP_TRY(hipEventCreate(&e1), "autotune");
+  Tuned best;
+  float best_ms = 1e30f;
+  static const uint32_t budgets[] = {4096, 8192, 16384, 24576, 32768, 49152, 65536, 98304, 131072};
+  for (uint32_t wg : {64u, 128u, 256u}) {
+    std::vector<uint32_t> seen;
+    for (uint32_t b : budgets) {
+      const size_t per_reg = (size_t)NC * wg * S;
+      const size_t win_lds = 2 * (size_t)WIN_WORDS * 4;
+      const size_t cap = c->max_lds > win_lds ? c->max_lds - win_lds : 0;
+      if (b > cap) continue;
+      const uint32_t rl = (uint32_t)std::min<size_t>(R, b / per_reg);
+      if (std::find(seen.begin(), seen.end(), rl) != seen.end()) continue;
+      seen.push_back(rl);
+      Tuned cand;
+      cand.wg = wg;
+      cand.lds_budget = b;
+      float ms = 1e30f;
+      for (int rep = 0; rep < 3 && rc == EHIP_OK; ++rep) {
+        HIP_TRY(hipEventRecord(e0, c->stream), "autotune");
+        rc = run(c, t, kind, K, L, c->stream, &cand);
+        HIP_TRY(hipEventRecord(e1, c->stream), "autotune");
+        HIP_TRY(hipEventSynchronize(e1), "autotune");
+        float m = 0.f;
+        HIP_TRY(hipEventElapsedTime(&m, e0, e1), "autotune");
+        if (rep > 0) ms = std::min(ms, m);  // first run warms caches
+      }
+      if (rc) break;
+      if (ms < best_ms) { best_ms = ms; best = cand; }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  t->tuned[tune_key(kind, K, what == 1)] = best;
+  out3[0] = best.wg;
+  out3[1] = best.lds_budget;
+  out3[2] = (uint32_t)std::min(best_ms * 1000.f, 4.0e9f);
+  return ok();
+}
+
+int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget) {
+  if (!t) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (wg != 0 && wg != 64 && wg != 128 && wg != 256)
+    return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
+  int key;
+  if (what >= 0 && what <= 2) key = what;
+  else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
+  else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
+  t->tuned[key].wg = wg;
+  t->tuned[key].lds_budget = wg ? lds_budget : 0;
+  return ok();
+}
+
+int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_t* out2) {
+  if (!t || !out2) return fail(EHIP_OTHER, "GPU error: null argument");
+  int key;
+  if (what >= 0 && what <= 2) key = what;
+  else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
+  else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
+  out2[0] = t->tuned[key].wg;
+  out2[1] = t->tuned[key].lds_budget;
+  return ok();
 }
 
 // ----------------------------- host-pointer API -----------------------------

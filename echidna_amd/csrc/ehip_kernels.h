@@ -6,10 +6,8 @@
 namespace ehip {
 
 struct Launch {
-  const uint32_t* prog;   // NW words per instruction (device)
-  const uint32_t* terms;  // side table (device): meta[T] | reg[T] | imm[T*IW]
-  uint32_t ninstr;
-  uint32_t nterms;        // T
+  const uint32_t* prog;   // windowed instruction stream (device, encode_stream)
+  uint32_t nwin;          // windows of WIN_WORDS words in the stream
   uint32_t R_lds;         // registers held in LDS
   uint32_t nin, nout;
   uint64_t B;             // points (lanes) in this launch

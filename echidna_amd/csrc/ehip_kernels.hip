@@ -1,23 +1,26 @@
 // Wave-uniform interpreters for the compiled register program.
 //
-// One lane evaluates one point (or one point-direction pair).  Every lane of
-// a workgroup runs the same instruction stream, so instruction words are
-// fetched with scalar loads into SGPRs (the next instruction is requested
-// while the current one executes) and every branch on them is uniform; only
-// the data-dependent conventions (zero-adjoint skip, zero-multiplier guard,
-// domain checks) are per-lane selects.
+// One lane evaluates one point (or one point-direction pair); every lane of a
+// workgroup runs the same instruction stream.  The stream (ehip_program.h,
+// encode_stream) is cut into windows that the workgroup stages in LDS with
+// coalesced vector loads: two window buffers, the next window arriving while
+// the current one executes, one barrier per window.  Instruction words are
+// read from LDS by every lane at the same address (broadcast) and made
+// wave-uniform with readfirstlane where they steer control flow; register
+// numbers and immediates are used straight from VGPRs.  Only the reference's
+// data-dependent conventions (zero-adjoint skip, zero-multiplier guard,
+// domain checks, NaN routing) are per-lane selects.
 //
 // Register file: registers [0, R_lds) live in LDS as [reg][component][lane]
 // (slot-major, lane-minor: one conflict-free LDS transaction per wave);
 // registers >= R_lds live in HBM as [reg][component][point] (slot-major,
 // point-minor: one coalesced transaction per wave).  The LDS-only variant
-// (every register fits) has no HBM path at all; the split variant keeps the
-// two paths in distinct address spaces so neither degrades to flat accesses.
+// has no HBM path; the split variant keeps the two paths in distinct address
+// spaces so neither degrades to flat accesses.
 //
-// Fused dot products read their term table (meta | reg | imm arrays) in
-// blocks of eight: eight register indices and eight immediates arrive in two
-// scalar loads, eight LDS reads issue back to back, then the products are
-// added in term order.
+// Fused dot products run their product terms in blocks of eight: eight LDS
+// reads issue back to back, then the products are added in term order (no
+// FMA, the reference's rounding sequence).
 #include <hip/hip_runtime.h>
 
 #include "ehip_kernels.h"
@@ -32,29 +35,34 @@ extern __shared__ __attribute__((aligned(16))) unsigned char g_smem[];
 #define AS3 __attribute__((address_space(3)))
 #define AS1 __attribute__((address_space(1)))
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <class F> __device__ __forceinline__ F imm_of(uint32_t lo, uint32_t hi);
 template <> __device__ __forceinline__ float imm_of<float>(uint32_t lo, uint32_t) { return __uint_as_float(lo); }
 template <> __device__ __forceinline__ double imm_of<double>(uint32_t lo, uint32_t hi) {
   return __hiloint2double((int)hi, (int)lo);
 }
-template <class F> __device__ __forceinline__ F imm_at(const uint32_t* immw, uint32_t k);
-template <> __device__ __forceinline__ float imm_at<float>(const uint32_t* immw, uint32_t k) {
-  return __uint_as_float(immw[k]);
-}
-template <> __device__ __forceinline__ double imm_at<double>(const uint32_t* immw, uint32_t k) {
-  return __hiloint2double((int)immw[2 * k + 1], (int)immw[2 * k]);
-}
 
+// Register file.  Register numbers may arrive in VGPRs (wave-uniform values);
+// the split variant makes them scalar before choosing LDS or HBM.
 template <class F, int NC, bool LDSONLY>
 struct Regs {
-  AS3 F* lds;
+  AS3 F* lane;  // this lane's column of the LDS register file
   AS1 F* gl;
-  uint32_t R_lds, wg, tid;
+  uint32_t R_lds, lsh;  // lsh = log2(workgroup size)
   uint64_t gstride, p;
   __device__ __forceinline__ void rd(uint32_t r, F* o) const {
-    if (LDSONLY || r < R_lds) {
+    if (LDSONLY) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) o[c] = lds[(r * NC + c) * wg + tid];
+      for (int c = 0; c < NC; ++c) o[c] = lane[(r * NC + c) << lsh];
+      return;
+    }
+    r = uni(r);
+    if (r < R_lds) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) o[c] = lane[(r * NC + c) << lsh];
     } else {
       const uint64_t base = (uint64_t)(r - R_lds) * NC;
 #pragma unroll
@@ -62,9 +70,15 @@ struct Regs {
     }
   }
   __device__ __forceinline__ void wr(uint32_t r, const F* v) const {
-    if (LDSONLY || r < R_lds) {
+    if (LDSONLY) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) lds[(r * NC + c) * wg + tid] = v[c];
+      for (int c = 0; c < NC; ++c) lane[(r * NC + c) << lsh] = v[c];
+      return;
+    }
+    r = uni(r);
+    if (r < R_lds) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) lane[(r * NC + c) << lsh] = v[c];
     } else {
       const uint64_t base = (uint64_t)(r - R_lds) * NC;
 #pragma unroll
@@ -73,66 +87,101 @@ struct Regs {
   }
 };
 
+// LDS layout: [window buffer 0 | window buffer 1 | register file].
+constexpr uint32_t WIN_BYTES = WIN_WORDS * 4;
+
 template <class F, int NC, bool LDSONLY>
 __device__ __forceinline__ Regs<F, NC, LDSONLY> make_regs(const Launch& L) {
   Regs<F, NC, LDSONLY> R;
-  R.lds = (AS3 F*)(g_smem);
+  R.lane = (AS3 F*)(g_smem + 2 * WIN_BYTES) + threadIdx.x;
   R.gl = (AS1 F*)(L.gregs);
   R.R_lds = L.R_lds;
-  R.wg = blockDim.x;
-  R.tid = threadIdx.x;
+  R.lsh = __builtin_ctz(blockDim.x);
   R.gstride = (uint64_t)gridDim.x * blockDim.x;
   R.p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   return R;
 }
 
-// Side-table views for instruction-relative access.
-template <class F>
-struct Terms {
-  const uint32_t* meta;
-  const uint32_t* reg;
-  const uint32_t* immw;
-  __device__ __forceinline__ Terms(const Launch& L, uint32_t off) {
-    constexpr uint32_t IW = sizeof(F) / 4;
-    meta = L.terms + off;
-    reg = L.terms + L.nterms + off;
-    immw = L.terms + 2 * L.nterms + (size_t)off * IW;
-  }
-  __device__ __forceinline__ F imm(uint32_t k) const { return imm_at<F>(immw, k); }
-  __device__ __forceinline__ uint32_t kind(uint32_t k) const { return meta[k] & 3u; }
-  __device__ __forceinline__ uint32_t u32(uint32_t k) const { return immw[(size_t)k * (sizeof(F) / 4)]; }
-  __device__ __forceinline__ uint32_t run(uint32_t k) const { return (meta[k] >> M_RUN_SHIFT) & 0xFFu; }
-};
-
-struct Ins {
-  uint32_t w[NW];
-};
-
-__device__ __forceinline__ Ins fetch(const uint32_t* prog, uint32_t pc) {
-  Ins i;
-  const uint32_t* p = prog + (size_t)pc * NW;
+// Streams the program window by window through LDS and calls
+// `exec(window, pos, w0)` for every instruction.
+template <class Exec>
+__device__ __forceinline__ void run_stream(const Launch& L, Exec exec) {
+  AS3 uint32_t* buf = (AS3 uint32_t*)g_smem;
+  const uint32_t tid = threadIdx.x, wg = blockDim.x;
+  const u32x4* src = reinterpret_cast<const u32x4*>(L.prog);
+  constexpr uint32_t Q = WIN_WORDS / 4;  // 16-byte vectors per window
+  constexpr uint32_t PER = (Q + 63) / 64;  // vectors per lane at 64 lanes
+  for (uint32_t i = tid; i < Q; i += wg) ((AS3 u32x4*)buf)[i] = src[i];
+  __syncthreads();
+  const uint32_t nwin = L.nwin;
+  for (uint32_t w = 0; w < nwin; ++w) {
+    AS3 const uint32_t* win = buf + (w & 1u) * WIN_WORDS;
+    // request the next window now; store it once this one has executed
+    u32x4 pre[PER];
+    const bool more = w + 1 < nwin;
+    if (more) {
 #pragma unroll
-  for (int k = 0; k < (int)NW; ++k) i.w[k] = p[k];
-  return i;
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t i = tid + k * wg;
+        if (i < Q) pre[k] = src[(size_t)(w + 1) * Q + i];
+      }
+    }
+    uint32_t pos = 0;
+    for (;;) {
+      const uint32_t w0 = uni(win[pos]);
+      const uint32_t op = w0 & 0xFFu;
+      if (op == I_WEND || op == I_END) break;
+      exec(win, pos, w0);
+      pos += w0 >> 24;
+    }
+    if (more) {
+      AS3 u32x4* nb = (AS3 u32x4*)(buf + ((w + 1) & 1u) * WIN_WORDS);
+#pragma unroll
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t i = tid + k * wg;
+        if (i < Q) nb[i] = pre[k];
+      }
+    }
+    __syncthreads();
+  }
 }
 
-// Batched input load: eight loads in flight, then eight register writes.
+// Word accessors for the current instruction.
+template <class F>
+struct Ins {
+  AS3 const uint32_t* p;
+  __device__ __forceinline__ uint32_t u(uint32_t k) const { return uni(p[k]); }  // uniform
+  __device__ __forceinline__ uint32_t v(uint32_t k) const { return p[k]; }       // VGPR copy
+  __device__ __forceinline__ F imm(uint32_t k) const { return imm_of<F>(p[k], sizeof(F) == 8 ? p[k + 1] : 0u); }
+};
+
+// Inline term k of a DOT / DOT_REV / LOADN: meta, reg, imm.
+template <class F>
+struct Term {
+  AS3 const uint32_t* p;
+  __device__ __forceinline__ Term(AS3 const uint32_t* ins, uint32_t k) : p(ins + 3 + k * (2 + sizeof(F) / 4)) {}
+  __device__ __forceinline__ uint32_t meta() const { return p[0]; }
+  __device__ __forceinline__ uint32_t reg() const { return p[1]; }
+  __device__ __forceinline__ F imm() const { return imm_of<F>(p[2], sizeof(F) == 8 ? p[3] : 0u); }
+  __device__ __forceinline__ uint32_t u32() const { return p[2]; }
+};
+
+// Batched input load: eight global loads in flight, then eight register writes.
 template <class F, int NC, bool LDSONLY, class MakeVal>
-__device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, const Launch& L, uint32_t n, uint32_t off,
+__device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, AS3 const uint32_t* ins, uint32_t n,
                                             MakeVal make) {
-  Terms<F> T(L, off);
   uint32_t k = 0;
   for (; k + 8 <= n; k += 8) {
     F v[8][NC];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) make(T.u32(k + j), v[j]);
+    for (int j = 0; j < 8; ++j) make(Term<F>(ins, k + j).u32(), v[j]);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) R.wr(T.reg[k + j], v[j]);
+    for (int j = 0; j < 8; ++j) R.wr(Term<F>(ins, k + j).reg(), v[j]);
   }
   for (; k < n; ++k) {
     F v[NC];
-    make(T.u32(k), v);
-    R.wr(T.reg[k], v);
+    make(Term<F>(ins, k).u32(), v);
+    R.wr(Term<F>(ins, k).reg(), v);
   }
 }
 
@@ -147,47 +196,43 @@ __global__ __launch_bounds__(256) void k_scalar(Launch L) {
   const F* X = reinterpret_cast<const F*>(L.x);
   F* Y = reinterpret_cast<F*>(L.y);
   F* G = reinterpret_cast<F*>(L.g);
-  Ins nxt = fetch(L.prog, 0);
-  for (uint32_t pc = 0; pc < L.ninstr; ++pc) {
-    const Ins cur = nxt;
-    if (pc + 1 < L.ninstr) nxt = fetch(L.prog, pc + 1);
-    const uint32_t w0 = cur.w[0], w1 = cur.w[1], w2 = cur.w[2], w3 = cur.w[3], w4 = cur.w[4], w5 = cur.w[5],
-                   w6 = cur.w[6], w7 = cur.w[7];
-    const uint32_t op = w0 & 0xFFu, fl = w0 >> 8;
+  run_stream(L, [&](AS3 const uint32_t* win, uint32_t pos, uint32_t w0) {
+    const Ins<F> I{win + pos};
+    AS3 const uint32_t* ip = win + pos;
+    const uint32_t op = w0 & 0xFFu, fl = (w0 >> 8) & 0xFFFFu;
     if (op < 64u) {
       F a, b = F(0), r;
-      if (fl & F_A_IMM) a = imm_of<F>(w2, w3); else R.rd(w2, &a);
-      if (fl & F_HAS_B) { if (fl & F_B_IMM) b = imm_of<F>(w4, w5); else R.rd(w4, &b); }
-      r = (op == OP_POWI) ? s_powi(a, (int32_t)w7) : s_eval<F>(op, a, b);
-      R.wr(w1, &r);
+      if (fl & F_A_IMM) a = I.imm(2); else R.rd(I.v(2), &a);
+      if (fl & F_HAS_B) { if (fl & F_B_IMM) b = I.imm(4); else R.rd(I.v(4), &b); }
+      r = (op == OP_POWI) ? s_powi(a, (int32_t)I.u(7)) : s_eval<F>(op, a, b);
+      R.wr(I.v(1), &r);
     } else if (op == I_DOT) {
-      const Terms<F> T(L, w3);
+      const uint32_t n = I.u(2), dst = I.v(1);
       F acc = F(0);
       bool have = (fl & F_DOT_CONT) != 0;
-      if (have) R.rd(w1, &acc);
+      if (have) R.rd(dst, &acc);
       uint32_t k = 0;
-      while (k < w2) {
-        if (T.run(k) >= 8) {
-          uint32_t rr[8];
-          F cc[8], v[8];
+      while (k < n) {
+        const uint32_t m = uni(Term<F>(ip, k).meta());
+        if (((m >> M_RUN_SHIFT) & 0xFFu) >= 8) {
+          F v[8];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) { rr[j] = T.reg[k + j]; cc[j] = T.imm(k + j); }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) R.rd(rr[j], &v[j]);
+          for (int j = 0; j < 8; ++j) R.rd(Term<F>(ip, k + j).reg(), &v[j]);
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const F t = v[j] * cc[j];
+            const F t = v[j] * Term<F>(ip, k + j).imm();
             acc = (j == 0 && !have) ? t : acc + t;
           }
           have = true;
           k += 8;
         } else {
-          const uint32_t kind = T.kind(k);
-          const F c = T.imm(k);
+          const Term<F> T(ip, k);
+          const uint32_t kind = m & 3u;
+          const F c = T.imm();
           F term = c;
           if (kind != T_IMM) {
             F v;
-            R.rd(T.reg[k], &v);
+            R.rd(T.reg(), &v);
             term = (kind == T_PROD) ? v * c : v;
           }
           acc = have ? acc + term : term;
@@ -195,83 +240,93 @@ __global__ __launch_bounds__(256) void k_scalar(Launch L) {
           ++k;
         }
       }
-      R.wr(w1, &acc);
+      R.wr(dst, &acc);
     } else if (op >= I_REV_BASE) {
       const uint32_t top = op - I_REV_BASE;
       F g;
-      R.rd(w1, &g);
+      R.rd(I.v(1), &g);
       const bool live = g != F(0);
       F a = F(0), b = F(0), r = F(0);
-      if (fl & F_A_IMM) a = imm_of<F>(w2, w3); else if (fl & F_RD_A) R.rd(w2, &a);
-      if (fl & F_B_IMM) b = imm_of<F>(w4, w5); else if (fl & F_RD_B) R.rd(w4, &b);
-      if (fl & F_RD_R) R.rd(w6, &r);
+      if (fl & F_A_IMM) a = I.imm(2); else if (fl & F_RD_A) R.rd(I.v(2), &a);
+      if (fl & F_B_IMM) b = I.imm(4); else if (fl & F_RD_B) R.rd(I.v(4), &b);
+      if (fl & F_RD_R) R.rd(I.v(6), &r);
       F da, db;
-      if (top == OP_POWI) { da = s_powi_partial((int32_t)w7, a, r); db = F(0); }
+      if (top == OP_POWI) { da = s_powi_partial((int32_t)I.u(7), a, r); db = F(0); }
       else s_partials<F>(top, a, b, r, da, db);
       if (fl & F_ACC_A) {
         F c = F(0);
-        if (!(fl & F_FIRST_A)) R.rd(w3, &c);
+        if (!(fl & F_FIRST_A)) R.rd(I.v(3), &c);
         if (live && da != F(0)) c = c + da * g;
-        R.wr(w3, &c);
+        R.wr(I.v(3), &c);
       }
       if (fl & F_ACC_B) {
         F c = F(0);
-        if (!(fl & F_FIRST_B)) R.rd(w5, &c);
+        if (!(fl & F_FIRST_B)) R.rd(I.v(5), &c);
         if (live && db != F(0)) c = c + db * g;
-        R.wr(w5, &c);
+        R.wr(I.v(5), &c);
       }
     } else if (op == I_DOT_REV) {
-      const Terms<F> T(L, w3);
+      const uint32_t n = I.u(2);
       F g;
-      R.rd(w1, &g);
+      R.rd(I.v(1), &g);
       const bool live = g != F(0);
       uint32_t k = 0;
-      while (k < w2) {
-        if (T.run(k) >= 8) {
+      while (k < n) {
+        const uint32_t m0 = uni(Term<F>(ip, k).meta());
+        if (((m0 >> M_RUN_SHIFT) & 0xFFu) >= 8) {
           // eight product terms with distinct adjoint registers; a first
           // contribution reads its register anyway and discards the value
-          uint32_t mm[8], rr[8];
-          F cc[8], cur[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) { mm[j] = T.meta[k + j]; rr[j] = T.reg[k + j]; cc[j] = T.imm(k + j); }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) R.rd(rr[j], &cur[j]);
+          // all term words are read before the first register write, which
+          // the compiler must otherwise assume may alias the window
+          F cur[8], cj[8];
+          uint32_t rj[8], mj[8];
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            F c0 = (mm[j] & M_FIRST) ? F(0) : cur[j];
-            if (live && cc[j] != F(0)) c0 = c0 + cc[j] * g;
-            R.wr(rr[j], &c0);
+            const Term<F> T(ip, k + j);
+            rj[j] = T.reg();
+            mj[j] = T.meta();
+            cj[j] = T.imm();
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) R.rd(rj[j], &cur[j]);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            F c0 = (mj[j] & M_FIRST) ? F(0) : cur[j];
+            if (live && cj[j] != F(0)) c0 = c0 + cj[j] * g;
+            R.wr(rj[j], &c0);
           }
           k += 8;
         } else {
-          const uint32_t m = T.meta[k];
-          const F c = T.imm(k);
+          const Term<F> T(ip, k);
+          const F c = T.imm();
           F cur = F(0);
-          if (!(m & M_FIRST)) R.rd(T.reg[k], &cur);
-          if ((m & 3u) == T_PROD) {
+          if (!(m0 & M_FIRST)) R.rd(T.reg(), &cur);
+          if ((m0 & 3u) == T_PROD) {
             if (live && c != F(0)) cur = cur + c * g;
           } else if (live) {
             cur = cur + F(1) * g;
           }
-          R.wr(T.reg[k], &cur);
+          R.wr(T.reg(), &cur);
           ++k;
         }
       }
     } else if (op == I_LOADN) {
-      load_inputs<F, 1, LDSONLY>(R, L, w2, w3, [&](uint32_t i, F* v) { v[0] = act ? X[p * L.xstride + i] : F(0); });
+      load_inputs<F, 1, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
+        v[0] = act ? X[p * L.xstride + i] : F(0);
+      });
     } else if (op == I_STORE_OUT) {
       F v;
-      if (fl & F_A_IMM) v = imm_of<F>(w2, w3); else R.rd(w2, &v);
-      if (act && Y) Y[p * L.nout + w1] = v;
+      if (fl & F_A_IMM) v = I.imm(2); else R.rd(I.v(2), &v);
+      if (act && Y) Y[p * L.nout + I.u(1)] = v;
     } else if (op == I_SEED) {
       F one = F(1);
-      R.wr(w1, &one);
+      R.wr(I.v(1), &one);
     } else if (op == I_STORE_GRAD) {
       F v;
-      if (fl & F_A_IMM) v = imm_of<F>(w2, w3); else R.rd(w2, &v);
-      if (act && G) G[p * L.nin + w1] = v;
+      if (fl & F_A_IMM) v = I.imm(2); else R.rd(I.v(2), &v);
+      if (act && G) G[p * L.nin + I.u(1)] = v;
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -294,68 +349,68 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
   F* H = reinterpret_cast<F*>(L.h);
   auto rdd = [&](uint32_t r) { Dual<F> d; F t[2]; R.rd(r, t); d.re = t[0]; d.eps = t[1]; return d; };
   auto wrd = [&](uint32_t r, Dual<F> d) { F t[2] = {d.re, d.eps}; R.wr(r, t); };
-  Ins nxt = fetch(L.prog, 0);
-  for (uint32_t pc = 0; pc < L.ninstr; ++pc) {
-    const Ins cur = nxt;
-    if (pc + 1 < L.ninstr) nxt = fetch(L.prog, pc + 1);
-    const uint32_t w0 = cur.w[0], w1 = cur.w[1], w2 = cur.w[2], w3 = cur.w[3], w4 = cur.w[4], w5 = cur.w[5],
-                   w6 = cur.w[6], w7 = cur.w[7];
-    const uint32_t op = w0 & 0xFFu, fl = w0 >> 8;
+  run_stream(L, [&](AS3 const uint32_t* win, uint32_t pos, uint32_t w0) {
+    const Ins<F> I{win + pos};
+    AS3 const uint32_t* ip = win + pos;
+    const uint32_t op = w0 & 0xFFu, fl = (w0 >> 8) & 0xFFFFu;
     if (op < 64u) {
       Dual<F> a, b = dc(F(0));
-      a = (fl & F_A_IMM) ? dc(imm_of<F>(w2, w3)) : rdd(w2);
-      if (fl & F_HAS_B) b = (fl & F_B_IMM) ? dc(imm_of<F>(w4, w5)) : rdd(w4);
-      Dual<F> r = (op == OP_POWI) ? d_powi(a, (int32_t)w7) : d_eval<F>(op, a, b);
-      wrd(w1, r);
+      a = (fl & F_A_IMM) ? dc(I.imm(2)) : rdd(I.v(2));
+      if (fl & F_HAS_B) b = (fl & F_B_IMM) ? dc(I.imm(4)) : rdd(I.v(4));
+      Dual<F> r = (op == OP_POWI) ? d_powi(a, (int32_t)I.u(7)) : d_eval<F>(op, a, b);
+      wrd(I.v(1), r);
     } else if (op == I_DOT) {
-      const Terms<F> T(L, w3);
+      const uint32_t n = I.u(2), dst = I.v(1);
       Dual<F> acc = dc(F(0));
       bool have = (fl & F_DOT_CONT) != 0;
-      if (have) acc = rdd(w1);
-      for (uint32_t k = 0; k < w2; ++k) {
-        const uint32_t m = T.meta[k], kind = m & 3u;
-        const Dual<F> c = dc(T.imm(k));
+      if (have) acc = rdd(dst);
+      for (uint32_t k = 0; k < n; ++k) {
+        const Term<F> T(ip, k);
+        const uint32_t m = uni(T.meta()), kind = m & 3u;
+        const Dual<F> c = dc(T.imm());
         Dual<F> term = c;
         if (kind != T_IMM) {
-          Dual<F> v = rdd(T.reg[k]);
+          Dual<F> v = rdd(T.reg());
           term = (kind == T_PROD) ? ((m & M_CONST_LEFT) ? d_mul(c, v) : d_mul(v, c)) : v;
         }
         acc = have ? d_add(acc, term) : term;
         have = true;
       }
-      wrd(w1, acc);
+      wrd(dst, acc);
     } else if (op >= I_REV_BASE) {
       const uint32_t top = op - I_REV_BASE;
-      const Dual<F> g = rdd(w1);
+      const Dual<F> g = rdd(I.v(1));
       const bool live = !d_zero(g);
       Dual<F> a = dc(F(0)), b = dc(F(0)), r = dc(F(0));
-      if (fl & F_A_IMM) a = dc(imm_of<F>(w2, w3)); else if (fl & F_RD_A) a = rdd(w2);
-      if (fl & F_B_IMM) b = dc(imm_of<F>(w4, w5)); else if (fl & F_RD_B) b = rdd(w4);
-      if (fl & F_RD_R) r = rdd(w6);
+      if (fl & F_A_IMM) a = dc(I.imm(2)); else if (fl & F_RD_A) a = rdd(I.v(2));
+      if (fl & F_B_IMM) b = dc(I.imm(4)); else if (fl & F_RD_B) b = rdd(I.v(4));
+      if (fl & F_RD_R) r = rdd(I.v(6));
       Dual<F> da, db;
-      if (top == OP_POWI) { da = d_powi_partial((int32_t)w7, a, r); db = dc(F(0)); }
+      if (top == OP_POWI) { da = d_powi_partial((int32_t)I.u(7), a, r); db = dc(F(0)); }
       else d_partials<F>(top, a, b, r, da, db);
       if (fl & F_ACC_A) {
-        Dual<F> c = (fl & F_FIRST_A) ? dc(F(0)) : rdd(w3);
+        Dual<F> c = (fl & F_FIRST_A) ? dc(F(0)) : rdd(I.v(3));
         if (live && !d_zero(da)) c = d_add(c, d_mul(da, g));
-        wrd(w3, c);
+        wrd(I.v(3), c);
       }
       if (fl & F_ACC_B) {
-        Dual<F> c = (fl & F_FIRST_B) ? dc(F(0)) : rdd(w5);
+        Dual<F> c = (fl & F_FIRST_B) ? dc(F(0)) : rdd(I.v(5));
         if (live && !d_zero(db)) c = d_add(c, d_mul(db, g));
-        wrd(w5, c);
+        wrd(I.v(5), c);
       }
     } else if (op == I_DOT_REV) {
-      const Terms<F> T(L, w3);
-      const Dual<F> g = rdd(w1);
+      const uint32_t n = I.u(2);
+      const Dual<F> g = rdd(I.v(1));
       const bool live = !d_zero(g);
       Dual<F> gd = g;
       uint32_t depth = 0;
-      for (uint32_t k = 0; k < w2; ++k) {
-        const uint32_t m = T.meta[k], kind = m & 3u, tdepth = m >> M_DEPTH_SHIFT;
-        const F c = T.imm(k);
+      for (uint32_t k = 0; k < n; ++k) {
+        const Term<F> T(ip, k);
+        const uint32_t m = uni(T.meta()), kind = m & 3u, tdepth = m >> M_DEPTH_SHIFT;
+        const F c = T.imm();
+        if (tdepth < depth) { gd = g; depth = 0; }
         while (depth < tdepth) { gd = dual_norm(gd); ++depth; }
-        Dual<F> cur = (m & M_FIRST) ? dc(F(0)) : rdd(T.reg[k]);
+        Dual<F> cur = (m & M_FIRST) ? dc(F(0)) : rdd(T.reg());
         if (live) {
           if (kind == T_PROD) {
             if (c != F(0)) cur = d_add(cur, d_mul(dc(c), dual_norm(gd)));
@@ -363,10 +418,10 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
             cur = d_add(cur, d_mul(dc(F(1)), gd));
           }
         }
-        wrd(T.reg[k], cur);
+        wrd(T.reg(), cur);
       }
     } else if (op == I_LOADN) {
-      load_inputs<F, 2, LDSONLY>(R, L, w2, w3, [&](uint32_t i, F* v) {
+      load_inputs<F, 2, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
         v[0] = F(0);
         v[1] = F(0);
         if (act) {
@@ -376,13 +431,13 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
         }
       });
     } else if (op == I_SEED) {
-      wrd(w1, dc(F(1)));
+      wrd(I.v(1), dc(F(1)));
     } else if (op == I_STORE_GRAD) {
-      Dual<F> v = (fl & F_A_IMM) ? dc(imm_of<F>(w2, w3)) : rdd(w2);
-      if (act) { G[p * L.nin + w1] = v.re; H[p * L.nin + w1] = v.eps; }
+      Dual<F> v = (fl & F_A_IMM) ? dc(I.imm(2)) : rdd(I.v(2));
+      if (act) { G[p * L.nin + I.u(1)] = v.re; H[p * L.nin + I.u(1)] = v.eps; }
     }
     // I_STORE_OUT: the HVP entry points return gradients and products only.
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -396,36 +451,38 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
   const F* X = reinterpret_cast<const F*>(L.x);
   const F* V = reinterpret_cast<const F*>(L.dirs);
   F* Y = reinterpret_cast<F*>(L.y);
-  Ins nxt = fetch(L.prog, 0);
-  for (uint32_t pc = 0; pc < L.ninstr; ++pc) {
-    const Ins cur = nxt;
-    if (pc + 1 < L.ninstr) nxt = fetch(L.prog, pc + 1);
-    const uint32_t w0 = cur.w[0], w1 = cur.w[1], w2 = cur.w[2], w3 = cur.w[3], w4 = cur.w[4], w5 = cur.w[5],
-                   w7 = cur.w[7];
-    const uint32_t op = w0 & 0xFFu, fl = w0 >> 8;
+  run_stream(L, [&](AS3 const uint32_t* win, uint32_t pos, uint32_t w0) {
+    const Ins<F> I{win + pos};
+    AS3 const uint32_t* ip = win + pos;
+    const uint32_t op = w0 & 0xFFu, fl = (w0 >> 8) & 0xFFFFu;
     if (op < 64u) {
       Jet<F, K> a, b = jc<F, K>(F(0));
-      if (fl & F_A_IMM) a = jc<F, K>(imm_of<F>(w2, w3)); else R.rd(w2, a.c);
-      if (fl & F_HAS_B) { if (fl & F_B_IMM) b = jc<F, K>(imm_of<F>(w4, w5)); else R.rd(w4, b.c); }
-      Jet<F, K> r = (op == OP_POWI) ? j_powi<F, K>(a, (int32_t)w7) : j_eval<F, K>(op, a, b);
-      R.wr(w1, r.c);
+      if (fl & F_A_IMM) a = jc<F, K>(I.imm(2)); else R.rd(I.v(2), a.c);
+      if (fl & F_HAS_B) { if (fl & F_B_IMM) b = jc<F, K>(I.imm(4)); else R.rd(I.v(4), b.c); }
+      Jet<F, K> r = (op == OP_POWI) ? j_powi<F, K>(a, (int32_t)I.u(7)) : j_eval<F, K>(op, a, b);
+      R.wr(I.v(1), r.c);
     } else if (op == I_DOT) {
-      const Terms<F> T(L, w3);
+      const uint32_t n = I.u(2), dst = I.v(1);
       Jet<F, K> acc = jc<F, K>(F(0));
       bool have = (fl & F_DOT_CONT) != 0;
-      if (have) R.rd(w1, acc.c);
+      if (have) R.rd(dst, acc.c);
       uint32_t k = 0;
-      while (k < w2) {
-        const uint32_t blk = (T.run(k) >= 4) ? 4u : 1u;
+      while (k < n) {
+        const uint32_t m0 = uni(Term<F>(ip, k).meta());
+        const uint32_t blk = (((m0 >> M_RUN_SHIFT) & 0xFFu) >= 4) ? 4u : 1u;
         Jet<F, K> v[4];
+        if (blk == 4) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if ((uint32_t)j < blk && T.kind(k + j) != T_IMM) R.rd(T.reg[k + j], v[j].c);
+          for (int j = 0; j < 4; ++j) R.rd(Term<F>(ip, k + j).reg(), v[j].c);
+        } else if ((m0 & 3u) != T_IMM) {
+          R.rd(Term<F>(ip, k).reg(), v[0].c);
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           if ((uint32_t)j >= blk) break;
-          const uint32_t m = T.meta[k + j], kind = m & 3u;
-          const Jet<F, K> c = jc<F, K>(T.imm(k + j));
+          const Term<F> T(ip, k + j);
+          const uint32_t m = uni(T.meta()), kind = m & 3u;
+          const Jet<F, K> c = jc<F, K>(T.imm());
           Jet<F, K> term = c;
           if (kind == T_PROD) {
             if (m & M_CONST_LEFT) jt::mul<F, K>(c.c, v[j].c, term.c);
@@ -443,9 +500,9 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
         }
         k += blk;
       }
-      R.wr(w1, acc.c);
+      R.wr(dst, acc.c);
     } else if (op == I_LOADN) {
-      load_inputs<F, K, LDSONLY>(R, L, w2, w3, [&](uint32_t i, F* v) {
+      load_inputs<F, K, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = F(0);
         if (act) {
@@ -457,13 +514,14 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
       });
     } else if (op == I_STORE_OUT) {
       Jet<F, K> v;
-      if (fl & F_A_IMM) v = jc<F, K>(imm_of<F>(w2, w3)); else R.rd(w2, v.c);
+      if (fl & F_A_IMM) v = jc<F, K>(I.imm(2)); else R.rd(I.v(2), v.c);
       if (act) {
+        const uint32_t j = I.u(1);
 #pragma unroll
-        for (int q = 0; q < K; ++q) Y[(p * L.nout + w1) * K + q] = v.c[q];
+        for (int q = 0; q < K; ++q) Y[(p * L.nout + j) * K + q] = v.c[q];
       }
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -473,7 +531,7 @@ static hipError_t launch_common(const void* fn, const Launch& L, uint32_t wg, si
   uint64_t grid = (L.B + wg - 1) / wg;
   if (grid == 0) grid = 1;
   void* args[] = {const_cast<Launch*>(&L)};
-  return hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(wg), args, lds, s);
+  return hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(wg), args, lds + 2 * WIN_BYTES, s);
 }
 
 #define EHIP_PICK(KERNEL, ...)                                                                     \

@@ -670,4 +670,72 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
   return out;
 }
 
+
+std::vector<uint32_t> encode_stream(const Program& p, bool f64) {
+  const uint32_t IW = f64 ? 2u : 1u;
+  const uint32_t TWW = 2 + IW;                     // words per inline term
+  const uint32_t MAXLEN = std::min<uint32_t>(255u, WIN_WORDS - 1);
+  const uint32_t max_terms = (MAXLEN - 3) / TWW;
+  const uint32_t T = p.nterms;
+  std::vector<uint32_t> out;
+  out.reserve(p.words.size() + (size_t)T * TWW + WIN_WORDS);
+  uint32_t used = 0;  // words used in the current window
+  auto need = [&](uint32_t len) {
+    // leave room for the closing I_WEND
+    if (used + len + 1 > WIN_WORDS) {
+      out.push_back(I_WEND);
+      ++used;
+      while (used < WIN_WORDS) { out.push_back(0u); ++used; }
+      used = 0;
+    }
+  };
+  auto emit = [&](const uint32_t* w, uint32_t len) {
+    need(len);
+    out.insert(out.end(), w, w + len);
+    used += len;
+  };
+  for (uint32_t i = 0; i < p.ninstr; ++i) {
+    const uint32_t* w = &p.words[(size_t)i * NW];
+    const uint32_t op = w[0] & 0xFFu;
+    const uint32_t fl = (w[0] >> 8) & 0xFFFFu;
+    if (op == I_DOT || op == I_DOT_REV || op == I_LOADN) {
+      const uint32_t n = w[2], off = w[3];
+      uint32_t done = 0;
+      while (done < n || (n == 0 && done == 0)) {
+        const uint32_t k = std::min(max_terms, n - done);
+        uint32_t pfl = fl;
+        if (op == I_DOT && done > 0) pfl |= F_DOT_CONT;
+        std::vector<uint32_t> ins;
+        ins.push_back(op | (pfl << 8) | ((3 + k * TWW) << 24));
+        ins.push_back(w[1]);
+        ins.push_back(k);
+        for (uint32_t j = 0; j < k; ++j) {
+          const uint32_t t = off + done + j;
+          uint32_t meta = p.terms[t];
+          // runs must not cross a piece boundary
+          uint32_t run = (meta >> M_RUN_SHIFT) & 0xFFu;
+          run = std::min(run, k - j);
+          meta = (meta & ~(0xFFu << M_RUN_SHIFT)) | (run << M_RUN_SHIFT);
+          ins.push_back(meta);
+          ins.push_back(p.terms[T + t]);
+          for (uint32_t q = 0; q < IW; ++q) ins.push_back(p.terms[2 * T + (size_t)t * IW + q]);
+        }
+        emit(ins.data(), (uint32_t)ins.size());
+        done += k;
+        if (n == 0) break;
+      }
+    } else {
+      uint32_t ins[NW];
+      std::memcpy(ins, w, sizeof ins);
+      ins[0] = (ins[0] & 0x00FFFFFFu) | (NW << 24);
+      emit(ins, NW);
+    }
+  }
+  need(1);
+  out.push_back(I_END);
+  ++used;
+  while (used < WIN_WORDS) { out.push_back(0u); ++used; }
+  return out;
+}
+
 }  // namespace ehip

@@ -48,6 +48,8 @@ enum : uint32_t {
   I_DOT = 68,
   I_DOT_REV = 69,
   I_LOADN = 70,
+  I_WEND = 71,   // end of a stream window: continue with the next one
+  I_END = 72,    // end of the program
   I_REV_BASE = 128,
 };
 
@@ -101,6 +103,17 @@ struct Program {
 };
 
 enum ProgMode { PM_FORWARD = 0, PM_GRADIENT = 1 };
+
+// Windowed instruction stream executed by the interpreters.  The program is
+// cut into windows of WIN_WORDS u32 words that a workgroup stages in LDS (two
+// buffers: one executing, the next arriving).  Every instruction lies inside
+// one window; its first word is op | flags << 8 | length << 24.  Ordinary
+// instructions keep their 8 words; DOT / DOT_REV / LOADN carry their terms
+// inline after a 3-word header (w1 register, w2 term count):
+// (meta, reg, imm[IW]) per term, and a chain longer than a window allows is
+// split into continuation pieces.  I_WEND closes a window, I_END the program.
+constexpr uint32_t WIN_WORDS = 512;
+std::vector<uint32_t> encode_stream(const Program& p, bool f64);
 
 // Build the program; `fuse` enables DOT fusion.  Never fails on a validated
 // tape.

@@ -166,11 +166,14 @@ int32_t ehip_laplacian_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const voi
 int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
 
 /* Compile a tape into its register program without a device (inspection
- * and tests).  Same validation as ehip_tape_upload.  Writes up to
- * `words_cap` program words and `terms_cap` side-table words; out4 =
- * {instructions, registers, program words, side-table entries T}; the side
- * table needs max(4, T*(2+IW)) words (IW = 1 f32, 2 f64).  Returns EHIP_OTHER
- * if a buffer is too small (out4 still holds the sizes). */
+ * and tests).  Same validation as ehip_tape_upload.  Mode 0 forward, 1
+ * gradient/HVP: writes up to `words_cap` program words (8 per instruction)
+ * and `terms_cap` side-table words; out4 = {instructions, registers, program
+ * words, side-table entries T}; the side table needs max(4, T*(2+IW)) words
+ * (IW = 1 f32, 2 f64).  Mode 2 forward, 3 gradient: writes the windowed
+ * instruction stream the kernels execute (512-word windows, dot terms
+ * inline) into `words`, out4[2] = its length; `terms` is unused.  Returns
+ * EHIP_OTHER if a buffer is too small (out4 still holds the sizes). */
 int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                              const double* constants, uint32_t num_ops, uint32_t num_inputs,
                              uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,
@@ -180,6 +183,24 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
 /* Launch tuning knobs (autotuner): workgroup size (64/128/256) and the
  * per-workgroup LDS budget for the register file in bytes.  0 = default. */
 int32_t ehip_set_launch(ehip_ctx* ctx, uint32_t wg_size, uint32_t lds_budget_bytes);
+
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed at
+ * 256 threads per block in src/gpu/cuda_backend.rs).  Times every launch
+ * shape -- workgroup size 64/128/256 x LDS budget for the register file --
+ * on the caller's device buffers for evaluation `what` (0 forward, 1
+ * gradient, 2 hvp, 3 taylor of `order`), keeps the fastest on the tape for
+ * later calls of that kind, and returns out3 = {workgroup size, LDS budget
+ * bytes, microseconds per launch}.  Buffers are as for the matching *_dev
+ * entry point (d_y may be null for gradient/hvp); their contents are
+ * overwritten.  Runs on the context stream and synchronizes. */
+int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t order, const void* d_x,
+                          uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
+                          uint32_t* out3);
+
+/* Restore (wg != 0) or clear (wg == 0) a tuned launch shape, e.g. from a
+ * persisted autotuner cache; ehip_get_tuned reads it back (out2 = {wg, lds}). */
+int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget);
+int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out2);
 
 /* Dot-product fusion for tapes uploaded after this call (default on).  Off
  * runs every tape entry as its own instruction; results are identical. */

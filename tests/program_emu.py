@@ -124,17 +124,79 @@ def split_terms(terms, nterms, f64):
     return out
 
 
+WIN_WORDS = 512
+I_WEND, I_END = 71, 72
+
+
+def decode_stream(stream, f64):
+    """Windowed stream -> [(w[8], inline terms)], checking the window layout.
+
+    Every instruction lies inside one window, a window ends with I_WEND (or
+    the final I_END) followed by zero padding, and the length field of each
+    header matches its contents.
+    """
+    st = [int(v) for v in np.asarray(stream, dtype=np.uint32)]
+    assert len(st) % WIN_WORDS == 0 and st, "stream is not whole windows"
+    tww = 4 if f64 else 3
+    out = []
+    nwin = len(st) // WIN_WORDS
+    for wi in range(nwin):
+        win = st[wi * WIN_WORDS:(wi + 1) * WIN_WORDS]
+        pos = 0
+        while True:
+            assert pos < WIN_WORDS, "instruction runs past its window"
+            w0 = win[pos]
+            op = w0 & 0xFF
+            if op in (I_WEND, I_END):
+                assert (op == I_END) == (wi == nwin - 1), "misplaced stream end"
+                assert all(v == 0 for v in win[pos + 1:]), "window padding is not zero"
+                break
+            ln = w0 >> 24
+            assert ln >= 3 and pos + ln < WIN_WORDS, "instruction crosses its window"
+            ins = win[pos:pos + ln]
+            head = [ins[0] & 0x00FFFFFF] + ins[1:min(ln, 8)] + [0] * max(0, 8 - ln)
+            tl = []
+            if op in (I_DOT, I_DOT_REV, I_LOADN):
+                k = ins[2]
+                assert ln == 3 + k * tww, "term count does not match length"
+                for j in range(k):
+                    t = ins[3 + j * tww:3 + (j + 1) * tww]
+                    tl.append((t[0], t[1], t[2], t[3] if f64 else 0))
+                head = [ins[0] & 0x00FFFFFF, ins[1], k, 0, 0, 0, 0, 0]
+            else:
+                assert ln == 8, "ordinary instructions are 8 words"
+            out.append((head, tl))
+            pos += ln
+    return out
+
+
 def run(words, terms, x, nin, nout, nregs, f64=True, mode=1, nterms=None):
     """Run the program over points ``x`` [B, nin]; returns (y [B,nout], g [B,nin])."""
+    tl = split_terms(terms, nterms, f64)
+    instrs = []
+    for w in words:
+        w = [int(v) for v in w]
+        op = w[0] & 0xFF
+        if op in (I_DOT, I_DOT_REV, I_LOADN):
+            instrs.append((w, tl[w[3]:w[3] + w[2]]))
+        else:
+            instrs.append((w, []))
+    return execute(instrs, x, nin, nout, nregs, f64)
+
+
+def run_stream(stream, x, nin, nout, nregs, f64=True):
+    """Run a windowed stream (``ehip_compile_program`` modes 2/3)."""
+    return execute(decode_stream(stream, f64), x, nin, nout, nregs, f64)
+
+
+def execute(instrs, x, nin, nout, nregs, f64=True):
     F = np.float64 if f64 else np.float32
-    terms = split_terms(terms, nterms, f64)
     B = x.shape[0]
     regs = np.full((nregs, B), np.nan, dtype=F)
     y = np.zeros((B, nout), F)
     g = np.zeros((B, nin), F)
     with np.errstate(all="ignore"):
-        for w in words:
-            w = [int(v) for v in w]
+        for w, terms in instrs:
             op, fl = w[0] & 0xFF, w[0] >> 8
 
             def val(lo, hi, imm_flag):
@@ -173,12 +235,12 @@ def run(words, terms, x, nin, nout, nregs, f64=True, mode=1, nterms=None):
                     regs[w[5]] = cur
             elif op == I_LOADN:
                 for k in range(w[2]):
-                    t0, t1, t2, t3 = terms[w[3] + k]
+                    t0, t1, t2, t3 = terms[k]
                     regs[t1] = x[:, t2]
             elif op == I_DOT:
                 acc = regs[w[1]].copy() if fl & F_DOT_CONT else None
                 for k in range(w[2]):
-                    t0, t1, t2, t3 = terms[w[3] + k]
+                    t0, t1, t2, t3 = terms[k]
                     c = F(_imm(t2, t3, f64))
                     kind = t0 & 0x3
                     term = (regs[t1] * c if kind == T_PROD else regs[t1].copy()) if kind != T_IMM else np.full(B, c, F)
@@ -188,7 +250,7 @@ def run(words, terms, x, nin, nout, nregs, f64=True, mode=1, nterms=None):
                 gg = regs[w[1]].copy()
                 live = gg != 0
                 for k in range(w[2]):
-                    t0, t1, t2, t3 = terms[w[3] + k]
+                    t0, t1, t2, t3 = terms[k]
                     c = F(_imm(t2, t3, f64))
                     kind = t0 & 0x3
                     cur = np.zeros(B, F) if (t0 >> 2) & 1 else regs[t1].copy()

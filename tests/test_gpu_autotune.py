@@ -1,0 +1,71 @@
+"""Autotuner and launch shapes: every workgroup size and LDS budget (all
+registers in LDS, split LDS/HBM, all in HBM) computes bit-identical results,
+because each lane runs the same arithmetic in the same order; only where the
+registers live changes.  The tuned choice is kept per tape and evaluation kind.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from echidna_amd import philox
+from echidna_amd import workloads as wl
+from echidna_amd.backend import TUNE_GRADIENT, TUNE_HVP, TUNE_TAYLOR, HipContext
+from echidna_amd.tape import GpuError, GpuTapeData
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 0), (64, 1024), (128, 8192), (256, 16384), (256, 131072), (128, 65536)]
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_launch_shapes_are_bit_identical(hip, f64):
+    tape = wl.tanh_mlp(12, 40, 31)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    F = np.float64 if f64 else np.float32
+    B = 3000
+    x = wl.points_normal(8, B, 12).astype(F)
+    v = philox.normal(9, philox.STREAM_DIRS, np.arange(B * 12)).reshape(B, 12).astype(F)
+    ref = None
+    for wg, lds in SHAPES:
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, wg, lds)
+        HipContext.set_tuned(bufs, TUNE_HVP, wg, lds)
+        HipContext.set_tuned(bufs, TUNE_TAYLOR, wg, lds, order=3)
+        y, g = hip.gradient_batch(bufs, x.reshape(-1), B)
+        gh, hv = hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B)
+        tk = hip.taylor_forward_kth_batch(bufs, x.reshape(-1), v.reshape(-1), B, 3)
+        out = [np.asarray(a).copy() for a in (y, g, gh, hv, *tk.coefficients)]
+        if ref is None:
+            ref = out
+        else:
+            for a, b in zip(out, ref):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (wg, lds)
+
+
+def test_autotune_keeps_choice_and_results(hip):
+    tape = wl.tanh_mlp(16, 32, 32)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    B = 1 << 14
+    x = torch.from_numpy(wl.points_normal(10, B, 16).astype(np.float32)).cuda()
+    y = torch.empty(B, dtype=torch.float32, device="cuda")
+    g = torch.empty(B, 16, dtype=torch.float32, device="cuda")
+    y0, g0 = hip.gradient_batch(bufs, x.cpu().numpy().reshape(-1), B)
+    res = hip.autotune_dev(bufs, TUNE_GRADIENT, B, x, d_y=y, d_g=g)
+    assert res["wg"] in (64, 128, 256) and res["us"] > 0
+    assert HipContext.get_tuned(bufs, TUNE_GRADIENT) == {"wg": res["wg"], "lds_budget": res["lds_budget"]}
+    y1, g1 = hip.gradient_batch(bufs, x.cpu().numpy().reshape(-1), B)
+    assert np.array_equal(y0, y1) and np.array_equal(g0, g1)
+    HipContext.set_tuned(bufs, TUNE_GRADIENT, 0, 0)
+    assert HipContext.get_tuned(bufs, TUNE_GRADIENT)["wg"] == 0
+
+
+def test_autotune_rejects_bad_arguments(hip):
+    tape = wl.tanh_mlp(4, 4, 33)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    with pytest.raises(GpuError, match="workgroup size"):
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 96, 0)
+    with pytest.raises(GpuError, match="unknown evaluation kind"):
+        HipContext.set_tuned(bufs, 7, 64, 0)
+    x = torch.zeros(4, 4, device="cuda")
+    with pytest.raises(GpuError, match="taylor order"):
+        hip.autotune_dev(bufs, TUNE_TAYLOR, 4, x, d_y=x, d_dirs=x, order=9)

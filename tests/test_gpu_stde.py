@@ -61,9 +61,19 @@ def test_laplacian_with_control_matches_oracle(hip):
     ref = po.laplacian_with_control(tape, x, dirs, diag_ref)
     assert close(r.estimate, ref["estimate"])
     assert close(r.sample_variance, ref["sample_variance"], 1e-9)
-    # with the exact diagonal, Rademacher residuals carry only off-diagonal terms
+    # Rademacher v_j^2 = 1, so the diagonal control shifts every sample by the
+    # same constant: the variance is unchanged and the estimate agrees
     plain = sg.laplacian_gpu(hip, bufs, x, dirs)
-    assert r.sample_variance < plain.sample_variance
+    assert close(r.sample_variance, plain.sample_variance, 1e-9)
+    assert close(r.estimate, plain.estimate, 1e-9)
+    # with Gaussian directions the exact diagonal removes the diagonal part of
+    # the variance (2 sum_j H_jj^2 of 2 ||H||_F^2)
+    g = philox.normal(6, philox.STREAM_DIRS, np.arange(512 * 10)).reshape(512, 10)
+    rg = sg.laplacian_with_control_gpu(hip, bufs, x, g, diag_ref)
+    pg = sg.laplacian_gpu(hip, bufs, x, g)
+    refg = po.laplacian_with_control(tape, x, g, diag_ref)
+    assert close(rg.sample_variance, refg["sample_variance"], 1e-9)
+    assert rg.sample_variance < pg.sample_variance
 
 
 @pytest.mark.parametrize("kind", [sg.DIR_RADEMACHER, sg.DIR_GAUSSIAN])

@@ -43,6 +43,31 @@ def test_program_matches_oracle(name, fuse, f64):
             assert np.all(np.abs(g - g_ref) <= tol * gs), (name, mode)
 
 
+@pytest.mark.parametrize("name", list(TAPES) + ["wide_mlp"])
+@pytest.mark.parametrize("f64", [True, False])
+def test_windowed_stream_runs_like_the_program(name, f64):
+    """The windowed stream (what the kernels execute) decodes into windows
+    that hold whole instructions and computes bit-identical results; long dot
+    products split into pieces across windows."""
+    tape = wl.tanh_mlp(6, 300, 2) if name == "wide_mlp" else TAPES[name]()
+    d = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
+    F = np.float64 if f64 else np.float32
+    x = (wl.points_normal(6, 7, tape.num_inputs) * 0.7).astype(F)
+    for mode in (0, 1):
+        words, terms, info = compile_program(d, mode=mode, fuse=True, f64=f64)
+        stream, _, sinfo = compile_program(d, mode=mode + 2, fuse=True, f64=f64)
+        assert sinfo["registers"] == info["registers"] and len(stream) % emu.WIN_WORDS == 0
+        y, g = emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64,
+                       nterms=info["nterms"])
+        ys, gs = emu.run_stream(stream, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64)
+        assert np.array_equal(y.view(np.uint8), ys.view(np.uint8)), (name, mode)
+        assert np.array_equal(g.view(np.uint8), gs.view(np.uint8)), (name, mode)
+    if name == "wide_mlp":
+        ins = emu.decode_stream(stream, f64)
+        assert len(stream) // emu.WIN_WORDS > 10
+        assert any(w[0] & 0xFF == emu.I_DOT and (w[0] >> 8) & emu.F_DOT_CONT for w, _ in ins)
+
+
 def test_fusion_collapses_mlp_dot_products():
     tape = wl.tanh_mlp(32, 64, 1)
     d = GpuTapeData.from_tape(tape)
