@@ -70,11 +70,19 @@ _SIGS = {
     "ehip_laplacian_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                                   ctypes.c_uint64, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "ehip_set_fusion": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "ehip_multi_batch_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp]),
     "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
     "ehip_set_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     "ehip_get_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
 }
+
+class EhipJob(ctypes.Structure):
+    """``ehip_job``: one tape's share of a multi-tape queue."""
+
+    _fields_ = [("tape", _vp), ("what", ctypes.c_int32), ("d_x", _vp), ("batch", ctypes.c_uint64),
+                ("d_y", _vp), ("d_g", _vp)]
+
 
 # Evaluation kinds of the autotuner.
 TUNE_FORWARD, TUNE_GRADIENT, TUNE_HVP, TUNE_TAYLOR = 0, 1, 2, 3
@@ -329,6 +337,18 @@ class HipContext:
         _check(load_library().ehip_gradient_batch_dev(self.handle, tape.handle, _vp(d_inputs.data_ptr()), batch,
                                                        _vp(d_outputs.data_ptr()), _vp(d_grads.data_ptr()),
                                                        _vp(stream or None)))
+
+    def multi_batch_dev(self, jobs, wg: int = 0, lds_budget: int = 0, stream: int = 0) -> None:
+        """Run many (tape, inputs, outputs) jobs in one persistent launch.
+
+        ``jobs``: sequence of (tape, what, d_x, batch, d_y, d_g) with device
+        tensors (what 0 forward, 1 gradient; d_y / d_g may be None).
+        """
+        arr = (EhipJob * max(len(jobs), 1))()
+        for k, (tape, what, d_x, batch, d_y, d_g) in enumerate(jobs):
+            arr[k] = EhipJob(tape.handle, what, _dp(d_x), batch, _dp(d_y), _dp(d_g))
+        _check(load_library().ehip_multi_batch_dev(self.handle, ctypes.cast(arr, _vp), len(jobs), wg, lds_budget,
+                                                   _vp(stream or None)))
 
     def autotune_dev(self, tape: "HipTapeBuffers", what: int, batch: int, d_x, d_y=0, d_g=0, d_h=0, d_dirs=0,
                      x_stride: Optional[int] = None, order: int = 0) -> dict:

@@ -83,6 +83,8 @@ struct ehip_ctx {
   DevBuf regs;                 // HBM register file
   DevBuf io[6];                // staging for host-pointer entry points
   DevBuf jets;                 // STDE per-sample jets
+  DevBuf qjobs, qtiles, qcount;  // multi-tape queue tables
+  uint32_t ncu = 256;
   uint32_t wg = 64;
   uint32_t lds_budget = 32768;
   bool fuse = true;
@@ -194,6 +196,7 @@ int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) {
   ehip_ctx* c = new ehip_ctx();
   c->device = device;
   c->max_lds = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 65536;
+  c->ncu = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 256;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -212,6 +215,9 @@ void ehip_ctx_destroy(ehip_ctx* c) {
   }
   c->regs.release();
   c->jets.release();
+  c->qjobs.release();
+  c->qtiles.release();
+  c->qcount.release();
   for (auto& b : c->io) b.release();
   delete c;
 }
@@ -439,6 +445,98 @@ int32_t ehip_taylor_kth_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d
   L.y = d_coeffs;
   rc = run(c, t, 2, (int)order, L, pick(c, stream));
   return rc ? rc : ok();
+}
+
+// ------------------------------ multi-tape queue ------------------------------
+
+int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, uint32_t wg, uint32_t lds_budget,
+                             void* stream) {
+  if (!c || (!jobs && njobs)) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (njobs == 0) return ok();
+  if (wg == 0) wg = 256;
+  if (wg != 64 && wg != 128 && wg != 256) return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
+  if (lds_budget == 0) lds_budget = 24576;
+  const bool f64 = jobs[0].tape && jobs[0].tape->f64;
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const ehip_job& J = jobs[j];
+    if (!J.tape || !J.d_x) return fail(EHIP_OTHER, "GPU error: job %u: null argument", j);
+    if (J.tape->f64 != f64) return fail(EHIP_OTHER, "GPU error: job %u: all tapes in one queue share a dtype", j);
+    if (J.what != 0 && J.what != 1) return fail(EHIP_OTHER, "GPU error: job %u: what must be 0 (forward) or 1 (gradient)", j);
+    if (J.what == 1 && !J.d_g) return fail(EHIP_OTHER, "GPU error: job %u: gradient job without a gradient buffer", j);
+    if (J.what == 0 && !J.d_y) return fail(EHIP_OTHER, "GPU error: job %u: forward job without an output buffer", j);
+    int32_t rc = check_batch(J.tape, J.batch);
+    if (rc) return fail(rc, "job %u: %s", j, g_last_error.c_str());
+    if (J.batch > (1ull << 40)) return fail(EHIP_OTHER, "GPU error: job %u: batch too large", j);
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = pick(c, stream);
+  const size_t S = f64 ? 8 : 4;
+  const size_t per_reg = (size_t)wg * S;
+  const size_t win_lds = 2 * (size_t)WIN_WORDS * 4;
+  const size_t cap = c->max_lds > win_lds ? c->max_lds - win_lds : 0;
+  const size_t budget = std::min<size_t>(lds_budget, cap);
+  // jobs, longest program first
+  std::vector<uint32_t> order(njobs);
+  for (uint32_t j = 0; j < njobs; ++j) order[j] = j;
+  auto prog_of = [&](uint32_t j) -> const DevProgram& {
+    return jobs[j].tape->prog[jobs[j].what == 1 ? PM_GRADIENT : PM_FORWARD];
+  };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return prog_of(a).nwin > prog_of(b).nwin; });
+  std::vector<Launch> L(njobs);
+  std::vector<QueueTile> tiles;
+  uint32_t max_hbm_regs = 0, max_rlds = 0;
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const ehip_job& J = jobs[j];
+    const DevProgram& dp = prog_of(j);
+    Launch& l = L[j];
+    l = Launch{};
+    l.prog = dp.d_stream;
+    l.nwin = dp.nwin;
+    l.nin = J.tape->nin;
+    l.nout = J.tape->nout;
+    l.B = J.batch;
+    l.x = J.d_x;
+    l.xstride = J.tape->nin;
+    l.y = J.d_y;
+    l.g = J.what == 1 ? J.d_g : nullptr;
+    const uint32_t R = dp.host.nregs;
+    l.R_lds = (uint32_t)std::min<size_t>(R, budget / per_reg);
+    max_rlds = std::max(max_rlds, l.R_lds);
+    max_hbm_regs = std::max(max_hbm_regs, R - l.R_lds);
+  }
+  for (uint32_t j : order)
+    for (uint64_t f = 0; f < jobs[j].batch; f += wg) tiles.push_back(QueueTile{j, 0u, f});
+  if (tiles.size() > 0xFFFFFFF0ull) return fail(EHIP_OTHER, "GPU error: too many tiles in one queue");
+  const size_t lds = (size_t)max_rlds * per_reg;
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, queue_kernel(f64), (int)wg, queue_lds_bytes(lds)),
+          "queue occupancy");
+  uint64_t grid = (uint64_t)std::max(per_cu, 1) * c->ncu;
+  grid = std::min<uint64_t>(grid, tiles.size());
+  const uint64_t lanes = grid * wg;
+  void* gregs = nullptr;
+  if (max_hbm_regs) {
+    HIP_TRY(c->regs.reserve((size_t)max_hbm_regs * lanes * S), "register file");
+    gregs = c->regs.p;
+  }
+  for (auto& l : L) l.gregs = gregs;
+  HIP_TRY(c->qjobs.reserve(L.size() * sizeof(Launch)), "queue tables");
+  HIP_TRY(c->qtiles.reserve(tiles.size() * sizeof(QueueTile)), "queue tables");
+  HIP_TRY(c->qcount.reserve(256), "queue tables");
+  HIP_TRY(hipMemcpyAsync(c->qjobs.p, L.data(), L.size() * sizeof(Launch), hipMemcpyHostToDevice, s), "queue tables");
+  HIP_TRY(hipMemcpyAsync(c->qtiles.p, tiles.data(), tiles.size() * sizeof(QueueTile), hipMemcpyHostToDevice, s),
+          "queue tables");
+  Queue Q;
+  Q.jobs = (const Launch*)c->qjobs.p;
+  Q.tiles = (const QueueTile*)c->qtiles.p;
+  Q.ntiles = (uint32_t)tiles.size();
+  Q.counter = (uint32_t*)c->qcount.p;
+  hipError_t e = launch_queue(f64, Q, (uint32_t)grid, wg, lds, s);
+  if (e != hipSuccess) return hip_fail(e, "queue launch");
+  // the host tables above are released on return
+  HIP_TRY(hipStreamSynchronize(s), "queue");
+  return ok();
 }
 
 // --------------------------------- autotuner ---------------------------------

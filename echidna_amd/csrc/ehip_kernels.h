@@ -38,6 +38,25 @@ struct StdeReduce {
 
 hipError_t launch_stde_reduce(bool f64, const StdeReduce& R, hipStream_t s);
 
+// Persistent multi-tape queue: tile `it` evaluates points
+// [first, first + workgroup size) of job `job`.
+struct QueueTile {
+  uint32_t job;
+  uint32_t pad;
+  uint64_t first;
+};
+
+struct Queue {
+  const Launch* jobs;      // device array; each job's B masks its last tile
+  const QueueTile* tiles;  // device array, in the order workgroups take them
+  uint32_t ntiles;
+  uint32_t* counter;       // next tile; cleared by launch_queue
+};
+
+const void* queue_kernel(bool f64);
+size_t queue_lds_bytes(size_t reg_lds);
+hipError_t launch_queue(bool f64, const Queue& Q, uint32_t grid, uint32_t wg, size_t lds, hipStream_t s);
+
 hipError_t launch_scalar(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);
 hipError_t launch_dual(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);
 hipError_t launch_jet(bool f64, int K, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);

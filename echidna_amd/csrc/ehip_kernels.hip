@@ -188,10 +188,10 @@ __device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, AS3 c
 // ---------------------------------------------------------------------------
 // Scalar interpreter: forward values, outputs, reverse adjoint sweep.
 // ---------------------------------------------------------------------------
+// `p` is the point this lane evaluates; R.p the lane's slot in the HBM
+// register file (the same in a one-shot launch, different in the queue).
 template <class F, bool LDSONLY>
-__global__ __launch_bounds__(256) void k_scalar(Launch L) {
-  const Regs<F, 1, LDSONLY> R = make_regs<F, 1, LDSONLY>(L);
-  const uint64_t p = R.p;
+__device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LDSONLY>& R, uint64_t p) {
   const bool act = p < L.B;
   const F* X = reinterpret_cast<const F*>(L.x);
   F* Y = reinterpret_cast<F*>(L.y);
@@ -327,6 +327,36 @@ __global__ __launch_bounds__(256) void k_scalar(Launch L) {
       if (act && G) G[p * L.nin + I.u(1)] = v;
     }
   });
+}
+
+template <class F, bool LDSONLY>
+__global__ __launch_bounds__(256) void k_scalar(Launch L) {
+  const Regs<F, 1, LDSONLY> R = make_regs<F, 1, LDSONLY>(L);
+  scalar_body<F, LDSONLY>(L, R, R.p);
+}
+
+// Many tapes in one persistent launch.  Each workgroup takes (job, tile)
+// items from a global counter until the queue is empty; the host orders the
+// tiles longest program first.  A workgroup keeps its slot in the HBM
+// register file for every tile it runs.
+template <class F>
+__global__ __launch_bounds__(256) void k_queue_scalar(Queue Q) {
+  __shared__ uint32_t item;
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(Q.counter, 1u);
+    __syncthreads();
+    const uint32_t it = uni(item);
+    if (it >= Q.ntiles) break;
+    const QueueTile t = Q.tiles[it];
+    const Launch L = Q.jobs[t.job];
+    const Regs<F, 1, false> R = make_regs<F, 1, false>(L);
+    scalar_body<F, false>(L, R, t.first + threadIdx.x);
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -544,6 +574,20 @@ hipError_t launch_scalar(bool f64, const Launch& L, uint32_t wg, size_t lds, hip
   const bool all_lds = L.gregs == nullptr;
   return launch_common(EHIP_PICK(k_scalar), L, wg, lds, s);
 }
+
+const void* queue_kernel(bool f64) {
+  return f64 ? (const void*)k_queue_scalar<double> : (const void*)k_queue_scalar<float>;
+}
+
+hipError_t launch_queue(bool f64, const Queue& Q, uint32_t grid, uint32_t wg, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, Q.counter, 1u, 0u);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  void* args[] = {const_cast<Queue*>(&Q)};
+  return hipLaunchKernel(queue_kernel(f64), dim3(grid), dim3(wg), args, lds + 2 * WIN_BYTES, s);
+}
+
+size_t queue_lds_bytes(size_t reg_lds) { return reg_lds + 2 * WIN_BYTES; }
 
 hipError_t launch_dual(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s) {
   const bool all_lds = L.gregs == nullptr;

@@ -184,6 +184,28 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
  * per-workgroup LDS budget for the register file in bytes.  0 = default. */
 int32_t ehip_set_launch(ehip_ctx* ctx, uint32_t wg_size, uint32_t lds_budget_bytes);
 
+/* One job of a multi-tape queue: `batch` points of `tape`, inputs [batch][nin]
+ * on the device; what = 0 forward (d_y [batch][nout]) or 1 gradient (d_g
+ * [batch][nin], d_y optional). */
+typedef struct ehip_job {
+  const ehip_tape* tape;
+  int32_t what;
+  const void* d_x;
+  uint64_t batch;
+  void* d_y;
+  void* d_g;
+} ehip_job;
+
+/* Many tapes of uneven length and point count in one persistent launch (no
+ * counterpart in the reference, which launches one tape per call:
+ * src/gpu/cuda_backend.rs:861-913).  Workgroups take (tape, point-tile) items
+ * from a device work queue, longest program first; tile width = workgroup
+ * size (64/128/256, 0 = 256) and `lds_budget` bytes of LDS hold each tile's
+ * first registers (0 = 24576).  All tapes share one dtype.  Results equal
+ * the per-tape *_dev calls bit for bit.  Synchronizes `stream` before return. */
+int32_t ehip_multi_batch_dev(ehip_ctx* ctx, const ehip_job* jobs, uint32_t njobs, uint32_t wg, uint32_t lds_budget,
+                             void* stream);
+
 /* Autotuner (no counterpart in the reference, whose launch shape is fixed at
  * 256 threads per block in src/gpu/cuda_backend.rs).  Times every launch
  * shape -- workgroup size 64/128/256 x LDS budget for the register file --
