@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -70,6 +71,24 @@ _SIGS = {
     "ehip_laplacian_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                                   ctypes.c_uint64, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "ehip_set_fusion": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "ehip_cv_create": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.c_float, ctypes.POINTER(_vp)]),
+    "ehip_cv_destroy": (None, [_vp]),
+    "ehip_cv_info": (ctypes.c_int32, [_vp, _c_u64p]),
+    "ehip_cv_get": (ctypes.c_int32, [_vp, ctypes.c_int32, _vp]),
+    "ehip_cv_set_params": (ctypes.c_int32, [_vp, _vp]),
+    "ehip_cv_set_gain": (ctypes.c_int32, [_vp, ctypes.c_float]),
+    "ehip_cv_gain": (ctypes.c_float, [_vp]),
+    "ehip_cv_forward_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_uint64, _vp, _vp]),
+    "ehip_cv_loss_grad_dev": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
+    "ehip_cv_adam_step": (ctypes.c_int32, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                           _vp]),
+    "ehip_cv_train_dev": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_float, _vp, _vp]),
+    "ehip_cv_make_tuples_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    "ehip_laplacian_cv_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32,
+                                                     ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, _vp]),
+    "ehip_hessian_trace_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "ehip_multi_batch_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp]),
     "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
@@ -220,6 +239,7 @@ class HipContext:
 
     def __init__(self, handle):
         self.handle = handle
+        self._children = weakref.WeakSet()  # objects to release before the context
 
     @staticmethod
     def new(device: int = 0) -> Optional["HipContext"]:
@@ -234,6 +254,8 @@ class HipContext:
 
     def close(self) -> None:
         if self.handle:
+            for child in list(self._children):
+                child.close()
             load_library().ehip_ctx_destroy(self.handle)
             self.handle = None
 

@@ -1,0 +1,308 @@
+// Kernels of the learned control variate (ehip_cv.h).
+//
+// GEMM: 64x64 output tile per 256-thread workgroup, four waves in a 2x2
+// grid, each wave a 32x32 block of v_mfma_f32_32x32x16_bf16.  Operand
+// layout of that instruction (checked on the device): lane l holds
+// A[m = l%32][k = 8*(l/32) + i] and B[k = 8*(l/32) + i][n = l%32] for
+// i < 8, and C[m][n] for n = l%32, m = 8*(j/4) + 4*(l/32) + j%4, j < 16.
+// Tiles of A and B go global -> registers -> LDS as BF16 with k contiguous
+// (row pitch 40 BF16 = 80 bytes keeps every 8-element operand read 16-byte
+// aligned); the next K step's global loads are in flight while the current
+// step's MFMAs run.
+#include <hip/hip_runtime.h>
+
+#include "ehip_cv.h"
+#include "philox.h"
+
+namespace ehip {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 64, BN = 64, BK = 32, PITCH = BK + 8;
+
+// Eight consecutive elements p[0..7] (stride 1) if all lie below `lim`
+// elements from the run start, else the in-range ones and zeros.
+__device__ __forceinline__ void load8(const float* p, int64_t avail, float* o) {
+  if (avail >= 8 && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0];
+    const float4 b = reinterpret_cast<const float4*>(p)[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = (j < avail) ? p[j] : 0.0f;
+  }
+}
+
+// Stage one operand tile (64 rows of the non-k dimension x 32 k) into
+// registers.  KC: k is the contiguous dimension of this operand.
+template <bool KC>
+__device__ __forceinline__ void fetch_tile(const float* base, int64_t s_row, int64_t s_k, int rows, int K, int r0,
+                                           int k0, float* o) {
+  const int t = threadIdx.x;
+  if (KC) {
+    const int row = t >> 2, kq = (t & 3) * 8;
+    const int gr = r0 + row, gk = k0 + kq;
+    if (gr < rows && gk < K) load8(base + (int64_t)gr * s_row + gk, K - gk, o);
+    else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) o[j] = 0.0f;
+    }
+  } else {
+    const int kr = t >> 3, rq = (t & 7) * 8;
+    const int gk = k0 + kr, gr = r0 + rq;
+    if (gk < K && gr < rows) load8(base + (int64_t)gk * s_k + gr, rows - gr, o);
+    else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) o[j] = 0.0f;
+    }
+  }
+}
+
+template <bool KC>
+__device__ __forceinline__ void store_tile(__bf16* lds, const float* o) {
+  const int t = threadIdx.x;
+  if (KC) {
+    const int row = t >> 2, kq = (t & 3) * 8;
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (__bf16)o[j];
+    *reinterpret_cast<bf16x8*>(lds + row * PITCH + kq) = v;
+  } else {
+    const int kr = t >> 3, rq = (t & 7) * 8;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lds[(rq + j) * PITCH + kr] = (__bf16)o[j];
+  }
+}
+
+template <bool AK, bool BKC>
+__global__ __launch_bounds__(256) void k_gemm(Gemm g) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[BM * PITCH];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * PITCH];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // A(m,k): rows = M, row stride sam, k stride sak; B(k,n): rows = N.
+  float ra[8], rb[8];
+  fetch_tile<AK>(g.A, g.sam, g.sak, g.M, g.K, m0, 0, ra);
+  fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, g.K, n0, 0, rb);
+  f32x16 acc = {};
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    __syncthreads();
+    store_tile<AK>(As, ra);
+    store_tile<BKC>(Bs, rb);
+    __syncthreads();
+    if (k0 + BK < g.K) {
+      fetch_tile<AK>(g.A, g.sam, g.sak, g.M, g.K, m0, k0 + BK, ra);
+      fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, g.K, n0, k0 + BK, rb);
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      const int ko = kk + 8 * (lane >> 5);
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(As + (wm + (lane & 31)) * PITCH + ko);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + (wn + (lane & 31)) * PITCH + ko);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+  }
+  const int n = n0 + wn + (lane & 31);
+  if (n >= g.N) return;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int m = m0 + wm + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+    if (m >= g.M) continue;
+    float v = acc[j];
+    float* c = g.C + (int64_t)m * g.ldc + n;
+    switch (g.epi) {
+      case EPI_BIAS: v = g.alpha * (v + g.bias[n]); break;
+      case EPI_BIAS_TANH: v = tanhf(v + g.bias[n]); break;
+      case EPI_DTANH: {
+        const float h = g.aux[(int64_t)m * g.ldaux + n];
+        v = v * (1.0f - h * h);
+        break;
+      }
+      default: break;
+    }
+    *c = g.accumulate ? *c + v : v;
+  }
+}
+
+hipError_t launch_gemm(const Gemm& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  const bool ak = g.sak == 1, bk = g.sbk == 1;
+  if ((!ak && g.sam != 1) || (!bk && g.sbn != 1)) return hipErrorInvalidValue;
+  const dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
+  const void* fn = ak ? (bk ? (const void*)k_gemm<true, true> : (const void*)k_gemm<true, false>)
+                      : (bk ? (const void*)k_gemm<false, true> : (const void*)k_gemm<false, false>);
+  void* args[] = {const_cast<Gemm*>(&g)};
+  return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
+}
+
+__global__ __launch_bounds__(256) void k_colsum(const float* X, int64_t ld, int64_t M, int64_t N, float scale,
+                                                float* out, int accumulate) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float acc = 0.0f;
+  for (int64_t m = 0; m < M; ++m) acc += X[m * ld + n];
+  acc *= scale;
+  out[n] = accumulate ? out[n] + acc : acc;
+}
+
+hipError_t launch_colsum(const float* X, int64_t ld, int64_t M, int64_t N, float scale, float* out, int accumulate,
+                         hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_colsum, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, s, X, ld, M, N, scale, out,
+                     accumulate);
+  return hipGetLastError();
+}
+
+// Block-wide sum of `cnt` per-thread values (cnt <= 34), result in every thread.
+template <int CNT>
+__device__ __forceinline__ void block_sum(float* v, float* red /* [4][CNT] */) {
+#pragma unroll
+  for (int j = 0; j < CNT; ++j) {
+    float x = v[j];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[j] = x;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < CNT; ++j) red[wave * CNT + j] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < CNT; ++j) v[j] = red[j] + red[CNT + j] + red[2 * CNT + j] + red[3 * CNT + j];
+}
+
+constexpr int RMAX = 32;
+
+__global__ __launch_bounds__(256) void k_cv_loss(CvLoss a) {
+  __shared__ float red[4 * (RMAX + 2)];
+  const uint64_t t = blockIdx.x;
+  if (t >= a.T) return;
+  const uint32_t n = a.n, r = a.r;
+  const float* L = a.L + t * (uint64_t)n * r;
+  const float* v = a.v + t * (uint64_t)n;
+  const float* hv = a.hv + t * (uint64_t)n;
+  // pass 1: u = L^T v, |L|_F^2, v.hv
+  float acc[RMAX + 2];
+#pragma unroll
+  for (int j = 0; j < RMAX + 2; ++j) acc[j] = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    const float vi = v[i];
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j)
+      if (j < (int)r) {
+        const float l = L[(uint64_t)i * r + j];
+        acc[j] += l * vi;
+        acc[RMAX] += l * l;
+      }
+    acc[RMAX + 1] += vi * hv[i];
+  }
+  block_sum<RMAX + 2>(acc, red);
+  float u[RMAX];
+  float uu = 0.0f;
+#pragma unroll
+  for (int j = 0; j < RMAX; ++j) {
+    u[j] = (j < (int)r) ? acc[j] : 0.0f;
+    uu += u[j] * u[j];
+  }
+  const float lnorm = acc[RMAX], vhv = acc[RMAX + 1];
+  // pass 2: e = hv - L u, |e|^2, w = L^T e
+  float acc2[RMAX + 2];
+#pragma unroll
+  for (int j = 0; j < RMAX + 2; ++j) acc2[j] = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    float lu = 0.0f;
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j)
+      if (j < (int)r) lu += L[(uint64_t)i * r + j] * u[j];
+    const float e = hv[i] - lu;
+    acc2[RMAX] += e * e;
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j)
+      if (j < (int)r) acc2[j] += L[(uint64_t)i * r + j] * e;
+  }
+  __syncthreads();
+  block_sum<RMAX + 2>(acc2, red);
+  if (threadIdx.x == 0) {
+    if (a.loss) a.loss[t] = acc2[RMAX] / (float)n;
+    if (a.vhv) a.vhv[t] = vhv;
+    if (a.cv) a.cv[t] = uu - lnorm;
+  }
+  if (!a.dL) return;
+  // pass 3: dL = gscale (e u^T + v w^T)
+  float* dL = a.dL + t * (uint64_t)n * r;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    float lu = 0.0f;
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j)
+      if (j < (int)r) lu += L[(uint64_t)i * r + j] * u[j];
+    const float e = hv[i] - lu, vi = v[i];
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j)
+      if (j < (int)r) dL[(uint64_t)i * r + j] = a.gscale * (e * u[j] + vi * acc2[j]);
+  }
+}
+
+hipError_t launch_cv_loss(const CvLoss& a, hipStream_t s) {
+  if (a.T == 0) return hipSuccess;
+  if (a.r == 0 || a.r > (uint32_t)RMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cv_loss, dim3((uint32_t)a.T), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* p, const float* g, float* m, float* v, uint64_t count, float lr_c,
+                                              float b1, float b2, float bc2s, float eps) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] = p[i] - lr_c * mi / (sqrtf(vi) / bc2s + eps);
+}
+
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, uint64_t count, float lr_c, float b1, float b2,
+                       float bc2s, float eps, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adam, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, s, p, g, m, v, count, lr_c, b1,
+                     b2, bc2s, eps);
+  return hipGetLastError();
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_philox_fill(F* out, uint64_t count, uint32_t kind, uint64_t seed,
+                                                     uint32_t stream, uint64_t offset) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (F)philox_draw(kind, seed, stream, offset + i);
+}
+
+hipError_t launch_philox_fill(void* out, bool f64, uint64_t count, uint32_t kind, uint64_t seed, uint32_t stream,
+                              uint64_t offset, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const dim3 grid((uint32_t)((count + 255) / 256));
+  if (f64) hipLaunchKernelGGL(k_philox_fill<double>, grid, dim3(256), 0, s, (double*)out, count, kind, seed, stream, offset);
+  else hipLaunchKernelGGL(k_philox_fill<float>, grid, dim3(256), 0, s, (float*)out, count, kind, seed, stream, offset);
+  return hipGetLastError();
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_to_f32(const F* src, uint64_t count, float* dst) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = (float)src[i];
+}
+
+hipError_t launch_to_f32(const void* src, bool f64, uint64_t count, float* dst, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const dim3 grid((uint32_t)((count + 255) / 256));
+  if (f64) hipLaunchKernelGGL(k_to_f32<double>, grid, dim3(256), 0, s, (const double*)src, count, dst);
+  else hipLaunchKernelGGL(k_to_f32<float>, grid, dim3(256), 0, s, (const float*)src, count, dst);
+  return hipGetLastError();
+}
+
+}  // namespace ehip

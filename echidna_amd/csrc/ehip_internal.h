@@ -1,0 +1,178 @@
+// Host-side internals shared by the C ABI translation units: status and
+// error text, device buffers, the context and tape handles, program upload
+// and the interpreter launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/echidna_hip.h"
+#include "ehip_kernels.h"
+#include "ehip_program.h"
+
+using namespace ehip;
+
+namespace ehip_detail {
+
+inline thread_local std::string g_last_error;
+
+inline int32_t fail(int32_t code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+inline int32_t ok() {
+  g_last_error.clear();
+  return EHIP_OK;
+}
+
+inline int32_t hip_fail(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory) return fail(EHIP_OUT_OF_MEMORY, "GPU out of memory (%s)", what);
+  return fail(EHIP_OTHER, "GPU error: %s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                     \
+  do {                                          \
+    hipError_t _e = (expr);                     \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  hipError_t reserve(size_t bytes) {
+    if (bytes <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) n = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace ehip_detail
+
+using namespace ehip_detail;
+
+struct ehip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;       // guards device state and knobs
+  std::mutex host_mu;  // serializes host-pointer calls (staging buffers)
+  DevBuf regs;                 // HBM register file
+  DevBuf io[6];                // staging for host-pointer entry points
+  DevBuf jets;                 // STDE per-sample jets
+  DevBuf qjobs, qtiles, qcount;  // multi-tape queue tables
+  DevBuf tup[4];               // control-variate tuple scratch
+  DevBuf cvx, cvl;             // control-variate inputs and factors
+  uint32_t ncu = 256;
+  uint32_t wg = 64;
+  uint32_t lds_budget = 32768;
+  bool fuse = true;
+  size_t max_lds = 65536;
+};
+
+struct DevProgram {
+  Program host;
+  uint32_t* d_stream = nullptr;  // windowed stream (encode_stream)
+  uint32_t nwin = 0;
+};
+
+// Launch shape chosen by the autotuner for one evaluation kind of one tape.
+struct Tuned {
+  uint32_t wg = 0;          // 0: not tuned, use the context defaults
+  uint32_t lds_budget = 0;  // bytes of LDS for the register file
+};
+
+struct ehip_tape {
+  ehip_ctx* ctx = nullptr;
+  bool f64 = false;
+  uint32_t nin = 0, nout = 0;
+  TapeHost host;
+  DevProgram prog[2];
+  Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
+};
+
+namespace ehip_detail {
+
+inline int32_t upload_program(DevProgram& dp, bool f64) {
+  const std::vector<uint32_t> st = encode_stream(dp.host, f64);
+  dp.nwin = (uint32_t)(st.size() / WIN_WORDS);
+  HIP_TRY(hipMalloc((void**)&dp.d_stream, st.size() * 4), "program upload");
+  HIP_TRY(hipMemcpy(dp.d_stream, st.data(), st.size() * 4, hipMemcpyHostToDevice), "program upload");
+  return EHIP_OK;
+}
+
+inline void free_program(DevProgram& dp) {
+  if (dp.d_stream) (void)hipFree(dp.d_stream);
+  dp.d_stream = nullptr;
+}
+
+// validate_batch_args (src/gpu/mod.rs:572-582)
+inline int32_t check_batch(const ehip_tape* t, uint64_t batch) {
+  if (batch == 0) return fail(EHIP_OTHER, "GPU error: batch_size must be non-zero");
+  if (t->nin == 0)
+    return fail(EHIP_OTHER, "GPU error: tape has no inputs (constant function); evaluate it on the CPU instead");
+  return EHIP_OK;
+}
+
+// Slot of `tuned` for an evaluation: 0 forward, 1 gradient, 2 hvp, 2+K taylor.
+inline int tune_key(int kind, int K, bool grad) { return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : 2 + K); }
+
+// Launch one interpreter over `L.B` lanes.  kind 0 scalar, 1 dual, 2 jet(K).
+inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s, const Tuned* force = nullptr) {
+  const DevProgram& dp = t->prog[kind == 0 && L.g == nullptr ? PM_FORWARD : (kind == 2 ? PM_FORWARD : PM_GRADIENT)];
+  L.prog = dp.d_stream;
+  L.nwin = dp.nwin;
+  const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
+  const size_t S = t->f64 ? 8 : 4;
+  const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
+  const uint32_t wg = tu.wg ? tu.wg : c->wg;
+  const size_t per_reg = (size_t)NC * wg * S;
+  // two instruction windows share the workgroup's LDS with the registers
+  const size_t win_lds = 2 * (size_t)WIN_WORDS * 4;
+  size_t budget = std::min<size_t>(tu.wg ? tu.lds_budget : c->lds_budget, c->max_lds > win_lds ? c->max_lds - win_lds : 0);
+  uint32_t R = dp.host.nregs;
+  uint32_t R_lds = (uint32_t)std::min<size_t>(R, budget / per_reg);
+  L.R_lds = R_lds;
+  size_t lds = (size_t)R_lds * per_reg;
+  uint64_t grid = (L.B + wg - 1) / wg;
+  uint64_t lanes = grid * wg;
+  if (R > R_lds) {
+    size_t bytes = (size_t)(R - R_lds) * NC * lanes * S;
+    HIP_TRY(c->regs.reserve(bytes), "register file");
+    L.gregs = c->regs.p;
+  } else {
+    L.gregs = nullptr;
+  }
+  hipError_t e;
+  if (kind == 0) e = launch_scalar(t->f64, L, wg, lds, s);
+  else if (kind == 1) e = launch_dual(t->f64, L, wg, lds, s);
+  else e = launch_jet(t->f64, K, L, wg, lds, s);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return EHIP_OK;
+}
+
+inline hipStream_t pick(ehip_ctx* c, void* stream) { return stream ? (hipStream_t)stream : c->stream; }
+
+}  // namespace ehip_detail
+
+using namespace ehip_detail;

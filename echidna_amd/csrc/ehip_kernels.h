@@ -34,6 +34,10 @@ struct StdeReduce {
   const void* diag;       // diagonal control variate [.. ][nin] or null
   uint64_t diag_stride;   // 0: one diagonal shared by all points
   double* out;            // [npoints][5]: value, estimate, variance, std error, count
+  const float* lcv;       // learned control variate L(x) [npoints][nin*rank] or null
+  uint32_t rank;          // r <= 32
+  double* out_plain;      // plain-estimator statistics of the same samples, or null
+  double sample_scale;    // samples are sample_scale * 2 c2 (0 means 1)
 };
 
 hipError_t launch_stde_reduce(bool f64, const StdeReduce& R, hipStream_t s);

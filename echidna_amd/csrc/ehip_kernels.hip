@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
         if (act) {
           const uint64_t row = L.S ? p / L.S : p;
           v[0] = X[row * L.xstride + i];
-          v[1] = L.rng ? (F)philox_direction(L.rng, L.seed, p * L.nin + i) : V[p * L.nin + i];
+          v[1] = L.rng ? (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i) : V[p * L.nin + i];
         }
       });
     } else if (op == I_SEED) {
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
           const uint64_t row = L.S ? p / L.S : p;
           v[0] = X[row * L.xstride + i];
           if (K > 1)
-            v[K > 1 ? 1 : 0] = L.rng ? (F)philox_direction(L.rng, L.seed, p * L.nin + i) : V[p * L.nin + i];
+            v[K > 1 ? 1 : 0] = L.rng ? (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i) : V[p * L.nin + i];
         }
       });
     } else if (op == I_STORE_OUT) {

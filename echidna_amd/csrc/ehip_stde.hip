@@ -11,7 +11,10 @@
 //
 // Samples: plain STDE uses 2*c2 (= v^T H v); the diagonal control variate
 // uses the residual 2*c2 - sum_j d_j v_j^2 with the exact tr(D) added to the
-// mean afterwards (laplacian_with_control_gpu's form).
+// mean afterwards (laplacian_with_control_gpu's form).  The learned control
+// variate subtracts v^T L L^T v - tr(L L^T) = |L^T v|^2 - |L|_F^2, which has
+// mean zero for any L when E[v v^T] = I, so the estimate stays unbiased; the
+// plain statistics of the same samples can be written alongside.
 #include <hip/hip_runtime.h>
 
 #include "ehip_kernels.h"
@@ -35,33 +38,15 @@ __device__ __forceinline__ Wf chan_merge(Wf a, Wf b) {
   return r;
 }
 
-template <class F>
-__global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
-  const uint64_t pt = blockIdx.x;
-  if (pt >= R.npoints) return;
-  const uint32_t lane = threadIdx.x;
-  const F* J = reinterpret_cast<const F*>(R.jets);
-  const F* D = reinterpret_cast<const F*>(R.dirs);
-  const F* DG = reinterpret_cast<const F*>(R.diag);
-  Wf w{0.0, 0.0, 0.0};
-  for (uint32_t s = lane; s < R.S; s += 64) {
-    const uint64_t row = pt * R.S + s;
-    double sample = 2.0 * (double)J[row * 3 + 2];
-    if (DG) {
-      const F* d = DG + pt * R.diag_stride;
-      double cv = 0.0;
-      for (uint32_t j = 0; j < R.nin; ++j) {
-        double v = R.rng ? philox_direction(R.rng, R.seed, row * R.nin + j) : (double)D[row * R.nin + j];
-        cv += (double)d[j] * v * v;
-      }
-      sample -= cv;
-    }
-    if (!isfinite(sample)) continue;
-    w.n += 1.0;
-    double delta = sample - w.mean;
-    w.mean += delta / w.n;
-    w.m2 += delta * (sample - w.mean);
-  }
+__device__ __forceinline__ void welford_add(Wf& w, double x) {
+  if (!isfinite(x)) return;
+  w.n += 1.0;
+  double delta = x - w.mean;
+  w.mean += delta / w.n;
+  w.m2 += delta * (x - w.mean);
+}
+
+__device__ __forceinline__ Wf wave_merge(Wf w) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     Wf o;
@@ -70,30 +55,92 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
     o.m2 = __shfl_xor(w.m2, off, 64);
     w = chan_merge(w, o);
   }
+  return w;
+}
+
+// WelfordAccumulator::finalize (src/stde/types.rs:96-113) plus an offset.
+__device__ __forceinline__ void finalize(const Wf& w, double value, double offset, double* out) {
+  out[0] = value;
+  if (w.n == 0.0) {
+    out[1] = __builtin_nan("");
+    out[2] = 0.0;
+    out[3] = 0.0;
+  } else if (w.n > 1.0) {
+    double var = w.m2 / (w.n - 1.0);
+    if (!(var > 0.0)) var = 0.0;
+    out[1] = w.mean + offset;
+    out[2] = var;
+    out[3] = sqrt(var / w.n);
+  } else {
+    out[1] = w.mean + offset;
+    out[2] = 0.0;
+    out[3] = 0.0;
+  }
+  out[4] = w.n;
+}
+
+constexpr int CV_RMAX = 32;
+
+template <class F>
+__global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
+  const uint64_t pt = blockIdx.x;
+  if (pt >= R.npoints) return;
+  const uint32_t lane = threadIdx.x;
+  const F* J = reinterpret_cast<const F*>(R.jets);
+  const F* D = reinterpret_cast<const F*>(R.dirs);
+  const F* DG = reinterpret_cast<const F*>(R.diag);
+  const float* LC = R.lcv ? R.lcv + pt * (uint64_t)R.nin * R.rank : nullptr;
+  // |L|_F^2 of this point, summed over the wave
+  double lnorm = 0.0;
+  if (LC) {
+    for (uint64_t k = lane; k < (uint64_t)R.nin * R.rank; k += 64) lnorm += (double)LC[k] * (double)LC[k];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) lnorm += __shfl_xor(lnorm, off, 64);
+  }
+  Wf w{0.0, 0.0, 0.0}, wp{0.0, 0.0, 0.0};
+  for (uint32_t s = lane; s < R.S; s += 64) {
+    const uint64_t row = pt * R.S + s;
+    double sample = 2.0 * (double)J[row * 3 + 2];
+    if (R.sample_scale != 0.0) sample *= R.sample_scale;
+    welford_add(wp, sample);
+    if (LC) {
+      double u[CV_RMAX];
+#pragma unroll
+      for (int j = 0; j < CV_RMAX; ++j) u[j] = 0.0;
+      for (uint32_t i = 0; i < R.nin; ++i) {
+        const double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, i) : (double)D[row * R.nin + i];
+        const float* li = LC + (uint64_t)i * R.rank;
+#pragma unroll
+        for (int j = 0; j < CV_RMAX; ++j)
+          if (j < (int)R.rank) u[j] += (double)li[j] * v;
+      }
+      double q = 0.0;
+#pragma unroll
+      for (int j = 0; j < CV_RMAX; ++j) q += u[j] * u[j];
+      sample -= q - lnorm;
+    }
+    if (DG) {
+      const F* d = DG + pt * R.diag_stride;
+      double cv = 0.0;
+      for (uint32_t j = 0; j < R.nin; ++j) {
+        double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, j) : (double)D[row * R.nin + j];
+        cv += (double)d[j] * v * v;
+      }
+      sample -= cv;
+    }
+    welford_add(w, sample);
+  }
+  w = wave_merge(w);
+  if (R.out_plain) wp = wave_merge(wp);
   if (lane == 0) {
-    double* out = R.out + pt * 5;
     double tr = 0.0;
     if (DG) {
       const F* d = DG + pt * R.diag_stride;
       for (uint32_t j = 0; j < R.nin; ++j) tr += (double)d[j];
     }
-    out[0] = (double)J[(pt * R.S) * 3 + 0];
-    if (w.n == 0.0) {
-      out[1] = __builtin_nan("");
-      out[2] = 0.0;
-      out[3] = 0.0;
-    } else if (w.n > 1.0) {
-      double var = w.m2 / (w.n - 1.0);
-      if (!(var > 0.0)) var = 0.0;
-      out[1] = w.mean + tr;
-      out[2] = var;
-      out[3] = sqrt(var / w.n);
-    } else {
-      out[1] = w.mean + tr;
-      out[2] = 0.0;
-      out[3] = 0.0;
-    }
-    out[4] = w.n;
+    const double value = (double)J[(pt * R.S) * 3 + 0];
+    finalize(w, value, tr, R.out + pt * 5);
+    if (R.out_plain) finalize(wp, value, 0.0, R.out_plain + pt * 5);
   }
 }
 

@@ -34,16 +34,31 @@ __host__ __device__ inline double philox_u53(uint32_t wa, uint32_t wb) {
   return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-enum : uint32_t { RNG_NONE = 0, RNG_RADEMACHER = 1, RNG_GAUSSIAN = 2 };
-constexpr uint32_t STREAM_DIRS = 7;
+// Direction sources: read from memory, Rademacher, Gaussian, or the basis
+// vectors (sample s of a point is e_s; needs S == n).
+enum : uint32_t { RNG_NONE = 0, RNG_RADEMACHER = 1, RNG_GAUSSIAN = 2, RNG_BASIS = 3 };
+constexpr uint32_t STREAM_POINTS = 1, STREAM_DIRS = 7, STREAM_CV = 9;
 
-// Direction component `idx` of the (seed, STREAM_DIRS) stream.
-__host__ __device__ inline double philox_direction(uint32_t kind, uint64_t seed, uint64_t idx) {
-  Philox4 w = philox4x32_10((uint32_t)idx, (uint32_t)(idx >> 32), 0u, 0u, (uint32_t)seed, STREAM_DIRS);
+// Draw `idx` of the (seed, stream) sequence: a Rademacher sign or a Gaussian.
+__host__ __device__ inline double philox_draw(uint32_t kind, uint64_t seed, uint32_t stream, uint64_t idx) {
+  Philox4 w = philox4x32_10((uint32_t)idx, (uint32_t)(idx >> 32), 0u, 0u, (uint32_t)seed, stream);
   if (kind == RNG_RADEMACHER) return (w.v[0] & 1u) ? -1.0 : 1.0;
   double u1 = philox_u53(w.v[0], w.v[1]);
   double u2 = philox_u53(w.v[2], w.v[3]);
   return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
+}
+
+// Direction component `idx` of the (seed, STREAM_DIRS) stream.
+__host__ __device__ inline double philox_direction(uint32_t kind, uint64_t seed, uint64_t idx) {
+  return philox_draw(kind, seed, STREAM_DIRS, idx);
+}
+
+// Component i of the direction of lane p (p = point * S + sample) with n
+// inputs: drawn at index p * n + i, or the basis vector e_(p mod S).
+__host__ __device__ inline double lane_direction(uint32_t kind, uint64_t seed, uint32_t S, uint64_t p, uint32_t n,
+                                                 uint32_t i) {
+  if (kind == RNG_BASIS) return (p % S) == i ? 1.0 : 0.0;
+  return philox_direction(kind, seed, p * n + i);
 }
 
 }  // namespace ehip

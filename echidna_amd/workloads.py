@@ -87,6 +87,25 @@ def tanh_mlp2(n: int, H1: int, H2: int, seed: int) -> BytecodeTape:
     return tape
 
 
+def softplus_ridge(n: int, H: int, seed: int) -> BytecodeTape:
+    """f(x) = (1/H) sum_j softplus(w_j . x + b_j): a convex function whose
+    Hessian (1/H) sum_j s_j(x) w_j w_j^T has rank <= H -- a tape a rank-H
+    learned control variate can match exactly."""
+    W, b, _ = mlp_weights(n, H, seed)
+
+    def f(x):
+        acc = BReverse.constant(0.0)
+        for j in range(H):
+            s = x[0] * float(W[j, 0])
+            for i in range(1, n):
+                s = s + x[i] * float(W[j, i])
+            acc = acc + _softplus(s + float(b[j])) * (1.0 / H)
+        return acc
+
+    tape, _ = record(f, [0.0] * n)
+    return tape
+
+
 def points_uniform(seed: int, B: int, n: int, lo: float, hi: float) -> np.ndarray:
     idx = np.arange(B * n, dtype=np.uint64)
     return philox.uniform_range(seed, philox.STREAM_POINTS, idx, lo, hi).reshape(B, n)

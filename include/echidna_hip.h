@@ -228,6 +228,62 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * runs every tape entry as its own instruction; results are identical. */
 int32_t ehip_set_fusion(ehip_ctx* ctx, int32_t enable);
 
+/* ------------------------------------------------------------------------
+ * Learned low-rank control variate for STDE.  No counterpart in the
+ * reference; it generalises laplacian_with_control (src/stde/laplacian.rs:
+ * 90-129, src/gpu/stde_gpu.rs:135-200) from a fixed diagonal to
+ * v^T L(x) L(x)^T v - tr(L L^T) with L(x) in R^{n x r} from an MLP
+ * n -> hidden -> hidden -> n*r (tanh hidden layers), trained on GPU with
+ * BF16-MFMA forward/backward GEMMs (FP32 accumulate) and Adam.
+ * ------------------------------------------------------------------------ */
+typedef struct ehip_cv ehip_cv;
+
+/* New network with W ~ N(0, 1/fan_in) drawn from Philox (seed, stream 9) at
+ * each parameter's flat index, the output layer scaled by out_scale, zero
+ * biases.  rank 1..=32. */
+int32_t ehip_cv_create(ehip_ctx* ctx, uint32_t n, uint32_t rank, uint32_t hidden, uint64_t seed, float out_scale,
+                       ehip_cv** out);
+void ehip_cv_destroy(ehip_cv* cv);
+/* out5 = {n, rank, hidden, parameter count, Adam steps taken}. */
+int32_t ehip_cv_info(const ehip_cv* cv, uint64_t* out5);
+/* Copy the flat FP32 vector [W1 | b1 | W2 | b2 | W3 | b3] (which = 0), the
+ * last gradients (1) or the Adam moments (2, 3) to the host. */
+int32_t ehip_cv_get(ehip_cv* cv, int32_t which, float* host_out);
+int32_t ehip_cv_set_params(ehip_cv* cv, const float* host_in);
+/* Output gain g: L(x) = g * (W3 h2 + b3), so L L^T scales with g^2 (the
+ * training loop then works on a target of unit scale).  Default 1. */
+int32_t ehip_cv_set_gain(ehip_cv* cv, float gain);
+float ehip_cv_gain(const ehip_cv* cv);
+/* L(x) for `batch` FP32 points: d_L [batch][n*r], L[i][j] at i*r + j. */
+int32_t ehip_cv_forward_dev(ehip_cv* cv, const float* d_x, uint64_t batch, float* d_L, void* stream);
+/* Gradients of mean_t ||hv_t - L(x_t) L(x_t)^T v_t||^2 / n into the
+ * network's gradient buffer; the mean loss to *d_loss (device) if non-null. */
+int32_t ehip_cv_loss_grad_dev(ehip_cv* cv, const float* d_x, const float* d_v, const float* d_hv, uint64_t batch,
+                              float* d_loss, void* stream);
+/* One Adam step (PyTorch form) on the last gradients. */
+int32_t ehip_cv_adam_step(ehip_cv* cv, float lr, float beta1, float beta2, float eps, void* stream);
+/* `epochs` passes over `ntuples` tuples in minibatches of `batch` (Adam,
+ * betas 0.9/0.999, eps 1e-8); per-step mean losses to d_loss_hist
+ * [epochs * ceil(ntuples / batch)] when non-null. */
+int32_t ehip_cv_train_dev(ehip_cv* cv, const float* d_x, const float* d_v, const float* d_hv, uint64_t ntuples,
+                          uint32_t batch, uint32_t epochs, float lr, float* d_loss_hist, void* stream);
+/* Training tuples from the tape: x_t ~ N(0, I) (Philox seed, stream 1),
+ * v_t ~ N(0, I) (seed, stream 7) and H(x_t) v_t from the HVP kernel, all
+ * written as FP32 [ntuples][n]. */
+int32_t ehip_cv_make_tuples_dev(ehip_ctx* ctx, const ehip_tape* tape, uint64_t ntuples, uint64_t seed, float* d_x,
+                                float* d_v, float* d_hv, void* stream);
+/* Batched STDE with the learned control variate (cv may be null: plain):
+ * S device-drawn directions per point (1 Rademacher, 2 Gaussian), results
+ * [npoints][5] as ehip_laplacian_batch_dev; d_plain_results (optional)
+ * receives the plain estimator's statistics of the same samples. */
+int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, ehip_cv* cv, const void* d_x,
+                                    uint64_t npoints, uint32_t S, uint32_t dir_kind, uint64_t seed,
+                                    double* d_results, double* d_plain_results, void* stream);
+/* Exact tr(H) per point from n basis-direction Taylor jets (results[p][1];
+ * layout as ehip_laplacian_batch_dev, with zero variance up to rounding). */
+int32_t ehip_hessian_trace_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const void* d_x, uint64_t npoints,
+                                     double* d_results, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

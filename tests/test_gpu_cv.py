@@ -1,0 +1,179 @@
+"""Learned control variate on the GPU: forward, gradients and Adam against
+the numpy restatement (BF16-rounded GEMM operands, tests/cv_ref.py), the
+training tuples against Philox and the HVP kernel, exact traces against the
+oracle, unbiasedness of the controlled estimator, the variance it removes
+after training, and the plain-STDE fallback.
+
+Bars: forward and gradients within 2e-3 of the reference in relative
+Frobenius norm per tensor (FP32 accumulation order and the occasional BF16
+tie flipped by it; a wrong index or a dropped term gives O(1) errors);
+Adam within 1e-6 relative; exact traces (f64 tape) within 1e-12 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import cv_ref
+import pyoracle as po
+from echidna_amd import cv as lcv
+from echidna_amd import philox
+from echidna_amd import workloads as wl
+from echidna_amd.tape import GpuError, GpuTapeData
+
+pytestmark = pytest.mark.gpu
+
+N, H, R = 24, 40, 4  # sizes off the 64x64x32 GEMM tile on purpose
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def random_cv(hip, seed=3, out_scale=1.0):
+    net = lcv.LearnedControlVariate(hip, N, R, H, seed=seed, out_scale=out_scale)
+    return net
+
+
+def test_init_matches_philox_formula(hip):
+    net = random_cv(hip, seed=11, out_scale=0.5)
+    assert np.array_equal(net.params(), cv_ref.init_params(N, H, R, 11, 0.5))
+
+
+def test_forward_matches_reference(hip):
+    net = random_cv(hip)
+    X = wl.points_normal(3, 37, N).astype(np.float32)
+    L = net.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    _, _, ref = cv_ref.forward(net.params(), X, N, H, R, bf=True)
+    assert L.shape == (37, N * R)
+    assert rel(L, ref) <= 2e-3
+
+
+def test_gradients_match_reference(hip):
+    net = random_cv(hip)
+    T = 53
+    X = wl.points_normal(4, T, N).astype(np.float32)
+    V = philox.normal(5, philox.STREAM_DIRS, np.arange(T * N)).reshape(T, N).astype(np.float32)
+    HV = (V * 0.3 + philox.normal(6, 1, np.arange(T * N)).reshape(T, N) * 0.1).astype(np.float32)
+    dev = [torch.from_numpy(a).cuda() for a in (X, V, HV)]
+    loss = net.loss_grad(*dev)
+    ref_loss, ref_g = cv_ref.loss_grad(net.params(), X, V, HV, N, H, R, bf=True)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    g = net.grads()
+    for k, (a, b) in enumerate(zip(net.split(g), net.split(ref_g))):
+        assert rel(a, b) <= 2e-3, k
+
+
+def test_gradients_chunked_equal_single_pass(hip):
+    """Batches above the 2048-tuple chunk accumulate gradients over chunks."""
+    net = random_cv(hip)
+    T = 2048 + 300
+    X = wl.points_normal(7, T, N).astype(np.float32)
+    V = philox.normal(8, philox.STREAM_DIRS, np.arange(T * N)).reshape(T, N).astype(np.float32)
+    HV = (0.5 * V).astype(np.float32)
+    loss = net.loss_grad(*[torch.from_numpy(a).cuda() for a in (X, V, HV)])
+    ref_loss, ref_g = cv_ref.loss_grad(net.params(), X, V, HV, N, H, R, bf=True)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    for k, (a, b) in enumerate(zip(net.split(net.grads()), net.split(ref_g))):
+        assert rel(a, b) <= 2e-3, k
+
+
+def test_adam_step_matches_reference(hip):
+    net = random_cv(hip)
+    T = 40
+    X = wl.points_normal(9, T, N).astype(np.float32)
+    V = philox.normal(10, philox.STREAM_DIRS, np.arange(T * N)).reshape(T, N).astype(np.float32)
+    HV = (0.2 * V).astype(np.float32)
+    dev = [torch.from_numpy(a).cuda() for a in (X, V, HV)]
+    p = net.params().astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in (1, 2):
+        net.loss_grad(*dev)
+        g = net.grads().astype(np.float64)
+        net.adam_step(1e-2)
+        p, m, v = cv_ref.adam(p, g, m, v, t, 1e-2)
+        assert np.max(np.abs(net.params() - p)) <= 1e-6 * max(1.0, np.max(np.abs(p)))
+    assert net.info()["steps"] == 2
+
+
+def test_tuples_match_philox_and_hvp(hip):
+    tape = wl.tanh_mlp(10, 12, 50)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    x, v, hv = lcv.make_tuples(hip, bufs, 300, seed=21)
+    xr = wl.points_normal(21, 300, 10).astype(np.float32)
+    vr = philox.normal(21, philox.STREAM_DIRS, np.arange(3000)).reshape(300, 10).astype(np.float32)
+    assert np.allclose(x.cpu().numpy(), xr, rtol=1e-6, atol=1e-7)
+    assert np.allclose(v.cpu().numpy(), vr, rtol=1e-6, atol=1e-7)
+    _, hv_ref = hip.hvp_batch_points(bufs, x.cpu().numpy().reshape(-1), v.cpu().numpy().reshape(-1), 300)
+    assert np.array_equal(hv.cpu().numpy().reshape(-1), np.asarray(hv_ref).reshape(-1))
+
+
+def test_exact_trace_matches_oracle(hip):
+    tape = wl.tanh_mlp(9, 11, 51)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    X = wl.points_normal(22, 17, 9)
+    tr = lcv.hessian_trace_batch(hip, bufs, torch.from_numpy(X).cuda())
+    for p in range(17):
+        _, d = po.hessian_diagonal(tape, X[p])
+        ref = float(np.sum(d))
+        assert abs(tr[p] - ref) <= 1e-12 * max(1.0, abs(ref)), p
+
+
+def test_controlled_estimator_is_unbiased(hip):
+    """An untrained (badly scaled) network changes the variance, not the mean."""
+    tape = wl.tanh_mlp(16, 24, 52)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    net = lcv.LearnedControlVariate(hip, 16, 4, 32, seed=5, out_scale=2.0)
+    P = 64
+    X = torch.from_numpy(wl.points_normal(23, P, 16)).cuda()
+    exact = lcv.hessian_trace_batch(hip, bufs, X)
+    res, plain = lcv.laplacian_batch(hip, bufs, X, 512, lcv.DIR_GAUSSIAN, 9, net, with_plain=True)
+    z = (res[:, 1] - exact) / res[:, 3]
+    assert np.all(np.abs(z) < 5.0)
+    assert abs(np.mean(z)) < 4.0 / np.sqrt(P)
+    assert np.all(res[:, 4] == 512) and np.all(plain[:, 4] == 512)
+    assert np.array_equal(res[:, 0], plain[:, 0])
+
+
+def test_training_removes_variance_on_a_low_rank_hessian(hip):
+    """softplus ridge with 4 units: H(x) has rank 4 and is PSD, so a rank-8
+    L L^T can absorb it; the trained control variate keeps a small fraction
+    of the plain variance and the estimate stays on the exact trace."""
+    tape = wl.softplus_ridge(16, 4, 53)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    est = lcv.LearnedStde(hip, bufs, rank=8, hidden=64, seed=7)
+    est.fit(ntuples=16384, epochs=6, batch=256, lr=3e-3, val_points=128, val_S=32)
+    assert est.history[-1] < est.history[0]
+    assert est.use_cv and est.variance_ratio < 0.25, est.variance_ratio
+    X64 = torch.from_numpy(wl.points_normal(24, 32, 16)).cuda()
+    bufs64 = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    exact = lcv.hessian_trace_batch(hip, bufs64, X64)
+    res = est.estimate(X64.float(), 64, seed=12)
+    z = (res[:, 1] - exact) / np.maximum(res[:, 3], 1e-12)
+    assert abs(np.mean(z)) < 4.0 / np.sqrt(32) + 0.2
+
+
+def test_fallback_to_plain_when_the_network_hurts(hip):
+    tape = wl.tanh_mlp(12, 16, 54)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    est = lcv.LearnedStde(hip, bufs, rank=4, hidden=16, seed=8)
+    est.fit(ntuples=256, epochs=0, out_scale=30.0, val_points=32, val_S=16)
+    assert est.variance_ratio > 1.0 and not est.use_cv
+    X = torch.from_numpy(wl.points_normal(25, 8, 12).astype(np.float32)).cuda()
+    a = est.estimate(X, 16, seed=3)
+    b = lcv.laplacian_batch(hip, bufs, X, 16, lcv.DIR_GAUSSIAN, 3, None)
+    assert np.array_equal(a, b)
+
+
+def test_cv_errors(hip):
+    with pytest.raises(GpuError, match="rank must be 1..=32"):
+        lcv.LearnedControlVariate(hip, 4, 33, 8)
+    tape = wl.tanh_mlp(5, 6, 55)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    net = lcv.LearnedControlVariate(hip, 6, 2, 8)
+    X = torch.zeros(3, 5, device="cuda")
+    with pytest.raises(GpuError, match="built for n=6"):
+        lcv.laplacian_batch(hip, bufs, X, 4, lcv.DIR_GAUSSIAN, 0, net)
+    with pytest.raises(GpuError, match="device-drawn directions"):
+        lcv.laplacian_batch(hip, bufs, X, 4, 0, 0, None)
