@@ -44,6 +44,14 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (calibrates HBM byte counters)")
+    ap.add_argument("--workload", choices=["mlp", "stde", "cv", "dags"], default="mlp",
+                    help="mlp: fwd+grad (default line); stde: plain GPU STDE on the n=256 H=256 MLP; "
+                         "cv: learned control variate vs plain on the n=1024 2x512 MLP; dags: 64 uneven "
+                         "random DAGs in one persistent launch vs one launch per tape")
+    ap.add_argument("--points", type=int, default=0, help="override the workload's point count")
+    ap.add_argument("--tuples-log2", type=int, default=16, help="cv: training tuples = 2^k")
+    ap.add_argument("--epochs", type=int, default=4, help="cv: training epochs")
+    ap.add_argument("--dag-scale", type=float, default=1.0, help="dags: shrink lengths and point counts")
     return ap.parse_args()
 
 
@@ -62,12 +70,208 @@ def cpu_baseline(tape, x: np.ndarray, sample: int):
             "sample": f"{sample} of the {x.shape[0]} points, f64 oracle fwd+grad, {dt:.2f} s"}
 
 
+def s_req(res: np.ndarray, rel: float = 1e-3) -> np.ndarray:
+    """Directions each point needs for a standard error of rel * |estimate|."""
+    return res[:, 2] / np.maximum((rel * np.abs(res[:, 1])) ** 2, 1e-300)
+
+
+def timed(fn, reps: int):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best, out
+
+
+def run_stde(args) -> dict:
+    """Plain GPU STDE: tanh MLP n=256 H=256 (seed 2), x ~ N(0, I), S = 64
+    Rademacher and Gaussian directions per point, f32 jets, f64 reduction;
+    error against the exact trace (f64 basis jets) on a 1024-point subset."""
+    import torch
+
+    from echidna_amd import cv as lcv
+    from echidna_amd import workloads as wl
+    from echidna_amd.backend import HipContext
+    from echidna_amd.tape import GpuTapeData
+
+    n, H, S = 256, 256, 64
+    P = args.points or 65536
+    tape = wl.tanh_mlp(n, H, 2)
+    ctx = HipContext.new(0)
+    bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
+    X = torch.from_numpy(wl.points_normal(2, P, n).astype(np.float32)).cuda()
+    out = {"metric": "Laplacians/s at 1e-3 rel err", "unit": "Laplacians/s", "n_gpus": 1, "higher_is_better": True,
+           "vs_baseline": None, "dtype": "f32 jets, f64 reduction",
+           "data": "synthetic: Philox points, random-init weights baked into the tape",
+           "config": {"workload": f"plain GPU STDE, tanh-MLP tape n={n} H={H} seed 2 ({tape.counts()['N']} "
+                                  f"computed entries), B={P} points, S={S}", "points": P, "S": S}}
+    for name, kind in (("rademacher", lcv.DIR_RADEMACHER), ("gaussian", lcv.DIR_GAUSSIAN)):
+        dt, res = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, kind, 2), max(1, args.steps // 10))
+        t_sample = dt / (P * S)
+        sr = s_req(res)
+        out[name] = {"seconds": dt, "jets_per_s": P * S / dt, "t_sample_ns": t_sample * 1e9,
+                     "mean_S_req": float(np.mean(sr)), "median_S_req": float(np.median(sr)),
+                     "laplacians_per_s": float(1.0 / (np.mean(sr) * t_sample)),
+                     "mean_rel_se_at_S": float(np.mean(res[:, 3] / np.abs(res[:, 1])))}
+        if name == "rademacher":
+            rad = res
+    # exact traces on a subset: f64 tape, n basis jets per point
+    k = min(P, 1024)
+    b64 = ctx.upload_tape(GpuTapeData.from_tape_f64(tape))
+    X64 = torch.from_numpy(wl.points_normal(2, k, n)).cuda()
+    t0 = time.perf_counter()
+    exact = lcv.hessian_trace_batch(ctx, b64, X64)
+    t_exact = time.perf_counter() - t0
+    z = (rad[:k, 1] - exact) / np.maximum(rad[:k, 3], 1e-300)
+    out["exact_check"] = {"points": k, "seconds": t_exact, "mean_abs_z": float(np.mean(np.abs(z))),
+                          "frac_within_3se": float(np.mean(np.abs(z) < 3.0)),
+                          "mean_rel_err": float(np.mean(np.abs(rad[:k, 1] - exact) / np.abs(exact)))}
+    out["value"] = out["rademacher"]["laplacians_per_s"]
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        from echidna_amd import philox
+
+        x0 = wl.points_normal(2, 1, n)[0]
+        dirs = philox.rademacher(2, philox.STREAM_DIRS, np.arange(S * n)).reshape(S, n)
+        pyoracle.laplacian_with_stats(tape, x0, dirs[:2])
+        t0 = time.perf_counter()
+        pyoracle.laplacian_with_stats(tape, x0, dirs)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": S / dt, "unit": "jets/s", "cores": 1, "kind": "port",
+                               "sample": f"1 point x {S} Rademacher directions, f64 oracle Taylor jets, {dt:.2f} s"}
+    return out
+
+
+def run_cv(args) -> dict:
+    """Learned control variate vs plain STDE: tanh MLP n=1024, 2x512 hidden
+    (seed 4), S = 16 Gaussian directions, x ~ N(0, I); CV network
+    1024 -> 256 -> 256 -> 1024*16 trained on 2^k tuples from the tape."""
+    import torch
+
+    from echidna_amd import cv as lcv
+    from echidna_amd import workloads as wl
+    from echidna_amd.backend import HipContext
+    from echidna_amd.tape import GpuTapeData
+
+    n, H, S, r = 1024, 512, 16, 16
+    P = args.points or 16384
+    t0 = time.perf_counter()
+    tape = wl.tanh_mlp2(n, H, H, 4)
+    t_record = time.perf_counter() - t0
+    ctx = HipContext.new(0)
+    t0 = time.perf_counter()
+    bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
+    t_upload = time.perf_counter() - t0
+    X = torch.from_numpy(wl.points_normal(4, P, n).astype(np.float32)).cuda()
+    print(f"# recorded {tape.counts()['N']} entries in {t_record:.1f} s, uploaded in {t_upload:.1f} s",
+          file=sys.stderr, flush=True)
+    dt_plain, plain = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, lcv.DIR_GAUSSIAN, 5), 1)
+    print(f"# plain STDE {dt_plain:.2f} s", file=sys.stderr, flush=True)
+    T = 1 << args.tuples_log2
+    # signed factor (8 columns +1, 8 columns -1): the MLP's Hessian is indefinite
+    est = lcv.LearnedStde(ctx, bufs, rank=r, hidden=256, seed=4, npos=r // 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, v, hv = lcv.make_tuples(ctx, bufs, T, 5)
+    t_tuples = time.perf_counter() - t0
+    del x, v, hv
+    print(f"# {T} tuples in {t_tuples:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    est.fit(ntuples=T, epochs=args.epochs, batch=512, lr=1e-3, val_points=256, val_S=S)
+    t_fit = time.perf_counter() - t0 - t_tuples  # fit regenerates the tuples
+    print(f"# trained in {t_fit:.2f} s, validation variance ratio {est.variance_ratio:.4f}", file=sys.stderr,
+          flush=True)
+    dt_cv, both = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, lcv.DIR_GAUSSIAN, 5, est.cv, with_plain=True), 1)
+    res_cv, res_pl = both
+    t_s_plain = dt_plain / (P * S)
+    t_s_cv = dt_cv / (P * S)
+    sr_plain, sr_cv = s_req(res_pl), s_req(res_cv)
+    lps_plain = 1.0 / (np.mean(sr_plain) * t_s_plain)
+    lps_cv = 1.0 / (np.mean(sr_cv) * t_s_cv) if est.use_cv else lps_plain
+    k = min(P, 64)
+    b64 = ctx.upload_tape(GpuTapeData.from_tape_f64(tape))
+    t0 = time.perf_counter()
+    exact = lcv.hessian_trace_batch(ctx, b64, X[:k].double())
+    t_exact = time.perf_counter() - t0
+    zc = (res_cv[:k, 1] - exact) / np.maximum(res_cv[:k, 3], 1e-300)
+    zp = (res_pl[:k, 1] - exact) / np.maximum(res_pl[:k, 3], 1e-300)
+    return {"metric": "Laplacians/s at 1e-3 rel err, learned CV vs plain", "value": float(lps_cv),
+            "unit": "Laplacians/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None,
+            "dtype": "f32 jets, f64 reduction, bf16 MFMA CV", "data": "synthetic: Philox points and tuples",
+            "config": {"workload": f"tanh-MLP tape n={n} 2x{H} seed 4 ({tape.counts()['N']} computed entries), "
+                                   f"B={P} points, S={S} Gaussian, CV {n}->256->256->{n}x{r} signed {r // 2}+{r // 2}",
+                       "points": P, "S": S, "rank": r, "tuples": T, "epochs": args.epochs},
+            "plain": {"laplacians_per_s": float(lps_plain), "seconds": dt_plain, "t_sample_ns": t_s_plain * 1e9,
+                      "mean_S_req": float(np.mean(sr_plain)), "mean_var": float(np.mean(res_pl[:, 2]))},
+            "learned_cv": {"laplacians_per_s": float(lps_cv), "seconds": dt_cv, "t_sample_ns": t_s_cv * 1e9,
+                           "mean_S_req": float(np.mean(sr_cv)), "mean_var": float(np.mean(res_cv[:, 2])),
+                           "variance_ratio": float(np.mean(res_cv[:, 2]) / np.mean(res_pl[:, 2])),
+                           "validation_variance_ratio": est.variance_ratio, "used": est.use_cv,
+                           "speedup": float(lps_cv / lps_plain)},
+            "training": {"tuple_seconds": t_tuples, "train_seconds": t_fit, "loss_first": float(est.history[0]),
+                         "loss_last": float(est.history[-1])},
+            "exact_check": {"points": k, "seconds": t_exact, "mean_abs_z_cv": float(np.mean(np.abs(zc))),
+                            "mean_abs_z_plain": float(np.mean(np.abs(zp)))},
+            "setup": {"record_s": t_record, "upload_s": t_upload}}
+
+
+def run_dags(args) -> dict:
+    """64 random DAGs (seed 3): lengths log-U[200, 20000], n ~ U[4, 128],
+    B_i log-U[1e3, 1e5], x ~ U[-1, 1]^n, f32 fwd+grad: one persistent
+    launch over the (tape, point-tile) queue vs one launch per tape."""
+    import torch
+
+    from echidna_amd import workloads as wl
+    from echidna_amd.backend import HipContext
+    from echidna_amd.tape import GpuTapeData
+
+    specs = wl.random_dag_suite(3, 64, args.dag_scale)
+    ctx = HipContext.new(0)
+    jobs, ops = [], 0
+    t0 = time.perf_counter()
+    for k, (length, n, B) in enumerate(specs):
+        tape = wl.random_dag(3, k, length, n)
+        bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
+        x = torch.from_numpy(wl.points_uniform(300 + k, B, n, -1.0, 1.0).astype(np.float32)).cuda()
+        jobs.append((bufs, 1, x, B, torch.empty(B, device="cuda"), torch.empty(B, n, device="cuda")))
+        ops += B * tape.counts()["N"]
+    t_setup = time.perf_counter() - t0
+    points = sum(j[3] for j in jobs)
+    reps = max(1, args.steps // 4)
+    dt_q, _ = timed(lambda: ctx.multi_batch_dev(jobs), reps)
+
+    def per_tape():
+        for b, _, x, B, y, g in jobs:
+            ctx.gradient_batch_dev(b, x, B, y, g)
+
+    dt_s, _ = timed(per_tape, reps)
+    return {"metric": "fwd+grad point-evals/s", "value": points / dt_q, "unit": "points/s", "n_gpus": 1,
+            "higher_is_better": True, "vs_baseline": None, "dtype": "f32", "data": "synthetic random DAGs",
+            "config": {"workload": f"64 random DAGs seed 3 scale {args.dag_scale}, {points} points, "
+                                   f"{ops:.3e} entry-evals", "points": points},
+            "queue": {"seconds": dt_q, "entry_evals_per_s": ops / dt_q},
+            "per_tape_launches": {"seconds": dt_s, "points_per_s": points / dt_s},
+            "speedup": dt_s / dt_q, "setup_s": t_setup}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(points shard by contiguous ranges across GPUs with no exchange)"}))
         return 2
+    if args.workload != "mlp":
+        fn = {"stde": run_stde, "cv": run_cv, "dags": run_dags}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return 0
     import torch
 
     from echidna_amd import workloads as wl

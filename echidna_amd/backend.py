@@ -61,7 +61,8 @@ _SIGS = {
     "ehip_compile_program": (ctypes.c_int32, [ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _c_f64p, ctypes.c_uint32,
                                               ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32,
                                               ctypes.c_int32, ctypes.c_int32, _c_u32p, ctypes.c_uint64, _c_u32p,
-                                              ctypes.c_uint64, _c_u64p]),
+                                              ctypes.c_uint64, _c_u32p, ctypes.c_uint64, _c_u64p]),
+    "ehip_set_dense": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "ehip_set_launch": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
     "ehip_laplacian": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint32, _c_f64p]),
     "ehip_laplacian_with_control": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp, _c_f64p]),
@@ -79,6 +80,7 @@ _SIGS = {
     "ehip_cv_set_params": (ctypes.c_int32, [_vp, _vp]),
     "ehip_cv_set_gain": (ctypes.c_int32, [_vp, ctypes.c_float]),
     "ehip_cv_gain": (ctypes.c_float, [_vp]),
+    "ehip_cv_set_signs": (ctypes.c_int32, [_vp, ctypes.c_uint32]),
     "ehip_cv_forward_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "ehip_cv_loss_grad_dev": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "ehip_cv_adam_step": (ctypes.c_int32, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
@@ -174,32 +176,37 @@ class TaylorKthBatchResult:
     order: int
 
 
-def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Optional[bool] = None):
+def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Optional[bool] = None,
+                    dense: bool = True):
     """Compile without a device.
 
     Modes 0/1 (forward/gradient) return (words[n,8], side table, info);
-    modes 2/3 return (windowed stream words, None, info).
+    modes 2/3 return (windowed stream words, None, info).  info["pool"] holds
+    the DENSE pool words.
     """
     lib = load_library()
     f64 = (np.asarray(data.constants).dtype == np.float64) if f64 is None else f64
     consts = np.ascontiguousarray(data.constants, dtype=np.float64)
     outs = _u32(data.output_indices)
-    info = (ctypes.c_uint64 * 4)()
+    info = (ctypes.c_uint64 * 5)()
+    fz = 0 if not fuse else (1 if dense else 2)
     args = [EHIP_F64 if f64 else EHIP_F32, _u32(data.opcodes), _u32(data.arg0), _u32(data.arg1), consts]
     ptrs = [args[0]] + [a.ctypes.data_as(_c_u32p) for a in args[1:4]] + [consts.ctypes.data_as(_c_f64p)]
     rc = lib.ehip_compile_program(*ptrs, data.num_ops, data.num_inputs, data.output_index,
-                                  outs.ctypes.data_as(_c_u32p), len(outs), mode, int(fuse),
-                                  None, 0, None, 0, info)
-    nw, T = int(info[2]), int(info[3])
+                                  outs.ctypes.data_as(_c_u32p), len(outs), mode, fz,
+                                  None, 0, None, 0, None, 0, info)
+    nw, T, npool = int(info[2]), int(info[3]), int(info[4])
     nt = max(4, T * (4 if f64 else 3))
     words = np.zeros(max(nw, 8), dtype=np.uint32)
     terms = np.zeros(nt, dtype=np.uint32)
+    pool = np.zeros(max(npool, 1), dtype=np.uint32)
     rc = lib.ehip_compile_program(*ptrs, data.num_ops, data.num_inputs, data.output_index,
-                                  outs.ctypes.data_as(_c_u32p), len(outs), mode, int(fuse),
+                                  outs.ctypes.data_as(_c_u32p), len(outs), mode, fz,
                                   words.ctypes.data_as(_c_u32p), len(words), terms.ctypes.data_as(_c_u32p),
-                                  len(terms), info)
+                                  len(terms), pool.ctypes.data_as(_c_u32p), len(pool), info)
     _check(rc)
-    info2 = {"instructions": int(info[0]), "registers": int(info[1]), "words": nw, "nterms": T}
+    info2 = {"instructions": int(info[0]), "registers": int(info[1]), "words": nw, "nterms": T,
+             "pool": pool[:npool]}
     if mode >= 2:
         return words[:nw], None, info2
     return words[:nw].reshape(-1, 8), terms[:nt], info2
@@ -267,6 +274,9 @@ class HipContext:
 
     def set_launch(self, wg: int = 0, lds_budget: int = 0) -> None:
         _check(load_library().ehip_set_launch(self.handle, wg, lds_budget))
+
+    def set_dense(self, enable: bool) -> None:
+        _check(load_library().ehip_set_dense(self.handle, int(enable)))
 
     def set_fusion(self, enable: bool) -> None:
         _check(load_library().ehip_set_fusion(self.handle, int(enable)))

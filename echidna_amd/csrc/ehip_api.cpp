@@ -69,6 +69,13 @@ int32_t ehip_set_launch(ehip_ctx* c, uint32_t wg, uint32_t lds_budget) {
   return ok();
 }
 
+int32_t ehip_set_dense(ehip_ctx* c, int32_t enable) {
+  if (!c) return fail(EHIP_OTHER, "GPU error: null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->dense = enable != 0;
+  return ok();
+}
+
 int32_t ehip_set_fusion(ehip_ctx* c, int32_t enable) {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -113,15 +120,17 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
   t->host.num_inputs = num_inputs;
   t->host.outputs = outs;
   t->host.f64 = t->f64;
-  bool fuse;
+  bool fuse, dense;
   {
     std::lock_guard<std::mutex> lk(c->mu);
     fuse = c->fuse;
+    dense = c->dense;
   }
-  t->prog[PM_FORWARD].host = compile_program(t->host, PM_FORWARD, fuse);
-  t->prog[PM_GRADIENT].host = compile_program(t->host, PM_GRADIENT, fuse);
+  t->prog[0].host = compile_program(t->host, PM_FORWARD, fuse, dense);
+  t->prog[1].host = compile_program(t->host, PM_GRADIENT, fuse, dense);
+  t->prog[2].host = compile_program(t->host, PM_GRADIENT, fuse, false);
   (void)hipSetDevice(c->device);
-  for (int m = 0; m < 2; ++m) {
+  for (int m = 0; m < 3; ++m) {
     int32_t rc = upload_program(t->prog[m], t->f64);
     if (rc != EHIP_OK) {
       for (auto& p : t->prog) free_program(p);
@@ -144,7 +153,7 @@ uint32_t ehip_num_inputs(const ehip_tape* t) { return t ? t->nin : 0; }
 int32_t ehip_tape_dtype(const ehip_tape* t) { return t ? (t->f64 ? EHIP_F64 : EHIP_F32) : -1; }
 
 int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) {
-  if (!t || !out4 || (mode != 0 && mode != 1)) return fail(EHIP_OTHER, "GPU error: bad argument");
+  if (!t || !out4 || mode < 0 || mode > 2) return fail(EHIP_OTHER, "GPU error: bad argument");
   const Program& p = t->prog[mode].host;
   out4[0] = p.ninstr;
   out4[1] = p.nregs;
@@ -157,8 +166,10 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
                              const double* constants, uint32_t num_ops, uint32_t num_inputs,
                              uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,
                              int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
-                             uint64_t terms_cap, uint64_t* out4) {
-  if (!out4 || mode < 0 || mode > 3 || (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
+                             uint64_t terms_cap, uint32_t* pool, uint64_t pool_cap, uint64_t* out5) {
+  uint64_t* out4 = out5;
+  if (!out4 || mode < 0 || mode > 3 || fuse < 0 || fuse > 2 ||
+      (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)
     if (opcodes[i] == 43u) return fail(EHIP_CUSTOM_OPS_NOT_SUPPORTED, "tape contains custom ops which cannot run on GPU");
@@ -179,23 +190,21 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
   th.num_inputs = num_inputs;
   th.outputs = outs;
   th.f64 = dtype == EHIP_F64;
-  Program p = compile_program(th, (mode & 1) == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0);
+  Program p = compile_program(th, (mode & 1) == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0, fuse == 1);
   out4[0] = p.ninstr;
   out4[1] = p.nregs;
   out4[3] = p.nterms;
-  if (mode >= 2) {
-    // the windowed stream the kernels execute; `terms` is not used
-    const std::vector<uint32_t> st = encode_stream(p, th.f64);
-    out4[2] = st.size();
-    if (st.size() > words_cap || !words) return fail(EHIP_OTHER, "GPU error: program buffers too small");
-    std::memcpy(words, st.data(), st.size() * 4);
-    return ok();
-  }
-  out4[2] = p.words.size();
-  if (p.words.size() > words_cap || p.terms.size() > terms_cap || !words || !terms)
-    return fail(EHIP_OTHER, "GPU error: program buffers too small");
-  std::memcpy(words, p.words.data(), p.words.size() * 4);
-  std::memcpy(terms, p.terms.data(), p.terms.size() * 4);
+  out5[4] = p.pool.size();
+  std::vector<uint32_t> st;
+  if (mode >= 2) st = encode_stream(p, th.f64);  // the windowed stream the kernels execute
+  const std::vector<uint32_t>& wv = mode >= 2 ? st : p.words;
+  out4[2] = wv.size();
+  const bool small = wv.size() > words_cap || !words || (mode < 2 && (p.terms.size() > terms_cap || !terms)) ||
+                     (!p.pool.empty() && (p.pool.size() > pool_cap || !pool));
+  if (small) return fail(EHIP_OTHER, "GPU error: program buffers too small");
+  std::memcpy(words, wv.data(), wv.size() * 4);
+  if (mode < 2) std::memcpy(terms, p.terms.data(), p.terms.size() * 4);
+  if (!p.pool.empty()) std::memcpy(pool, p.pool.data(), p.pool.size() * 4);
   return ok();
 }
 
@@ -317,7 +326,7 @@ int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, 
   std::vector<uint32_t> order(njobs);
   for (uint32_t j = 0; j < njobs; ++j) order[j] = j;
   auto prog_of = [&](uint32_t j) -> const DevProgram& {
-    return jobs[j].tape->prog[jobs[j].what == 1 ? PM_GRADIENT : PM_FORWARD];
+    return jobs[j].tape->prog[prog_index(0, jobs[j].what == 1)];
   };
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return prog_of(a).nwin > prog_of(b).nwin; });
   std::vector<Launch> L(njobs);
@@ -329,6 +338,7 @@ int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, 
     Launch& l = L[j];
     l = Launch{};
     l.prog = dp.d_stream;
+    l.pool = dp.d_pool;
     l.nwin = dp.nwin;
     l.nin = J.tape->nin;
     l.nout = J.tape->nout;
@@ -426,7 +436,7 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
   (void)hipSetDevice(c->device);
   const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
   const size_t S = t->f64 ? 8 : 4;
-  const uint32_t R = t->prog[(kind == 0 && what == 1) || kind == 1 ? PM_GRADIENT : PM_FORWARD].host.nregs;
+  const uint32_t R = t->prog[prog_index(kind, what == 1)].host.nregs;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0), "autotune");
   HIP_TRY(hipEventCreate(&e1), "autotune");

@@ -39,9 +39,11 @@ hipError_t launch_gemm(const Gemm& g, hipStream_t s);
 hipError_t launch_colsum(const float* X, int64_t ld, int64_t M, int64_t N, float scale, float* out, int accumulate,
                          hipStream_t s);
 
-// Per tuple t: u = L^T v, e = hv - L u, w = L^T e;
-//   loss[t] = |e|^2 / n, vhv[t] = v.hv, cv[t] = |u|^2 - |L|_F^2,
-//   dL = gscale * (e u^T + v w^T)   (gscale = -2 / (n * T) for the batch mean).
+// Per tuple t, with S = diag(+1 x npos, -1 x (r - npos)):
+//   u = L^T v, e = hv - L S u, w = L^T e;
+//   loss[t] = |e|^2 / n, vhv[t] = v.hv, cv[t] = u^T S u - tr(L S L^T),
+//   dL = gscale * (e (S u)^T + v (S w)^T)   (gscale = -2 / (n * T) for the
+//   batch mean of |hv - L S L^T v|^2 / n).
 struct CvLoss {
   const float* L;  // [T][n*r]
   const float* v;  // [T][n]
@@ -50,7 +52,7 @@ struct CvLoss {
   float* loss;     // [T] or null
   float* vhv;      // [T] or null
   float* cv;       // [T] or null
-  uint32_t n, r;
+  uint32_t n, r, npos;
   uint64_t T;
   float gscale;
 };

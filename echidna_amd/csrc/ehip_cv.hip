@@ -198,20 +198,22 @@ __global__ __launch_bounds__(256) void k_cv_loss(CvLoss a) {
       if (j < (int)r) {
         const float l = L[(uint64_t)i * r + j];
         acc[j] += l * vi;
-        acc[RMAX] += l * l;
+        acc[RMAX] += (j < (int)a.npos) ? l * l : -(l * l);
       }
     acc[RMAX + 1] += vi * hv[i];
   }
   block_sum<RMAX + 2>(acc, red);
-  float u[RMAX];
+  // su = S u with S = diag(+1 for the first npos columns, -1 after)
+  float su[RMAX];
   float uu = 0.0f;
 #pragma unroll
   for (int j = 0; j < RMAX; ++j) {
-    u[j] = (j < (int)r) ? acc[j] : 0.0f;
-    uu += u[j] * u[j];
+    const float u = (j < (int)r) ? acc[j] : 0.0f;
+    su[j] = (j < (int)a.npos) ? u : -u;
+    uu += su[j] * u;
   }
   const float lnorm = acc[RMAX], vhv = acc[RMAX + 1];
-  // pass 2: e = hv - L u, |e|^2, w = L^T e
+  // pass 2: e = hv - L S u, |e|^2, w = L^T e
   float acc2[RMAX + 2];
 #pragma unroll
   for (int j = 0; j < RMAX + 2; ++j) acc2[j] = 0.0f;
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void k_cv_loss(CvLoss a) {
     float lu = 0.0f;
 #pragma unroll
     for (int j = 0; j < RMAX; ++j)
-      if (j < (int)r) lu += L[(uint64_t)i * r + j] * u[j];
+      if (j < (int)r) lu += L[(uint64_t)i * r + j] * su[j];
     const float e = hv[i] - lu;
     acc2[RMAX] += e * e;
 #pragma unroll
@@ -234,17 +236,20 @@ __global__ __launch_bounds__(256) void k_cv_loss(CvLoss a) {
     if (a.cv) a.cv[t] = uu - lnorm;
   }
   if (!a.dL) return;
-  // pass 3: dL = gscale (e u^T + v w^T)
+  // pass 3: dL = gscale (e (S u)^T + v (S w)^T)
   float* dL = a.dL + t * (uint64_t)n * r;
   for (uint32_t i = threadIdx.x; i < n; i += 256) {
     float lu = 0.0f;
 #pragma unroll
     for (int j = 0; j < RMAX; ++j)
-      if (j < (int)r) lu += L[(uint64_t)i * r + j] * u[j];
+      if (j < (int)r) lu += L[(uint64_t)i * r + j] * su[j];
     const float e = hv[i] - lu, vi = v[i];
 #pragma unroll
     for (int j = 0; j < RMAX; ++j)
-      if (j < (int)r) dL[(uint64_t)i * r + j] = a.gscale * (e * u[j] + vi * acc2[j]);
+      if (j < (int)r) {
+        const float sw = (j < (int)a.npos) ? acc2[j] : -acc2[j];
+        dL[(uint64_t)i * r + j] = a.gscale * (e * su[j] + vi * sw);
+      }
   }
 }
 

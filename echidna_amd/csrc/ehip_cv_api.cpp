@@ -27,6 +27,7 @@ struct ehip_cv {
   uint32_t cap = 0;
   uint64_t step = 0;
   float gain = 1.0f;  // L = gain * (W3 h2 + b3)
+  uint32_t npos = 0;  // columns with sign +1 (the rest -1); r = L L^T
 };
 
 namespace {
@@ -113,6 +114,7 @@ int32_t cv_backward_chunk(ehip_cv* cv, const float* X, const float* V, const flo
   l.loss = a.loss;
   l.n = cv->n;
   l.r = cv->r;
+  l.npos = cv->npos;
   l.T = T;
   // gradient with respect to the pre-gain output W3 h2 + b3
   l.gscale = -2.0f * cv->gain / ((float)n * (float)Ttotal);
@@ -184,6 +186,7 @@ int32_t ehip_cv_create(ehip_ctx* c, uint32_t n, uint32_t rank, uint32_t hidden, 
   cv->n = n;
   cv->r = rank;
   cv->H = hidden;
+  cv->npos = rank;
   const uint64_t H = hidden, nr = (uint64_t)n * rank;
   const uint64_t sizes[6] = {H * n, H, H * H, H, nr * H, nr};
   cv->off[0] = 0;
@@ -260,6 +263,14 @@ int32_t ehip_cv_set_gain(ehip_cv* cv, float gain) {
 }
 
 float ehip_cv_gain(const ehip_cv* cv) { return cv ? cv->gain : 0.0f; }
+
+int32_t ehip_cv_set_signs(ehip_cv* cv, uint32_t npos) {
+  if (!cv) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (npos > cv->r) return fail(EHIP_OTHER, "GPU error: npos must be <= rank (%u), got %u", cv->r, npos);
+  std::lock_guard<std::mutex> lk(cv->ctx->mu);
+  cv->npos = npos;
+  return ok();
+}
 
 int32_t ehip_cv_set_params(ehip_cv* cv, const float* host_in) {
   if (!cv || !host_in) return fail(EHIP_OTHER, "GPU error: null argument");
@@ -439,6 +450,7 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
     }
     R.lcv = Lp;
     R.rank = cv->r;
+    R.npos = cv->npos;
   }
   HIP_TRY(launch_stde_reduce(t->f64, R, s), "stde reduce");
   return ok();

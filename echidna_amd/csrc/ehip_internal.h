@@ -87,12 +87,14 @@ struct ehip_ctx {
   uint32_t wg = 64;
   uint32_t lds_budget = 32768;
   bool fuse = true;
+  bool dense = true;
   size_t max_lds = 65536;
 };
 
 struct DevProgram {
   Program host;
   uint32_t* d_stream = nullptr;  // windowed stream (encode_stream)
+  uint32_t* d_pool = nullptr;    // DENSE operand lists and weights
   uint32_t nwin = 0;
 };
 
@@ -107,7 +109,7 @@ struct ehip_tape {
   bool f64 = false;
   uint32_t nin = 0, nout = 0;
   TapeHost host;
-  DevProgram prog[2];
+  DevProgram prog[3];  // forward, gradient (both with DENSE), HVP (dot-level)
   Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
 };
 
@@ -118,12 +120,18 @@ inline int32_t upload_program(DevProgram& dp, bool f64) {
   dp.nwin = (uint32_t)(st.size() / WIN_WORDS);
   HIP_TRY(hipMalloc((void**)&dp.d_stream, st.size() * 4), "program upload");
   HIP_TRY(hipMemcpy(dp.d_stream, st.data(), st.size() * 4, hipMemcpyHostToDevice), "program upload");
+  const std::vector<uint32_t>& pool = dp.host.pool;
+  HIP_TRY(hipMalloc((void**)&dp.d_pool, std::max<size_t>(pool.size(), 4) * 4), "program upload");
+  if (!pool.empty())
+    HIP_TRY(hipMemcpy(dp.d_pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice), "program upload");
   return EHIP_OK;
 }
 
 inline void free_program(DevProgram& dp) {
   if (dp.d_stream) (void)hipFree(dp.d_stream);
+  if (dp.d_pool) (void)hipFree(dp.d_pool);
   dp.d_stream = nullptr;
+  dp.d_pool = nullptr;
 }
 
 // validate_batch_args (src/gpu/mod.rs:572-582)
@@ -135,13 +143,17 @@ inline int32_t check_batch(const ehip_tape* t, uint64_t batch) {
 }
 
 // Slot of `tuned` for an evaluation: 0 forward, 1 gradient, 2 hvp, 2+K taylor.
+// Program of an evaluation: scalar forward / gradient, dual (HVP), jet.
+inline int prog_index(int kind, bool grad) { return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : 0); }
+
 inline int tune_key(int kind, int K, bool grad) { return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : 2 + K); }
 
 // Launch one interpreter over `L.B` lanes.  kind 0 scalar, 1 dual, 2 jet(K).
 inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s, const Tuned* force = nullptr) {
-  const DevProgram& dp = t->prog[kind == 0 && L.g == nullptr ? PM_FORWARD : (kind == 2 ? PM_FORWARD : PM_GRADIENT)];
+  const DevProgram& dp = t->prog[prog_index(kind, L.g != nullptr)];
   L.prog = dp.d_stream;
   L.nwin = dp.nwin;
+  L.pool = dp.d_pool;
   const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
   const size_t S = t->f64 ? 8 : 4;
   const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];

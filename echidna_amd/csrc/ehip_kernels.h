@@ -7,6 +7,7 @@ namespace ehip {
 
 struct Launch {
   const uint32_t* prog;   // windowed instruction stream (device, encode_stream)
+  const uint32_t* pool;   // DENSE operand lists and weights (device)
   uint32_t nwin;          // windows of WIN_WORDS words in the stream
   uint32_t R_lds;         // registers held in LDS
   uint32_t nin, nout;
@@ -36,6 +37,7 @@ struct StdeReduce {
   double* out;            // [npoints][5]: value, estimate, variance, std error, count
   const float* lcv;       // learned control variate L(x) [npoints][nin*rank] or null
   uint32_t rank;          // r <= 32
+  uint32_t npos;          // columns j < npos enter with +1, the rest with -1
   double* out_plain;      // plain-estimator statistics of the same samples, or null
   double sample_scale;    // samples are sample_scale * 2 c2 (0 means 1)
 };

@@ -186,6 +186,107 @@ __device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, AS3 c
 }
 
 // ---------------------------------------------------------------------------
+// DENSE / DENSE_REV (ehip_program.h): m dot products over one operand list.
+// Each member's accumulator sees its terms in program order, with the same
+// operations as its DOT (term 0 initialises, then acc + term), so results
+// are bit-identical; an operand register is read once per block of JB
+// members.  Operand lists and weights come from the pool through
+// wave-uniform (scalar) loads.
+// ---------------------------------------------------------------------------
+// JET: products follow the Taylor product with a constant jet (jt::mul,
+// the operation the jet interpreter's DOT performs); otherwise v * c.
+template <class F, int NC, int JB, bool JET, class Regs_>
+__device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, uint32_t m, uint32_t T,
+                                          uint32_t off_regs, uint32_t off_w) {
+  const uint32_t* PR = pool + off_regs;
+  const F* PW = reinterpret_cast<const F*>(pool + off_w);
+  for (uint32_t j0 = 0; j0 < m; j0 += JB) {
+    const uint32_t jb = (m - j0 < (uint32_t)JB) ? m - j0 : (uint32_t)JB;
+    F acc[JB][NC];
+    for (uint32_t p = 0; p < T; ++p) {
+      const uint32_t kr = PR[p];
+      const uint32_t kind = kr >> P_KIND_SHIFT;
+      F x[NC];
+      if (kind != T_IMM) R.rd(kr & P_REG_MASK, x);
+      const bool cleft = (kr & P_CLEFT) != 0;
+      const F* w = PW + (uint64_t)p * m + j0;
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        if ((uint32_t)j >= jb) break;
+        const F c = w[j];
+        F term[NC];
+        if (kind == T_PROD) {
+          if (JET) {
+            F cj[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) cj[q] = (q == 0) ? c : F(0);
+            if (cleft) jt::mul<F, NC>(cj, x, term);
+            else jt::mul<F, NC>(x, cj, term);
+          } else {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) term[q] = x[q] * c;
+          }
+        } else if (kind == T_REG) {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) term[q] = x[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) term[q] = (q == 0) ? c : F(0);
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      if ((uint32_t)j >= jb) break;
+      R.wr(PR[T + j0 + j], acc[j]);
+    }
+  }
+}
+
+template <class F, bool LDSONLY>
+__device__ __forceinline__ void dense_rev(const Regs<F, 1, LDSONLY>& R, const uint32_t* pool, uint32_t m, uint32_t T,
+                                          uint32_t off_regs, uint32_t off_w) {
+  constexpr int IB = 16;
+  const uint32_t* PR = pool + off_regs;
+  const F* PW = reinterpret_cast<const F*>(pool + off_w);
+  for (uint32_t i0 = 0; i0 < T; i0 += IB) {
+    const uint32_t ib = (T - i0 < (uint32_t)IB) ? T - i0 : (uint32_t)IB;
+    F cur[IB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      cur[i] = F(0);
+      if ((uint32_t)i >= ib) break;
+      const uint32_t kr = PR[i0 + i];
+      if (!(kr & P_FIRST)) R.rd(kr & P_REG_MASK, &cur[i]);
+    }
+    for (uint32_t j = 0; j < m; ++j) {
+      F g;
+      R.rd(PR[T + j], &g);
+      const bool live = g != F(0);
+      const F* w = PW + (uint64_t)j * T + i0;
+#pragma unroll
+      for (int i = 0; i < IB; ++i) {
+        if ((uint32_t)i >= ib) break;
+        const uint32_t kind = PR[i0 + i] >> P_KIND_SHIFT;
+        const F c = w[i];
+        if (kind == T_PROD) {
+          if (live && c != F(0)) cur[i] = cur[i] + c * g;
+        } else if (live) {
+          cur[i] = cur[i] + F(1) * g;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      if ((uint32_t)i >= ib) break;
+      R.wr(PR[i0 + i] & P_REG_MASK, &cur[i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Scalar interpreter: forward values, outputs, reverse adjoint sweep.
 // ---------------------------------------------------------------------------
 // `p` is the point this lane evaluates; R.p the lane's slot in the HBM
@@ -310,6 +411,10 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
           ++k;
         }
       }
+    } else if (op == I_DENSE) {
+      dense_fwd<F, 1, 16, false>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4));
+    } else if (op == I_DENSE_REV) {
+      dense_rev<F, LDSONLY>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4));
     } else if (op == I_LOADN) {
       load_inputs<F, 1, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
         v[0] = act ? X[p * L.xstride + i] : F(0);
@@ -531,6 +636,8 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
         k += blk;
       }
       R.wr(dst, acc.c);
+    } else if (op == I_DENSE) {
+      dense_fwd<F, K, (K <= 2 ? 16 : (K == 3 ? 8 : 4)), true>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4));
     } else if (op == I_LOADN) {
       load_inputs<F, K, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
 #pragma unroll

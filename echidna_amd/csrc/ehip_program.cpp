@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <queue>
 
 namespace ehip {
@@ -61,7 +62,145 @@ struct AIns {
   int64_t dst = -1, g = -1, r = -1, adjA = -1, adjB = -1;
   Opnd a, b;
   std::vector<ATerm> terms;
+  bool full = false;             // DOT / DOT_REV covering its whole chain
+  std::vector<int64_t> members;  // DENSE: destinations; DENSE_REV: adjoints g_j
+  std::vector<double> W;         // DENSE / DENSE_REV weights [member][term]
 };
+
+constexpr size_t DENSE_MIN = 4;  // members worth a DENSE instruction
+
+// Replace groups of full DOTs (and their DOT_REVs) that share one term
+// structure by DENSE (DENSE_REV) instructions.  A forward group moves to its
+// first member's position: every member reads the same operands, defined
+// before that point.  A reverse group moves to its last member's position
+// when no instruction in between touches the operand adjoints, so each
+// adjoint receives the same contributions in the same order.
+void group_dense(std::vector<AIns>& prog, int64_t adj_offset) {
+  struct Key {
+    std::vector<int64_t> k;
+    bool operator<(const Key& o) const { return k < o.k; }
+  };
+  auto key_of = [](const AIns& x, bool& ok) {
+    Key key;
+    ok = true;
+    std::vector<int64_t> vids;
+    for (const ATerm& tm : x.terms) {
+      key.k.push_back((int64_t)tm.kind | (tm.const_left ? 4 : 0));
+      key.k.push_back(tm.kind == T_IMM ? -1 : tm.vid);
+      if (tm.kind != T_IMM) vids.push_back(tm.vid);
+    }
+    std::sort(vids.begin(), vids.end());
+    if (std::adjacent_find(vids.begin(), vids.end()) != vids.end()) ok = false;
+    if (vids.empty()) ok = false;
+    return key;
+  };
+  std::map<Key, std::vector<size_t>> groups;
+  for (size_t p = 0; p < prog.size(); ++p) {
+    const AIns& x = prog[p];
+    if (x.op != I_DOT || !x.full) continue;
+    bool ok;
+    Key k = key_of(x, ok);
+    if (ok) groups[k].push_back(p);
+  }
+  std::vector<char> drop(prog.size(), 0);
+  std::vector<std::pair<size_t, AIns>> place;  // position -> new instruction
+  // DOT_REV lookup by root adjoint
+  std::map<int64_t, size_t> rev_at;
+  std::map<int64_t, int> rev_count;
+  for (size_t p = 0; p < prog.size(); ++p)
+    if (prog[p].op == I_DOT_REV) {
+      rev_count[prog[p].g]++;
+      if (prog[p].full) rev_at[prog[p].g] = p;
+    }
+  for (auto& kv : groups) {
+    const std::vector<size_t>& mem = kv.second;
+    if (mem.size() < DENSE_MIN) continue;
+    const AIns& x0 = prog[mem[0]];
+    const size_t T = x0.terms.size();
+    AIns d;
+    d.op = I_DENSE;
+    d.terms = x0.terms;
+    for (size_t j = 0; j < mem.size(); ++j) {
+      const AIns& x = prog[mem[j]];
+      d.members.push_back(x.dst);
+      for (size_t q = 0; q < T; ++q) d.W.push_back(x.terms[q].imm);
+      if (j > 0) drop[mem[j]] = 1;
+    }
+    drop[mem[0]] = 1;
+    place.push_back({mem[0], std::move(d)});
+    // reverse: every member has exactly one DOT_REV, the whole chain
+    std::vector<size_t> rp;
+    bool rev_ok = true;
+    for (size_t j : mem) {
+      // the DOT_REV of member j reads the adjoint of its root
+      const int64_t g = prog[j].dst + adj_offset;
+      auto it = rev_at.find(g);
+      if (it == rev_at.end() || rev_count[g] != 1) { rev_ok = false; break; }
+      rp.push_back(it->second);
+    }
+    if (!rev_ok || rp.empty()) continue;
+    std::vector<size_t> order(rp);
+    std::sort(order.begin(), order.end());
+    // the operand adjoints: the same for every member (same operands)
+    const AIns& r0 = prog[order[0]];
+    std::vector<int64_t> adj;
+    for (const ATerm& tm : r0.terms) adj.push_back(tm.vid);
+    std::vector<int64_t> sadj(adj);
+    std::sort(sadj.begin(), sadj.end());
+    bool same = true;
+    for (size_t q : order) {
+      const AIns& r = prog[q];
+      if (r.terms.size() != adj.size()) { same = false; break; }
+      std::vector<int64_t> a2;
+      for (const ATerm& tm : r.terms) a2.push_back(tm.vid);
+      std::sort(a2.begin(), a2.end());
+      if (a2 != sadj) { same = false; break; }
+      for (size_t k = 0; k < r.terms.size(); ++k)
+        if (r.terms[k].first && q != order[0]) same = false;
+    }
+    if (!same) continue;
+    auto touches = [&](const AIns& x) {
+      auto hit = [&](int64_t v) { return v >= 0 && std::binary_search(sadj.begin(), sadj.end(), v); };
+      if (hit(x.dst) || hit(x.g) || hit(x.r) || hit(x.adjA) || hit(x.adjB)) return true;
+      if (!x.a.imm && hit(x.a.vid)) return true;
+      if (!x.b.imm && hit(x.b.vid)) return true;
+      for (const ATerm& tm : x.terms)
+        if (hit(tm.vid)) return true;
+      for (int64_t v : x.members)
+        if (hit(v)) return true;
+      return false;
+    };
+    bool clear = true;
+    for (size_t q = order.front() + 1; q < order.back() && clear; ++q)
+      if (!std::binary_search(order.begin(), order.end(), q) && touches(prog[q])) clear = false;
+    if (!clear) continue;
+    // terms in r0's order (operands distinct, so the order across operands
+    // does not matter); members in application order
+    AIns dr;
+    dr.op = I_DENSE_REV;
+    dr.terms = r0.terms;
+    for (size_t q : order) {
+      const AIns& r = prog[q];
+      dr.members.push_back(r.g);
+      std::map<int64_t, const ATerm*> by;
+      for (const ATerm& tm : r.terms) by[tm.vid] = &tm;
+      for (const ATerm& tm : r0.terms) dr.W.push_back(by[tm.vid]->imm);
+      drop[q] = 1;
+    }
+    place.push_back({order.back(), std::move(dr)});
+  }
+  if (place.empty()) return;
+  std::map<size_t, AIns*> at;
+  for (auto& pr : place) at[pr.first] = &pr.second;
+  std::vector<AIns> out;
+  out.reserve(prog.size());
+  for (size_t p = 0; p < prog.size(); ++p) {
+    auto it = at.find(p);
+    if (it != at.end()) out.push_back(std::move(*it->second));
+    if (!drop[p]) out.push_back(std::move(prog[p]));
+  }
+  prog.swap(out);
+}
 
 // One chain of `acc + term` links rooted at an Add.
 struct Term {
@@ -120,7 +259,7 @@ std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const u
   return "";
 }
 
-Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
+Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense) {
   const uint32_t N = (uint32_t)t.opcodes.size();
   const uint32_t ni = t.num_inputs;
   const bool grad = (mode == PM_GRADIENT);
@@ -225,6 +364,33 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
         ch.count++;
       };
       add_ord(i);
+      // Products absorbed here must keep the absorbed ordinals contiguous.
+      // When both operands are products (a chain's first link), they are
+      // judged together: x0*c0, x1*c1, Add is contiguous as a pair though
+      // neither product is alone.
+      bool take_prod[2] = {false, false};
+      {
+        std::vector<uint32_t> cand;
+        for (int k = 0; k < 2; ++k)
+          if (k != take_chain && !is_const(ops[k]) && prod_candidate(ops[k], i)) cand.push_back((uint32_t)k);
+        auto fits = [&](const std::vector<uint32_t>& ks) {
+          uint32_t lo = ch.lo, hi = ch.hi, cnt = ch.count;
+          for (uint32_t k : ks) { lo = std::min(lo, ord[ops[k]]); hi = std::max(hi, ord[ops[k]]); ++cnt; }
+          return cnt == hi - lo + 1;
+        };
+        if (cand.size() == 2 && fits(cand)) {
+          take_prod[0] = take_prod[1] = true;
+        } else {
+          // otherwise the later product first, then the other if it still fits
+          std::sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return ord[ops[a]] > ord[ops[b]]; });
+          std::vector<uint32_t> taken;
+          for (uint32_t k : cand) {
+            taken.push_back(k);
+            if (fits(taken)) take_prod[k] = true;
+            else taken.pop_back();
+          }
+        }
+      }
       for (int k = 0; k < 2; ++k) {
         if (k == take_chain) continue;
         uint32_t o = ops[k];
@@ -234,9 +400,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
         if (is_const(o)) {
           tm.kind = T_IMM; tm.slot = o; tm.imm = t.values[o]; tm.contrib = i;
         } else if (prod_candidate(o, i)) {
-          // keep it only if the product stays contiguous with the chain
-          uint32_t lo = std::min(ch.lo, ord[o]), hi = std::max(ch.hi, ord[o]);
-          if (ch.count + 1 == hi - lo + 1) {
+          if (take_prod[k]) {
             bool cl = is_const(t.arg0[o]);
             tm.kind = T_PROD;
             tm.slot = cl ? t.arg1[o] : t.arg0[o];
@@ -341,6 +505,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
     AIns x;
     x.op = I_DOT;
     x.dst = VAL(r);
+    x.full = lo == 0 && hi == (uint32_t)c.terms.size();
     if (lo > 0) x.flags |= F_DOT_CONT;
     for (uint32_t k = lo; k < hi; ++k) {
       const Term& tm = c.terms[k];
@@ -442,6 +607,9 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
       AIns x;
       x.op = I_DOT_REV;
       x.g = ADJ(r);
+      size_t nonimm = 0;
+      for (const Term& tm : c.terms) nonimm += tm.kind != T_IMM;
+      x.full = terms_.size() == nonimm;
       for (const Term* tm : terms_) {
         ATerm at;
         at.kind = tm->kind;
@@ -511,6 +679,8 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
     }
   }
 
+  if (fuse && dense) group_dense(prog, (int64_t)N);
+
   // ---- live ranges ----
   const size_t NV = 2 * (size_t)N;
   std::vector<int64_t> def(NV, -1), last(NV, -1);
@@ -538,6 +708,16 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
       if (x.flags & F_DOT_CONT) rd(x.dst, p);
       wr(x.dst, p);
     } else if (x.op == I_DOT_REV) {
+      for (auto& tm : x.terms) {
+        if (tm.first) wr(tm.vid, p);
+        else rd(tm.vid, p);
+        rd(tm.vid, p);
+      }
+    } else if (x.op == I_DENSE) {
+      for (auto& tm : x.terms) rd(tm.vid, p);
+      for (int64_t v : x.members) wr(v, p);
+    } else if (x.op == I_DENSE_REV) {
+      for (int64_t v : x.members) rd(v, p);
       for (auto& tm : x.terms) {
         if (tm.first) wr(tm.vid, p);
         else rd(tm.vid, p);
@@ -641,6 +821,32 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse) {
           }
           t_meta[base + k] |= run << M_RUN_SHIFT;
         }
+        break;
+      }
+      case I_DENSE:
+      case I_DENSE_REV: {
+        const size_t T = x.terms.size(), m = x.members.size();
+        w[1] = (uint32_t)m;
+        w[2] = (uint32_t)T;
+        w[3] = (uint32_t)out.pool.size();
+        for (auto& tm : x.terms)
+          out.pool.push_back((tm.kind << P_KIND_SHIFT) | (tm.first ? P_FIRST : 0u) |
+                             (tm.const_left ? P_CLEFT : 0u) | (tm.vid >= 0 ? R(tm.vid) : 0u));
+        for (int64_t v : x.members) out.pool.push_back(R(v));
+        if (t.f64 && (out.pool.size() & 1)) out.pool.push_back(0u);  // 8-byte aligned weights
+        w[4] = (uint32_t)out.pool.size();
+        // DENSE: [T][m] (a term's weights for all members together);
+        // DENSE_REV: [m][T] (a member's weights for all adjoints together)
+        for (size_t a = 0; a < (x.op == I_DENSE ? T : m); ++a)
+          for (size_t b = 0; b < (x.op == I_DENSE ? m : T); ++b) {
+            const size_t j = x.op == I_DENSE ? b : a, q = x.op == I_DENSE ? a : b;
+            double v = x.W[j * T + q];
+            if (x.terms[q].kind == T_REG) v = 1.0;
+            uint32_t lo = 0, hi = 0;
+            put_imm(v, lo, hi);
+            out.pool.push_back(lo);
+            if (t.f64) out.pool.push_back(hi);
+          }
         break;
       }
       default:

@@ -38,6 +38,20 @@ namespace ehip {
 //                     into the side table below
 //   LOADN           : w2 count, w3 first side-table index; reg[] holds the
 //                     destination registers, imm[] the input indices
+//   DENSE           : w1 m members, w2 T terms, w3 pool offset of the term
+//                     words then the m destination registers, w4 pool offset
+//                     of the weights [T][m] (term-major); IW words per weight
+//   DENSE_REV       : w1 m members, w2 T terms, w3 pool offset of the term
+//                     words (adjoint registers) then the m member adjoints
+//                     g_j in application order, w4 weights [m][T]
+// A DENSE replaces m DOT instructions that share one term structure (same
+// kinds, same operand registers, all distinct; weights differ): member j
+// computes exactly what its DOT computed, term by term in the same order, so
+// results are bit-identical; the operand registers are read once per block of
+// members instead of once per member.  DENSE_REV likewise replaces the m
+// DOT_REVs, applying member contributions to each adjoint in their original
+// order.  Pool term word: kind << 30 | first << 29 | const_left << 28 |
+// register.
 constexpr uint32_t NW = 8;
 
 enum : uint32_t {
@@ -50,6 +64,8 @@ enum : uint32_t {
   I_LOADN = 70,
   I_WEND = 71,   // end of a stream window: continue with the next one
   I_END = 72,    // end of the program
+  I_DENSE = 73,
+  I_DENSE_REV = 74,
   I_REV_BASE = 128,
 };
 
@@ -93,9 +109,12 @@ struct TapeHost {
   bool f64 = false;
 };
 
+constexpr uint32_t P_KIND_SHIFT = 30, P_FIRST = 1u << 29, P_CLEFT = 1u << 28, P_REG_MASK = (1u << 28) - 1;
+
 struct Program {
   std::vector<uint32_t> words;  // NW words per instruction
   std::vector<uint32_t> terms;  // meta[T] | reg[T] | imm[T*IW]
+  std::vector<uint32_t> pool;   // DENSE / DENSE_REV operand lists and weights
   uint32_t ninstr = 0;
   uint32_t nregs = 0;
   uint64_t dot_terms = 0;
@@ -115,9 +134,9 @@ enum ProgMode { PM_FORWARD = 0, PM_GRADIENT = 1 };
 constexpr uint32_t WIN_WORDS = 512;
 std::vector<uint32_t> encode_stream(const Program& p, bool f64);
 
-// Build the program; `fuse` enables DOT fusion.  Never fails on a validated
-// tape.
-Program compile_program(const TapeHost& t, ProgMode mode, bool fuse);
+// Build the program; `fuse` enables DOT fusion, `dense` the grouping of
+// fused dot products into DENSE / DENSE_REV.  Never fails on a validated tape.
+Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense = true);
 
 // GpuTapeData::validate (src/gpu/mod.rs:397-564); returns "" when valid.
 std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,

@@ -12,9 +12,10 @@
 // Samples: plain STDE uses 2*c2 (= v^T H v); the diagonal control variate
 // uses the residual 2*c2 - sum_j d_j v_j^2 with the exact tr(D) added to the
 // mean afterwards (laplacian_with_control_gpu's form).  The learned control
-// variate subtracts v^T L L^T v - tr(L L^T) = |L^T v|^2 - |L|_F^2, which has
-// mean zero for any L when E[v v^T] = I, so the estimate stays unbiased; the
-// plain statistics of the same samples can be written alongside.
+// variate subtracts v^T L S L^T v - tr(L S L^T) (S = diag(+-1); S = I is
+// |L^T v|^2 - |L|_F^2), which has mean zero for any L when E[v v^T] = I, so
+// the estimate stays unbiased; the plain statistics of the same samples can
+// be written alongside.
 #include <hip/hip_runtime.h>
 
 #include "ehip_kernels.h"
@@ -93,7 +94,10 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
   // |L|_F^2 of this point, summed over the wave
   double lnorm = 0.0;
   if (LC) {
-    for (uint64_t k = lane; k < (uint64_t)R.nin * R.rank; k += 64) lnorm += (double)LC[k] * (double)LC[k];
+    for (uint64_t k = lane; k < (uint64_t)R.nin * R.rank; k += 64) {
+      const double l2 = (double)LC[k] * (double)LC[k];
+      lnorm += (k % R.rank) < R.npos ? l2 : -l2;
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) lnorm += __shfl_xor(lnorm, off, 64);
   }
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
       }
       double q = 0.0;
 #pragma unroll
-      for (int j = 0; j < CV_RMAX; ++j) q += u[j] * u[j];
+      for (int j = 0; j < CV_RMAX; ++j) q += (j < (int)R.npos) ? u[j] * u[j] : -(u[j] * u[j]);
       sample -= q - lnorm;
     }
     if (DG) {

@@ -46,6 +46,7 @@ class LearnedControlVariate:
         self.handle = h
         self.ctx = ctx
         self.n, self.rank, self.hidden = n, rank, hidden
+        self.npos = rank
         ctx._children.add(self)  # released before the context
 
     def close(self) -> None:
@@ -62,6 +63,11 @@ class LearnedControlVariate:
     def set_gain(self, gain: float) -> None:
         """L(x) = gain * (W3 h2 + b3)."""
         _check(load_library().ehip_cv_set_gain(self.handle, gain))
+
+    def set_signs(self, npos: int) -> None:
+        """Use v^T L S L^T v - tr(L S L^T), S = diag(+1 x npos, -1 x (rank - npos))."""
+        _check(load_library().ehip_cv_set_signs(self.handle, npos))
+        self.npos = npos
 
     def __del__(self):
         try:
@@ -200,6 +206,7 @@ class LearnedStde:
     rank: int = 16
     hidden: int = 256
     seed: int = 0
+    npos: Optional[int] = None  # signed factor columns with +1; None: rank (L L^T)
     cv: Optional[LearnedControlVariate] = None
     variance_ratio: float = float("nan")
     use_cv: bool = False
@@ -219,6 +226,8 @@ class LearnedStde:
         if out_scale is None:
             out_scale = (n * self.rank) ** -0.25 / 0.6
         self.cv = LearnedControlVariate(self.ctx, n, self.rank, self.hidden, self.seed, out_scale)
+        if self.npos is not None:
+            self.cv.set_signs(self.npos)
         x, v, hv = make_tuples(self.ctx, self.tape, ntuples, self.seed + 1)
         k = min(ntuples, 4096)
         hs = hv[:k].double().cpu().numpy()

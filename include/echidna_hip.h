@@ -161,24 +161,27 @@ int32_t ehip_laplacian_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const voi
                                  const void* d_diag, uint64_t diag_stride, double* d_results, void* stream);
 
 /* Introspection of the compiled register program (for tests and the
- * traffic model): mode 0 forward, 1 gradient/HVP. out[0]=instructions,
+ * traffic model): mode 0 forward, 1 gradient, 2 HVP. out[0]=instructions,
  * out[1]=registers, out[2]=program words, out[3]=fused dot terms. */
 int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
 
 /* Compile a tape into its register program without a device (inspection
- * and tests).  Same validation as ehip_tape_upload.  Mode 0 forward, 1
- * gradient/HVP: writes up to `words_cap` program words (8 per instruction)
- * and `terms_cap` side-table words; out4 = {instructions, registers, program
- * words, side-table entries T}; the side table needs max(4, T*(2+IW)) words
- * (IW = 1 f32, 2 f64).  Mode 2 forward, 3 gradient: writes the windowed
- * instruction stream the kernels execute (512-word windows, dot terms
- * inline) into `words`, out4[2] = its length; `terms` is unused.  Returns
- * EHIP_OTHER if a buffer is too small (out4 still holds the sizes). */
+ * and tests).  Same validation as ehip_tape_upload.  fuse: 0 none, 1 dot
+ * fusion and DENSE grouping (what uploads use), 2 dot fusion only.  Mode 0
+ * forward, 1 gradient: writes up to `words_cap` program words (8 per
+ * instruction) and `terms_cap` side-table words; the side table needs
+ * max(4, T*(2+IW)) words (IW = 1 f32, 2 f64).  Mode 2 forward, 3 gradient:
+ * writes the windowed instruction stream the kernels execute (512-word
+ * windows, dot terms inline) into `words`; `terms` is unused.  Every mode
+ * writes the DENSE pool (operand lists and weights) into `pool`.  out5 =
+ * {instructions, registers, program or stream words, side-table entries T,
+ * pool words}.  Returns EHIP_OTHER if a buffer is too small (out5 still
+ * holds the sizes). */
 int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                              const double* constants, uint32_t num_ops, uint32_t num_inputs,
                              uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,
                              int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
-                             uint64_t terms_cap, uint64_t* out4);
+                             uint64_t terms_cap, uint32_t* pool, uint64_t pool_cap, uint64_t* out5);
 
 /* Launch tuning knobs (autotuner): workgroup size (64/128/256) and the
  * per-workgroup LDS budget for the register file in bytes.  0 = default. */
@@ -224,6 +227,11 @@ int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t
 int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget);
 int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out2);
 
+/* DENSE grouping for tapes uploaded after this call (default on): fused dot
+ * products sharing one operand list run as one register-blocked instruction
+ * (ehip_program.h); results are identical with it off. */
+int32_t ehip_set_dense(ehip_ctx* ctx, int32_t enable);
+
 /* Dot-product fusion for tapes uploaded after this call (default on).  Off
  * runs every tape entry as its own instruction; results are identical. */
 int32_t ehip_set_fusion(ehip_ctx* ctx, int32_t enable);
@@ -254,6 +262,10 @@ int32_t ehip_cv_set_params(ehip_cv* cv, const float* host_in);
  * training loop then works on a target of unit scale).  Default 1. */
 int32_t ehip_cv_set_gain(ehip_cv* cv, float gain);
 float ehip_cv_gain(const ehip_cv* cv);
+/* Signed factor: the control variate becomes v^T L S L^T v - tr(L S L^T)
+ * with S = diag(+1 for the first npos columns, -1 for the rest), still mean
+ * zero, able to follow indefinite Hessians.  Default npos = rank (L L^T). */
+int32_t ehip_cv_set_signs(ehip_cv* cv, uint32_t npos);
 /* L(x) for `batch` FP32 points: d_L [batch][n*r], L[i][j] at i*r + j. */
 int32_t ehip_cv_forward_dev(ehip_cv* cv, const float* d_x, uint64_t batch, float* d_L, void* stream);
 /* Gradients of mean_t ||hv_t - L(x_t) L(x_t)^T v_t||^2 / n into the

@@ -46,18 +46,19 @@ def forward(flat, X, n, H, r, bf=True, gain=1.0):
     return H1, H2, Lf
 
 
-def loss_grad(flat, X, V, HV, n, H, r, bf=True, gain=1.0):
-    """Mean over tuples of ||hv - L L^T v||^2 / n and its flat gradient,
-    L = gain * (W3 h2 + b3)."""
+def loss_grad(flat, X, V, HV, n, H, r, bf=True, gain=1.0, npos=None):
+    """Mean over tuples of ||hv - L S L^T v||^2 / n and its flat gradient,
+    L = gain * (W3 h2 + b3), S = diag(+1 x npos, -1 x (r - npos))."""
+    sgn = np.where(np.arange(r) < (r if npos is None else npos), 1.0, -1.0)
     W1, b1, W2, b2, W3, b3 = split(flat, n, H, r)
     T = X.shape[0]
     H1, H2, Lf = forward(flat, X, n, H, r, bf, gain)
     L = Lf.reshape(T, n, r)
     V = np.asarray(V, np.float64)
     HV = np.asarray(HV, np.float64)
-    u = np.einsum("tir,ti->tr", L, V)
+    u = np.einsum("tir,ti->tr", L, V) * sgn
     e = HV - np.einsum("tir,tr->ti", L, u)
-    w = np.einsum("tir,ti->tr", L, e)
+    w = np.einsum("tir,ti->tr", L, e) * sgn
     loss = np.mean(np.sum(e * e, axis=1) / n)
     gs = -2.0 * gain / (n * T)
     dL = gs * (np.einsum("ti,tr->tir", e, u) + np.einsum("ti,tr->tir", V, w))

@@ -126,6 +126,8 @@ def split_terms(terms, nterms, f64):
 
 WIN_WORDS = 512
 I_WEND, I_END = 71, 72
+I_DENSE, I_DENSE_REV = 73, 74
+P_KIND_SHIFT, P_FIRST, P_CLEFT, P_REG_MASK = 30, 1 << 29, 1 << 28, (1 << 28) - 1
 
 
 def decode_stream(stream, f64):
@@ -170,7 +172,7 @@ def decode_stream(stream, f64):
     return out
 
 
-def run(words, terms, x, nin, nout, nregs, f64=True, mode=1, nterms=None):
+def run(words, terms, x, nin, nout, nregs, f64=True, mode=1, nterms=None, pool=None):
     """Run the program over points ``x`` [B, nin]; returns (y [B,nout], g [B,nin])."""
     tl = split_terms(terms, nterms, f64)
     instrs = []
@@ -181,15 +183,28 @@ def run(words, terms, x, nin, nout, nregs, f64=True, mode=1, nterms=None):
             instrs.append((w, tl[w[3]:w[3] + w[2]]))
         else:
             instrs.append((w, []))
-    return execute(instrs, x, nin, nout, nregs, f64)
+    return execute(instrs, x, nin, nout, nregs, f64, pool)
 
 
-def run_stream(stream, x, nin, nout, nregs, f64=True):
+def run_stream(stream, x, nin, nout, nregs, f64=True, pool=None):
     """Run a windowed stream (``ehip_compile_program`` modes 2/3)."""
-    return execute(decode_stream(stream, f64), x, nin, nout, nregs, f64)
+    return execute(decode_stream(stream, f64), x, nin, nout, nregs, f64, pool)
 
 
-def execute(instrs, x, nin, nout, nregs, f64=True):
+def dense_parts(w, pool, f64):
+    """(term words, member registers, weights) of a DENSE / DENSE_REV."""
+    pool = np.asarray(pool, dtype=np.uint32)
+    m, T, o_r, o_w = w[1], w[2], w[3], w[4]
+    tw = [int(v) for v in pool[o_r:o_r + T]]
+    mem = [int(v) for v in pool[o_r + T:o_r + T + m]]
+    if f64:
+        W = pool[o_w:o_w + 2 * m * T].view(np.float64)
+    else:
+        W = pool[o_w:o_w + m * T].view(np.float32)
+    return tw, mem, W
+
+
+def execute(instrs, x, nin, nout, nregs, f64=True, pool=None):
     F = np.float64 if f64 else np.float32
     B = x.shape[0]
     regs = np.full((nregs, B), np.nan, dtype=F)
@@ -260,6 +275,36 @@ def execute(instrs, x, nin, nout, nregs, f64=True):
                     else:
                         cur[live] = cur[live] + F(1) * gg[live]
                     regs[t1] = cur
+            elif op == I_DENSE:
+                tw, mem, W = dense_parts(w, pool, f64)
+                m, T = len(mem), len(tw)
+                W = W.reshape(T, m)
+                for j in range(m):
+                    acc = None
+                    for p, word in enumerate(tw):
+                        kind, r = word >> P_KIND_SHIFT, word & P_REG_MASK
+                        c = F(W[p, j])
+                        term = regs[r] * c if kind == T_PROD else (regs[r].copy() if kind == T_REG
+                                                                   else np.full(B, c, F))
+                        acc = term if acc is None else acc + term
+                    regs[mem[j]] = acc
+            elif op == I_DENSE_REV:
+                tw, mem, W = dense_parts(w, pool, f64)
+                m, T = len(mem), len(tw)
+                W = W.reshape(m, T)
+                for i, word in enumerate(tw):
+                    kind, r = word >> P_KIND_SHIFT, word & P_REG_MASK
+                    cur = np.zeros(B, F) if word & P_FIRST else regs[r].copy()
+                    for j in range(m):
+                        gg = regs[mem[j]]
+                        live = gg != 0
+                        c = F(W[j, i])
+                        if kind == T_PROD:
+                            if c != 0:
+                                cur[live] = cur[live] + c * gg[live]
+                        else:
+                            cur[live] = cur[live] + F(1) * gg[live]
+                    regs[r] = cur
             elif op == I_LOAD:
                 regs[w[1]] = x[:, w[2]]
             elif op == I_STORE_OUT:

@@ -6,23 +6,25 @@ Rademacher sign vectors."""
 import itertools
 
 import numpy as np
+import pytest
 
 import cv_ref
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("npos", [None, 1, 0])
+def test_gradient_matches_finite_differences(npos):
     n, H, r, T = 3, 4, 2, 5
     rng = np.random.default_rng(0)
     flat = rng.normal(size=H * n + H + H * H + H + n * r * H + n * r) * 0.7
     X, V, HV = rng.normal(size=(T, n)), rng.normal(size=(T, n)), rng.normal(size=(T, n))
-    loss, g = cv_ref.loss_grad(flat, X, V, HV, n, H, r, bf=False)
+    loss, g = cv_ref.loss_grad(flat, X, V, HV, n, H, r, bf=False, gain=1.3, npos=npos)
     h = 1e-6
     fd = np.empty_like(flat)
     for k in range(flat.size):
         e = np.zeros_like(flat)
         e[k] = h
-        lp, _ = cv_ref.loss_grad(flat + e, X, V, HV, n, H, r, bf=False)
-        lm, _ = cv_ref.loss_grad(flat - e, X, V, HV, n, H, r, bf=False)
+        lp, _ = cv_ref.loss_grad(flat + e, X, V, HV, n, H, r, bf=False, gain=1.3, npos=npos)
+        lm, _ = cv_ref.loss_grad(flat - e, X, V, HV, n, H, r, bf=False, gain=1.3, npos=npos)
         fd[k] = (lp - lm) / (2 * h)
     assert np.max(np.abs(fd - g)) <= 1e-7 * max(1.0, np.max(np.abs(g)))
 
@@ -32,8 +34,9 @@ def test_control_variate_has_mean_zero_over_all_rademacher_vectors():
     n, r = 5, 3
     L = rng.normal(size=(n, r))
     vs = np.array(list(itertools.product([-1.0, 1.0], repeat=n)))
-    cv = np.sum((vs @ L) ** 2, axis=1) - np.sum(L * L)
-    assert abs(cv.mean()) <= 1e-12
+    for s in (np.ones(r), np.array([1.0, -1.0, -1.0])):
+        cv = np.sum(s * (vs @ L) ** 2, axis=1) - np.sum(s * L * L)
+        assert abs(cv.mean()) <= 1e-12
 
 
 def test_loss_is_frobenius_distance_over_all_rademacher_vectors():

@@ -69,3 +69,26 @@ def test_autotune_rejects_bad_arguments(hip):
     x = torch.zeros(4, 4, device="cuda")
     with pytest.raises(GpuError, match="taylor order"):
         hip.autotune_dev(bufs, TUNE_TAYLOR, 4, x, d_y=x, d_dirs=x, order=9)
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_dense_grouping_is_bit_identical_on_device(hip, f64):
+    """Uploads with and without DENSE grouping give the same bits for values,
+    gradients and Taylor jets."""
+    tape = wl.tanh_mlp2(20, 24, 18, 35)
+    F = np.float64 if f64 else np.float32
+    B = 700
+    x = wl.points_normal(12, B, 20).astype(F)
+    v = philox.normal(13, philox.STREAM_DIRS, np.arange(B * 20)).reshape(B, 20).astype(F)
+    outs = []
+    for dense in (True, False):
+        hip.set_dense(dense)
+        try:
+            bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+        finally:
+            hip.set_dense(True)
+        y, g = hip.gradient_batch(bufs, x.reshape(-1), B)
+        tk = hip.taylor_forward_kth_batch(bufs, x.reshape(-1), v.reshape(-1), B, 3)
+        outs.append([np.asarray(a).copy() for a in (y, g, *tk.coefficients)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))

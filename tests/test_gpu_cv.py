@@ -64,6 +64,31 @@ def test_gradients_match_reference(hip):
         assert rel(a, b) <= 2e-3, k
 
 
+def test_signed_factor_gradients_and_unbiasedness(hip):
+    """S = diag(+1, +1, -1, -1): gradients against the reference and the
+    controlled estimate still centred on the exact trace."""
+    net = random_cv(hip, seed=4)
+    net.set_signs(2)
+    net.set_gain(0.7)
+    T = 45
+    X = wl.points_normal(11, T, N).astype(np.float32)
+    V = philox.normal(12, philox.STREAM_DIRS, np.arange(T * N)).reshape(T, N).astype(np.float32)
+    HV = (V * -0.4).astype(np.float32)
+    loss = net.loss_grad(*[torch.from_numpy(a).cuda() for a in (X, V, HV)])
+    ref_loss, ref_g = cv_ref.loss_grad(net.params(), X, V, HV, N, H, R, bf=True, gain=0.7, npos=2)
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    for k, (a, b) in enumerate(zip(net.split(net.grads()), net.split(ref_g))):
+        assert rel(a, b) <= 2e-3, k
+    tape = wl.tanh_mlp(N, 20, 56)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    P = 48
+    Xp = torch.from_numpy(wl.points_normal(26, P, N)).cuda()
+    exact = lcv.hessian_trace_batch(hip, bufs, Xp)
+    res = lcv.laplacian_batch(hip, bufs, Xp, 512, lcv.DIR_GAUSSIAN, 13, net)
+    z = (res[:, 1] - exact) / res[:, 3]
+    assert np.all(np.abs(z) < 5.0) and abs(np.mean(z)) < 4.0 / np.sqrt(P)
+
+
 def test_gradients_chunked_equal_single_pass(hip):
     """Batches above the 2048-tuple chunk accumulate gradients over chunks."""
     net = random_cv(hip)
@@ -169,6 +194,8 @@ def test_fallback_to_plain_when_the_network_hurts(hip):
 def test_cv_errors(hip):
     with pytest.raises(GpuError, match="rank must be 1..=32"):
         lcv.LearnedControlVariate(hip, 4, 33, 8)
+    with pytest.raises(GpuError, match="npos must be <= rank"):
+        lcv.LearnedControlVariate(hip, 4, 3, 8).set_signs(4)
     tape = wl.tanh_mlp(5, 6, 55)
     bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
     net = lcv.LearnedControlVariate(hip, 6, 2, 8)

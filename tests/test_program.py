@@ -34,7 +34,8 @@ def test_program_matches_oracle(name, fuse, f64):
     y_ref, g_ref = po.gradient(tape, x, dtype=F)
     for mode in (0, 1):
         words, terms, info = compile_program(d, mode=mode, fuse=fuse, f64=f64)
-        y, g = emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64, nterms=info["nterms"])
+        y, g = emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64,
+                       nterms=info["nterms"], pool=info["pool"])
         tol = 1e-13 if f64 else 2e-5
         scale = np.maximum(np.abs(y_ref), 1.0)
         assert np.all(np.abs(y - y_ref) <= tol * scale), (name, mode)
@@ -58,14 +59,52 @@ def test_windowed_stream_runs_like_the_program(name, f64):
         stream, _, sinfo = compile_program(d, mode=mode + 2, fuse=True, f64=f64)
         assert sinfo["registers"] == info["registers"] and len(stream) % emu.WIN_WORDS == 0
         y, g = emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64,
-                       nterms=info["nterms"])
-        ys, gs = emu.run_stream(stream, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64)
+                       nterms=info["nterms"], pool=info["pool"])
+        ys, gs = emu.run_stream(stream, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64,
+                                pool=sinfo["pool"])
         assert np.array_equal(y.view(np.uint8), ys.view(np.uint8)), (name, mode)
         assert np.array_equal(g.view(np.uint8), gs.view(np.uint8)), (name, mode)
     if name == "wide_mlp":
         ins = emu.decode_stream(stream, f64)
         assert len(stream) // emu.WIN_WORDS > 10
         assert any(w[0] & 0xFF == emu.I_DOT and (w[0] >> 8) & emu.F_DOT_CONT for w, _ in ins)
+
+
+@pytest.mark.parametrize("name", ["mlp32", "mlp2", "ridge"])
+@pytest.mark.parametrize("f64", [True, False])
+def test_dense_grouping_is_bit_identical(name, f64):
+    """DENSE / DENSE_REV replace groups of DOTs sharing one operand list;
+    every value and adjoint is computed by the same operations in the same
+    order, so the emulated results match the DOT-level program bit for bit."""
+    tape = wl.softplus_ridge(16, 6, 53) if name == "ridge" else TAPES[name]()
+    d = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
+    F = np.float64 if f64 else np.float32
+    x = (wl.points_normal(12, 11, tape.num_inputs) * 0.8).astype(F)
+    for mode in (0, 1):
+        outs = []
+        for dense in (True, False):
+            words, terms, info = compile_program(d, mode=mode, dense=dense, f64=f64)
+            ops = [int(w[0]) & 0xFF for w in words]
+            assert (emu.I_DENSE in ops) == dense
+            assert (emu.I_DENSE_REV in ops) == (dense and mode == 1)
+            outs.append(emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64,
+                                nterms=info["nterms"], pool=info["pool"]))
+        for a, b in zip(outs[0], outs[1]):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (name, mode)
+
+
+def test_dense_needs_distinct_operands():
+    """A dot product that uses one operand twice keeps its DOT form."""
+    def f(v):
+        s = v[0] * 0.0
+        for j in range(6):
+            s = s + (v[0] * (0.1 + j) + v[0] * (0.2 * j) + v[1] * 0.3).tanh()
+        return s
+
+    tape, _ = record(f, [0.3, 0.4])
+    words, _, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=1)
+    ops = [int(w[0]) & 0xFF for w in words]
+    assert emu.I_DENSE not in ops and emu.I_DENSE_REV not in ops
 
 
 def test_fusion_collapses_mlp_dot_products():
@@ -100,7 +139,7 @@ def test_register_reuse_bounds_live_set():
     assert info["registers"] <= 3
     x = np.array([[0.3], [1.1]])
     words, terms, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=1)
-    y, g = emu.run(words, terms, x, 1, 1, info["registers"], nterms=info["nterms"])
+    y, g = emu.run(words, terms, x, 1, 1, info["registers"], nterms=info["nterms"], pool=info["pool"])
     y_ref, g_ref = po.gradient(tape, x)
     assert np.allclose(y, y_ref, rtol=1e-13) and np.allclose(g, g_ref, rtol=1e-12)
 
@@ -109,6 +148,6 @@ def test_multi_output_program():
     tape, _ = record_multi(lambda v: [v[0] * v[1], v[0].sin() + v[1], v[1]], [1.0, 2.0])
     words, terms, info = compile_program(GpuTapeData.from_tape_f64(tape), mode=1)
     x = np.array([[0.5, 2.0], [1.5, -1.0]])
-    y, g = emu.run(words, terms, x, 2, 3, info["registers"], nterms=info["nterms"])
+    y, g = emu.run(words, terms, x, 2, 3, info["registers"], nterms=info["nterms"], pool=info["pool"])
     y_ref, g_ref = po.gradient(tape, x)
     assert np.allclose(y, y_ref, rtol=1e-14) and np.allclose(g, g_ref, rtol=1e-14)
