@@ -193,97 +193,137 @@ __device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, AS3 c
 // members.  Operand lists and weights come from the pool through
 // wave-uniform (scalar) loads.
 // ---------------------------------------------------------------------------
-// JET: products follow the Taylor product with a constant jet (jt::mul,
-// the operation the jet interpreter's DOT performs); otherwise v * c.
+// One block of JB members (FULL: exactly JB, no per-member bounds checks).
+// JET: products follow the Taylor product with a constant jet (jt::mul, the
+// operation the jet interpreter's DOT performs); otherwise v * c.
+template <class F, int NC, int JB, bool JET, bool FULL, class Regs_>
+__device__ __forceinline__ void dense_fwd_block(const Regs_& R, const uint32_t* PR, const F* PW, uint32_t m,
+                                                uint32_t T, uint32_t j0, uint32_t jb) {
+  F acc[JB][NC];
+  for (uint32_t p = 0; p < T; ++p) {
+    const uint32_t kr = PR[p];
+    const uint32_t kind = kr >> P_KIND_SHIFT;
+    const F* w = PW + (uint64_t)p * m + j0;
+    if (kind == T_PROD) {
+      F x[NC];
+      R.rd(kr & P_REG_MASK, x);
+      const bool cleft = (kr & P_CLEFT) != 0;
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        if (!(FULL || (uint32_t)j < jb)) continue;
+        const F c = w[j];
+        F term[NC];
+        if (JET) {
+          F cj[NC];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) cj[q] = (q == 0) ? c : F(0);
+          if (cleft) jt::mul<F, NC>(cj, x, term);
+          else jt::mul<F, NC>(x, cj, term);
+        } else {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) term[q] = x[q] * c;
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
+      }
+    } else if (kind == T_REG) {
+      F x[NC];
+      R.rd(kr & P_REG_MASK, x);
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        if (!(FULL || (uint32_t)j < jb)) continue;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? x[q] : acc[j][q] + x[q];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        if (!(FULL || (uint32_t)j < jb)) continue;
+        const F c = w[j];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          const F term = (q == 0) ? c : F(0);
+          acc[j][q] = (p == 0) ? term : acc[j][q] + term;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < JB; ++j) {
+    if (!(FULL || (uint32_t)j < jb)) continue;
+    R.wr(PR[T + j0 + j], acc[j]);
+  }
+}
+
 template <class F, int NC, int JB, bool JET, class Regs_>
 __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, uint32_t m, uint32_t T,
                                           uint32_t off_regs, uint32_t off_w) {
   const uint32_t* PR = pool + off_regs;
   const F* PW = reinterpret_cast<const F*>(pool + off_w);
-  for (uint32_t j0 = 0; j0 < m; j0 += JB) {
-    const uint32_t jb = (m - j0 < (uint32_t)JB) ? m - j0 : (uint32_t)JB;
-    F acc[JB][NC];
-    for (uint32_t p = 0; p < T; ++p) {
-      const uint32_t kr = PR[p];
-      const uint32_t kind = kr >> P_KIND_SHIFT;
-      F x[NC];
-      if (kind != T_IMM) R.rd(kr & P_REG_MASK, x);
-      const bool cleft = (kr & P_CLEFT) != 0;
-      const F* w = PW + (uint64_t)p * m + j0;
-#pragma unroll
-      for (int j = 0; j < JB; ++j) {
-        if ((uint32_t)j >= jb) break;
-        const F c = w[j];
-        F term[NC];
-        if (kind == T_PROD) {
-          if (JET) {
-            F cj[NC];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) cj[q] = (q == 0) ? c : F(0);
-            if (cleft) jt::mul<F, NC>(cj, x, term);
-            else jt::mul<F, NC>(x, cj, term);
-          } else {
-#pragma unroll
-            for (int q = 0; q < NC; ++q) term[q] = x[q] * c;
-          }
-        } else if (kind == T_REG) {
-#pragma unroll
-          for (int q = 0; q < NC; ++q) term[q] = x[q];
-        } else {
-#pragma unroll
-          for (int q = 0; q < NC; ++q) term[q] = (q == 0) ? c : F(0);
-        }
-#pragma unroll
-        for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < JB; ++j) {
-      if ((uint32_t)j >= jb) break;
-      R.wr(PR[T + j0 + j], acc[j]);
-    }
-  }
+  uint32_t j0 = 0;
+  for (; j0 + JB <= m; j0 += JB) dense_fwd_block<F, NC, JB, JET, true>(R, PR, PW, m, T, j0, JB);
+  if (j0 < m) dense_fwd_block<F, NC, JB, JET, false>(R, PR, PW, m, T, j0, m - j0);
 }
 
-template <class F, bool LDSONLY>
-__device__ __forceinline__ void dense_rev(const Regs<F, 1, LDSONLY>& R, const uint32_t* pool, uint32_t m, uint32_t T,
-                                          uint32_t off_regs, uint32_t off_w) {
-  constexpr int IB = 16;
-  const uint32_t* PR = pool + off_regs;
-  const F* PW = reinterpret_cast<const F*>(pool + off_w);
-  for (uint32_t i0 = 0; i0 < T; i0 += IB) {
-    const uint32_t ib = (T - i0 < (uint32_t)IB) ? T - i0 : (uint32_t)IB;
-    F cur[IB];
+// One block of IB adjoints.  Fast path, taken per member when the compiler
+// found every term a product with a non-zero weight (D_ALL_PROD_NZ) and the
+// member's adjoint is finite and non-zero in every lane: then the reference's
+// zero-adjoint skip and zero-multiplier guard cannot trigger and the
+// contribution is c * g unconditionally.
+template <class F, int IB, bool FULL, bool LDSONLY>
+__device__ __forceinline__ void dense_rev_block(const Regs<F, 1, LDSONLY>& R, const uint32_t* PR, const F* PW,
+                                                uint32_t m, uint32_t T, uint32_t i0, uint32_t ib, bool fastw) {
+  F cur[IB];
+  uint32_t kr[IB];
 #pragma unroll
-    for (int i = 0; i < IB; ++i) {
-      cur[i] = F(0);
-      if ((uint32_t)i >= ib) break;
-      const uint32_t kr = PR[i0 + i];
-      if (!(kr & P_FIRST)) R.rd(kr & P_REG_MASK, &cur[i]);
-    }
-    for (uint32_t j = 0; j < m; ++j) {
-      F g;
-      R.rd(PR[T + j], &g);
-      const bool live = g != F(0);
-      const F* w = PW + (uint64_t)j * T + i0;
+  for (int i = 0; i < IB; ++i) {
+    cur[i] = F(0);
+    kr[i] = 0;
+    if (!(FULL || (uint32_t)i < ib)) continue;
+    kr[i] = PR[i0 + i];
+    if (!(kr[i] & P_FIRST)) R.rd(kr[i] & P_REG_MASK, &cur[i]);
+  }
+  for (uint32_t j = 0; j < m; ++j) {
+    F g;
+    R.rd(PR[T + j], &g);
+    const F* w = PW + (uint64_t)j * T + i0;
+    if (fastw && __all(g != F(0) && isfinite(g))) {
 #pragma unroll
       for (int i = 0; i < IB; ++i) {
-        if ((uint32_t)i >= ib) break;
-        const uint32_t kind = PR[i0 + i] >> P_KIND_SHIFT;
+        if (!(FULL || (uint32_t)i < ib)) continue;
+        cur[i] = cur[i] + w[i] * g;
+      }
+    } else {
+      const bool live = g != F(0);
+#pragma unroll
+      for (int i = 0; i < IB; ++i) {
+        if (!(FULL || (uint32_t)i < ib)) continue;
         const F c = w[i];
-        if (kind == T_PROD) {
+        if ((kr[i] >> P_KIND_SHIFT) == T_PROD) {
           if (live && c != F(0)) cur[i] = cur[i] + c * g;
         } else if (live) {
           cur[i] = cur[i] + F(1) * g;
         }
       }
     }
-#pragma unroll
-    for (int i = 0; i < IB; ++i) {
-      if ((uint32_t)i >= ib) break;
-      R.wr(PR[i0 + i] & P_REG_MASK, &cur[i]);
-    }
   }
+#pragma unroll
+  for (int i = 0; i < IB; ++i) {
+    if (!(FULL || (uint32_t)i < ib)) continue;
+    R.wr(kr[i] & P_REG_MASK, &cur[i]);
+  }
+}
+
+template <class F, bool LDSONLY>
+__device__ __forceinline__ void dense_rev(const Regs<F, 1, LDSONLY>& R, const uint32_t* pool, uint32_t m, uint32_t T,
+                                          uint32_t off_regs, uint32_t off_w, uint32_t dflags) {
+  constexpr int IB = sizeof(F) == 8 ? 4 : 8;
+  const uint32_t* PR = pool + off_regs;
+  const F* PW = reinterpret_cast<const F*>(pool + off_w);
+  const bool fastw = (dflags & D_ALL_PROD_NZ) != 0;
+  uint32_t i0 = 0;
+  for (; i0 + IB <= T; i0 += IB) dense_rev_block<F, IB, true, LDSONLY>(R, PR, PW, m, T, i0, IB, fastw);
+  if (i0 < T) dense_rev_block<F, IB, false, LDSONLY>(R, PR, PW, m, T, i0, T - i0, fastw);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,7 +454,7 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
     } else if (op == I_DENSE) {
       dense_fwd<F, 1, 16, false>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4));
     } else if (op == I_DENSE_REV) {
-      dense_rev<F, LDSONLY>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4));
+      dense_rev<F, LDSONLY>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4), I.u(5));
     } else if (op == I_LOADN) {
       load_inputs<F, 1, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
         v[0] = act ? X[p * L.xstride + i] : F(0);

@@ -837,16 +837,20 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
         w[4] = (uint32_t)out.pool.size();
         // DENSE: [T][m] (a term's weights for all members together);
         // DENSE_REV: [m][T] (a member's weights for all adjoints together)
+        bool prod_nz = true;
         for (size_t a = 0; a < (x.op == I_DENSE ? T : m); ++a)
           for (size_t b = 0; b < (x.op == I_DENSE ? m : T); ++b) {
             const size_t j = x.op == I_DENSE ? b : a, q = x.op == I_DENSE ? a : b;
             double v = x.W[j * T + q];
             if (x.terms[q].kind == T_REG) v = 1.0;
+            if (x.terms[q].kind != T_PROD) prod_nz = false;
             uint32_t lo = 0, hi = 0;
             put_imm(v, lo, hi);
+            if ((t.f64 ? v : (double)(float)v) == 0.0) prod_nz = false;
             out.pool.push_back(lo);
             if (t.f64) out.pool.push_back(hi);
           }
+        w[5] = prod_nz ? D_ALL_PROD_NZ : 0u;
         break;
       }
       default:

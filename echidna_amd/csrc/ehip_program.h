@@ -43,7 +43,7 @@ namespace ehip {
 //                     of the weights [T][m] (term-major); IW words per weight
 //   DENSE_REV       : w1 m members, w2 T terms, w3 pool offset of the term
 //                     words (adjoint registers) then the m member adjoints
-//                     g_j in application order, w4 weights [m][T]
+//                     g_j in application order, w4 weights [m][T], w5 flags
 // A DENSE replaces m DOT instructions that share one term structure (same
 // kinds, same operand registers, all distinct; weights differ): member j
 // computes exactly what its DOT computed, term by term in the same order, so
@@ -110,6 +110,9 @@ struct TapeHost {
 };
 
 constexpr uint32_t P_KIND_SHIFT = 30, P_FIRST = 1u << 29, P_CLEFT = 1u << 28, P_REG_MASK = (1u << 28) - 1;
+// DENSE_REV w5: every term is a product and no weight is zero (in the
+// program's dtype), so only the adjoints can trigger the reference guards.
+constexpr uint32_t D_ALL_PROD_NZ = 1u;
 
 struct Program {
   std::vector<uint32_t> words;  // NW words per instruction
