@@ -91,6 +91,21 @@ _SIGS = {
     "ehip_laplacian_cv_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32,
                                                      ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, _vp]),
     "ehip_hessian_trace_batch_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
+    "ehip_hessian_sparsity": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp,
+                                               ctypes.c_uint64, _c_u64p]),
+    "ehip_jacobian_sparsity": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp,
+                                                ctypes.c_uint32, _vp, _vp, ctypes.c_uint64, _c_u64p]),
+    "ehip_greedy_coloring": (ctypes.c_int32, [ctypes.c_uint32, _vp, _vp, ctypes.c_uint64, _vp, _c_u32p]),
+    "ehip_column_coloring": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, ctypes.c_uint64, _vp,
+                                              _c_u32p]),
+    "ehip_row_coloring": (ctypes.c_int32, [ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, ctypes.c_uint64, _vp,
+                                           _c_u32p]),
+    "ehip_sparse_hessian": (ctypes.c_int32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_uint64, _c_u64p]),
+    "ehip_sparse_jacobian": (ctypes.c_int32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_uint64, _c_u64p]),
+    "ehip_kink_count": (ctypes.c_uint32, [_vp]),
+    "ehip_kink_info": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32]),
+    "ehip_forward_nonsmooth_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    "ehip_forward_nonsmooth": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "ehip_multi_batch_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp]),
     "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),

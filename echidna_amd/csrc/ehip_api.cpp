@@ -129,8 +129,11 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
   t->prog[0].host = compile_program(t->host, PM_FORWARD, fuse, dense);
   t->prog[1].host = compile_program(t->host, PM_GRADIENT, fuse, dense);
   t->prog[2].host = compile_program(t->host, PM_GRADIENT, fuse, false);
+  t->prog[3].host = compile_program(t->host, PM_KINK, fuse, dense);
+  for (uint32_t i = t->host.num_inputs; i < num_ops; ++i)
+    if (is_nonsmooth_op(t->host.opcodes[i])) t->nkinks++;
   (void)hipSetDevice(c->device);
-  for (int m = 0; m < 3; ++m) {
+  for (int m = 0; m < 4; ++m) {
     int32_t rc = upload_program(t->prog[m], t->f64);
     if (rc != EHIP_OK) {
       for (auto& p : t->prog) free_program(p);
@@ -290,6 +293,44 @@ int32_t ehip_taylor_kth_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d
   L.dirs = d_dirs;
   L.y = d_coeffs;
   rc = run(c, t, 2, (int)order, L, pick(c, stream));
+  return rc ? rc : ok();
+}
+
+// --------------------------- nonsmooth kink records ---------------------------
+
+uint32_t ehip_kink_count(const ehip_tape* t) { return t ? t->nkinks : 0; }
+
+int32_t ehip_kink_info(const ehip_tape* t, uint32_t* tape_index, uint32_t* opcode, uint32_t cap) {
+  if (!t || (t->nkinks && (!tape_index || !opcode))) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (cap < t->nkinks) return fail(EHIP_OTHER, "GPU error: kink buffers too small (need %u)", t->nkinks);
+  uint32_t k = 0;
+  for (uint32_t i = t->host.num_inputs; i < (uint32_t)t->host.opcodes.size(); ++i)
+    if (is_nonsmooth_op(t->host.opcodes[i])) {
+      tape_index[k] = i;
+      opcode[k] = t->host.opcodes[i];
+      ++k;
+    }
+  return ok();
+}
+
+int32_t ehip_forward_nonsmooth_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t batch, void* d_out,
+                                   void* d_switch, int8_t* d_branch, void* stream) {
+  if (!c || !t || !d_x || (t->nkinks && (!d_switch || !d_branch))) return fail(EHIP_OTHER, "GPU error: null argument");
+  int32_t rc = check_batch(t, batch);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  Launch L{};
+  L.nin = t->nin;
+  L.nout = t->nout;
+  L.B = batch;
+  L.x = d_x;
+  L.xstride = t->nin;
+  L.y = d_out;
+  L.ksw = d_switch;
+  L.kbr = d_branch;
+  L.nkinks = t->nkinks;
+  rc = run(c, t, 3, 1, L, pick(c, stream));
   return rc ? rc : ok();
 }
 
@@ -555,6 +596,29 @@ int32_t ehip_forward_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs, 
   TRY(ehip_forward_batch_dev(c, t, dx, batch, dy, nullptr));
   std::lock_guard<std::mutex> lk(c->mu);
   TRY(h.fetch(outputs, dy, (size_t)batch * t->nout * h.S));
+  TRY(h.sync());
+  return ok();
+}
+
+int32_t ehip_forward_nonsmooth(ehip_ctx* c, const ehip_tape* t, const void* inputs, uint32_t batch, void* outputs,
+                               void* switching, int8_t* branches) {
+  if (!c || !t || !inputs || !outputs || (t->nkinks && (!switching || !branches)))
+    return fail(EHIP_OTHER, "GPU error: null argument");
+  TRY(check_batch(t, batch));
+  std::lock_guard<std::mutex> hlk(c->host_mu);
+  HostCall h{c, t->f64 ? 8u : 4u};
+  void *dx, *dy, *ds, *db;
+  const size_t nk = (size_t)batch * t->nkinks;
+  TRY(h.stage(0, inputs, (size_t)batch * t->nin * h.S, &dx));
+  TRY(h.stage(1, nullptr, (size_t)batch * t->nout * h.S, &dy));
+  TRY(h.stage(2, nullptr, std::max<size_t>(nk * h.S, 8), &ds));
+  TRY(h.stage(3, nullptr, std::max<size_t>(nk, 8), &db));
+  TRY(ehip_forward_nonsmooth_dev(c, t, dx, batch, dy, ds, (int8_t*)db, nullptr));
+  TRY(h.fetch(outputs, dy, (size_t)batch * t->nout * h.S));
+  if (nk) {
+    TRY(h.fetch(switching, ds, nk * h.S));
+    TRY(h.fetch(branches, db, nk));
+  }
   TRY(h.sync());
   return ok();
 }

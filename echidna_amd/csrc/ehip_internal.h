@@ -109,7 +109,8 @@ struct ehip_tape {
   bool f64 = false;
   uint32_t nin = 0, nout = 0;
   TapeHost host;
-  DevProgram prog[3];  // forward, gradient (both with DENSE), HVP (dot-level)
+  DevProgram prog[4];  // forward, gradient (both with DENSE), HVP (dot-level), kinks
+  uint32_t nkinks = 0;
   Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
 };
 
@@ -143,10 +144,15 @@ inline int32_t check_batch(const ehip_tape* t, uint64_t batch) {
 }
 
 // Slot of `tuned` for an evaluation: 0 forward, 1 gradient, 2 hvp, 2+K taylor.
-// Program of an evaluation: scalar forward / gradient, dual (HVP), jet.
-inline int prog_index(int kind, bool grad) { return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : 0); }
+// Program of an evaluation: kind 0 scalar forward / gradient, 1 dual (HVP),
+// 2 jet, 3 scalar forward with kink records.
+inline int prog_index(int kind, bool grad) {
+  return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : (kind == 3 ? 3 : 0));
+}
 
-inline int tune_key(int kind, int K, bool grad) { return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : 2 + K); }
+inline int tune_key(int kind, int K, bool grad) {
+  return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : (kind == 3 ? 0 : 2 + K));
+}
 
 // Launch one interpreter over `L.B` lanes.  kind 0 scalar, 1 dual, 2 jet(K).
 inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s, const Tuned* force = nullptr) {
@@ -154,7 +160,7 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   L.prog = dp.d_stream;
   L.nwin = dp.nwin;
   L.pool = dp.d_pool;
-  const uint32_t NC = (kind == 0) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
+  const uint32_t NC = (kind == 0 || kind == 3) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
   const size_t S = t->f64 ? 8 : 4;
   const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
   const uint32_t wg = tu.wg ? tu.wg : c->wg;
@@ -176,7 +182,7 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
     L.gregs = nullptr;
   }
   hipError_t e;
-  if (kind == 0) e = launch_scalar(t->f64, L, wg, lds, s);
+  if (kind == 0 || kind == 3) e = launch_scalar(t->f64, L, wg, lds, s);
   else if (kind == 1) e = launch_dual(t->f64, L, wg, lds, s);
   else e = launch_jet(t->f64, K, L, wg, lds, s);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -19,6 +19,9 @@ struct Launch {
   void* g;                // gradients
   void* h;                // Hessian-vector products
   void* gregs;            // HBM register file for registers >= R_lds
+  void* ksw;              // kink switching values [B][nkinks] (KINK program)
+  int8_t* kbr;            // kink branches [B][nkinks]
+  uint32_t nkinks;
   uint32_t S;             // lanes per x row (0: one row per lane)
   uint32_t rng;           // RNG_NONE: read dirs; else generate (philox.h)
   uint64_t seed;

@@ -451,6 +451,39 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
           ++k;
         }
       }
+    } else if (op == I_KINK) {
+      // KinkEntry of forward_nonsmooth (src/bytecode_tape/forward.rs:87-187)
+      F a, b = F(0);
+      if (fl & F_A_IMM) a = I.imm(2); else R.rd(I.v(2), &a);
+      if (fl & F_HAS_B) { if (fl & F_B_IMM) b = I.imm(4); else R.rd(I.v(4), &b); }
+      const uint32_t kop = I.u(6);
+      F sw;
+      bool up;
+      if (kop == OP_ABS || kop == OP_SIGNUM) {
+        sw = a;
+        up = a >= F(0);
+      } else if (kop == OP_MAX) {
+        sw = a - b;
+        up = a >= b || b != b;
+      } else if (kop == OP_MIN) {
+        sw = a - b;
+        up = a <= b || b != b;
+      } else if (kop == OP_ROUND) {
+        const F sh = a + F(0.5);
+        sw = sh - round(sh);
+        up = a - floor(a) < F(0.5);
+      } else if (kop == OP_FRACT) {
+        sw = a - round(a);
+        up = a - trunc(a) >= F(0);
+      } else {  // floor, ceil, trunc
+        sw = a - round(a);
+        up = a - floor(a) < F(0.5);
+      }
+      if (act) {
+        const uint64_t o = p * L.nkinks + I.u(1);
+        reinterpret_cast<F*>(L.ksw)[o] = sw;
+        L.kbr[o] = up ? int8_t(1) : int8_t(-1);
+      }
     } else if (op == I_DENSE) {
       dense_fwd<F, 1, 16, false>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4));
     } else if (op == I_DENSE_REV) {

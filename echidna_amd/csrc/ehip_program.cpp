@@ -63,6 +63,7 @@ struct AIns {
   Opnd a, b;
   std::vector<ATerm> terms;
   bool full = false;             // DOT / DOT_REV covering its whole chain
+  uint32_t kop = 0;              // KINK: opcode of the nonsmooth entry
   std::vector<int64_t> members;  // DENSE: destinations; DENSE_REV: adjoints g_j
   std::vector<double> W;         // DENSE / DENSE_REV weights [member][term]
 };
@@ -284,6 +285,9 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
   std::vector<char> need(N, 0);
   for (uint32_t o : t.outputs)
     if (!is_const(o)) need[o] = 1;
+  if (mode == PM_KINK)
+    for (uint32_t i = ni; i < N; ++i)
+      if (is_nonsmooth_op(t.opcodes[i])) need[i] = 1;
   if (grad) {
     for (uint32_t i = ni; i < N; ++i) {
       if (!reach[i] || is_const(i)) continue;
@@ -523,6 +527,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
     for (auto& pc : pieces_at[(size_t)(pos + 1)]) emit_piece(pc.first, pc.second.first, pc.second.second);
   };
   flush_pieces(-1);
+  uint32_t nkink = 0;
   for (uint32_t i = ni; i < N; ++i) {
     if (!need[i] || is_const(i) || absorbed[i]) continue;
     if (split_root[i]) {
@@ -543,6 +548,20 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
       if (x.b.imm) x.flags |= F_B_IMM;
     }
     prog.push_back(std::move(x));
+    if (mode == PM_KINK && is_nonsmooth_op(op)) {
+      AIns k;
+      k.op = I_KINK;
+      k.aux = nkink++;
+      k.kop = op;
+      k.a = opnd(t.arg0[i]);
+      if (k.a.imm) k.flags |= F_A_IMM;
+      if (binop(i)) {
+        k.flags |= F_HAS_B;
+        k.b = opnd(t.arg1[i]);
+        if (k.b.imm) k.flags |= F_B_IMM;
+      }
+      prog.push_back(std::move(k));
+    }
     flush_pieces(i);
   }
   for (uint32_t j = 0; j < (uint32_t)t.outputs.size(); ++j) {
@@ -823,6 +842,16 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
         }
         break;
       }
+      case I_KINK:
+        w[1] = x.aux;
+        if (x.a.imm) put_imm(x.a.v, w[2], w[3]);
+        else w[2] = R(x.a.vid);
+        if (x.flags & F_HAS_B) {
+          if (x.b.imm) put_imm(x.b.v, w[4], w[5]);
+          else w[4] = R(x.b.vid);
+        }
+        w[6] = x.kop;
+        break;
       case I_DENSE:
       case I_DENSE_REV: {
         const size_t T = x.terms.size(), m = x.members.size();

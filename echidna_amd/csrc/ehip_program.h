@@ -38,6 +38,7 @@ namespace ehip {
 //                     into the side table below
 //   LOADN           : w2 count, w3 first side-table index; reg[] holds the
 //                     destination registers, imm[] the input indices
+//   KINK            : w1 kink index, w2/w3 a, w4/w5 b (Max/Min), w6 opcode
 //   DENSE           : w1 m members, w2 T terms, w3 pool offset of the term
 //                     words then the m destination registers, w4 pool offset
 //                     of the weights [T][m] (term-major); IW words per weight
@@ -66,6 +67,7 @@ enum : uint32_t {
   I_END = 72,    // end of the program
   I_DENSE = 73,
   I_DENSE_REV = 74,
+  I_KINK = 75,
   I_REV_BASE = 128,
 };
 
@@ -124,7 +126,11 @@ struct Program {
   uint32_t nterms = 0;          // T
 };
 
-enum ProgMode { PM_FORWARD = 0, PM_GRADIENT = 1 };
+// PM_KINK: forward values plus one KINK record per nonsmooth entry of the
+// whole tape (forward_nonsmooth, src/bytecode_tape/forward.rs:87-187).
+enum ProgMode { PM_FORWARD = 0, PM_GRADIENT = 1, PM_KINK = 2 };
+
+inline bool is_nonsmooth_op(uint32_t op) { return op == 10 || op == 11 || (op >= 36 && op <= 42); }
 
 // Windowed instruction stream executed by the interpreters.  The program is
 // cut into windows of WIN_WORDS u32 words that a workgroup stages in LDS (two

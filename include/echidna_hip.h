@@ -237,6 +237,56 @@ int32_t ehip_set_dense(ehip_ctx* ctx, int32_t enable);
 int32_t ehip_set_fusion(ehip_ctx* ctx, int32_t enable);
 
 /* ------------------------------------------------------------------------
+ * Structure for the sparse derivative paths (bit-exact with the reference)
+ * and the GPU sparse Jacobian / Hessian.  Pattern detection works on the
+ * flattened tape arrays (no device needed).  Two-call pattern for sizes:
+ * call with cap 0 to learn *nnz (the call then returns EHIP_OTHER), then
+ * again.
+ * ------------------------------------------------------------------------ */
+/* BytecodeTape::detect_sparsity (src/bytecode_tape/sparse.rs:42-52,
+ * 463-600): lower-triangle COO (rows >= cols), sorted and deduplicated. */
+int32_t ehip_hessian_sparsity(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1, uint32_t num_ops,
+                              uint32_t num_inputs, uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz);
+/* BytecodeTape::detect_jacobian_sparsity (src/bytecode_tape/sparse.rs:54-67,
+ * 733-776): (output, input) pairs, outputs in order, inputs ascending. */
+int32_t ehip_jacobian_sparsity(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1, uint32_t num_ops,
+                               uint32_t num_inputs, const uint32_t* output_indices, uint32_t num_outputs,
+                               uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz);
+/* greedy_coloring (src/sparse.rs:57-98): distance-2 coloring of a symmetric
+ * pattern; column_coloring / row_coloring (src/sparse.rs:268-300) of a
+ * Jacobian pattern.  Decreasing-degree stable order, lowest free color. */
+int32_t ehip_greedy_coloring(uint32_t dim, const uint32_t* rows, const uint32_t* cols, uint64_t nnz,
+                             uint32_t* colors, uint32_t* num_colors);
+int32_t ehip_column_coloring(uint32_t num_outputs, uint32_t num_inputs, const uint32_t* rows, const uint32_t* cols,
+                             uint64_t nnz, uint32_t* colors, uint32_t* num_colors);
+int32_t ehip_row_coloring(uint32_t num_outputs, uint32_t num_inputs, const uint32_t* rows, const uint32_t* cols,
+                          uint64_t nnz, uint32_t* colors, uint32_t* num_colors);
+/* GpuBackend::sparse_hessian (src/gpu/mod.rs:158-172, cuda_backend.rs:
+ * 397-441): value, gradient [n], pattern and values at x (host, n). */
+int32_t ehip_sparse_hessian(ehip_ctx* ctx, const ehip_tape* tape, const void* x, void* value, void* gradient,
+                            uint32_t* rows, uint32_t* cols, void* values, uint64_t cap, uint64_t* nnz);
+/* GpuBackend::sparse_jacobian (src/gpu/mod.rs:138-148, cuda_backend.rs:
+ * 306-395): outputs [m], pattern and values at x (host, n). */
+int32_t ehip_sparse_jacobian(ehip_ctx* ctx, const ehip_tape* tape, const void* x, void* outputs, uint32_t* rows,
+                             uint32_t* cols, void* values, uint64_t cap, uint64_t* nnz);
+
+/* ------------------------------------------------------------------------
+ * Nonsmooth kink records per point (forward_nonsmooth,
+ * src/bytecode_tape/forward.rs:87-187): one record per Abs / Min / Max /
+ * Signum / Floor / Ceil / Round / Trunc / Fract entry of the whole tape, in
+ * tape order.  The entries are structural (ehip_kink_info); the switching
+ * values and branches (+1 / -1) are per point.
+ * ------------------------------------------------------------------------ */
+uint32_t ehip_kink_count(const ehip_tape* tape);
+int32_t ehip_kink_info(const ehip_tape* tape, uint32_t* tape_index, uint32_t* opcode, uint32_t cap);
+/* Forward values [batch][m] plus switching [batch][K] and branch [batch][K]
+ * (K = ehip_kink_count); device buffers / host buffers. */
+int32_t ehip_forward_nonsmooth_dev(ehip_ctx* ctx, const ehip_tape* tape, const void* d_x, uint64_t batch,
+                                   void* d_outputs, void* d_switching, int8_t* d_branches, void* stream);
+int32_t ehip_forward_nonsmooth(ehip_ctx* ctx, const ehip_tape* tape, const void* inputs, uint32_t batch,
+                               void* outputs, void* switching, int8_t* branches);
+
+/* ------------------------------------------------------------------------
  * Learned low-rank control variate for STDE.  No counterpart in the
  * reference; it generalises laplacian_with_control (src/stde/laplacian.rs:
  * 90-129, src/gpu/stde_gpu.rs:135-200) from a fixed diagonal to

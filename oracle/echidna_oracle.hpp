@@ -29,6 +29,7 @@
 // tests and closed forms only.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1211,5 +1212,240 @@ template <class F> struct Welford {
     }
   }
 };
+
+// ---------------------------------------------------------------------------
+// Structure: sparsity patterns and colorings
+// (src/bytecode_tape/sparse.rs:463-776, src/sparse.rs:57-413).
+// ---------------------------------------------------------------------------
+inline bool sp_binary(uint32_t op) {  // is_binary_op, sparse.rs:702-718
+  return op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV || op == OP_REM || op == OP_POWF ||
+         op == OP_ATAN2 || op == OP_HYPOT || op == OP_MAX || op == OP_MIN;
+}
+
+// 0 linear, 1 unary nonlinear, 2 binary nonlinear, 3 zero derivative
+inline int sp_class(uint32_t op) {  // classify_op, sparse.rs:643-700
+  switch (op) {
+    case OP_ADD: case OP_SUB: case OP_NEG: case OP_FRACT: return 0;
+    case OP_MUL: case OP_DIV: case OP_POWF: case OP_ATAN2: case OP_HYPOT: return 2;
+    case OP_ABS: case OP_SIGNUM: case OP_FLOOR: case OP_CEIL: case OP_ROUND: case OP_TRUNC:
+    case OP_MAX: case OP_MIN: case OP_REM: return 3;
+    default: return 1;
+  }
+}
+
+struct DepSets {
+  uint32_t n = 0, words = 0;
+  std::vector<std::vector<uint64_t>> d;
+  std::vector<uint32_t> bits(uint32_t i) const {  // extract_bits
+    std::vector<uint32_t> r;
+    for (uint32_t w = 0; w < words; ++w)
+      for (uint32_t b = 0; b < 64; ++b)
+        if ((d[i][w] >> b) & 1u) {
+          uint32_t pos = w * 64 + b;
+          if (pos < n) r.push_back(pos);
+        }
+    return r;
+  }
+  void unite(uint32_t dst, uint32_t src) {  // union_into
+    if (dst == src) return;
+    for (uint32_t w = 0; w < words; ++w) d[dst][w] |= d[src][w];
+  }
+};
+
+// detect_sparsity_impl / detect_jacobian_sparsity_impl; `pairs` null for the
+// Jacobian walk (every op propagates).
+inline DepSets sp_walk(const TapeView& t, std::vector<std::pair<uint32_t, uint32_t>>* pairs) {
+  DepSets D;
+  D.n = t.num_inputs;
+  D.words = (t.num_inputs + 63) / 64;
+  D.d.assign(t.num_ops, std::vector<uint64_t>(D.words, 0));
+  auto push_upper = [&](const std::vector<uint32_t>& b) {
+    for (size_t ii = 0; ii < b.size(); ++ii)
+      for (size_t jj = 0; jj <= ii; ++jj)
+        pairs->push_back(b[ii] >= b[jj] ? std::make_pair(b[ii], b[jj]) : std::make_pair(b[jj], b[ii]));
+  };
+  uint32_t input_idx = 0;
+  for (uint32_t i = 0; i < t.num_ops; ++i) {
+    uint32_t op = t.opcodes[i];
+    if (op == OP_INPUT) {
+      D.d[i][input_idx / 64] |= 1ull << (input_idx % 64);
+      ++input_idx;
+      continue;
+    }
+    if (op == OP_CONST) continue;
+    uint32_t a = t.arg0[i], b = t.arg1[i];
+    bool bin = sp_binary(op) && b != UNUSED;
+    if (!pairs) {
+      D.unite(i, a);
+      if (bin) D.unite(i, b);
+      continue;
+    }
+    int cls = sp_class(op);
+    if (cls == 0 || cls == 3) {
+      D.unite(i, a);
+      if (bin) D.unite(i, b);
+    } else if (cls == 1) {
+      D.unite(i, a);
+      push_upper(D.bits(i));
+    } else {
+      std::vector<uint32_t> ba = D.bits(a), bb = D.bits(b);
+      D.unite(i, a);
+      D.unite(i, b);
+      for (uint32_t va : ba)
+        for (uint32_t vb : bb) pairs->push_back(va >= vb ? std::make_pair(va, vb) : std::make_pair(vb, va));
+      if (op != OP_MUL) {
+        push_upper(ba);
+        push_upper(bb);
+      }
+    }
+  }
+  return D;
+}
+
+inline void hessian_pattern(const TapeView& t, std::vector<uint32_t>& rows, std::vector<uint32_t>& cols) {
+  std::vector<std::pair<uint32_t, uint32_t>> p;
+  sp_walk(t, &p);
+  std::sort(p.begin(), p.end());
+  p.erase(std::unique(p.begin(), p.end()), p.end());
+  rows.clear();
+  cols.clear();
+  for (auto& e : p) { rows.push_back(e.first); cols.push_back(e.second); }
+}
+
+inline void jacobian_pattern(const TapeView& t, std::vector<uint32_t>& rows, std::vector<uint32_t>& cols) {
+  DepSets D = sp_walk(t, nullptr);
+  rows.clear();
+  cols.clear();
+  for (uint32_t o = 0; o < t.num_outputs; ++o)
+    for (uint32_t c : D.bits(t.output_indices[o])) { rows.push_back(o); cols.push_back(c); }
+}
+
+// greedy_distance1_coloring (sparse.rs:357-413), with its u64 bitset fast path
+// and the sorted-list fallback for colors >= 64.
+inline uint32_t distance1_coloring(const std::vector<std::vector<uint32_t>>& adj, std::vector<uint32_t>& colors) {
+  const size_t n = adj.size();
+  std::vector<size_t> order(n);
+  for (size_t v = 0; v < n; ++v) order[v] = v;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return adj[b].size() < adj[a].size(); });
+  colors.assign(n, 0xFFFFFFFFu);
+  uint32_t num = 0;
+  for (size_t v : order) {
+    uint64_t used = 0;
+    bool fallback = false;
+    for (uint32_t u : adj[v]) {
+      uint32_t c = colors[u];
+      if (c == 0xFFFFFFFFu) continue;
+      if (c < 64) used |= 1ull << c;
+      else fallback = true;
+    }
+    uint32_t color;
+    if (!fallback) {
+      uint64_t freeb = ~used;
+      color = freeb ? (uint32_t)__builtin_ctzll(freeb) : 64u;
+    } else {
+      std::vector<uint32_t> uv;
+      for (uint32_t u : adj[v])
+        if (colors[u] != 0xFFFFFFFFu) uv.push_back(colors[u]);
+      std::sort(uv.begin(), uv.end());
+      uv.erase(std::unique(uv.begin(), uv.end()), uv.end());
+      color = 0;
+      for (uint32_t u : uv) {
+        if (u != color) break;
+        ++color;
+      }
+    }
+    colors[v] = color;
+    if (color + 1 > num) num = color + 1;
+  }
+  return num;
+}
+
+inline uint32_t greedy_coloring(uint32_t n, const std::vector<uint32_t>& rows, const std::vector<uint32_t>& cols,
+                                std::vector<uint32_t>& colors) {
+  colors.clear();
+  if (n == 0) return 0;
+  std::vector<std::vector<uint32_t>> adj(n), adj2(n);
+  for (size_t k = 0; k < rows.size(); ++k)
+    if (rows[k] != cols[k]) { adj[rows[k]].push_back(cols[k]); adj[cols[k]].push_back(rows[k]); }
+  for (uint32_t v = 0; v < n; ++v) {
+    for (uint32_t u : adj[v]) adj2[v].push_back(u);
+    for (uint32_t u : adj[v])
+      for (uint32_t w : adj[u])
+        if (w != v) adj2[v].push_back(w);
+    std::sort(adj2[v].begin(), adj2[v].end());
+    adj2[v].erase(std::unique(adj2[v].begin(), adj2[v].end()), adj2[v].end());
+  }
+  return distance1_coloring(adj2, colors);
+}
+
+inline uint32_t intersection_coloring(const std::vector<uint32_t>& gk, const std::vector<uint32_t>& ck,
+                                      uint32_t gdim, uint32_t cdim, std::vector<uint32_t>& colors) {
+  colors.clear();
+  if (cdim == 0) return 0;
+  std::vector<std::vector<uint32_t>> groups(gdim), adj(cdim);
+  for (size_t k = 0; k < gk.size(); ++k) groups[gk[k]].push_back(ck[k]);
+  for (auto& m : groups)
+    for (size_t i = 0; i < m.size(); ++i)
+      for (size_t j = i + 1; j < m.size(); ++j) { adj[m[i]].push_back(m[j]); adj[m[j]].push_back(m[i]); }
+  for (auto& l : adj) {
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+  }
+  return distance1_coloring(adj, colors);
+}
+
+// ---------------------------------------------------------------------------
+// forward_nonsmooth (src/bytecode_tape/forward.rs:87-187): one record per
+// nonsmooth entry of the whole tape, in tape order.
+// ---------------------------------------------------------------------------
+inline bool is_nonsmooth(uint32_t op) {
+  return op == OP_ABS || op == OP_MIN || op == OP_MAX || op == OP_SIGNUM || op == OP_FLOOR || op == OP_CEIL ||
+         op == OP_ROUND || op == OP_TRUNC || op == OP_FRACT;
+}
+
+template <class F> inline void forward_nonsmooth(const TapeView& t, const F* x, std::vector<F>& v, std::vector<F>& sw,
+                                                 std::vector<int8_t>& br) {
+  forward_into<F>(t, x, v);
+  sw.clear();
+  br.clear();
+  const F half = F(0.5);
+  for (uint32_t i = 0; i < t.num_ops; ++i) {
+    uint32_t op = t.opcodes[i];
+    if (!is_nonsmooth(op)) continue;
+    F a = v[t.arg0[i]];
+    switch (op) {
+      case OP_ABS:
+      case OP_SIGNUM: sw.push_back(a); br.push_back(a >= F(0) ? 1 : -1); break;
+      case OP_MAX: {
+        F b = v[t.arg1[i]];
+        sw.push_back(a - b);
+        br.push_back((a >= b || b != b) ? 1 : -1);
+        break;
+      }
+      case OP_MIN: {
+        F b = v[t.arg1[i]];
+        sw.push_back(a - b);
+        br.push_back((a <= b || b != b) ? 1 : -1);
+        break;
+      }
+      case OP_FLOOR:
+      case OP_CEIL:
+      case OP_TRUNC:
+        sw.push_back(a - std::round(a));
+        br.push_back(a - std::floor(a) < half ? 1 : -1);
+        break;
+      case OP_FRACT:
+        sw.push_back(a - std::round(a));
+        br.push_back(a - std::trunc(a) >= F(0) ? 1 : -1);
+        break;
+      case OP_ROUND: {
+        F sh = a + half;
+        sw.push_back(sh - std::round(sh));
+        br.push_back(a - std::floor(a) < half ? 1 : -1);
+        break;
+      }
+    }
+  }
+}
 
 }  // namespace orc

@@ -224,4 +224,70 @@ int orc_hessian_f64(TAPE_ARGS, const double* x, double* H) {
   return 0;
 }
 
+
+// Structure.  Sizes first: call with cap 0 to get *nnz.
+int orc_hessian_pattern(TAPE_ARGS, uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz) {
+  std::vector<uint32_t> r, c;
+  hessian_pattern(TAPE, r, c);
+  *nnz = r.size();
+  if (r.size() > cap) return 1;
+  std::copy(r.begin(), r.end(), rows);
+  std::copy(c.begin(), c.end(), cols);
+  return 0;
+}
+
+int orc_jacobian_pattern(TAPE_ARGS, uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz) {
+  std::vector<uint32_t> r, c;
+  jacobian_pattern(TAPE, r, c);
+  *nnz = r.size();
+  if (r.size() > cap) return 1;
+  std::copy(r.begin(), r.end(), rows);
+  std::copy(c.begin(), c.end(), cols);
+  return 0;
+}
+
+uint32_t orc_greedy_coloring(uint32_t n, const uint32_t* rows, const uint32_t* cols, uint64_t nnz, uint32_t* colors) {
+  std::vector<uint32_t> r(rows, rows + nnz), c(cols, cols + nnz), out;
+  uint32_t k = greedy_coloring(n, r, c, out);
+  std::copy(out.begin(), out.end(), colors);
+  return k;
+}
+
+uint32_t orc_intersection_coloring(const uint32_t* gk, const uint32_t* ck, uint64_t nnz, uint32_t gdim, uint32_t cdim,
+                                   uint32_t* colors) {
+  std::vector<uint32_t> g(gk, gk + nnz), c(ck, ck + nnz), out;
+  uint32_t k = intersection_coloring(g, c, gdim, cdim, out);
+  std::copy(out.begin(), out.end(), colors);
+  return k;
+}
+
+// Kinks of B points: values [B][nout], switching [B][K], branch [B][K].
+int orc_nonsmooth_f64(TAPE_ARGS, const double* x, uint64_t B, uint32_t K, double* y, double* sw, int8_t* br) {
+  TapeView t = TAPE;
+  std::vector<double> v, s;
+  std::vector<int8_t> b;
+  for (uint64_t p = 0; p < B; ++p) {
+    forward_nonsmooth<double>(t, x + p * t.num_inputs, v, s, b);
+    if (s.size() != K) return 1;
+    for (uint32_t j = 0; j < t.num_outputs; ++j) y[p * t.num_outputs + j] = v[t.output_indices[j]];
+    std::copy(s.begin(), s.end(), sw + p * K);
+    std::copy(b.begin(), b.end(), br + p * K);
+  }
+  return 0;
+}
+
+int orc_nonsmooth_f32(TAPE_ARGS, const float* x, uint64_t B, uint32_t K, float* y, float* sw, int8_t* br) {
+  TapeView t = TAPE;
+  std::vector<float> v, s;
+  std::vector<int8_t> b;
+  for (uint64_t p = 0; p < B; ++p) {
+    forward_nonsmooth<float>(t, x + p * t.num_inputs, v, s, b);
+    if (s.size() != K) return 1;
+    for (uint32_t j = 0; j < t.num_outputs; ++j) y[p * t.num_outputs + j] = v[t.output_indices[j]];
+    std::copy(s.begin(), s.end(), sw + p * K);
+    std::copy(b.begin(), b.end(), br + p * K);
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -173,3 +173,75 @@ def hessian(tape, x):
     H = np.zeros((t.num_inputs, t.num_inputs))
     lib().orc_hessian_f64(*t.args(), _ptr(x, f64p), _ptr(H, f64p))
     return H
+
+
+# ---------------------------------------------------------------------------
+# Structure and kinks (src/bytecode_tape/sparse.rs, src/sparse.rs,
+# src/bytecode_tape/forward.rs:87-187)
+# ---------------------------------------------------------------------------
+def _pattern(fn, tape):
+    t = OracleTape(tape)
+    nnz = ctypes.c_uint64(0)
+    fn(*t.args(), None, None, ctypes.c_uint64(0), ctypes.byref(nnz))
+    n = int(nnz.value)
+    rows = np.zeros(max(n, 1), np.uint32)
+    cols = np.zeros(max(n, 1), np.uint32)
+    fn(*t.args(), _ptr(rows, u32p), _ptr(cols, u32p), ctypes.c_uint64(n), ctypes.byref(nnz))
+    return rows[:n].copy(), cols[:n].copy()
+
+
+def hessian_pattern(tape):
+    """detect_sparsity: (rows, cols) lower triangle, sorted, deduplicated."""
+    return _pattern(lib().orc_hessian_pattern, tape)
+
+
+def jacobian_pattern(tape):
+    """detect_jacobian_sparsity: (output rows, input cols)."""
+    return _pattern(lib().orc_jacobian_pattern, tape)
+
+
+def greedy_coloring(n, rows, cols):
+    rows = np.ascontiguousarray(rows, np.uint32)
+    cols = np.ascontiguousarray(cols, np.uint32)
+    colors = np.zeros(max(n, 1), np.uint32)
+    f = lib().orc_greedy_coloring
+    f.restype = ctypes.c_uint32
+    k = f(ctypes.c_uint32(n), _ptr(rows, u32p), _ptr(cols, u32p), ctypes.c_uint64(len(rows)), _ptr(colors, u32p))
+    return colors[:n].copy(), int(k)
+
+
+def _intersection(gk, ck, gdim, cdim):
+    gk = np.ascontiguousarray(gk, np.uint32)
+    ck = np.ascontiguousarray(ck, np.uint32)
+    colors = np.zeros(max(cdim, 1), np.uint32)
+    f = lib().orc_intersection_coloring
+    f.restype = ctypes.c_uint32
+    k = f(_ptr(gk, u32p), _ptr(ck, u32p), ctypes.c_uint64(len(gk)), ctypes.c_uint32(gdim), ctypes.c_uint32(cdim),
+          _ptr(colors, u32p))
+    return colors[:cdim].copy(), int(k)
+
+
+def column_coloring(num_outputs, num_inputs, rows, cols):
+    return _intersection(rows, cols, num_outputs, num_inputs)
+
+
+def row_coloring(num_outputs, num_inputs, rows, cols):
+    return _intersection(cols, rows, num_inputs, num_outputs)
+
+
+def nonsmooth(tape, X, dtype=np.float64):
+    """forward_nonsmooth per point: (outputs [B,m], switching [B,K], branch [B,K])."""
+    t = OracleTape(tape)
+    sfx, p = _dt(dtype)
+    X = np.ascontiguousarray(X, dtype=dtype).reshape(-1, t.num_inputs)
+    B = X.shape[0]
+    nonsmooth_ops = {10, 11, 36, 37, 38, 39, 40, 41, 42}
+    K = int(sum(1 for o in t.opcodes[t.num_inputs:] if int(o) in nonsmooth_ops))
+    y = np.zeros((B, t.num_outputs), dtype)
+    sw = np.zeros((B, max(K, 1)), dtype)
+    br = np.zeros((B, max(K, 1)), np.int8)
+    rc = getattr(lib(), "orc_nonsmooth_" + sfx)(*t.args(), _ptr(X, p), ctypes.c_uint64(B), ctypes.c_uint32(K),
+                                                 _ptr(y, p), _ptr(sw, p), _ptr(br, ctypes.POINTER(ctypes.c_int8)))
+    if rc != 0:
+        raise ValueError("kink count mismatch")
+    return y, sw[:, :K], br[:, :K]
