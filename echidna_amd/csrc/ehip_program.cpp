@@ -2,6 +2,7 @@
 #include "ehip_program.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -68,7 +69,8 @@ struct AIns {
   std::vector<double> W;         // DENSE / DENSE_REV weights [member][term]
 };
 
-constexpr size_t DENSE_MIN = 4;  // members worth a DENSE instruction
+constexpr size_t DENSE_MIN = 4;     // members worth a DENSE instruction
+constexpr size_t DENSE_CHUNK = 16;  // members per forward DENSE
 
 // Replace groups of full DOTs (and their DOT_REVs) that share one term
 // structure by DENSE (DENSE_REV) instructions.  A forward group moves to its
@@ -76,7 +78,7 @@ constexpr size_t DENSE_MIN = 4;  // members worth a DENSE instruction
 // before that point.  A reverse group moves to its last member's position
 // when no instruction in between touches the operand adjoints, so each
 // adjoint receives the same contributions in the same order.
-void group_dense(std::vector<AIns>& prog, int64_t adj_offset) {
+void group_dense(std::vector<AIns>& prog, int64_t adj_offset, size_t chunk) {
   struct Key {
     std::vector<int64_t> k;
     bool operator<(const Key& o) const { return k < o.k; }
@@ -118,17 +120,27 @@ void group_dense(std::vector<AIns>& prog, int64_t adj_offset) {
     if (mem.size() < DENSE_MIN) continue;
     const AIns& x0 = prog[mem[0]];
     const size_t T = x0.terms.size();
-    AIns d;
-    d.op = I_DENSE;
-    d.terms = x0.terms;
-    for (size_t j = 0; j < mem.size(); ++j) {
-      const AIns& x = prog[mem[j]];
-      d.members.push_back(x.dst);
-      for (size_t q = 0; q < T; ++q) d.W.push_back(x.terms[q].imm);
-      if (j > 0) drop[mem[j]] = 1;
+    // Forward chunks of `chunk` members, each at its first member's position:
+    // the kernels block members by at most 16 anyway, and a chunk keeps at
+    // most `chunk` new values live at once (forward programs; a gradient
+    // program keeps the layer's values for the reverse sweep regardless).
+    for (size_t c0 = 0; c0 < mem.size(); c0 += chunk) {
+      const size_t c1 = std::min(mem.size(), c0 + chunk);
+      AIns d;
+      d.op = I_DENSE;
+      d.terms = x0.terms;
+      for (size_t j = c0; j < c1; ++j) {
+        const AIns& x = prog[mem[j]];
+        d.members.push_back(x.dst);
+        for (size_t q = 0; q < T; ++q) d.W.push_back(x.terms[q].imm);
+        drop[mem[j]] = 1;
+      }
+      if (c1 - c0 == 1) {  // a lone member stays a DOT
+        drop[mem[c0]] = 0;
+        continue;
+      }
+      place.push_back({mem[c0], std::move(d)});
     }
-    drop[mem[0]] = 1;
-    place.push_back({mem[0], std::move(d)});
     // reverse: every member has exactly one DOT_REV, the whole chain
     std::vector<size_t> rp;
     bool rev_ok = true;
@@ -698,7 +710,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
     }
   }
 
-  if (fuse && dense) group_dense(prog, (int64_t)N);
+  if (fuse && dense) group_dense(prog, (int64_t)N, grad ? SIZE_MAX : DENSE_CHUNK);
 
   // ---- live ranges ----
   const size_t NV = 2 * (size_t)N;
