@@ -335,6 +335,17 @@ def main() -> int:
     wall = time.perf_counter() - t0
     kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
 
+    # the same steps replayed from one captured hipGraph (reported beside)
+    from echidna_amd.backend import TUNE_GRADIENT as _TG
+    graph = ctx.graph_capture(bufs, _TG, B, dx, d_y=dy, d_g=dg, repeats=args.steps)
+    graph.launch()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    graph.launch()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t1) * 1e3 / args.steps
+    graph.close()
+
     S = 8 if f64 else 4
     bytes_per_point = S * (6 * counts["n"] + 5 * counts["N"] + counts["m"])
     achieved = bytes_per_point * B / (kernel_ms * 1e-3)
@@ -362,6 +373,7 @@ def main() -> int:
                    "points": B, "inputs": n, "hidden": H, "computed_entries": counts["N"],
                    "program": bufs.program_info(1), "parallelism": "none (1 GPU)",
                    "launch": tuned or {"wg": args.wg, "lds_budget": args.lds}},
+        "graph_ms_per_step": graph_ms,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel_ms": kernel_ms, "algorithmic_bytes_per_point": bytes_per_point},

@@ -107,6 +107,10 @@ _SIGS = {
     "ehip_forward_nonsmooth_dev": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
     "ehip_forward_nonsmooth": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "ehip_multi_batch_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _vp]),
+    "ehip_graph_capture": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
+                                            ctypes.c_uint64, _vp, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "ehip_graph_launch": (ctypes.c_int32, [_vp, _vp]),
+    "ehip_graph_destroy": (None, [_vp]),
     "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
     "ehip_set_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
@@ -252,6 +256,31 @@ class HipTapeBuffers:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class HipGraph:
+    """A captured hipGraph of repeated evaluations (``ehip_graph``)."""
+
+    def __init__(self, ctx: "HipContext", handle, tape=None):
+        self.ctx = ctx
+        self.handle = handle
+        self.tape = tape  # the recorded launches read the tape's program
+
+    def launch(self, stream: int = 0) -> None:
+        _check(load_library().ehip_graph_launch(self.handle, _vp(stream or None)))
+
+    def close(self) -> None:
+        if self.handle:
+            if self.ctx.handle:
+                load_library().ehip_graph_destroy(self.handle)
+            self.handle = None
+            self.ctx._children.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 
@@ -405,6 +434,17 @@ class HipContext:
         _check(load_library().ehip_autotune_dev(self.handle, tape.handle, what, order, _dp(d_x), xs, _dp(d_dirs),
                                                 batch, _dp(d_y), _dp(d_g), _dp(d_h), out))
         return {"wg": int(out[0]), "lds_budget": int(out[1]), "us": int(out[2])}
+
+    def graph_capture(self, tape: "HipTapeBuffers", what: int, batch: int, d_x, d_y=0, d_g=0, d_h=0, d_dirs=0,
+                      x_stride: Optional[int] = None, order: int = 0, repeats: int = 1) -> "HipGraph":
+        """Record ``repeats`` launches of ``what`` (TUNE_*) into a hipGraph."""
+        h = _vp()
+        xs = tape.num_inputs if x_stride is None else x_stride
+        _check(load_library().ehip_graph_capture(self.handle, tape.handle, what, order, _dp(d_x), xs, _dp(d_dirs),
+                                                 batch, _dp(d_y), _dp(d_g), _dp(d_h), repeats, ctypes.byref(h)))
+        g = HipGraph(self, h, tape)
+        self._children.add(g)
+        return g
 
     @staticmethod
     def set_tuned(tape: "HipTapeBuffers", what: int, wg: int, lds_budget: int, order: int = 0) -> None:

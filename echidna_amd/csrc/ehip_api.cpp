@@ -546,6 +546,77 @@ int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_
   return ok();
 }
 
+// ------------------------------- hipGraph capture -------------------------------
+
+struct ehip_graph {
+  ehip_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevBuf regs;  // the graph's own HBM register file
+};
+
+int32_t ehip_graph_capture(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t order, const void* d_x,
+                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
+                           uint32_t repeats, ehip_graph** out) {
+  if (!c || !t || !d_x || !out) return fail(EHIP_OTHER, "GPU error: null argument");
+  *out = nullptr;
+  if (repeats == 0) return fail(EHIP_OTHER, "GPU error: repeats must be non-zero");
+  int32_t rc = check_batch(t, batch);
+  if (rc) return rc;
+  Launch L;
+  int kind = 0, K = 1;
+  rc = tune_launch(t, what, order, d_x, x_stride, d_dirs, batch, d_y, d_g, d_h, L, kind, K);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  ehip_graph* gr = new ehip_graph();
+  gr->ctx = c;
+  // one eager launch sizes the graph's register file (no allocation may
+  // happen while the stream is being captured)
+  rc = run(c, t, kind, K, L, c->stream, nullptr, &gr->regs);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipStreamSynchronize(c->stream);
+  if (!rc && e == hipSuccess) e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (!rc && e == hipSuccess) {
+    for (uint32_t k = 0; k < repeats && !rc; ++k) rc = run(c, t, kind, K, L, c->stream, nullptr, &gr->regs);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    gr->graph = g;
+    if (e2 != hipSuccess && !rc) e = e2;
+  }
+  if (!rc && e == hipSuccess) e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+  if (rc || e != hipSuccess) {
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    gr->regs.release();
+    delete gr;
+    return rc ? rc : hip_fail(e, "graph capture");
+  }
+  *out = gr;
+  return ok();
+}
+
+int32_t ehip_graph_launch(ehip_graph* gr, void* stream) {
+  if (!gr) return fail(EHIP_OTHER, "GPU error: null argument");
+  std::lock_guard<std::mutex> lk(gr->ctx->mu);
+  (void)hipSetDevice(gr->ctx->device);
+  HIP_TRY(hipGraphLaunch(gr->exec, pick(gr->ctx, stream)), "graph launch");
+  return ok();
+}
+
+void ehip_graph_destroy(ehip_graph* gr) {
+  if (!gr) return;
+  {
+    std::lock_guard<std::mutex> lk(gr->ctx->mu);
+    (void)hipSetDevice(gr->ctx->device);
+    (void)hipDeviceSynchronize();
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    gr->regs.release();
+  }
+  delete gr;
+}
+
 // ----------------------------- host-pointer API -----------------------------
 
 namespace {

@@ -155,7 +155,11 @@ inline int tune_key(int kind, int K, bool grad) {
 }
 
 // Launch one interpreter over `L.B` lanes.  kind 0 scalar, 1 dual, 2 jet(K).
-inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s, const Tuned* force = nullptr) {
+// `ws` (optional): a caller-owned HBM register file used instead of the
+// context's grow-only one (graphs keep their own so a later reallocation
+// cannot pull the memory from under a recorded launch).
+inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s, const Tuned* force = nullptr,
+                   DevBuf* ws = nullptr) {
   const DevProgram& dp = t->prog[prog_index(kind, L.g != nullptr)];
   L.prog = dp.d_stream;
   L.nwin = dp.nwin;
@@ -176,8 +180,9 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   uint64_t lanes = grid * wg;
   if (R > R_lds) {
     size_t bytes = (size_t)(R - R_lds) * NC * lanes * S;
-    HIP_TRY(c->regs.reserve(bytes), "register file");
-    L.gregs = c->regs.p;
+    DevBuf& rf = ws ? *ws : c->regs;
+    HIP_TRY(rf.reserve(bytes), "register file");
+    L.gregs = rf.p;
   } else {
     L.gregs = nullptr;
   }

@@ -222,6 +222,19 @@ int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
                           uint32_t* out3);
 
+/* hipGraph capture of repeated evaluations (north star; no counterpart in
+ * the reference).  Records `repeats` launches of evaluation `what` (as for
+ * ehip_autotune_dev) over fixed device buffers into one graph with its own
+ * register workspace; ehip_graph_launch replays it on `stream` (null: the
+ * context stream).  The recorded launches contain no memset nodes.  A graph
+ * must be destroyed before its tape and context. */
+typedef struct ehip_graph ehip_graph;
+int32_t ehip_graph_capture(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t order, const void* d_x,
+                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
+                           uint32_t repeats, ehip_graph** out);
+int32_t ehip_graph_launch(ehip_graph* graph, void* stream);
+void ehip_graph_destroy(ehip_graph* graph);
+
 /* Restore (wg != 0) or clear (wg == 0) a tuned launch shape, e.g. from a
  * persisted autotuner cache; ehip_get_tuned reads it back (out2 = {wg, lds}). */
 int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget);
