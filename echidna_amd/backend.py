@@ -115,6 +115,13 @@ _SIGS = {
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
     "ehip_set_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     "ehip_get_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
+    "ehip_tape_to_json": (ctypes.c_int32, [ctypes.c_int32, _vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.c_uint32, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _c_u64p]),
+    "ehip_tape_from_json": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int32, _vp, _vp, _vp, _vp,
+                                             ctypes.c_uint64, _vp, ctypes.c_uint64, _c_u64p, _c_u32p, _c_u32p,
+                                             _c_u64p]),
+    "ehip_tape_upload_json": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_uint64,
+                                               ctypes.POINTER(_vp)]),
 }
 
 class EhipJob(ctypes.Structure):
@@ -342,6 +349,14 @@ class HipContext:
         return HipTapeBuffers(self, h, f64, int(lib.ehip_num_inputs(h)), int(lib.ehip_num_outputs(h)))
 
     upload_tape_f64 = upload_tape
+
+    def upload_tape_json(self, text, f64: bool = True) -> HipTapeBuffers:
+        """``serde_json::from_str::<BytecodeTape<F>>`` then ``upload_tape``: one C call."""
+        raw = text.encode() if isinstance(text, str) else bytes(text)
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.ehip_tape_upload_json(self.handle, EHIP_F64 if f64 else EHIP_F32, raw, len(raw), ctypes.byref(h)))
+        return HipTapeBuffers(self, h, f64, int(lib.ehip_num_inputs(h)), int(lib.ehip_num_outputs(h)))
 
     def num_outputs(self, tape: HipTapeBuffers) -> int:
         return int(load_library().ehip_num_outputs(tape.handle))

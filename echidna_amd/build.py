@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libechidna_hip.so")
 
 SOURCES = ["ehip_api.cpp", "ehip_program.cpp", "ehip_kernels.hip", "ehip_stde.hip", "ehip_cv.hip",
-           "ehip_cv_api.cpp", "ehip_sparse.cpp"]
+           "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # Reference arithmetic never contracts a*b+c into an FMA.

@@ -300,6 +300,35 @@ int32_t ehip_forward_nonsmooth(ehip_ctx* ctx, const ehip_tape* tape, const void*
                                void* outputs, void* switching, int8_t* branches);
 
 /* ------------------------------------------------------------------------
+ * Tape wire format: the serde JSON form of BytecodeTape
+ * (src/bytecode_tape/serde_support.rs:11-100), host only (no device).
+ * ------------------------------------------------------------------------ */
+/* Serialize (serde_support.rs:11-31): serde_json's compact layout of
+ * opcodes (variant names), arg_indices, values (non-finite -> null),
+ * num_inputs, num_variables (= num_entries), output_index, output_indices,
+ * custom_second_args ({}).  A Custom opcode returns
+ * EHIP_CUSTOM_OPS_NOT_SUPPORTED with the reference's message.  `values` has
+ * num_entries elements of `dtype`.  buf == NULL: *len = text length only;
+ * otherwise cap must be >= *len + 1 (a NUL is written). */
+int32_t ehip_tape_to_json(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                          const void* values, uint64_t num_entries, uint32_t num_inputs, uint32_t output_index,
+                          const uint32_t* output_indices, uint64_t num_output_indices, char* buf, uint64_t cap,
+                          uint64_t* len);
+/* Deserialize (serde_support.rs:33-100): parse, refuse Custom opcodes,
+ * non-empty custom_second_args and a num_variables stamp != opcodes.len(),
+ * then BytecodeTape::validate (src/bytecode_tape/mod.rs:575-703); every
+ * rejection is EHIP_OTHER with the reference's message.  Sizing call: all
+ * array pointers NULL fills the four counts only.  output_indices is the
+ * payload's list as written (possibly empty). */
+int32_t ehip_tape_from_json(const char* json, uint64_t len, int32_t dtype, uint32_t* opcodes, uint32_t* arg0,
+                            uint32_t* arg1, void* values, uint64_t cap_entries, uint32_t* output_indices,
+                            uint64_t cap_outputs, uint64_t* num_entries, uint32_t* num_inputs,
+                            uint32_t* output_index, uint64_t* num_output_indices);
+/* serde_json::from_str::<BytecodeTape<F>> followed by upload_tape /
+ * upload_tape_f64 (f32 values are the f64 parse rounded to f32). */
+int32_t ehip_tape_upload_json(ehip_ctx* ctx, int32_t dtype, const char* json, uint64_t len, ehip_tape** out);
+
+/* ------------------------------------------------------------------------
  * Learned low-rank control variate for STDE.  No counterpart in the
  * reference; it generalises laplacian_with_control (src/stde/laplacian.rs:
  * 90-129, src/gpu/stde_gpu.rs:135-200) from a fixed diagonal to
