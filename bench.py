@@ -49,8 +49,8 @@ def parse():
                          "cv: learned control variate vs plain on the n=1024 2x512 MLP; dags: 64 uneven "
                          "random DAGs in one persistent launch vs one launch per tape")
     ap.add_argument("--points", type=int, default=0, help="override the workload's point count")
-    ap.add_argument("--tuples-log2", type=int, default=16, help="cv: training tuples = 2^k")
-    ap.add_argument("--epochs", type=int, default=4, help="cv: training epochs")
+    ap.add_argument("--tuples-log2", type=int, default=20, help="cv: training tuples = 2^k")
+    ap.add_argument("--epochs", type=int, default=10, help="cv: training epochs")
     ap.add_argument("--dag-scale", type=float, default=1.0, help="dags: shrink lengths and point counts")
     return ap.parse_args()
 
@@ -70,9 +70,22 @@ def cpu_baseline(tape, x: np.ndarray, sample: int):
             "sample": f"{sample} of the {x.shape[0]} points, f64 oracle fwd+grad, {dt:.2f} s"}
 
 
-def s_req(res: np.ndarray, rel: float = 1e-3) -> np.ndarray:
-    """Directions each point needs for a standard error of rel * |estimate|."""
-    return res[:, 2] / np.maximum((rel * np.abs(res[:, 1])) ** 2, 1e-300)
+def s_req(var: np.ndarray, lap: np.ndarray, rel: float = 1e-3) -> np.ndarray:
+    """Directions each point needs for a standard error of rel * |Laplacian|:
+    sigma^2 / (rel * Laplacian)^2, with the exact f64 Laplacian as the
+    denominator (an estimate there would put each method's own noise into
+    the other's comparison, worst where the Laplacian is near zero)."""
+    return var / np.maximum((rel * np.abs(lap)) ** 2, 1e-300)
+
+
+def lap_rate(var: np.ndarray, lap: np.ndarray, t_sample: float) -> dict:
+    """Laplacians/s at 1e-3 relative standard error over a set of points:
+    one Laplacian costs S_req * t_sample; the mean over points is the
+    throughput of the whole set, the median that of a typical point."""
+    sr = s_req(var, lap)
+    return {"mean_S_req": float(np.mean(sr)), "median_S_req": float(np.median(sr)),
+            "laplacians_per_s": float(1.0 / (np.mean(sr) * t_sample)),
+            "median_point_laplacians_per_s": float(1.0 / (np.median(sr) * t_sample))}
 
 
 def timed(fn, reps: int):
@@ -111,16 +124,6 @@ def run_stde(args) -> dict:
            "data": "synthetic: Philox points, random-init weights baked into the tape",
            "config": {"workload": f"plain GPU STDE, tanh-MLP tape n={n} H={H} seed 2 ({tape.counts()['N']} "
                                   f"computed entries), B={P} points, S={S}", "points": P, "S": S}}
-    for name, kind in (("rademacher", lcv.DIR_RADEMACHER), ("gaussian", lcv.DIR_GAUSSIAN)):
-        dt, res = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, kind, 2), max(1, args.steps // 10))
-        t_sample = dt / (P * S)
-        sr = s_req(res)
-        out[name] = {"seconds": dt, "jets_per_s": P * S / dt, "t_sample_ns": t_sample * 1e9,
-                     "mean_S_req": float(np.mean(sr)), "median_S_req": float(np.median(sr)),
-                     "laplacians_per_s": float(1.0 / (np.mean(sr) * t_sample)),
-                     "mean_rel_se_at_S": float(np.mean(res[:, 3] / np.abs(res[:, 1])))}
-        if name == "rademacher":
-            rad = res
     # exact traces on a subset: f64 tape, n basis jets per point
     k = min(P, 1024)
     b64 = ctx.upload_tape(GpuTapeData.from_tape_f64(tape))
@@ -128,6 +131,14 @@ def run_stde(args) -> dict:
     t0 = time.perf_counter()
     exact = lcv.hessian_trace_batch(ctx, b64, X64)
     t_exact = time.perf_counter() - t0
+    for name, kind in (("rademacher", lcv.DIR_RADEMACHER), ("gaussian", lcv.DIR_GAUSSIAN)):
+        dt, res = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, kind, 2), max(1, args.steps // 10))
+        t_sample = dt / (P * S)
+        out[name] = {"seconds": dt, "jets_per_s": P * S / dt, "t_sample_ns": t_sample * 1e9,
+                     **lap_rate(res[:k, 2], exact, t_sample),
+                     "mean_rel_se_at_S": float(np.mean(res[:k, 3] / np.abs(exact)))}
+        if name == "rademacher":
+            rad = res
     z = (rad[:k, 1] - exact) / np.maximum(rad[:k, 3], 1e-300)
     out["exact_check"] = {"points": k, "seconds": t_exact, "mean_abs_z": float(np.mean(np.abs(z))),
                           "frac_within_3se": float(np.mean(np.abs(z) < 3.0)),
@@ -185,7 +196,10 @@ def run_cv(args) -> dict:
     del x, v, hv
     print(f"# {T} tuples in {t_tuples:.2f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    est.fit(ntuples=T, epochs=args.epochs, batch=512, lr=1e-3, val_points=256, val_S=S)
+    # start near C = 0 (an untrained network neither adds nor removes
+    # variance); cosine-decayed Adam over the tuples
+    est.fit(ntuples=T, epochs=args.epochs, batch=1024, lr=2e-3, val_points=256, val_S=S, out_scale=0.03,
+            schedule="cosine")
     t_fit = time.perf_counter() - t0 - t_tuples  # fit regenerates the tuples
     print(f"# trained in {t_fit:.2f} s, validation variance ratio {est.variance_ratio:.4f}", file=sys.stderr,
           flush=True)
@@ -193,14 +207,21 @@ def run_cv(args) -> dict:
     res_cv, res_pl = both
     t_s_plain = dt_plain / (P * S)
     t_s_cv = dt_cv / (P * S)
-    sr_plain, sr_cv = s_req(res_pl), s_req(res_cv)
-    lps_plain = 1.0 / (np.mean(sr_plain) * t_s_plain)
-    lps_cv = 1.0 / (np.mean(sr_cv) * t_s_cv) if est.use_cv else lps_plain
-    k = min(P, 64)
+    k = min(P, 1024)
     b64 = ctx.upload_tape(GpuTapeData.from_tape_f64(tape))
     t0 = time.perf_counter()
     exact = lcv.hessian_trace_batch(ctx, b64, X[:k].double())
     t_exact = time.perf_counter() - t0
+    rate_plain = lap_rate(res_pl[:k, 2], exact, t_s_plain)
+    rate_cv = lap_rate(res_cv[:k, 2], exact, t_s_cv)
+    lps_plain = rate_plain["laplacians_per_s"]
+    lps_cv = rate_cv["laplacians_per_s"] if est.use_cv else lps_plain
+    t0 = time.perf_counter()
+    bound = rank_bound(ctx, b64, est.cv, X[:16], r, r // 2)
+    bound["seconds"] = time.perf_counter() - t0
+    print(f"# rank-{r} bound: best variance ratio {bound['best_ratio']:.4f} "
+          f"(fixed signs {bound['best_ratio_signed']:.4f}), trained network {bound['network_ratio']:.4f}",
+          file=sys.stderr, flush=True)
     zc = (res_cv[:k, 1] - exact) / np.maximum(res_cv[:k, 3], 1e-300)
     zp = (res_pl[:k, 1] - exact) / np.maximum(res_pl[:k, 3], 1e-300)
     return {"metric": "Laplacians/s at 1e-3 rel err, learned CV vs plain", "value": float(lps_cv),
@@ -209,18 +230,55 @@ def run_cv(args) -> dict:
             "config": {"workload": f"tanh-MLP tape n={n} 2x{H} seed 4 ({tape.counts()['N']} computed entries), "
                                    f"B={P} points, S={S} Gaussian, CV {n}->256->256->{n}x{r} signed {r // 2}+{r // 2}",
                        "points": P, "S": S, "rank": r, "tuples": T, "epochs": args.epochs},
-            "plain": {"laplacians_per_s": float(lps_plain), "seconds": dt_plain, "t_sample_ns": t_s_plain * 1e9,
-                      "mean_S_req": float(np.mean(sr_plain)), "mean_var": float(np.mean(res_pl[:, 2]))},
-            "learned_cv": {"laplacians_per_s": float(lps_cv), "seconds": dt_cv, "t_sample_ns": t_s_cv * 1e9,
-                           "mean_S_req": float(np.mean(sr_cv)), "mean_var": float(np.mean(res_cv[:, 2])),
+            "plain": {**rate_plain, "seconds": dt_plain, "t_sample_ns": t_s_plain * 1e9,
+                      "mean_var": float(np.mean(res_pl[:, 2]))},
+            "learned_cv": {**rate_cv, "seconds": dt_cv, "t_sample_ns": t_s_cv * 1e9,
+                           "mean_var": float(np.mean(res_cv[:, 2])),
                            "variance_ratio": float(np.mean(res_cv[:, 2]) / np.mean(res_pl[:, 2])),
                            "validation_variance_ratio": est.variance_ratio, "used": est.use_cv,
-                           "speedup": float(lps_cv / lps_plain)},
+                           "speedup": float(lps_cv / lps_plain),
+                           "median_point_speedup": float(rate_cv["median_point_laplacians_per_s"]
+                                                         / rate_plain["median_point_laplacians_per_s"])},
             "training": {"tuple_seconds": t_tuples, "train_seconds": t_fit, "loss_first": float(est.history[0]),
                          "loss_last": float(est.history[-1])},
             "exact_check": {"points": k, "seconds": t_exact, "mean_abs_z_cv": float(np.mean(np.abs(zc))),
                             "mean_abs_z_plain": float(np.mean(np.abs(zp)))},
+            "rank_bound": bound,
             "setup": {"record_s": t_record, "upload_s": t_upload}}
+
+
+def rank_bound(ctx, b64, cvnet, X, r: int, npos: int) -> dict:
+    """What any rank-r control variate could remove on these points.
+
+    The exact f64 Hessian of each point comes from n GPU HVPs along the basis
+    vectors.  For Gaussian directions Var(v^T A v) = 2 |A|_F^2, so the best
+    rank-r C = L S L^T leaves |H - C|_F^2 = sum of all but the r largest
+    squared eigenvalues (Eckart-Young); with S fixed to npos positive and
+    r - npos negative signs, the npos largest positive and r - npos most
+    negative eigenvalues are removed.  The trained network's own C(x) is
+    scored the same way."""
+    k, n = X.shape
+    xs = np.repeat(X.double().cpu().numpy(), n, axis=0)
+    dirs = np.tile(np.eye(n), (k, 1))
+    _, hv = ctx.hvp_batch_points(b64, xs, dirs, k * n)
+    Hs = hv.reshape(k, n, n)
+    Hs = 0.5 * (Hs + Hs.transpose(0, 2, 1))
+    L = cvnet.forward(X.float()).double().cpu().numpy().reshape(k, n, r)
+    sgn = np.where(np.arange(r) < npos, 1.0, -1.0)
+    fro2 = res_any = res_signed = res_net = 0.0
+    for p in range(k):
+        lam = np.linalg.eigvalsh(Hs[p])
+        sq = np.sort(lam * lam)[::-1]
+        pos = np.sort(lam[lam > 0])[::-1]
+        neg = np.sort(-lam[lam < 0])[::-1]
+        f2 = float(np.sum(sq))
+        fro2 += f2
+        res_any += f2 - float(np.sum(sq[:r]))
+        res_signed += f2 - float(np.sum(pos[:npos] ** 2) + np.sum(neg[:r - npos] ** 2))
+        C = (L[p] * sgn) @ L[p].T
+        res_net += float(np.sum((Hs[p] - C) ** 2))
+    return {"points": k, "best_ratio": res_any / fro2, "best_ratio_signed": res_signed / fro2,
+            "network_ratio": res_net / fro2}
 
 
 def run_dags(args) -> dict:

@@ -139,14 +139,24 @@ class LearnedControlVariate:
     def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
         _check(load_library().ehip_cv_adam_step(self.handle, lr, beta1, beta2, eps, _vp(None)))
 
-    def train(self, d_x, d_v, d_hv, batch: int = 512, epochs: int = 1, lr: float = 1e-3) -> np.ndarray:
-        """Minibatch Adam over the tuples; returns the per-step mean losses."""
+    def train(self, d_x, d_v, d_hv, batch: int = 512, epochs: int = 1, lr: float = 1e-3,
+              schedule: str = "constant") -> np.ndarray:
+        """Minibatch Adam over the tuples; returns the per-step mean losses.
+
+        ``schedule="cosine"`` sets epoch e's rate to lr * (1 + cos(pi e / epochs)) / 2.
+        """
         torch = _torch()
         T = d_x.shape[0]
         steps = (T + batch - 1) // batch
         hist = torch.zeros(epochs * steps, dtype=torch.float32, device=d_x.device)
-        _check(load_library().ehip_cv_train_dev(self.handle, _dp(d_x), _dp(d_v), _dp(d_hv), T, batch, epochs, lr,
-                                                _dp(hist), _vp(None)))
+        if schedule not in ("constant", "cosine"):
+            raise ValueError(f"unknown schedule {schedule!r}")
+        rates = [lr] if schedule == "constant" else [lr * 0.5 * (1.0 + np.cos(np.pi * e / epochs))
+                                                     for e in range(epochs)]
+        per = epochs if schedule == "constant" else 1
+        for e, rate in enumerate(rates):
+            _check(load_library().ehip_cv_train_dev(self.handle, _dp(d_x), _dp(d_v), _dp(d_hv), T, batch, per,
+                                                    float(rate), _dp(hist[e * steps:]), _vp(None)))
         torch.cuda.synchronize()
         return hist.cpu().numpy()
 
@@ -213,7 +223,8 @@ class LearnedStde:
     history: List[float] = field(default_factory=list)
 
     def fit(self, ntuples: int, epochs: int = 4, batch: int = 512, lr: float = 1e-3, val_points: int = 256,
-            val_S: int = 16, out_scale: Optional[float] = None, kind: int = DIR_GAUSSIAN) -> "LearnedStde":
+            val_S: int = 16, out_scale: Optional[float] = None, kind: int = DIR_GAUSSIAN,
+            schedule: str = "constant") -> "LearnedStde":
         """Train on ``ntuples`` tuples, then validate.
 
         Scale: the output gain is set to sqrt(s) with s = rms|Hv| / rms|v|
@@ -235,7 +246,7 @@ class LearnedStde:
         s = float(np.sqrt(np.mean(hs * hs) / max(np.mean(vs * vs), 1e-300)))
         if np.isfinite(s) and s > 0:
             self.cv.set_gain(float(np.sqrt(s)))
-        self.history = list(self.cv.train(x, v, hv, batch, epochs, lr))
+        self.history = list(self.cv.train(x, v, hv, batch, epochs, lr, schedule))
         del x, v, hv
         xv = self.validation_points(val_points)
         cvr, plain = laplacian_batch(self.ctx, self.tape, xv, val_S, kind, self.seed + 3, self.cv, with_plain=True)

@@ -122,6 +122,23 @@ def test_adam_step_matches_reference(hip):
     assert net.info()["steps"] == 2
 
 
+def test_cosine_schedule_is_per_epoch_adam(hip):
+    """schedule="cosine" equals epoch-by-epoch training at lr (1 + cos(pi e / E)) / 2."""
+    T, E, lr = 96, 3, 1e-2
+    X = wl.points_normal(11, T, N).astype(np.float32)
+    V = philox.normal(12, philox.STREAM_DIRS, np.arange(T * N)).reshape(T, N).astype(np.float32)
+    HV = (0.3 * V).astype(np.float32)
+    dev = [torch.from_numpy(a).cuda() for a in (X, V, HV)]
+    a, b = random_cv(hip, seed=5), random_cv(hip, seed=5)
+    ha = a.train(*dev, batch=32, epochs=E, lr=lr, schedule="cosine")
+    hb = np.concatenate([b.train(*dev, batch=32, epochs=1, lr=lr * 0.5 * (1 + np.cos(np.pi * e / E)))
+                         for e in range(E)])
+    assert len(ha) == E * 3 and np.array_equal(ha, hb)
+    assert np.array_equal(a.params(), b.params()) and a.info()["steps"] == E * 3
+    with pytest.raises(ValueError, match="unknown schedule"):
+        a.train(*dev, schedule="linear")
+
+
 def test_tuples_match_philox_and_hvp(hip):
     tape = wl.tanh_mlp(10, 12, 50)
     bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
