@@ -1,10 +1,16 @@
-"""Build libechidna_hip.so (HIP for gfx950) in-tree with hipcc."""
+"""Build libechidna_hip.so (HIP for gfx950) in-tree with hipcc.
+
+Each source compiles to its own object (in parallel, in a temporary
+directory), then hipcc links the shared library next to this file.
+"""
 
 from __future__ import annotations
 
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -13,26 +19,47 @@ LIB = os.path.join(HERE, "libechidna_hip.so")
 SOURCES = ["ehip_api.cpp", "ehip_program.cpp", "ehip_kernels.hip", "ehip_stde.hip", "ehip_cv.hip",
            "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp"]
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # Reference arithmetic never contracts a*b+c into an FMA.
     "-ffp-contract=off",
     # Keep the compiler from forming packed FP32 instructions.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+# The interpreters are one loop around every opcode's code.  Machine LICM
+# hoists each math routine's polynomial constants out of that loop into
+# VGPRs held for the whole kernel (f64 VOP3 instructions take no 64-bit
+# literals), which took the f64 interpreter to 256 VGPRs plus AGPR spills
+# and one wave per SIMD; without it each constant is materialised where it
+# is used and the f64 interpreter needs about 100 VGPRs.
+EXTRA = {"ehip_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
-def build(verbose: bool = False) -> str:
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc, *FLAGS, "-o", tmp, *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libechidna_hip.so")
+    return r.returncode
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8"))))
+    with tempfile.TemporaryDirectory(prefix="ehip_build_") as tmpdir:
+        objs, cmds = [], []
+        for s in SOURCES:
+            obj = os.path.join(tmpdir, s + ".o")
+            objs.append(obj)
+            cmds.append([hipcc, *FLAGS, *EXTRA.get(s, []), "-c", "-o", obj, os.path.join(CSRC, s)])
+        with ThreadPoolExecutor(jobs) as ex:
+            rcs = list(ex.map(lambda c: _run(c, verbose), cmds))
+        if any(rcs):
+            raise RuntimeError("hipcc failed building libechidna_hip.so")
+        tmp = LIB + ".tmp"
+        if _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose):
+            raise RuntimeError("hipcc failed linking libechidna_hip.so")
     os.replace(tmp, LIB)
     return LIB
 
