@@ -409,9 +409,13 @@ def main() -> int:
     achieved = bytes_per_point * B / (kernel_ms * 1e-3)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_mlp32_fwdgrad_{args.dtype}.json")
+    launch = tuned or {"wg": args.wg, "lds_budget": args.lds}
     if os.path.exists(tpath) and args.batch_log2 == 20:
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        # counters hold for the launch shape they were taken with
+        if prof.get("launch") == {"wg": launch.get("wg"), "lds_budget": launch.get("lds_budget")}:
+            traffic = prof.get("hbm_bytes_per_launch")
 
     out = {
         "metric": "fwd+grad point-evals/s",
@@ -430,7 +434,7 @@ def main() -> int:
                                f"B=2^{args.batch_log2} points, fwd+grad",
                    "points": B, "inputs": n, "hidden": H, "computed_entries": counts["N"],
                    "program": bufs.program_info(1), "parallelism": "none (1 GPU)",
-                   "launch": tuned or {"wg": args.wg, "lds_budget": args.lds}},
+                   "launch": launch},
         "graph_ms_per_step": graph_ms,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,

@@ -32,7 +32,8 @@ FLAGS = [
 # literals), which took the f64 interpreter to 256 VGPRs plus AGPR spills
 # and one wave per SIMD; without it each constant is materialised where it
 # is used and the f64 interpreter needs about 100 VGPRs.
-EXTRA = {"ehip_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
+NO_LICM = ["-mllvm", "-disable-machine-licm"]
+EXTRA = {"ehip_kernels.hip": NO_LICM, "ehip_stde.hip": NO_LICM}
 
 
 def _run(cmd, verbose):
