@@ -34,6 +34,9 @@ extern __shared__ __attribute__((aligned(16))) unsigned char g_smem[];
 
 #define AS3 __attribute__((address_space(3)))
 #define AS1 __attribute__((address_space(1)))
+// Constant address space: read-only for the whole launch, so wave-uniform
+// loads through it become scalar loads (SGPR results, scalar cache).
+#define AS4 __attribute__((address_space(4)))
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -193,11 +196,63 @@ __device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, AS3 c
 // members.  Operand lists and weights come from the pool through
 // wave-uniform (scalar) loads.
 // ---------------------------------------------------------------------------
-// One block of JB members (FULL: exactly JB, no per-member bounds checks).
-// JET: products follow the Taylor product with a constant jet (jt::mul, the
-// operation the jet interpreter's DOT performs); otherwise v * c.
-template <class F, int NC, int JB, bool JET, bool FULL, class Regs_>
-__device__ __forceinline__ void dense_fwd_block(const Regs_& R, const uint32_t* PR, const F* PW, uint32_t m,
+// Value programs.  One term of a block of JB members (FULL: exactly JB, no
+// per-member bounds checks): FIRST initialises the accumulators with it,
+// otherwise it is added.  Pool words and weights arrive through scalar loads.
+template <class F, int JB, bool FULL, bool FIRST, class Regs_>
+__device__ __forceinline__ void dense_fwd_term(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
+                                               uint32_t p, uint32_t j0, uint32_t jb, F (&acc)[JB]) {
+  const uint32_t kr = PR[p];
+  const uint32_t kind = kr >> P_KIND_SHIFT;
+  AS4 const F* w = PW + (uint64_t)p * m + j0;
+  if (kind == T_PROD) {
+    F x;
+    R.rd(kr & P_REG_MASK, &x);
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      if (!(FULL || (uint32_t)j < jb)) continue;
+      const F term = x * w[j];
+      acc[j] = FIRST ? term : acc[j] + term;
+    }
+  } else if (kind == T_REG) {
+    F x;
+    R.rd(kr & P_REG_MASK, &x);
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      if (!(FULL || (uint32_t)j < jb)) continue;
+      acc[j] = FIRST ? x : acc[j] + x;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      if (!(FULL || (uint32_t)j < jb)) continue;
+      acc[j] = FIRST ? w[j] : acc[j] + w[j];
+    }
+  }
+}
+
+// The first term peeled, so the loop body is a plain acc + term.
+template <class F, int JB, bool FULL, class Regs_>
+__device__ __forceinline__ void dense_fwd_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
+                                                uint32_t T, uint32_t j0, uint32_t jb) {
+  F acc[JB];
+  dense_fwd_term<F, JB, FULL, true>(R, PR, PW, m, 0, j0, jb, acc);
+  for (uint32_t p = 1; p < T; ++p) dense_fwd_term<F, JB, FULL, false>(R, PR, PW, m, p, j0, jb, acc);
+#pragma unroll
+  for (int j = 0; j < JB; ++j) {
+    if (!(FULL || (uint32_t)j < jb)) continue;
+    R.wr(PR[T + j0 + j], &acc[j]);
+  }
+}
+
+// Jet programs: products follow the Taylor product with a constant jet
+// (jt::mul, the operation the jet interpreter's DOT performs); one loop with
+// a per-term select over generic pool loads.  The value-program form raises
+// the jet interpreter's register use (115 instead of 96 VGPRs at order 3 in
+// f32, 276 instead of 169 at order 2 in f64), costing more occupancy than
+// it saves.
+template <class F, int NC, int JB, bool FULL, class Regs_>
+__device__ __forceinline__ void dense_jet_block(const Regs_& R, const uint32_t* PR, const F* PW, uint32_t m,
                                                 uint32_t T, uint32_t j0, uint32_t jb) {
   F acc[JB][NC];
   for (uint32_t p = 0; p < T; ++p) {
@@ -212,17 +267,11 @@ __device__ __forceinline__ void dense_fwd_block(const Regs_& R, const uint32_t* 
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
         const F c = w[j];
-        F term[NC];
-        if (JET) {
-          F cj[NC];
+        F term[NC], cj[NC];
 #pragma unroll
-          for (int q = 0; q < NC; ++q) cj[q] = (q == 0) ? c : F(0);
-          if (cleft) jt::mul<F, NC>(cj, x, term);
-          else jt::mul<F, NC>(x, cj, term);
-        } else {
-#pragma unroll
-          for (int q = 0; q < NC; ++q) term[q] = x[q] * c;
-        }
+        for (int q = 0; q < NC; ++q) cj[q] = (q == 0) ? c : F(0);
+        if (cleft) jt::mul<F, NC>(cj, x, term);
+        else jt::mul<F, NC>(x, cj, term);
 #pragma unroll
         for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
       }
@@ -258,11 +307,19 @@ __device__ __forceinline__ void dense_fwd_block(const Regs_& R, const uint32_t* 
 template <class F, int NC, int JB, bool JET, class Regs_>
 __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, uint32_t m, uint32_t T,
                                           uint32_t off_regs, uint32_t off_w) {
-  const uint32_t* PR = pool + off_regs;
-  const F* PW = reinterpret_cast<const F*>(pool + off_w);
   uint32_t j0 = 0;
-  for (; j0 + JB <= m; j0 += JB) dense_fwd_block<F, NC, JB, JET, true>(R, PR, PW, m, T, j0, JB);
-  if (j0 < m) dense_fwd_block<F, NC, JB, JET, false>(R, PR, PW, m, T, j0, m - j0);
+  if constexpr (JET) {
+    const uint32_t* PR = pool + off_regs;
+    const F* PW = reinterpret_cast<const F*>(pool + off_w);
+    for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true>(R, PR, PW, m, T, j0, JB);
+    if (j0 < m) dense_jet_block<F, NC, JB, false>(R, PR, PW, m, T, j0, m - j0);
+  } else {
+    static_assert(NC == 1, "value programs carry one component");
+    AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
+    AS4 const F* PW = reinterpret_cast<AS4 const F*>((AS4 const uint32_t*)pool + off_w);
+    for (; j0 + JB <= m; j0 += JB) dense_fwd_block<F, JB, true>(R, PR, PW, m, T, j0, JB);
+    if (j0 < m) dense_fwd_block<F, JB, false>(R, PR, PW, m, T, j0, m - j0);
+  }
 }
 
 // One block of IB adjoints.  Fast path, taken per member when the compiler
@@ -271,7 +328,7 @@ __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, 
 // zero-adjoint skip and zero-multiplier guard cannot trigger and the
 // contribution is c * g unconditionally.
 template <class F, int IB, bool FULL, bool LDSONLY>
-__device__ __forceinline__ void dense_rev_block(const Regs<F, 1, LDSONLY>& R, const uint32_t* PR, const F* PW,
+__device__ __forceinline__ void dense_rev_block(const Regs<F, 1, LDSONLY>& R, AS4 const uint32_t* PR, AS4 const F* PW,
                                                 uint32_t m, uint32_t T, uint32_t i0, uint32_t ib, bool fastw) {
   F cur[IB];
   uint32_t kr[IB];
@@ -286,7 +343,7 @@ __device__ __forceinline__ void dense_rev_block(const Regs<F, 1, LDSONLY>& R, co
   for (uint32_t j = 0; j < m; ++j) {
     F g;
     R.rd(PR[T + j], &g);
-    const F* w = PW + (uint64_t)j * T + i0;
+    AS4 const F* w = PW + (uint64_t)j * T + i0;
     if (fastw && __all(g != F(0) && isfinite(g))) {
 #pragma unroll
       for (int i = 0; i < IB; ++i) {
@@ -318,8 +375,8 @@ template <class F, bool LDSONLY>
 __device__ __forceinline__ void dense_rev(const Regs<F, 1, LDSONLY>& R, const uint32_t* pool, uint32_t m, uint32_t T,
                                           uint32_t off_regs, uint32_t off_w, uint32_t dflags) {
   constexpr int IB = sizeof(F) == 8 ? 4 : 8;
-  const uint32_t* PR = pool + off_regs;
-  const F* PW = reinterpret_cast<const F*>(pool + off_w);
+  AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
+  AS4 const F* PW = reinterpret_cast<AS4 const F*>((AS4 const uint32_t*)pool + off_w);
   const bool fastw = (dflags & D_ALL_PROD_NZ) != 0;
   uint32_t i0 = 0;
   for (; i0 + IB <= T; i0 += IB) dense_rev_block<F, IB, true, LDSONLY>(R, PR, PW, m, T, i0, IB, fastw);
