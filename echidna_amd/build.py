@@ -18,6 +18,10 @@ LIB = os.path.join(HERE, "libechidna_hip.so")
 
 SOURCES = ["ehip_api.cpp", "ehip_program.cpp", "ehip_kernels.hip", "ehip_stde.hip", "ehip_cv.hip",
            "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp"]
+# The interpreters are built once per program-fetch mode, each in its own
+# namespace (object name, defines).
+VARIANTS = {"ehip_kernels.hip": [("ehip_kernels_lds", ["-DEHIP_SFETCH=0", "-DEHIP_NS=lds_fetch"]),
+                                 ("ehip_kernels_sfetch", ["-DEHIP_SFETCH=1", "-DEHIP_NS=scalar_fetch"])]}
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # Reference arithmetic never contracts a*b+c into an FMA.
@@ -51,15 +55,17 @@ def build(verbose: bool = False) -> str:
     with tempfile.TemporaryDirectory(prefix="ehip_build_") as tmpdir:
         objs, cmds = [], []
         for s in SOURCES:
-            obj = os.path.join(tmpdir, s + ".o")
-            objs.append(obj)
-            cmds.append([hipcc, *FLAGS, *EXTRA.get(s, []), "-c", "-o", obj, os.path.join(CSRC, s)])
+            for name, defs in VARIANTS.get(s, [(s, [])]):
+                obj = os.path.join(tmpdir, name + ".o")
+                objs.append(obj)
+                cmds.append([hipcc, *FLAGS, *EXTRA.get(s, []), *defs, "-c", "-o", obj, os.path.join(CSRC, s)])
         with ThreadPoolExecutor(jobs) as ex:
             rcs = list(ex.map(lambda c: _run(c, verbose), cmds))
         if any(rcs):
             raise RuntimeError("hipcc failed building libechidna_hip.so")
         tmp = LIB + ".tmp"
-        if _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose):
+        if _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs],
+                verbose):
             raise RuntimeError("hipcc failed linking libechidna_hip.so")
     os.replace(tmp, LIB)
     return LIB

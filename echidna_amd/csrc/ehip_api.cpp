@@ -69,6 +69,14 @@ int32_t ehip_set_launch(ehip_ctx* c, uint32_t wg, uint32_t lds_budget) {
   return ok();
 }
 
+int32_t ehip_set_fetch(ehip_ctx* c, int32_t mode) {
+  if (!c) return fail(EHIP_OTHER, "GPU error: null context");
+  if (mode != 0 && mode != 1) return fail(EHIP_OTHER, "GPU error: program fetch must be 0 (LDS) or 1 (scalar)");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->fetch = (uint32_t)mode;
+  return ok();
+}
+
 int32_t ehip_set_dense(ehip_ctx* c, int32_t enable) {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -360,7 +368,8 @@ int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, 
   hipStream_t s = pick(c, stream);
   const size_t S = f64 ? 8 : 4;
   const size_t per_reg = (size_t)wg * S;
-  const size_t win_lds = 2 * (size_t)WIN_WORDS * 4;
+  const bool sf = c->fetch != 0;
+  const size_t win_lds = window_lds(sf);
   const size_t cap = c->max_lds > win_lds ? c->max_lds - win_lds : 0;
   const size_t budget = std::min<size_t>(lds_budget, cap);
   // jobs, longest program first
@@ -398,7 +407,7 @@ int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, 
   if (tiles.size() > 0xFFFFFFF0ull) return fail(EHIP_OTHER, "GPU error: too many tiles in one queue");
   const size_t lds = (size_t)max_rlds * per_reg;
   int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, queue_kernel(f64), (int)wg, queue_lds_bytes(lds)),
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, queue_kernel(f64, sf), (int)wg, queue_lds_bytes(sf, lds)),
           "queue occupancy");
   uint64_t grid = (uint64_t)std::max(per_cu, 1) * c->ncu;
   grid = std::min<uint64_t>(grid, tiles.size());
@@ -420,7 +429,7 @@ int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, 
   Q.tiles = (const QueueTile*)c->qtiles.p;
   Q.ntiles = (uint32_t)tiles.size();
   Q.counter = (uint32_t*)c->qcount.p;
-  hipError_t e = launch_queue(f64, Q, (uint32_t)grid, wg, lds, s);
+  hipError_t e = launch_queue(f64, sf, Q, (uint32_t)grid, wg, lds, s);
   if (e != hipSuccess) return hip_fail(e, "queue launch");
   // the host tables above are released on return
   HIP_TRY(hipStreamSynchronize(s), "queue");
@@ -465,8 +474,8 @@ int32_t tune_launch(const ehip_tape* t, int32_t what, uint32_t order, const void
 
 int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t order, const void* d_x,
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
-                          uint32_t* out3) {
-  if (!c || !t || !d_x || !out3) return fail(EHIP_OTHER, "GPU error: null argument");
+                          uint32_t* out4) {
+  if (!c || !t || !d_x || !out4) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, batch);
   if (rc) return rc;
   Launch L;
@@ -484,11 +493,12 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
   Tuned best;
   float best_ms = 1e30f;
   static const uint32_t budgets[] = {4096, 8192, 16384, 24576, 32768, 49152, 65536, 98304, 131072};
+  for (uint32_t fetch : {0u, 1u})
   for (uint32_t wg : {64u, 128u, 256u}) {
     std::vector<uint32_t> seen;
     for (uint32_t b : budgets) {
       const size_t per_reg = (size_t)NC * wg * S;
-      const size_t win_lds = 2 * (size_t)WIN_WORDS * 4;
+      const size_t win_lds = window_lds(fetch != 0);
       const size_t cap = c->max_lds > win_lds ? c->max_lds - win_lds : 0;
       if (b > cap) continue;
       const uint32_t rl = (uint32_t)std::min<size_t>(R, b / per_reg);
@@ -497,6 +507,7 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
       Tuned cand;
       cand.wg = wg;
       cand.lds_budget = b;
+      cand.fetch = fetch;
       float ms = 1e30f;
       for (int rep = 0; rep < 3 && rc == EHIP_OK; ++rep) {
         HIP_TRY(hipEventRecord(e0, c->stream), "autotune");
@@ -516,33 +527,38 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   t->tuned[tune_key(kind, K, what == 1)] = best;
-  out3[0] = best.wg;
-  out3[1] = best.lds_budget;
-  out3[2] = (uint32_t)std::min(best_ms * 1000.f, 4.0e9f);
+  out4[0] = best.wg;
+  out4[1] = best.lds_budget;
+  out4[2] = (uint32_t)std::min(best_ms * 1000.f, 4.0e9f);
+  out4[3] = best.fetch;
   return ok();
 }
 
-int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget) {
+int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
+                       uint32_t fetch) {
   if (!t) return fail(EHIP_OTHER, "GPU error: null argument");
   if (wg != 0 && wg != 64 && wg != 128 && wg != 256)
     return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
+  if (fetch > 1) return fail(EHIP_OTHER, "GPU error: program fetch must be 0 (LDS) or 1 (scalar)");
   int key;
   if (what >= 0 && what <= 2) key = what;
   else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
   else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
   t->tuned[key].wg = wg;
   t->tuned[key].lds_budget = wg ? lds_budget : 0;
+  t->tuned[key].fetch = wg ? fetch : 0;
   return ok();
 }
 
-int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_t* out2) {
-  if (!t || !out2) return fail(EHIP_OTHER, "GPU error: null argument");
+int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_t* out3) {
+  if (!t || !out3) return fail(EHIP_OTHER, "GPU error: null argument");
   int key;
   if (what >= 0 && what <= 2) key = what;
   else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
   else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
-  out2[0] = t->tuned[key].wg;
-  out2[1] = t->tuned[key].lds_budget;
+  out3[0] = t->tuned[key].wg;
+  out3[1] = t->tuned[key].lds_budget;
+  out3[2] = t->tuned[key].fetch;
   return ok();
 }
 

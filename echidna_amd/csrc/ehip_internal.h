@@ -86,6 +86,7 @@ struct ehip_ctx {
   uint32_t ncu = 256;
   uint32_t wg = 64;
   uint32_t lds_budget = 32768;
+  uint32_t fetch = 0;  // program fetch of untuned launches and the queue (0 LDS, 1 scalar)
   bool fuse = true;
   bool dense = true;
   size_t max_lds = 65536;
@@ -102,6 +103,7 @@ struct DevProgram {
 struct Tuned {
   uint32_t wg = 0;          // 0: not tuned, use the context defaults
   uint32_t lds_budget = 0;  // bytes of LDS for the register file
+  uint32_t fetch = 0;       // program fetch: 0 LDS windows, 1 scalar cache
 };
 
 struct ehip_tape {
@@ -168,9 +170,10 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   const size_t S = t->f64 ? 8 : 4;
   const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
   const uint32_t wg = tu.wg ? tu.wg : c->wg;
+  const bool sf = (tu.wg ? tu.fetch : c->fetch) != 0;
   const size_t per_reg = (size_t)NC * wg * S;
-  // two instruction windows share the workgroup's LDS with the registers
-  const size_t win_lds = 2 * (size_t)WIN_WORDS * 4;
+  // LDS fetch: two instruction windows share the workgroup's LDS with the registers
+  const size_t win_lds = window_lds(sf);
   size_t budget = std::min<size_t>(tu.wg ? tu.lds_budget : c->lds_budget, c->max_lds > win_lds ? c->max_lds - win_lds : 0);
   uint32_t R = dp.host.nregs;
   uint32_t R_lds = (uint32_t)std::min<size_t>(R, budget / per_reg);
@@ -187,9 +190,9 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
     L.gregs = nullptr;
   }
   hipError_t e;
-  if (kind == 0 || kind == 3) e = launch_scalar(t->f64, L, wg, lds, s);
-  else if (kind == 1) e = launch_dual(t->f64, L, wg, lds, s);
-  else e = launch_jet(t->f64, K, L, wg, lds, s);
+  if (kind == 0 || kind == 3) e = launch_scalar(t->f64, sf, L, wg, lds, s);
+  else if (kind == 1) e = launch_dual(t->f64, sf, L, wg, lds, s);
+  else e = launch_jet(t->f64, sf, K, L, wg, lds, s);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return EHIP_OK;
 }

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ehip_program.h"
+
 namespace ehip {
 
 struct Launch {
@@ -62,12 +64,47 @@ struct Queue {
   uint32_t* counter;       // next tile; cleared by launch_queue
 };
 
-const void* queue_kernel(bool f64);
-size_t queue_lds_bytes(size_t reg_lds);
-hipError_t launch_queue(bool f64, const Queue& Q, uint32_t grid, uint32_t wg, size_t lds, hipStream_t s);
+// The interpreters are compiled twice, once per program-fetch mode
+// (ehip_kernels.hip with EHIP_SFETCH 0 / 1): lds_fetch stages instruction
+// windows in LDS per workgroup, scalar_fetch reads the stream through the
+// scalar cache.  `lds` is the register file's LDS; the launchers add the
+// window buffers where the mode needs them.
+#define EHIP_FETCH_LAUNCHERS                                                                                \
+  const void* queue_kernel(bool f64);                                                                      \
+  size_t queue_lds_bytes(size_t reg_lds);                                                                  \
+  hipError_t launch_queue(bool f64, const Queue& Q, uint32_t grid, uint32_t wg, size_t lds, hipStream_t s); \
+  hipError_t launch_scalar(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);             \
+  hipError_t launch_dual(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);               \
+  hipError_t launch_jet(bool f64, int K, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);
+namespace lds_fetch {
+EHIP_FETCH_LAUNCHERS
+}
+namespace scalar_fetch {
+EHIP_FETCH_LAUNCHERS
+}
+#undef EHIP_FETCH_LAUNCHERS
 
-hipError_t launch_scalar(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);
-hipError_t launch_dual(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);
-hipError_t launch_jet(bool f64, int K, const Launch& L, uint32_t wg, size_t lds, hipStream_t s);
+// LDS the instruction windows take beside the register file.
+inline size_t window_lds(bool sfetch) { return sfetch ? 0 : 2 * (size_t)WIN_WORDS * 4; }
+
+inline const void* queue_kernel(bool f64, bool sfetch) {
+  return sfetch ? scalar_fetch::queue_kernel(f64) : lds_fetch::queue_kernel(f64);
+}
+inline size_t queue_lds_bytes(bool sfetch, size_t reg_lds) {
+  return sfetch ? scalar_fetch::queue_lds_bytes(reg_lds) : lds_fetch::queue_lds_bytes(reg_lds);
+}
+inline hipError_t launch_queue(bool f64, bool sfetch, const Queue& Q, uint32_t grid, uint32_t wg, size_t lds,
+                               hipStream_t s) {
+  return sfetch ? scalar_fetch::launch_queue(f64, Q, grid, wg, lds, s) : lds_fetch::launch_queue(f64, Q, grid, wg, lds, s);
+}
+inline hipError_t launch_scalar(bool f64, bool sfetch, const Launch& L, uint32_t wg, size_t lds, hipStream_t s) {
+  return sfetch ? scalar_fetch::launch_scalar(f64, L, wg, lds, s) : lds_fetch::launch_scalar(f64, L, wg, lds, s);
+}
+inline hipError_t launch_dual(bool f64, bool sfetch, const Launch& L, uint32_t wg, size_t lds, hipStream_t s) {
+  return sfetch ? scalar_fetch::launch_dual(f64, L, wg, lds, s) : lds_fetch::launch_dual(f64, L, wg, lds, s);
+}
+inline hipError_t launch_jet(bool f64, bool sfetch, int K, const Launch& L, uint32_t wg, size_t lds, hipStream_t s) {
+  return sfetch ? scalar_fetch::launch_jet(f64, K, L, wg, lds, s) : lds_fetch::launch_jet(f64, K, L, wg, lds, s);
+}
 
 }  // namespace ehip

@@ -29,6 +29,10 @@
 #include "philox.h"
 
 namespace ehip {
+#ifndef EHIP_NS
+#define EHIP_NS lds_fetch
+#endif
+namespace EHIP_NS {
 
 extern __shared__ __attribute__((aligned(16))) unsigned char g_smem[];
 
@@ -90,13 +94,25 @@ struct Regs {
   }
 };
 
-// LDS layout: [window buffer 0 | window buffer 1 | register file].
+// LDS layout: [window buffer 0 | window buffer 1 (LDS fetch only) | register file].
 constexpr uint32_t WIN_BYTES = WIN_WORDS * 4;
+
+#ifndef EHIP_SFETCH
+#define EHIP_SFETCH 0
+#endif
+// Program fetch.  LDS: each window is staged once per workgroup in LDS
+// (double-buffered) and instruction words are read from there.  Scalar: the
+// stream is read in place through the constant address space, so words
+// arrive in SGPRs through the scalar cache with no LDS round trip and no
+// readfirstlane.
+constexpr bool SFETCH = EHIP_SFETCH != 0;
+// LDS before the register file: the two instruction windows (LDS fetch).
+constexpr uint32_t WIN_LDS = SFETCH ? 0u : 2u * WIN_BYTES;
 
 template <class F, int NC, bool LDSONLY>
 __device__ __forceinline__ Regs<F, NC, LDSONLY> make_regs(const Launch& L) {
   Regs<F, NC, LDSONLY> R;
-  R.lane = (AS3 F*)(g_smem + 2 * WIN_BYTES) + threadIdx.x;
+  R.lane = (AS3 F*)(g_smem + WIN_LDS) + threadIdx.x;
   R.gl = (AS1 F*)(L.gregs);
   R.R_lds = L.R_lds;
   R.lsh = __builtin_ctz(blockDim.x);
@@ -105,10 +121,28 @@ __device__ __forceinline__ Regs<F, NC, LDSONLY> make_regs(const Launch& L) {
   return R;
 }
 
-// Streams the program window by window through LDS and calls
-// `exec(window, pos, w0)` for every instruction.
+
+
+// Streams the program window by window and calls `exec(window, pos, w0)`
+// for every instruction.
 template <class Exec>
 __device__ __forceinline__ void run_stream(const Launch& L, Exec exec) {
+  if constexpr (SFETCH) {
+    AS4 const uint32_t* prog = (AS4 const uint32_t*)L.prog;
+    const uint32_t nwin = L.nwin;
+    for (uint32_t w = 0; w < nwin; ++w) {
+      AS4 const uint32_t* win = prog + (size_t)w * WIN_WORDS;
+      uint32_t pos = 0;
+      for (;;) {
+        const uint32_t w0 = win[pos];
+        const uint32_t op = w0 & 0xFFu;
+        if (op == I_WEND || op == I_END) break;
+        exec(win, pos, w0);
+        pos += w0 >> 24;
+      }
+    }
+    return;
+  }
   AS3 uint32_t* buf = (AS3 uint32_t*)g_smem;
   const uint32_t tid = threadIdx.x, wg = blockDim.x;
   const u32x4* src = reinterpret_cast<const u32x4*>(L.prog);
@@ -150,41 +184,46 @@ __device__ __forceinline__ void run_stream(const Launch& L, Exec exec) {
 }
 
 // Word accessors for the current instruction.
-template <class F>
+template <class F, class P>
 struct Ins {
-  AS3 const uint32_t* p;
+  P p;
   __device__ __forceinline__ uint32_t u(uint32_t k) const { return uni(p[k]); }  // uniform
   __device__ __forceinline__ uint32_t v(uint32_t k) const { return p[k]; }       // VGPR copy
   __device__ __forceinline__ F imm(uint32_t k) const { return imm_of<F>(p[k], sizeof(F) == 8 ? p[k + 1] : 0u); }
 };
 
 // Inline term k of a DOT / DOT_REV / LOADN: meta, reg, imm.
-template <class F>
+template <class F, class P>
 struct Term {
-  AS3 const uint32_t* p;
-  __device__ __forceinline__ Term(AS3 const uint32_t* ins, uint32_t k) : p(ins + 3 + k * (2 + sizeof(F) / 4)) {}
+  P p;
+  __device__ __forceinline__ Term(P ins, uint32_t k) : p(ins + 3 + k * (2 + sizeof(F) / 4)) {}
   __device__ __forceinline__ uint32_t meta() const { return p[0]; }
   __device__ __forceinline__ uint32_t reg() const { return p[1]; }
   __device__ __forceinline__ F imm() const { return imm_of<F>(p[2], sizeof(F) == 8 ? p[3] : 0u); }
   __device__ __forceinline__ uint32_t u32() const { return p[2]; }
 };
 
+template <class F, class P>
+__device__ __forceinline__ Term<F, P> term(P ins, uint32_t k) {
+  return Term<F, P>(ins, k);
+}
+
 // Batched input load: eight global loads in flight, then eight register writes.
-template <class F, int NC, bool LDSONLY, class MakeVal>
-__device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, AS3 const uint32_t* ins, uint32_t n,
+template <class F, int NC, bool LDSONLY, class P, class MakeVal>
+__device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, P ins, uint32_t n,
                                             MakeVal make) {
   uint32_t k = 0;
   for (; k + 8 <= n; k += 8) {
     F v[8][NC];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) make(Term<F>(ins, k + j).u32(), v[j]);
+    for (int j = 0; j < 8; ++j) make(term<F>(ins, k + j).u32(), v[j]);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) R.wr(Term<F>(ins, k + j).reg(), v[j]);
+    for (int j = 0; j < 8; ++j) R.wr(term<F>(ins, k + j).reg(), v[j]);
   }
   for (; k < n; ++k) {
     F v[NC];
-    make(Term<F>(ins, k).u32(), v);
-    R.wr(Term<F>(ins, k).reg(), v);
+    make(term<F>(ins, k).u32(), v);
+    R.wr(term<F>(ins, k).reg(), v);
   }
 }
 
@@ -394,9 +433,9 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
   const F* X = reinterpret_cast<const F*>(L.x);
   F* Y = reinterpret_cast<F*>(L.y);
   F* G = reinterpret_cast<F*>(L.g);
-  run_stream(L, [&](AS3 const uint32_t* win, uint32_t pos, uint32_t w0) {
-    const Ins<F> I{win + pos};
-    AS3 const uint32_t* ip = win + pos;
+  run_stream(L, [&](auto win, uint32_t pos, uint32_t w0) {
+    const Ins<F, decltype(win)> I{win + pos};
+    const auto ip = win + pos;
     const uint32_t op = w0 & 0xFFu, fl = (w0 >> 8) & 0xFFFFu;
     if (op < 64u) {
       F a, b = F(0), r;
@@ -411,20 +450,20 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
       if (have) R.rd(dst, &acc);
       uint32_t k = 0;
       while (k < n) {
-        const uint32_t m = uni(Term<F>(ip, k).meta());
+        const uint32_t m = uni(term<F>(ip, k).meta());
         if (((m >> M_RUN_SHIFT) & 0xFFu) >= 8) {
           F v[8];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) R.rd(Term<F>(ip, k + j).reg(), &v[j]);
+          for (int j = 0; j < 8; ++j) R.rd(term<F>(ip, k + j).reg(), &v[j]);
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const F t = v[j] * Term<F>(ip, k + j).imm();
+            const F t = v[j] * term<F>(ip, k + j).imm();
             acc = (j == 0 && !have) ? t : acc + t;
           }
           have = true;
           k += 8;
         } else {
-          const Term<F> T(ip, k);
+          const auto T = term<F>(ip, k);
           const uint32_t kind = m & 3u;
           const F c = T.imm();
           F term = c;
@@ -470,7 +509,7 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
       const bool live = g != F(0);
       uint32_t k = 0;
       while (k < n) {
-        const uint32_t m0 = uni(Term<F>(ip, k).meta());
+        const uint32_t m0 = uni(term<F>(ip, k).meta());
         if (((m0 >> M_RUN_SHIFT) & 0xFFu) >= 8) {
           // eight product terms with distinct adjoint registers; a first
           // contribution reads its register anyway and discards the value
@@ -480,7 +519,7 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
           uint32_t rj[8], mj[8];
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const Term<F> T(ip, k + j);
+            const auto T = term<F>(ip, k + j);
             rj[j] = T.reg();
             mj[j] = T.meta();
             cj[j] = T.imm();
@@ -495,7 +534,7 @@ __device__ __forceinline__ void scalar_body(const Launch& L, const Regs<F, 1, LD
           }
           k += 8;
         } else {
-          const Term<F> T(ip, k);
+          const auto T = term<F>(ip, k);
           const F c = T.imm();
           F cur = F(0);
           if (!(m0 & M_FIRST)) R.rd(T.reg(), &cur);
@@ -614,9 +653,9 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
   F* H = reinterpret_cast<F*>(L.h);
   auto rdd = [&](uint32_t r) { Dual<F> d; F t[2]; R.rd(r, t); d.re = t[0]; d.eps = t[1]; return d; };
   auto wrd = [&](uint32_t r, Dual<F> d) { F t[2] = {d.re, d.eps}; R.wr(r, t); };
-  run_stream(L, [&](AS3 const uint32_t* win, uint32_t pos, uint32_t w0) {
-    const Ins<F> I{win + pos};
-    AS3 const uint32_t* ip = win + pos;
+  run_stream(L, [&](auto win, uint32_t pos, uint32_t w0) {
+    const Ins<F, decltype(win)> I{win + pos};
+    const auto ip = win + pos;
     const uint32_t op = w0 & 0xFFu, fl = (w0 >> 8) & 0xFFFFu;
     if (op < 64u) {
       Dual<F> a, b = dc(F(0));
@@ -630,7 +669,7 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
       bool have = (fl & F_DOT_CONT) != 0;
       if (have) acc = rdd(dst);
       for (uint32_t k = 0; k < n; ++k) {
-        const Term<F> T(ip, k);
+        const auto T = term<F>(ip, k);
         const uint32_t m = uni(T.meta()), kind = m & 3u;
         const Dual<F> c = dc(T.imm());
         Dual<F> term = c;
@@ -670,7 +709,7 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
       Dual<F> gd = g;
       uint32_t depth = 0;
       for (uint32_t k = 0; k < n; ++k) {
-        const Term<F> T(ip, k);
+        const auto T = term<F>(ip, k);
         const uint32_t m = uni(T.meta()), kind = m & 3u, tdepth = m >> M_DEPTH_SHIFT;
         const F c = T.imm();
         if (tdepth < depth) { gd = g; depth = 0; }
@@ -716,9 +755,9 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
   const F* X = reinterpret_cast<const F*>(L.x);
   const F* V = reinterpret_cast<const F*>(L.dirs);
   F* Y = reinterpret_cast<F*>(L.y);
-  run_stream(L, [&](AS3 const uint32_t* win, uint32_t pos, uint32_t w0) {
-    const Ins<F> I{win + pos};
-    AS3 const uint32_t* ip = win + pos;
+  run_stream(L, [&](auto win, uint32_t pos, uint32_t w0) {
+    const Ins<F, decltype(win)> I{win + pos};
+    const auto ip = win + pos;
     const uint32_t op = w0 & 0xFFu, fl = (w0 >> 8) & 0xFFFFu;
     if (op < 64u) {
       Jet<F, K> a, b = jc<F, K>(F(0));
@@ -733,19 +772,19 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
       if (have) R.rd(dst, acc.c);
       uint32_t k = 0;
       while (k < n) {
-        const uint32_t m0 = uni(Term<F>(ip, k).meta());
+        const uint32_t m0 = uni(term<F>(ip, k).meta());
         const uint32_t blk = (((m0 >> M_RUN_SHIFT) & 0xFFu) >= 4) ? 4u : 1u;
         Jet<F, K> v[4];
         if (blk == 4) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) R.rd(Term<F>(ip, k + j).reg(), v[j].c);
+          for (int j = 0; j < 4; ++j) R.rd(term<F>(ip, k + j).reg(), v[j].c);
         } else if ((m0 & 3u) != T_IMM) {
-          R.rd(Term<F>(ip, k).reg(), v[0].c);
+          R.rd(term<F>(ip, k).reg(), v[0].c);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           if ((uint32_t)j >= blk) break;
-          const Term<F> T(ip, k + j);
+          const auto T = term<F>(ip, k + j);
           const uint32_t m = uni(T.meta()), kind = m & 3u;
           const Jet<F, K> c = jc<F, K>(T.imm());
           Jet<F, K> term = c;
@@ -798,7 +837,7 @@ static hipError_t launch_common(const void* fn, const Launch& L, uint32_t wg, si
   uint64_t grid = (L.B + wg - 1) / wg;
   if (grid == 0) grid = 1;
   void* args[] = {const_cast<Launch*>(&L)};
-  return hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(wg), args, lds + 2 * WIN_BYTES, s);
+  return hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(wg), args, lds + WIN_LDS, s);
 }
 
 #define EHIP_PICK(KERNEL, ...)                                                                     \
@@ -821,10 +860,10 @@ hipError_t launch_queue(bool f64, const Queue& Q, uint32_t grid, uint32_t wg, si
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   void* args[] = {const_cast<Queue*>(&Q)};
-  return hipLaunchKernel(queue_kernel(f64), dim3(grid), dim3(wg), args, lds + 2 * WIN_BYTES, s);
+  return hipLaunchKernel(queue_kernel(f64), dim3(grid), dim3(wg), args, lds + WIN_LDS, s);
 }
 
-size_t queue_lds_bytes(size_t reg_lds) { return reg_lds + 2 * WIN_BYTES; }
+size_t queue_lds_bytes(size_t reg_lds) { return reg_lds + WIN_LDS; }
 
 hipError_t launch_dual(bool f64, const Launch& L, uint32_t wg, size_t lds, hipStream_t s) {
   const bool all_lds = L.gregs == nullptr;
@@ -843,4 +882,5 @@ hipError_t launch_jet(bool f64, int K, const Launch& L, uint32_t wg, size_t lds,
   }
 }
 
+}  // namespace EHIP_NS
 }  // namespace ehip

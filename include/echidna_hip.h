@@ -187,6 +187,11 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
  * per-workgroup LDS budget for the register file in bytes.  0 = default. */
 int32_t ehip_set_launch(ehip_ctx* ctx, uint32_t wg_size, uint32_t lds_budget_bytes);
 
+/* Program fetch for untuned launches and the multi-tape queue: 0 stages the
+ * instruction stream in LDS windows per workgroup, 1 reads it through the
+ * scalar cache per wave (no window LDS).  Results are identical. */
+int32_t ehip_set_fetch(ehip_ctx* ctx, int32_t mode);
+
 /* One job of a multi-tape queue: `batch` points of `tape`, inputs [batch][nin]
  * on the device; what = 0 forward (d_y [batch][nout]) or 1 gradient (d_g
  * [batch][nin], d_y optional). */
@@ -214,13 +219,14 @@ int32_t ehip_multi_batch_dev(ehip_ctx* ctx, const ehip_job* jobs, uint32_t njobs
  * shape -- workgroup size 64/128/256 x LDS budget for the register file --
  * on the caller's device buffers for evaluation `what` (0 forward, 1
  * gradient, 2 hvp, 3 taylor of `order`), keeps the fastest on the tape for
- * later calls of that kind, and returns out3 = {workgroup size, LDS budget
- * bytes, microseconds per launch}.  Buffers are as for the matching *_dev
+ * later calls of that kind, and returns out4 = {workgroup size, LDS budget
+ * bytes, microseconds per launch, program fetch (0 LDS, 1 scalar)}.  The
+ * shapes are workgroup size 64/128/256 x LDS budget x program fetch.  Buffers are as for the matching *_dev
  * entry point (d_y may be null for gradient/hvp); their contents are
  * overwritten.  Runs on the context stream and synchronizes. */
 int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t order, const void* d_x,
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
-                          uint32_t* out3);
+                          uint32_t* out4);
 
 /* hipGraph capture of repeated evaluations (north star; no counterpart in
  * the reference).  Records `repeats` launches of evaluation `what` (as for
@@ -236,9 +242,11 @@ int32_t ehip_graph_launch(ehip_graph* graph, void* stream);
 void ehip_graph_destroy(ehip_graph* graph);
 
 /* Restore (wg != 0) or clear (wg == 0) a tuned launch shape, e.g. from a
- * persisted autotuner cache; ehip_get_tuned reads it back (out2 = {wg, lds}). */
-int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget);
-int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out2);
+ * persisted autotuner cache; ehip_get_tuned reads it back
+ * (out3 = {wg, lds, fetch}). */
+int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
+                       uint32_t fetch);
+int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out3);
 
 /* DENSE grouping for tapes uploaded after this call (default on): fused dot
  * products sharing one operand list run as one register-blocked instruction

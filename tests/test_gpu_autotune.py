@@ -1,7 +1,9 @@
-"""Autotuner and launch shapes: every workgroup size and LDS budget (all
-registers in LDS, split LDS/HBM, all in HBM) computes bit-identical results,
-because each lane runs the same arithmetic in the same order; only where the
-registers live changes.  The tuned choice is kept per tape and evaluation kind.
+"""Autotuner and launch shapes: every workgroup size, LDS budget (all
+registers in LDS, split LDS/HBM, all in HBM) and program fetch (LDS windows
+or the scalar cache) computes bit-identical results, because each lane runs
+the same arithmetic in the same order; only where the registers and the
+instruction words live changes.  The tuned choice is kept per tape and
+evaluation kind.
 """
 
 import numpy as np
@@ -15,7 +17,8 @@ from echidna_amd.tape import GpuError, GpuTapeData
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(64, 0), (64, 1024), (128, 8192), (256, 16384), (256, 131072), (128, 65536)]
+SHAPES = [(64, 0, 0), (64, 1024, 1), (128, 8192, 0), (256, 16384, 1), (256, 131072, 0), (128, 65536, 1),
+          (256, 0, 1), (64, 65536, 0)]
 
 
 @pytest.mark.parametrize("f64", [False, True])
@@ -27,10 +30,10 @@ def test_launch_shapes_are_bit_identical(hip, f64):
     x = wl.points_normal(8, B, 12).astype(F)
     v = philox.normal(9, philox.STREAM_DIRS, np.arange(B * 12)).reshape(B, 12).astype(F)
     ref = None
-    for wg, lds in SHAPES:
-        HipContext.set_tuned(bufs, TUNE_GRADIENT, wg, lds)
-        HipContext.set_tuned(bufs, TUNE_HVP, wg, lds)
-        HipContext.set_tuned(bufs, TUNE_TAYLOR, wg, lds, order=3)
+    for wg, lds, fetch in SHAPES:
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, wg, lds, fetch=fetch)
+        HipContext.set_tuned(bufs, TUNE_HVP, wg, lds, fetch=fetch)
+        HipContext.set_tuned(bufs, TUNE_TAYLOR, wg, lds, order=3, fetch=fetch)
         y, g = hip.gradient_batch(bufs, x.reshape(-1), B)
         gh, hv = hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B)
         tk = hip.taylor_forward_kth_batch(bufs, x.reshape(-1), v.reshape(-1), B, 3)
@@ -39,7 +42,7 @@ def test_launch_shapes_are_bit_identical(hip, f64):
             ref = out
         else:
             for a, b in zip(out, ref):
-                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (wg, lds)
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (wg, lds, fetch)
 
 
 def test_autotune_keeps_choice_and_results(hip):
@@ -51,8 +54,9 @@ def test_autotune_keeps_choice_and_results(hip):
     g = torch.empty(B, 16, dtype=torch.float32, device="cuda")
     y0, g0 = hip.gradient_batch(bufs, x.cpu().numpy().reshape(-1), B)
     res = hip.autotune_dev(bufs, TUNE_GRADIENT, B, x, d_y=y, d_g=g)
-    assert res["wg"] in (64, 128, 256) and res["us"] > 0
-    assert HipContext.get_tuned(bufs, TUNE_GRADIENT) == {"wg": res["wg"], "lds_budget": res["lds_budget"]}
+    assert res["wg"] in (64, 128, 256) and res["us"] > 0 and res["fetch"] in (0, 1)
+    assert HipContext.get_tuned(bufs, TUNE_GRADIENT) == {"wg": res["wg"], "lds_budget": res["lds_budget"],
+                                                         "fetch": res["fetch"]}
     y1, g1 = hip.gradient_batch(bufs, x.cpu().numpy().reshape(-1), B)
     assert np.array_equal(y0, y1) and np.array_equal(g0, g1)
     HipContext.set_tuned(bufs, TUNE_GRADIENT, 0, 0)
@@ -66,6 +70,10 @@ def test_autotune_rejects_bad_arguments(hip):
         HipContext.set_tuned(bufs, TUNE_GRADIENT, 96, 0)
     with pytest.raises(GpuError, match="unknown evaluation kind"):
         HipContext.set_tuned(bufs, 7, 64, 0)
+    with pytest.raises(GpuError, match="program fetch"):
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=2)
+    with pytest.raises(GpuError, match="program fetch"):
+        hip.set_fetch(3)
     x = torch.zeros(4, 4, device="cuda")
     with pytest.raises(GpuError, match="taylor order"):
         hip.autotune_dev(bufs, TUNE_TAYLOR, 4, x, d_y=x, d_dirs=x, order=9)

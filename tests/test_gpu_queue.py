@@ -32,8 +32,8 @@ def upload(hip, tapes, f64):
 
 
 @pytest.mark.parametrize("f64", [False, True])
-@pytest.mark.parametrize("wg,lds", [(256, 0), (64, 4096), (128, 65536)])
-def test_queue_matches_per_tape_launches(hip, f64, wg, lds):
+@pytest.mark.parametrize("wg,lds,fetch", [(256, 0, 0), (64, 4096, 0), (128, 65536, 0), (256, 0, 1), (128, 16384, 1)])
+def test_queue_matches_per_tape_launches(hip, f64, wg, lds, fetch):
     tapes, xs = suite(0.04, 24, f64)
     bufs = upload(hip, tapes, f64)
     tdt = torch.float64 if f64 else torch.float32
@@ -45,7 +45,11 @@ def test_queue_matches_per_tape_launches(hip, f64, wg, lds):
         dg = torch.full((B, n), float("nan"), dtype=tdt, device="cuda") if k % 3 else None
         jobs.append((b, 1 if dg is not None else 0, dx, B, dy, dg))
         outs.append((dx, dy, dg))
-    hip.multi_batch_dev(jobs, wg=wg, lds_budget=lds)
+    hip.set_fetch(fetch)
+    try:
+        hip.multi_batch_dev(jobs, wg=wg, lds_budget=lds)
+    finally:
+        hip.set_fetch(0)
     for (b, what, dx, B, dy, dg), x, tape in zip(jobs, xs, tapes):
         ry = torch.empty_like(dy)
         if what == 1:
