@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--batch-log2", type=int, default=20)
     ap.add_argument("--wg", type=int, default=0)
     ap.add_argument("--lds", type=int, default=0)
+    ap.add_argument("--fetch", type=int, choices=[0, 1], default=0,
+                    help="with --wg/--lds: program fetch, 0 LDS windows, 1 scalar cache")
     ap.add_argument("--no-fuse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=32768)
@@ -358,6 +360,7 @@ def main() -> int:
         return 1
     if args.wg or args.lds:
         ctx.set_launch(args.wg, args.lds)
+        ctx.set_fetch(args.fetch)
     if args.no_fuse:
         ctx.set_fusion(False)
     data = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
@@ -418,12 +421,13 @@ def main() -> int:
     achieved = bytes_per_point * B / (kernel_ms * 1e-3)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_mlp32_fwdgrad_{args.dtype}.json")
-    launch = tuned or {"wg": args.wg, "lds_budget": args.lds}
+    launch = tuned or {"wg": args.wg, "lds_budget": args.lds, "fetch": args.fetch}
     if os.path.exists(tpath) and args.batch_log2 == 20:
         with open(tpath) as f:
             prof = json.load(f)
         # counters hold for the launch shape they were taken with
-        if prof.get("launch") == {"wg": launch.get("wg"), "lds_budget": launch.get("lds_budget")}:
+        shape = {k: launch.get(k) for k in ("wg", "lds_budget", "fetch")}
+        if prof.get("launch") == shape:
             traffic = prof.get("hbm_bytes_per_launch")
 
     out = {
