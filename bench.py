@@ -90,6 +90,22 @@ def lap_rate(var: np.ndarray, lap: np.ndarray, t_sample: float) -> dict:
             "median_point_laplacians_per_s": float(1.0 / (np.median(sr) * t_sample))}
 
 
+def tune_jets(ctx, bufs, X, lanes: int) -> dict:
+    """Autotune the order-3 jet launch (the STDE samples) on `lanes` lanes:
+    rows of X repeated, Gaussian directions."""
+    import torch
+
+    from echidna_amd.backend import TUNE_TAYLOR
+
+    n = X.shape[1]
+    rows = X[torch.arange(lanes, device=X.device) % X.shape[0]].contiguous()
+    dirs = torch.randn(lanes, n, dtype=X.dtype, device=X.device)
+    coeff = torch.empty(lanes, bufs.num_outputs * 3, dtype=X.dtype, device=X.device)
+    res = ctx.autotune_dev(bufs, TUNE_TAYLOR, lanes, rows, d_y=coeff, d_dirs=dirs, order=3)
+    torch.cuda.synchronize()
+    return res
+
+
 def timed(fn, reps: int):
     import torch
 
@@ -121,11 +137,13 @@ def run_stde(args) -> dict:
     ctx = HipContext.new(0)
     bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
     X = torch.from_numpy(wl.points_normal(2, P, n).astype(np.float32)).cuda()
+    tuned = tune_jets(ctx, bufs, X, 1 << 18)
     out = {"metric": "Laplacians/s at 1e-3 rel err", "unit": "Laplacians/s", "n_gpus": 1, "higher_is_better": True,
            "vs_baseline": None, "dtype": "f32 jets, f64 reduction",
            "data": "synthetic: Philox points, random-init weights baked into the tape",
            "config": {"workload": f"plain GPU STDE, tanh-MLP tape n={n} H={H} seed 2 ({tape.counts()['N']} "
-                                  f"computed entries), B={P} points, S={S}", "points": P, "S": S}}
+                                  f"computed entries), B={P} points, S={S}", "points": P, "S": S,
+                       "launch": tuned}}
     # exact traces on a subset: f64 tape, n basis jets per point
     k = min(P, 1024)
     b64 = ctx.upload_tape(GpuTapeData.from_tape_f64(tape))
@@ -184,6 +202,7 @@ def run_cv(args) -> dict:
     bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
     t_upload = time.perf_counter() - t0
     X = torch.from_numpy(wl.points_normal(4, P, n).astype(np.float32)).cuda()
+    tuned = tune_jets(ctx, bufs, X, 1 << 16)
     print(f"# recorded {tape.counts()['N']} entries in {t_record:.1f} s, uploaded in {t_upload:.1f} s",
           file=sys.stderr, flush=True)
     dt_plain, plain = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, lcv.DIR_GAUSSIAN, 5), 1)
@@ -231,7 +250,7 @@ def run_cv(args) -> dict:
             "dtype": "f32 jets, f64 reduction, bf16 MFMA CV", "data": "synthetic: Philox points and tuples",
             "config": {"workload": f"tanh-MLP tape n={n} 2x{H} seed 4 ({tape.counts()['N']} computed entries), "
                                    f"B={P} points, S={S} Gaussian, CV {n}->256->256->{n}x{r} signed {r // 2}+{r // 2}",
-                       "points": P, "S": S, "rank": r, "tuples": T, "epochs": args.epochs},
+                       "points": P, "S": S, "rank": r, "tuples": T, "epochs": args.epochs, "launch": tuned},
             "plain": {**rate_plain, "seconds": dt_plain, "t_sample_ns": t_s_plain * 1e9,
                       "mean_var": float(np.mean(res_pl[:, 2]))},
             "learned_cv": {**rate_cv, "seconds": dt_cv, "t_sample_ns": t_s_cv * 1e9,

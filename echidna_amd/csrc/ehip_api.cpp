@@ -61,11 +61,11 @@ void ehip_ctx_destroy(ehip_ctx* c) {
 
 int32_t ehip_set_launch(ehip_ctx* c, uint32_t wg, uint32_t lds_budget) {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
-  if (wg == 0) wg = 64;
+  if (wg == 0) wg = 256;
   if (wg != 64 && wg != 128 && wg != 256) return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
   std::lock_guard<std::mutex> lk(c->mu);
   c->wg = wg;
-  c->lds_budget = lds_budget ? lds_budget : 32768;
+  c->lds_budget = lds_budget ? lds_budget : 24576;
   return ok();
 }
 

@@ -84,8 +84,10 @@ struct ehip_ctx {
   DevBuf tup[4];               // control-variate tuple scratch
   DevBuf cvx, cvl;             // control-variate inputs and factors
   uint32_t ncu = 256;
-  uint32_t wg = 64;
-  uint32_t lds_budget = 32768;
+  // untuned launches: 256-lane workgroups and 24 KB of register LDS, the
+  // shape the autotuner picks for the MLP fwd+grad and order-3 jet programs
+  uint32_t wg = 256;
+  uint32_t lds_budget = 24576;
   uint32_t fetch = 0;  // program fetch of untuned launches and the queue (0 LDS, 1 scalar)
   bool fuse = true;
   bool dense = true;

@@ -183,8 +183,9 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
                              int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
                              uint64_t terms_cap, uint32_t* pool, uint64_t pool_cap, uint64_t* out5);
 
-/* Launch tuning knobs (autotuner): workgroup size (64/128/256) and the
- * per-workgroup LDS budget for the register file in bytes.  0 = default. */
+/* Launch shape of untuned launches: workgroup size (64/128/256) and the
+ * per-workgroup LDS budget for the register file in bytes.  0 = default
+ * (256 lanes, 24576 bytes). */
 int32_t ehip_set_launch(ehip_ctx* ctx, uint32_t wg_size, uint32_t lds_budget_bytes);
 
 /* Program fetch for untuned launches and the multi-tape queue: 0 stages the
