@@ -144,7 +144,7 @@ def run_stde(args) -> dict:
     ctx = HipContext.new(0)
     bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
     X = torch.from_numpy(wl.points_normal(2, P, n).astype(np.float32)).cuda()
-    tuned = tune_jets(ctx, bufs, X, 1 << 18)
+    tuned = tune_jets(ctx, bufs, X, 1 << 20)
     out = {"metric": "Laplacians/s at 1e-3 rel err", "unit": "Laplacians/s", "n_gpus": 1, "higher_is_better": True,
            "vs_baseline": None, "dtype": "f32 jets, f64 reduction",
            "data": "synthetic: Philox points, random-init weights baked into the tape",

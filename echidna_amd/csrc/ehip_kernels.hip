@@ -238,45 +238,38 @@ __device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, P ins
 // Value programs.  One term of a block of JB members (FULL: exactly JB, no
 // per-member bounds checks): FIRST initialises the accumulators with it,
 // otherwise it is added.  Pool words and weights arrive through scalar loads.
+// Every kind runs the same straight-line x * w: the pool holds weight 1 for a
+// plain register term, and an immediate term uses x = 1; x * 1 and 1 * c are
+// exact, so each member sees the bits its DOT would produce, and the
+// accumulators stay in place across terms.
 template <class F, int JB, bool FULL, bool FIRST, class Regs_>
-__device__ __forceinline__ void dense_fwd_term(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
-                                               uint32_t p, uint32_t j0, uint32_t jb, F (&acc)[JB]) {
-  const uint32_t kr = PR[p];
-  const uint32_t kind = kr >> P_KIND_SHIFT;
+__device__ __forceinline__ void dense_fwd_term(const Regs_& R, uint32_t kr, AS4 const F* PW, uint32_t m, uint32_t p,
+                                               uint32_t j0, uint32_t jb, F (&acc)[JB]) {
   AS4 const F* w = PW + (uint64_t)p * m + j0;
-  if (kind == T_PROD) {
-    F x;
-    R.rd(kr & P_REG_MASK, &x);
+  F x = F(1);
+  if ((kr >> P_KIND_SHIFT) != T_IMM) R.rd(kr & P_REG_MASK, &x);
 #pragma unroll
-    for (int j = 0; j < JB; ++j) {
-      if (!(FULL || (uint32_t)j < jb)) continue;
-      const F term = x * w[j];
-      acc[j] = FIRST ? term : acc[j] + term;
-    }
-  } else if (kind == T_REG) {
-    F x;
-    R.rd(kr & P_REG_MASK, &x);
-#pragma unroll
-    for (int j = 0; j < JB; ++j) {
-      if (!(FULL || (uint32_t)j < jb)) continue;
-      acc[j] = FIRST ? x : acc[j] + x;
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < JB; ++j) {
-      if (!(FULL || (uint32_t)j < jb)) continue;
-      acc[j] = FIRST ? w[j] : acc[j] + w[j];
-    }
+  for (int j = 0; j < JB; ++j) {
+    if (!(FULL || (uint32_t)j < jb)) continue;
+    const F term = x * w[j];
+    acc[j] = FIRST ? term : acc[j] + term;
   }
 }
 
-// The first term peeled, so the loop body is a plain acc + term.
+// The first term peeled, so the loop body is a plain acc + term; each
+// term's operand word is loaded one term ahead.
 template <class F, int JB, bool FULL, class Regs_>
 __device__ __forceinline__ void dense_fwd_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
                                                 uint32_t T, uint32_t j0, uint32_t jb) {
   F acc[JB];
-  dense_fwd_term<F, JB, FULL, true>(R, PR, PW, m, 0, j0, jb, acc);
-  for (uint32_t p = 1; p < T; ++p) dense_fwd_term<F, JB, FULL, false>(R, PR, PW, m, p, j0, jb, acc);
+  uint32_t kr = PR[0];
+  uint32_t next = PR[T > 1 ? 1 : 0];
+  dense_fwd_term<F, JB, FULL, true>(R, kr, PW, m, 0, j0, jb, acc);
+  for (uint32_t p = 1; p < T; ++p) {
+    kr = next;
+    next = PR[p + 1 < T ? p + 1 : p];
+    dense_fwd_term<F, JB, FULL, false>(R, kr, PW, m, p, j0, jb, acc);
+  }
 #pragma unroll
   for (int j = 0; j < JB; ++j) {
     if (!(FULL || (uint32_t)j < jb)) continue;
