@@ -354,52 +354,52 @@ __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, 
   }
 }
 
-// One block of IB adjoints.  Fast path, taken per member when the compiler
-// found every term a product with a non-zero weight (D_ALL_PROD_NZ) and the
-// member's adjoint is finite and non-zero in every lane: then the reference's
-// zero-adjoint skip and zero-multiplier guard cannot trigger and the
-// contribution is c * g unconditionally.
+// One block of IB adjoints.  The reference skips an op whose adjoint is
+// zero and a contribution whose partial (here the weight; 1 for a plain
+// register term) is zero; every other contribution is cur + c * g, Inf and
+// NaN adjoints included.  Fast path, taken per member when the compiler
+// found every weight non-zero (D_ALL_PROD_NZ) and the member's adjoint is
+// non-zero in every lane: then no guard can trigger.  Otherwise each
+// contribution is a select between cur + c * g and cur.  The next member's
+// register index is loaded one member ahead.
 template <class F, int IB, bool FULL, bool LDSONLY>
 __device__ __forceinline__ void dense_rev_block(const Regs<F, 1, LDSONLY>& R, AS4 const uint32_t* PR, AS4 const F* PW,
                                                 uint32_t m, uint32_t T, uint32_t i0, uint32_t ib, bool fastw) {
   F cur[IB];
-  uint32_t kr[IB];
 #pragma unroll
   for (int i = 0; i < IB; ++i) {
     cur[i] = F(0);
-    kr[i] = 0;
     if (!(FULL || (uint32_t)i < ib)) continue;
-    kr[i] = PR[i0 + i];
-    if (!(kr[i] & P_FIRST)) R.rd(kr[i] & P_REG_MASK, &cur[i]);
+    const uint32_t kr = PR[i0 + i];
+    if (!(kr & P_FIRST)) R.rd(kr & P_REG_MASK, &cur[i]);
   }
+  uint32_t next = PR[T];
   for (uint32_t j = 0; j < m; ++j) {
     F g;
-    R.rd(PR[T + j], &g);
+    R.rd(next, &g);
+    next = PR[T + (j + 1 < m ? j + 1 : j)];
     AS4 const F* w = PW + (uint64_t)j * T + i0;
-    if (fastw && __all(g != F(0) && isfinite(g))) {
+    const bool live = g != F(0);
+    if (fastw && __all(live)) {
 #pragma unroll
       for (int i = 0; i < IB; ++i) {
         if (!(FULL || (uint32_t)i < ib)) continue;
         cur[i] = cur[i] + w[i] * g;
       }
     } else {
-      const bool live = g != F(0);
 #pragma unroll
       for (int i = 0; i < IB; ++i) {
         if (!(FULL || (uint32_t)i < ib)) continue;
         const F c = w[i];
-        if ((kr[i] >> P_KIND_SHIFT) == T_PROD) {
-          if (live && c != F(0)) cur[i] = cur[i] + c * g;
-        } else if (live) {
-          cur[i] = cur[i] + F(1) * g;
-        }
+        const F t = cur[i] + c * g;
+        cur[i] = (live && c != F(0)) ? t : cur[i];
       }
     }
   }
 #pragma unroll
   for (int i = 0; i < IB; ++i) {
     if (!(FULL || (uint32_t)i < ib)) continue;
-    R.wr(kr[i] & P_REG_MASK, &cur[i]);
+    R.wr(PR[i0 + i] & P_REG_MASK, &cur[i]);
   }
 }
 

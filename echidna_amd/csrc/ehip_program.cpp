@@ -883,7 +883,10 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
           for (size_t b = 0; b < (x.op == I_DENSE ? m : T); ++b) {
             const size_t j = x.op == I_DENSE ? b : a, q = x.op == I_DENSE ? a : b;
             double v = x.W[j * T + q];
-            if (x.terms[q].kind == T_REG) v = 1.0;
+            // a plain register term adds its operand (forward) or the whole
+            // adjoint (reverse): weight 1, so the interpreters run c * x or
+            // c * g for every kind
+            if (x.terms[q].kind == T_REG || (x.op == I_DENSE_REV && x.terms[q].kind != T_PROD)) v = 1.0;
             if (x.terms[q].kind != T_PROD) prod_nz = false;
             uint32_t lo = 0, hi = 0;
             put_imm(v, lo, hi);

@@ -100,3 +100,52 @@ def test_dense_grouping_is_bit_identical_on_device(hip, f64):
         outs.append([np.asarray(a).copy() for a in (y, g, *tk.coefficients)])
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_dense_guards_are_bit_identical_on_device(hip, f64):
+    """Zero, infinite and NaN adjoints and zero weights inside DENSE groups:
+    gradients with and without grouping agree bit for bit across both program
+    fetch modes, except the sign bit of NaN results, which IEEE 754 leaves
+    unspecified (the device gives +NaN on one path and -NaN on the other for
+    0 + c * NaN); NaN positions, Inf values and every finite value agree."""
+    import dense_cases as dc
+
+    tape = dc.tape()
+    F = np.float64 if f64 else np.float32
+    B = 500
+    x = dc.points(B, 3).astype(F)
+    outs = []
+    for dense in (True, False):
+        hip.set_dense(dense)
+        try:
+            bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+        finally:
+            hip.set_dense(True)
+        for fetch in (0, 1):
+            HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 24576, fetch=fetch)
+            y, g = hip.gradient_batch(bufs, x.reshape(-1), B)
+            outs.append((np.asarray(y).copy(), np.asarray(g).copy()))
+    g0 = outs[0][1]
+    g0r = g0.reshape(B, -1)
+    assert np.isinf(g0r).any() and np.isnan(g0r).any() and np.isfinite(g0r).all(axis=1).any()
+    U = np.uint64 if f64 else np.uint32
+    sign = U(1) << U(63 if f64 else 31)
+
+    def bits(a):  # NaN sign cleared
+        b = a.view(U).copy()
+        b[np.isnan(a)] &= ~sign
+        return b
+
+    for y, g in outs[1:]:
+        assert np.array_equal(bits(y), bits(outs[0][0]))
+        assert np.array_equal(bits(g), bits(g0))
+    if f64:  # and the same NaN / Inf pattern and values as the oracle
+        import pyoracle as po
+
+        with np.errstate(all="ignore"):
+            _, ref = po.gradient(tape, x.astype(np.float64))
+        assert np.array_equal(np.isnan(g0r), np.isnan(ref))
+        fin = np.isfinite(ref)
+        assert np.array_equal(g0r[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
+        assert np.max(np.abs(g0r[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-12

@@ -93,6 +93,30 @@ def test_dense_grouping_is_bit_identical(name, f64):
             assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (name, mode)
 
 
+@pytest.mark.parametrize("f64", [True, False])
+def test_dense_guards_are_bit_identical(f64):
+    """Zero and infinite adjoints, zero weights and NaN partials inside DENSE
+    groups: the grouped program matches the DOT-level one bit for bit."""
+    import dense_cases as dc
+
+    tape = dc.tape()
+    d = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
+    F = np.float64 if f64 else np.float32
+    x = dc.points(60).astype(F)
+    outs = []
+    for dense in (True, False):
+        words, terms, info = compile_program(d, mode=1, dense=dense, f64=f64)
+        ops = [int(w[0]) & 0xFF for w in words]
+        assert (emu.I_DENSE_REV in ops) == dense
+        with np.errstate(all="ignore"):
+            outs.append(emu.run(words, terms, x, tape.num_inputs, tape.num_outputs(), info["registers"], f64=f64,
+                                nterms=info["nterms"], pool=info["pool"]))
+    g = outs[0][1]
+    assert np.isinf(g).any() and np.isnan(g).any() and np.isfinite(g).all(axis=1).any()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
 def test_dense_needs_distinct_operands():
     """A dot product that uses one operand twice keeps its DOT form."""
     def f(v):
