@@ -48,7 +48,8 @@ namespace ehip {
 // A DENSE replaces m DOT instructions that share one term structure (same
 // kinds, same operand registers, all distinct; weights differ): member j
 // computes exactly what its DOT computed, term by term in the same order, so
-// results are bit-identical; the operand registers are read once per block of
+// results are bit-identical (a NaN's sign bit, which IEEE 754 leaves open,
+// aside); the operand registers are read once per block of
 // members instead of once per member.  DENSE_REV likewise replaces the m
 // DOT_REVs, applying member contributions to each adjoint in their original
 // order.  Pool term word: kind << 30 | first << 29 | const_left << 28 |
