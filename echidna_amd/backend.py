@@ -124,6 +124,11 @@ _SIGS = {
                                              _c_u64p]),
     "ehip_tape_upload_json": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_uint64,
                                                ctypes.POINTER(_vp)]),
+    "ehip_tape_to_cbor": (ctypes.c_int32, [ctypes.c_int32, _vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.c_uint32, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _c_u64p]),
+    "ehip_tape_from_cbor": (ctypes.c_int32, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int32, _vp, _vp, _vp, _vp,
+                                             ctypes.c_uint64, _vp, ctypes.c_uint64, _c_u64p, _c_u32p, _c_u32p,
+                                             _c_u64p]),
 }
 
 class EhipJob(ctypes.Structure):

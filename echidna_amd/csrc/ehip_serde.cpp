@@ -19,7 +19,12 @@
 // (src/bytecode_tape/mod.rs:575-703) with the reference's messages.
 // Decimal values are converted with correct rounding; an f32 tape's values
 // are parsed as f64 and then rounded to f32, as serde_json does for f32.
+//
+// The same data model is also read and written as CBOR (RFC 8949), the
+// binary form the reference round-trips through ciborium
+// (tests/serde.rs:156-170), with the same checks and messages.
 #include <clocale>
+#include <limits>
 #include <cmath>
 #include <cstdlib>
 #include <locale.h>
@@ -603,6 +608,364 @@ int32_t read_tape(const char* json, uint64_t len, TapeData& d) {
   return EHIP_OK;
 }
 
+// ---- CBOR (ciborium's layout of the same serde data model) ------------------
+//
+// RFC 8949 encoding as ciborium writes it: the struct as a definite map
+// with text keys in field order, unit variants as text strings, sequences
+// as definite arrays, unsigned integers in their shortest head, and floats
+// in the shortest of binary16 / binary32 / binary64 that holds the value
+// exactly (NaN as the binary16 quiet NaN 0x7e00).  Unlike JSON, CBOR
+// carries Inf and NaN, so such tapes round-trip.
+
+void cbor_head(std::string& o, uint32_t major, uint64_t v) {
+  const char m = (char)(major << 5);
+  if (v < 24) {
+    o.push_back((char)(m | (char)v));
+  } else if (v <= 0xFF) {
+    o.push_back((char)(m | 24));
+    o.push_back((char)v);
+  } else if (v <= 0xFFFF) {
+    o.push_back((char)(m | 25));
+    for (int s = 8; s >= 0; s -= 8) o.push_back((char)(v >> s));
+  } else if (v <= 0xFFFFFFFFull) {
+    o.push_back((char)(m | 26));
+    for (int s = 24; s >= 0; s -= 8) o.push_back((char)(v >> s));
+  } else {
+    o.push_back((char)(m | 27));
+    for (int s = 56; s >= 0; s -= 8) o.push_back((char)(v >> s));
+  }
+}
+
+void cbor_text(std::string& o, const char* t) {
+  const size_t n = std::strlen(t);
+  cbor_head(o, 3, n);
+  o.append(t, n);
+}
+
+// binary16 bits of v when v is exactly representable there.
+bool half_of(double v, uint16_t& h) {
+  const uint16_t sign = std::signbit(v) ? 0x8000 : 0;
+  const double a = std::fabs(v);
+  if (a == 0) { h = sign; return true; }
+  if (std::isinf(a)) { h = sign | 0x7C00; return true; }
+  int e;
+  const double fr = std::frexp(a, &e);  // a = fr * 2^e, fr in [0.5, 1)
+  const int ex = e - 1;                 // a = (2 fr) * 2^ex
+  if (ex >= -14 && ex <= 15) {          // normal: 10 fraction bits
+    const double mant = (2 * fr - 1) * 1024;
+    if (mant != std::floor(mant)) return false;
+    h = sign | (uint16_t)((ex + 15) << 10) | (uint16_t)mant;
+    return true;
+  }
+  if (ex < -14 && ex >= -24) {  // subnormal: a = k * 2^-24
+    const double k = std::ldexp(a, 24);
+    if (k != std::floor(k)) return false;
+    h = sign | (uint16_t)k;
+    return true;
+  }
+  return false;
+}
+
+double half_value(uint16_t h) {
+  const int e = (h >> 10) & 0x1F, f = h & 0x3FF;
+  double v;
+  if (e == 0) v = std::ldexp((double)f, -24);
+  else if (e == 31) v = f ? std::numeric_limits<double>::quiet_NaN() : std::numeric_limits<double>::infinity();
+  else v = std::ldexp(1.0 + f / 1024.0, e - 15);
+  return (h & 0x8000) ? -v : v;
+}
+
+void cbor_float(std::string& o, double v) {
+  uint16_t h;
+  if (std::isnan(v)) {
+    o.push_back((char)0xF9);
+    o.push_back((char)0x7E);
+    o.push_back((char)0x00);
+  } else if (half_of(v, h)) {
+    o.push_back((char)0xF9);
+    o.push_back((char)(h >> 8));
+    o.push_back((char)h);
+  } else if ((double)(float)v == v) {
+    const float f = (float)v;
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    o.push_back((char)0xFA);
+    for (int s = 24; s >= 0; s -= 8) o.push_back((char)(b >> s));
+  } else {
+    uint64_t b;
+    std::memcpy(&b, &v, 8);
+    o.push_back((char)0xFB);
+    for (int s = 56; s >= 0; s -= 8) o.push_back((char)(b >> s));
+  }
+}
+
+struct CborReader {
+  const uint8_t* b;
+  const uint8_t* p;
+  const uint8_t* e;
+  std::string err;
+
+  bool fail(const char* fmt, ...) {
+    char buf[300];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    char at[48];
+    std::snprintf(at, sizeof at, " at byte %zu", (size_t)(p - b));
+    err = std::string(buf) + at;
+    return false;
+  }
+  static const char* kind(uint32_t major) {
+    static const char* const k[8] = {"integer", "negative integer", "bytes", "string", "array", "map", "tag",
+                                     "float or simple value"};
+    return k[major & 7];
+  }
+  // Head of the next item: major type, argument, and whether the length is
+  // indefinite (additional information 31).
+  bool head(uint32_t& major, uint64_t& arg, bool& indef) {
+    if (p >= e) return fail("unexpected end of input");
+    const uint8_t ib = *p++;
+    major = ib >> 5;
+    const uint32_t ai = ib & 31;
+    indef = false;
+    if (ai < 24) { arg = ai; return true; }
+    if (ai == 31) {
+      if (major == 0 || major == 1 || major == 6) return fail("invalid indefinite length");
+      indef = true;
+      arg = 0;
+      return true;
+    }
+    if (ai > 27) return fail("reserved additional information %u", ai);
+    const int n = 1 << (ai - 24);
+    if (e - p < n) return fail("unexpected end of input");
+    arg = 0;
+    for (int i = 0; i < n; ++i) arg = (arg << 8) | *p++;
+    return true;
+  }
+  bool peek_break() { return p < e && *p == 0xFF; }
+  bool u32(uint32_t& v, const char* field) {
+    uint32_t mj; uint64_t a; bool ind;
+    if (!head(mj, a, ind)) return false;
+    if (mj == 1) return fail("invalid value: negative integer, expected u32 (field `%s`)", field);
+    if (mj != 0) return fail("invalid type: %s, expected u32 (field `%s`)", kind(mj), field);
+    if (a > 0xFFFFFFFFull) return fail("invalid value: integer `%llu`, expected u32 (field `%s`)",
+                                       (unsigned long long)a, field);
+    v = (uint32_t)a;
+    return true;
+  }
+  bool f64(double& v) {
+    const uint8_t* at = p;
+    uint32_t mj; uint64_t a; bool ind;
+    if (!head(mj, a, ind)) return false;
+    if (mj == 0) { v = (double)a; return true; }
+    if (mj == 1) { v = -1.0 - (double)a; return true; }
+    if (mj == 7) {
+      const uint8_t ai = *at & 31;
+      if (ai == 25) { v = half_value((uint16_t)a); return true; }
+      if (ai == 26) { uint32_t b = (uint32_t)a; float f; std::memcpy(&f, &b, 4); v = f; return true; }
+      if (ai == 27) { std::memcpy(&v, &a, 8); return true; }
+    }
+    return fail("invalid type: %s, expected f64", kind(mj));
+  }
+  bool text(std::string& s) {
+    uint32_t mj; uint64_t a; bool ind;
+    if (!head(mj, a, ind)) return false;
+    if (mj != 3) return fail("invalid type: %s, expected a string", kind(mj));
+    if (ind) return fail("indefinite-length strings are not supported");
+    if ((uint64_t)(e - p) < a) return fail("unexpected end of input");
+    s.assign((const char*)p, (size_t)a);
+    p += a;
+    return true;
+  }
+  bool skip(int depth) {
+    if (depth > 128) return fail("recursion limit exceeded");
+    uint32_t mj; uint64_t a; bool ind;
+    if (!head(mj, a, ind)) return false;
+    switch (mj) {
+      case 0: case 1: return true;
+      case 2: case 3:
+        if (ind) {
+          while (!peek_break()) if (!skip(depth + 1)) return false;
+          ++p;
+          return true;
+        }
+        if ((uint64_t)(e - p) < a) return fail("unexpected end of input");
+        p += a;
+        return true;
+      case 4: case 5: {
+        const uint64_t per = mj == 5 ? 2 : 1;
+        if (ind) {
+          while (!peek_break()) if (!skip(depth + 1)) return false;
+          ++p;
+          return true;
+        }
+        for (uint64_t i = 0; i < a * per; ++i) if (!skip(depth + 1)) return false;
+        return true;
+      }
+      case 6: return skip(depth + 1);
+      default: return true;  // simple values and floats: head only
+    }
+  }
+  // Items of an array (definite or indefinite), each read by elem().
+  template <typename Fn>
+  bool array(Fn&& elem, const char* field) {
+    uint32_t mj; uint64_t a; bool ind;
+    if (!head(mj, a, ind)) return false;
+    if (mj != 4) return fail("invalid type: %s, expected a sequence (field `%s`)", kind(mj), field);
+    if (ind) {
+      while (!peek_break()) if (!elem()) return false;
+      ++p;
+      return true;
+    }
+    // every item takes at least one byte
+    if (a > (uint64_t)(e - p))
+      return fail("unexpected end of input: %llu items declared for field `%s`", (unsigned long long)a, field);
+    for (uint64_t i = 0; i < a; ++i) if (!elem()) return false;
+    return true;
+  }
+  bool opcode(uint32_t& op) {
+    std::string name;
+    if (p < e && (*p >> 5) == 5) {  // {"Name": null}
+      uint32_t mj; uint64_t a; bool ind;
+      if (!head(mj, a, ind)) return false;
+      if (ind || a != 1) return fail("invalid length, expected a one-entry map for an enum");
+      if (!text(name)) return false;
+      if (p >= e || *p != 0xF6) return fail("invalid type, expected unit variant OpCode::%s", name.c_str());
+      ++p;
+    } else if (!text(name)) {
+      return false;
+    }
+    op = kNumOps;
+    for (uint32_t i = 0; i < kNumOps; ++i)
+      if (name == kOpName[i]) op = i;
+    if (op == kNumOps) return fail("unknown variant `%s`", name.c_str());
+    return true;
+  }
+};
+
+bool parse_tape_cbor(CborReader& r, TapeData& d) {
+  enum { F_OPC, F_ARGS, F_VALS, F_NIN, F_NVAR, F_OUT, F_OUTS, F_CSA, NF };
+  static const char* const names[NF] = {"opcodes",      "arg_indices",   "values",         "num_inputs",
+                                        "num_variables", "output_index", "output_indices", "custom_second_args"};
+  bool seen[NF] = {};
+  uint32_t mj; uint64_t n; bool ind;
+  if (!r.head(mj, n, ind)) return false;
+  if (mj != 5) return r.fail("invalid type: %s, expected struct TapeData", CborReader::kind(mj));
+  for (uint64_t k = 0; ind || k < n; ++k) {
+    if (ind && r.peek_break()) { ++r.p; break; }
+    std::string key;
+    if (!r.text(key)) return false;
+    int f = -1;
+    for (int i = 0; i < NF; ++i)
+      if (key == names[i]) f = i;
+    if (f < 0) {
+      if (!r.skip(1)) return false;
+      continue;
+    }
+    if (seen[f]) return r.fail("duplicate field `%s`", names[f]);
+    seen[f] = true;
+    bool ok = true;
+    switch (f) {
+      case F_OPC:
+        ok = r.array([&] { uint32_t op; if (!r.opcode(op)) return false; d.opcodes.push_back(op); return true; },
+                     "opcodes");
+        break;
+      case F_ARGS:
+        ok = r.array([&] {
+          uint32_t m2; uint64_t a2; bool i2;
+          if (!r.head(m2, a2, i2)) return false;
+          if (m2 != 4) return r.fail("invalid type: %s, expected an array of length 2", CborReader::kind(m2));
+          if (i2 || a2 != 2) return r.fail("invalid length, expected an array of length 2");
+          uint32_t a = 0, b = 0;
+          if (!r.u32(a, "arg_indices") || !r.u32(b, "arg_indices")) return false;
+          d.arg0.push_back(a);
+          d.arg1.push_back(b);
+          return true;
+        }, "arg_indices");
+        break;
+      case F_VALS:
+        ok = r.array([&] { double v; if (!r.f64(v)) return false; d.values.push_back(v); return true; }, "values");
+        break;
+      case F_NIN: ok = r.u32(d.num_inputs, "num_inputs"); break;
+      case F_NVAR: ok = r.u32(d.num_variables, "num_variables"); break;
+      case F_OUT: ok = r.u32(d.output_index, "output_index"); break;
+      case F_OUTS:
+        ok = r.array([&] { uint32_t v; if (!r.u32(v, "output_indices")) return false; d.output_indices.push_back(v);
+                           return true; }, "output_indices");
+        break;
+      case F_CSA: {
+        uint32_t m2; uint64_t a2; bool i2;
+        if (!r.head(m2, a2, i2)) return false;
+        if (m2 != 5) return r.fail("invalid type: %s, expected a map (field `custom_second_args`)",
+                                   CborReader::kind(m2));
+        for (uint64_t q = 0; i2 || q < a2; ++q) {
+          if (i2 && r.peek_break()) { ++r.p; break; }
+          uint32_t kk, vv;
+          if (!r.u32(kk, "custom_second_args") || !r.u32(vv, "custom_second_args")) return false;
+          if (d.n_custom_second++ == 0) { d.first_custom_key = kk; d.first_custom_val = vv; }
+        }
+        break;
+      }
+    }
+    if (!ok) return false;
+  }
+  for (int f = 0; f <= F_OUT; ++f)
+    if (!seen[f]) return r.fail("missing field `%s`", names[f]);
+  if (r.p != r.e) return r.fail("trailing data");
+  return true;
+}
+
+int32_t read_tape_cbor(const uint8_t* buf, uint64_t len, TapeData& d) {
+  if (!buf && len) return fail(EHIP_OTHER, "tape CBOR: null buffer");
+  CborReader r{buf, buf, buf + len, {}};
+  if (!parse_tape_cbor(r, d)) return fail(EHIP_OTHER, "tape CBOR: %s", r.err.c_str());
+  std::string msg = check_tape(d);
+  if (!msg.empty()) return fail(EHIP_OTHER, "tape CBOR: %s", msg.c_str());
+  return EHIP_OK;
+}
+
+// The tape's fields after the Custom-op refusal the serializer shares.
+int32_t check_writable(const uint32_t* opcodes, uint64_t n, const char* what) {
+  for (uint64_t i = 0; i < n; ++i) {
+    if (opcodes[i] == OP_CUSTOM_)
+      return fail(EHIP_CUSTOM_OPS_NOT_SUPPORTED,
+                  "cannot serialize a BytecodeTape containing custom ops; custom ops must be re-registered "
+                  "after deserialization");
+    if (opcodes[i] >= kNumOps)
+      return fail(EHIP_OTHER, "tape %s: opcodes[%llu] (%u) is not an OpCode", what, (unsigned long long)i, opcodes[i]);
+  }
+  return EHIP_OK;
+}
+
+int32_t copy_out(const TapeData& d, int32_t dtype, uint32_t* opcodes, uint32_t* arg0, uint32_t* arg1, void* values,
+                 uint64_t cap_entries, uint32_t* output_indices, uint64_t cap_outputs, uint64_t* num_entries,
+                 uint32_t* num_inputs, uint32_t* output_index, uint64_t* num_output_indices, const char* what) {
+  const uint64_t n = d.opcodes.size(), m = d.output_indices.size();
+  *num_entries = n;
+  *num_inputs = d.num_inputs;
+  *output_index = d.output_index;
+  *num_output_indices = m;
+  if (!opcodes && !arg0 && !arg1 && !values && !output_indices) return ok();  // sizing call
+  if (cap_entries < n || cap_outputs < m || (n && (!opcodes || !arg0 || !arg1 || !values)) ||
+      (m && !output_indices))
+    return fail(EHIP_OTHER, "tape %s: result buffers too small (%llu entries, %llu outputs)", what,
+                (unsigned long long)n, (unsigned long long)m);
+  if (n) {
+    std::memcpy(opcodes, d.opcodes.data(), n * 4);
+    std::memcpy(arg0, d.arg0.data(), n * 4);
+    std::memcpy(arg1, d.arg1.data(), n * 4);
+  }
+  if (dtype == EHIP_F64) {
+    if (n) std::memcpy(values, d.values.data(), n * 8);
+  } else {
+    float* v = (float*)values;
+    for (uint64_t i = 0; i < n; ++i) v[i] = (float)d.values[i];
+  }
+  if (m) std::memcpy(output_indices, d.output_indices.data(), m * 4);
+  return ok();
+}
+
 }  // namespace
 
 extern "C" {
@@ -616,14 +979,8 @@ int32_t ehip_tape_to_json(int32_t dtype, const uint32_t* opcodes, const uint32_t
   if (num_entries && (!opcodes || !arg0 || !arg1 || !values)) return fail(EHIP_OTHER, "tape JSON: null tape array");
   if (num_output_indices && !output_indices) return fail(EHIP_OTHER, "tape JSON: null output_indices");
   if (num_entries > 0xFFFFFFFFull) return fail(EHIP_OTHER, "tape JSON: more than u32::MAX entries");
-  for (uint64_t i = 0; i < num_entries; ++i) {
-    if (opcodes[i] == OP_CUSTOM_)
-      return fail(EHIP_CUSTOM_OPS_NOT_SUPPORTED,
-                  "cannot serialize a BytecodeTape containing custom ops; custom ops must be re-registered "
-                  "after deserialization");
-    if (opcodes[i] >= kNumOps) return fail(EHIP_OTHER, "tape JSON: opcodes[%llu] (%u) is not an OpCode",
-                                           (unsigned long long)i, opcodes[i]);
-  }
+  int32_t rc = check_writable(opcodes, num_entries, "JSON");
+  if (rc) return rc;
   std::string s;
   s.reserve(64 + num_entries * 48);
   s += "{\"opcodes\":[";
@@ -676,29 +1033,8 @@ int32_t ehip_tape_from_json(const char* json, uint64_t len, int32_t dtype, uint3
   TapeData d;
   int32_t rc = read_tape(json, len, d);
   if (rc) return rc;
-  const uint64_t n = d.opcodes.size(), m = d.output_indices.size();
-  *num_entries = n;
-  *num_inputs = d.num_inputs;
-  *output_index = d.output_index;
-  *num_output_indices = m;
-  if (!opcodes && !arg0 && !arg1 && !values && !output_indices) return ok();  // sizing call
-  if (cap_entries < n || cap_outputs < m || (n && (!opcodes || !arg0 || !arg1 || !values)) ||
-      (m && !output_indices))
-    return fail(EHIP_OTHER, "tape JSON: result buffers too small (%llu entries, %llu outputs)",
-                (unsigned long long)n, (unsigned long long)m);
-  if (n) {
-    std::memcpy(opcodes, d.opcodes.data(), n * 4);
-    std::memcpy(arg0, d.arg0.data(), n * 4);
-    std::memcpy(arg1, d.arg1.data(), n * 4);
-  }
-  if (dtype == EHIP_F64) {
-    if (n) std::memcpy(values, d.values.data(), n * 8);
-  } else {
-    float* v = (float*)values;
-    for (uint64_t i = 0; i < n; ++i) v[i] = (float)d.values[i];
-  }
-  if (m) std::memcpy(output_indices, d.output_indices.data(), m * 4);
-  return ok();
+  return copy_out(d, dtype, opcodes, arg0, arg1, values, cap_entries, output_indices, cap_outputs, num_entries,
+                  num_inputs, output_index, num_output_indices, "JSON");
 }
 
 int32_t ehip_tape_upload_json(ehip_ctx* ctx, int32_t dtype, const char* json, uint64_t len, ehip_tape** out) {
@@ -718,6 +1054,67 @@ int32_t ehip_tape_upload_json(ehip_ctx* ctx, int32_t dtype, const char* json, ui
   }
   return ehip_tape_upload(ctx, dtype, d.opcodes.data(), d.arg0.data(), d.arg1.data(), vals, n, d.num_inputs, n,
                           d.output_index, d.output_indices.data(), (uint32_t)d.output_indices.size(), out);
+}
+
+int32_t ehip_tape_to_cbor(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                          const void* values, uint64_t num_entries, uint32_t num_inputs, uint32_t output_index,
+                          const uint32_t* output_indices, uint64_t num_output_indices, uint8_t* buf, uint64_t cap,
+                          uint64_t* len) {
+  if (!len) return fail(EHIP_OTHER, "tape CBOR: null length pointer");
+  if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "tape CBOR: dtype must be EHIP_F32 or EHIP_F64");
+  if (num_entries && (!opcodes || !arg0 || !arg1 || !values)) return fail(EHIP_OTHER, "tape CBOR: null tape array");
+  if (num_output_indices && !output_indices) return fail(EHIP_OTHER, "tape CBOR: null output_indices");
+  if (num_entries > 0xFFFFFFFFull) return fail(EHIP_OTHER, "tape CBOR: more than u32::MAX entries");
+  int32_t rc = check_writable(opcodes, num_entries, "CBOR");
+  if (rc) return rc;
+  std::string o;
+  o.reserve(64 + num_entries * 24);
+  cbor_head(o, 5, 8);
+  cbor_text(o, "opcodes");
+  cbor_head(o, 4, num_entries);
+  for (uint64_t i = 0; i < num_entries; ++i) cbor_text(o, kOpName[opcodes[i]]);
+  cbor_text(o, "arg_indices");
+  cbor_head(o, 4, num_entries);
+  for (uint64_t i = 0; i < num_entries; ++i) {
+    cbor_head(o, 4, 2);
+    cbor_head(o, 0, arg0[i]);
+    cbor_head(o, 0, arg1[i]);
+  }
+  cbor_text(o, "values");
+  cbor_head(o, 4, num_entries);
+  for (uint64_t i = 0; i < num_entries; ++i)
+    cbor_float(o, dtype == EHIP_F64 ? ((const double*)values)[i] : (double)((const float*)values)[i]);
+  cbor_text(o, "num_inputs");
+  cbor_head(o, 0, num_inputs);
+  cbor_text(o, "num_variables");
+  cbor_head(o, 0, num_entries);
+  cbor_text(o, "output_index");
+  cbor_head(o, 0, output_index);
+  cbor_text(o, "output_indices");
+  cbor_head(o, 4, num_output_indices);
+  for (uint64_t j = 0; j < num_output_indices; ++j) cbor_head(o, 0, output_indices[j]);
+  cbor_text(o, "custom_second_args");
+  cbor_head(o, 5, 0);
+  *len = o.size();
+  if (!buf) return ok();
+  if (cap < o.size()) return fail(EHIP_OTHER, "tape CBOR: buffer too small (%llu bytes, need %llu)",
+                                  (unsigned long long)cap, (unsigned long long)o.size());
+  std::memcpy(buf, o.data(), o.size());
+  return ok();
+}
+
+int32_t ehip_tape_from_cbor(const uint8_t* cbor, uint64_t len, int32_t dtype, uint32_t* opcodes, uint32_t* arg0,
+                            uint32_t* arg1, void* values, uint64_t cap_entries, uint32_t* output_indices,
+                            uint64_t cap_outputs, uint64_t* num_entries, uint32_t* num_inputs,
+                            uint32_t* output_index, uint64_t* num_output_indices) {
+  if (!num_entries || !num_inputs || !output_index || !num_output_indices)
+    return fail(EHIP_OTHER, "tape CBOR: null result pointer");
+  if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "tape CBOR: dtype must be EHIP_F32 or EHIP_F64");
+  TapeData d;
+  int32_t rc = read_tape_cbor(cbor, len, d);
+  if (rc) return rc;
+  return copy_out(d, dtype, opcodes, arg0, arg1, values, cap_entries, output_indices, cap_outputs, num_entries,
+                  num_inputs, output_index, num_output_indices, "CBOR");
 }
 
 }  // extern "C"

@@ -1,7 +1,8 @@
 """Serde wire formats (Python mirror).
 
-``tape_to_json`` / ``tape_from_json`` call the library's tape reader and
-writer (``ehip_tape_to_json`` / ``ehip_tape_from_json``), which follow
+``tape_to_json`` / ``tape_from_json`` (and the CBOR pair ``tape_to_cbor`` /
+``tape_from_cbor``, the binary form the reference round-trips through
+ciborium) call the library's tape readers and writers, which follow
 ``impl Serialize / Deserialize for BytecodeTape``
 (``src/bytecode_tape/serde_support.rs:11-100``): the same field names and
 order, opcodes as variant names, and the same rejections with the same
@@ -36,9 +37,7 @@ def _p(a: np.ndarray):
     return _vp(a.ctypes.data) if a.size else _vp(None)
 
 
-def tape_to_json(tape: BytecodeTape, f32: bool = False) -> str:
-    """``serde_json::to_string(&tape)`` for a ``BytecodeTape<f64>`` (``f32``:
-    the values rounded to f32, as a ``BytecodeTape<f32>`` would carry them)."""
+def _encode(fn: str, tape: BytecodeTape, f32: bool) -> bytes:
     n = len(tape.opcodes)
     op = np.asarray(tape.opcodes, np.uint32)
     a0 = np.asarray(tape.arg0, np.uint32)
@@ -46,34 +45,52 @@ def tape_to_json(tape: BytecodeTape, f32: bool = False) -> str:
     with np.errstate(over="ignore"):
         vals = np.asarray(tape.values, np.float64).astype(np.float32 if f32 else np.float64)
     outs = np.asarray(tape.output_indices, np.uint32)
-    lib = load_library()
+    fn = getattr(load_library(), fn)
     args = (EHIP_F32 if f32 else EHIP_F64, _p(op), _p(a0), _p(a1), _p(vals), n, tape.num_inputs, tape.output_index,
             _p(outs), len(outs))
     ln = _c_u64(0)
-    _check(lib.ehip_tape_to_json(*args, None, 0, ctypes.byref(ln)))
+    _check(fn(*args, None, 0, ctypes.byref(ln)))
     buf = ctypes.create_string_buffer(int(ln.value) + 1)
-    _check(lib.ehip_tape_to_json(*args, buf, int(ln.value) + 1, ctypes.byref(ln)))
-    return buf.raw[:int(ln.value)].decode()
+    _check(fn(*args, buf, int(ln.value) + 1, ctypes.byref(ln)))
+    return buf.raw[:int(ln.value)]
+
+
+def tape_to_json(tape: BytecodeTape, f32: bool = False) -> str:
+    """``serde_json::to_string(&tape)`` for a ``BytecodeTape<f64>`` (``f32``:
+    the values rounded to f32, as a ``BytecodeTape<f32>`` would carry them)."""
+    return _encode("ehip_tape_to_json", tape, f32).decode()
+
+
+def tape_to_cbor(tape: BytecodeTape, f32: bool = False) -> bytes:
+    """``ciborium::into_writer(&tape, ..)``: the same model as CBOR (RFC 8949)."""
+    return _encode("ehip_tape_to_cbor", tape, f32)
 
 
 def tape_from_json(text, f32: bool = False) -> BytecodeTape:
     """``serde_json::from_str::<BytecodeTape<F>>``; a rejected payload raises
     ``GpuError(OTHER)`` carrying the reference's message."""
-    raw = text.encode() if isinstance(text, str) else bytes(text)
-    lib = load_library()
+    return _decode("ehip_tape_from_json", text.encode() if isinstance(text, str) else bytes(text), f32)
+
+
+def tape_from_cbor(data: bytes, f32: bool = False) -> BytecodeTape:
+    """``ciborium::from_reader::<BytecodeTape<F>>``, with the same checks."""
+    return _decode("ehip_tape_from_cbor", bytes(data), f32)
+
+
+def _decode(fn: str, raw: bytes, f32: bool) -> BytecodeTape:
+    fn = getattr(load_library(), fn)
     dt = EHIP_F32 if f32 else EHIP_F64
     n, ni, oi, m = _c_u64(0), _c_u32(0), _c_u32(0), _c_u64(0)
     counts = (ctypes.byref(n), ctypes.byref(ni), ctypes.byref(oi), ctypes.byref(m))
-    _check(lib.ehip_tape_from_json(raw, len(raw), dt, None, None, None, None, 0, None, 0, *counts))
+    _check(fn(raw, len(raw), dt, None, None, None, None, 0, None, 0, *counts))
     N, M = int(n.value), int(m.value)
     op = np.zeros(max(N, 1), np.uint32)
     a0 = np.zeros(max(N, 1), np.uint32)
     a1 = np.zeros(max(N, 1), np.uint32)
     vals = np.zeros(max(N, 1), np.float32 if f32 else np.float64)
     outs = np.zeros(max(M, 1), np.uint32)
-    _check(lib.ehip_tape_from_json(raw, len(raw), dt, _vp(op.ctypes.data), _vp(a0.ctypes.data),
-                                   _vp(a1.ctypes.data), _vp(vals.ctypes.data), max(N, 1), _vp(outs.ctypes.data),
-                                   max(M, 1), *counts))
+    _check(fn(raw, len(raw), dt, _vp(op.ctypes.data), _vp(a0.ctypes.data), _vp(a1.ctypes.data),
+              _vp(vals.ctypes.data), max(N, 1), _vp(outs.ctypes.data), max(M, 1), *counts))
     t = BytecodeTape()
     t.opcodes = op[:N].tolist()
     t.arg0 = a0[:N].tolist()

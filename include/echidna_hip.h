@@ -336,6 +336,20 @@ int32_t ehip_tape_from_json(const char* json, uint64_t len, int32_t dtype, uint3
 /* serde_json::from_str::<BytecodeTape<F>> followed by upload_tape /
  * upload_tape_f64 (f32 values are the f64 parse rounded to f32). */
 int32_t ehip_tape_upload_json(ehip_ctx* ctx, int32_t dtype, const char* json, uint64_t len, ehip_tape** out);
+/* The same model as CBOR (RFC 8949), the binary form the reference
+ * round-trips through ciborium (tests/serde.rs:156-170): a definite map with
+ * text keys in field order, variant names as text, shortest unsigned heads,
+ * floats in the shortest of binary16/32/64 that holds them exactly (NaN as
+ * binary16 0x7e00; Inf and NaN round-trip).  Same two-call sizing, checks
+ * and messages as the JSON pair; no terminator is written. */
+int32_t ehip_tape_to_cbor(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                          const void* values, uint64_t num_entries, uint32_t num_inputs, uint32_t output_index,
+                          const uint32_t* output_indices, uint64_t num_output_indices, uint8_t* buf, uint64_t cap,
+                          uint64_t* len);
+int32_t ehip_tape_from_cbor(const uint8_t* cbor, uint64_t len, int32_t dtype, uint32_t* opcodes, uint32_t* arg0,
+                            uint32_t* arg1, void* values, uint64_t cap_entries, uint32_t* output_indices,
+                            uint64_t cap_outputs, uint64_t* num_entries, uint32_t* num_inputs,
+                            uint32_t* output_index, uint64_t* num_output_indices);
 
 /* ------------------------------------------------------------------------
  * Learned low-rank control variate for STDE.  No counterpart in the

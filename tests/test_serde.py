@@ -5,7 +5,9 @@ payload of regression #26, and each corrupt-payload rejection.  The layout
 is pinned by a hand-derived serde_json text of the rosenbrock tape, and the
 number text by shortest round-trip digits: Python's repr (f64) and numpy's
 dragon4 (f32) must give the same digits, and every value reads back
-bit-identically."""
+bit-identically.  The CBOR form (the reference's ciborium round trip) is
+pinned by RFC 8949 Appendix A's encodings and by the same rejections,
+re-encoded with a small independent encoder."""
 
 import json
 import re
@@ -364,3 +366,145 @@ def test_nonsmooth_info_roundtrip():
     info2 = sd.NonsmoothInfo.from_json(info.to_json())
     assert info2 == info and info2.signature() == info.signature()
     assert json.loads(info.to_json())["kinks"][0]["opcode"] == "Abs"
+
+
+# ---- CBOR (ciborium's binary form; tests/serde.rs:156-170) -------------------
+
+def cbor_encode(v) -> bytes:
+    """Small independent RFC 8949 encoder for crafting payloads (floats always
+    binary64, which every reader must accept)."""
+    import struct
+
+    def head(major, n):
+        if n < 24:
+            return bytes([major << 5 | n])
+        for ai, w in ((24, 1), (25, 2), (26, 4), (27, 8)):
+            if n < 1 << (8 * w):
+                return bytes([major << 5 | ai]) + n.to_bytes(w, "big")
+        raise ValueError(n)
+
+    if v is None:
+        return b"\xf6"
+    if isinstance(v, bool):
+        return b"\xf5" if v else b"\xf4"
+    if isinstance(v, int):
+        return head(0, v) if v >= 0 else head(1, -1 - v)
+    if isinstance(v, float):
+        return b"\xfb" + struct.pack(">d", v)
+    if isinstance(v, str):
+        return head(3, len(v.encode())) + v.encode()
+    if isinstance(v, list):
+        return head(4, len(v)) + b"".join(cbor_encode(x) for x in v)
+    if isinstance(v, dict):
+        return head(5, len(v)) + b"".join(cbor_encode(int(k) if isinstance(k, str) and k.isdigit() else k)
+                                          + cbor_encode(x) for k, x in v.items())
+    raise TypeError(type(v))
+
+
+def _values_bytes(blob: bytes) -> bytes:
+    """The encoded items of the `values` array (after its key and head)."""
+    key = b"\x66values"
+    i = blob.index(key) + len(key)
+    j = blob.index(b"\x6anum_inputs")
+    head = blob[i]
+    skip = 1 if head & 31 < 24 else 1 + (1 << ((head & 31) - 24))
+    return blob[i + skip:j]
+
+
+# RFC 8949 Appendix A: preferred (shortest exact) float encodings
+RFC_FLOATS = [(0.0, "f90000"), (-0.0, "f98000"), (1.0, "f93c00"), (1.1, "fb3ff199999999999a"), (1.5, "f93e00"),
+              (65504.0, "f97bff"), (100000.0, "fa47c35000"), (3.4028234663852886e38, "fa7f7fffff"),
+              (1.0e300, "fb7e37e43c8800759c"), (5.960464477539063e-8, "f90001"), (0.00006103515625, "f90400"),
+              (-4.0, "f9c400"), (-4.1, "fbc010666666666666"), (float("inf"), "f97c00"), (float("nan"), "f97e00"),
+              (float("-inf"), "f9fc00")]
+
+
+@pytest.mark.parametrize("v,hexs", RFC_FLOATS)
+def test_cbor_float_encoding_matches_rfc8949(v, hexs):
+    assert _values_bytes(sd.tape_to_cbor(_const_tape([v]))).hex() == hexs
+
+
+def test_cbor_integer_and_text_encoding_matches_rfc8949():
+    t = _const_tape([1.0])
+    t.output_indices = [0, 1, 10, 23, 24, 25, 100, 1000, 1000000, 4294967295]
+    blob = sd.tape_to_cbor(t)
+    key = b"\x6eoutput_indices"
+    i = blob.index(key) + len(key)
+    body = blob[i:blob.index(b"\x72custom_second_args")]
+    assert body.hex() == "8a" + "00" + "01" + "0a" + "17" + "1818" + "1819" + "1864" + "1903e8" + "1a000f4240" + "1affffffff"
+    assert blob[:1] == b"\xa8" and b"\x65Const" in blob  # map(8); text(5) "Const"
+    assert blob.endswith(b"\x72custom_second_args\xa0")
+
+
+def test_cbor_roundtrip_gradient():
+    tape = rosen_tape()
+    tape2 = sd.tape_from_cbor(sd.tape_to_cbor(tape))
+    assert same_tape(tape, tape2)
+    x = np.array([[1.5, 2.5]])
+    assert np.array_equal(po.gradient(tape, x)[1], po.gradient(tape2, x)[1])
+
+
+@pytest.mark.parametrize("name", ["mlp", "dag", "multi"])
+def test_cbor_and_json_agree(name):
+    if name == "mlp":
+        tape = wl.tanh_mlp(6, 5, 9)
+    elif name == "dag":
+        tape = wl.random_dag(3, 1, 500, 20)
+    else:
+        tape, _ = record_multi(lambda v: [v[0] * v[1], v[1].sin(), v[0].powi(-3), v[1].atan2(v[0])], [0.3, 0.7])
+    blob = sd.tape_to_cbor(tape)
+    a, b = sd.tape_from_cbor(blob), sd.tape_from_json(sd.tape_to_json(tape))
+    assert same_tape(a, tape) and same_tape(b, tape)
+    assert sd.tape_to_cbor(a) == blob
+    assert len(blob) < len(sd.tape_to_json(tape))
+    t32 = sd.tape_from_cbor(sd.tape_to_cbor(tape, f32=True), f32=True)
+    assert np.array_equal(np.asarray(t32.values, np.float32), np.asarray(tape.values).astype(np.float32))
+
+
+def test_cbor_carries_non_finite_values():
+    t = BytecodeTape()
+    t.new_input(1.0)
+    t.push_const(float("inf"))
+    t._push_entry(OpCode.Mul, 0, 1, float("nan"))
+    t.set_output(2)
+    t2 = sd.tape_from_cbor(sd.tape_to_cbor(t))
+    assert t2.values[0] == 1.0 and t2.values[1] == float("inf") and np.isnan(t2.values[2])
+
+
+def test_cbor_reads_any_float_width_and_integer_values():
+    v = tape_value()
+    v["values"] = [float(x) for x in v["values"]]  # binary64 everywhere
+    v["values"][2] = 1  # an integer where a float is expected
+    assert same_tape(sd.tape_from_cbor(cbor_encode(v)), rosen_tape())
+
+
+@pytest.mark.parametrize("mutate,match", [
+    (lambda v: v.__setitem__("num_variables", v["num_variables"] + 1), r"num_variables \(12\) != opcodes.len\(\) \(11\)"),
+    (lambda v: v["arg_indices"].__setitem__(0, [0, 0]), r"must be \[UNUSED, UNUSED\] for an Input entry"),
+    (lambda v: v["opcodes"].__setitem__(10, "Input"), r"opcodes\[10\] is Input, but only the first 2"),
+    (lambda v: v.__setitem__("output_index", 0xFFFFFFFF), r"output_index \(4294967295\) >= tape length"),
+    (lambda v: v["arg_indices"][10].__setitem__(0, 10), r"arg_indices\[10\]\[0\] \(10\) must reference an earlier"),
+    (lambda v: v["custom_second_args"].__setitem__("3", 1), "custom_second_args must be empty"),
+    (lambda v: v["arg_indices"].append([0, 0]), r"arg_indices.len\(\) \(12\) != opcodes.len\(\) \(11\)"),
+    (lambda v: v["opcodes"].__setitem__(10, "Custom"), r"opcodes\[10\] is Custom"),
+    (lambda v: v.pop("values"), "missing field `values`"),
+    (lambda v: v.__setitem__("num_inputs", -1), "negative integer, expected u32"),
+    (lambda v: v["opcodes"].__setitem__(3, "Subtract"), "unknown variant `Subtract`"),
+    (lambda v: v["values"].__setitem__(2, "1.0"), "invalid type: string, expected f64"),
+])
+def test_cbor_rejections(mutate, match):
+    v = tape_value()
+    mutate(v)
+    with pytest.raises(GpuError, match=match):
+        sd.tape_from_cbor(cbor_encode(v))
+
+
+def test_cbor_truncated_and_trailing_payloads():
+    blob = sd.tape_to_cbor(rosen_tape())
+    for cut in (1, 10, len(blob) // 2, len(blob) - 1):
+        with pytest.raises(GpuError, match="unexpected end of input"):
+            sd.tape_from_cbor(blob[:cut])
+    with pytest.raises(GpuError, match="trailing data"):
+        sd.tape_from_cbor(blob + b"\x00")
+    with pytest.raises(GpuError, match="expected struct TapeData"):
+        sd.tape_from_cbor(b"\x80")
