@@ -295,6 +295,26 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, const uint32_t* 
       F x[NC];
       R.rd(kr & P_REG_MASK, x);
       const bool cleft = (kr & P_CLEFT) != 0;
+      bool fin = true;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) fin = fin && isfinite(x[q]);
+      if (__all(fin)) {
+        // With every coefficient of x finite the Taylor product with the
+        // constant jet [c, 0, ..] reduces exactly to 0 + x_k * c: its other
+        // products are signed zeros, and a sum that starts at +0 absorbs
+        // them (either operand order).
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+          if (!(FULL || (uint32_t)j < jb)) continue;
+          const F c = w[j];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) {
+            const F term = F(0) + x[q] * c;
+            acc[j][q] = (p == 0) ? term : acc[j][q] + term;
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
