@@ -47,8 +47,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="repeat CPU passes for at least this long")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one 1 GiB device-to-device copy (calibrates HBM byte counters)")
-    ap.add_argument("--workload", choices=["mlp", "stde", "cv", "dags"], default="mlp",
-                    help="mlp: fwd+grad (default line); stde: plain GPU STDE on the n=256 H=256 MLP; "
+    ap.add_argument("--workload", choices=["mlp", "hvp", "stde", "cv", "dags"], default="mlp",
+                    help="mlp: fwd+grad (default line); hvp: one Hessian-vector product per point on the "
+                         "same MLP; stde: plain GPU STDE on the n=256 H=256 MLP; "
                          "cv: learned control variate vs plain on the n=1024 2x512 MLP; dags: 64 uneven "
                          "random DAGs in one persistent launch vs one launch per tape")
     ap.add_argument("--points", type=int, default=0, help="override the workload's point count")
@@ -314,6 +315,43 @@ def rank_bound(ctx, b64, cvnet, X, r: int, npos: int) -> dict:
             "network_ratio": res_net / fro2}
 
 
+def run_hvp(args) -> dict:
+    """Forward-over-reverse HVP, one (x, v) per point: tanh MLP n=32 H=64
+    seed 1, B=2^20, x, v ~ N(0, I) (config 2's third evaluation)."""
+    import torch
+
+    from echidna_amd import philox
+    from echidna_amd import workloads as wl
+    from echidna_amd.backend import TUNE_HVP, HipContext
+    from echidna_amd.tape import GpuTapeData
+
+    n, H = 32, 64
+    B = 1 << args.batch_log2
+    f64 = args.dtype == "f64"
+    tape = wl.tanh_mlp(n, H, 1)
+    counts = tape.counts()
+    ctx = HipContext.new(0)
+    bufs = ctx.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    dt = np.float64 if f64 else np.float32
+    x = torch.from_numpy(wl.points_normal(1, B, n).astype(dt)).cuda()
+    v = torch.from_numpy(philox.normal(1, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n).astype(dt)).cuda()
+    g = torch.empty_like(x)
+    hv = torch.empty_like(x)
+    tuned = ctx.autotune_dev(bufs, TUNE_HVP, B, x, d_g=g, d_h=hv, d_dirs=v)
+    dt_s, _ = timed(lambda: ctx.hvp_batch_dev(bufs, x, n, v, B, g, hv), max(1, args.steps))
+    S = 8 if f64 else 4
+    per_point = S * (12 * counts["n"] + 10 * counts["N"])
+    return {"metric": "HVP point-evals/s", "value": B / dt_s, "unit": "points/s", "n_gpus": 1,
+            "higher_is_better": True, "vs_baseline": None, "dtype": args.dtype,
+            "data": "synthetic: Philox points and directions, random-init weights baked into the tape",
+            "config": {"workload": f"tanh-MLP tape n={n} H={H} seed 1 (N={counts['N']} computed entries), "
+                                   f"B=2^{args.batch_log2} points, one HVP each", "points": B, "launch": tuned},
+            "ms_per_step": dt_s * 1e3,
+            "roofline": {"bound": "hbm", "achieved": per_point * B / dt_s / 1e9, "peak": HBM_PEAK / 1e9,
+                         "unit": "GB/s", "frac": per_point * B / dt_s / HBM_PEAK,
+                         "algorithmic_bytes_per_point": per_point}}
+
+
 def run_dags(args) -> dict:
     """64 random DAGs (seed 3): lengths log-U[200, 20000], n ~ U[4, 128],
     B_i log-U[1e3, 1e5], x ~ U[-1, 1]^n, f32 fwd+grad: one persistent
@@ -369,7 +407,7 @@ def main() -> int:
                                    "(points shard by contiguous ranges across GPUs with no exchange)"}))
         return 2
     if args.workload != "mlp":
-        fn = {"stde": run_stde, "cv": run_cv, "dags": run_dags}[args.workload]
+        fn = {"hvp": run_hvp, "stde": run_stde, "cv": run_cv, "dags": run_dags}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return 0
     import torch
