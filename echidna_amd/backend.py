@@ -214,8 +214,9 @@ def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Op
     """Compile without a device.
 
     Modes 0/1 (forward/gradient) return (words[n,8], side table, info);
-    modes 2/3 return (windowed stream words, None, info).  info["pool"] holds
-    the DENSE pool words.
+    modes 2/3 return (windowed stream words, None, info); adding 4 to a
+    gradient mode compiles the dual interpreter's HVP program.  info["pool"]
+    holds the DENSE pool words.
     """
     lib = load_library()
     f64 = (np.asarray(data.constants).dtype == np.float64) if f64 is None else f64
@@ -240,7 +241,7 @@ def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Op
     _check(rc)
     info2 = {"instructions": int(info[0]), "registers": int(info[1]), "words": nw, "nterms": T,
              "pool": pool[:npool]}
-    if mode >= 2:
+    if mode & 2:
         return words[:nw], None, info2
     return words[:nw].reshape(-1, 8), terms[:nt], info2
 

@@ -136,7 +136,7 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
   }
   t->prog[0].host = compile_program(t->host, PM_FORWARD, fuse, dense);
   t->prog[1].host = compile_program(t->host, PM_GRADIENT, fuse, dense);
-  t->prog[2].host = compile_program(t->host, PM_GRADIENT, fuse, false);
+  t->prog[2].host = compile_program(t->host, PM_GRADIENT, fuse, dense, true);
   t->prog[3].host = compile_program(t->host, PM_KINK, fuse, dense);
   for (uint32_t i = t->host.num_inputs; i < num_ops; ++i)
     if (is_nonsmooth_op(t->host.opcodes[i])) t->nkinks++;
@@ -179,7 +179,7 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
                              int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
                              uint64_t terms_cap, uint32_t* pool, uint64_t pool_cap, uint64_t* out5) {
   uint64_t* out4 = out5;
-  if (!out4 || mode < 0 || mode > 3 || fuse < 0 || fuse > 2 ||
+  if (!out4 || mode < 0 || mode > 7 || mode == 4 || mode == 6 || fuse < 0 || fuse > 2 ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)
@@ -201,20 +201,20 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
   th.num_inputs = num_inputs;
   th.outputs = outs;
   th.f64 = dtype == EHIP_F64;
-  Program p = compile_program(th, (mode & 1) == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0, fuse == 1);
+  Program p = compile_program(th, (mode & 1) == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0, fuse == 1, (mode & 4) != 0);
   out4[0] = p.ninstr;
   out4[1] = p.nregs;
   out4[3] = p.nterms;
   out5[4] = p.pool.size();
   std::vector<uint32_t> st;
-  if (mode >= 2) st = encode_stream(p, th.f64);  // the windowed stream the kernels execute
-  const std::vector<uint32_t>& wv = mode >= 2 ? st : p.words;
+  if (mode & 2) st = encode_stream(p, th.f64);  // the windowed stream the kernels execute
+  const std::vector<uint32_t>& wv = (mode & 2) ? st : p.words;
   out4[2] = wv.size();
-  const bool small = wv.size() > words_cap || !words || (mode < 2 && (p.terms.size() > terms_cap || !terms)) ||
+  const bool small = wv.size() > words_cap || !words || (!(mode & 2) && (p.terms.size() > terms_cap || !terms)) ||
                      (!p.pool.empty() && (p.pool.size() > pool_cap || !pool));
   if (small) return fail(EHIP_OTHER, "GPU error: program buffers too small");
   std::memcpy(words, wv.data(), wv.size() * 4);
-  if (mode < 2) std::memcpy(terms, p.terms.data(), p.terms.size() * 4);
+  if (!(mode & 2)) std::memcpy(terms, p.terms.data(), p.terms.size() * 4);
   if (!p.pool.empty()) std::memcpy(pool, p.pool.data(), p.pool.size() * 4);
   return ok();
 }

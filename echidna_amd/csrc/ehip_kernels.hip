@@ -655,6 +655,118 @@ __device__ __forceinline__ Dual<F> dual_norm(Dual<F> x) {
   return d_add(dc(F(0)), d_mul(dc(F(1)), x));
 }
 
+// Dual programs.  DENSE: each member's DOT in dual arithmetic (the product
+// with the constant dual (c, 0) and d_add), term by term in program order,
+// so every member gets the bits its DOT gives; an operand is read once per
+// block of JB members.
+template <class F, int JB, bool FULL, bool LDSONLY>
+__device__ __forceinline__ void dense_dual_block(const Regs<F, 2, LDSONLY>& R, AS4 const uint32_t* PR,
+                                                 AS4 const F* PW, uint32_t m, uint32_t T, uint32_t j0, uint32_t jb) {
+  Dual<F> acc[JB];
+  for (uint32_t p = 0; p < T; ++p) {
+    const uint32_t kr = PR[p];
+    const uint32_t kind = kr >> P_KIND_SHIFT;
+    AS4 const F* w = PW + (uint64_t)p * m + j0;
+    Dual<F> x = dc(F(0));
+    if (kind != T_IMM) {
+      F t[2];
+      R.rd(kr & P_REG_MASK, t);
+      x = Dual<F>{t[0], t[1]};
+    }
+    const bool cleft = (kr & P_CLEFT) != 0;
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      if (!(FULL || (uint32_t)j < jb)) continue;
+      const Dual<F> c = dc(w[j]);
+      Dual<F> term = c;
+      if (kind == T_PROD) term = cleft ? d_mul(c, x) : d_mul(x, c);
+      else if (kind == T_REG) term = x;
+      acc[j] = (p == 0) ? term : d_add(acc[j], term);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < JB; ++j) {
+    if (!(FULL || (uint32_t)j < jb)) continue;
+    F t[2] = {acc[j].re, acc[j].eps};
+    R.wr(PR[T + j0 + j], t);
+  }
+}
+
+// DENSE_REV: the members' DOT_REVs applied to each operand adjoint in their
+// original order.  A DOT_REV hands a term at depth d the adjoint normalised
+// d times by 0 + 1 * g (one Add link each), and a product term normalises
+// once more.  One normalisation N(g) already gives every later one's value,
+// so a product contributes c * N(g) and a plain term g (depth 0, P_DEEP
+// clear) or N(g).  Skips as in the reference: an all-zero adjoint (per lane)
+// and a zero weight (uniform).
+template <class F, int IB, bool FULL, bool LDSONLY>
+__device__ __forceinline__ void dense_dual_rev_block(const Regs<F, 2, LDSONLY>& R, AS4 const uint32_t* PR,
+                                                     AS4 const F* PW, uint32_t m, uint32_t T, uint32_t i0,
+                                                     uint32_t ib) {
+  static_assert(IB <= 32, "one flag bit per adjoint");
+  Dual<F> cur[IB];
+  uint32_t prod = 0, deep = 0;
+#pragma unroll
+  for (int i = 0; i < IB; ++i) {
+    cur[i] = dc(F(0));
+    if (!(FULL || (uint32_t)i < ib)) continue;
+    const uint32_t kr = PR[i0 + i];
+    if ((kr >> P_KIND_SHIFT) == T_PROD) prod |= 1u << i;
+    if (kr & P_DEEP) deep |= 1u << i;
+    if (!(kr & P_FIRST)) {
+      F t[2];
+      R.rd(kr & P_REG_MASK, t);
+      cur[i] = Dual<F>{t[0], t[1]};
+    }
+  }
+  for (uint32_t j = 0; j < m; ++j) {
+    F t[2];
+    R.rd(PR[T + j], t);
+    const Dual<F> g{t[0], t[1]};
+    const bool live = !d_zero(g);
+    const Dual<F> n1 = dual_norm(g);
+    AS4 const F* w = PW + (uint64_t)j * T + i0;
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      if (!(FULL || (uint32_t)i < ib)) continue;
+      Dual<F> nx;
+      if (prod & (1u << i)) {
+        const F c = w[i];
+        if (c == F(0)) continue;
+        nx = d_add(cur[i], d_mul(dc(c), n1));
+      } else {
+        nx = d_add(cur[i], d_mul(dc(F(1)), (deep & (1u << i)) ? n1 : g));
+      }
+      cur[i].re = live ? nx.re : cur[i].re;
+      cur[i].eps = live ? nx.eps : cur[i].eps;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < IB; ++i) {
+    if (!(FULL || (uint32_t)i < ib)) continue;
+    F t[2] = {cur[i].re, cur[i].eps};
+    R.wr(PR[i0 + i] & P_REG_MASK, t);
+  }
+}
+
+template <class F, bool LDSONLY>
+__device__ __forceinline__ void dense_dual(const Regs<F, 2, LDSONLY>& R, const uint32_t* pool, uint32_t m, uint32_t T,
+                                           uint32_t off_regs, uint32_t off_w, bool rev) {
+  constexpr int B = 8;
+  AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
+  AS4 const F* PW = reinterpret_cast<AS4 const F*>((AS4 const uint32_t*)pool + off_w);
+  const uint32_t n = rev ? T : m;
+  uint32_t k0 = 0;
+  for (; k0 + B <= n; k0 += B) {
+    if (rev) dense_dual_rev_block<F, B, true, LDSONLY>(R, PR, PW, m, T, k0, B);
+    else dense_dual_block<F, B, true, LDSONLY>(R, PR, PW, m, T, k0, B);
+  }
+  if (k0 < n) {
+    if (rev) dense_dual_rev_block<F, B, false, LDSONLY>(R, PR, PW, m, T, k0, n - k0);
+    else dense_dual_block<F, B, false, LDSONLY>(R, PR, PW, m, T, k0, n - k0);
+  }
+}
+
 template <class F, bool LDSONLY>
 __global__ __launch_bounds__(256) void k_dual(Launch L) {
   const Regs<F, 2, LDSONLY> R = make_regs<F, 2, LDSONLY>(L);
@@ -694,6 +806,8 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
         have = true;
       }
       wrd(dst, acc);
+    } else if (op == I_DENSE || op == I_DENSE_REV) {
+      dense_dual<F, LDSONLY>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4), op == I_DENSE_REV);
     } else if (op >= I_REV_BASE) {
       const uint32_t top = op - I_REV_BASE;
       const Dual<F> g = rdd(I.v(1));

@@ -78,7 +78,7 @@ constexpr size_t DENSE_CHUNK = 16;  // members per forward DENSE
 // before that point.  A reverse group moves to its last member's position
 // when no instruction in between touches the operand adjoints, so each
 // adjoint receives the same contributions in the same order.
-void group_dense(std::vector<AIns>& prog, int64_t adj_offset, size_t chunk) {
+void group_dense(std::vector<AIns>& prog, int64_t adj_offset, size_t chunk, bool dual) {
   struct Key {
     std::vector<int64_t> k;
     bool operator<(const Key& o) const { return k < o.k; }
@@ -170,6 +170,16 @@ void group_dense(std::vector<AIns>& prog, int64_t adj_offset, size_t chunk) {
       if (a2 != sadj) { same = false; break; }
       for (size_t k = 0; k < r.terms.size(); ++k)
         if (r.terms[k].first && q != order[0]) same = false;
+    }
+    // dual programs: every member must reach each operand through the same
+    // kind of link (depth 0 or >= 1), the one property of the chain shape
+    // the dual DENSE_REV keeps
+    if (dual) {
+      std::map<int64_t, bool> deep;
+      for (const ATerm& tm : r0.terms) deep[tm.vid] = tm.depth >= 1;
+      for (size_t q : order)
+        for (const ATerm& tm : prog[q].terms)
+          if (deep[tm.vid] != (tm.depth >= 1)) same = false;
     }
     if (!same) continue;
     auto touches = [&](const AIns& x) {
@@ -272,7 +282,7 @@ std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const u
   return "";
 }
 
-Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense) {
+Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense, bool dual) {
   const uint32_t N = (uint32_t)t.opcodes.size();
   const uint32_t ni = t.num_inputs;
   const bool grad = (mode == PM_GRADIENT);
@@ -710,7 +720,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
     }
   }
 
-  if (fuse && dense) group_dense(prog, (int64_t)N, grad ? SIZE_MAX : DENSE_CHUNK);
+  if (fuse && dense) group_dense(prog, (int64_t)N, grad ? SIZE_MAX : DENSE_CHUNK, dual);
 
   // ---- live ranges ----
   const size_t NV = 2 * (size_t)N;
@@ -872,7 +882,9 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense)
         w[3] = (uint32_t)out.pool.size();
         for (auto& tm : x.terms)
           out.pool.push_back((tm.kind << P_KIND_SHIFT) | (tm.first ? P_FIRST : 0u) |
-                             (tm.const_left ? P_CLEFT : 0u) | (tm.vid >= 0 ? R(tm.vid) : 0u));
+                             ((dual && x.op == I_DENSE_REV) ? (tm.depth >= 1 ? P_DEEP : 0u)
+                                                            : (tm.const_left ? P_CLEFT : 0u)) |
+                             (tm.vid >= 0 ? R(tm.vid) : 0u));
         for (int64_t v : x.members) out.pool.push_back(R(v));
         if (t.f64 && (out.pool.size() & 1)) out.pool.push_back(0u);  // 8-byte aligned weights
         w[4] = (uint32_t)out.pool.size();

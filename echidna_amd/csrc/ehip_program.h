@@ -113,6 +113,11 @@ struct TapeHost {
 };
 
 constexpr uint32_t P_KIND_SHIFT = 30, P_FIRST = 1u << 29, P_CLEFT = 1u << 28, P_REG_MASK = (1u << 28) - 1;
+// DENSE_REV term words of dual programs use bit 28 for P_DEEP instead: the
+// term reaches its operand through at least one Add link of its chain (DOT_REV
+// depth >= 1), so its adjoint arrives normalised once as 0 + 1 * g.  One
+// normalisation is all that matters: applying it again changes no value.
+constexpr uint32_t P_DEEP = 1u << 28;
 // DENSE_REV w5: every term is a product and no weight is zero (in the
 // program's dtype), so only the adjoints can trigger the reference guards.
 constexpr uint32_t D_ALL_PROD_NZ = 1u;
@@ -146,7 +151,9 @@ std::vector<uint32_t> encode_stream(const Program& p, bool f64);
 
 // Build the program; `fuse` enables DOT fusion, `dense` the grouping of
 // fused dot products into DENSE / DENSE_REV.  Never fails on a validated tape.
-Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense = true);
+// `dual`: the program runs on the dual (HVP) interpreter, whose DENSE_REV
+// also needs each term's P_DEEP flag.
+Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense = true, bool dual = false);
 
 // GpuTapeData::validate (src/gpu/mod.rs:397-564); returns "" when valid.
 std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,

@@ -173,7 +173,9 @@ int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
  * max(4, T*(2+IW)) words (IW = 1 f32, 2 f64).  Mode 2 forward, 3 gradient:
  * writes the windowed instruction stream the kernels execute (512-word
  * windows, dot terms inline) into `words`; `terms` is unused.  Every mode
- * writes the DENSE pool (operand lists and weights) into `pool`.  out5 =
+ * writes the DENSE pool (operand lists and weights) into `pool`.  Adding 4
+ * to a gradient mode compiles the HVP program of the dual interpreter
+ * (DENSE_REV terms carry P_DEEP).  out5 =
  * {instructions, registers, program or stream words, side-table entries T,
  * pool words}.  Returns EHIP_OTHER if a buffer is too small (out5 still
  * holds the sizes). */

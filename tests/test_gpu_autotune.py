@@ -149,3 +149,52 @@ def test_dense_guards_are_bit_identical_on_device(hip, f64):
         fin = np.isfinite(ref)
         assert np.array_equal(g0r[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
         assert np.max(np.abs(g0r[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-12
+
+
+def _bits(a, f64):
+    """Bits with the sign of NaNs cleared (IEEE 754 leaves it unspecified)."""
+    U = np.uint64 if f64 else np.uint32
+    b = np.asarray(a).view(U).copy()
+    b[np.isnan(a)] &= ~(U(1) << U(63 if f64 else 31))
+    return b
+
+
+@pytest.mark.parametrize("name", ["mlp2", "guards"])
+@pytest.mark.parametrize("f64", [False, True])
+def test_dense_hvp_matches_dot_program_and_oracle(hip, name, f64):
+    """HVPs (one x per direction) with the dual interpreter's DENSE groups
+    equal the dot-level program's bit for bit (NaN signs aside), in both
+    program fetch modes, and match the f64 oracle."""
+    import dense_cases as dc
+    import pyoracle as po
+
+    tape = wl.tanh_mlp2(20, 24, 18, 35) if name == "mlp2" else dc.tape()
+    n = tape.num_inputs
+    F = np.float64 if f64 else np.float32
+    B = 400
+    x = (wl.points_normal(14, B, n) if name == "mlp2" else dc.points(B, 5)).astype(F)
+    v = philox.normal(15, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n).astype(F)
+    outs = []
+    for dense in (True, False):
+        hip.set_dense(dense)
+        try:
+            bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+        finally:
+            hip.set_dense(True)
+        for fetch in (0, 1):
+            HipContext.set_tuned(bufs, TUNE_HVP, 128, 8192, fetch=fetch)
+            g, hv = hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B)
+            outs.append((np.asarray(g).copy(), np.asarray(hv).copy()))
+    for g, hv in outs[1:]:
+        assert np.array_equal(_bits(g, f64), _bits(outs[0][0], f64))
+        assert np.array_equal(_bits(hv, f64), _bits(outs[0][1], f64))
+    if f64:
+        g, hv = (a.reshape(B, n) for a in outs[0])
+        with np.errstate(all="ignore"):
+            _, ref = po.hvp(tape, x, v)
+        ref = ref.reshape(B, n)
+        got = hv
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        fin = np.isfinite(ref)
+        assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
+        assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-11

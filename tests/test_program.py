@@ -117,6 +117,28 @@ def test_dense_guards_are_bit_identical(f64):
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+@pytest.mark.parametrize("name", ["mlp32", "mlp2"])
+def test_hvp_programs_group_dense_layers(name):
+    """The dual (HVP) program groups the same dense layers as the gradient
+    program; its DENSE_REV term words carry P_DEEP for operands reached
+    through an Add link (every product operand of an MLP layer's chain)."""
+    tape = TAPES[name]()
+    d = GpuTapeData.from_tape(tape)
+    grad_words, _, _ = compile_program(d, mode=1)
+    hvp_words, _, hinfo = compile_program(d, mode=5)
+    count = lambda words, op: sum(1 for w in words if int(w[0]) & 0xFF == op)
+    assert count(hvp_words, emu.I_DENSE) == count(grad_words, emu.I_DENSE) > 0
+    assert count(hvp_words, emu.I_DENSE_REV) == count(grad_words, emu.I_DENSE_REV) > 0
+    _, _, plain = compile_program(d, mode=5, dense=False)
+    assert hinfo["instructions"] < plain["instructions"]
+    pool = hinfo["pool"]
+    for w in hvp_words:
+        if int(w[0]) & 0xFF == emu.I_DENSE_REV:
+            T, off = int(w[2]), int(w[3])
+            words = pool[off:off + T]
+            assert np.all((words >> 30) == 0) and np.all(words & (1 << 28))  # products, all deep
+
+
 def test_dense_needs_distinct_operands():
     """A dot product that uses one operand twice keeps its DOT form."""
     def f(v):
