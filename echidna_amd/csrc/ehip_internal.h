@@ -113,7 +113,7 @@ struct ehip_tape {
   bool f64 = false;
   uint32_t nin = 0, nout = 0;
   TapeHost host;
-  DevProgram prog[4];  // forward, gradient (both with DENSE), HVP (dot-level), kinks
+  DevProgram prog[4];  // forward, gradient, HVP (dual interpreter), kinks; all with DENSE groups
   uint32_t nkinks = 0;
   Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
 };
