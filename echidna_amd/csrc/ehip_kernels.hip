@@ -279,18 +279,17 @@ __device__ __forceinline__ void dense_fwd_block(const Regs_& R, AS4 const uint32
 
 // Jet programs: products follow the Taylor product with a constant jet
 // (jt::mul, the operation the jet interpreter's DOT performs); one loop with
-// a per-term select over generic pool loads.  The value-program form raises
-// the jet interpreter's register use (115 instead of 96 VGPRs at order 3 in
-// f32, 276 instead of 169 at order 2 in f64), costing more occupancy than
-// it saves.
+// a per-term select (the peeled value-program form raises the jet
+// interpreter's register use: 115 instead of 96 VGPRs at order 3 in f32),
+// pool words and weights through scalar loads.
 template <class F, int NC, int JB, bool FULL, class Regs_>
-__device__ __forceinline__ void dense_jet_block(const Regs_& R, const uint32_t* PR, const F* PW, uint32_t m,
+__device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
                                                 uint32_t T, uint32_t j0, uint32_t jb) {
   F acc[JB][NC];
   for (uint32_t p = 0; p < T; ++p) {
     const uint32_t kr = PR[p];
     const uint32_t kind = kr >> P_KIND_SHIFT;
-    const F* w = PW + (uint64_t)p * m + j0;
+    AS4 const F* w = PW + (uint64_t)p * m + j0;
     if (kind == T_PROD) {
       F x[NC];
       R.rd(kr & P_REG_MASK, x);
@@ -361,8 +360,8 @@ __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, 
                                           uint32_t off_regs, uint32_t off_w) {
   uint32_t j0 = 0;
   if constexpr (JET) {
-    const uint32_t* PR = pool + off_regs;
-    const F* PW = reinterpret_cast<const F*>(pool + off_w);
+    AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
+    AS4 const F* PW = reinterpret_cast<AS4 const F*>((AS4 const uint32_t*)pool + off_w);
     for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true>(R, PR, PW, m, T, j0, JB);
     if (j0 < m) dense_jet_block<F, NC, JB, false>(R, PR, PW, m, T, j0, m - j0);
   } else {
