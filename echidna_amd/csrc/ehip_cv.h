@@ -31,6 +31,11 @@ struct Gemm {
   int64_t ldaux;
   int32_t accumulate;
   float alpha;
+  // Split-K workspace (optional): with room for ksplit * M * N floats the
+  // launcher may split a long K over `ksplit` slices of the grid, write each
+  // slice's sums there and add them in slice order before the epilogue.
+  float* part;
+  int64_t part_cap;  // floats
 };
 
 hipError_t launch_gemm(const Gemm& g, hipStream_t s);

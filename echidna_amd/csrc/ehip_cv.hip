@@ -76,26 +76,45 @@ __device__ __forceinline__ void store_tile(__bf16* lds, const float* o) {
   }
 }
 
+// Epilogue of one output element (shared by the direct and split-K paths).
+__device__ __forceinline__ void gemm_store(const Gemm& g, int m, int n, float v) {
+  float* c = g.C + (int64_t)m * g.ldc + n;
+  switch (g.epi) {
+    case EPI_BIAS: v = g.alpha * (v + g.bias[n]); break;
+    case EPI_BIAS_TANH: v = tanhf(v + g.bias[n]); break;
+    case EPI_DTANH: {
+      const float h = g.aux[(int64_t)m * g.ldaux + n];
+      v = v * (1.0f - h * h);
+      break;
+    }
+    default: break;
+  }
+  *c = g.accumulate ? *c + v : v;
+}
+
+// blockIdx.z = K slice [kz * kc, (kz + 1) * kc) (kc = K when not split).
+// Split: raw sums to part[kz][m][n]; otherwise the epilogue.
 template <bool AK, bool BKC>
-__global__ __launch_bounds__(256) void k_gemm(Gemm g) {
+__global__ __launch_bounds__(256) void k_gemm(Gemm g, int kc) {
   __shared__ __attribute__((aligned(16))) __bf16 As[BM * PITCH];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * PITCH];
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int kb = blockIdx.z * kc, ke = min(g.K, kb + kc);
   // A(m,k): rows = M, row stride sam, k stride sak; B(k,n): rows = N.
   float ra[8], rb[8];
-  fetch_tile<AK>(g.A, g.sam, g.sak, g.M, g.K, m0, 0, ra);
-  fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, g.K, n0, 0, rb);
+  fetch_tile<AK>(g.A, g.sam, g.sak, g.M, ke, m0, kb, ra);
+  fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, ke, n0, kb, rb);
   f32x16 acc = {};
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
+  for (int k0 = kb; k0 < ke; k0 += BK) {
     __syncthreads();
     store_tile<AK>(As, ra);
     store_tile<BKC>(Bs, rb);
     __syncthreads();
-    if (k0 + BK < g.K) {
-      fetch_tile<AK>(g.A, g.sam, g.sak, g.M, g.K, m0, k0 + BK, ra);
-      fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, g.K, n0, k0 + BK, rb);
+    if (k0 + BK < ke) {
+      fetch_tile<AK>(g.A, g.sam, g.sak, g.M, ke, m0, k0 + BK, ra);
+      fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, ke, n0, k0 + BK, rb);
     }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
@@ -111,47 +130,75 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g) {
   for (int j = 0; j < 16; ++j) {
     const int m = m0 + wm + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
     if (m >= g.M) continue;
-    float v = acc[j];
-    float* c = g.C + (int64_t)m * g.ldc + n;
-    switch (g.epi) {
-      case EPI_BIAS: v = g.alpha * (v + g.bias[n]); break;
-      case EPI_BIAS_TANH: v = tanhf(v + g.bias[n]); break;
-      case EPI_DTANH: {
-        const float h = g.aux[(int64_t)m * g.ldaux + n];
-        v = v * (1.0f - h * h);
-        break;
-      }
-      default: break;
-    }
-    *c = g.accumulate ? *c + v : v;
+    if (gridDim.z > 1) g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = acc[j];
+    else gemm_store(g, m, n, acc[j]);
   }
+}
+
+// Split-K second pass: the slices' sums added in slice order, then the epilogue.
+__global__ __launch_bounds__(256) void k_gemm_reduce(Gemm g, int nsplit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t MN = (int64_t)g.M * g.N;
+  if (i >= MN) return;
+  float v = g.part[i];
+  for (int z = 1; z < nsplit; ++z) v += g.part[z * MN + i];
+  gemm_store(g, (int)(i / g.N), (int)(i % g.N), v);
 }
 
 hipError_t launch_gemm(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const bool ak = g.sak == 1, bk = g.sbk == 1;
   if ((!ak && g.sam != 1) || (!bk && g.sbn != 1)) return hipErrorInvalidValue;
-  const dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
+  const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
+  // a long K over too few tiles to fill the chip: split it (256 CUs)
+  int split = 1;
+  const int64_t MN = (int64_t)g.M * g.N;
+  while (split < 16 && (int64_t)tm * tn * split < 512 && g.K / (split * 2) >= 512 &&
+         g.part && g.part_cap >= MN * split * 2)
+    split *= 2;
+  const int kc = split > 1 ? ((g.K + split - 1) / split + BK - 1) / BK * BK : g.K;
+  const dim3 grid(tn, tm, split);
   const void* fn = ak ? (bk ? (const void*)k_gemm<true, true> : (const void*)k_gemm<true, false>)
                       : (bk ? (const void*)k_gemm<false, true> : (const void*)k_gemm<false, false>);
-  void* args[] = {const_cast<Gemm*>(&g)};
-  return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
+  int kcv = kc;
+  void* args[] = {const_cast<Gemm*>(&g), &kcv};
+  hipError_t e = hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
+  if (e != hipSuccess || split == 1) return e;
+  int sv = split;
+  void* rargs[] = {const_cast<Gemm*>(&g), &sv};
+  return hipLaunchKernel((const void*)k_gemm_reduce, dim3((uint32_t)((MN + 255) / 256)), dim3(256), rargs, 0, s);
 }
 
+// Column sums of a row-major [M][N] block (bias gradients): a workgroup owns
+// 64 columns; its four waves take every fourth row, each lane keeping eight
+// independent partial sums (eight loads in flight, rows coalesced across
+// the wave), then the four waves' sums meet in LDS.
 __global__ __launch_bounds__(256) void k_colsum(const float* X, int64_t ld, int64_t M, int64_t N, float scale,
                                                 float* out, int accumulate) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  float acc = 0.0f;
-  for (int64_t m = 0; m < M; ++m) acc += X[m * ld + n];
-  acc *= scale;
-  out[n] = accumulate ? out[n] + acc : acc;
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  const int64_t n = (int64_t)blockIdx.x * 64 + c;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (n < N) {
+    int64_t m = rg;
+    for (; m + 28 < M; m += 32) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += X[(m + 4 * u) * ld + n];
+    }
+    for (; m < M; m += 4) acc[0] += X[m * ld + n];
+  }
+  part[rg][c] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  __syncthreads();
+  if (rg == 0 && n < N) {
+    const float t = (((part[0][c] + part[1][c]) + part[2][c]) + part[3][c]) * scale;
+    out[n] = accumulate ? out[n] + t : t;
+  }
 }
 
 hipError_t launch_colsum(const float* X, int64_t ld, int64_t M, int64_t N, float scale, float* out, int accumulate,
                          hipStream_t s) {
   if (N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_colsum, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, s, X, ld, M, N, scale, out,
+  hipLaunchKernelGGL(k_colsum, dim3((uint32_t)((N + 63) / 64)), dim3(256), 0, s, X, ld, M, N, scale, out,
                      accumulate);
   return hipGetLastError();
 }

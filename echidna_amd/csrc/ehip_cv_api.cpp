@@ -24,6 +24,7 @@ struct ehip_cv {
   uint64_t off[7] = {};  // W1 b1 W2 b2 W3 b3 end
   DevBuf params, grads, mom1, mom2;
   DevBuf act;       // H1, H2, Lf, dLf, dZ2, dZ1, loss for `cap` tuples
+  DevBuf ws;        // split-K partial sums of the dZ2 GEMM (16 slices)
   uint32_t cap = 0;
   uint64_t step = 0;
   float gain = 1.0f;  // L = gain * (W3 h2 + b3)
@@ -42,6 +43,7 @@ int32_t cv_reserve(ehip_cv* cv, uint32_t T) {
   if (T <= cv->cap) return EHIP_OK;
   const size_t per = 4ull * cv->H + 2ull * cv->n * cv->r + 1;
   HIP_TRY(cv->act.reserve(per * T * sizeof(float)), "control variate activations");
+  HIP_TRY(cv->ws.reserve(16ull * T * cv->H * sizeof(float)), "control variate workspace");
   cv->cap = T;
   return EHIP_OK;
 }
@@ -124,9 +126,12 @@ int32_t cv_backward_chunk(ehip_cv* cv, const float* X, const float* V, const flo
   g.accumulate = accumulate;
   HIP_TRY(launch_gemm(g, s), "cv backward");
   HIP_TRY(launch_colsum(a.dLf, nr, T, nr, 1.0f, G(cv, 5), accumulate, s), "cv backward");
-  // dZ2 = (dLf W3) * (1 - H2^2)
-  HIP_TRY(launch_gemm(gemm(a.dLf, nr, 1, P(cv, 4), H, 1, a.dZ2, H, (int)T, H, nr, EPI_DTANH, nullptr, a.H2, H), s),
-          "cv backward");
+  // dZ2 = (dLf W3) * (1 - H2^2): K = n r is long and M x N small, so the
+  // launcher may split K over the workspace
+  g = gemm(a.dLf, nr, 1, P(cv, 4), H, 1, a.dZ2, H, (int)T, H, nr, EPI_DTANH, nullptr, a.H2, H);
+  g.part = (float*)cv->ws.p;
+  g.part_cap = (int64_t)(cv->ws.n / sizeof(float));
+  HIP_TRY(launch_gemm(g, s), "cv backward");
   g = gemm(a.dZ2, 1, H, a.H1, H, 1, G(cv, 2), H, H, H, (int)T);
   g.accumulate = accumulate;
   HIP_TRY(launch_gemm(g, s), "cv backward");
@@ -230,6 +235,7 @@ void ehip_cv_destroy(ehip_cv* cv) {
   cv->mom1.release();
   cv->mom2.release();
   cv->act.release();
+  cv->ws.release();
   delete cv;
 }
 
