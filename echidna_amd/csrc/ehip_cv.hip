@@ -300,10 +300,108 @@ __global__ __launch_bounds__(256) void k_cv_loss(CvLoss a) {
   }
 }
 
+// The same three passes for a compile-time rank R (a multiple of 4): each
+// lane loads its rows of L as float4s (a wave reads contiguous rows) and
+// both inner loops of a pass reuse the loaded row.
+template <int R>
+__global__ __launch_bounds__(256) void k_cv_loss_r(CvLoss a) {
+  static_assert(R % 4 == 0 && R <= RMAX, "rank");
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __shared__ float red[4 * (R + 2)];
+  const uint64_t t = blockIdx.x;
+  if (t >= a.T) return;
+  const uint32_t n = a.n, npos = a.npos;
+  const f4* L = reinterpret_cast<const f4*>(a.L + t * (uint64_t)n * R);
+  const float* v = a.v + t * (uint64_t)n;
+  const float* hv = a.hv + t * (uint64_t)n;
+  auto row = [&](uint32_t i, float* l) {
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q) {
+      const f4 x = L[(uint64_t)i * (R / 4) + q];
+      l[4 * q] = x.x; l[4 * q + 1] = x.y; l[4 * q + 2] = x.z; l[4 * q + 3] = x.w;
+    }
+  };
+  // pass 1: u = L^T v, |L|_F^2 with the signs of S, v.hv
+  float acc[R + 2];
+#pragma unroll
+  for (int j = 0; j < R + 2; ++j) acc[j] = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    float l[R];
+    row(i, l);
+    const float vi = v[i];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      acc[j] += l[j] * vi;
+      acc[R] += (j < (int)npos) ? l[j] * l[j] : -(l[j] * l[j]);
+    }
+    acc[R + 1] += vi * hv[i];
+  }
+  block_sum<R + 2>(acc, red);
+  float su[R];
+  float uu = 0.0f;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    su[j] = (j < (int)npos) ? acc[j] : -acc[j];
+    uu += su[j] * acc[j];
+  }
+  const float lnorm = acc[R], vhv = acc[R + 1];
+  // pass 2: e = hv - L S u, |e|^2, w = L^T e
+  float acc2[R + 2];
+#pragma unroll
+  for (int j = 0; j < R + 2; ++j) acc2[j] = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    float l[R];
+    row(i, l);
+    float lu = 0.0f;
+#pragma unroll
+    for (int j = 0; j < R; ++j) lu += l[j] * su[j];
+    const float e = hv[i] - lu;
+    acc2[R] += e * e;
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc2[j] += l[j] * e;
+  }
+  __syncthreads();
+  block_sum<R + 2>(acc2, red);
+  if (threadIdx.x == 0) {
+    if (a.loss) a.loss[t] = acc2[R] / (float)n;
+    if (a.vhv) a.vhv[t] = vhv;
+    if (a.cv) a.cv[t] = uu - lnorm;
+  }
+  if (!a.dL) return;
+  // pass 3: dL = gscale (e (S u)^T + v (S w)^T)
+  f4* dL = reinterpret_cast<f4*>(a.dL + t * (uint64_t)n * R);
+  float sw[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) sw[j] = (j < (int)npos) ? acc2[j] : -acc2[j];
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    float l[R];
+    row(i, l);
+    float lu = 0.0f;
+#pragma unroll
+    for (int j = 0; j < R; ++j) lu += l[j] * su[j];
+    const float e = hv[i] - lu, vi = v[i];
+#pragma unroll
+    for (int q = 0; q < R / 4; ++q) {
+      f4 o;
+      o.x = a.gscale * (e * su[4 * q] + vi * sw[4 * q]);
+      o.y = a.gscale * (e * su[4 * q + 1] + vi * sw[4 * q + 1]);
+      o.z = a.gscale * (e * su[4 * q + 2] + vi * sw[4 * q + 2]);
+      o.w = a.gscale * (e * su[4 * q + 3] + vi * sw[4 * q + 3]);
+      dL[(uint64_t)i * (R / 4) + q] = o;
+    }
+  }
+}
+
 hipError_t launch_cv_loss(const CvLoss& a, hipStream_t s) {
   if (a.T == 0) return hipSuccess;
   if (a.r == 0 || a.r > (uint32_t)RMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_cv_loss, dim3((uint32_t)a.T), dim3(256), 0, s, a);
+  // float4 rows need 16-byte aligned tuples
+  const bool al = ((reinterpret_cast<uintptr_t>(a.L) | reinterpret_cast<uintptr_t>(a.dL)) & 15u) == 0;
+  const dim3 grid((uint32_t)a.T), block(256);
+  if (al && a.r == 8) hipLaunchKernelGGL(k_cv_loss_r<8>, grid, block, 0, s, a);
+  else if (al && a.r == 16) hipLaunchKernelGGL(k_cv_loss_r<16>, grid, block, 0, s, a);
+  else if (al && a.r == 32) hipLaunchKernelGGL(k_cv_loss_r<32>, grid, block, 0, s, a);
+  else hipLaunchKernelGGL(k_cv_loss, grid, block, 0, s, a);
   return hipGetLastError();
 }
 

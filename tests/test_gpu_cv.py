@@ -103,16 +103,19 @@ def test_gradients_chunked_equal_single_pass(hip):
         assert rel(a, b) <= 2e-3, k
 
 
-def test_split_k_gradients_match_reference(hip):
-    """n r = 4096 over a 300-tuple batch: the dZ2 product runs split over K
-    (few output tiles, long K); gradients still match the reference."""
-    n2, r2, h2, T = 512, 8, 64, 300
+@pytest.mark.parametrize("r2", [8, 16])
+def test_split_k_gradients_match_reference(hip, r2):
+    """n r >= 4096 over a 300-tuple batch: the dZ2 product runs split over K
+    (few output tiles, long K), and the loss runs its rank-specialised
+    kernel with signed columns; gradients still match the reference."""
+    n2, h2, T = 512, 64, 300
     net = lcv.LearnedControlVariate(hip, n2, r2, h2, seed=11, out_scale=1.0)
+    net.set_signs(r2 // 2)
     X = wl.points_normal(21, T, n2).astype(np.float32)
     V = philox.normal(22, philox.STREAM_DIRS, np.arange(T * n2)).reshape(T, n2).astype(np.float32)
     HV = (0.5 * V).astype(np.float32)
     loss = net.loss_grad(*[torch.from_numpy(a).cuda() for a in (X, V, HV)])
-    ref_loss, ref_g = cv_ref.loss_grad(net.params(), X, V, HV, n2, h2, r2, bf=True)
+    ref_loss, ref_g = cv_ref.loss_grad(net.params(), X, V, HV, n2, h2, r2, bf=True, npos=r2 // 2)
     assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
     for k, (a, b) in enumerate(zip(net.split(net.grads()), net.split(ref_g))):
         assert rel(a, b) <= 2e-3, k
