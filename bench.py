@@ -354,8 +354,9 @@ def run_hvp(args) -> dict:
 
 def run_dags(args) -> dict:
     """64 random DAGs (seed 3): lengths log-U[200, 20000], n ~ U[4, 128],
-    B_i log-U[1e3, 1e5], x ~ U[-1, 1]^n, f32 fwd+grad: one persistent
-    launch over the (tape, point-tile) queue vs one launch per tape."""
+    B_i log-U[1e3, 1e5], x ~ U[-1, 1]^n, fwd+grad in --dtype: one
+    persistent launch over the (tape, point-tile) queue vs one launch per
+    tape."""
     import torch
 
     from echidna_amd import workloads as wl
@@ -363,14 +364,17 @@ def run_dags(args) -> dict:
     from echidna_amd.tape import GpuTapeData
 
     specs = wl.random_dag_suite(3, 64, args.dag_scale)
+    f64 = args.dtype == "f64"
+    dt, tdt = (np.float64, torch.float64) if f64 else (np.float32, torch.float32)
     ctx = HipContext.new(0)
     jobs, ops = [], 0
     t0 = time.perf_counter()
     for k, (length, n, B) in enumerate(specs):
         tape = wl.random_dag(3, k, length, n)
-        bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
-        x = torch.from_numpy(wl.points_uniform(300 + k, B, n, -1.0, 1.0).astype(np.float32)).cuda()
-        jobs.append((bufs, 1, x, B, torch.empty(B, device="cuda"), torch.empty(B, n, device="cuda")))
+        bufs = ctx.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+        x = torch.from_numpy(wl.points_uniform(300 + k, B, n, -1.0, 1.0).astype(dt)).cuda()
+        jobs.append((bufs, 1, x, B, torch.empty(B, dtype=tdt, device="cuda"),
+                     torch.empty(B, n, dtype=tdt, device="cuda")))
         ops += B * tape.counts()["N"]
     t_setup = time.perf_counter() - t0
     points = sum(j[3] for j in jobs)
@@ -391,7 +395,7 @@ def run_dags(args) -> dict:
 
     dt_s, _ = timed(per_tape, reps)
     return {"metric": "fwd+grad point-evals/s", "value": points / dt_q, "unit": "points/s", "n_gpus": 1,
-            "higher_is_better": True, "vs_baseline": None, "dtype": "f32", "data": "synthetic random DAGs",
+            "higher_is_better": True, "vs_baseline": None, "dtype": args.dtype, "data": "synthetic random DAGs",
             "config": {"workload": f"64 random DAGs seed 3 scale {args.dag_scale}, {points} points, "
                                    f"{ops:.3e} entry-evals", "points": points,
                        "fetch": "scalar" if fetch else "lds"},
