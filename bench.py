@@ -39,8 +39,11 @@ def parse():
     ap.add_argument("--batch-log2", type=int, default=20)
     ap.add_argument("--wg", type=int, default=0)
     ap.add_argument("--lds", type=int, default=0)
-    ap.add_argument("--fetch", type=int, choices=[0, 1], default=0,
-                    help="with --wg/--lds: program fetch, 0 LDS windows, 1 scalar cache")
+    ap.add_argument("--fetch", type=int, choices=[0, 1, 2], default=0,
+                    help="with --wg/--lds: program fetch, 0 LDS windows, 1 scalar cache, "
+                         "2 tape-specialised kernel (mlp)")
+    ap.add_argument("--no-spec", action="store_true",
+                    help="mlp: leave the tape-specialised kernel out (interpreters only)")
     ap.add_argument("--no-fuse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="points per CPU-baseline pass")
@@ -433,11 +436,19 @@ def main() -> int:
         return 1
     if args.wg or args.lds:
         ctx.set_launch(args.wg, args.lds)
-        ctx.set_fetch(args.fetch)
+        if args.fetch != 2:
+            ctx.set_fetch(args.fetch)
     if args.no_fuse:
         ctx.set_fusion(False)
     data = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
     bufs = ctx.upload_tape(data)
+    from echidna_amd.backend import TUNE_GRADIENT
+    spec = None
+    if not args.no_spec:
+        # straight-line kernel for this tape (hipRTC), built before any timing
+        spec = ctx.specialize(bufs, TUNE_GRADIENT)
+    if args.fetch == 2 and args.wg:
+        ctx.set_tuned(bufs, TUNE_GRADIENT, args.wg, 0, fetch=2)
     tdt = torch.float64 if f64 else torch.float32
     dx = torch.from_numpy(x.astype(np.float64 if f64 else np.float32)).cuda()
     dy = torch.empty(B, dtype=tdt, device="cuda")
@@ -458,7 +469,6 @@ def main() -> int:
     tuned = None
     if not (args.wg or args.lds):
         # launch shape from the autotuner, chosen before the timed region
-        from echidna_amd.backend import TUNE_GRADIENT
         tuned = ctx.autotune_dev(bufs, TUNE_GRADIENT, B, dx, d_y=dy, d_g=dg)
         torch.cuda.synchronize()
     with torch.cuda.stream(stream):
@@ -520,7 +530,7 @@ def main() -> int:
                                f"B=2^{args.batch_log2} points, fwd+grad",
                    "points": B, "inputs": n, "hidden": H, "computed_entries": counts["N"],
                    "program": bufs.program_info(1), "parallelism": "none (1 GPU)",
-                   "launch": launch},
+                   "launch": launch, "specialized": spec},
         "graph_ms_per_step": graph_ms,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,

@@ -113,6 +113,10 @@ _SIGS = {
     "ehip_graph_destroy": (None, [_vp]),
     "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
+    "ehip_tape_specialize": (ctypes.c_int32, [_vp, ctypes.c_int32, _c_u32p]),
+    "ehip_spec_source": (ctypes.c_int32, [ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _c_f64p, ctypes.c_uint32,
+                                          ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32, ctypes.c_int32,
+                                          ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _c_u64p]),
     "ehip_set_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_uint32]),
     "ehip_set_fetch": (ctypes.c_int32, [_vp, ctypes.c_int32]),
@@ -244,6 +248,28 @@ def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Op
     if mode & 2:
         return words[:nw], None, info2
     return words[:nw].reshape(-1, 8), terms[:nt], info2
+
+
+def spec_source(data: GpuTapeData, what: int = 1, f64: Optional[bool] = None, arch: Optional[str] = None):
+    """The tape-specialised kernel's source (no device needed).
+
+    With ``arch`` (e.g. "gfx950") the source is also compiled by hipRTC.
+    Returns (source text, {"code_bytes", "compile_ms"}).
+    """
+    lib = load_library()
+    f64 = (np.asarray(data.constants).dtype == np.float64) if f64 is None else f64
+    consts = np.ascontiguousarray(data.constants, dtype=np.float64)
+    outs = _u32(data.output_indices)
+    arrs = [_u32(data.opcodes), _u32(data.arg0), _u32(data.arg1)]
+    info = (ctypes.c_uint64 * 3)()
+    call = lambda a, buf, cap: lib.ehip_spec_source(
+        EHIP_F64 if f64 else EHIP_F32, *[x.ctypes.data_as(_c_u32p) for x in arrs], consts.ctypes.data_as(_c_f64p),
+        data.num_ops, data.num_inputs, data.output_index, outs.ctypes.data_as(_c_u32p), len(outs), what, a, buf,
+        cap, info)
+    _check(call(None, None, 0))
+    buf = ctypes.create_string_buffer(int(info[0]) + 1)
+    _check(call(arch.encode() if arch else None, buf, len(buf)))
+    return buf.value.decode(), {"code_bytes": int(info[1]), "compile_ms": info[2] / 1000.0}
 
 
 class HipTapeBuffers:
@@ -474,9 +500,22 @@ class HipContext:
         return g
 
     @staticmethod
+    def specialize(tape: "HipTapeBuffers", what: int = 1) -> dict:
+        """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient).
+
+        Select it with ``set_tuned(..., fetch=2)`` or let ``autotune_dev``
+        consider it.
+        """
+        out = (ctypes.c_uint32 * 4)()
+        _check(load_library().ehip_tape_specialize(tape.handle, what, out))
+        return {"vgprs": int(out[0]), "scratch": int(out[1]), "compile_ms": out[2] / 1000.0,
+                "code_bytes": int(out[3])}
+
+    @staticmethod
     def set_tuned(tape: "HipTapeBuffers", what: int, wg: int, lds_budget: int, order: int = 0,
                   fetch: int = 0) -> None:
-        """Pin a launch shape; ``fetch`` 0 stages the program in LDS windows, 1 reads it through the scalar cache."""
+        """Pin a launch shape; ``fetch`` 0 stages the program in LDS windows, 1 reads it through the
+        scalar cache, 2 runs the tape-specialised kernel (``specialize``)."""
         _check(load_library().ehip_set_tuned(tape.handle, what, order, wg, lds_budget, fetch))
 
     @staticmethod

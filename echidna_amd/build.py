@@ -17,7 +17,10 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libechidna_hip.so")
 
 SOURCES = ["ehip_api.cpp", "ehip_program.cpp", "ehip_kernels.hip", "ehip_stde.hip", "ehip_cv.hip",
-           "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp"]
+           "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp", "ehip_spec.cpp"]
+# Headers whose text the library carries for hipRTC (tape-specialised kernels):
+# (file, C symbol).
+EMBED = [("ehip_math.h", "ehip_math_h_text")]
 # The interpreters are built once per program-fetch mode, each in its own
 # namespace (object name, defines).
 VARIANTS = {"ehip_kernels.hip": [("ehip_kernels_lds", ["-DEHIP_SFETCH=0", "-DEHIP_NS=lds_fetch"]),
@@ -49,6 +52,18 @@ def _run(cmd, verbose):
     return r.returncode
 
 
+def _embed_source(path: str) -> str:
+    """A C++ source defining each EMBED header's text as a char array."""
+    out = []
+    for name, sym in EMBED:
+        text = open(os.path.join(CSRC, name), "rb").read()
+        body = ",".join(str(b) for b in text)
+        out.append(f'extern "C" const char {sym}[] = {{{body},0}};\n')
+    with open(path, "w") as f:
+        f.write("".join(out))
+    return path
+
+
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8"))))
@@ -59,12 +74,16 @@ def build(verbose: bool = False) -> str:
                 obj = os.path.join(tmpdir, name + ".o")
                 objs.append(obj)
                 cmds.append([hipcc, *FLAGS, *EXTRA.get(s, []), *defs, "-c", "-o", obj, os.path.join(CSRC, s)])
+        emb = _embed_source(os.path.join(tmpdir, "ehip_embed.cpp"))
+        objs.append(os.path.join(tmpdir, "ehip_embed.o"))
+        cmds.append([hipcc, "-O1", "-fPIC", "-c", "-o", objs[-1], emb])
         with ThreadPoolExecutor(jobs) as ex:
             rcs = list(ex.map(lambda c: _run(c, verbose), cmds))
         if any(rcs):
             raise RuntimeError("hipcc failed building libechidna_hip.so")
         tmp = LIB + ".tmp"
-        if _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs],
+        if _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs,
+                 "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"],
                 verbose):
             raise RuntimeError("hipcc failed linking libechidna_hip.so")
     os.replace(tmp, LIB)

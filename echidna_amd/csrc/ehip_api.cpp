@@ -156,6 +156,7 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
 void ehip_tape_free(ehip_tape* t) {
   if (!t) return;
   for (auto& p : t->prog) free_program(p);
+  for (auto& k : t->spec) spec_free(k);
   delete t;
 }
 
@@ -523,6 +524,25 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
     }
     if (rc) break;
   }
+  // the tape-specialised kernel, where one has been compiled (ehip_tape_specialize)
+  if (rc == EHIP_OK && kind == 0 && t->spec[what == 1].fn)
+    for (uint32_t wg : {64u, 128u, 256u}) {
+      Tuned cand;
+      cand.wg = wg;
+      cand.fetch = 2;
+      float ms = 1e30f;
+      for (int rep = 0; rep < 3 && rc == EHIP_OK; ++rep) {
+        HIP_TRY(hipEventRecord(e0, c->stream), "autotune");
+        rc = run(c, t, kind, K, L, c->stream, &cand);
+        HIP_TRY(hipEventRecord(e1, c->stream), "autotune");
+        HIP_TRY(hipEventSynchronize(e1), "autotune");
+        float m = 0.f;
+        HIP_TRY(hipEventElapsedTime(&m, e0, e1), "autotune");
+        if (rep > 0) ms = std::min(ms, m);
+      }
+      if (rc) break;
+      if (ms < best_ms) { best_ms = ms; best = cand; }
+    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
@@ -539,11 +559,14 @@ int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, 
   if (!t) return fail(EHIP_OTHER, "GPU error: null argument");
   if (wg != 0 && wg != 64 && wg != 128 && wg != 256)
     return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
-  if (fetch > 1) return fail(EHIP_OTHER, "GPU error: program fetch must be 0 (LDS) or 1 (scalar)");
+  if (fetch > 2)
+    return fail(EHIP_OTHER, "GPU error: program fetch must be 0 (LDS), 1 (scalar) or 2 (specialised kernel)");
   int key;
   if (what >= 0 && what <= 2) key = what;
   else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
   else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
+  if (fetch == 2 && wg && (what > 1 || !t->spec[what].fn))
+    return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
   t->tuned[key].wg = wg;
   t->tuned[key].lds_budget = wg ? lds_budget : 0;
   t->tuned[key].fetch = wg ? fetch : 0;

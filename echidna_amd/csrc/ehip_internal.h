@@ -105,8 +105,26 @@ struct DevProgram {
 struct Tuned {
   uint32_t wg = 0;          // 0: not tuned, use the context defaults
   uint32_t lds_budget = 0;  // bytes of LDS for the register file
-  uint32_t fetch = 0;       // program fetch: 0 LDS windows, 1 scalar cache
+  uint32_t fetch = 0;       // program fetch: 0 LDS windows, 1 scalar cache, 2 tape-specialised kernel
 };
+
+// A tape-specialised kernel (straight-line code compiled for one tape by
+// hipRTC, ehip_spec.cpp): ehip_spec(X, xstride, Y, G, B).
+struct SpecKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  uint32_t vgprs = 0, scratch = 0, code_bytes = 0;
+  double compile_ms = 0;
+};
+
+// Longest tape the specialiser accepts (compile time and code size grow
+// with the tape).
+constexpr uint32_t SPEC_MAX_ENTRIES = 1u << 16;
+
+namespace ehip_detail {
+hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStream_t st);
+void spec_free(SpecKernel& s);
+}  // namespace ehip_detail
 
 struct ehip_tape {
   ehip_ctx* ctx = nullptr;
@@ -116,6 +134,7 @@ struct ehip_tape {
   DevProgram prog[4];  // forward, gradient, HVP (dual interpreter), kinks; all with DENSE groups
   uint32_t nkinks = 0;
   Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
+  SpecKernel spec[2];  // tape-specialised forward and gradient kernels, once compiled
 };
 
 namespace ehip_detail {
@@ -172,7 +191,15 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   const size_t S = t->f64 ? 8 : 4;
   const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
   const uint32_t wg = tu.wg ? tu.wg : c->wg;
-  const bool sf = (tu.wg ? tu.fetch : c->fetch) != 0;
+  const uint32_t fm = tu.wg ? tu.fetch : c->fetch;
+  if (fm == 2 && kind == 0) {
+    const SpecKernel& sk = t->spec[L.g ? 1 : 0];
+    if (!sk.fn) return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
+    hipError_t e = spec_launch(sk, L, wg, s);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return EHIP_OK;
+  }
+  const bool sf = fm == 1;
   const size_t per_reg = (size_t)NC * wg * S;
   // LDS fetch: two instruction windows share the workgroup's LDS with the registers
   const size_t win_lds = window_lds(sf);

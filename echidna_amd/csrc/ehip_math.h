@@ -10,8 +10,18 @@
 // from the ROCm device math library (ocml); powi uses repeated squaring like
 // llvm.powi on the host.
 #pragma once
+// Also compiled at run time by hipRTC (tape-specialised kernels), which
+// supplies the HIP runtime declarations and fixed-width integers itself.
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#define INT32_MIN (-2147483647 - 1)
+#endif
 
 namespace ehip {
 

@@ -1,0 +1,322 @@
+// Tape-specialised kernels: the whole forward (and reverse) sweep of one
+// tape written out as straight-line HIP and compiled at run time by hipRTC
+// for the device's architecture.  Every tape entry becomes a named value
+// that the compiler keeps in VGPRs (no register file in LDS or HBM), every
+// constant becomes an instruction literal, and the program stream
+// disappears.  Worth it for short tapes evaluated over many points, where the
+// interpreter's register traffic and dispatch cost dominate; the compile
+// costs a fraction of a second to tens of seconds, paid once per tape.
+//
+// The arithmetic is the interpreters' own: every entry calls the same
+// s_eval / s_partials / s_powi / s_powi_partial device functions (the
+// embedded text of ehip_math.h), add/sub/mul/neg are written as the plain
+// operators those functions apply, and the sweeps follow the reference's
+// order exactly:
+//   forward   forward_into/forward_dispatch (src/bytecode_tape/forward.rs:11-43, 194-229)
+//   reverse   reverse_sweep_core (src/bytecode_tape/reverse.rs:32-105): adjoints
+//             start at zero, output 0 is seeded with 1, entries are visited
+//             last to first, an entry whose adjoint is zero is skipped, and
+//             a partial that is zero adds nothing.
+// Two exact simplifications shape the generated reverse sweep:
+//   * an entry whose adjoint nothing has written yet is zero at run time for
+//     every point, so it is skipped when generating the code;
+//   * an update g_a = g_a + c * g with c a finite non-zero constant (add,
+//     sub, neg, multiplication by a recorded constant) is emitted without the
+//     zero test: when g is +-0, c * g is +-0 and g_a + (+-0) == g_a bit for bit,
+//     because an adjoint that starts at +0 and only ever has values added to
+//     it is never -0 under round-to-nearest.
+// Updates to entries that depend on no input are dropped: their adjoints
+// flow only into other such entries, never into the gradient.
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cinttypes>
+#include <cmath>
+
+#include "ehip_internal.h"
+
+extern "C" const char ehip_math_h_text[];  // ehip_math.h, embedded when the library is built
+
+namespace ehip_detail {
+
+namespace {
+
+// opcodes (src/opcode.rs:21-130) the generator treats specially
+constexpr uint32_t OP_INPUT = 0, OP_CONST = 1, OP_ADD = 2, OP_SUB = 3, OP_MUL = 4, OP_NEG = 12, OP_POWI = 16;
+constexpr uint32_t UNUSED = 0xFFFFFFFFu;
+
+struct Gen {
+  const TapeHost& h;
+  const bool f64;
+  std::string out;
+  char buf[256];
+
+  Gen(const TapeHost& th) : h(th), f64(th.f64) { out.reserve(h.opcodes.size() * 96); }
+
+  void put(const char* fmt, ...) __attribute__((format(printf, 2, 3))) {
+    va_list ap;
+    va_start(ap, fmt);
+    const int n = std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    out.append(buf, (size_t)std::min(n, (int)sizeof buf - 1));
+  }
+
+  // A constant as a bit-exact literal of the tape's dtype.
+  std::string lit(double v) const {
+    char b[64];
+    if (f64) {
+      uint64_t u;
+      std::memcpy(&u, &v, 8);
+      std::snprintf(b, sizeof b, "__builtin_bit_cast(double, 0x%016" PRIx64 "ull)", u);
+    } else {
+      const float f = (float)v;
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      std::snprintf(b, sizeof b, "__builtin_bit_cast(float, 0x%08" PRIx32 "u)", u);
+    }
+    return b;
+  }
+
+  // Value of constant entry `i` in the tape's dtype.
+  double cval(uint32_t i) const { return f64 ? h.values[i] : (double)(float)h.values[i]; }
+};
+
+}  // namespace
+
+std::string spec_source(const TapeHost& h, bool grad) {
+  Gen g(h);
+  const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
+  g.put("// generated: %u entries, %u inputs, %u outputs, %s%s\n", N, n, m, h.f64 ? "f64" : "f32",
+        grad ? ", value and gradient of output 0" : ", values");
+  g.put("#include \"ehip_math.h\"\n");
+  g.put("typedef %s F;\n", h.f64 ? "double" : "float");
+  g.put("extern \"C\" __global__ void __launch_bounds__(256) ehip_spec(const F* __restrict__ X, "
+        "unsigned long long xs, F* __restrict__ Y, F* __restrict__ G, unsigned long long B) {\n");
+  g.put("const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n");
+  g.put("if (p >= B) return;\n");
+  g.put("const F* __restrict__ x = X + p * xs;\n");
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t op = h.opcodes[i], a = h.arg0[i], b = h.arg1[i];
+    switch (op) {
+      case OP_INPUT: g.put("const F v%u = x[%u];\n", i, i); break;
+      case OP_CONST: g.out += "const F v" + std::to_string(i) + " = " + g.lit(h.values[i]) + ";\n"; break;
+      case OP_ADD: g.put("const F v%u = v%u + v%u;\n", i, a, b); break;
+      case OP_SUB: g.put("const F v%u = v%u - v%u;\n", i, a, b); break;
+      case OP_MUL: g.put("const F v%u = v%u * v%u;\n", i, a, b); break;
+      case OP_NEG: g.put("const F v%u = -v%u;\n", i, a); break;
+      case OP_POWI: g.put("const F v%u = ehip::s_powi<F>(v%u, %d);\n", i, a, (int32_t)b); break;
+      default:
+        if (b != UNUSED) g.put("const F v%u = ehip::s_eval<F>(%uu, v%u, v%u);\n", i, op, a, b);
+        else g.put("const F v%u = ehip::s_eval<F>(%uu, v%u, F(0));\n", i, op, a);
+    }
+  }
+  g.put("if (Y) {\nF* __restrict__ y = Y + p * %uull;\n", m);
+  for (uint32_t k = 0; k < m; ++k) g.put("y[%u] = v%u;\n", k, h.outputs[k]);
+  g.put("}\n");
+  if (grad) {
+    // active[i]: entry i depends on an input
+    std::vector<uint8_t> active(N, 0), touched(N, 0);
+    for (uint32_t i = 0; i < N; ++i) {
+      const uint32_t op = h.opcodes[i];
+      if (op == OP_INPUT) active[i] = 1;
+      else if (op != OP_CONST)
+        active[i] = active[h.arg0[i]] || (op != OP_POWI && h.arg1[i] != UNUSED && active[h.arg1[i]]);
+    }
+    const uint32_t seed = h.outputs[0];
+    g.put("F g%u = F(1);\n", seed);
+    touched[seed] = 1;
+    auto declare = [&](uint32_t a) {
+      if (!touched[a]) {
+        g.put("F g%u = F(0);\n", a);
+        touched[a] = 1;
+      }
+    };
+    for (uint32_t ii = N; ii-- > 0;) {
+      const uint32_t op = h.opcodes[ii];
+      if (op == OP_INPUT || op == OP_CONST || !touched[ii]) continue;
+      const uint32_t a = h.arg0[ii], b = h.arg1[ii];
+      const bool ua = active[a];
+      const bool ub = op != OP_POWI && b != UNUSED && active[b];
+      if (!ua && !ub) continue;
+      // static partials (finite, in the tape's dtype) where the op fixes them
+      bool stat = false;
+      double ca = 0, cb = 0;
+      if (op == OP_ADD) stat = true, ca = 1, cb = 1;
+      else if (op == OP_SUB) stat = true, ca = 1, cb = -1;
+      else if (op == OP_NEG) stat = true, ca = -1;
+      else if (op == OP_MUL && h.opcodes[b] == OP_CONST && std::isfinite(g.cval(b))) stat = true, ca = g.cval(b);
+      else if (op == OP_MUL && h.opcodes[a] == OP_CONST && std::isfinite(g.cval(a))) stat = true, cb = g.cval(a);
+      if (stat) {
+        if (ua && ca != 0.0) {
+          declare(a);
+          g.out += "g" + std::to_string(a) + " = g" + std::to_string(a) + " + " + g.lit(ca) + " * g" +
+                   std::to_string(ii) + ";\n";
+        }
+        if (ub && cb != 0.0) {
+          declare(b);
+          g.out += "g" + std::to_string(b) + " = g" + std::to_string(b) + " + " + g.lit(cb) + " * g" +
+                   std::to_string(ii) + ";\n";
+        }
+        continue;
+      }
+      if (ua) declare(a);
+      if (ub) declare(b);
+      if (op == OP_POWI) {
+        g.put("{ const F da = ehip::s_powi_partial<F>(%d, v%u, v%u);\n", (int32_t)b, a, ii);
+        g.put("if (g%u != F(0) && da != F(0)) g%u = g%u + da * g%u; }\n", ii, a, a, ii);
+        continue;
+      }
+      if (b != UNUSED) g.put("{ F da, db; ehip::s_partials<F>(%uu, v%u, v%u, v%u, da, db);\n", op, a, b, ii);
+      else g.put("{ F da, db; ehip::s_partials<F>(%uu, v%u, F(0), v%u, da, db);\n", op, a, ii);
+      g.put("const bool lv = g%u != F(0);\n", ii);
+      if (ua) g.put("if (lv && da != F(0)) g%u = g%u + da * g%u;\n", a, a, ii);
+      if (ub) g.put("if (lv && db != F(0)) g%u = g%u + db * g%u;\n", b, b, ii);
+      g.put("}\n");
+    }
+    g.put("F* __restrict__ gp = G + p * %uull;\n", n);
+    for (uint32_t k = 0; k < n; ++k) {
+      if (touched[k]) g.put("gp[%u] = g%u;\n", k, k);
+      else g.put("gp[%u] = F(0);\n", k);
+    }
+  }
+  g.put("}\n");
+  return g.out;
+}
+
+int32_t spec_compile(const std::string& src, const char* arch, std::vector<char>& code, std::string& log,
+                     double* ms) {
+  const auto t0 = std::chrono::steady_clock::now();
+  hiprtcProgram prog;
+  const char* hdrs[] = {ehip_math_h_text};
+  const char* names[] = {"ehip_math.h"};
+  if (hiprtcCreateProgram(&prog, src.c_str(), "ehip_spec.hip", 1, hdrs, names) != HIPRTC_SUCCESS)
+    return fail(EHIP_SHADER_COMPILATION, "hipRTC: could not create the program");
+  std::string archopt = std::string("--offload-arch=") + arch;
+  const char* opts[] = {archopt.c_str(), "-O3", "-std=c++17",
+                        // reference arithmetic: no a*b+c contraction, denormals kept
+                        "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
+                        // no packed FP32 instructions
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  log.assign(ls, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  if (r != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return fail(EHIP_SHADER_COMPILATION, "hipRTC compile failed: %.400s", log.c_str());
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  code.resize(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return EHIP_OK;
+}
+
+void spec_free(SpecKernel& s) {
+  if (s.mod) (void)hipModuleUnload(s.mod);
+  s = SpecKernel{};
+}
+
+hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStream_t st) {
+  const void* X = L.x;
+  unsigned long long xs = L.xstride;
+  void* Y = L.y;
+  void* G = L.g;
+  unsigned long long B = L.B;
+  void* args[] = {(void*)&X, (void*)&xs, (void*)&Y, (void*)&G, (void*)&B};
+  const uint64_t grid = (L.B + wg - 1) / wg;
+  return hipModuleLaunchKernel(s.fn, (uint32_t)grid, 1, 1, wg, 1, 1, 0, st, args, nullptr);
+}
+
+}  // namespace ehip_detail
+
+using namespace ehip_detail;
+
+int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
+  if (!t || !out4) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (what != 0 && what != 1) return fail(EHIP_OTHER, "GPU error: specialised kernels exist for forward (0) and gradient (1)");
+  if (t->host.opcodes.size() > SPEC_MAX_ENTRIES)
+    return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%zu entries > %u)", t->host.opcodes.size(),
+                SPEC_MAX_ENTRIES);
+  ehip_ctx* c = t->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, c->device), "device properties");
+  std::string arch = prop.gcnArchName;
+  arch = arch.substr(0, arch.find(':'));
+  const std::string src = spec_source(t->host, what == 1);
+  std::vector<char> code;
+  std::string log;
+  double ms = 0;
+  int32_t rc = spec_compile(src, arch.c_str(), code, log, &ms);
+  if (rc) return rc;
+  SpecKernel s;
+  HIP_TRY(hipModuleLoadData(&s.mod, code.data()), "load specialised kernel");
+  hipError_t e = hipModuleGetFunction(&s.fn, s.mod, "ehip_spec");
+  if (e != hipSuccess) {
+    spec_free(s);
+    return hip_fail(e, "load specialised kernel");
+  }
+  int v = 0;
+  if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, s.fn) == hipSuccess) s.vgprs = (uint32_t)v;
+  if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, s.fn) == hipSuccess) s.scratch = (uint32_t)v;
+  s.code_bytes = (uint32_t)code.size();
+  s.compile_ms = ms;
+  spec_free(t->spec[what]);
+  t->spec[what] = s;
+  out4[0] = s.vgprs;
+  out4[1] = s.scratch;
+  out4[2] = (uint32_t)std::min(ms * 1000.0, 4.0e9);
+  out4[3] = s.code_bytes;
+  return ok();
+}
+
+int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                         const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
+                         const uint32_t* output_indices, uint32_t num_outputs, int32_t what, const char* arch,
+                         char* src, uint64_t src_cap, uint64_t* out3) {
+  if (!out3 || (what != 0 && what != 1) || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
+      (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
+    return fail(EHIP_OTHER, "GPU error: bad argument");
+  for (uint32_t i = 0; i < num_ops; ++i)
+    if (opcodes[i] == 43u) return fail(EHIP_CUSTOM_OPS_NOT_SUPPORTED, "tape contains custom ops which cannot run on GPU");
+  std::vector<uint32_t> outs;
+  if (num_outputs == 0) outs.push_back(output_index);
+  else outs.assign(output_indices, output_indices + num_outputs);
+  std::string msg = validate_tape(opcodes, arg0, arg1, num_ops, num_inputs, num_ops, num_ops, output_index,
+                                  outs.data(), (uint32_t)outs.size());
+  if (!msg.empty()) return fail(EHIP_OTHER, "%s", msg.c_str());
+  if (num_ops == 0) return fail(EHIP_OTHER, "invalid GpuTapeData: empty tape");
+  if (num_ops > SPEC_MAX_ENTRIES)
+    return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%u entries > %u)", num_ops, SPEC_MAX_ENTRIES);
+  TapeHost th;
+  th.opcodes.assign(opcodes, opcodes + num_ops);
+  th.arg0.assign(arg0, arg0 + num_ops);
+  th.arg1.assign(arg1, arg1 + num_ops);
+  th.values.assign(constants, constants + num_ops);
+  th.num_inputs = num_inputs;
+  th.outputs = outs;
+  th.f64 = dtype == EHIP_F64;
+  const std::string s = spec_source(th, what == 1);
+  out3[0] = s.size();
+  out3[1] = 0;
+  out3[2] = 0;
+  if (arch) {
+    std::vector<char> code;
+    std::string log;
+    double ms = 0;
+    int32_t rc = spec_compile(s, arch, code, log, &ms);
+    if (rc) return rc;
+    out3[1] = code.size();
+    out3[2] = (uint64_t)(ms * 1000.0);
+  }
+  if (src) {
+    if (src_cap < s.size() + 1) return fail(EHIP_OTHER, "GPU error: source buffer too small");
+    std::memcpy(src, s.c_str(), s.size() + 1);
+  }
+  return ok();
+}

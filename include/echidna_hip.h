@@ -223,8 +223,10 @@ int32_t ehip_multi_batch_dev(ehip_ctx* ctx, const ehip_job* jobs, uint32_t njobs
  * on the caller's device buffers for evaluation `what` (0 forward, 1
  * gradient, 2 hvp, 3 taylor of `order`), keeps the fastest on the tape for
  * later calls of that kind, and returns out4 = {workgroup size, LDS budget
- * bytes, microseconds per launch, program fetch (0 LDS, 1 scalar)}.  The
- * shapes are workgroup size 64/128/256 x LDS budget x program fetch.  Buffers are as for the matching *_dev
+ * bytes, microseconds per launch, program fetch (0 LDS, 1 scalar, 2 the
+ * tape-specialised kernel)}.  The shapes are workgroup size 64/128/256 x LDS
+ * budget x program fetch, plus the specialised kernel at each workgroup size
+ * when ehip_tape_specialize has built one.  Buffers are as for the matching *_dev
  * entry point (d_y may be null for gradient/hvp); their contents are
  * overwritten.  Runs on the context stream and synchronizes. */
 int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t order, const void* d_x,
@@ -246,10 +248,35 @@ void ehip_graph_destroy(ehip_graph* graph);
 
 /* Restore (wg != 0) or clear (wg == 0) a tuned launch shape, e.g. from a
  * persisted autotuner cache; ehip_get_tuned reads it back
- * (out3 = {wg, lds, fetch}). */
+ * (out3 = {wg, lds, fetch}).  fetch 2 selects the tape-specialised kernel
+ * (forward and gradient, once ehip_tape_specialize has built it). */
 int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
                        uint32_t fetch);
 int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out3);
+
+/* Tape-specialised kernels (north star: short tapes keep their slots in
+ * registers; no counterpart in the reference, whose CUDA backend generates
+ * source at run time per Taylor order, src/gpu/taylor_codegen.rs, but never
+ * per tape).  Writes the
+ * tape's forward (what 0) or value-and-gradient (what 1) sweep as
+ * straight-line HIP, every entry a named value kept in VGPRs and every
+ * constant an instruction literal, and compiles it with hipRTC for the
+ * tape's device.  Arithmetic and order are the interpreters', so results
+ * are the same.  Selected per evaluation with ehip_set_tuned(fetch = 2) or
+ * by ehip_autotune_dev.  out4 = {VGPRs, scratch bytes per lane, compile
+ * microseconds, code object bytes}.  Tapes longer than 65536 entries are
+ * refused.  Not thread-safe against evaluations of the same tape. */
+int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t* out4);
+
+/* The source ehip_tape_specialize compiles, from the flattened tape arrays
+ * with no device (tape arguments as ehip_compile_program).  When `arch` is
+ * non-null (e.g. "gfx950") the source is also compiled with hipRTC for it.
+ * out3 = {source bytes (without the terminating zero), code object bytes,
+ * compile microseconds}; `src` (may be null) receives the text. */
+int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
+                         const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
+                         const uint32_t* output_indices, uint32_t num_outputs, int32_t what, const char* arch,
+                         char* src, uint64_t src_cap, uint64_t* out3);
 
 /* DENSE grouping for tapes uploaded after this call (default on): fused dot
  * products sharing one operand list run as one register-blocked instruction

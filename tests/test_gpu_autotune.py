@@ -71,6 +71,8 @@ def test_autotune_rejects_bad_arguments(hip):
     with pytest.raises(GpuError, match="unknown evaluation kind"):
         HipContext.set_tuned(bufs, 7, 64, 0)
     with pytest.raises(GpuError, match="program fetch"):
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=3)
+    with pytest.raises(GpuError, match="no specialised kernel"):
         HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=2)
     with pytest.raises(GpuError, match="program fetch"):
         hip.set_fetch(3)

@@ -1,0 +1,165 @@
+"""Tape-specialised kernels (straight-line code per tape, compiled by hipRTC).
+
+They run the interpreters' arithmetic in the interpreters' order, so values
+and gradients equal the interpreter's bit for bit, except the sign bit of
+NaN results, which IEEE 754 leaves unspecified; and they match the f64
+oracle within 1e-12 relative (f64) like every other path.  Checked on
+tapes that use every opcode, on the guard tape (zero, infinite and NaN
+adjoints), on random DAGs and on a multi-output map; through the autotuner,
+hipGraph replays and ragged batches.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+from echidna_amd import workloads as wl
+from echidna_amd.backend import TUNE_FORWARD, TUNE_GRADIENT, HipContext
+from echidna_amd.tape import GpuError, GpuTapeData, record_multi
+from test_structure import TAPES
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a, f64):
+    """Bits with the sign of NaNs cleared."""
+    U = np.uint64 if f64 else np.uint32
+    b = np.asarray(a).view(U).copy()
+    b[np.isnan(a)] &= ~(U(1) << U(63 if f64 else 31))
+    return b
+
+
+def _points(name, B, n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1.2, 1.2, (B, n))
+    if name == "guards":
+        import dense_cases as dc
+
+        x = dc.points(B, seed)
+    x[:3] = 0.0  # zero rows: zero adjoints and kinks
+    return x
+
+
+def _tape(name):
+    if name == "guards":
+        import dense_cases as dc
+
+        return dc.tape()
+    return TAPES[name]()
+
+
+def _both(hip, bufs, x, B, what):
+    """(interpreter, specialised) results of forward (0) or gradient (1)."""
+    outs = []
+    for fetch in (0, 2):
+        if fetch == 2:
+            HipContext.set_tuned(bufs, what, 128, 0, fetch=2)
+        else:
+            HipContext.set_tuned(bufs, what, 256, 24576, fetch=1)
+        if what == TUNE_GRADIENT:
+            y, g = hip.gradient_batch(bufs, x.reshape(-1), B)
+            outs.append((np.asarray(y).copy(), np.asarray(g).copy()))
+        else:
+            outs.append((np.asarray(hip.forward_batch(bufs, x.reshape(-1), B)).copy(),))
+    return outs
+
+
+@pytest.mark.parametrize("name", ["kitchen_sink", "rosenbrock", "mlp", "dag0", "dag1", "guards"])
+@pytest.mark.parametrize("f64", [False, True])
+def test_specialised_equals_interpreter_and_oracle(hip, name, f64):
+    tape = _tape(name)
+    n = tape.num_inputs
+    F = np.float64 if f64 else np.float32
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    for what in (TUNE_FORWARD, TUNE_GRADIENT):
+        info = HipContext.specialize(bufs, what)
+        assert info["vgprs"] > 0 and info["code_bytes"] > 0
+    B = 1000  # not a multiple of any workgroup size
+    x = _points(name, B, n, 21).astype(F)
+    with np.errstate(all="ignore"):
+        for what in (TUNE_FORWARD, TUNE_GRADIENT):
+            interp, spec = _both(hip, bufs, x, B, what)
+            for a, b in zip(interp, spec):
+                assert np.array_equal(_bits(a, f64), _bits(b, f64)), (name, what)
+        if f64:
+            y, g = spec[0].reshape(B), spec[1].reshape(B, n)
+            y_ref, g_ref = po.gradient(tape, x)
+            for got, ref in ((y, y_ref[:, 0]), (g, g_ref)):
+                assert np.array_equal(np.isnan(got), np.isnan(ref))
+                fin = np.isfinite(ref)
+                assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
+                assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin])),
+                              initial=0.0) <= 1e-12
+
+
+def vector_map(v):
+    return [v[0] * v[1] + v[2].sin(), v[1] * 2.5 - v[0], v[2].exp() * v[0], v[1]]
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_specialised_multi_output(hip, f64):
+    """All outputs are written; the gradient is output 0's (the reference
+    seeds output_indices[0]); an output that is an input passes through."""
+    tape, _ = record_multi(vector_map, [0.1, 0.2, 0.3])
+    F = np.float64 if f64 else np.float32
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    HipContext.specialize(bufs, TUNE_GRADIENT)
+    HipContext.specialize(bufs, TUNE_FORWARD)
+    B = 257
+    x = np.random.default_rng(4).uniform(-2, 2, (B, 3)).astype(F)
+    for what in (TUNE_FORWARD, TUNE_GRADIENT):
+        interp, spec = _both(hip, bufs, x, B, what)
+        for a, b in zip(interp, spec):
+            assert np.array_equal(_bits(a, f64), _bits(b, f64))
+    y = spec[0].reshape(B, 4)
+    assert np.array_equal(y[:, 3], x[:, 1])
+    g = spec[1].reshape(B, 3)
+    assert np.array_equal(g[:, 1], x[:, 0].astype(F))  # d(x0 x1 + sin x2)/dx1
+
+
+def test_specialised_autotune_graph_and_ragged_batches(hip):
+    """The autotuner may pick the specialised kernel; whichever it picks, the
+    results stay equal to the interpreter's, also replayed from a graph and
+    for batches of 1 and 65 points."""
+    tape = wl.tanh_mlp(16, 32, 32)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    B = 1 << 14
+    xn = wl.points_normal(10, B, 16).astype(np.float32)
+    HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 24576, fetch=1)
+    y0, g0 = hip.gradient_batch(bufs, xn.reshape(-1), B)
+    info = HipContext.specialize(bufs, TUNE_GRADIENT)
+    assert info["scratch"] == 0
+    x = torch.from_numpy(xn).cuda()
+    y = torch.empty(B, dtype=torch.float32, device="cuda")
+    g = torch.empty(B, 16, dtype=torch.float32, device="cuda")
+    res = hip.autotune_dev(bufs, TUNE_GRADIENT, B, x, d_y=y, d_g=g)
+    assert res["fetch"] in (0, 1, 2)
+    HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=2)
+    y.zero_()
+    g.zero_()
+    graph = hip.graph_capture(bufs, TUNE_GRADIENT, B, x, d_y=y, d_g=g, repeats=2)
+    for _ in range(3):
+        y.zero_()
+        g.zero_()
+        torch.cuda.synchronize()
+        graph.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(y.cpu().numpy(), y0) and np.array_equal(g.cpu().numpy().reshape(-1), g0)
+    graph.close()
+    for b in (1, 65):
+        yb, gb = hip.gradient_batch(bufs, xn[:b].reshape(-1), b)
+        assert np.array_equal(yb, y0[:b]) and np.array_equal(gb, g0[: b * 16])
+
+
+def test_specialise_rejects(hip):
+    tape = wl.tanh_mlp(4, 4, 33)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    with pytest.raises(GpuError, match="specialised kernels exist"):
+        HipContext.specialize(bufs, 2)
+    with pytest.raises(GpuError, match="no specialised kernel"):
+        HipContext.set_tuned(bufs, TUNE_FORWARD, 64, 0, fetch=2)
+    HipContext.specialize(bufs, TUNE_FORWARD)
+    HipContext.set_tuned(bufs, TUNE_FORWARD, 64, 0, fetch=2)
+    with pytest.raises(GpuError, match="no specialised kernel"):
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=2)

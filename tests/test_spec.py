@@ -1,0 +1,76 @@
+"""Tape-specialised kernel source (no device): generation and hipRTC
+compilation for gfx950 on the host, literal constants exact to the bit, and
+the reverse sweep's shape (zero tests only where a partial is not a fixed
+finite constant)."""
+
+import re
+
+import numpy as np
+import pytest
+
+from echidna_amd import backend as be
+from echidna_amd.tape import GpuError, GpuTapeData, record
+from test_structure import TAPES
+
+
+def _data(tape, f64):
+    return GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
+
+
+@pytest.mark.parametrize("name", ["kitchen_sink", "rosenbrock", "mlp", "dag0", "guards"])
+@pytest.mark.parametrize("f64", [False, True])
+def test_source_compiles_for_gfx950(name, f64):
+    if name == "guards":
+        import dense_cases as dc
+
+        tape = dc.tape()
+    else:
+        tape = TAPES[name]()
+    for what in (0, 1):
+        src, info = be.spec_source(_data(tape, f64), what, f64=f64, arch="gfx950")
+        assert info["code_bytes"] > 0
+        assert ("typedef double F;" if f64 else "typedef float F;") in src
+        stores = re.findall(r"^gp\[(\d+)\] = ", src, re.M)
+        assert (what == 1) == (len(stores) == tape.num_inputs)
+
+
+def test_constants_are_exact_literals():
+    tape, _ = record(lambda v: v[0] * 0.1 + v[1] * -0.0 + v[1] * float("inf"), [1.0, 2.0])
+    src32, _ = be.spec_source(_data(tape, False), 1, f64=False)
+    src64, _ = be.spec_source(_data(tape, True), 1, f64=True)
+    assert "__builtin_bit_cast(float, 0x3dcccccdu)" in src32  # 0.1f
+    assert "__builtin_bit_cast(double, 0x3fb999999999999aull)" in src64  # 0.1
+    assert "__builtin_bit_cast(double, 0x8000000000000000ull)" in src64  # -0.0
+    assert "__builtin_bit_cast(float, 0x7f800000u)" in src32  # inf
+    # multiplication by -0.0 has a zero partial: no update from it; by inf a
+    # guarded one (0 * inf would be NaN)
+    assert "0x80000000u) * g" not in src32
+    assert "if (lv && da != F(0))" in src32
+
+
+def test_reverse_sweep_shape():
+    """Adds, subs and products with constants update unconditionally; other
+    ops keep the reference's zero tests; entries off the path to output 0
+    and constant subexpressions get no adjoint."""
+    def f(v):
+        dead = v[2].exp()  # recorded, never reaches the output
+        c = 3.0
+        return (v[0] * 2.0 - v[1]).tanh() + v[0] * v[1]
+
+    tape, _ = record(f, [0.5, 0.25, 1.0])
+    src, _ = be.spec_source(_data(tape, True), 1, f64=True)
+    rev = src.split("F g", 1)[1]
+    assert rev.count("s_partials") == 2  # tanh and the variable product
+    assert "gp[2] = F(0);" in src  # x2 only feeds the dead exp
+    assert "s_partials<F>(17u" not in src  # no reverse for exp
+
+
+def test_source_rejects_bad_input():
+    tape, _ = record(lambda v: v[0] * v[1], [1.0, 2.0])
+    with pytest.raises(GpuError, match="bad argument"):
+        be.spec_source(_data(tape, False), 2, f64=False)
+    d = _data(tape, False)
+    d.arg0 = np.array(d.arg0, dtype=np.uint32)
+    d.arg0[-1] = 99
+    with pytest.raises(GpuError):
+        be.spec_source(d, 1, f64=False)
