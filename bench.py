@@ -43,7 +43,7 @@ def parse():
                     help="with --wg/--lds: program fetch, 0 LDS windows, 1 scalar cache, "
                          "2 tape-specialised kernel (mlp)")
     ap.add_argument("--no-spec", action="store_true",
-                    help="mlp: leave the tape-specialised kernel out (interpreters only)")
+                    help="mlp, hvp: leave the tape-specialised kernel out (interpreters only)")
     ap.add_argument("--no-fuse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="points per CPU-baseline pass")
@@ -340,6 +340,7 @@ def run_hvp(args) -> dict:
     v = torch.from_numpy(philox.normal(1, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n).astype(dt)).cuda()
     g = torch.empty_like(x)
     hv = torch.empty_like(x)
+    spec = None if args.no_spec else ctx.specialize(bufs, TUNE_HVP)
     tuned = ctx.autotune_dev(bufs, TUNE_HVP, B, x, d_g=g, d_h=hv, d_dirs=v)
     dt_s, _ = timed(lambda: ctx.hvp_batch_dev(bufs, x, n, v, B, g, hv), max(1, args.steps))
     S = 8 if f64 else 4
@@ -348,7 +349,8 @@ def run_hvp(args) -> dict:
             "higher_is_better": True, "vs_baseline": None, "dtype": args.dtype,
             "data": "synthetic: Philox points and directions, random-init weights baked into the tape",
             "config": {"workload": f"tanh-MLP tape n={n} H={H} seed 1 (N={counts['N']} computed entries), "
-                                   f"B=2^{args.batch_log2} points, one HVP each", "points": B, "launch": tuned},
+                                   f"B=2^{args.batch_log2} points, one HVP each", "points": B, "launch": tuned,
+                       "specialized": spec},
             "ms_per_step": dt_s * 1e3,
             "roofline": {"bound": "hbm", "achieved": per_point * B / dt_s / 1e9, "peak": HBM_PEAK / 1e9,
                          "unit": "GB/s", "frac": per_point * B / dt_s / HBM_PEAK,

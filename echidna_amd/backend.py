@@ -501,7 +501,7 @@ class HipContext:
 
     @staticmethod
     def specialize(tape: "HipTapeBuffers", what: int = 1) -> dict:
-        """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient).
+        """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient, 2 hvp).
 
         Select it with ``set_tuned(..., fetch=2)`` or let ``autotune_dev``
         consider it.

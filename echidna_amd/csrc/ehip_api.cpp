@@ -525,7 +525,7 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
     if (rc) break;
   }
   // the tape-specialised kernel, where one has been compiled (ehip_tape_specialize)
-  if (rc == EHIP_OK && kind == 0 && t->spec[what == 1].fn)
+  if (rc == EHIP_OK && (kind == 0 || kind == 1) && t->spec[kind == 1 ? 2 : (what == 1 ? 1 : 0)].fn)
     for (uint32_t wg : {64u, 128u, 256u}) {
       Tuned cand;
       cand.wg = wg;
@@ -565,7 +565,7 @@ int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, 
   if (what >= 0 && what <= 2) key = what;
   else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
   else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
-  if (fetch == 2 && wg && (what > 1 || !t->spec[what].fn))
+  if (fetch == 2 && wg && (what > 2 || !t->spec[what].fn))
     return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
   t->tuned[key].wg = wg;
   t->tuned[key].lds_budget = wg ? lds_budget : 0;

@@ -109,12 +109,14 @@ struct Tuned {
 };
 
 // A tape-specialised kernel (straight-line code compiled for one tape by
-// hipRTC, ehip_spec.cpp): ehip_spec(X, xstride, Y, G, B).
+// hipRTC, ehip_spec.cpp): ehip_spec(X, xstride, Y, G, B) for values and
+// gradients.
 struct SpecKernel {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
   uint32_t vgprs = 0, scratch = 0, code_bytes = 0;
   double compile_ms = 0;
+  bool dual = false;  // HVP kernel: ehip_spec(X, xstride, V, G, H, B)
 };
 
 // Longest tape the specialiser accepts (compile time and code size grow
@@ -134,7 +136,7 @@ struct ehip_tape {
   DevProgram prog[4];  // forward, gradient, HVP (dual interpreter), kinks; all with DENSE groups
   uint32_t nkinks = 0;
   Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
-  SpecKernel spec[2];  // tape-specialised forward and gradient kernels, once compiled
+  SpecKernel spec[3];  // tape-specialised forward, gradient and HVP kernels, once compiled
 };
 
 namespace ehip_detail {
@@ -192,8 +194,8 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
   const uint32_t wg = tu.wg ? tu.wg : c->wg;
   const uint32_t fm = tu.wg ? tu.fetch : c->fetch;
-  if (fm == 2 && kind == 0) {
-    const SpecKernel& sk = t->spec[L.g ? 1 : 0];
+  if (fm == 2 && (kind == 0 || kind == 1)) {
+    const SpecKernel& sk = t->spec[kind == 1 ? 2 : (L.g ? 1 : 0)];
     if (!sk.fn) return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
     hipError_t e = spec_launch(sk, L, wg, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");

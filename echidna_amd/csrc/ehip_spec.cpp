@@ -83,37 +83,62 @@ struct Gen {
 
 }  // namespace
 
-std::string spec_source(const TapeHost& h, bool grad) {
+// what: 0 values of every output; 1 values and the gradient of output 0;
+// 2 gradient and Hessian-vector product of output 0 (tangent-over-reverse
+// on Dual numbers: forward_tangent, src/bytecode_tape/tangent.rs:97-147, then
+// reverse_tangent_inner, tangent.rs:191-262, as in hvp, tangent.rs:268-310).
+std::string spec_source(const TapeHost& h, int what) {
   Gen g(h);
+  const bool dual = what == 2;
   const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
-  g.put("// generated: %u entries, %u inputs, %u outputs, %s%s\n", N, n, m, h.f64 ? "f64" : "f32",
-        grad ? ", value and gradient of output 0" : ", values");
+  static const char* kWhat[] = {"values", "value and gradient of output 0", "gradient and HVP of output 0"};
+  g.put("// generated: %u entries, %u inputs, %u outputs, %s, %s\n", N, n, m, h.f64 ? "f64" : "f32", kWhat[what]);
   g.put("#include \"ehip_math.h\"\n");
   g.put("typedef %s F;\n", h.f64 ? "double" : "float");
-  g.put("extern \"C\" __global__ void __launch_bounds__(256) ehip_spec(const F* __restrict__ X, "
-        "unsigned long long xs, F* __restrict__ Y, F* __restrict__ G, unsigned long long B) {\n");
+  g.put("typedef ehip::Dual<F> D;\n");
+  if (dual)
+    g.put("extern \"C\" __global__ void __launch_bounds__(256) ehip_spec(const F* __restrict__ X, "
+          "unsigned long long xs, const F* __restrict__ V, F* __restrict__ G, F* __restrict__ H, "
+          "unsigned long long B) {\n");
+  else
+    g.put("extern \"C\" __global__ void __launch_bounds__(256) ehip_spec(const F* __restrict__ X, "
+          "unsigned long long xs, F* __restrict__ Y, F* __restrict__ G, unsigned long long B) {\n");
   g.put("const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n");
   g.put("if (p >= B) return;\n");
   g.put("const F* __restrict__ x = X + p * xs;\n");
+  if (dual) g.put("const F* __restrict__ v = V + p * %uull;\n", n);
+  const char* T = dual ? "D" : "F";
   for (uint32_t i = 0; i < N; ++i) {
     const uint32_t op = h.opcodes[i], a = h.arg0[i], b = h.arg1[i];
-    switch (op) {
-      case OP_INPUT: g.put("const F v%u = x[%u];\n", i, i); break;
-      case OP_CONST: g.out += "const F v" + std::to_string(i) + " = " + g.lit(h.values[i]) + ";\n"; break;
-      case OP_ADD: g.put("const F v%u = v%u + v%u;\n", i, a, b); break;
-      case OP_SUB: g.put("const F v%u = v%u - v%u;\n", i, a, b); break;
-      case OP_MUL: g.put("const F v%u = v%u * v%u;\n", i, a, b); break;
-      case OP_NEG: g.put("const F v%u = -v%u;\n", i, a); break;
-      case OP_POWI: g.put("const F v%u = ehip::s_powi<F>(v%u, %d);\n", i, a, (int32_t)b); break;
-      default:
-        if (b != UNUSED) g.put("const F v%u = ehip::s_eval<F>(%uu, v%u, v%u);\n", i, op, a, b);
-        else g.put("const F v%u = ehip::s_eval<F>(%uu, v%u, F(0));\n", i, op, a);
+    if (op == OP_INPUT) {
+      if (dual) g.put("const D v%u = {x[%u], v[%u]};\n", i, i, i);
+      else g.put("const F v%u = x[%u];\n", i, i);
+    } else if (op == OP_CONST) {
+      g.out += std::string("const ") + T + " v" + std::to_string(i) + " = " + (dual ? "{" : "") +
+               g.lit(h.values[i]) + (dual ? ", F(0)};\n" : ";\n");
+    } else if (op == OP_POWI) {
+      g.put("const %s v%u = ehip::%s<F>(v%u, %d);\n", T, i, dual ? "d_powi" : "s_powi", a, (int32_t)b);
+    } else if (dual) {
+      if (b != UNUSED) g.put("const D v%u = ehip::d_eval<F>(%uu, v%u, v%u);\n", i, op, a, b);
+      else g.put("const D v%u = ehip::d_eval<F>(%uu, v%u, D{F(0), F(0)});\n", i, op, a);
+    } else {
+      switch (op) {
+        case OP_ADD: g.put("const F v%u = v%u + v%u;\n", i, a, b); break;
+        case OP_SUB: g.put("const F v%u = v%u - v%u;\n", i, a, b); break;
+        case OP_MUL: g.put("const F v%u = v%u * v%u;\n", i, a, b); break;
+        case OP_NEG: g.put("const F v%u = -v%u;\n", i, a); break;
+        default:
+          if (b != UNUSED) g.put("const F v%u = ehip::s_eval<F>(%uu, v%u, v%u);\n", i, op, a, b);
+          else g.put("const F v%u = ehip::s_eval<F>(%uu, v%u, F(0));\n", i, op, a);
+      }
     }
   }
-  g.put("if (Y) {\nF* __restrict__ y = Y + p * %uull;\n", m);
-  for (uint32_t k = 0; k < m; ++k) g.put("y[%u] = v%u;\n", k, h.outputs[k]);
-  g.put("}\n");
-  if (grad) {
+  if (!dual) {
+    g.put("if (Y) {\nF* __restrict__ y = Y + p * %uull;\n", m);
+    for (uint32_t k = 0; k < m; ++k) g.put("y[%u] = v%u;\n", k, h.outputs[k]);
+    g.put("}\n");
+  }
+  if (what >= 1) {
     // active[i]: entry i depends on an input
     std::vector<uint8_t> active(N, 0), touched(N, 0);
     for (uint32_t i = 0; i < N; ++i) {
@@ -123,14 +148,22 @@ std::string spec_source(const TapeHost& h, bool grad) {
         active[i] = active[h.arg0[i]] || (op != OP_POWI && h.arg1[i] != UNUSED && active[h.arg1[i]]);
     }
     const uint32_t seed = h.outputs[0];
-    g.put("F g%u = F(1);\n", seed);
+    g.put(dual ? "D g%u = {F(1), F(0)};\n" : "F g%u = F(1);\n", seed);
     touched[seed] = 1;
     auto declare = [&](uint32_t a) {
       if (!touched[a]) {
-        g.put("F g%u = F(0);\n", a);
+        g.put(dual ? "D g%u = {F(0), F(0)};\n" : "F g%u = F(0);\n", a);
         touched[a] = 1;
       }
     };
+    // g_a = g_a + P * g_i with a partial P fixed when generating
+    auto static_update = [&](uint32_t a, uint32_t i, const std::string& P) {
+      declare(a);
+      const std::string ga = "g" + std::to_string(a), gi = "g" + std::to_string(i);
+      if (dual) g.out += ga + " = ehip::d_add(" + ga + ", ehip::d_mul(" + P + ", " + gi + "));\n";
+      else g.out += ga + " = " + ga + " + " + P + " * " + gi + ";\n";
+    };
+    const std::string one = g.lit(1.0), mone = g.lit(-1.0), mzero = g.lit(-0.0);
     for (uint32_t ii = N; ii-- > 0;) {
       const uint32_t op = h.opcodes[ii];
       if (op == OP_INPUT || op == OP_CONST || !touched[ii]) continue;
@@ -138,45 +171,67 @@ std::string spec_source(const TapeHost& h, bool grad) {
       const bool ua = active[a];
       const bool ub = op != OP_POWI && b != UNUSED && active[b];
       if (!ua && !ub) continue;
-      // static partials (finite, in the tape's dtype) where the op fixes them
+      // Partials the op fixes, as reverse_partials computes them (ehip_math.h
+      // s_partials / d_partials): finite and non-zero, or zero (no update).
       bool stat = false;
-      double ca = 0, cb = 0;
-      if (op == OP_ADD) stat = true, ca = 1, cb = 1;
-      else if (op == OP_SUB) stat = true, ca = 1, cb = -1;
-      else if (op == OP_NEG) stat = true, ca = -1;
-      else if (op == OP_MUL && h.opcodes[b] == OP_CONST && std::isfinite(g.cval(b))) stat = true, ca = g.cval(b);
-      else if (op == OP_MUL && h.opcodes[a] == OP_CONST && std::isfinite(g.cval(a))) stat = true, cb = g.cval(a);
+      std::string pa, pb;  // empty: zero partial
+      auto scal = [&](double c) { return dual ? "D{" + g.lit(c) + ", F(0)}" : g.lit(c); };
+      const std::string p_one = dual ? "D{" + one + ", F(0)}" : one;
+      const std::string p_mone = dual ? "D{" + mone + ", " + mzero + "}" : mone;  // d_neg(one) = {-1, -0}
+      if (op == OP_ADD) stat = true, pa = p_one, pb = p_one;
+      else if (op == OP_SUB) stat = true, pa = p_one, pb = p_mone;
+      else if (op == OP_NEG) stat = true, pa = p_mone;
+      else if (op == OP_MUL && h.opcodes[b] == OP_CONST && std::isfinite(g.cval(b))) {
+        stat = true;
+        if (g.cval(b) != 0.0) pa = scal(g.cval(b));
+      } else if (op == OP_MUL && h.opcodes[a] == OP_CONST && std::isfinite(g.cval(a))) {
+        stat = true;
+        if (g.cval(a) != 0.0) pb = scal(g.cval(a));
+      }
       if (stat) {
-        if (ua && ca != 0.0) {
-          declare(a);
-          g.out += "g" + std::to_string(a) + " = g" + std::to_string(a) + " + " + g.lit(ca) + " * g" +
-                   std::to_string(ii) + ";\n";
-        }
-        if (ub && cb != 0.0) {
-          declare(b);
-          g.out += "g" + std::to_string(b) + " = g" + std::to_string(b) + " + " + g.lit(cb) + " * g" +
-                   std::to_string(ii) + ";\n";
-        }
+        if (ua && !pa.empty()) static_update(a, ii, pa);
+        if (ub && !pb.empty()) static_update(b, ii, pb);
         continue;
       }
       if (ua) declare(a);
       if (ub) declare(b);
+      const std::string gi = "g" + std::to_string(ii);
+      const std::string live = dual ? "!ehip::d_zero(" + gi + ")" : gi + " != F(0)";
+      auto nz = [&](const char* d) { return dual ? std::string("!ehip::d_zero(") + d + ")" : std::string(d) + " != F(0)"; };
+      auto upd = [&](uint32_t t, const char* d) {
+        const std::string gt = "g" + std::to_string(t);
+        return dual ? gt + " = ehip::d_add(" + gt + ", ehip::d_mul(" + d + ", " + gi + "));"
+                    : gt + " = " + gt + " + " + d + " * " + gi + ";";
+      };
       if (op == OP_POWI) {
-        g.put("{ const F da = ehip::s_powi_partial<F>(%d, v%u, v%u);\n", (int32_t)b, a, ii);
-        g.put("if (g%u != F(0) && da != F(0)) g%u = g%u + da * g%u; }\n", ii, a, a, ii);
+        g.put("{ const %s da = ehip::%s<F>(%d, v%u, v%u);\n", T, dual ? "d_powi_partial" : "s_powi_partial",
+              (int32_t)b, a, ii);
+        g.out += "if (" + live + " && " + nz("da") + ") " + upd(a, "da") + " }\n";
         continue;
       }
-      if (b != UNUSED) g.put("{ F da, db; ehip::s_partials<F>(%uu, v%u, v%u, v%u, da, db);\n", op, a, b, ii);
-      else g.put("{ F da, db; ehip::s_partials<F>(%uu, v%u, F(0), v%u, da, db);\n", op, a, ii);
-      g.put("const bool lv = g%u != F(0);\n", ii);
-      if (ua) g.put("if (lv && da != F(0)) g%u = g%u + da * g%u;\n", a, a, ii);
-      if (ub) g.put("if (lv && db != F(0)) g%u = g%u + db * g%u;\n", b, b, ii);
+      const char* zero = dual ? "D{F(0), F(0)}" : "F(0)";
+      if (b != UNUSED)
+        g.put("{ %s da, db; ehip::%s<F>(%uu, v%u, v%u, v%u, da, db);\n", T, dual ? "d_partials" : "s_partials", op,
+              a, b, ii);
+      else
+        g.put("{ %s da, db; ehip::%s<F>(%uu, v%u, %s, v%u, da, db);\n", T, dual ? "d_partials" : "s_partials", op,
+              a, zero, ii);
+      g.out += "const bool lv = " + live + ";\n";
+      if (ua) g.out += "if (lv && " + nz("da") + ") " + upd(a, "da") + "\n";
+      if (ub) g.out += "if (lv && " + nz("db") + ") " + upd(b, "db") + "\n";
       g.put("}\n");
     }
     g.put("F* __restrict__ gp = G + p * %uull;\n", n);
+    if (dual) g.put("F* __restrict__ hp = H + p * %uull;\n", n);
     for (uint32_t k = 0; k < n; ++k) {
-      if (touched[k]) g.put("gp[%u] = g%u;\n", k, k);
-      else g.put("gp[%u] = F(0);\n", k);
+      if (!touched[k]) {
+        g.put("gp[%u] = F(0);\n", k);
+        if (dual) g.put("hp[%u] = F(0);\n", k);
+      } else if (dual) {
+        g.put("gp[%u] = g%u.re;\nhp[%u] = g%u.eps;\n", k, k, k, k);
+      } else {
+        g.put("gp[%u] = g%u;\n", k, k);
+      }
     }
   }
   g.put("}\n");
@@ -223,12 +278,15 @@ void spec_free(SpecKernel& s) {
 hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStream_t st) {
   const void* X = L.x;
   unsigned long long xs = L.xstride;
+  const void* V = L.dirs;
   void* Y = L.y;
   void* G = L.g;
+  void* H = L.h;
   unsigned long long B = L.B;
-  void* args[] = {(void*)&X, (void*)&xs, (void*)&Y, (void*)&G, (void*)&B};
+  void* args_fwd[] = {(void*)&X, (void*)&xs, (void*)&Y, (void*)&G, (void*)&B};
+  void* args_hvp[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&G, (void*)&H, (void*)&B};
   const uint64_t grid = (L.B + wg - 1) / wg;
-  return hipModuleLaunchKernel(s.fn, (uint32_t)grid, 1, 1, wg, 1, 1, 0, st, args, nullptr);
+  return hipModuleLaunchKernel(s.fn, (uint32_t)grid, 1, 1, wg, 1, 1, 0, st, s.dual ? args_hvp : args_fwd, nullptr);
 }
 
 }  // namespace ehip_detail
@@ -237,7 +295,9 @@ using namespace ehip_detail;
 
 int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
   if (!t || !out4) return fail(EHIP_OTHER, "GPU error: null argument");
-  if (what != 0 && what != 1) return fail(EHIP_OTHER, "GPU error: specialised kernels exist for forward (0) and gradient (1)");
+  if (what < 0 || what > 2)
+    return fail(EHIP_OTHER, "GPU error: specialised kernels exist for forward (0), gradient (1) and hvp (2)");
+  if (what == 2 && t->nout != 1) return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only");
   if (t->host.opcodes.size() > SPEC_MAX_ENTRIES)
     return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%zu entries > %u)", t->host.opcodes.size(),
                 SPEC_MAX_ENTRIES);
@@ -248,7 +308,7 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
   HIP_TRY(hipGetDeviceProperties(&prop, c->device), "device properties");
   std::string arch = prop.gcnArchName;
   arch = arch.substr(0, arch.find(':'));
-  const std::string src = spec_source(t->host, what == 1);
+  const std::string src = spec_source(t->host, what);
   std::vector<char> code;
   std::string log;
   double ms = 0;
@@ -266,6 +326,7 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
   if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, s.fn) == hipSuccess) s.scratch = (uint32_t)v;
   s.code_bytes = (uint32_t)code.size();
   s.compile_ms = ms;
+  s.dual = what == 2;
   spec_free(t->spec[what]);
   t->spec[what] = s;
   out4[0] = s.vgprs;
@@ -279,7 +340,7 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
                          const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
                          const uint32_t* output_indices, uint32_t num_outputs, int32_t what, const char* arch,
                          char* src, uint64_t src_cap, uint64_t* out3) {
-  if (!out3 || (what != 0 && what != 1) || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
+  if (!out3 || what < 0 || what > 2 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)
@@ -291,6 +352,7 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
                                   outs.data(), (uint32_t)outs.size());
   if (!msg.empty()) return fail(EHIP_OTHER, "%s", msg.c_str());
   if (num_ops == 0) return fail(EHIP_OTHER, "invalid GpuTapeData: empty tape");
+  if (what == 2 && outs.size() != 1) return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only");
   if (num_ops > SPEC_MAX_ENTRIES)
     return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%u entries > %u)", num_ops, SPEC_MAX_ENTRIES);
   TapeHost th;
@@ -301,7 +363,7 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
   th.num_inputs = num_inputs;
   th.outputs = outs;
   th.f64 = dtype == EHIP_F64;
-  const std::string s = spec_source(th, what == 1);
+  const std::string s = spec_source(th, what);
   out3[0] = s.size();
   out3[1] = 0;
   out3[2] = 0;

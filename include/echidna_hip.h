@@ -249,7 +249,7 @@ void ehip_graph_destroy(ehip_graph* graph);
 /* Restore (wg != 0) or clear (wg == 0) a tuned launch shape, e.g. from a
  * persisted autotuner cache; ehip_get_tuned reads it back
  * (out3 = {wg, lds, fetch}).  fetch 2 selects the tape-specialised kernel
- * (forward and gradient, once ehip_tape_specialize has built it). */
+ * (forward, gradient and hvp, once ehip_tape_specialize has built it). */
 int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
                        uint32_t fetch);
 int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out3);
@@ -258,12 +258,12 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * registers; no counterpart in the reference, whose CUDA backend generates
  * source at run time per Taylor order, src/gpu/taylor_codegen.rs, but never
  * per tape).  Writes the
- * tape's forward (what 0) or value-and-gradient (what 1) sweep as
- * straight-line HIP, every entry a named value kept in VGPRs and every
- * constant an instruction literal, and compiles it with hipRTC for the
- * tape's device.  Arithmetic and order are the interpreters', so results
- * are the same.  Selected per evaluation with ehip_set_tuned(fetch = 2) or
- * by ehip_autotune_dev.  out4 = {VGPRs, scratch bytes per lane, compile
+ * tape's forward (what 0), value-and-gradient (what 1) or gradient-and-HVP
+ * (what 2, tangent-over-reverse on dual numbers) sweep as straight-line HIP,
+ * every entry a named value kept in VGPRs and every constant an instruction
+ * literal, and compiles it with hipRTC for the tape's device.  Arithmetic
+ * and order are the interpreters', so results are the same.  Selected per
+ * evaluation with ehip_set_tuned(fetch = 2) or by ehip_autotune_dev.  out4 = {VGPRs, scratch bytes per lane, compile
  * microseconds, code object bytes}.  Tapes longer than 65536 entries are
  * refused.  Not thread-safe against evaluations of the same tape. */
 int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t* out4);

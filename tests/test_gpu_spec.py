@@ -156,10 +156,52 @@ def test_specialise_rejects(hip):
     tape = wl.tanh_mlp(4, 4, 33)
     bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
     with pytest.raises(GpuError, match="specialised kernels exist"):
-        HipContext.specialize(bufs, 2)
+        HipContext.specialize(bufs, 3)
     with pytest.raises(GpuError, match="no specialised kernel"):
         HipContext.set_tuned(bufs, TUNE_FORWARD, 64, 0, fetch=2)
     HipContext.specialize(bufs, TUNE_FORWARD)
     HipContext.set_tuned(bufs, TUNE_FORWARD, 64, 0, fetch=2)
     with pytest.raises(GpuError, match="no specialised kernel"):
         HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=2)
+
+
+@pytest.mark.parametrize("name", ["kitchen_sink", "rosenbrock", "mlp", "dag0", "guards"])
+@pytest.mark.parametrize("f64", [False, True])
+def test_specialised_hvp_equals_interpreter_and_oracle(hip, name, f64):
+    """Gradient and H v from the specialised dual kernel, one x per point and
+    one shared x (the reference's hvp_batch), against the dual interpreter
+    (bits, NaN signs aside) and the f64 oracle."""
+    from echidna_amd import philox
+    from echidna_amd.backend import TUNE_HVP
+
+    tape = _tape(name)
+    n = tape.num_inputs
+    F = np.float64 if f64 else np.float32
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    info = HipContext.specialize(bufs, TUNE_HVP)
+    assert info["vgprs"] > 0
+    B = 700
+    x = _points(name, B, n, 23).astype(F)
+    v = philox.normal(16, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n).astype(F)
+    v[5] = 0.0  # a zero direction: H v = 0 exactly
+    outs = {}
+    with np.errstate(all="ignore"):
+        for fetch in (1, 2):
+            if fetch == 2:
+                HipContext.set_tuned(bufs, TUNE_HVP, 64, 0, fetch=2)
+            else:
+                HipContext.set_tuned(bufs, TUNE_HVP, 128, 8192, fetch=1)
+            per_point = hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B)
+            shared = hip.hvp_batch(bufs, x[7], v.reshape(-1), B)
+            outs[fetch] = [np.asarray(a).copy() for a in (*per_point, *shared)]
+        for a, b in zip(outs[1], outs[2]):
+            assert np.array_equal(_bits(a, f64), _bits(b, f64)), name
+        if f64:
+            g, hv = (a.reshape(B, n) for a in outs[2][:2])
+            ref_g, ref_hv = po.hvp(tape, x, v)
+            for got, ref in ((g, ref_g.reshape(B, n)), (hv, ref_hv.reshape(B, n))):
+                assert np.array_equal(np.isnan(got), np.isnan(ref))
+                fin = np.isfinite(ref)
+                assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
+                assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin])),
+                              initial=0.0) <= 1e-11

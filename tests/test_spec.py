@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from echidna_amd import backend as be
-from echidna_amd.tape import GpuError, GpuTapeData, record
+from echidna_amd.tape import GpuError, GpuTapeData, record, record_multi
 from test_structure import TAPES
 
 
@@ -26,12 +26,14 @@ def test_source_compiles_for_gfx950(name, f64):
         tape = dc.tape()
     else:
         tape = TAPES[name]()
-    for what in (0, 1):
+    for what in (0, 1, 2):
         src, info = be.spec_source(_data(tape, f64), what, f64=f64, arch="gfx950")
         assert info["code_bytes"] > 0
         assert ("typedef double F;" if f64 else "typedef float F;") in src
         stores = re.findall(r"^gp\[(\d+)\] = ", src, re.M)
-        assert (what == 1) == (len(stores) == tape.num_inputs)
+        hstores = re.findall(r"^hp\[(\d+)\] = ", src, re.M)
+        assert len(stores) == (tape.num_inputs if what else 0)
+        assert len(hstores) == (tape.num_inputs if what == 2 else 0)
 
 
 def test_constants_are_exact_literals():
@@ -68,9 +70,23 @@ def test_reverse_sweep_shape():
 def test_source_rejects_bad_input():
     tape, _ = record(lambda v: v[0] * v[1], [1.0, 2.0])
     with pytest.raises(GpuError, match="bad argument"):
-        be.spec_source(_data(tape, False), 2, f64=False)
+        be.spec_source(_data(tape, False), 3, f64=False)
     d = _data(tape, False)
     d.arg0 = np.array(d.arg0, dtype=np.uint32)
     d.arg0[-1] = 99
     with pytest.raises(GpuError):
         be.spec_source(d, 1, f64=False)
+
+
+def test_dual_source_shape():
+    """The HVP kernel seeds inputs with (x, v), keeps the reference's Dual
+    zero tests for runtime partials and writes SUB's partial as d_neg(one) =
+    {-1, -0}."""
+    tape, _ = record(lambda v: (v[0] - v[1]).sin() * 3.0, [0.5, 0.25])
+    src, _ = be.spec_source(_data(tape, True), 2, f64=True)
+    assert "const D v0 = {x[0], v[0]};" in src
+    assert "!ehip::d_zero(" in src and "ehip::d_partials<F>(24u" in src
+    assert "D{__builtin_bit_cast(double, 0xbff0000000000000ull), __builtin_bit_cast(double, 0x8000000000000000ull)}" in src
+    multi, _ = record_multi(lambda v: [v[0] * v[1], v[1]], [1.0, 2.0])
+    with pytest.raises(GpuError, match="scalar-output"):
+        be.spec_source(_data(multi, False), 2, f64=False)
