@@ -512,8 +512,9 @@ def main() -> int:
             prof = json.load(f)
         # counters hold for the launch shape they were taken with
         shape = {k: launch.get(k) for k in ("wg", "lds_budget", "fetch")}
-        if prof.get("launch") == shape:
-            traffic = prof.get("hbm_bytes_per_launch")
+        for entry in (prof if isinstance(prof, list) else [prof]):
+            if entry.get("launch") == shape:
+                traffic = entry.get("hbm_bytes_per_launch")
 
     out = {
         "metric": "fwd+grad point-evals/s",
