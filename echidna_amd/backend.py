@@ -114,6 +114,7 @@ _SIGS = {
     "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
     "ehip_tape_specialize": (ctypes.c_int32, [_vp, ctypes.c_int32, _c_u32p]),
+    "ehip_set_specialize": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
     "ehip_spec_source": (ctypes.c_int32, [ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _c_f64p, ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32, ctypes.c_int32,
                                           ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _c_u64p]),
@@ -499,12 +500,17 @@ class HipContext:
         self._children.add(g)
         return g
 
+    def set_specialize(self, max_entries: int, what_mask: int = 0b011) -> None:
+        """Specialise later uploads of at most ``max_entries`` entries (0 = off)
+        for the evaluations in ``what_mask`` (bit 0 forward, 1 gradient, 2 hvp)."""
+        _check(load_library().ehip_set_specialize(self.handle, max_entries, what_mask))
+
     @staticmethod
     def specialize(tape: "HipTapeBuffers", what: int = 1) -> dict:
         """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient, 2 hvp).
 
-        Select it with ``set_tuned(..., fetch=2)`` or let ``autotune_dev``
-        consider it.
+        Untuned evaluations of that kind then run it; ``set_tuned`` (fetch 2
+        or an interpreter shape) and ``autotune_dev`` choose explicitly.
         """
         out = (ctypes.c_uint32 * 4)()
         _check(load_library().ehip_tape_specialize(tape.handle, what, out))

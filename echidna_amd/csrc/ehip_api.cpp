@@ -149,7 +149,34 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
       return rc;
     }
   }
+  uint32_t smax, smask;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    smax = c->spec_max;
+    smask = c->spec_mask;
+  }
+  if (num_ops <= smax)
+    for (int32_t what = 0; what < 3; ++what) {
+      if (!(smask & (1u << what)) || (what == 2 && t->nout != 1)) continue;
+      uint32_t info[4];
+      int32_t rc = ehip_tape_specialize(t, what, info);
+      if (rc != EHIP_OK) {
+        ehip_tape_free(t);
+        return rc;
+      }
+    }
   *out = t;
+  return ok();
+}
+
+int32_t ehip_set_specialize(ehip_ctx* c, uint32_t max_entries, uint32_t what_mask) {
+  if (!c) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (what_mask > 7) return fail(EHIP_OTHER, "GPU error: what_mask holds bits 0 forward, 1 gradient, 2 hvp");
+  if (max_entries > SPEC_MAX_ENTRIES)
+    return fail(EHIP_OTHER, "GPU error: max_entries above the specialiser's limit (%u)", SPEC_MAX_ENTRIES);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->spec_max = what_mask ? max_entries : 0;
+  c->spec_mask = what_mask;
   return ok();
 }
 

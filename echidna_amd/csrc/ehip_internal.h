@@ -89,6 +89,7 @@ struct ehip_ctx {
   uint32_t wg = 256;
   uint32_t lds_budget = 24576;
   uint32_t fetch = 0;  // program fetch of untuned launches and the queue (0 LDS, 1 scalar)
+  uint32_t spec_max = 0, spec_mask = 0;  // specialise uploads of at most spec_max entries (ehip_set_specialize)
   bool fuse = true;
   bool dense = true;
   size_t max_lds = 65536;
@@ -193,7 +194,9 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   const size_t S = t->f64 ? 8 : 4;
   const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
   const uint32_t wg = tu.wg ? tu.wg : c->wg;
-  const uint32_t fm = tu.wg ? tu.fetch : c->fetch;
+  // untuned evaluations run the tape-specialised kernel once it is built
+  const bool has_spec = kind == 0 ? t->spec[L.g ? 1 : 0].fn != nullptr : (kind == 1 && t->spec[2].fn != nullptr);
+  const uint32_t fm = tu.wg ? tu.fetch : (has_spec ? 2u : c->fetch);
   if (fm == 2 && (kind == 0 || kind == 1)) {
     const SpecKernel& sk = t->spec[kind == 1 ? 2 : (L.g ? 1 : 0)];
     if (!sk.fn) return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");

@@ -271,7 +271,10 @@ int32_t spec_compile(const std::string& src, const char* arch, std::vector<char>
 }
 
 void spec_free(SpecKernel& s) {
-  if (s.mod) (void)hipModuleUnload(s.mod);
+  if (s.mod) {
+    (void)hipDeviceSynchronize();  // no launch of the old code may still be in flight
+    (void)hipModuleUnload(s.mod);
+  }
   s = SpecKernel{};
 }
 
@@ -302,18 +305,20 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
     return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%zu entries > %u)", t->host.opcodes.size(),
                 SPEC_MAX_ENTRIES);
   ehip_ctx* c = t->ctx;
-  std::lock_guard<std::mutex> lk(c->mu);
-  (void)hipSetDevice(c->device);
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, c->device), "device properties");
   std::string arch = prop.gcnArchName;
   arch = arch.substr(0, arch.find(':'));
+  // generate and compile without the context lock (seconds for a few
+  // thousand entries); load and swap under it
   const std::string src = spec_source(t->host, what);
   std::vector<char> code;
   std::string log;
   double ms = 0;
   int32_t rc = spec_compile(src, arch.c_str(), code, log, &ms);
   if (rc) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
   SpecKernel s;
   HIP_TRY(hipModuleLoadData(&s.mod, code.data()), "load specialised kernel");
   hipError_t e = hipModuleGetFunction(&s.fn, s.mod, "ehip_spec");

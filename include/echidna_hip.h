@@ -262,11 +262,20 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * (what 2, tangent-over-reverse on dual numbers) sweep as straight-line HIP,
  * every entry a named value kept in VGPRs and every constant an instruction
  * literal, and compiles it with hipRTC for the tape's device.  Arithmetic
- * and order are the interpreters', so results are the same.  Selected per
- * evaluation with ehip_set_tuned(fetch = 2) or by ehip_autotune_dev.  out4 = {VGPRs, scratch bytes per lane, compile
+ * and order are the interpreters', so results are the same.  Once built it
+ * runs every untuned evaluation of that kind; ehip_set_tuned (fetch 2, or
+ * 0/1 for the interpreters) and ehip_autotune_dev choose explicitly.  out4 = {VGPRs, scratch bytes per lane, compile
  * microseconds, code object bytes}.  Tapes longer than 65536 entries are
- * refused.  Not thread-safe against evaluations of the same tape. */
+ * refused.  The compile runs outside the context lock; building a kernel
+ * again replaces the old one after the device is idle, which invalidates
+ * graphs captured with it. */
 int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t* out4);
+
+/* Specialise every tape uploaded after this call whose entry count is at
+ * most max_entries (<= 65536; 0 = off, the default), for the evaluations in
+ * what_mask (bit 0 forward, bit 1 gradient, bit 2 hvp; hvp only for
+ * scalar-output tapes).  A compile failure fails the upload. */
+int32_t ehip_set_specialize(ehip_ctx* ctx, uint32_t max_entries, uint32_t what_mask);
 
 /* The source ehip_tape_specialize compiles, from the flattened tape arrays
  * with no device (tape arguments as ehip_compile_program).  When `arch` is

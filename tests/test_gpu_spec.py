@@ -205,3 +205,34 @@ def test_specialised_hvp_equals_interpreter_and_oracle(hip, name, f64):
                 assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
                 assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin])),
                               initial=0.0) <= 1e-11
+
+
+def test_upload_time_specialisation_and_default_use(hip):
+    """ehip_set_specialize builds the kernels at upload for short tapes only;
+    a built kernel runs untuned evaluations, and an interpreter shape pinned
+    with set_tuned still gives the same results."""
+    from echidna_amd.backend import TUNE_HVP
+
+    short = wl.tanh_mlp(8, 6, 40)
+    long_ = wl.tanh_mlp(8, 64, 41)
+    with pytest.raises(GpuError, match="what_mask"):
+        hip.set_specialize(100, 8)
+    with pytest.raises(GpuError, match="limit"):
+        hip.set_specialize(1 << 20, 1)
+    hip.set_specialize(len(short.opcodes), 0b111)
+    try:
+        bs = hip.upload_tape(GpuTapeData.from_tape(short))
+        bl = hip.upload_tape(GpuTapeData.from_tape(long_))
+    finally:
+        hip.set_specialize(0, 0)
+    for what in (TUNE_FORWARD, TUNE_GRADIENT, TUNE_HVP):
+        HipContext.set_tuned(bs, what, 256, 0, fetch=2)  # built at upload
+        HipContext.set_tuned(bs, what, 0, 0)
+        with pytest.raises(GpuError, match="no specialised kernel"):
+            HipContext.set_tuned(bl, what, 256, 0, fetch=2)
+    B = 333
+    x = wl.points_normal(42, B, 8).astype(np.float32)
+    y_spec, g_spec = hip.gradient_batch(bs, x.reshape(-1), B)  # untuned: specialised
+    HipContext.set_tuned(bs, TUNE_GRADIENT, 128, 4096, fetch=0)
+    y_int, g_int = hip.gradient_batch(bs, x.reshape(-1), B)
+    assert np.array_equal(y_spec, y_int) and np.array_equal(g_spec, g_int)
