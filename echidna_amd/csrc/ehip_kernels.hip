@@ -286,6 +286,10 @@ template <class F, int NC, int JB, bool FULL, class Regs_>
 __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
                                                 uint32_t T, uint32_t j0, uint32_t jb) {
   F acc[JB][NC];
+  // true once a product term has entered the sums: a Taylor product's
+  // coefficients are sums that start at +0, so from then on no accumulator
+  // can be -0 (x + y is -0 only when both are), and acc + (0 + t) == acc + t
+  bool absorbed = false;
   for (uint32_t p = 0; p < T; ++p) {
     const uint32_t kr = PR[p];
     const uint32_t kind = kr >> P_KIND_SHIFT;
@@ -302,18 +306,30 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
         // constant jet [c, 0, ..] reduces exactly to 0 + x_k * c: its other
         // products are signed zeros, and a sum that starts at +0 absorbs
         // them (either operand order).
+        if (absorbed) {
 #pragma unroll
-        for (int j = 0; j < JB; ++j) {
-          if (!(FULL || (uint32_t)j < jb)) continue;
-          const F c = w[j];
+          for (int j = 0; j < JB; ++j) {
+            if (!(FULL || (uint32_t)j < jb)) continue;
+            const F c = w[j];
 #pragma unroll
-          for (int q = 0; q < NC; ++q) {
-            const F term = F(0) + x[q] * c;
-            acc[j][q] = (p == 0) ? term : acc[j][q] + term;
+            for (int q = 0; q < NC; ++q) acc[j][q] = acc[j][q] + x[q] * c;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < JB; ++j) {
+            if (!(FULL || (uint32_t)j < jb)) continue;
+            const F c = w[j];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+              const F term = F(0) + x[q] * c;
+              acc[j][q] = (p == 0) ? term : acc[j][q] + term;
+            }
           }
         }
+        absorbed = true;
         continue;
       }
+      absorbed = true;
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
