@@ -102,26 +102,47 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
     for (int off = 32; off >= 1; off >>= 1) lnorm += __shfl_xor(lnorm, off, 64);
   }
   Wf w{0.0, 0.0, 0.0}, wp{0.0, 0.0, 0.0};
-  for (uint32_t s = lane; s < R.S; s += 64) {
-    const uint64_t row = pt * R.S + s;
+  // With a learned control variate each sample needs u = L^T v over all n
+  // coordinates.  G lanes share one sample (G = 64 / samples per pass, a power
+  // of two): lane part g sums coordinates g, g + G, .., and the G partial sums
+  // meet by shuffles inside the lane group.  Without one, a lane takes whole
+  // samples.
+  uint32_t spp = 64, G = 1;  // samples per pass, lanes per sample
+  if (LC)
+    while (spp / 2 >= R.S && spp > 1) spp /= 2;
+  G = 64 / spp;
+  const uint32_t part = lane % G;
+  for (uint32_t s0 = 0; s0 < R.S; s0 += spp) {
+    const uint32_t s = s0 + lane / G;
+    const bool has = s < R.S;
+    const uint64_t row = pt * R.S + (has ? s : 0);
     double sample = 2.0 * (double)J[row * 3 + 2];
     if (R.sample_scale != 0.0) sample *= R.sample_scale;
-    welford_add(wp, sample);
     if (LC) {
       double u[CV_RMAX];
 #pragma unroll
       for (int j = 0; j < CV_RMAX; ++j) u[j] = 0.0;
-      for (uint32_t i = 0; i < R.nin; ++i) {
-        const double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, i) : (double)D[row * R.nin + i];
-        const float* li = LC + (uint64_t)i * R.rank;
+      if (has)
+        for (uint32_t i = part; i < R.nin; i += G) {
+          const double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, i) : (double)D[row * R.nin + i];
+          const float* li = LC + (uint64_t)i * R.rank;
+#pragma unroll
+          for (int j = 0; j < CV_RMAX; ++j)
+            if (j < (int)R.rank) u[j] += (double)li[j] * v;
+        }
+      for (uint32_t off = G / 2; off >= 1; off >>= 1)
 #pragma unroll
         for (int j = 0; j < CV_RMAX; ++j)
-          if (j < (int)R.rank) u[j] += (double)li[j] * v;
-      }
+          if (j < (int)R.rank) u[j] += __shfl_xor(u[j], (int)off, 64);
       double q = 0.0;
 #pragma unroll
       for (int j = 0; j < CV_RMAX; ++j) q += (j < (int)R.npos) ? u[j] * u[j] : -(u[j] * u[j]);
+      if (!has || part != 0) continue;
+      welford_add(wp, sample);
       sample -= q - lnorm;
+    } else {
+      if (!has) continue;
+      welford_add(wp, sample);
     }
     if (DG) {
       const F* d = DG + pt * R.diag_stride;
