@@ -76,6 +76,25 @@ struct Regs {
       for (int c = 0; c < NC; ++c) o[c] = gl[(base + c) * gstride + p];
     }
   }
+  // the first N components only
+  template <int N>
+  __device__ __forceinline__ void rd_n(uint32_t r, F* o) const {
+    static_assert(N <= NC, "components");
+    if (LDSONLY) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) o[c] = lane[(r * NC + c) << lsh];
+      return;
+    }
+    r = uni(r);
+    if (r < R_lds) {
+#pragma unroll
+      for (int c = 0; c < N; ++c) o[c] = lane[(r * NC + c) << lsh];
+    } else {
+      const uint64_t base = (uint64_t)(r - R_lds) * NC;
+#pragma unroll
+      for (int c = 0; c < N; ++c) o[c] = gl[(base + c) * gstride + p];
+    }
+  }
   __device__ __forceinline__ void wr(uint32_t r, const F* v) const {
     if (LDSONLY) {
 #pragma unroll
@@ -289,9 +308,17 @@ __device__ __forceinline__ void dense_fwd_block(const Regs_& R, AS4 const uint32
 // a per-term select (the peeled value-program form raises the jet
 // interpreter's register use: 115 instead of 96 VGPRs at order 3 in f32),
 // pool words and weights through scalar loads.
-template <class F, int NC, int JB, bool FULL, class Regs_>
+// NZ: how many leading coefficients of the operands may be non-zero.  NZ <
+// NC when every operand is a tape input (D_INPUT_OPERANDS): jets seed inputs
+// as (x, v, 0, ..), so coefficients NZ.. are +0 and are not read.  With
+// finite weights, an accumulator of such an order then holds +0, or NaN
+// once a non-finite operand went through the full product, and every
+// further term adds +0 to it, which changes neither; so those accumulators
+// are set once and only the full product touches them.
+template <class F, int NC, int JB, bool FULL, int NZ, class Regs_>
 __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
                                                 uint32_t T, uint32_t j0, uint32_t jb) {
+  static_assert(NZ >= 1 && NZ <= NC, "coefficients");
   F acc[JB][NC];
   // true once a product term has entered the sums: a Taylor product's
   // coefficients are sums that start at +0, so from then on no accumulator
@@ -301,13 +328,21 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
     const uint32_t kr = PR[p];
     const uint32_t kind = kr >> P_KIND_SHIFT;
     AS4 const F* w = PW + (uint64_t)p * m + j0;
+    if (p == 0 && NZ < NC) {
+#pragma unroll
+      for (int j = 0; j < JB; ++j)
+#pragma unroll
+        for (int q = NZ; q < NC; ++q) acc[j][q] = F(0);
+    }
     if (kind == T_PROD) {
       F x[NC];
-      R.rd(kr & P_REG_MASK, x);
+      R.template rd_n<NZ>(kr & P_REG_MASK, x);
+#pragma unroll
+      for (int q = NZ; q < NC; ++q) x[q] = F(0);
       const bool cleft = (kr & P_CLEFT) != 0;
       bool fin = true;
 #pragma unroll
-      for (int q = 0; q < NC; ++q) fin = fin && isfinite(x[q]);
+      for (int q = 0; q < NZ; ++q) fin = fin && isfinite(x[q]);
       if (__all(fin)) {
         // With every coefficient of x finite the Taylor product with the
         // constant jet [c, 0, ..] reduces exactly to 0 + x_k * c: its other
@@ -319,7 +354,7 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
             if (!(FULL || (uint32_t)j < jb)) continue;
             const F c = w[j];
 #pragma unroll
-            for (int q = 0; q < NC; ++q) acc[j][q] = acc[j][q] + x[q] * c;
+            for (int q = 0; q < NZ; ++q) acc[j][q] = acc[j][q] + x[q] * c;
           }
         } else {
 #pragma unroll
@@ -327,7 +362,7 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
             if (!(FULL || (uint32_t)j < jb)) continue;
             const F c = w[j];
 #pragma unroll
-            for (int q = 0; q < NC; ++q) {
+            for (int q = 0; q < NZ; ++q) {
               const F term = F(0) + x[q] * c;
               acc[j][q] = (p == 0) ? term : acc[j][q] + term;
             }
@@ -351,12 +386,12 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
       }
     } else if (kind == T_REG) {
       F x[NC];
-      R.rd(kr & P_REG_MASK, x);
+      R.template rd_n<NZ>(kr & P_REG_MASK, x);
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
 #pragma unroll
-        for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? x[q] : acc[j][q] + x[q];
+        for (int q = 0; q < NZ; ++q) acc[j][q] = (p == 0) ? x[q] : acc[j][q] + x[q];
       }
     } else {
 #pragma unroll
@@ -364,7 +399,7 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
         if (!(FULL || (uint32_t)j < jb)) continue;
         const F c = w[j];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) {
+        for (int q = 0; q < NZ; ++q) {
           const F term = (q == 0) ? c : F(0);
           acc[j][q] = (p == 0) ? term : acc[j][q] + term;
         }
@@ -380,13 +415,19 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
 
 template <class F, int NC, int JB, bool JET, class Regs_>
 __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, uint32_t m, uint32_t T,
-                                          uint32_t off_regs, uint32_t off_w) {
+                                          uint32_t off_regs, uint32_t off_w, uint32_t dflags = 0) {
   uint32_t j0 = 0;
   if constexpr (JET) {
     AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
     AS4 const F* PW = reinterpret_cast<AS4 const F*>((AS4 const uint32_t*)pool + off_w);
-    for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true>(R, PR, PW, m, T, j0, JB);
-    if (j0 < m) dense_jet_block<F, NC, JB, false>(R, PR, PW, m, T, j0, m - j0);
+    constexpr int NZI = NC >= 3 ? 2 : NC;  // input operands: (x, v, 0, ..)
+    if (NZI < NC && (dflags & D_INPUT_OPERANDS)) {
+      for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NZI>(R, PR, PW, m, T, j0, JB);
+      if (j0 < m) dense_jet_block<F, NC, JB, false, NZI>(R, PR, PW, m, T, j0, m - j0);
+      return;
+    }
+    for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NC>(R, PR, PW, m, T, j0, JB);
+    if (j0 < m) dense_jet_block<F, NC, JB, false, NC>(R, PR, PW, m, T, j0, m - j0);
   } else {
     static_assert(NC == 1, "value programs carry one component");
     AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
@@ -955,7 +996,8 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
       }
       R.wr(dst, acc.c);
     } else if (op == I_DENSE) {
-      dense_fwd<F, K, (K <= 2 ? 16 : (K == 3 ? jet3_block<F>() : 4)), true>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4));
+      dense_fwd<F, K, (K <= 2 ? 16 : (K == 3 ? jet3_block<F>() : 4)), true>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4),
+                                                                          I.u(5));
     } else if (op == I_LOADN) {
       load_inputs<F, K, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
 #pragma unroll

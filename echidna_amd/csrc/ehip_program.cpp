@@ -2,6 +2,7 @@
 #include "ehip_program.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -891,6 +892,9 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
         // DENSE: [T][m] (a term's weights for all members together);
         // DENSE_REV: [m][T] (a member's weights for all adjoints together)
         bool prod_nz = true;
+        bool in_ops = x.op == I_DENSE;
+        for (auto& tm : x.terms)
+          if (tm.kind != T_IMM && !(tm.vid >= 0 && tm.vid < (int64_t)t.num_inputs)) in_ops = false;
         for (size_t a = 0; a < (x.op == I_DENSE ? T : m); ++a)
           for (size_t b = 0; b < (x.op == I_DENSE ? m : T); ++b) {
             const size_t j = x.op == I_DENSE ? b : a, q = x.op == I_DENSE ? a : b;
@@ -903,10 +907,11 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
             uint32_t lo = 0, hi = 0;
             put_imm(v, lo, hi);
             if ((t.f64 ? v : (double)(float)v) == 0.0) prod_nz = false;
+            if (!std::isfinite(t.f64 ? v : (double)(float)v)) in_ops = false;
             out.pool.push_back(lo);
             if (t.f64) out.pool.push_back(hi);
           }
-        w[5] = prod_nz ? D_ALL_PROD_NZ : 0u;
+        w[5] = (prod_nz ? D_ALL_PROD_NZ : 0u) | (in_ops ? D_INPUT_OPERANDS : 0u);
         break;
       }
       default:

@@ -121,6 +121,10 @@ constexpr uint32_t P_DEEP = 1u << 28;
 // DENSE_REV w5: every term is a product and no weight is zero (in the
 // program's dtype), so only the adjoints can trigger the reference guards.
 constexpr uint32_t D_ALL_PROD_NZ = 1u;
+// DENSE w5: every operand is a tape input and every weight is finite (in
+// the program's dtype).  Jet interpreters seed inputs as (x, v, 0, ..), so
+// the operands' coefficients of order >= 2 are +0 there.
+constexpr uint32_t D_INPUT_OPERANDS = 2u;
 
 struct Program {
   std::vector<uint32_t> words;  // NW words per instruction

@@ -200,3 +200,43 @@ def test_dense_hvp_matches_dot_program_and_oracle(hip, name, f64):
         fin = np.isfinite(ref)
         assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
         assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-11
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_dense_jets_over_inputs_with_nonfinite_points(hip, f64):
+    """Order-3 jets through DENSE groups whose operands are tape inputs (the
+    seeds' second coefficients are zero and are not read): bit-identical to
+    the ungrouped program (NaN signs aside) also when some inputs and
+    directions are infinite or NaN, which sends terms through the full Taylor
+    product; f64 against the oracle."""
+    import pyoracle as po
+
+    tape = wl.tanh_mlp2(20, 24, 18, 35)
+    F = np.float64 if f64 else np.float32
+    B = 640
+    x = wl.points_normal(17, B, 20)
+    v = philox.normal(18, philox.STREAM_DIRS, np.arange(B * 20)).reshape(B, 20)
+    x[3, 5] = np.inf
+    x[70:72, 0] = np.nan
+    v[130, 2] = -np.inf
+    x[200:264, 7] = np.inf  # a whole wave of lanes
+    x, v = x.astype(F), v.astype(F)
+    outs = []
+    for dense in (True, False):
+        hip.set_dense(dense)
+        try:
+            bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+        finally:
+            hip.set_dense(True)
+        tk = hip.taylor_forward_kth_batch(bufs, x.reshape(-1), v.reshape(-1), B, 3)
+        outs.append([np.asarray(c).copy() for c in tk.coefficients])
+    for a, b in zip(*outs):
+        assert np.array_equal(_bits(a, f64), _bits(b, f64))
+    if f64:
+        with np.errstate(all="ignore"):
+            ref = po.taylor(tape, x, v, K=3)[:, 0, :]
+        got = np.stack(outs[0], axis=1).reshape(B, 3)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        fin = np.isfinite(ref)
+        assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
+        assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-12
