@@ -384,15 +384,20 @@ def run_dags(args) -> dict:
     t_setup = time.perf_counter() - t0
     points = sum(j[3] for j in jobs)
     reps = max(1, args.steps // 4)
-    # program fetch for the queue (and the per-tape launches it is compared
-    # with): LDS windows or the scalar cache, whichever runs the queue faster
-    fetch_ms = {}
+    # launch shape of the queue, tuned before timing: program fetch (LDS
+    # windows or the scalar cache) x tile width (workgroup size) x LDS budget
+    # for the register file; the per-tape launches it is compared with run
+    # the same shape
+    shapes = {}
     for mode in (0, 1):
         ctx.set_fetch(mode)
-        fetch_ms[mode], _ = timed(lambda: ctx.multi_batch_dev(jobs), 1)
-    fetch = min(fetch_ms, key=fetch_ms.get)
+        for wg in (64, 128, 256):
+            for lds in (8192, 16384, 24576, 32768, 49152, 65536):
+                shapes[(mode, wg, lds)], _ = timed(lambda: ctx.multi_batch_dev(jobs, wg=wg, lds_budget=lds), 2)
+    fetch, wg_q, lds_q = min(shapes, key=shapes.get)
     ctx.set_fetch(fetch)
-    dt_q, _ = timed(lambda: ctx.multi_batch_dev(jobs), reps)
+    ctx.set_launch(wg_q, lds_q)
+    dt_q, _ = timed(lambda: ctx.multi_batch_dev(jobs, wg=wg_q, lds_budget=lds_q), reps)
 
     def per_tape():
         for b, _, x, B, y, g in jobs:
@@ -403,7 +408,7 @@ def run_dags(args) -> dict:
             "higher_is_better": True, "vs_baseline": None, "dtype": args.dtype, "data": "synthetic random DAGs",
             "config": {"workload": f"64 random DAGs seed 3 scale {args.dag_scale}, {points} points, "
                                    f"{ops:.3e} entry-evals", "points": points,
-                       "fetch": "scalar" if fetch else "lds"},
+                       "fetch": "scalar" if fetch else "lds", "wg": wg_q, "lds_budget": lds_q},
             "queue": {"seconds": dt_q, "entry_evals_per_s": ops / dt_q},
             "per_tape_launches": {"seconds": dt_s, "points_per_s": points / dt_s},
             "speedup": dt_s / dt_q, "setup_s": t_setup}

@@ -182,6 +182,7 @@ int32_t ehip_set_specialize(ehip_ctx* c, uint32_t max_entries, uint32_t what_mas
 
 void ehip_tape_free(ehip_tape* t) {
   if (!t) return;
+  if (t->ctx) (void)hipSetDevice(t->ctx->device);
   for (auto& p : t->prog) free_program(p);
   for (auto& k : t->spec) spec_free(k);
   delete t;
