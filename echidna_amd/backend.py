@@ -113,11 +113,11 @@ _SIGS = {
     "ehip_graph_destroy": (None, [_vp]),
     "ehip_autotune_dev": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_uint32, _vp, ctypes.c_uint64, _vp,
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
-    "ehip_tape_specialize": (ctypes.c_int32, [_vp, ctypes.c_int32, _c_u32p]),
+    "ehip_tape_specialize": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
     "ehip_set_specialize": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
     "ehip_spec_source": (ctypes.c_int32, [ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _c_f64p, ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32, ctypes.c_int32,
-                                          ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _c_u64p]),
+                                          ctypes.c_uint32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _c_u64p]),
     "ehip_set_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_uint32]),
     "ehip_set_fetch": (ctypes.c_int32, [_vp, ctypes.c_int32]),
@@ -251,7 +251,8 @@ def compile_program(data: GpuTapeData, mode: int = 1, fuse: bool = True, f64: Op
     return words[:nw].reshape(-1, 8), terms[:nt], info2
 
 
-def spec_source(data: GpuTapeData, what: int = 1, f64: Optional[bool] = None, arch: Optional[str] = None):
+def spec_source(data: GpuTapeData, what: int = 1, f64: Optional[bool] = None, arch: Optional[str] = None,
+                waves: int = 0):
     """The tape-specialised kernel's source (no device needed).
 
     With ``arch`` (e.g. "gfx950") the source is also compiled by hipRTC.
@@ -265,8 +266,8 @@ def spec_source(data: GpuTapeData, what: int = 1, f64: Optional[bool] = None, ar
     info = (ctypes.c_uint64 * 3)()
     call = lambda a, buf, cap: lib.ehip_spec_source(
         EHIP_F64 if f64 else EHIP_F32, *[x.ctypes.data_as(_c_u32p) for x in arrs], consts.ctypes.data_as(_c_f64p),
-        data.num_ops, data.num_inputs, data.output_index, outs.ctypes.data_as(_c_u32p), len(outs), what, a, buf,
-        cap, info)
+        data.num_ops, data.num_inputs, data.output_index, outs.ctypes.data_as(_c_u32p), len(outs), what, waves, a,
+        buf, cap, info)
     _check(call(None, None, 0))
     buf = ctypes.create_string_buffer(int(info[0]) + 1)
     _check(call(arch.encode() if arch else None, buf, len(buf)))
@@ -506,16 +507,17 @@ class HipContext:
         _check(load_library().ehip_set_specialize(self.handle, max_entries, what_mask))
 
     @staticmethod
-    def specialize(tape: "HipTapeBuffers", what: int = 1) -> dict:
+    def specialize(tape: "HipTapeBuffers", what: int = 1, waves: int = 0) -> dict:
         """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient, 2 hvp).
 
         Untuned evaluations of that kind then run it; ``set_tuned`` (fetch 2
         or an interpreter shape) and ``autotune_dev`` choose explicitly.
+        ``waves``: occupancy target 1..8, or 0 for the automatic choice.
         """
-        out = (ctypes.c_uint32 * 4)()
-        _check(load_library().ehip_tape_specialize(tape.handle, what, out))
+        out = (ctypes.c_uint32 * 5)()
+        _check(load_library().ehip_tape_specialize(tape.handle, what, waves, out))
         return {"vgprs": int(out[0]), "scratch": int(out[1]), "compile_ms": out[2] / 1000.0,
-                "code_bytes": int(out[3])}
+                "code_bytes": int(out[3]), "waves_per_eu": int(out[4])}
 
     @staticmethod
     def set_tuned(tape: "HipTapeBuffers", what: int, wg: int, lds_budget: int, order: int = 0,

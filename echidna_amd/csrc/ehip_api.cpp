@@ -158,8 +158,8 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
   if (num_ops <= smax)
     for (int32_t what = 0; what < 3; ++what) {
       if (!(smask & (1u << what)) || (what == 2 && t->nout != 1)) continue;
-      uint32_t info[4];
-      int32_t rc = ehip_tape_specialize(t, what, info);
+      uint32_t info[5];
+      int32_t rc = ehip_tape_specialize(t, what, 0, info);
       if (rc != EHIP_OK) {
         ehip_tape_free(t);
         return rc;

@@ -118,6 +118,7 @@ struct SpecKernel {
   uint32_t vgprs = 0, scratch = 0, code_bytes = 0;
   double compile_ms = 0;
   bool dual = false;  // HVP kernel: ehip_spec(X, xstride, V, G, H, B)
+  uint32_t waves = 0;  // occupancy target it was compiled for (0: the compiler's choice)
 };
 
 // Longest tape the specialiser accepts (compile time and code size grow

@@ -87,21 +87,25 @@ struct Gen {
 // 2 gradient and Hessian-vector product of output 0 (tangent-over-reverse
 // on Dual numbers: forward_tangent, src/bytecode_tape/tangent.rs:97-147, then
 // reverse_tangent_inner, tangent.rs:191-262, as in hvp, tangent.rs:268-310).
-std::string spec_source(const TapeHost& h, int what) {
+std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
   Gen g(h);
   const bool dual = what == 2;
   const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
   static const char* kWhat[] = {"values", "value and gradient of output 0", "gradient and HVP of output 0"};
   g.put("// generated: %u entries, %u inputs, %u outputs, %s, %s\n", N, n, m, h.f64 ? "f64" : "f32", kWhat[what]);
   g.put("#include \"ehip_math.h\"\n");
+  // occupancy target: the compiler keeps at most 512 / waves registers per
+  // lane, spilling the rest
+  if (waves) g.put("#define EHIP_OCC __attribute__((amdgpu_waves_per_eu(%u)))\n", waves);
+  else g.put("#define EHIP_OCC\n");
   g.put("typedef %s F;\n", h.f64 ? "double" : "float");
   g.put("typedef ehip::Dual<F> D;\n");
   if (dual)
-    g.put("extern \"C\" __global__ void __launch_bounds__(256) ehip_spec(const F* __restrict__ X, "
+    g.put("extern \"C\" __global__ void __launch_bounds__(256) EHIP_OCC ehip_spec(const F* __restrict__ X, "
           "unsigned long long xs, const F* __restrict__ V, F* __restrict__ G, F* __restrict__ H, "
           "unsigned long long B) {\n");
   else
-    g.put("extern \"C\" __global__ void __launch_bounds__(256) ehip_spec(const F* __restrict__ X, "
+    g.put("extern \"C\" __global__ void __launch_bounds__(256) EHIP_OCC ehip_spec(const F* __restrict__ X, "
           "unsigned long long xs, F* __restrict__ Y, F* __restrict__ G, unsigned long long B) {\n");
   g.put("const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n");
   g.put("if (p >= B) return;\n");
@@ -296,22 +300,17 @@ hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStr
 
 using namespace ehip_detail;
 
-int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
-  if (!t || !out4) return fail(EHIP_OTHER, "GPU error: null argument");
-  if (what < 0 || what > 2)
-    return fail(EHIP_OTHER, "GPU error: specialised kernels exist for forward (0), gradient (1) and hvp (2)");
-  if (what == 2 && t->nout != 1) return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only");
-  if (t->host.opcodes.size() > SPEC_MAX_ENTRIES)
-    return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%zu entries > %u)", t->host.opcodes.size(),
-                SPEC_MAX_ENTRIES);
-  ehip_ctx* c = t->ctx;
-  hipDeviceProp_t prop;
-  HIP_TRY(hipGetDeviceProperties(&prop, c->device), "device properties");
-  std::string arch = prop.gcnArchName;
-  arch = arch.substr(0, arch.find(':'));
-  // generate and compile without the context lock (seconds for a few
-  // thousand entries); load and swap under it
-  const std::string src = spec_source(t->host, what);
+namespace {
+
+// Waves per SIMD a kernel of `regs` registers per lane (VGPRs and AGPRs,
+// allocated in granules of 8 out of 512) can keep resident.
+uint32_t occupancy(uint32_t regs) {
+  const uint32_t g = (std::max<uint32_t>(regs, 1) + 7) / 8 * 8;
+  return std::min<uint32_t>(8, 512 / g);
+}
+
+int32_t spec_build(ehip_ctx* c, const TapeHost& h, int what, uint32_t waves, const std::string& arch, SpecKernel& s) {
+  const std::string src = spec_source(h, what, waves);
   std::vector<char> code;
   std::string log;
   double ms = 0;
@@ -319,7 +318,6 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
-  SpecKernel s;
   HIP_TRY(hipModuleLoadData(&s.mod, code.data()), "load specialised kernel");
   hipError_t e = hipModuleGetFunction(&s.fn, s.mod, "ehip_spec");
   if (e != hipSuccess) {
@@ -332,20 +330,69 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t* out4) {
   s.code_bytes = (uint32_t)code.size();
   s.compile_ms = ms;
   s.dual = what == 2;
+  s.waves = waves;
+  return EHIP_OK;
+}
+
+}  // namespace
+
+int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, uint32_t* out5) {
+  if (!t || !out5) return fail(EHIP_OTHER, "GPU error: null argument");
+  if (what < 0 || what > 2)
+    return fail(EHIP_OTHER, "GPU error: specialised kernels exist for forward (0), gradient (1) and hvp (2)");
+  if (what == 2 && t->nout != 1) return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only");
+  if (waves_per_eu > 8) return fail(EHIP_OTHER, "GPU error: waves_per_eu must be 0 (automatic) or 1..8");
+  if (t->host.opcodes.size() > SPEC_MAX_ENTRIES)
+    return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%zu entries > %u)", t->host.opcodes.size(),
+                SPEC_MAX_ENTRIES);
+  ehip_ctx* c = t->ctx;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, c->device), "device properties");
+  std::string arch = prop.gcnArchName;
+  arch = arch.substr(0, arch.find(':'));
+  // compile without the context lock (seconds for a few thousand entries);
+  // load under it
+  SpecKernel s;
+  int32_t rc = spec_build(c, t->host, what, waves_per_eu, arch, s);
+  if (rc) return rc;
+  if (waves_per_eu == 0) {
+    // Automatic: below three waves per SIMD, build again for one wave more
+    // and keep that.  On the n=32 H=64 MLP: f64 fwd+grad 1 -> 2 waves 748 ->
+    // 1,096 M points/s with 28 VGPRs spilled; f64 HVP 1 -> 2, 567 -> 744;
+    // f32 HVP 2 -> 3, 1,329 -> 1,486; the f32 gradient at 3 waves lost 5%
+    // at 4.
+    const uint32_t occ = occupancy(s.vgprs);
+    if (occ < 3) {
+      SpecKernel s2;
+      rc = spec_build(c, t->host, what, occ + 1, arch, s2);
+      if (rc) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        spec_free(s);
+        return rc;
+      }
+      s2.compile_ms += s.compile_ms;
+      std::lock_guard<std::mutex> lk(c->mu);
+      spec_free(s);
+      s = s2;
+    }
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
   spec_free(t->spec[what]);
   t->spec[what] = s;
-  out4[0] = s.vgprs;
-  out4[1] = s.scratch;
-  out4[2] = (uint32_t)std::min(ms * 1000.0, 4.0e9);
-  out4[3] = s.code_bytes;
+  out5[0] = s.vgprs;
+  out5[1] = s.scratch;
+  out5[2] = (uint32_t)std::min(s.compile_ms * 1000.0, 4.0e9);
+  out5[3] = s.code_bytes;
+  out5[4] = s.waves;
   return ok();
 }
 
 int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                          const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
-                         const uint32_t* output_indices, uint32_t num_outputs, int32_t what, const char* arch,
-                         char* src, uint64_t src_cap, uint64_t* out3) {
-  if (!out3 || what < 0 || what > 2 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
+                         const uint32_t* output_indices, uint32_t num_outputs, int32_t what,
+                         uint32_t waves_per_eu, const char* arch, char* src, uint64_t src_cap, uint64_t* out3) {
+  if (!out3 || what < 0 || what > 2 || waves_per_eu > 8 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)
@@ -368,7 +415,7 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
   th.num_inputs = num_inputs;
   th.outputs = outs;
   th.f64 = dtype == EHIP_F64;
-  const std::string s = spec_source(th, what);
+  const std::string s = spec_source(th, what, waves_per_eu);
   out3[0] = s.size();
   out3[1] = 0;
   out3[2] = 0;

@@ -264,12 +264,17 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * literal, and compiles it with hipRTC for the tape's device.  Arithmetic
  * and order are the interpreters', so results are the same.  Once built it
  * runs every untuned evaluation of that kind; ehip_set_tuned (fetch 2, or
- * 0/1 for the interpreters) and ehip_autotune_dev choose explicitly.  out4 = {VGPRs, scratch bytes per lane, compile
- * microseconds, code object bytes}.  Tapes longer than 65536 entries are
- * refused.  The compile runs outside the context lock; building a kernel
- * again replaces the old one after the device is idle, which invalidates
- * graphs captured with it. */
-int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t* out4);
+ * 0/1 for the interpreters) and ehip_autotune_dev choose explicitly.
+ * waves_per_eu (1..8) asks the compiler for that occupancy, spilling what
+ * does not fit; 0 = automatic: when the compiler's own choice allows fewer
+ * than three waves per SIMD, the kernel is built again for one wave more
+ * and that build is kept.  out5 = {VGPRs and AGPRs, scratch bytes per
+ * lane, compile microseconds (all builds), code object bytes, the
+ * occupancy target used (0: the compiler's)}.  Tapes longer than 65536
+ * entries are refused.  The compile runs outside the context lock; building
+ * a kernel again replaces the old one after the device is idle, which
+ * invalidates graphs captured with it. */
+int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t waves_per_eu, uint32_t* out5);
 
 /* Specialise every tape uploaded after this call whose entry count is at
  * most max_entries (<= 65536; 0 = off, the default), for the evaluations in
@@ -284,8 +289,8 @@ int32_t ehip_set_specialize(ehip_ctx* ctx, uint32_t max_entries, uint32_t what_m
  * compile microseconds}; `src` (may be null) receives the text. */
 int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                          const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
-                         const uint32_t* output_indices, uint32_t num_outputs, int32_t what, const char* arch,
-                         char* src, uint64_t src_cap, uint64_t* out3);
+                         const uint32_t* output_indices, uint32_t num_outputs, int32_t what,
+                         uint32_t waves_per_eu, const char* arch, char* src, uint64_t src_cap, uint64_t* out3);
 
 /* DENSE grouping for tapes uploaded after this call (default on): fused dot
  * products sharing one operand list run as one register-blocked instruction

@@ -236,3 +236,45 @@ def test_upload_time_specialisation_and_default_use(hip):
     HipContext.set_tuned(bs, TUNE_GRADIENT, 128, 4096, fetch=0)
     y_int, g_int = hip.gradient_batch(bs, x.reshape(-1), B)
     assert np.array_equal(y_spec, y_int) and np.array_equal(g_spec, g_int)
+
+
+@pytest.mark.parametrize("waves", [2, 4])
+def test_occupancy_targets_keep_results(hip, waves):
+    """Kernels built for a higher occupancy (the compiler spills what does
+    not fit) compute the same bits as the interpreters."""
+    from echidna_amd import philox
+    from echidna_amd.backend import TUNE_HVP
+
+    tape = wl.tanh_mlp(12, 40, 31)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    B = 500
+    x = wl.points_normal(44, B, 12)
+    v = philox.normal(45, philox.STREAM_DIRS, np.arange(B * 12)).reshape(B, 12)
+    HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 24576, fetch=1)
+    HipContext.set_tuned(bufs, TUNE_HVP, 256, 24576, fetch=1)
+    ref = [np.asarray(a).copy() for a in (*hip.gradient_batch(bufs, x.reshape(-1), B),
+                                          *hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B))]
+    for what in (TUNE_GRADIENT, TUNE_HVP):
+        info = HipContext.specialize(bufs, what, waves=waves)
+        assert info["waves_per_eu"] == waves and info["vgprs"] <= 512 // waves
+        HipContext.set_tuned(bufs, what, 128, 0, fetch=2)
+    got = [np.asarray(a).copy() for a in (*hip.gradient_batch(bufs, x.reshape(-1), B),
+                                          *hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B))]
+    for a, b in zip(ref, got):
+        assert np.array_equal(_bits(a, True), _bits(b, True))
+
+
+def test_automatic_occupancy_policy(hip):
+    """The f64 n=32 H=64 MLP gradient needs more than 256 registers at the
+    compiler's own choice (one wave per SIMD): the automatic policy rebuilds
+    it for two waves."""
+    tape = wl.tanh_mlp(32, 64, 1)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    info = HipContext.specialize(bufs, TUNE_GRADIENT)
+    assert info["waves_per_eu"] == 2 and info["vgprs"] <= 256
+    B = 300
+    x = wl.points_normal(46, B, 32)
+    y_s, g_s = hip.gradient_batch(bufs, x.reshape(-1), B)
+    HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 24576, fetch=1)
+    y_i, g_i = hip.gradient_batch(bufs, x.reshape(-1), B)
+    assert np.array_equal(y_s, y_i) and np.array_equal(g_s, g_i)

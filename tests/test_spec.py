@@ -90,3 +90,12 @@ def test_dual_source_shape():
     multi, _ = record_multi(lambda v: [v[0] * v[1], v[1]], [1.0, 2.0])
     with pytest.raises(GpuError, match="scalar-output"):
         be.spec_source(_data(multi, False), 2, f64=False)
+
+
+def test_occupancy_target_in_source():
+    tape, _ = record(lambda v: v[0] * v[1] + v[0].sin(), [0.5, 0.25])
+    src0, _ = be.spec_source(_data(tape, True), 1, f64=True)
+    src2, info = be.spec_source(_data(tape, True), 1, f64=True, arch="gfx950", waves=2)
+    assert "#define EHIP_OCC\n" in src0 and "amdgpu_waves_per_eu(2)" in src2 and info["code_bytes"] > 0
+    with pytest.raises(GpuError, match="bad argument"):
+        be.spec_source(_data(tape, True), 1, f64=True, waves=9)
