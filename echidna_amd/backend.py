@@ -503,12 +503,14 @@ class HipContext:
 
     def set_specialize(self, max_entries: int, what_mask: int = 0b011) -> None:
         """Specialise later uploads of at most ``max_entries`` entries (0 = off)
-        for the evaluations in ``what_mask`` (bit 0 forward, 1 gradient, 2 hvp)."""
+        for the evaluations in ``what_mask`` (bit 0 forward, 1 gradient, 2 hvp,
+        3..7 Taylor order 1..5)."""
         _check(load_library().ehip_set_specialize(self.handle, max_entries, what_mask))
 
     @staticmethod
     def specialize(tape: "HipTapeBuffers", what: int = 1, waves: int = 0) -> dict:
-        """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient, 2 hvp).
+        """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient, 2 hvp,
+        3..7 Taylor jets of order 1..5, which the STDE entry points use too).
 
         Untuned evaluations of that kind then run it; ``set_tuned`` (fetch 2
         or an interpreter shape) and ``autotune_dev`` choose explicitly.

@@ -20,7 +20,7 @@ SOURCES = ["ehip_api.cpp", "ehip_program.cpp", "ehip_kernels.hip", "ehip_stde.hi
            "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp", "ehip_spec.cpp"]
 # Headers whose text the library carries for hipRTC (tape-specialised kernels):
 # (file, C symbol).
-EMBED = [("ehip_math.h", "ehip_math_h_text")]
+EMBED = [("ehip_math.h", "ehip_math_h_text"), ("philox.h", "philox_h_text")]
 # The interpreters are built once per program-fetch mode, each in its own
 # namespace (object name, defines).
 VARIANTS = {"ehip_kernels.hip": [("ehip_kernels_lds", ["-DEHIP_SFETCH=0", "-DEHIP_NS=lds_fetch"]),

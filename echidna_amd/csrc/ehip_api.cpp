@@ -156,7 +156,7 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
     smask = c->spec_mask;
   }
   if (num_ops <= smax)
-    for (int32_t what = 0; what < 3; ++what) {
+    for (int32_t what = 0; what < 8; ++what) {
       if (!(smask & (1u << what)) || (what == 2 && t->nout != 1)) continue;
       uint32_t info[5];
       int32_t rc = ehip_tape_specialize(t, what, 0, info);
@@ -171,7 +171,8 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
 
 int32_t ehip_set_specialize(ehip_ctx* c, uint32_t max_entries, uint32_t what_mask) {
   if (!c) return fail(EHIP_OTHER, "GPU error: null argument");
-  if (what_mask > 7) return fail(EHIP_OTHER, "GPU error: what_mask holds bits 0 forward, 1 gradient, 2 hvp");
+  if (what_mask > 255)
+    return fail(EHIP_OTHER, "GPU error: what_mask holds bits 0 forward, 1 gradient, 2 hvp, 3..7 taylor order 1..5");
   if (max_entries > SPEC_MAX_ENTRIES)
     return fail(EHIP_OTHER, "GPU error: max_entries above the specialiser's limit (%u)", SPEC_MAX_ENTRIES);
   std::lock_guard<std::mutex> lk(c->mu);
@@ -553,7 +554,7 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
     if (rc) break;
   }
   // the tape-specialised kernel, where one has been compiled (ehip_tape_specialize)
-  if (rc == EHIP_OK && (kind == 0 || kind == 1) && t->spec[kind == 1 ? 2 : (what == 1 ? 1 : 0)].fn)
+  if (rc == EHIP_OK && t->spec[tune_key(kind, K, what == 1)].fn)
     for (uint32_t wg : {64u, 128u, 256u}) {
       Tuned cand;
       cand.wg = wg;
@@ -593,7 +594,7 @@ int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, 
   if (what >= 0 && what <= 2) key = what;
   else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
   else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
-  if (fetch == 2 && wg && (what > 2 || !t->spec[what].fn))
+  if (fetch == 2 && wg && !t->spec[key].fn)
     return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
   t->tuned[key].wg = wg;
   t->tuned[key].lds_budget = wg ? lds_budget : 0;

@@ -118,6 +118,7 @@ struct SpecKernel {
   uint32_t vgprs = 0, scratch = 0, code_bytes = 0;
   double compile_ms = 0;
   bool dual = false;  // HVP kernel: ehip_spec(X, xstride, V, G, H, B)
+  uint32_t order = 0;  // Taylor kernel of this order: ehip_spec(X, xstride, V, Y, B, S, rng, seed)
   uint32_t waves = 0;  // occupancy target it was compiled for (0: the compiler's choice)
 };
 
@@ -138,7 +139,7 @@ struct ehip_tape {
   DevProgram prog[4];  // forward, gradient, HVP (dual interpreter), kinks; all with DENSE groups
   uint32_t nkinks = 0;
   Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
-  SpecKernel spec[3];  // tape-specialised forward, gradient and HVP kernels, once compiled
+  SpecKernel spec[8];  // tape-specialised kernels, once compiled, indexed like `tuned`
 };
 
 namespace ehip_detail {
@@ -196,10 +197,12 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   const Tuned& tu = force ? *force : t->tuned[tune_key(kind, K, L.g != nullptr)];
   const uint32_t wg = tu.wg ? tu.wg : c->wg;
   // untuned evaluations run the tape-specialised kernel once it is built
-  const bool has_spec = kind == 0 ? t->spec[L.g ? 1 : 0].fn != nullptr : (kind == 1 && t->spec[2].fn != nullptr);
+  // (kinks have none; the other kinds share the tuning key's slot)
+  const int slot = kind == 3 ? -1 : tune_key(kind, K, L.g != nullptr);
+  const bool has_spec = slot >= 0 && t->spec[slot].fn != nullptr;
   const uint32_t fm = tu.wg ? tu.fetch : (has_spec ? 2u : c->fetch);
-  if (fm == 2 && (kind == 0 || kind == 1)) {
-    const SpecKernel& sk = t->spec[kind == 1 ? 2 : (L.g ? 1 : 0)];
+  if (fm == 2 && slot >= 0) {
+    const SpecKernel& sk = t->spec[slot];
     if (!sk.fn) return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
     hipError_t e = spec_launch(sk, L, wg, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");

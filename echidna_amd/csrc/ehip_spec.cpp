@@ -35,7 +35,9 @@
 
 #include "ehip_internal.h"
 
-extern "C" const char ehip_math_h_text[];  // ehip_math.h, embedded when the library is built
+// ehip_math.h and philox.h, embedded when the library is built
+extern "C" const char ehip_math_h_text[];
+extern "C" const char philox_h_text[];
 
 namespace ehip_detail {
 
@@ -83,11 +85,55 @@ struct Gen {
 
 }  // namespace
 
+// Taylor jets of order K for every output (forward_tangent at T =
+// Taylor<F, K>, src/bytecode_tape/tangent.rs:97-147, as taylor_forward_kth
+// does per lane): lane p seeds input i with (x_i, v_i, 0, ..) where x is row
+// p / S of X (S = 0: row p) and v is read from V or drawn on the device
+// (philox.h lane_direction), the interpreters' launch convention.
+std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
+  Gen g(h);
+  const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
+  g.put("// generated: %u entries, %u inputs, %u outputs, %s, Taylor jets of order %d\n", N, n, m,
+        h.f64 ? "f64" : "f32", K);
+  g.put("#include \"ehip_math.h\"\n#include \"philox.h\"\n");
+  if (waves) g.put("#define EHIP_OCC __attribute__((amdgpu_waves_per_eu(%u)))\n", waves);
+  else g.put("#define EHIP_OCC\n");
+  g.put("typedef %s F;\n", h.f64 ? "double" : "float");
+  g.put("typedef ehip::Jet<F, %d> J;\n", K);
+  g.put("extern \"C\" __global__ void __launch_bounds__(256) EHIP_OCC ehip_spec(const F* __restrict__ X, "
+        "unsigned long long xs, const F* __restrict__ V, F* __restrict__ Y, unsigned long long B, unsigned S, "
+        "unsigned rng, unsigned long long seed) {\n");
+  g.put("const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n");
+  g.put("if (p >= B) return;\n");
+  g.put("const F* __restrict__ x = X + (S ? p / S : p) * xs;\n");
+  g.put("auto seed_jet = [&](unsigned i) {\n  J r = ehip::jc<F, %d>(x[i]);\n", K);
+  if (K > 1)
+    g.put("  r.c[1] = rng ? (F)ehip::lane_direction(rng, seed, S, p, %uu, i) : V[p * %uull + i];\n", n, n);
+  g.put("  return r;\n};\n");
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t op = h.opcodes[i], a = h.arg0[i], b = h.arg1[i];
+    if (op == OP_INPUT) g.put("const J v%u = seed_jet(%u);\n", i, i);
+    else if (op == OP_CONST)
+      g.out += "const J v" + std::to_string(i) + " = ehip::jc<F, " + std::to_string(K) + ">(" + g.lit(h.values[i]) +
+               ");\n";
+    else if (op == OP_POWI) g.put("const J v%u = ehip::j_powi<F, %d>(v%u, %d);\n", i, K, a, (int32_t)b);
+    else if (b != UNUSED) g.put("const J v%u = ehip::j_eval<F, %d>(%uu, v%u, v%u);\n", i, K, op, a, b);
+    else g.put("const J v%u = ehip::j_eval<F, %d>(%uu, v%u, ehip::jc<F, %d>(F(0)));\n", i, K, op, a, K);
+  }
+  g.put("F* __restrict__ y = Y + p * %uull;\n", m * K);
+  for (uint32_t j = 0; j < m; ++j)
+    for (int q = 0; q < K; ++q) g.put("y[%u] = v%u.c[%d];\n", j * K + q, h.outputs[j], q);
+  g.put("}\n");
+  return g.out;
+}
+
 // what: 0 values of every output; 1 values and the gradient of output 0;
 // 2 gradient and Hessian-vector product of output 0 (tangent-over-reverse
 // on Dual numbers: forward_tangent, src/bytecode_tape/tangent.rs:97-147, then
-// reverse_tangent_inner, tangent.rs:191-262, as in hvp, tangent.rs:268-310).
+// reverse_tangent_inner, tangent.rs:191-262, as in hvp, tangent.rs:268-310);
+// 3..7 Taylor jets of order what - 2.
 std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
+  if (what >= 3) return spec_source_jets(h, what - 2, waves);
   Gen g(h);
   const bool dual = what == 2;
   const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
@@ -246,9 +292,9 @@ int32_t spec_compile(const std::string& src, const char* arch, std::vector<char>
                      double* ms) {
   const auto t0 = std::chrono::steady_clock::now();
   hiprtcProgram prog;
-  const char* hdrs[] = {ehip_math_h_text};
-  const char* names[] = {"ehip_math.h"};
-  if (hiprtcCreateProgram(&prog, src.c_str(), "ehip_spec.hip", 1, hdrs, names) != HIPRTC_SUCCESS)
+  const char* hdrs[] = {ehip_math_h_text, philox_h_text};
+  const char* names[] = {"ehip_math.h", "philox.h"};
+  if (hiprtcCreateProgram(&prog, src.c_str(), "ehip_spec.hip", 2, hdrs, names) != HIPRTC_SUCCESS)
     return fail(EHIP_SHADER_COMPILATION, "hipRTC: could not create the program");
   std::string archopt = std::string("--offload-arch=") + arch;
   const char* opts[] = {archopt.c_str(), "-O3", "-std=c++17",
@@ -290,10 +336,15 @@ hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStr
   void* G = L.g;
   void* H = L.h;
   unsigned long long B = L.B;
+  unsigned S = L.S, rng = L.rng;
+  unsigned long long seed = L.seed;
   void* args_fwd[] = {(void*)&X, (void*)&xs, (void*)&Y, (void*)&G, (void*)&B};
   void* args_hvp[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&G, (void*)&H, (void*)&B};
+  void* args_jet[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&Y, (void*)&B, (void*)&S, (void*)&rng, (void*)&seed};
   const uint64_t grid = (L.B + wg - 1) / wg;
-  return hipModuleLaunchKernel(s.fn, (uint32_t)grid, 1, 1, wg, 1, 1, 0, st, s.dual ? args_hvp : args_fwd, nullptr);
+  if (grid == 0 || grid > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
+  return hipModuleLaunchKernel(s.fn, (uint32_t)grid, 1, 1, wg, 1, 1, 0, st,
+                               s.order ? args_jet : (s.dual ? args_hvp : args_fwd), nullptr);
 }
 
 }  // namespace ehip_detail
@@ -330,6 +381,7 @@ int32_t spec_build(ehip_ctx* c, const TapeHost& h, int what, uint32_t waves, con
   s.code_bytes = (uint32_t)code.size();
   s.compile_ms = ms;
   s.dual = what == 2;
+  s.order = what >= 3 ? (uint32_t)(what - 2) : 0u;
   s.waves = waves;
   return EHIP_OK;
 }
@@ -338,8 +390,9 @@ int32_t spec_build(ehip_ctx* c, const TapeHost& h, int what, uint32_t waves, con
 
 int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, uint32_t* out5) {
   if (!t || !out5) return fail(EHIP_OTHER, "GPU error: null argument");
-  if (what < 0 || what > 2)
-    return fail(EHIP_OTHER, "GPU error: specialised kernels exist for forward (0), gradient (1) and hvp (2)");
+  if (what < 0 || what > 7)
+    return fail(EHIP_OTHER,
+                "GPU error: specialised kernels exist for forward (0), gradient (1), hvp (2) and taylor order 1..5 (3..7)");
   if (what == 2 && t->nout != 1) return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only");
   if (waves_per_eu > 8) return fail(EHIP_OTHER, "GPU error: waves_per_eu must be 0 (automatic) or 1..8");
   if (t->host.opcodes.size() > SPEC_MAX_ENTRIES)
@@ -392,7 +445,7 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
                          const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
                          const uint32_t* output_indices, uint32_t num_outputs, int32_t what,
                          uint32_t waves_per_eu, const char* arch, char* src, uint64_t src_cap, uint64_t* out3) {
-  if (!out3 || what < 0 || what > 2 || waves_per_eu > 8 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
+  if (!out3 || what < 0 || what > 7 || waves_per_eu > 8 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)

@@ -5,8 +5,12 @@
 // in binary64 (cosine branch) from words (0,1) and (2,3); Rademacher signs
 // come from bit 0 of word 0.
 #pragma once
+// Also compiled at run time by hipRTC (tape-specialised jet kernels), which
+// supplies the HIP runtime declarations and fixed-width integers itself.
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace ehip {
 

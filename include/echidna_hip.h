@@ -249,7 +249,7 @@ void ehip_graph_destroy(ehip_graph* graph);
 /* Restore (wg != 0) or clear (wg == 0) a tuned launch shape, e.g. from a
  * persisted autotuner cache; ehip_get_tuned reads it back
  * (out3 = {wg, lds, fetch}).  fetch 2 selects the tape-specialised kernel
- * (forward, gradient and hvp, once ehip_tape_specialize has built it). */
+ * (every evaluation but kinks, once ehip_tape_specialize has built it). */
 int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
                        uint32_t fetch);
 int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out3);
@@ -258,8 +258,9 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * registers; no counterpart in the reference, whose CUDA backend generates
  * source at run time per Taylor order, src/gpu/taylor_codegen.rs, but never
  * per tape).  Writes the
- * tape's forward (what 0), value-and-gradient (what 1) or gradient-and-HVP
- * (what 2, tangent-over-reverse on dual numbers) sweep as straight-line HIP,
+ * tape's forward (what 0), value-and-gradient (what 1), gradient-and-HVP
+ * (what 2, tangent-over-reverse on dual numbers) or Taylor-jet sweep of
+ * order 1..5 (what 3..7, also serving the STDE entry points) as straight-line HIP,
  * every entry a named value kept in VGPRs and every constant an instruction
  * literal, and compiles it with hipRTC for the tape's device.  Arithmetic
  * and order are the interpreters', so results are the same.  Once built it
@@ -278,8 +279,9 @@ int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t waves_per_e
 
 /* Specialise every tape uploaded after this call whose entry count is at
  * most max_entries (<= 65536; 0 = off, the default), for the evaluations in
- * what_mask (bit 0 forward, bit 1 gradient, bit 2 hvp; hvp only for
- * scalar-output tapes).  A compile failure fails the upload. */
+ * what_mask (bit 0 forward, bit 1 gradient, bit 2 hvp, bits 3..7 taylor
+ * order 1..5; hvp only for scalar-output tapes).  A compile failure fails
+ * the upload. */
 int32_t ehip_set_specialize(ehip_ctx* ctx, uint32_t max_entries, uint32_t what_mask);
 
 /* The source ehip_tape_specialize compiles, from the flattened tape arrays

@@ -15,7 +15,7 @@ import torch
 
 import pyoracle as po
 from echidna_amd import workloads as wl
-from echidna_amd.backend import TUNE_FORWARD, TUNE_GRADIENT, HipContext
+from echidna_amd.backend import TUNE_FORWARD, TUNE_GRADIENT, TUNE_TAYLOR, HipContext
 from echidna_amd.tape import GpuError, GpuTapeData, record_multi
 from test_structure import TAPES
 
@@ -156,7 +156,7 @@ def test_specialise_rejects(hip):
     tape = wl.tanh_mlp(4, 4, 33)
     bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
     with pytest.raises(GpuError, match="specialised kernels exist"):
-        HipContext.specialize(bufs, 3)
+        HipContext.specialize(bufs, 8)
     with pytest.raises(GpuError, match="no specialised kernel"):
         HipContext.set_tuned(bufs, TUNE_FORWARD, 64, 0, fetch=2)
     HipContext.specialize(bufs, TUNE_FORWARD)
@@ -216,7 +216,7 @@ def test_upload_time_specialisation_and_default_use(hip):
     short = wl.tanh_mlp(8, 6, 40)
     long_ = wl.tanh_mlp(8, 64, 41)
     with pytest.raises(GpuError, match="what_mask"):
-        hip.set_specialize(100, 8)
+        hip.set_specialize(100, 256)
     with pytest.raises(GpuError, match="limit"):
         hip.set_specialize(1 << 20, 1)
     hip.set_specialize(len(short.opcodes), 0b111)
@@ -278,3 +278,54 @@ def test_automatic_occupancy_policy(hip):
     HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 24576, fetch=1)
     y_i, g_i = hip.gradient_batch(bufs, x.reshape(-1), B)
     assert np.array_equal(y_s, y_i) and np.array_equal(g_s, g_i)
+
+
+@pytest.mark.parametrize("name,orders", [("kitchen_sink", (1, 2, 3, 4, 5)), ("rosenbrock", (3,)), ("mlp", (2, 3)),
+                                         ("dag0", (3,)), ("guards", (3,))])
+@pytest.mark.parametrize("f64", [False, True])
+def test_specialised_jets_equal_interpreter_and_oracle(hip, name, orders, f64):
+    from echidna_amd import philox
+
+    tape = _tape(name)
+    n = tape.num_inputs
+    F = np.float64 if f64 else np.float32
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    B = 700
+    x = _points(name, B, n, 25).astype(F)
+    v = philox.normal(17, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n).astype(F)
+    with np.errstate(all="ignore"):
+        for K in orders:
+            HipContext.set_tuned(bufs, TUNE_TAYLOR, 128, 8192, order=K, fetch=1)
+            ref = hip.taylor_forward_kth_batch(bufs, x.reshape(-1), v.reshape(-1), B, K)
+            info = HipContext.specialize(bufs, 2 + K)
+            assert info["vgprs"] > 0
+            HipContext.set_tuned(bufs, TUNE_TAYLOR, 0, 0, order=K)  # untuned: the specialised kernel
+            got = hip.taylor_forward_kth_batch(bufs, x.reshape(-1), v.reshape(-1), B, K)
+            for a, b in zip(ref.coefficients, got.coefficients):
+                assert np.array_equal(_bits(a, f64), _bits(b, f64)), (name, K)
+            if f64 and K == 3:
+                m = len(ref.coefficients[0]) // B
+                jets = np.stack(got.coefficients, axis=1).reshape(B, m, 3)
+                oj = po.taylor(tape, x, v, K=3)
+                fin = np.isfinite(oj)
+                assert np.array_equal(np.isnan(jets), np.isnan(oj))
+                assert np.max(np.abs(jets[fin] - oj[fin]) / np.maximum(1.0, np.abs(oj[fin])), initial=0.0) <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["rademacher", "gaussian"])
+def test_specialised_jets_drive_stde(hip, kind):
+    """The STDE entry points run the specialised order-3 kernel with device
+    directions (S lanes per point) and give the interpreter's estimates."""
+    from echidna_amd import cv as lcv
+
+    tape = wl.tanh_mlp(12, 16, 61)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    P, S = 40, 64
+    X = torch.from_numpy(wl.points_normal(62, P, 12).astype(np.float32)).cuda()
+    d = lcv.DIR_GAUSSIAN if kind == "gaussian" else lcv.DIR_RADEMACHER
+    HipContext.set_tuned(bufs, TUNE_TAYLOR, 256, 24576, order=3, fetch=1)
+    ref = np.asarray(lcv.laplacian_batch(hip, bufs, X, S, d, 63, None)).copy()
+    HipContext.specialize(bufs, 5)
+    HipContext.set_tuned(bufs, TUNE_TAYLOR, 64, 0, order=3, fetch=2)
+    got = np.asarray(lcv.laplacian_batch(hip, bufs, X, S, d, 63, None)).copy()
+    assert np.array_equal(ref, got)

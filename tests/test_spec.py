@@ -26,13 +26,13 @@ def test_source_compiles_for_gfx950(name, f64):
         tape = dc.tape()
     else:
         tape = TAPES[name]()
-    for what in (0, 1, 2):
+    for what in (0, 1, 2, 5):  # 5: Taylor jets of order 3
         src, info = be.spec_source(_data(tape, f64), what, f64=f64, arch="gfx950")
         assert info["code_bytes"] > 0
         assert ("typedef double F;" if f64 else "typedef float F;") in src
         stores = re.findall(r"^gp\[(\d+)\] = ", src, re.M)
         hstores = re.findall(r"^hp\[(\d+)\] = ", src, re.M)
-        assert len(stores) == (tape.num_inputs if what else 0)
+        assert len(stores) == (tape.num_inputs if what in (1, 2) else 0)
         assert len(hstores) == (tape.num_inputs if what == 2 else 0)
 
 
@@ -70,7 +70,7 @@ def test_reverse_sweep_shape():
 def test_source_rejects_bad_input():
     tape, _ = record(lambda v: v[0] * v[1], [1.0, 2.0])
     with pytest.raises(GpuError, match="bad argument"):
-        be.spec_source(_data(tape, False), 3, f64=False)
+        be.spec_source(_data(tape, False), 8, f64=False)
     d = _data(tape, False)
     d.arg0 = np.array(d.arg0, dtype=np.uint32)
     d.arg0[-1] = 99
@@ -99,3 +99,14 @@ def test_occupancy_target_in_source():
     assert "#define EHIP_OCC\n" in src0 and "amdgpu_waves_per_eu(2)" in src2 and info["code_bytes"] > 0
     with pytest.raises(GpuError, match="bad argument"):
         be.spec_source(_data(tape, True), 1, f64=True, waves=9)
+
+
+@pytest.mark.parametrize("order", [1, 2, 4, 5])
+def test_jet_source_compiles_for_every_order(order):
+    """Jets of every order on the tape that uses every opcode; inputs are
+    seeded (x, v, 0, ..) with v read or drawn like the interpreters."""
+    tape = TAPES["kitchen_sink"]()
+    src, info = be.spec_source(_data(tape, False), 2 + order, f64=False, arch="gfx950")
+    assert info["code_bytes"] > 0 and f"typedef ehip::Jet<F, {order}> J;" in src
+    assert ("ehip::lane_direction" in src) == (order > 1)
+    assert src.count("y[") == len(_data(tape, False).output_indices) * order
