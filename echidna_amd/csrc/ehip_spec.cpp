@@ -51,16 +51,23 @@ struct Gen {
   const TapeHost& h;
   const bool f64;
   std::string out;
-  char buf[256];
 
   Gen(const TapeHost& th) : h(th), f64(th.f64) { out.reserve(h.opcodes.size() * 96); }
 
+  // printf-style append, sized exactly
   void put(const char* fmt, ...) __attribute__((format(printf, 2, 3))) {
-    va_list ap;
+    va_list ap, ap2;
     va_start(ap, fmt);
-    const int n = std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_copy(ap2, ap);
+    const int n = std::vsnprintf(nullptr, 0, fmt, ap);
     va_end(ap);
-    out.append(buf, (size_t)std::min(n, (int)sizeof buf - 1));
+    if (n > 0) {
+      const size_t at = out.size();
+      out.resize(at + (size_t)n + 1);
+      std::vsnprintf(&out[at], (size_t)n + 1, fmt, ap2);
+      out.resize(at + (size_t)n);
+    }
+    va_end(ap2);
   }
 
   // A constant as a bit-exact literal of the tape's dtype.
