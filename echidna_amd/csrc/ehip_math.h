@@ -71,7 +71,8 @@ template <class F> EH_DEV bool s_exact_i32(F x, int32_t* out) {
 }
 // tanh and its derivative, the reference's 1/cosh^2 (src/opcode.rs:463-466).
 // f32: from e = exp(-2|a|) and m = expm1(-2|a|) = e - 1, tanh|a| = -m / (2 + m)
-// and 1/cosh^2 a = 4e / (1 + e)^2, each a few ulp from the exact value
+// and 1/cosh^2 a = 4e / (1 + e)^2 (the divisions by the 1-ulp hardware
+// reciprocal of a value in [1, 2]), each a few ulp from the exact value
 // (within the reference's 16-ulp budget for tanh), with no cancellation near
 // 0 (m, not e - 1) and no overflow (e <= 1).  About a third of the
 // instructions of the device library's tanhf and coshf, which work in
@@ -82,8 +83,9 @@ template <class F> EH_DEV bool s_exact_i32(F x, int32_t* out) {
 // raised the specialised kernel's registers (512 against 284) and slowed it.
 template <class F> EH_DEV F s_tanh(F a) {
   if constexpr (sizeof(F) == 4) {
+    // 2 + m lies in [1, 2]: the hardware reciprocal (1 ulp) needs no scaling
     const F m = expm1(F(-2) * fabs(a));
-    return copysign(-m / (F(2) + m), a);
+    return copysign(-m * __builtin_amdgcn_rcpf(F(2) + m), a);
   } else {
     return tanh(a);
   }
@@ -91,8 +93,8 @@ template <class F> EH_DEV F s_tanh(F a) {
 template <class F> EH_DEV F s_dtanh(F a) {
   if constexpr (sizeof(F) == 4) {
     const F e = exp(F(-2) * fabs(a));
-    const F q = F(1) + e;
-    return (F(4) * e) / (q * q);
+    const F r = __builtin_amdgcn_rcpf(F(1) + e);  // 1 + e in [1, 2]
+    return (F(4) * e) * (r * r);
   } else {
     const F c = cosh(a);
     return F(1) / (c * c);
