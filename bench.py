@@ -326,7 +326,7 @@ def run_hvp(args) -> dict:
     from echidna_amd import philox
     from echidna_amd import workloads as wl
     from echidna_amd.backend import TUNE_HVP, HipContext
-    from echidna_amd.tape import GpuTapeData
+    from echidna_amd.tape import GpuError, GpuTapeData
 
     n, H = 32, 64
     B = 1 << args.batch_log2
@@ -340,7 +340,12 @@ def run_hvp(args) -> dict:
     v = torch.from_numpy(philox.normal(1, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n).astype(dt)).cuda()
     g = torch.empty_like(x)
     hv = torch.empty_like(x)
-    spec = None if args.no_spec else ctx.specialize(bufs, TUNE_HVP)
+    spec = None
+    if not args.no_spec:
+        try:
+            spec = ctx.specialize(bufs, TUNE_HVP)
+        except GpuError as e:
+            spec = {"error": str(e)}
     tuned = ctx.autotune_dev(bufs, TUNE_HVP, B, x, d_g=g, d_h=hv, d_dirs=v)
     dt_s, _ = timed(lambda: ctx.hvp_batch_dev(bufs, x, n, v, B, g, hv), max(1, args.steps))
     S = 8 if f64 else 4
@@ -428,7 +433,7 @@ def main() -> int:
 
     from echidna_amd import workloads as wl
     from echidna_amd.backend import HipContext
-    from echidna_amd.tape import GpuTapeData
+    from echidna_amd.tape import GpuError, GpuTapeData
 
     f64 = args.dtype == "f64"
     B = 1 << args.batch_log2
@@ -452,8 +457,13 @@ def main() -> int:
     from echidna_amd.backend import TUNE_GRADIENT
     spec = None
     if not args.no_spec:
-        # straight-line kernel for this tape (hipRTC), built before any timing
-        spec = ctx.specialize(bufs, TUNE_GRADIENT)
+        # straight-line kernel for this tape (hipRTC), built before any timing;
+        # should the compile fail, the interpreters (also HIP) run and the
+        # line says why
+        try:
+            spec = ctx.specialize(bufs, TUNE_GRADIENT)
+        except GpuError as e:
+            spec = {"error": str(e)}
     if args.fetch == 2 and args.wg:
         ctx.set_tuned(bufs, TUNE_GRADIENT, args.wg, 0, fetch=2)
     tdt = torch.float64 if f64 else torch.float32
