@@ -268,6 +268,15 @@ template <class F> EH_DEV bool d_zero(Dual<F> a) { return a.re == F(0) && a.eps 
 template <class F> EH_DEV Dual<F> d_chain(Dual<F> x, F f, F df) {
   return {f, (x.eps == F(0)) ? F(0) : x.eps * df};
 }
+// Dual tanh in f32 may use s_tanh / s_dtanh when no lane of the wave has
+// |x| > 40 or a non-finite x (a wave-uniform test).  There cosh^2 cannot
+// overflow, every value the reference's dual formulas produce is finite, and
+// the cheaper forms give the same NaN / Inf pattern for every tangent; past
+// it the wave runs the reference's formulas.
+template <class F> EH_DEV bool d_tanh_fast(F x) {
+  if constexpr (sizeof(F) == 4) return !__any(!(fabs(x) <= F(40)));
+  else return false;
+}
 template <class F> EH_DEV Dual<F> d_powi(Dual<F> x, int32_t n) {
   if (n == 0) return dc(F(1));
   F val = s_powi(x.re, n);
@@ -339,7 +348,11 @@ template <class F> EH_DEV Dual<F> d_eval(uint32_t op, Dual<F> a, Dual<F> b) {
     case OP_ATAN: return d_chain(a, atan(x), k_atan_deriv(x));
     case OP_SINH: return d_chain(a, sinh(x), cosh(x));
     case OP_COSH: return d_chain(a, cosh(x), sinh(x));
-    case OP_TANH: { F c = cosh(x); return d_chain(a, tanh(x), F(1) / (c * c)); }
+    case OP_TANH: {
+      if (d_tanh_fast(x)) return d_chain(a, s_tanh(x), s_dtanh(x));
+      F c = cosh(x);
+      return d_chain(a, tanh(x), F(1) / (c * c));
+    }
     case OP_ASINH: return d_chain(a, asinh(x), k_asinh_deriv(x));
     case OP_ACOSH: return d_chain(a, acosh(x), k_acosh_deriv(x));
     case OP_ATANH: return d_chain(a, atanh(x), k_atanh_deriv(x));
@@ -415,7 +428,18 @@ template <class F> EH_DEV void d_partials(uint32_t op, Dual<F> a, Dual<F> b, Dua
       return;
     case OP_SINH: da = d_eval(OP_COSH, a, a); return;
     case OP_COSH: da = d_eval(OP_SINH, a, a); return;
-    case OP_TANH: { Dual<F> c = d_eval(OP_COSH, a, a); da = d_div(one, d_mul(c, c)); return; }
+    case OP_TANH: {
+      if (d_tanh_fast(a.re)) {
+        // 1/cosh^2 and its tangent -2 tanh / cosh^2 * eps; a zero tangent
+        // stays +0 as in the reference's dual division
+        const F t = s_tanh(a.re), d = s_dtanh(a.re);
+        da = {d, (a.eps == F(0)) ? F(0) : -(F(2) * t * d) * a.eps};
+        return;
+      }
+      Dual<F> c = d_eval(OP_COSH, a, a);
+      da = d_div(one, d_mul(c, c));
+      return;
+    }
     case OP_ASINH:
       if (fabs(a.re) > F(1e8)) {
         Dual<F> inv = d_div(one, a);
