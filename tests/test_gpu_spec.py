@@ -16,7 +16,7 @@ import torch
 import pyoracle as po
 from echidna_amd import workloads as wl
 from echidna_amd.backend import TUNE_FORWARD, TUNE_GRADIENT, TUNE_TAYLOR, HipContext
-from echidna_amd.tape import GpuError, GpuTapeData, record_multi
+from echidna_amd.tape import GpuError, GpuTapeData, record, record_multi
 from test_structure import TAPES
 
 pytestmark = pytest.mark.gpu
@@ -329,3 +329,33 @@ def test_specialised_jets_drive_stde(hip, kind):
     HipContext.set_tuned(bufs, TUNE_TAYLOR, 64, 0, order=3, fetch=2)
     got = np.asarray(lcv.laplacian_batch(hip, bufs, X, S, d, 63, None)).copy()
     assert np.array_equal(ref, got)
+
+
+@pytest.mark.parametrize("spec", [False, True])
+def test_f32_dual_tanh_guard_keeps_reference_nan_pattern(hip, spec):
+    """f32 HVPs through tanh at arguments past cosh^2's overflow (|x| > 44.4)
+    mixed with ordinary ones in the same waves: the waves with a large
+    argument run the reference's dual formulas, so NaN and Inf positions
+    equal the f32 oracle's; finite values agree within the f32 bound."""
+    from echidna_amd.backend import TUNE_HVP
+
+    tape, _ = record(lambda v: (v[0] * 1.5 + v[1]).tanh() * v[1] + v[0].tanh(), [0.1, 0.2])
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    if spec:
+        HipContext.specialize(bufs, TUNE_HVP)
+    B = 512
+    rng = np.random.default_rng(70)
+    x = rng.uniform(-2, 2, (B, 2))
+    x[::9, 0] = rng.uniform(-70, 70, len(x[::9]))  # some lanes past the overflow
+    x[300:364, 0] = 0.5  # a whole wave of ordinary arguments
+    v = rng.normal(size=(B, 2))
+    v[::13] = np.inf
+    x, v = x.astype(np.float32), v.astype(np.float32)
+    with np.errstate(all="ignore"):
+        g, hv = hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B)
+        rg, rhv = po.hvp(tape, x, v, dtype=np.float32)
+    for got, ref in ((np.asarray(g).reshape(B, 2), rg.reshape(B, 2)), (np.asarray(hv).reshape(B, 2), rhv.reshape(B, 2))):
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        assert np.array_equal(np.isinf(got), np.isinf(ref))
+        fin = np.isfinite(ref)
+        assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin])), initial=0.0) <= 2e-5
