@@ -233,7 +233,7 @@ int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
                           uint32_t* out4);
 
-/* hipGraph capture of repeated evaluations (north star; no counterpart in
+/* hipGraph capture of repeated evaluations (no counterpart in
  * the reference).  Records `repeats` launches of evaluation `what` (as for
  * ehip_autotune_dev) over fixed device buffers into one graph with its own
  * register workspace; ehip_graph_launch replays it on `stream` (null: the
@@ -254,8 +254,8 @@ int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t w
                        uint32_t fetch);
 int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out3);
 
-/* Tape-specialised kernels (north star: short tapes keep their slots in
- * registers; no counterpart in the reference, whose CUDA backend generates
+/* Tape-specialised kernels (short tapes keep their slots in registers; no
+ * counterpart in the reference, whose CUDA backend generates
  * source at run time per Taylor order, src/gpu/taylor_codegen.rs, but never
  * per tape).  Writes the
  * tape's forward (what 0), value-and-gradient (what 1), gradient-and-HVP
