@@ -64,7 +64,7 @@ def parse():
 
 def cpu_baseline(tape, x: np.ndarray, sample: int, seconds: float):
     """Oracle fwd+grad on the first `sample` points, all host threads (at
-    most 16, the box's CPU share), repeated until `seconds` have passed."""
+    most 16), repeated until `seconds` have passed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -422,7 +422,7 @@ def run_dags(args) -> dict:
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(points shard by contiguous ranges across GPUs with no exchange)"}))
         return 2
     if args.workload != "mlp":

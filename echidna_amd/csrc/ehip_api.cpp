@@ -24,7 +24,7 @@ int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(EHIP_NO_DEVICE, "no suitable GPU device found");
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-    return fail(EHIP_NO_DEVICE, "no suitable GPU device found (device %d is %s; this build targets gfx950)", device,
+    return fail(EHIP_NO_DEVICE, "no suitable GPU device found (device %d is %s; the library targets gfx950)", device,
                 prop.gcnArchName);
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
   ehip_ctx* c = new ehip_ctx();
