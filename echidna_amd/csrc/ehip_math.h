@@ -72,8 +72,9 @@ template <class F> EH_DEV bool s_exact_i32(F x, int32_t* out) {
 // tanh and its derivative, the reference's 1/cosh^2 (src/opcode.rs:463-466).
 // f32: from e = exp(-2|a|) and m = expm1(-2|a|) = e - 1, tanh|a| = -m / (2 + m)
 // and 1/cosh^2 a = 4e / (1 + e)^2 (the divisions by the 1-ulp hardware
-// reciprocal of a value in [1, 2]), each a few ulp from the exact value
-// (within the reference's 16-ulp budget for tanh), with no cancellation near
+// reciprocal of a value in [1, 2]): at most 3 and 4 ulp from the correctly
+// rounded value over 2^20 arguments in [-50, 50] (the reference's budget
+// for tanh is 16 ulp), with no cancellation near
 // 0 (m, not e - 1) and no overflow (e <= 1).  About a third of the
 // instructions of the device library's tanhf and coshf, which work in
 // extended precision; the n=32 H=64 MLP fwd+grad went from 2.24 to 3.95 G

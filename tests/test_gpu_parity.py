@@ -241,3 +241,36 @@ def test_gradient_of_input_and_const_outputs(hip):
     y, g = hip.gradient_batch(bufs, np.array([[3.0, 4.0]]), 1)
     y_ref, g_ref = po.gradient(tape, np.array([[3.0, 4.0]]))
     assert y[0] == y_ref[0, 0] and list(g) == list(g_ref[0])
+
+
+def test_f32_tanh_pair_ulp_sweep(hip):
+    """The f32 scalar tanh and its partial (from exp(-2|a|) and expm1(-2|a|))
+    over 2^20 arguments in [-50, 50] and special values, against the f64
+    oracle rounded to f32: within 4 ulp for both, inside the reference's
+    16-ulp budget for tanh; both paths (interpreter and specialised kernel)."""
+    from echidna_amd.backend import TUNE_GRADIENT, HipContext
+
+    tape, _ = record(lambda v: v[0].tanh(), [0.5])
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    x = np.random.default_rng(80).uniform(-50, 50, 1 << 20)
+    x[:12] = [0.0, -0.0, 1e-30, -1e-30, 1e-7, 0.5, 20.0, 44.0, 44.5, 60.0, np.inf, -np.inf]
+    x = x.astype(np.float32)
+    y_ref, g_ref = po.gradient(tape, x.reshape(-1, 1).astype(np.float64))
+    yr, gr = y_ref[:, 0].astype(np.float32), g_ref[:, 0].astype(np.float32)
+    worst = {}
+    for spec in (False, True):
+        if spec:
+            HipContext.specialize(bufs, TUNE_GRADIENT)
+        y, g = hip.gradient_batch(bufs, x, len(x))
+        for name, got, ref in (("value", y, yr), ("derivative", g, gr)):
+            gi = np.asarray(got, np.float32).view(np.int32).astype(np.int64)
+            ri = ref.view(np.int32).astype(np.int64)
+            gi = np.where(gi < 0, -(gi & 0x7FFFFFFF), gi)
+            ri = np.where(ri < 0, -(ri & 0x7FFFFFFF), ri)
+            worst[(spec, name)] = int(np.max(np.abs(gi - ri)))
+    print("max ulp", worst)
+    assert all(w <= 4 for w in worst.values()), worst
+    # signed zeros and infinities exactly
+    y, g = hip.gradient_batch(bufs, x[:12], 12)
+    assert np.signbit(y[1]) and not np.signbit(y[0]) and y[10] == 1.0 and y[11] == -1.0
+    assert g[0] == 1.0 and g[10] == 0.0 and g[11] == 0.0
