@@ -82,10 +82,33 @@ template <class F> EH_DEV bool s_exact_i32(F x, int32_t* out) {
 // cosh^2 overflows (|a| > 44.4) the derivative is the subnormal value
 // instead of 0.  f64 keeps the library's tanh and cosh: there the exp pair
 // raised the specialised kernel's registers (512 against 284) and slowed it.
+// e = exp(y) and m = expm1(y) for y <= 0 (or NaN) from one range reduction:
+// y = k ln2 + r with |r| <= ln2 / 2 (Cody-Waite, two-part ln2), p = expm1(r)
+// by its Taylor series to r^7 (truncation below 2e-8 relative), then
+// e = 2^k (1 + p) and m = 2^k p + (2^k - 1), which has no cancellation: for
+// k = 0 it is p, for k < 0 the two terms add with the same sign.  y below
+// -200 gives e = 0, m = -1; NaN propagates.
+EH_DEV void exp_pair_f32(float y, float& e, float& m) {
+  y = (y < -200.0f) ? -200.0f : y;
+  const float kf = rintf(y * 1.44269504f);
+  float r = fmaf(-kf, 0.693145752f, y);
+  r = fmaf(-kf, 1.42860677e-6f, r);
+  float p = fmaf(r, 1.98412698e-4f, 1.38888889e-3f);
+  p = fmaf(r, p, 8.33333377e-3f);
+  p = fmaf(r, p, 4.16666679e-2f);
+  p = fmaf(r, p, 1.66666672e-1f);
+  p = fmaf(r, p, 0.5f);
+  p = r * p;
+  p = fmaf(r, p, r);  // r + r^2 (1/2 + r/6 + ..)
+  const int k = (int)kf;
+  e = ldexpf(1.0f + p, k);
+  m = ldexpf(p, k) + (ldexpf(1.0f, k) - 1.0f);
+}
 template <class F> EH_DEV F s_tanh(F a) {
   if constexpr (sizeof(F) == 4) {
     // 2 + m lies in [1, 2]: the hardware reciprocal (1 ulp) needs no scaling
-    const F m = expm1(F(-2) * fabs(a));
+    F e, m;
+    exp_pair_f32(F(-2) * fabs(a), e, m);
     return copysign(-m * __builtin_amdgcn_rcpf(F(2) + m), a);
   } else {
     return tanh(a);
@@ -93,7 +116,8 @@ template <class F> EH_DEV F s_tanh(F a) {
 }
 template <class F> EH_DEV F s_dtanh(F a) {
   if constexpr (sizeof(F) == 4) {
-    const F e = exp(F(-2) * fabs(a));
+    F e, m;
+    exp_pair_f32(F(-2) * fabs(a), e, m);
     const F r = __builtin_amdgcn_rcpf(F(1) + e);  // 1 + e in [1, 2]
     return (F(4) * e) * (r * r);
   } else {

@@ -416,13 +416,14 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, 
   int32_t rc = spec_build(c, t->host, what, waves_per_eu, arch, s);
   if (rc) return rc;
   if (waves_per_eu == 0) {
-    // Automatic: below three waves per SIMD, build again for one wave more
-    // and keep that.  On the n=32 H=64 MLP: f64 fwd+grad 1 -> 2 waves 748 ->
-    // 1,096 M points/s with 28 VGPRs spilled; f64 HVP 1 -> 2, 567 -> 744;
-    // f32 HVP 2 -> 3, 1,329 -> 1,486; the f32 gradient at 3 waves lost 5%
-    // at 4.
+    // Automatic: a build that allows one wave per SIMD is built again for
+    // two and that build is kept.  On the n=32 H=64 MLP: f64 fwd+grad 748 ->
+    // 1,096 M points/s with 28 VGPRs spilled, f64 HVP 567 -> 744.  From two
+    // waves to three the spills cost as much as the wave gains or more
+    // (f32 HVP +10% once and -3% later, f32 fwd+grad -26%), so two-wave
+    // builds stay as the compiler made them.
     const uint32_t occ = occupancy(s.vgprs);
-    if (occ < 3) {
+    if (occ < 2) {
       SpecKernel s2;
       rc = spec_build(c, t->host, what, occ + 1, arch, s2);
       if (rc) {

@@ -267,9 +267,8 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * runs every untuned evaluation of that kind; ehip_set_tuned (fetch 2, or
  * 0/1 for the interpreters) and ehip_autotune_dev choose explicitly.
  * waves_per_eu (1..8) asks the compiler for that occupancy, spilling what
- * does not fit; 0 = automatic: when the compiler's own choice allows fewer
- * than three waves per SIMD, the kernel is built again for one wave more
- * and that build is kept.  out5 = {VGPRs and AGPRs, scratch bytes per
+ * does not fit; 0 = automatic: when the compiler's own choice allows one
+ * wave per SIMD, the kernel is built again for two and that build is kept.  out5 = {VGPRs and AGPRs, scratch bytes per
  * lane, compile microseconds (all builds), code object bytes, the
  * occupancy target used (0: the compiler's)}.  Tapes longer than 65536
  * entries are refused.  The compile runs outside the context lock; building
