@@ -119,7 +119,7 @@ _SIGS = {
                                           ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32, ctypes.c_int32,
                                           ctypes.c_uint32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _c_u64p]),
     "ehip_set_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                        ctypes.c_uint32]),
+                                        ctypes.c_uint32, ctypes.c_uint32]),
     "ehip_set_fetch": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "ehip_get_tuned": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
     "ehip_tape_to_json": (ctypes.c_int32, [ctypes.c_int32, _vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32,
@@ -484,11 +484,12 @@ class HipContext:
     def autotune_dev(self, tape: "HipTapeBuffers", what: int, batch: int, d_x, d_y=0, d_g=0, d_h=0, d_dirs=0,
                      x_stride: Optional[int] = None, order: int = 0) -> dict:
         """Pick the fastest launch shape for ``what`` (TUNE_*) on these buffers."""
-        out = (ctypes.c_uint32 * 4)()
+        out = (ctypes.c_uint32 * 5)()
         xs = tape.num_inputs if x_stride is None else x_stride
         _check(load_library().ehip_autotune_dev(self.handle, tape.handle, what, order, _dp(d_x), xs, _dp(d_dirs),
                                                 batch, _dp(d_y), _dp(d_g), _dp(d_h), out))
-        return {"wg": int(out[0]), "lds_budget": int(out[1]), "us": int(out[2]), "fetch": int(out[3])}
+        return {"wg": int(out[0]), "lds_budget": int(out[1]), "us": int(out[2]), "fetch": int(out[3]),
+                "points_per_lane": int(out[4])}
 
     def graph_capture(self, tape: "HipTapeBuffers", what: int, batch: int, d_x, d_y=0, d_g=0, d_h=0, d_dirs=0,
                       x_stride: Optional[int] = None, order: int = 0, repeats: int = 1) -> "HipGraph":
@@ -523,16 +524,17 @@ class HipContext:
 
     @staticmethod
     def set_tuned(tape: "HipTapeBuffers", what: int, wg: int, lds_budget: int, order: int = 0,
-                  fetch: int = 0) -> None:
+                  fetch: int = 0, points_per_lane: int = 0) -> None:
         """Pin a launch shape; ``fetch`` 0 stages the program in LDS windows, 1 reads it through the
-        scalar cache, 2 runs the tape-specialised kernel (``specialize``)."""
-        _check(load_library().ehip_set_tuned(tape.handle, what, order, wg, lds_budget, fetch))
+        scalar cache, 2 runs the tape-specialised kernel (``specialize``), which alone takes
+        ``points_per_lane`` above 1."""
+        _check(load_library().ehip_set_tuned(tape.handle, what, order, wg, lds_budget, fetch, points_per_lane))
 
     @staticmethod
     def get_tuned(tape: "HipTapeBuffers", what: int, order: int = 0) -> dict:
-        out = (ctypes.c_uint32 * 3)()
+        out = (ctypes.c_uint32 * 4)()
         _check(load_library().ehip_get_tuned(tape.handle, what, order, out))
-        return {"wg": int(out[0]), "lds_budget": int(out[1]), "fetch": int(out[2])}
+        return {"wg": int(out[0]), "lds_budget": int(out[1]), "fetch": int(out[2]), "points_per_lane": int(out[3])}
 
     def hvp_batch_dev(self, tape, d_x, x_stride: int, d_dirs, batch: int, d_grads, d_hvps, stream: int = 0) -> None:
         _check(load_library().ehip_hvp_batch_dev(self.handle, tape.handle, _vp(d_x.data_ptr()), x_stride,

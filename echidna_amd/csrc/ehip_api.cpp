@@ -504,8 +504,8 @@ int32_t tune_launch(const ehip_tape* t, int32_t what, uint32_t order, const void
 
 int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t order, const void* d_x,
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
-                          uint32_t* out4) {
-  if (!c || !t || !d_x || !out4) return fail(EHIP_OTHER, "GPU error: null argument");
+                          uint32_t* out5) {
+  if (!c || !t || !d_x || !out5) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, batch);
   if (rc) return rc;
   Launch L;
@@ -555,10 +555,12 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
   }
   // the tape-specialised kernel, where one has been compiled (ehip_tape_specialize)
   if (rc == EHIP_OK && t->spec[tune_key(kind, K, what == 1)].fn)
-    for (uint32_t wg : {64u, 128u, 256u}) {
+    for (uint32_t wg : {64u, 128u, 256u})
+    for (uint32_t ppl : {1u, 2u, 4u}) {
       Tuned cand;
       cand.wg = wg;
       cand.fetch = 2;
+      cand.ppl = ppl;
       float ms = 1e30f;
       for (int rep = 0; rep < 3 && rc == EHIP_OK; ++rep) {
         HIP_TRY(hipEventRecord(e0, c->stream), "autotune");
@@ -576,15 +578,16 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   t->tuned[tune_key(kind, K, what == 1)] = best;
-  out4[0] = best.wg;
-  out4[1] = best.lds_budget;
-  out4[2] = (uint32_t)std::min(best_ms * 1000.f, 4.0e9f);
-  out4[3] = best.fetch;
+  out5[0] = best.wg;
+  out5[1] = best.lds_budget;
+  out5[2] = (uint32_t)std::min(best_ms * 1000.f, 4.0e9f);
+  out5[3] = best.fetch;
+  out5[4] = best.ppl ? best.ppl : 1u;
   return ok();
 }
 
 int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
-                       uint32_t fetch) {
+                       uint32_t fetch, uint32_t points_per_lane) {
   if (!t) return fail(EHIP_OTHER, "GPU error: null argument");
   if (wg != 0 && wg != 64 && wg != 128 && wg != 256)
     return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
@@ -596,21 +599,25 @@ int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, 
   else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
   if (fetch == 2 && wg && !t->spec[key].fn)
     return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
+  if (points_per_lane > 64 || (points_per_lane > 1 && fetch != 2))
+    return fail(EHIP_OTHER, "GPU error: points_per_lane must be 0 or 1, or up to 64 with the specialised kernel");
+  t->tuned[key].ppl = (wg && fetch == 2) ? points_per_lane : 0u;
   t->tuned[key].wg = wg;
   t->tuned[key].lds_budget = wg ? lds_budget : 0;
   t->tuned[key].fetch = wg ? fetch : 0;
   return ok();
 }
 
-int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_t* out3) {
-  if (!t || !out3) return fail(EHIP_OTHER, "GPU error: null argument");
+int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_t* out4) {
+  if (!t || !out4) return fail(EHIP_OTHER, "GPU error: null argument");
   int key;
   if (what >= 0 && what <= 2) key = what;
   else if (what == 3 && order >= 1 && order <= 5) key = 2 + (int)order;
   else return fail(EHIP_OTHER, "GPU error: unknown evaluation kind %d", what);
-  out3[0] = t->tuned[key].wg;
-  out3[1] = t->tuned[key].lds_budget;
-  out3[2] = t->tuned[key].fetch;
+  out4[0] = t->tuned[key].wg;
+  out4[1] = t->tuned[key].lds_budget;
+  out4[2] = t->tuned[key].fetch;
+  out4[3] = t->tuned[key].ppl ? t->tuned[key].ppl : 1u;
   return ok();
 }
 

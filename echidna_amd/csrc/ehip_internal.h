@@ -107,6 +107,7 @@ struct Tuned {
   uint32_t wg = 0;          // 0: not tuned, use the context defaults
   uint32_t lds_budget = 0;  // bytes of LDS for the register file
   uint32_t fetch = 0;       // program fetch: 0 LDS windows, 1 scalar cache, 2 tape-specialised kernel
+  uint32_t ppl = 0;         // points per lane of the tape-specialised kernel (0: one)
 };
 
 // A tape-specialised kernel (straight-line code compiled for one tape by
@@ -127,7 +128,7 @@ struct SpecKernel {
 constexpr uint32_t SPEC_MAX_ENTRIES = 1u << 16;
 
 namespace ehip_detail {
-hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStream_t st);
+hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, uint32_t ppl, hipStream_t st);
 void spec_free(SpecKernel& s);
 }  // namespace ehip_detail
 
@@ -204,7 +205,7 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   if (fm == 2 && slot >= 0) {
     const SpecKernel& sk = t->spec[slot];
     if (!sk.fn) return fail(EHIP_OTHER, "GPU error: no specialised kernel for this evaluation (ehip_tape_specialize)");
-    hipError_t e = spec_launch(sk, L, wg, s);
+    hipError_t e = spec_launch(sk, L, wg, tu.wg ? tu.ppl : 0u, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return EHIP_OK;
   }

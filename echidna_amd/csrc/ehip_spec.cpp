@@ -110,8 +110,10 @@ std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
   g.put("extern \"C\" __global__ void __launch_bounds__(256) EHIP_OCC ehip_spec(const F* __restrict__ X, "
         "unsigned long long xs, const F* __restrict__ V, F* __restrict__ Y, unsigned long long B, unsigned S, "
         "unsigned rng, unsigned long long seed) {\n");
-  g.put("const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n");
-  g.put("if (p >= B) return;\n");
+  // points per lane: a grid-stride loop over the points (the launch sizes
+  // the grid)
+  g.put("for (unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; p < B;\n"
+        "     p += (unsigned long long)gridDim.x * blockDim.x) {\n");
   g.put("const F* __restrict__ x = X + (S ? p / S : p) * xs;\n");
   g.put("auto seed_jet = [&](unsigned i) {\n  J r = ehip::jc<F, %d>(x[i]);\n", K);
   if (K > 1)
@@ -130,7 +132,7 @@ std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
   g.put("F* __restrict__ y = Y + p * %uull;\n", m * K);
   for (uint32_t j = 0; j < m; ++j)
     for (int q = 0; q < K; ++q) g.put("y[%u] = v%u.c[%d];\n", j * K + q, h.outputs[j], q);
-  g.put("}\n");
+  g.put("}\n}\n");
   return g.out;
 }
 
@@ -160,8 +162,10 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
   else
     g.put("extern \"C\" __global__ void __launch_bounds__(256) EHIP_OCC ehip_spec(const F* __restrict__ X, "
           "unsigned long long xs, F* __restrict__ Y, F* __restrict__ G, unsigned long long B) {\n");
-  g.put("const unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n");
-  g.put("if (p >= B) return;\n");
+  // points per lane: a grid-stride loop over the points (the launch sizes
+  // the grid)
+  g.put("for (unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; p < B;\n"
+        "     p += (unsigned long long)gridDim.x * blockDim.x) {\n");
   g.put("const F* __restrict__ x = X + p * xs;\n");
   if (dual) g.put("const F* __restrict__ v = V + p * %uull;\n", n);
   const char* T = dual ? "D" : "F";
@@ -291,7 +295,7 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
       }
     }
   }
-  g.put("}\n");
+  g.put("}\n}\n");
   return g.out;
 }
 
@@ -335,7 +339,7 @@ void spec_free(SpecKernel& s) {
   s = SpecKernel{};
 }
 
-hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStream_t st) {
+hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, uint32_t ppl, hipStream_t st) {
   const void* X = L.x;
   unsigned long long xs = L.xstride;
   const void* V = L.dirs;
@@ -348,7 +352,8 @@ hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, hipStr
   void* args_fwd[] = {(void*)&X, (void*)&xs, (void*)&Y, (void*)&G, (void*)&B};
   void* args_hvp[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&G, (void*)&H, (void*)&B};
   void* args_jet[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&Y, (void*)&B, (void*)&S, (void*)&rng, (void*)&seed};
-  const uint64_t grid = (L.B + wg - 1) / wg;
+  const uint64_t per = (uint64_t)wg * (ppl ? ppl : 1u);  // points per workgroup
+  const uint64_t grid = (L.B + per - 1) / per;
   if (grid == 0 || grid > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;
   return hipModuleLaunchKernel(s.fn, (uint32_t)grid, 1, 1, wg, 1, 1, 0, st,
                                s.order ? args_jet : (s.dual ? args_hvp : args_fwd), nullptr);

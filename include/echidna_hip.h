@@ -222,16 +222,18 @@ int32_t ehip_multi_batch_dev(ehip_ctx* ctx, const ehip_job* jobs, uint32_t njobs
  * shape -- workgroup size 64/128/256 x LDS budget for the register file --
  * on the caller's device buffers for evaluation `what` (0 forward, 1
  * gradient, 2 hvp, 3 taylor of `order`), keeps the fastest on the tape for
- * later calls of that kind, and returns out4 = {workgroup size, LDS budget
+ * later calls of that kind, and returns out5 = {workgroup size, LDS budget
  * bytes, microseconds per launch, program fetch (0 LDS, 1 scalar, 2 the
- * tape-specialised kernel)}.  The shapes are workgroup size 64/128/256 x LDS
- * budget x program fetch, plus the specialised kernel at each workgroup size
- * when ehip_tape_specialize has built one.  Buffers are as for the matching *_dev
+ * tape-specialised kernel), points per lane}.  The shapes are workgroup size
+ * 64/128/256 x LDS budget x program fetch, plus the specialised kernel at
+ * each workgroup size with 1, 2 or 4 points per lane (a grid-stride loop)
+ * when ehip_tape_specialize has built one; the interpreters run one point
+ * per lane.  Buffers are as for the matching *_dev
  * entry point (d_y may be null for gradient/hvp); their contents are
  * overwritten.  Runs on the context stream and synchronizes. */
 int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t order, const void* d_x,
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
-                          uint32_t* out4);
+                          uint32_t* out5);
 
 /* hipGraph capture of repeated evaluations (no counterpart in
  * the reference).  Records `repeats` launches of evaluation `what` (as for
@@ -248,11 +250,13 @@ void ehip_graph_destroy(ehip_graph* graph);
 
 /* Restore (wg != 0) or clear (wg == 0) a tuned launch shape, e.g. from a
  * persisted autotuner cache; ehip_get_tuned reads it back
- * (out3 = {wg, lds, fetch}).  fetch 2 selects the tape-specialised kernel
- * (every evaluation but kinks, once ehip_tape_specialize has built it). */
+ * (out4 = {wg, lds, fetch, points per lane}).  fetch 2 selects the
+ * tape-specialised kernel (every evaluation but kinks, once
+ * ehip_tape_specialize has built it), which alone takes points_per_lane
+ * above 1 (up to 64; 0 means 1). */
 int32_t ehip_set_tuned(ehip_tape* tape, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
-                       uint32_t fetch);
-int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out3);
+                       uint32_t fetch, uint32_t points_per_lane);
+int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint32_t* out4);
 
 /* Tape-specialised kernels (short tapes keep their slots in registers; no
  * counterpart in the reference, whose CUDA backend generates

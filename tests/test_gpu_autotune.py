@@ -56,7 +56,7 @@ def test_autotune_keeps_choice_and_results(hip):
     res = hip.autotune_dev(bufs, TUNE_GRADIENT, B, x, d_y=y, d_g=g)
     assert res["wg"] in (64, 128, 256) and res["us"] > 0 and res["fetch"] in (0, 1)
     assert HipContext.get_tuned(bufs, TUNE_GRADIENT) == {"wg": res["wg"], "lds_budget": res["lds_budget"],
-                                                         "fetch": res["fetch"]}
+                                                         "fetch": res["fetch"], "points_per_lane": 1}
     y1, g1 = hip.gradient_batch(bufs, x.cpu().numpy().reshape(-1), B)
     assert np.array_equal(y0, y1) and np.array_equal(g0, g1)
     HipContext.set_tuned(bufs, TUNE_GRADIENT, 0, 0)

@@ -359,3 +359,36 @@ def test_f32_dual_tanh_guard_keeps_reference_nan_pattern(hip, spec):
         assert np.array_equal(np.isinf(got), np.isinf(ref))
         fin = np.isfinite(ref)
         assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin])), initial=0.0) <= 2e-5
+
+
+@pytest.mark.parametrize("ppl", [2, 4, 7])
+def test_points_per_lane_keeps_results(hip, ppl):
+    """The specialised kernels loop over several points per lane (grid
+    stride): the same values, gradients, HVPs and jets, also for a batch that
+    is not a multiple of the workgroup size times the points per lane."""
+    from echidna_amd import philox
+    from echidna_amd.backend import TUNE_HVP
+
+    tape = wl.tanh_mlp(10, 12, 71)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    B = 1000
+    x = wl.points_normal(72, B, 10).astype(np.float32)
+    v = philox.normal(73, philox.STREAM_DIRS, np.arange(B * 10)).reshape(B, 10).astype(np.float32)
+
+    def run():
+        y, g = hip.gradient_batch(bufs, x.reshape(-1), B)
+        gh, hv = hip.hvp_batch_points(bufs, x.reshape(-1), v.reshape(-1), B)
+        tk = hip.taylor_forward_kth_batch(bufs, x.reshape(-1), v.reshape(-1), B, 3)
+        return [np.asarray(a).copy() for a in (y, g, gh, hv, *tk.coefficients)]
+
+    for what in (TUNE_GRADIENT, TUNE_HVP, 5):
+        HipContext.specialize(bufs, what)
+    ref = run()
+    HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=2, points_per_lane=ppl)
+    HipContext.set_tuned(bufs, TUNE_HVP, 128, 0, fetch=2, points_per_lane=ppl)
+    HipContext.set_tuned(bufs, TUNE_TAYLOR, 64, 0, order=3, fetch=2, points_per_lane=ppl)
+    assert HipContext.get_tuned(bufs, TUNE_GRADIENT)["points_per_lane"] == ppl
+    for a, b in zip(ref, run()):
+        assert np.array_equal(a, b)
+    with pytest.raises(GpuError, match="points_per_lane"):
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 64, 0, fetch=1, points_per_lane=2)
