@@ -76,6 +76,15 @@ struct Regs {
       for (int c = 0; c < NC; ++c) o[c] = gl[(base + c) * gstride + p];
     }
   }
+  // component C only
+  template <int C>
+  __device__ __forceinline__ F rd_c(uint32_t r) const {
+    static_assert(C < NC, "component");
+    if (LDSONLY) return lane[(r * NC + C) << lsh];
+    r = uni(r);
+    if (r < R_lds) return lane[(r * NC + C) << lsh];
+    return gl[((uint64_t)(r - R_lds) * NC + C) * gstride + p];
+  }
   // the first N components only
   template <int N>
   __device__ __forceinline__ void rd_n(uint32_t r, F* o) const {
@@ -315,9 +324,13 @@ __device__ __forceinline__ void dense_fwd_block(const Regs_& R, AS4 const uint32
 // once a non-finite operand went through the full product, and every
 // further term adds +0 to it, which changes neither; so those accumulators
 // are set once and only the full product touches them.
+// XR (optional, with NZ < NC): the wave's common input row; operand
+// values then come from XR[IX[p]] by scalar loads, and only the first-order
+// coefficients from the register file.
 template <class F, int NC, int JB, bool FULL, int NZ, class Regs_>
 __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
-                                                uint32_t T, uint32_t j0, uint32_t jb) {
+                                                uint32_t T, uint32_t j0, uint32_t jb, AS4 const F* XR = nullptr,
+                                                AS4 const uint32_t* IX = nullptr) {
   static_assert(NZ >= 1 && NZ <= NC, "coefficients");
   F acc[JB][NC];
   // true once a product term has entered the sums: a Taylor product's
@@ -336,7 +349,16 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
     }
     if (kind == T_PROD) {
       F x[NC];
-      R.template rd_n<NZ>(kr & P_REG_MASK, x);
+      if constexpr (NZ < NC && NZ >= 2) {
+        if (XR) {
+          x[0] = XR[IX[p]];
+          x[1] = R.template rd_c<1>(kr & P_REG_MASK);
+        } else {
+          R.template rd_n<NZ>(kr & P_REG_MASK, x);
+        }
+      } else {
+        R.template rd_n<NZ>(kr & P_REG_MASK, x);
+      }
 #pragma unroll
       for (int q = NZ; q < NC; ++q) x[q] = F(0);
       const bool cleft = (kr & P_CLEFT) != 0;
@@ -386,7 +408,16 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
       }
     } else if (kind == T_REG) {
       F x[NC];
-      R.template rd_n<NZ>(kr & P_REG_MASK, x);
+      if constexpr (NZ < NC && NZ >= 2) {
+        if (XR) {
+          x[0] = XR[IX[p]];
+          x[1] = R.template rd_c<1>(kr & P_REG_MASK);
+        } else {
+          R.template rd_n<NZ>(kr & P_REG_MASK, x);
+        }
+      } else {
+        R.template rd_n<NZ>(kr & P_REG_MASK, x);
+      }
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
@@ -415,15 +446,19 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
 
 template <class F, int NC, int JB, bool JET, class Regs_>
 __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, uint32_t m, uint32_t T,
-                                          uint32_t off_regs, uint32_t off_w, uint32_t dflags = 0) {
+                                          uint32_t off_regs, uint32_t off_w, uint32_t dflags = 0,
+                                          uint32_t off_ix = 0, AS4 const F* XR = nullptr) {
   uint32_t j0 = 0;
   if constexpr (JET) {
     AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
     AS4 const F* PW = reinterpret_cast<AS4 const F*>((AS4 const uint32_t*)pool + off_w);
     constexpr int NZI = NC >= 3 ? 2 : NC;  // input operands: (x, v, 0, ..)
     if (NZI < NC && (dflags & D_INPUT_OPERANDS)) {
-      for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NZI>(R, PR, PW, m, T, j0, JB);
-      if (j0 < m) dense_jet_block<F, NC, JB, false, NZI>(R, PR, PW, m, T, j0, m - j0);
+      // operand values from the wave's common input row when it has one
+      AS4 const uint32_t* IX = (AS4 const uint32_t*)pool + off_ix;
+      AS4 const F* xr = XR;
+      for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NZI>(R, PR, PW, m, T, j0, JB, xr, IX);
+      if (j0 < m) dense_jet_block<F, NC, JB, false, NZI>(R, PR, PW, m, T, j0, m - j0, xr, IX);
       return;
     }
     for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NC>(R, PR, PW, m, T, j0, JB);
@@ -943,6 +978,15 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
   const uint64_t p = R.p;
   const bool act = p < L.B;
   const F* X = reinterpret_cast<const F*>(L.x);
+  // The wave's common input row, when every lane reads the same one: x
+  // shared by all points, or S lanes per point with S a multiple of 64 (a
+  // wave is then one point's samples) in a wave inside the batch.  DENSE
+  // over inputs then reads operand values from it by scalar loads.
+  const uint64_t p0 = p - (threadIdx.x & 63u);
+  const bool one_row = uni((L.xstride == 0 || (L.S >= 64 && (L.S & 63u) == 0 && p0 < L.B)) ? 1u : 0u) != 0;
+  const uint64_t roff = (one_row && L.xstride) ? (L.S ? p0 / L.S : p0) * L.xstride : 0;
+  const uint64_t roff_u = ((uint64_t)uni((uint32_t)(roff >> 32)) << 32) | uni((uint32_t)roff);
+  AS4 const F* XR = one_row ? (AS4 const F*)L.x + roff_u : nullptr;
   const F* V = reinterpret_cast<const F*>(L.dirs);
   F* Y = reinterpret_cast<F*>(L.y);
   run_stream(L, [&](auto win, uint32_t pos, uint32_t w0) {
@@ -997,7 +1041,7 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
       R.wr(dst, acc.c);
     } else if (op == I_DENSE) {
       dense_fwd<F, K, (K <= 2 ? 16 : (K == 3 ? jet3_block<F>() : 4)), true>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4),
-                                                                          I.u(5));
+                                                                          I.u(5), I.u(6), XR);
     } else if (op == I_LOADN) {
       load_inputs<F, K, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
 #pragma unroll

@@ -912,6 +912,12 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
             if (t.f64) out.pool.push_back(hi);
           }
         w[5] = (prod_nz ? D_ALL_PROD_NZ : 0u) | (in_ops ? D_INPUT_OPERANDS : 0u);
+        if (in_ops) {
+          // the operands' input indices, for interpreters that read an
+          // operand's value straight from the input row
+          w[6] = (uint32_t)out.pool.size();
+          for (auto& tm : x.terms) out.pool.push_back(tm.kind == T_IMM ? 0u : (uint32_t)tm.vid);
+        }
         break;
       }
       default:

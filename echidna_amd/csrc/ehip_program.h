@@ -123,7 +123,8 @@ constexpr uint32_t P_DEEP = 1u << 28;
 constexpr uint32_t D_ALL_PROD_NZ = 1u;
 // DENSE w5: every operand is a tape input and every weight is finite (in
 // the program's dtype).  Jet interpreters seed inputs as (x, v, 0, ..), so
-// the operands' coefficients of order >= 2 are +0 there.
+// the operands' coefficients of order >= 2 are +0 there.  w6 then holds the
+// pool offset of the T operands' input indices.
 constexpr uint32_t D_INPUT_OPERANDS = 2u;
 
 struct Program {
