@@ -332,35 +332,55 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
                                                 uint32_t T, uint32_t j0, uint32_t jb, AS4 const F* XR = nullptr,
                                                 AS4 const uint32_t* IX = nullptr) {
   static_assert(NZ >= 1 && NZ <= NC, "coefficients");
+  static_assert(NZ == NC || NZ == 2, "input operands carry (x, v)");
   F acc[JB][NC];
   // true once a product term has entered the sums: a Taylor product's
   // coefficients are sums that start at +0, so from then on no accumulator
   // can be -0 (x + y is -0 only when both are), and acc + (0 + t) == acc + t
   bool absorbed = false;
-  for (uint32_t p = 0; p < T; ++p) {
-    const uint32_t kr = PR[p];
-    const uint32_t kind = kr >> P_KIND_SHIFT;
-    AS4 const F* w = PW + (uint64_t)p * m + j0;
-    if (p == 0 && NZ < NC) {
+  // A term's operand coefficients are read one term ahead of its arithmetic:
+  // the register-file load (HBM past the LDS budget) and the scalar loads of
+  // its input-row value then overlap the previous term's products instead of
+  // stalling the wave once per term.  Immediate terms read nothing.
+  auto fetch = [&](uint32_t p, uint32_t kr, F* x) {
+    if ((kr >> P_KIND_SHIFT) == T_IMM) {
 #pragma unroll
-      for (int j = 0; j < JB; ++j)
-#pragma unroll
-        for (int q = NZ; q < NC; ++q) acc[j][q] = F(0);
+      for (int q = 0; q < NZ; ++q) x[q] = F(0);
+      return;
     }
-    if (kind == T_PROD) {
-      F x[NC];
-      if constexpr (NZ < NC && NZ >= 2) {
-        if (XR) {
-          x[0] = XR[IX[p]];
-          x[1] = R.template rd_c<1>(kr & P_REG_MASK);
-        } else {
-          R.template rd_n<NZ>(kr & P_REG_MASK, x);
-        }
-      } else {
-        R.template rd_n<NZ>(kr & P_REG_MASK, x);
+    if constexpr (NZ < NC) {
+      if (XR) {
+        x[0] = XR[IX[p]];
+        x[1] = R.template rd_c<1>(kr & P_REG_MASK);
+        return;
       }
+    }
+    R.template rd_n<NZ>(kr & P_REG_MASK, x);
+  };
+  uint32_t knext = 0;
+  F xn[NZ];
+  if (T > 0) {
+    knext = PR[0];
+    fetch(0, knext, xn);
+  }
+  if (NZ < NC) {
 #pragma unroll
-      for (int q = NZ; q < NC; ++q) x[q] = F(0);
+    for (int j = 0; j < JB; ++j)
+#pragma unroll
+      for (int q = NZ; q < NC; ++q) acc[j][q] = F(0);
+  }
+  for (uint32_t p = 0; p < T; ++p) {
+    const uint32_t kr = knext;
+    const uint32_t kind = kr >> P_KIND_SHIFT;
+    F x[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) x[q] = q < NZ ? xn[q] : F(0);
+    if (p + 1 < T) {
+      knext = PR[p + 1];
+      fetch(p + 1, knext, xn);
+    }
+    AS4 const F* w = PW + (uint64_t)p * m + j0;
+    if (kind == T_PROD) {
       const bool cleft = (kr & P_CLEFT) != 0;
       bool fin = true;
 #pragma unroll
@@ -390,34 +410,22 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
             }
           }
         }
-        absorbed = true;
-        continue;
+      } else {
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+          if (!(FULL || (uint32_t)j < jb)) continue;
+          const F c = w[j];
+          F term[NC], cj[NC];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) cj[q] = (q == 0) ? c : F(0);
+          if (cleft) jt::mul<F, NC>(cj, x, term);
+          else jt::mul<F, NC>(x, cj, term);
+#pragma unroll
+          for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
+        }
       }
       absorbed = true;
-#pragma unroll
-      for (int j = 0; j < JB; ++j) {
-        if (!(FULL || (uint32_t)j < jb)) continue;
-        const F c = w[j];
-        F term[NC], cj[NC];
-#pragma unroll
-        for (int q = 0; q < NC; ++q) cj[q] = (q == 0) ? c : F(0);
-        if (cleft) jt::mul<F, NC>(cj, x, term);
-        else jt::mul<F, NC>(x, cj, term);
-#pragma unroll
-        for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
-      }
     } else if (kind == T_REG) {
-      F x[NC];
-      if constexpr (NZ < NC && NZ >= 2) {
-        if (XR) {
-          x[0] = XR[IX[p]];
-          x[1] = R.template rd_c<1>(kr & P_REG_MASK);
-        } else {
-          R.template rd_n<NZ>(kr & P_REG_MASK, x);
-        }
-      } else {
-        R.template rd_n<NZ>(kr & P_REG_MASK, x);
-      }
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
