@@ -277,12 +277,17 @@ __device__ __forceinline__ void load_inputs(const Regs<F, NC, LDSONLY>& R, P ins
 // plain register term, and an immediate term uses x = 1; x * 1 and 1 * c are
 // exact, so each member sees the bits its DOT would produce, and the
 // accumulators stay in place across terms.
-template <class F, int JB, bool FULL, bool FIRST, class Regs_>
-__device__ __forceinline__ void dense_fwd_term(const Regs_& R, uint32_t kr, AS4 const F* PW, uint32_t m, uint32_t p,
-                                               uint32_t j0, uint32_t jb, F (&acc)[JB]) {
-  AS4 const F* w = PW + (uint64_t)p * m + j0;
+template <class F, class Regs_>
+__device__ __forceinline__ F dense_operand(const Regs_& R, uint32_t kr) {
   F x = F(1);
   if ((kr >> P_KIND_SHIFT) != T_IMM) R.rd(kr & P_REG_MASK, &x);
+  return x;
+}
+
+template <class F, int JB, bool FULL, bool FIRST>
+__device__ __forceinline__ void dense_fwd_term(F x, AS4 const F* PW, uint32_t m, uint32_t p, uint32_t j0, uint32_t jb,
+                                               F (&acc)[JB]) {
+  AS4 const F* w = PW + (uint64_t)p * m + j0;
 #pragma unroll
   for (int j = 0; j < JB; ++j) {
     if (!(FULL || (uint32_t)j < jb)) continue;
@@ -292,18 +297,20 @@ __device__ __forceinline__ void dense_fwd_term(const Regs_& R, uint32_t kr, AS4 
 }
 
 // The first term peeled, so the loop body is a plain acc + term; each
-// term's operand word is loaded one term ahead.
+// term's operand is read one term ahead of its products, so that its
+// register-file load overlaps the previous term's arithmetic.  A DENSE has
+// at least one term.
 template <class F, int JB, bool FULL, class Regs_>
 __device__ __forceinline__ void dense_fwd_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
                                                 uint32_t T, uint32_t j0, uint32_t jb) {
   F acc[JB];
-  uint32_t kr = PR[0];
-  uint32_t next = PR[T > 1 ? 1 : 0];
-  dense_fwd_term<F, JB, FULL, true>(R, kr, PW, m, 0, j0, jb, acc);
+  F x = dense_operand<F>(R, PR[0]);
+  F xn = T > 1 ? dense_operand<F>(R, PR[1]) : F(0);
+  dense_fwd_term<F, JB, FULL, true>(x, PW, m, 0, j0, jb, acc);
   for (uint32_t p = 1; p < T; ++p) {
-    kr = next;
-    next = PR[p + 1 < T ? p + 1 : p];
-    dense_fwd_term<F, JB, FULL, false>(R, kr, PW, m, p, j0, jb, acc);
+    x = xn;
+    if (p + 1 < T) xn = dense_operand<F>(R, PR[p + 1]);
+    dense_fwd_term<F, JB, FULL, false>(x, PW, m, p, j0, jb, acc);
   }
 #pragma unroll
   for (int j = 0; j < JB; ++j) {
@@ -499,11 +506,12 @@ __device__ __forceinline__ void dense_rev_block(const Regs<F, 1, LDSONLY>& R, AS
     const uint32_t kr = PR[i0 + i];
     if (!(kr & P_FIRST)) R.rd(kr & P_REG_MASK, &cur[i]);
   }
-  uint32_t next = PR[T];
+  // each member's adjoint is read one member ahead of its contributions
+  F gn;
+  R.rd(PR[T], &gn);
   for (uint32_t j = 0; j < m; ++j) {
-    F g;
-    R.rd(next, &g);
-    next = PR[T + (j + 1 < m ? j + 1 : j)];
+    const F g = gn;
+    if (j + 1 < m) R.rd(PR[T + j + 1], &gn);
     AS4 const F* w = PW + (uint64_t)j * T + i0;
     const bool live = g != F(0);
     if (fastw && __all(live)) {
@@ -769,15 +777,30 @@ template <class F, int JB, bool FULL, bool LDSONLY>
 __device__ __forceinline__ void dense_dual_block(const Regs<F, 2, LDSONLY>& R, AS4 const uint32_t* PR,
                                                  AS4 const F* PW, uint32_t m, uint32_t T, uint32_t j0, uint32_t jb) {
   Dual<F> acc[JB];
+  // a term's operand is read one term ahead of its products
+  auto fetch = [&](uint32_t kr, F* t) {
+    if ((kr >> P_KIND_SHIFT) != T_IMM) {
+      R.rd(kr & P_REG_MASK, t);
+    } else {
+      t[0] = F(0);
+      t[1] = F(0);
+    }
+  };
+  uint32_t knext = 0;
+  F tn[2] = {F(0), F(0)};
+  if (T > 0) {
+    knext = PR[0];
+    fetch(knext, tn);
+  }
   for (uint32_t p = 0; p < T; ++p) {
-    const uint32_t kr = PR[p];
+    const uint32_t kr = knext;
     const uint32_t kind = kr >> P_KIND_SHIFT;
     AS4 const F* w = PW + (uint64_t)p * m + j0;
     Dual<F> x = dc(F(0));
-    if (kind != T_IMM) {
-      F t[2];
-      R.rd(kr & P_REG_MASK, t);
-      x = Dual<F>{t[0], t[1]};
+    if (kind != T_IMM) x = Dual<F>{tn[0], tn[1]};
+    if (p + 1 < T) {
+      knext = PR[p + 1];
+      fetch(knext, tn);
     }
     const bool cleft = (kr & P_CLEFT) != 0;
 #pragma unroll
@@ -825,10 +848,12 @@ __device__ __forceinline__ void dense_dual_rev_block(const Regs<F, 2, LDSONLY>& 
       cur[i] = Dual<F>{t[0], t[1]};
     }
   }
+  // each member's adjoint is read one member ahead of its contributions
+  F tn[2];
+  R.rd(PR[T], tn);
   for (uint32_t j = 0; j < m; ++j) {
-    F t[2];
-    R.rd(PR[T + j], t);
-    const Dual<F> g{t[0], t[1]};
+    const Dual<F> g{tn[0], tn[1]};
+    if (j + 1 < m) R.rd(PR[T + j + 1], tn);
     const bool live = !d_zero(g);
     const Dual<F> n1 = dual_norm(g);
     AS4 const F* w = PW + (uint64_t)j * T + i0;
