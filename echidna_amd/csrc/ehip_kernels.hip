@@ -331,16 +331,27 @@ __device__ __forceinline__ void dense_fwd_block(const Regs_& R, AS4 const uint32
 // once a non-finite operand went through the full product, and every
 // further term adds +0 to it, which changes neither; so those accumulators
 // are set once and only the full product touches them.
-// XR (optional, with NZ < NC): the wave's common input row; operand
-// values then come from XR[IX[p]] by scalar loads, and only the first-order
-// coefficients from the register file.
-template <class F, int NC, int JB, bool FULL, int NZ, class Regs_>
+// ROW (with NZ < NC): XR is the wave's common input row; operand values
+// then come from XR[IX[p]] by scalar loads, and only the first-order
+// coefficients from the register file.  The members' value coefficients are
+// then the same in every lane, so lane l computes only member l mod JB's,
+// term by term with the operations and the order every lane would use, and
+// the JB results are broadcast by shuffles at the end of the block.
+template <class F, int NC, int JB, bool FULL, int NZ, bool ROW, class Regs_>
 __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32_t* PR, AS4 const F* PW, uint32_t m,
                                                 uint32_t T, uint32_t j0, uint32_t jb, AS4 const F* XR = nullptr,
                                                 AS4 const uint32_t* IX = nullptr) {
   static_assert(NZ >= 1 && NZ <= NC, "coefficients");
   static_assert(NZ == NC || NZ == 2, "input operands carry (x, v)");
+  static_assert(!ROW || NZ < NC, "a common row holds input operands only");
+  static_assert((JB & (JB - 1)) == 0 && JB <= 64, "members per lane group");
+  constexpr int Q0 = ROW ? 1 : 0;  // first coefficient kept per member
   F acc[JB][NC];
+  // ROW: this lane's member, its weight for the current and the next term,
+  // and its value-coefficient accumulator
+  const uint32_t lane_m = (uint32_t)(threadIdx.x & (JB - 1));
+  const uint32_t mj = (FULL || lane_m < jb) ? lane_m : jb - 1;
+  F cl = F(0), cln = F(0), a0 = F(0);
   // true once a product term has entered the sums: a Taylor product's
   // coefficients are sums that start at +0, so from then on no accumulator
   // can be -0 (x + y is -0 only when both are), and acc + (0 + t) == acc + t
@@ -355,20 +366,19 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
       for (int q = 0; q < NZ; ++q) x[q] = F(0);
       return;
     }
-    if constexpr (NZ < NC) {
-      if (XR) {
-        x[0] = XR[IX[p]];
-        x[1] = R.template rd_c<1>(kr & P_REG_MASK);
-        return;
-      }
+    if constexpr (ROW) {
+      x[0] = XR[IX[p]];
+      x[1] = R.template rd_c<1>(kr & P_REG_MASK);
+    } else {
+      R.template rd_n<NZ>(kr & P_REG_MASK, x);
     }
-    R.template rd_n<NZ>(kr & P_REG_MASK, x);
   };
   uint32_t knext = 0;
   F xn[NZ];
   if (T > 0) {
     knext = PR[0];
     fetch(0, knext, xn);
+    if constexpr (ROW) cln = PW[j0 + mj];
   }
   if (NZ < NC) {
 #pragma unroll
@@ -382,9 +392,11 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
     F x[NC];
 #pragma unroll
     for (int q = 0; q < NC; ++q) x[q] = q < NZ ? xn[q] : F(0);
+    if constexpr (ROW) cl = cln;
     if (p + 1 < T) {
       knext = PR[p + 1];
       fetch(p + 1, knext, xn);
+      if constexpr (ROW) cln = PW[(uint64_t)(p + 1) * m + j0 + mj];
     }
     AS4 const F* w = PW + (uint64_t)p * m + j0;
     if (kind == T_PROD) {
@@ -398,26 +410,36 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
         // products are signed zeros, and a sum that starts at +0 absorbs
         // them (either operand order).
         if (absorbed) {
+          if constexpr (ROW) a0 = a0 + x[0] * cl;
 #pragma unroll
           for (int j = 0; j < JB; ++j) {
             if (!(FULL || (uint32_t)j < jb)) continue;
             const F c = w[j];
 #pragma unroll
-            for (int q = 0; q < NZ; ++q) acc[j][q] = acc[j][q] + x[q] * c;
+            for (int q = Q0; q < NZ; ++q) acc[j][q] = acc[j][q] + x[q] * c;
           }
         } else {
+          if constexpr (ROW) {
+            const F term = F(0) + x[0] * cl;
+            a0 = (p == 0) ? term : a0 + term;
+          }
 #pragma unroll
           for (int j = 0; j < JB; ++j) {
             if (!(FULL || (uint32_t)j < jb)) continue;
             const F c = w[j];
 #pragma unroll
-            for (int q = 0; q < NZ; ++q) {
+            for (int q = Q0; q < NZ; ++q) {
               const F term = F(0) + x[q] * c;
               acc[j][q] = (p == 0) ? term : acc[j][q] + term;
             }
           }
         }
       } else {
+        if constexpr (ROW) {
+          // the value coefficient of the full Taylor product: 0 + a0 * b0
+          const F term = cleft ? F(0) + cl * x[0] : F(0) + x[0] * cl;
+          a0 = (p == 0) ? term : a0 + term;
+        }
 #pragma unroll
         for (int j = 0; j < JB; ++j) {
           if (!(FULL || (uint32_t)j < jb)) continue;
@@ -428,29 +450,37 @@ __device__ __forceinline__ void dense_jet_block(const Regs_& R, AS4 const uint32
           if (cleft) jt::mul<F, NC>(cj, x, term);
           else jt::mul<F, NC>(x, cj, term);
 #pragma unroll
-          for (int q = 0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
+          for (int q = Q0; q < NC; ++q) acc[j][q] = (p == 0) ? term[q] : acc[j][q] + term[q];
         }
       }
       absorbed = true;
     } else if (kind == T_REG) {
+      if constexpr (ROW) a0 = (p == 0) ? x[0] : a0 + x[0];
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
 #pragma unroll
-        for (int q = 0; q < NZ; ++q) acc[j][q] = (p == 0) ? x[q] : acc[j][q] + x[q];
+        for (int q = Q0; q < NZ; ++q) acc[j][q] = (p == 0) ? x[q] : acc[j][q] + x[q];
       }
     } else {
+      if constexpr (ROW) a0 = (p == 0) ? cl : a0 + cl;
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         if (!(FULL || (uint32_t)j < jb)) continue;
         const F c = w[j];
 #pragma unroll
-        for (int q = 0; q < NZ; ++q) {
+        for (int q = Q0; q < NZ; ++q) {
           const F term = (q == 0) ? c : F(0);
           acc[j][q] = (p == 0) ? term : acc[j][q] + term;
         }
       }
     }
+  }
+  if constexpr (ROW) {
+    // member j's value coefficient from lane j (every lane of the wave
+    // runs this block: the row is common to the whole wave)
+#pragma unroll
+    for (int j = 0; j < JB; ++j) acc[j][0] = __shfl(a0, j, 64);
   }
 #pragma unroll
   for (int j = 0; j < JB; ++j) {
@@ -468,16 +498,22 @@ __device__ __forceinline__ void dense_fwd(const Regs_& R, const uint32_t* pool, 
     AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;
     AS4 const F* PW = reinterpret_cast<AS4 const F*>((AS4 const uint32_t*)pool + off_w);
     constexpr int NZI = NC >= 3 ? 2 : NC;  // input operands: (x, v, 0, ..)
-    if (NZI < NC && (dflags & D_INPUT_OPERANDS)) {
-      // operand values from the wave's common input row when it has one
-      AS4 const uint32_t* IX = (AS4 const uint32_t*)pool + off_ix;
-      AS4 const F* xr = XR;
-      for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NZI>(R, PR, PW, m, T, j0, JB, xr, IX);
-      if (j0 < m) dense_jet_block<F, NC, JB, false, NZI>(R, PR, PW, m, T, j0, m - j0, xr, IX);
-      return;
+    if constexpr (NZI < NC) {
+      if (dflags & D_INPUT_OPERANDS) {
+        // operand values from the wave's common input row when it has one
+        AS4 const uint32_t* IX = (AS4 const uint32_t*)pool + off_ix;
+        if (XR) {
+          for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NZI, true>(R, PR, PW, m, T, j0, JB, XR, IX);
+          if (j0 < m) dense_jet_block<F, NC, JB, false, NZI, true>(R, PR, PW, m, T, j0, m - j0, XR, IX);
+          return;
+        }
+        for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NZI, false>(R, PR, PW, m, T, j0, JB);
+        if (j0 < m) dense_jet_block<F, NC, JB, false, NZI, false>(R, PR, PW, m, T, j0, m - j0);
+        return;
+      }
     }
-    for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NC>(R, PR, PW, m, T, j0, JB);
-    if (j0 < m) dense_jet_block<F, NC, JB, false, NC>(R, PR, PW, m, T, j0, m - j0);
+    for (; j0 + JB <= m; j0 += JB) dense_jet_block<F, NC, JB, true, NC, false>(R, PR, PW, m, T, j0, JB);
+    if (j0 < m) dense_jet_block<F, NC, JB, false, NC, false>(R, PR, PW, m, T, j0, m - j0);
   } else {
     static_assert(NC == 1, "value programs carry one component");
     AS4 const uint32_t* PR = (AS4 const uint32_t*)pool + off_regs;

@@ -240,3 +240,56 @@ def test_dense_jets_over_inputs_with_nonfinite_points(hip, f64):
         fin = np.isfinite(ref)
         assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
         assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin]))) <= 1e-12
+
+
+@pytest.mark.parametrize("f64", [False, True])
+@pytest.mark.parametrize("row", ["finite", "nonfinite"])
+def test_dense_jets_over_a_shared_input_row(hip, f64, row):
+    """Order-3 jets with one input row shared by every lane (x_stride 0, as
+    the single-point STDE entry points and S = 64 batches run them): the
+    DENSE groups over inputs then take operand values from that row, and
+    each lane accumulates one member's value coefficient for the whole wave.
+    Bit-identical to the ungrouped program (NaN signs aside) for a finite
+    row and for a row holding Inf and NaN, with infinite and NaN directions
+    in single lanes and in a whole wave, a partial last wave, and the layer
+    split into full and partial blocks of members; f64 also against the
+    oracle."""
+    import pyoracle as po
+
+    n = 20
+    tape = wl.tanh_mlp2(n, 37, 18, 41)
+    F = np.float64 if f64 else np.float32
+    B = 200  # three full waves and a partial one
+    x = wl.points_normal(19, 1, n)[0]
+    if row == "nonfinite":
+        x[4] = np.inf
+        x[11] = np.nan
+    v = philox.normal(20, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n)
+    v[5, 3] = np.inf
+    v[77, 0] = np.nan
+    v[128:192, 9] = -np.inf  # a whole wave
+    xd = torch.tensor(x.astype(F), device="cuda")
+    vd = torch.tensor(v.astype(F), device="cuda")
+    outs = []
+    for dense in (True, False):
+        hip.set_dense(dense)
+        try:
+            bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+        finally:
+            hip.set_dense(True)
+        for fetch in (0, 1):
+            HipContext.set_tuned(bufs, TUNE_TAYLOR, 64, 4096, order=3, fetch=fetch)
+            c = torch.zeros(B * 3, dtype=xd.dtype, device="cuda")
+            hip.taylor_kth_batch_dev(bufs, xd, 0, vd, B, 3, c)
+            torch.cuda.synchronize()
+            outs.append(c.cpu().numpy().reshape(B, 3))
+    for o in outs[1:]:
+        assert np.array_equal(_bits(o, f64), _bits(outs[0], f64))
+    if f64:
+        with np.errstate(all="ignore"):
+            ref = po.taylor(tape, np.tile(x, (B, 1)), v, K=3)[:, 0, :]
+        got = outs[0]
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        fin = np.isfinite(ref)
+        assert np.array_equal(got[~fin & ~np.isnan(ref)], ref[~fin & ~np.isnan(ref)])
+        assert np.max(np.abs(got[fin] - ref[fin]) / np.maximum(1.0, np.abs(ref[fin])), initial=0.0) <= 1e-12
