@@ -115,6 +115,7 @@ _SIGS = {
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
     "ehip_tape_specialize": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
     "ehip_set_specialize": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
+    "ehip_spec_error": (ctypes.c_char_p, [_vp, ctypes.c_int32]),
     "ehip_spec_source": (ctypes.c_int32, [ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _c_f64p, ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32, ctypes.c_int32,
                                           ctypes.c_uint32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _c_u64p]),
@@ -290,6 +291,10 @@ class HipTapeBuffers:
         _check(load_library().ehip_program_info(self.handle, mode, out))
         return {"instructions": int(out[0]), "registers": int(out[1]), "words": int(out[2]),
                 "dot_terms": int(out[3])}
+
+    def spec_error(self, what: int) -> str:
+        """Why the upload-time specialisation of ``what`` failed ("" if it did not)."""
+        return load_library().ehip_spec_error(self.handle, what).decode()
 
     def free(self) -> None:
         if self.handle:

@@ -15,7 +15,7 @@ extern "C" {
 const char* ehip_last_error(void) { return g_last_error.c_str(); }
 const char* ehip_version(void) { return "echidna-hip 0.1 (gfx950)"; }
 
-int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) {
+int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) try {
   if (!out) return fail(EHIP_OTHER, "GPU error: null output pointer");
   *out = nullptr;
   int n = 0;
@@ -38,9 +38,11 @@ int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) {
   }
   *out = c;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-void ehip_ctx_destroy(ehip_ctx* c) {
+void ehip_ctx_destroy(ehip_ctx* c) try {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) {
@@ -57,9 +59,11 @@ void ehip_ctx_destroy(ehip_ctx* c) {
   c->cvl.release();
   for (auto& b : c->io) b.release();
   delete c;
+} catch (...) {
+  (void)exception_status();
 }
 
-int32_t ehip_set_launch(ehip_ctx* c, uint32_t wg, uint32_t lds_budget) {
+int32_t ehip_set_launch(ehip_ctx* c, uint32_t wg, uint32_t lds_budget) try {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
   if (wg == 0) wg = 256;
   if (wg != 64 && wg != 128 && wg != 256) return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
@@ -67,34 +71,42 @@ int32_t ehip_set_launch(ehip_ctx* c, uint32_t wg, uint32_t lds_budget) {
   c->wg = wg;
   c->lds_budget = lds_budget ? lds_budget : 24576;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_set_fetch(ehip_ctx* c, int32_t mode) {
+int32_t ehip_set_fetch(ehip_ctx* c, int32_t mode) try {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
   if (mode != 0 && mode != 1) return fail(EHIP_OTHER, "GPU error: program fetch must be 0 (LDS) or 1 (scalar)");
   std::lock_guard<std::mutex> lk(c->mu);
   c->fetch = (uint32_t)mode;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_set_dense(ehip_ctx* c, int32_t enable) {
+int32_t ehip_set_dense(ehip_ctx* c, int32_t enable) try {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
   std::lock_guard<std::mutex> lk(c->mu);
   c->dense = enable != 0;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_set_fusion(ehip_ctx* c, int32_t enable) {
+int32_t ehip_set_fusion(ehip_ctx* c, int32_t enable) try {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
   std::lock_guard<std::mutex> lk(c->mu);
   c->fuse = enable != 0;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0,
                          const uint32_t* arg1, const void* constants, uint32_t num_ops, uint32_t num_inputs,
                          uint32_t num_variables, uint32_t output_index, const uint32_t* output_indices,
-                         uint32_t num_outputs, ehip_tape** out) {
+                         uint32_t num_outputs, ehip_tape** out) try {
   if (!c || !out) return fail(EHIP_OTHER, "GPU error: null argument");
   *out = nullptr;
   if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "GPU error: dtype must be EHIP_F32 or EHIP_F64");
@@ -158,18 +170,18 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
   if (num_ops <= smax)
     for (int32_t what = 0; what < 8; ++what) {
       if (!(smask & (1u << what)) || (what == 2 && t->nout != 1)) continue;
+      // a failed build leaves this evaluation on the interpreters, which
+      // run every tape; the reason is kept on the tape
       uint32_t info[5];
-      int32_t rc = ehip_tape_specialize(t, what, 0, info);
-      if (rc != EHIP_OK) {
-        ehip_tape_free(t);
-        return rc;
-      }
+      if (ehip_tape_specialize(t, what, 0, info) != EHIP_OK) t->spec_error[what] = g_last_error;
     }
   *out = t;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_set_specialize(ehip_ctx* c, uint32_t max_entries, uint32_t what_mask) {
+int32_t ehip_set_specialize(ehip_ctx* c, uint32_t max_entries, uint32_t what_mask) try {
   if (!c) return fail(EHIP_OTHER, "GPU error: null argument");
   if (what_mask > 255)
     return fail(EHIP_OTHER, "GPU error: what_mask holds bits 0 forward, 1 gradient, 2 hvp, 3..7 taylor order 1..5");
@@ -179,21 +191,30 @@ int32_t ehip_set_specialize(ehip_ctx* c, uint32_t max_entries, uint32_t what_mas
   c->spec_max = what_mask ? max_entries : 0;
   c->spec_mask = what_mask;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-void ehip_tape_free(ehip_tape* t) {
+void ehip_tape_free(ehip_tape* t) try {
   if (!t) return;
   if (t->ctx) (void)hipSetDevice(t->ctx->device);
   for (auto& p : t->prog) free_program(p);
   for (auto& k : t->spec) spec_free(k);
+  for (auto& k : t->retired) spec_free(k);
   delete t;
+} catch (...) {
+  (void)exception_status();
+}
+
+const char* ehip_spec_error(const ehip_tape* t, int32_t what) {
+  return (t && what >= 0 && what < 8) ? t->spec_error[what].c_str() : "";
 }
 
 uint32_t ehip_num_outputs(const ehip_tape* t) { return t ? t->nout : 0; }
 uint32_t ehip_num_inputs(const ehip_tape* t) { return t ? t->nin : 0; }
 int32_t ehip_tape_dtype(const ehip_tape* t) { return t ? (t->f64 ? EHIP_F64 : EHIP_F32) : -1; }
 
-int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) {
+int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) try {
   if (!t || !out4 || mode < 0 || mode > 2) return fail(EHIP_OTHER, "GPU error: bad argument");
   const Program& p = t->prog[mode].host;
   out4[0] = p.ninstr;
@@ -201,13 +222,15 @@ int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) {
   out4[2] = p.words.size();
   out4[3] = p.dot_terms;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                              const double* constants, uint32_t num_ops, uint32_t num_inputs,
                              uint32_t output_index, const uint32_t* output_indices, uint32_t num_outputs,
                              int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
-                             uint64_t terms_cap, uint32_t* pool, uint64_t pool_cap, uint64_t* out5) {
+                             uint64_t terms_cap, uint32_t* pool, uint64_t pool_cap, uint64_t* out5) try {
   uint64_t* out4 = out5;
   if (!out4 || mode < 0 || mode > 7 || mode == 4 || mode == 6 || fuse < 0 || fuse > 2 ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
@@ -247,12 +270,14 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
   if (!(mode & 2)) std::memcpy(terms, p.terms.data(), p.terms.size() * 4);
   if (!p.pool.empty()) std::memcpy(pool, p.pool.data(), p.pool.size() * 4);
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 // ---------------------------- device-pointer API ----------------------------
 
 int32_t ehip_forward_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_in, uint64_t batch, void* d_out,
-                               void* stream) {
+                               void* stream) try {
   if (!c || !t) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, batch);
   if (rc) return rc;
@@ -267,10 +292,12 @@ int32_t ehip_forward_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_in
   L.y = d_out;
   rc = run(c, t, 0, 1, L, pick(c, stream));
   return rc ? rc : ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_gradient_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_in, uint64_t batch, void* d_out,
-                                void* d_grads, void* stream) {
+                                void* d_grads, void* stream) try {
   if (!c || !t || !d_grads) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, batch);
   if (rc) return rc;
@@ -286,10 +313,12 @@ int32_t ehip_gradient_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_i
   L.g = d_grads;
   rc = run(c, t, 0, 1, L, pick(c, stream));
   return rc ? rc : ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_hvp_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t x_stride, const void* d_dirs,
-                           uint64_t batch, void* d_grads, void* d_hvps, void* stream) {
+                           uint64_t batch, void* d_grads, void* d_hvps, void* stream) try {
   if (!c || !t || !d_x || !d_dirs || !d_grads || !d_hvps) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, batch);
   if (rc) return rc;
@@ -309,11 +338,13 @@ int32_t ehip_hvp_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uin
   L.h = d_hvps;
   rc = run(c, t, 1, 2, L, pick(c, stream));
   return rc ? rc : ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_taylor_kth_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t x_stride,
                                   const void* d_dirs, uint64_t batch, uint32_t order, void* d_coeffs,
-                                  void* stream) {
+                                  void* stream) try {
   if (!c || !t || !d_x || !d_dirs || !d_coeffs) return fail(EHIP_OTHER, "GPU error: null argument");
   if (order < 1 || order > 5)
     return fail(EHIP_OTHER, "GPU error: taylor_forward_kth_batch supports order 1..=5, got %u", order);
@@ -332,13 +363,15 @@ int32_t ehip_taylor_kth_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d
   L.y = d_coeffs;
   rc = run(c, t, 2, (int)order, L, pick(c, stream));
   return rc ? rc : ok();
+} catch (...) {
+  return exception_status();
 }
 
 // --------------------------- nonsmooth kink records ---------------------------
 
 uint32_t ehip_kink_count(const ehip_tape* t) { return t ? t->nkinks : 0; }
 
-int32_t ehip_kink_info(const ehip_tape* t, uint32_t* tape_index, uint32_t* opcode, uint32_t cap) {
+int32_t ehip_kink_info(const ehip_tape* t, uint32_t* tape_index, uint32_t* opcode, uint32_t cap) try {
   if (!t || (t->nkinks && (!tape_index || !opcode))) return fail(EHIP_OTHER, "GPU error: null argument");
   if (cap < t->nkinks) return fail(EHIP_OTHER, "GPU error: kink buffers too small (need %u)", t->nkinks);
   uint32_t k = 0;
@@ -349,10 +382,12 @@ int32_t ehip_kink_info(const ehip_tape* t, uint32_t* tape_index, uint32_t* opcod
       ++k;
     }
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_forward_nonsmooth_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t batch, void* d_out,
-                                   void* d_switch, int8_t* d_branch, void* stream) {
+                                   void* d_switch, int8_t* d_branch, void* stream) try {
   if (!c || !t || !d_x || (t->nkinks && (!d_switch || !d_branch))) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, batch);
   if (rc) return rc;
@@ -370,12 +405,14 @@ int32_t ehip_forward_nonsmooth_dev(ehip_ctx* c, const ehip_tape* t, const void* 
   L.nkinks = t->nkinks;
   rc = run(c, t, 3, 1, L, pick(c, stream));
   return rc ? rc : ok();
+} catch (...) {
+  return exception_status();
 }
 
 // ------------------------------ multi-tape queue ------------------------------
 
 int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, uint32_t wg, uint32_t lds_budget,
-                             void* stream) {
+                             void* stream) try {
   if (!c || (!jobs && njobs)) return fail(EHIP_OTHER, "GPU error: null argument");
   if (njobs == 0) return ok();
   if (wg == 0) wg = 256;
@@ -464,6 +501,8 @@ int32_t ehip_multi_batch_dev(ehip_ctx* c, const ehip_job* jobs, uint32_t njobs, 
   // the host tables above are released on return
   HIP_TRY(hipStreamSynchronize(s), "queue");
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 // --------------------------------- autotuner ---------------------------------
@@ -504,7 +543,7 @@ int32_t tune_launch(const ehip_tape* t, int32_t what, uint32_t order, const void
 
 int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t order, const void* d_x,
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
-                          uint32_t* out5) {
+                          uint32_t* out5) try {
   if (!c || !t || !d_x || !out5) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, batch);
   if (rc) return rc;
@@ -584,10 +623,12 @@ int32_t ehip_autotune_dev(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t orde
   out5[3] = best.fetch;
   out5[4] = best.ppl ? best.ppl : 1u;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, uint32_t lds_budget,
-                       uint32_t fetch, uint32_t points_per_lane) {
+                       uint32_t fetch, uint32_t points_per_lane) try {
   if (!t) return fail(EHIP_OTHER, "GPU error: null argument");
   if (wg != 0 && wg != 64 && wg != 128 && wg != 256)
     return fail(EHIP_OTHER, "GPU error: workgroup size must be 64, 128 or 256");
@@ -606,9 +647,11 @@ int32_t ehip_set_tuned(ehip_tape* t, int32_t what, uint32_t order, uint32_t wg, 
   t->tuned[key].lds_budget = wg ? lds_budget : 0;
   t->tuned[key].fetch = wg ? fetch : 0;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_t* out4) {
+int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_t* out4) try {
   if (!t || !out4) return fail(EHIP_OTHER, "GPU error: null argument");
   int key;
   if (what >= 0 && what <= 2) key = what;
@@ -619,6 +662,8 @@ int32_t ehip_get_tuned(const ehip_tape* t, int32_t what, uint32_t order, uint32_
   out4[2] = t->tuned[key].fetch;
   out4[3] = t->tuned[key].ppl ? t->tuned[key].ppl : 1u;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 // ------------------------------- hipGraph capture -------------------------------
@@ -632,7 +677,7 @@ struct ehip_graph {
 
 int32_t ehip_graph_capture(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t order, const void* d_x,
                            uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
-                           uint32_t repeats, ehip_graph** out) {
+                           uint32_t repeats, ehip_graph** out) try {
   if (!c || !t || !d_x || !out) return fail(EHIP_OTHER, "GPU error: null argument");
   *out = nullptr;
   if (repeats == 0) return fail(EHIP_OTHER, "GPU error: repeats must be non-zero");
@@ -669,17 +714,21 @@ int32_t ehip_graph_capture(ehip_ctx* c, ehip_tape* t, int32_t what, uint32_t ord
   }
   *out = gr;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_graph_launch(ehip_graph* gr, void* stream) {
+int32_t ehip_graph_launch(ehip_graph* gr, void* stream) try {
   if (!gr) return fail(EHIP_OTHER, "GPU error: null argument");
   std::lock_guard<std::mutex> lk(gr->ctx->mu);
   (void)hipSetDevice(gr->ctx->device);
   HIP_TRY(hipGraphLaunch(gr->exec, pick(gr->ctx, stream)), "graph launch");
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-void ehip_graph_destroy(ehip_graph* gr) {
+void ehip_graph_destroy(ehip_graph* gr) try {
   if (!gr) return;
   {
     std::lock_guard<std::mutex> lk(gr->ctx->mu);
@@ -690,6 +739,8 @@ void ehip_graph_destroy(ehip_graph* gr) {
     gr->regs.release();
   }
   delete gr;
+} catch (...) {
+  (void)exception_status();
 }
 
 // ----------------------------- host-pointer API -----------------------------
@@ -727,7 +778,7 @@ struct HostCall {
     if (_r) return _r;      \
   } while (0)
 
-int32_t ehip_forward_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs, uint32_t batch, void* outputs) {
+int32_t ehip_forward_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs, uint32_t batch, void* outputs) try {
   if (!c || !t || !inputs || !outputs) return fail(EHIP_OTHER, "GPU error: null argument");
   TRY(check_batch(t, batch));
   std::lock_guard<std::mutex> hlk(c->host_mu);
@@ -744,10 +795,12 @@ int32_t ehip_forward_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs, 
   TRY(h.fetch(outputs, dy, (size_t)batch * t->nout * h.S));
   TRY(h.sync());
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_forward_nonsmooth(ehip_ctx* c, const ehip_tape* t, const void* inputs, uint32_t batch, void* outputs,
-                               void* switching, int8_t* branches) {
+                               void* switching, int8_t* branches) try {
   if (!c || !t || !inputs || !outputs || (t->nkinks && (!switching || !branches)))
     return fail(EHIP_OTHER, "GPU error: null argument");
   TRY(check_batch(t, batch));
@@ -767,10 +820,12 @@ int32_t ehip_forward_nonsmooth(ehip_ctx* c, const ehip_tape* t, const void* inpu
   }
   TRY(h.sync());
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_gradient_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs, uint32_t batch, void* outputs,
-                            void* grads) {
+                            void* grads) try {
   if (!c || !t || !inputs || !outputs || !grads) return fail(EHIP_OTHER, "GPU error: null argument");
   TRY(check_batch(t, batch));
   std::lock_guard<std::mutex> hlk(c->host_mu);
@@ -789,6 +844,8 @@ int32_t ehip_gradient_batch(ehip_ctx* c, const ehip_tape* t, const void* inputs,
   TRY(h.fetch(grads, dg, (size_t)batch * t->nin * h.S));
   TRY(h.sync());
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 static int32_t hvp_host(ehip_ctx* c, const ehip_tape* t, const void* x, uint64_t xrows, const void* dirs,
@@ -816,17 +873,21 @@ static int32_t hvp_host(ehip_ctx* c, const ehip_tape* t, const void* x, uint64_t
 }
 
 int32_t ehip_hvp_batch(ehip_ctx* c, const ehip_tape* t, const void* x, const void* dirs, uint32_t batch,
-                       void* grads, void* hvps) {
+                       void* grads, void* hvps) try {
   return hvp_host(c, t, x, 1, dirs, batch, grads, hvps);
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_hvp_batch_points(ehip_ctx* c, const ehip_tape* t, const void* xs, const void* dirs, uint32_t batch,
-                              void* grads, void* hvps) {
+                              void* grads, void* hvps) try {
   return hvp_host(c, t, xs, batch, dirs, batch, grads, hvps);
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_taylor_kth_batch(ehip_ctx* c, const ehip_tape* t, const void* primals, const void* seeds,
-                              uint32_t batch, uint32_t order, void* coeffs) {
+                              uint32_t batch, uint32_t order, void* coeffs) try {
   if (!c || !t || !primals || !seeds || !coeffs) return fail(EHIP_OTHER, "GPU error: null argument");
   if (order < 1 || order > 5)
     return fail(EHIP_OTHER, "GPU error: taylor_forward_kth_batch supports order 1..=5, got %u", order);
@@ -856,13 +917,15 @@ int32_t ehip_taylor_kth_batch(ehip_ctx* c, const ehip_tape* t, const void* prima
     for (uint32_t k = 0; k < order; ++k)
       std::memcpy(dst + ((size_t)k * total_out + i) * h.S, raw.data() + (i * order + k) * h.S, h.S);
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 // ------------------------------- GPU STDE ----------------------------------
 
 int32_t ehip_laplacian_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t npoints, uint32_t S,
                                  uint32_t dir_kind, uint64_t seed, const void* d_dirs, const void* d_diag,
-                                 uint64_t diag_stride, double* d_results, void* stream) {
+                                 uint64_t diag_stride, double* d_results, void* stream) try {
   if (!c || !t || !d_x || !d_results) return fail(EHIP_OTHER, "GPU error: null argument");
   if (t->nout != 1)
     return fail(EHIP_OTHER, "GPU error: laplacian_gpu requires a scalar-output tape; got %u outputs", t->nout);
@@ -907,6 +970,8 @@ int32_t ehip_laplacian_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_
   R.out = d_results;
   HIP_TRY(launch_stde_reduce(t->f64, R, s), "stde reduce");
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 static int32_t laplacian_host(ehip_ctx* c, const ehip_tape* t, const void* x, uint64_t npoints, const void* dirs,
@@ -935,24 +1000,30 @@ static int32_t laplacian_host(ehip_ctx* c, const ehip_tape* t, const void* x, ui
 }
 
 int32_t ehip_laplacian(ehip_ctx* c, const ehip_tape* t, const void* x, const void* dirs, uint32_t S,
-                       double* result5) {
+                       double* result5) try {
   if (S > 0 && !dirs) return fail(EHIP_OTHER, "GPU error: null directions");
   return laplacian_host(c, t, x, 1, dirs, S, 0, 0, nullptr, 0, result5);
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_laplacian_with_control(ehip_ctx* c, const ehip_tape* t, const void* x, const void* dirs, uint32_t S,
-                                    const void* control_diagonal, double* result5) {
+                                    const void* control_diagonal, double* result5) try {
   if (!control_diagonal) return fail(EHIP_OTHER, "GPU error: null control diagonal");
   if (S > 0 && !dirs) return fail(EHIP_OTHER, "GPU error: null directions");
   return laplacian_host(c, t, x, 1, dirs, S, 0, 0, control_diagonal, 1, result5);
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_laplacian_batch(ehip_ctx* c, const ehip_tape* t, const void* xs, uint64_t npoints, uint32_t S,
-                             uint32_t dir_kind, uint64_t seed, const void* dirs, double* results) {
+                             uint32_t dir_kind, uint64_t seed, const void* dirs, double* results) try {
   return laplacian_host(c, t, xs, npoints, dirs, S, dir_kind, seed, nullptr, 0, results);
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_hessian_diagonal(ehip_ctx* c, const ehip_tape* t, const void* x, void* value, void* diag) {
+int32_t ehip_hessian_diagonal(ehip_ctx* c, const ehip_tape* t, const void* x, void* value, void* diag) try {
   if (!c || !t || !x || !value || !diag) return fail(EHIP_OTHER, "GPU error: null argument");
   if (t->nout != 1)
     return fail(EHIP_OTHER, "GPU error: hessian_diagonal_gpu requires a scalar-output tape; got %u outputs", t->nout);
@@ -979,6 +1050,8 @@ int32_t ehip_hessian_diagonal(ehip_ctx* c, const ehip_tape* t, const void* x, vo
     for (uint32_t j = 0; j < n; ++j) ((float*)diag)[j] = 2.0f * o[2 * n + j];
   }
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 }  // extern "C"

@@ -179,7 +179,7 @@ int32_t cv_adam(ehip_cv* cv, float lr, float b1, float b2, float eps, hipStream_
 extern "C" {
 
 int32_t ehip_cv_create(ehip_ctx* c, uint32_t n, uint32_t rank, uint32_t hidden, uint64_t seed, float out_scale,
-                       ehip_cv** out) {
+                       ehip_cv** out) try {
   if (!c || !out) return fail(EHIP_OTHER, "GPU error: null argument");
   *out = nullptr;
   if (n == 0 || hidden == 0) return fail(EHIP_OTHER, "GPU error: control variate needs n > 0 and hidden > 0");
@@ -223,9 +223,11 @@ int32_t ehip_cv_create(ehip_ctx* c, uint32_t n, uint32_t rank, uint32_t hidden, 
   }
   *out = cv;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-void ehip_cv_destroy(ehip_cv* cv) {
+void ehip_cv_destroy(ehip_cv* cv) try {
   if (!cv) return;
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   (void)hipSetDevice(cv->ctx->device);
@@ -237,9 +239,11 @@ void ehip_cv_destroy(ehip_cv* cv) {
   cv->act.release();
   cv->ws.release();
   delete cv;
+} catch (...) {
+  (void)exception_status();
 }
 
-int32_t ehip_cv_info(const ehip_cv* cv, uint64_t* out5) {
+int32_t ehip_cv_info(const ehip_cv* cv, uint64_t* out5) try {
   if (!cv || !out5) return fail(EHIP_OTHER, "GPU error: null argument");
   out5[0] = cv->n;
   out5[1] = cv->r;
@@ -247,10 +251,12 @@ int32_t ehip_cv_info(const ehip_cv* cv, uint64_t* out5) {
   out5[3] = cv->nparams;
   out5[4] = cv->step;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 // which: 0 parameters, 1 gradients, 2 first moments, 3 second moments.
-int32_t ehip_cv_get(ehip_cv* cv, int32_t which, float* host_out) {
+int32_t ehip_cv_get(ehip_cv* cv, int32_t which, float* host_out) try {
   if (!cv || !host_out || which < 0 || which > 3) return fail(EHIP_OTHER, "GPU error: bad argument");
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   (void)hipSetDevice(cv->ctx->device);
@@ -258,36 +264,44 @@ int32_t ehip_cv_get(ehip_cv* cv, int32_t which, float* host_out) {
   HIP_TRY(hipStreamSynchronize(cv->ctx->stream), "cv get");
   HIP_TRY(hipMemcpy(host_out, b[which]->p, cv->nparams * sizeof(float), hipMemcpyDeviceToHost), "cv get");
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_cv_set_gain(ehip_cv* cv, float gain) {
+int32_t ehip_cv_set_gain(ehip_cv* cv, float gain) try {
   if (!cv) return fail(EHIP_OTHER, "GPU error: null argument");
   if (!(gain > 0.0f) || !std::isfinite(gain)) return fail(EHIP_OTHER, "GPU error: gain must be positive and finite");
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   cv->gain = gain;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 float ehip_cv_gain(const ehip_cv* cv) { return cv ? cv->gain : 0.0f; }
 
-int32_t ehip_cv_set_signs(ehip_cv* cv, uint32_t npos) {
+int32_t ehip_cv_set_signs(ehip_cv* cv, uint32_t npos) try {
   if (!cv) return fail(EHIP_OTHER, "GPU error: null argument");
   if (npos > cv->r) return fail(EHIP_OTHER, "GPU error: npos must be <= rank (%u), got %u", cv->r, npos);
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   cv->npos = npos;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_cv_set_params(ehip_cv* cv, const float* host_in) {
+int32_t ehip_cv_set_params(ehip_cv* cv, const float* host_in) try {
   if (!cv || !host_in) return fail(EHIP_OTHER, "GPU error: null argument");
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   (void)hipSetDevice(cv->ctx->device);
   HIP_TRY(hipStreamSynchronize(cv->ctx->stream), "cv set");
   HIP_TRY(hipMemcpy(cv->params.p, host_in, cv->nparams * sizeof(float), hipMemcpyHostToDevice), "cv set");
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_cv_forward_dev(ehip_cv* cv, const float* d_x, uint64_t batch, float* d_L, void* stream) {
+int32_t ehip_cv_forward_dev(ehip_cv* cv, const float* d_x, uint64_t batch, float* d_L, void* stream) try {
   if (!cv || !d_x || !d_L) return fail(EHIP_OTHER, "GPU error: null argument");
   if (batch == 0) return ok();
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
@@ -304,28 +318,34 @@ int32_t ehip_cv_forward_dev(ehip_cv* cv, const float* d_x, uint64_t batch, float
     if (rc) return rc;
   }
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_cv_loss_grad_dev(ehip_cv* cv, const float* d_x, const float* d_v, const float* d_hv, uint64_t batch,
-                              float* d_loss, void* stream) {
+                              float* d_loss, void* stream) try {
   if (!cv || !d_x || !d_v || !d_hv) return fail(EHIP_OTHER, "GPU error: null argument");
   if (batch == 0) return fail(EHIP_OTHER, "GPU error: batch_size must be non-zero");
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   (void)hipSetDevice(cv->ctx->device);
   int32_t rc = cv_loss_grad(cv, d_x, d_v, d_hv, batch, d_loss, pick(cv->ctx, stream));
   return rc ? rc : ok();
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_cv_adam_step(ehip_cv* cv, float lr, float beta1, float beta2, float eps, void* stream) {
+int32_t ehip_cv_adam_step(ehip_cv* cv, float lr, float beta1, float beta2, float eps, void* stream) try {
   if (!cv) return fail(EHIP_OTHER, "GPU error: null argument");
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   (void)hipSetDevice(cv->ctx->device);
   int32_t rc = cv_adam(cv, lr, beta1, beta2, eps, pick(cv->ctx, stream));
   return rc ? rc : ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_cv_train_dev(ehip_cv* cv, const float* d_x, const float* d_v, const float* d_hv, uint64_t ntuples,
-                          uint32_t batch, uint32_t epochs, float lr, float* d_loss_hist, void* stream) {
+                          uint32_t batch, uint32_t epochs, float lr, float* d_loss_hist, void* stream) try {
   if (!cv || !d_x || !d_v || !d_hv) return fail(EHIP_OTHER, "GPU error: null argument");
   if (ntuples == 0 || batch == 0) return fail(EHIP_OTHER, "GPU error: ntuples and batch must be non-zero");
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
@@ -345,10 +365,12 @@ int32_t ehip_cv_train_dev(ehip_cv* cv, const float* d_x, const float* d_v, const
     }
   }
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_cv_make_tuples_dev(ehip_ctx* c, const ehip_tape* t, uint64_t ntuples, uint64_t seed, float* d_x,
-                                float* d_v, float* d_hv, void* stream) {
+                                float* d_v, float* d_hv, void* stream) try {
   if (!c || !t || !d_x || !d_v || !d_hv) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_batch(t, ntuples);
   if (rc) return rc;
@@ -391,11 +413,13 @@ int32_t ehip_cv_make_tuples_dev(ehip_ctx* c, const ehip_tape* t, uint64_t ntuple
     HIP_TRY(hipStreamSynchronize(s), "tuple convert");  // scratch is reused by later calls
   }
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv, const void* d_x, uint64_t npoints,
                                     uint32_t S, uint32_t dir_kind, uint64_t seed, double* d_results,
-                                    double* d_plain_results, void* stream) {
+                                    double* d_plain_results, void* stream) try {
   if (!c || !t || !d_x || !d_results) return fail(EHIP_OTHER, "GPU error: null argument");
   if (t->nout != 1)
     return fail(EHIP_OTHER, "GPU error: laplacian_gpu requires a scalar-output tape; got %u outputs", t->nout);
@@ -460,10 +484,12 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
   }
   HIP_TRY(launch_stde_reduce(t->f64, R, s), "stde reduce");
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_hessian_trace_batch_dev(ehip_ctx* c, const ehip_tape* t, const void* d_x, uint64_t npoints,
-                                     double* d_results, void* stream) {
+                                     double* d_results, void* stream) try {
   if (!c || !t || !d_x || !d_results) return fail(EHIP_OTHER, "GPU error: null argument");
   if (t->nout != 1)
     return fail(EHIP_OTHER, "GPU error: hessian trace requires a scalar-output tape; got %u outputs", t->nout);
@@ -497,6 +523,8 @@ int32_t ehip_hessian_trace_batch_dev(ehip_ctx* c, const ehip_tape* t, const void
   R.out = d_results;
   HIP_TRY(launch_stde_reduce(t->f64, R, s), "stde reduce");
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 }  // extern "C"

@@ -8,7 +8,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -22,14 +24,37 @@ namespace ehip_detail {
 
 inline thread_local std::string g_last_error;
 
-inline int32_t fail(int32_t code, const char* fmt, ...) {
+inline int32_t fail(int32_t code, const char* fmt, ...) noexcept {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
   std::vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  g_last_error = buf;
+  // the message must not be able to throw: with no memory for its text the
+  // status alone reports the failure
+  try {
+    g_last_error = buf;
+  } catch (...) {
+    g_last_error.clear();
+  }
   return code;
+}
+
+// Every extern "C" entry point that can allocate is a function-try-block
+// whose handler returns this: no C++ exception crosses the boundary.
+// std::bad_alloc (host memory exhausted while compiling a program, building
+// a kernel's source or parsing a wire payload) is GpuError::OutOfMemory;
+// anything else is GpuError::Other with the exception's text.
+inline int32_t exception_status() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return fail(EHIP_OUT_OF_MEMORY, "GPU out of memory (host allocation failed)");
+  } catch (const std::exception& e) {
+    return fail(EHIP_OTHER, "GPU error: %s", e.what());
+  } catch (...) {
+    return fail(EHIP_OTHER, "GPU error: unknown exception");
+  }
 }
 
 inline int32_t ok() {
@@ -141,6 +166,8 @@ struct ehip_tape {
   uint32_t nkinks = 0;
   Tuned tuned[8];  // forward, gradient, hvp, taylor order 1..5
   SpecKernel spec[8];  // tape-specialised kernels, once compiled, indexed like `tuned`
+  std::vector<SpecKernel> retired;  // kernels replaced by a later build, unloaded with the tape
+  std::string spec_error[8];  // why an upload-time specialisation failed (the interpreters run instead)
 };
 
 namespace ehip_detail {

@@ -783,6 +783,9 @@ __global__ __launch_bounds__(256) void k_queue_scalar(Queue Q) {
     if (threadIdx.x == 0) item = atomicAdd(Q.counter, 1u);
     __syncthreads();
     const uint32_t it = uni(item);
+    // every wave has read `item` before thread 0 may overwrite it for the
+    // next tile (the scalar-fetch body has no barrier of its own)
+    __syncthreads();
     if (it >= Q.ntiles) break;
     const QueueTile t = Q.tiles[it];
     const Launch L = Q.jobs[t.job];

@@ -973,7 +973,7 @@ extern "C" {
 int32_t ehip_tape_to_json(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                           const void* values, uint64_t num_entries, uint32_t num_inputs, uint32_t output_index,
                           const uint32_t* output_indices, uint64_t num_output_indices, char* buf, uint64_t cap,
-                          uint64_t* len) {
+                          uint64_t* len) try {
   if (!len) return fail(EHIP_OTHER, "tape JSON: null length pointer");
   if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "tape JSON: dtype must be EHIP_F32 or EHIP_F64");
   if (num_entries && (!opcodes || !arg0 || !arg1 || !values)) return fail(EHIP_OTHER, "tape JSON: null tape array");
@@ -1021,12 +1021,14 @@ int32_t ehip_tape_to_json(int32_t dtype, const uint32_t* opcodes, const uint32_t
   std::memcpy(buf, s.data(), s.size());
   buf[s.size()] = 0;
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_tape_from_json(const char* json, uint64_t len, int32_t dtype, uint32_t* opcodes, uint32_t* arg0,
                             uint32_t* arg1, void* values, uint64_t cap_entries, uint32_t* output_indices,
                             uint64_t cap_outputs, uint64_t* num_entries, uint32_t* num_inputs,
-                            uint32_t* output_index, uint64_t* num_output_indices) {
+                            uint32_t* output_index, uint64_t* num_output_indices) try {
   if (!num_entries || !num_inputs || !output_index || !num_output_indices)
     return fail(EHIP_OTHER, "tape JSON: null result pointer");
   if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "tape JSON: dtype must be EHIP_F32 or EHIP_F64");
@@ -1035,9 +1037,11 @@ int32_t ehip_tape_from_json(const char* json, uint64_t len, int32_t dtype, uint3
   if (rc) return rc;
   return copy_out(d, dtype, opcodes, arg0, arg1, values, cap_entries, output_indices, cap_outputs, num_entries,
                   num_inputs, output_index, num_output_indices, "JSON");
+} catch (...) {
+  return exception_status();
 }
 
-int32_t ehip_tape_upload_json(ehip_ctx* ctx, int32_t dtype, const char* json, uint64_t len, ehip_tape** out) {
+int32_t ehip_tape_upload_json(ehip_ctx* ctx, int32_t dtype, const char* json, uint64_t len, ehip_tape** out) try {
   if (!ctx || !out) return fail(EHIP_OTHER, "GPU error: null argument");
   *out = nullptr;
   if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "GPU error: dtype must be EHIP_F32 or EHIP_F64");
@@ -1054,12 +1058,14 @@ int32_t ehip_tape_upload_json(ehip_ctx* ctx, int32_t dtype, const char* json, ui
   }
   return ehip_tape_upload(ctx, dtype, d.opcodes.data(), d.arg0.data(), d.arg1.data(), vals, n, d.num_inputs, n,
                           d.output_index, d.output_indices.data(), (uint32_t)d.output_indices.size(), out);
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_tape_to_cbor(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                           const void* values, uint64_t num_entries, uint32_t num_inputs, uint32_t output_index,
                           const uint32_t* output_indices, uint64_t num_output_indices, uint8_t* buf, uint64_t cap,
-                          uint64_t* len) {
+                          uint64_t* len) try {
   if (!len) return fail(EHIP_OTHER, "tape CBOR: null length pointer");
   if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "tape CBOR: dtype must be EHIP_F32 or EHIP_F64");
   if (num_entries && (!opcodes || !arg0 || !arg1 || !values)) return fail(EHIP_OTHER, "tape CBOR: null tape array");
@@ -1101,12 +1107,14 @@ int32_t ehip_tape_to_cbor(int32_t dtype, const uint32_t* opcodes, const uint32_t
                                   (unsigned long long)cap, (unsigned long long)o.size());
   std::memcpy(buf, o.data(), o.size());
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_tape_from_cbor(const uint8_t* cbor, uint64_t len, int32_t dtype, uint32_t* opcodes, uint32_t* arg0,
                             uint32_t* arg1, void* values, uint64_t cap_entries, uint32_t* output_indices,
                             uint64_t cap_outputs, uint64_t* num_entries, uint32_t* num_inputs,
-                            uint32_t* output_index, uint64_t* num_output_indices) {
+                            uint32_t* output_index, uint64_t* num_output_indices) try {
   if (!num_entries || !num_inputs || !output_index || !num_output_indices)
     return fail(EHIP_OTHER, "tape CBOR: null result pointer");
   if (dtype != EHIP_F32 && dtype != EHIP_F64) return fail(EHIP_OTHER, "tape CBOR: dtype must be EHIP_F32 or EHIP_F64");
@@ -1115,6 +1123,8 @@ int32_t ehip_tape_from_cbor(const uint8_t* cbor, uint64_t len, int32_t dtype, ui
   if (rc) return rc;
   return copy_out(d, dtype, opcodes, arg0, arg1, values, cap_entries, output_indices, cap_outputs, num_entries,
                   num_inputs, output_index, num_output_indices, "CBOR");
+} catch (...) {
+  return exception_status();
 }
 
 }  // extern "C"

@@ -226,26 +226,30 @@ int32_t jacobian_pattern(const TapeHost& th, uint32_t* rows, uint32_t* cols, uin
 extern "C" {
 
 int32_t ehip_hessian_sparsity(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1, uint32_t num_ops,
-                              uint32_t num_inputs, uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz) {
+                              uint32_t num_inputs, uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz) try {
   if (!nnz) return fail(EHIP_OTHER, "GPU error: null argument");
   TapeHost th;
   int32_t rc = raw_tape(opcodes, arg0, arg1, num_ops, num_inputs, nullptr, 0, th);
   return rc ? rc : hessian_pattern(th, rows, cols, cap, nnz);
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_jacobian_sparsity(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1, uint32_t num_ops,
                                uint32_t num_inputs, const uint32_t* output_indices, uint32_t num_outputs,
-                               uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz) {
+                               uint32_t* rows, uint32_t* cols, uint64_t cap, uint64_t* nnz) try {
   if (!nnz) return fail(EHIP_OTHER, "GPU error: null argument");
   TapeHost th;
   int32_t rc = raw_tape(opcodes, arg0, arg1, num_ops, num_inputs, output_indices, num_outputs, th);
   return rc ? rc : jacobian_pattern(th, rows, cols, cap, nnz);
+} catch (...) {
+  return exception_status();
 }
 
 // greedy_coloring (src/sparse.rs:57-98): distance-2 coloring of the
 // interaction graph.
 int32_t ehip_greedy_coloring(uint32_t dim, const uint32_t* rows, const uint32_t* cols, uint64_t nnz,
-                             uint32_t* colors, uint32_t* num_colors) {
+                             uint32_t* colors, uint32_t* num_colors) try {
   if (!num_colors || (dim && !colors)) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_pattern(rows, cols, nnz, dim, dim);
   if (rc) return rc;
@@ -266,24 +270,30 @@ int32_t ehip_greedy_coloring(uint32_t dim, const uint32_t* rows, const uint32_t*
   }
   *num_colors = distance1(adj2, colors);
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_column_coloring(uint32_t num_outputs, uint32_t num_inputs, const uint32_t* rows, const uint32_t* cols,
-                             uint64_t nnz, uint32_t* colors, uint32_t* num_colors) {
+                             uint64_t nnz, uint32_t* colors, uint32_t* num_colors) try {
   if (!num_colors || (num_inputs && !colors)) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_pattern(rows, cols, nnz, num_outputs, num_inputs);
   if (rc) return rc;
   *num_colors = intersection_coloring(rows, cols, nnz, num_outputs, num_inputs, colors);
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 int32_t ehip_row_coloring(uint32_t num_outputs, uint32_t num_inputs, const uint32_t* rows, const uint32_t* cols,
-                          uint64_t nnz, uint32_t* colors, uint32_t* num_colors) {
+                          uint64_t nnz, uint32_t* colors, uint32_t* num_colors) try {
   if (!num_colors || (num_outputs && !colors)) return fail(EHIP_OTHER, "GPU error: null argument");
   int32_t rc = check_pattern(rows, cols, nnz, num_outputs, num_inputs);
   if (rc) return rc;
   *num_colors = intersection_coloring(cols, rows, nnz, num_inputs, num_outputs, colors);
   return ok();
+} catch (...) {
+  return exception_status();
 }
 
 }  // extern "C"
@@ -364,7 +374,7 @@ extern "C" {
 // distance-2 coloring on the host, one HVP per color on the GPU; entries
 // recovered as hvp[color(col)][row].
 int32_t ehip_sparse_hessian(ehip_ctx* c, const ehip_tape* t, const void* x, void* value, void* grad, uint32_t* rows,
-                            uint32_t* cols, void* vals, uint64_t cap, uint64_t* nnz) {
+                            uint32_t* cols, void* vals, uint64_t cap, uint64_t* nnz) try {
   if (!c || !t || !x || !value || !grad || !nnz) return fail(EHIP_OTHER, "GPU error: null argument");
   if (t->nout != 1)
     return fail(EHIP_OTHER, "GPU error: sparse_hessian requires a scalar-output tape; got %u outputs", t->nout);
@@ -374,18 +384,22 @@ int32_t ehip_sparse_hessian(ehip_ctx* c, const ehip_tape* t, const void* x, void
                                     (double*)vals, cap, nnz);
   return sparse_hessian_t<float>(c, t, (const float*)x, (float*)value, (float*)grad, rows, cols, (float*)vals, cap,
                                  nnz);
+} catch (...) {
+  return exception_status();
 }
 
 // GpuBackend::sparse_jacobian (src/gpu/cuda_backend.rs:306-395): pattern and
 // column coloring on the host, one tangent sweep per color on the GPU;
 // entries recovered as tangent_out[color(col)][row].
 int32_t ehip_sparse_jacobian(ehip_ctx* c, const ehip_tape* t, const void* x, void* outputs, uint32_t* rows,
-                             uint32_t* cols, void* vals, uint64_t cap, uint64_t* nnz) {
+                             uint32_t* cols, void* vals, uint64_t cap, uint64_t* nnz) try {
   if (!c || !t || !x || !outputs || !nnz) return fail(EHIP_OTHER, "GPU error: null argument");
   if (t->nin == 0) return fail(EHIP_OTHER, "GPU error: tape has no inputs (constant function); evaluate it on the CPU instead");
   if (t->f64)
     return sparse_jacobian_t<double>(c, t, (const double*)x, (double*)outputs, rows, cols, (double*)vals, cap, nnz);
   return sparse_jacobian_t<float>(c, t, (const float*)x, (float*)outputs, rows, cols, (float*)vals, cap, nnz);
+} catch (...) {
+  return exception_status();
 }
 
 }  // extern "C"

@@ -400,7 +400,7 @@ int32_t spec_build(ehip_ctx* c, const TapeHost& h, int what, uint32_t waves, con
 
 }  // namespace
 
-int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, uint32_t* out5) {
+int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, uint32_t* out5) try {
   if (!t || !out5) return fail(EHIP_OTHER, "GPU error: null argument");
   if (what < 0 || what > 7)
     return fail(EHIP_OTHER,
@@ -444,7 +444,9 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, 
   }
   std::lock_guard<std::mutex> lk(c->mu);
   (void)hipSetDevice(c->device);
-  spec_free(t->spec[what]);
+  // a graph captured earlier may still launch the kernel this one
+  // replaces: its module stays loaded until the tape is freed
+  if (t->spec[what].mod) t->retired.push_back(t->spec[what]);
   t->spec[what] = s;
   out5[0] = s.vgprs;
   out5[1] = s.scratch;
@@ -453,11 +455,14 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, 
   out5[4] = s.waves;
   return ok();
 }
+catch (...) {
+  return exception_status();
+}
 
 int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,
                          const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
                          const uint32_t* output_indices, uint32_t num_outputs, int32_t what,
-                         uint32_t waves_per_eu, const char* arch, char* src, uint64_t src_cap, uint64_t* out3) {
+                         uint32_t waves_per_eu, const char* arch, char* src, uint64_t src_cap, uint64_t* out3) try {
   if (!out3 || what < 0 || what > 7 || waves_per_eu > 8 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
@@ -499,4 +504,6 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
     std::memcpy(src, s.c_str(), s.size() + 1);
   }
   return ok();
+} catch (...) {
+  return exception_status();
 }

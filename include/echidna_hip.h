@@ -275,17 +275,24 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * wave per SIMD, the kernel is built again for two and that build is kept.  out5 = {VGPRs and AGPRs, scratch bytes per
  * lane, compile microseconds (all builds), code object bytes, the
  * occupancy target used (0: the compiler's)}.  Tapes longer than 65536
- * entries are refused.  The compile runs outside the context lock; building
- * a kernel again replaces the old one after the device is idle, which
- * invalidates graphs captured with it. */
+ * entries are refused.  The compile runs outside the context lock.  Building
+ * a kernel again makes the new one serve later calls; the one it replaces
+ * stays loaded until the tape is freed, so graphs captured with it replay
+ * unchanged. */
 int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t waves_per_eu, uint32_t* out5);
 
 /* Specialise every tape uploaded after this call whose entry count is at
  * most max_entries (<= 65536; 0 = off, the default), for the evaluations in
  * what_mask (bit 0 forward, bit 1 gradient, bit 2 hvp, bits 3..7 taylor
- * order 1..5; hvp only for scalar-output tapes).  A compile failure fails
- * the upload. */
+ * order 1..5; hvp only for scalar-output tapes).  A compile failure does
+ * not fail the upload: that evaluation runs on the interpreters, and
+ * ehip_spec_error gives the reason. */
 int32_t ehip_set_specialize(ehip_ctx* ctx, uint32_t max_entries, uint32_t what_mask);
+
+/* Why the upload-time specialisation of evaluation `what` (as for
+ * ehip_tape_specialize) failed on this tape; "" when it did not fail or was
+ * not asked for.  The text lives as long as the tape. */
+const char* ehip_spec_error(const ehip_tape* tape, int32_t what);
 
 /* The source ehip_tape_specialize compiles, from the flattened tape arrays
  * with no device (tape arguments as ehip_compile_program).  When `arch` is

@@ -66,3 +66,39 @@ def test_python_validate_matches_reference_messages():
     d.arg1[2] = 5
     with pytest.raises(GpuError, match="should be the UNUSED sentinel"):
         d.validate()
+
+
+_OOM_CHILD = r"""
+import ctypes, resource, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from echidna_amd import backend
+lib = backend.load_library()
+n = 1 << 21
+# x0 * x0 * ... : a chain of 2M products (valid tape, about 60 MB of host
+# program state to compile)
+op = np.full(n, 4, np.uint32); op[0] = 0
+a0 = np.arange(-1, n - 1, dtype=np.int64).astype(np.uint32); a0[0] = 0xFFFFFFFF
+a1 = np.zeros(n, np.uint32); a1[0] = 0xFFFFFFFF
+c = np.zeros(n)
+out5 = (ctypes.c_uint64 * 5)()
+P = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))
+vm = [int(l.split()[1]) for l in open("/proc/self/status") if l.startswith("VmSize")][0] * 1024
+resource.setrlimit(resource.RLIMIT_AS, (vm + (24 << 20), resource.getrlimit(resource.RLIMIT_AS)[1]))
+rc = lib.ehip_compile_program(0, P(op, ctypes.c_uint32), P(a0, ctypes.c_uint32), P(a1, ctypes.c_uint32),
+                              P(c, ctypes.c_double), n, 1, n - 1, None, 0, 1, 1, None, 0, None, 0, None, 0, out5)
+resource.setrlimit(resource.RLIMIT_AS, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+print(rc, lib.ehip_last_error().decode(), flush=True)
+"""
+
+
+def test_host_allocation_failure_is_a_status_not_an_abort():
+    """A C++ exception never crosses the C ABI: std::bad_alloc while compiling
+    a program under a tight address-space limit comes back as status 3
+    (GpuError::OutOfMemory, src/gpu/mod.rs:276-303), not std::terminate."""
+    import sys as _sys
+    r = subprocess.run([_sys.executable, "-c", _OOM_CHILD, ROOT], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rc, msg = r.stdout.strip().split(" ", 1)
+    assert int(rc) == backend.GpuError.OUT_OF_MEMORY, r.stdout
+    assert "out of memory" in msg
