@@ -1172,6 +1172,84 @@ template <class F> inline void hvp(const TapeView& t, const F* x, const F* v, F*
   }
 }
 
+// Magnitudes of the terms behind each result (no reference counterpart;
+// the scale a parity bar is stated against where results cancel).
+// Forward, every entry r = f(a, b) carries
+//   E_r = |da| E_a + |db| E_b + |r|    (E = |x_i| at inputs, |c| at constants)
+//   T_r = |da| T_a + |db| T_b          (T = |v_i| at inputs, 0 at constants)
+// with da, db the reverse partials: E / 2^53 bounds the rounding error of
+// the value to first order (running error analysis) and T is the sum of the
+// absolute terms of the tangent grad . v.  Reverse, the adjoints of the
+// absolute partials (evaluated as Dual numbers on (value, T), each component
+// taken in absolute value) give, per input, the sum of the absolute terms of
+// the gradient (.re) and of H v (.eps); c2 = v^T H v / 2 gets
+// sum_i |v_i| (H v)_i^abs / 2.
+template <class F> inline void magnitudes(const TapeView& t, const F* x, const F* v, F* val_mag /*[m]*/,
+                                          F* grad_mag /*[n]*/, F* hv_mag /*[n]*/, F* c1_mag, F* c2_mag) {
+  typedef Dual<F> D;
+  std::vector<F> val;
+  forward_into<F>(t, x, val);
+  const uint32_t N = t.num_ops;
+  std::vector<F> E(N), T(N);
+  for (uint32_t i = 0; i < N; ++i) {
+    uint32_t op = t.opcodes[i];
+    if (op == OP_INPUT) { E[i] = m_abs(x[i]); T[i] = m_abs(v[i]); continue; }
+    if (op == OP_CONST) { E[i] = m_abs((F)t.values[i]); T[i] = F(0); continue; }
+    uint32_t ai = t.arg0[i], bi = t.arg1[i];
+    F a = val[ai], r = val[i];
+    F da, db = F(0);
+    if (op == OP_POWI) {
+      da = powi_reverse_partial<F>((int32_t)bi, a, r);
+      E[i] = m_abs(da) * E[ai] + m_abs(r);
+      T[i] = m_abs(da) * T[ai];
+      continue;
+    }
+    F b = (bi != UNUSED) ? val[bi] : F(0);
+    reverse_partials<F>(op, a, b, r, da, db);
+    E[i] = m_abs(da) * E[ai] + m_abs(r);
+    T[i] = m_abs(da) * T[ai];
+    if (bi != UNUSED) {
+      E[i] = E[i] + m_abs(db) * E[bi];
+      T[i] = T[i] + m_abs(db) * T[bi];
+    }
+  }
+  for (uint32_t j = 0; j < t.num_outputs; ++j) val_mag[j] = E[t.output_indices[j]];
+  const uint32_t seed = t.output_indices[0];
+  if (c1_mag) *c1_mag = T[seed];
+  std::vector<D> A(N, D{F(0), F(0)});
+  A[seed] = D{F(1), F(0)};
+  auto acc = [&](uint32_t k, D p, D g) {
+    const F pr = m_abs(p.re), pe = m_abs(p.eps);
+    A[k].re = A[k].re + pr * g.re;
+    A[k].eps = A[k].eps + pr * g.eps + pe * g.re;
+  };
+  for (int64_t ii = (int64_t)N - 1; ii >= 0; --ii) {
+    uint32_t i = (uint32_t)ii;
+    uint32_t op = t.opcodes[i];
+    if (op == OP_INPUT || op == OP_CONST) continue;
+    D g = A[i];
+    if (g.re == F(0) && g.eps == F(0)) continue;
+    uint32_t ai = t.arg0[i], bi = t.arg1[i];
+    D a{val[ai], T[ai]}, r{val[i], T[i]};
+    if (op == OP_POWI) {
+      acc(ai, powi_reverse_partial<D>((int32_t)bi, a, r), g);
+      continue;
+    }
+    D b = (bi != UNUSED) ? D{val[bi], T[bi]} : D{F(0), F(0)};
+    D da, db;
+    reverse_partials<D>(op, a, b, r, da, db);
+    acc(ai, da, g);
+    if (bi != UNUSED) acc(bi, db, g);
+  }
+  F c2 = F(0);
+  for (uint32_t i = 0; i < t.num_inputs; ++i) {
+    if (grad_mag) grad_mag[i] = A[i].re;
+    if (hv_mag) hv_mag[i] = A[i].eps;
+    c2 = c2 + m_abs(v[i]) * A[i].eps;
+  }
+  if (c2_mag) *c2_mag = c2 / F(2);
+}
+
 // taylor_jet_2nd (src/stde/jet.rs:26-47) generalised to every output.
 template <class F, int K> inline void taylor_jets(const TapeView& t, const F* x, const F* v, F* out /*[m][K]*/) {
   std::vector<Taylor<F, K>> in(t.num_inputs), buf;

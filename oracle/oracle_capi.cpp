@@ -121,6 +121,22 @@ int orc_gradient_f32(TAPE_ARGS, const float* x, uint64_t B, float* y, float* g, 
   gradient_batch<float>(TAPE, x, B, y, g, nt);
   return 0;
 }
+// Per point: magnitudes of the terms behind every output value, each
+// gradient and H v component, and the jet coefficients c1, c2 along v
+// (orc::magnitudes).  out: val [B][m], grad [B][n], hv [B][n], c12 [B][2].
+int orc_magnitudes_f64(TAPE_ARGS, const double* x, uint64_t xstride, const double* v, uint64_t B, double* val,
+                       double* grad, double* hv, double* c12, int nt) {
+  TapeView t = TAPE;
+  par_for(B, nt, [&](uint64_t lo, uint64_t hi) {
+    for (uint64_t p = lo; p < hi; ++p) {
+      const uint64_t n = t.num_inputs;
+      magnitudes<double>(t, x + p * xstride, v + p * n, val + p * t.num_outputs, grad + p * n, hv + p * n,
+                         c12 + 2 * p, c12 + 2 * p + 1);
+    }
+  });
+  return 0;
+}
+
 // Per-point x when xstride == nin; a single shared x when xstride == 0.
 int orc_hvp_f64(TAPE_ARGS, const double* x, uint64_t xstride, const double* v, uint64_t B, double* g,
                 double* hv, int nt) {

@@ -245,3 +245,25 @@ def nonsmooth(tape, X, dtype=np.float64):
     if rc != 0:
         raise ValueError("kink count mismatch")
     return y, sw[:, :K], br[:, :K]
+
+
+def magnitudes(tape, x, v=None, nthreads=1):
+    """Sums of the absolute terms behind each result (the cancellation floor
+    of a parity bar): dict with value [B,m], grad [B,n], hv [B,n], c1 [B],
+    c2 [B] (hv, c1 and c2 along v; v = 0 when not given)."""
+    t = OracleTape(tape)
+    n = t.num_inputs
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, n)
+    B = x.shape[0]
+    v = np.zeros((B, n)) if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1, n)
+    if v.shape[0] != B:
+        x = np.ascontiguousarray(np.broadcast_to(x, (v.shape[0], n)))
+        B = v.shape[0]
+    val = np.zeros((B, t.num_outputs))
+    g = np.zeros((B, n))
+    hv = np.zeros((B, n))
+    c12 = np.zeros((B, 2))
+    lib().orc_magnitudes_f64(*t.args(), _ptr(x, f64p), ctypes.c_uint64(n), _ptr(v, f64p), ctypes.c_uint64(B),
+                             _ptr(val, f64p), _ptr(g, f64p), _ptr(hv, f64p), _ptr(c12, f64p),
+                             ctypes.c_int(nthreads))
+    return {"value": val, "grad": g, "hv": hv, "c1": c12[:, 0], "c2": c12[:, 1]}

@@ -1,0 +1,202 @@
+"""The benchmark's workloads at their full sizes, against the oracle.
+
+* Headline (tanh MLP n=32 H=64 seed 1, B = 2^20, fwd+grad): the launch the
+  benchmark times -- the tape-specialised kernel at the autotuned shape --
+  and every points-per-lane variant of it, bit for bit against the register
+  interpreter over the whole batch, and a strided sample of 4,096 points
+  against the f64 oracle.  f64 the same way.
+* Plain STDE on the n=256 H=256 MLP (seed 2): B = 65,536 points, S = 64
+  Rademacher and Gaussian directions drawn on the device, f32 jets with an
+  f64 reduction; three points against the oracle's laplacian_with_stats on
+  the same directions (regenerated from Philox), and the estimates of 256
+  points against the exact f64 traces (z-scores).
+* The 64 random DAGs of seed 3 (lengths 200..20,000, 1.34 M points) in one
+  persistent launch, f32 and f64: every job bit for bit against its own
+  per-tape launch, and 32 points of every tape against the oracle.
+* The n=1024 2x512 MLP (seed 4, 1.57 M computed entries): plain STDE at
+  B = 16,384, S = 16 Gaussian against the oracle on two points; the learned
+  control variate end to end on 2^14 tuples (trained on the device,
+  unbiased against exact traces, used only when its variance ratio is
+  below 1).
+
+Bars: f64 as tests/parity_bar.py states (1e-12 of the larger of |oracle|
+and the sum of the absolute terms behind it).  f32 against the f64 oracle:
+2e-5 of the same scale for values and gradients (tanh MLP, 32..64-term
+sums); 1e-4 for STDE estimates relative to max(|estimate|, SE) and per-jet
+samples; DAG sample points within 1e-3 of the scale (sums over up to
+20,000 f32 operations).  Each test prints its measured worst ratio.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+from echidna_amd import cv as lcv
+from echidna_amd import philox
+from echidna_amd import workloads as wl
+from echidna_amd.backend import TUNE_GRADIENT, HipContext
+from echidna_amd.tape import GpuTapeData
+from parity_bar import assert_f64
+
+pytestmark = pytest.mark.gpu
+
+F32_TOL = 2e-5
+
+
+def _bits(t):
+    return t.cpu().numpy().view(np.uint8)
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_headline_specialised_kernel_full_batch(hip, f64):
+    n, H, B = 32, 64, 1 << 20
+    tape = wl.tanh_mlp(n, H, 1)
+    dt, tdt = (np.float64, torch.float64) if f64 else (np.float32, torch.float32)
+    x = wl.points_normal(1, B, n).astype(dt)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    HipContext.specialize(bufs, TUNE_GRADIENT)
+    dx = torch.from_numpy(x).cuda()
+
+    def run():
+        y = torch.full((B,), float("nan"), dtype=tdt, device="cuda")
+        g = torch.full((B, n), float("nan"), dtype=tdt, device="cuda")
+        hip.gradient_batch_dev(bufs, dx, B, y, g)
+        torch.cuda.synchronize()
+        return y, g
+
+    tuned = hip.autotune_dev(bufs, TUNE_GRADIENT, B, dx, d_y=torch.empty(B, dtype=tdt, device="cuda"),
+                             d_g=torch.empty(B, n, dtype=tdt, device="cuda"))
+    y0, g0 = run()  # the benchmark's launch
+    HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 24576, fetch=1)  # the interpreter
+    yi, gi = run()
+    assert np.array_equal(_bits(y0), _bits(yi)) and np.array_equal(_bits(g0), _bits(gi)), tuned
+    for ppl in (1, 2, 4):
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 0, fetch=2, points_per_lane=ppl)
+        y, g = run()
+        assert np.array_equal(_bits(y), _bits(yi)) and np.array_equal(_bits(g), _bits(gi)), ppl
+    idx = np.unique(np.concatenate([np.arange(0, B, 257), [B - 1]]))[:4096]
+    y_ref, g_ref = po.gradient(tape, x[idx].astype(np.float64), nthreads=8)
+    mag = po.magnitudes(tape, x[idx].astype(np.float64), nthreads=8)
+    yd, gd = y0.cpu().numpy()[idx], g0.cpu().numpy()[idx]
+    if f64:
+        wy = assert_f64(yd, y_ref[:, 0], mag["value"][:, 0], "value")
+        wg = assert_f64(gd, g_ref, mag["grad"], "gradient")
+    else:
+        wy = assert_f64(yd, y_ref[:, 0], mag["value"][:, 0], "value", tol=F32_TOL)
+        wg = assert_f64(gd, g_ref, mag["grad"], "gradient", tol=F32_TOL)
+    rel = np.max(np.abs(gd - g_ref) / np.maximum(np.abs(g_ref), 1e-300))
+    print(f"headline {'f64' if f64 else 'f32'} launch {tuned}: worst ratio value {wy:.3g} gradient {wg:.3g}; "
+          f"max plain relative gradient error {rel:.3g}")
+
+
+@pytest.fixture(scope="module")
+def mlp256():
+    return wl.tanh_mlp(256, 256, 2)
+
+
+@pytest.mark.parametrize("kind", [lcv.DIR_RADEMACHER, lcv.DIR_GAUSSIAN])
+def test_stde_n256_full_batch(hip, mlp256, kind):
+    n, P, S, seed = 256, 65536, 64, 2
+    X = wl.points_normal(2, P, n).astype(np.float32)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(mlp256))
+    dX = torch.from_numpy(X).cuda()
+    res = lcv.laplacian_batch(hip, bufs, dX, S, kind, seed)
+    assert np.all(res[:, 4] == S)
+    draw = philox.rademacher if kind == lcv.DIR_RADEMACHER else philox.normal
+    worst = 0.0
+    for p in (0, 31337, P - 1):
+        idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
+        dirs = draw(seed, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        ref = po.laplacian_with_stats(mlp256, X[p].astype(np.float64), dirs)
+        scale = max(abs(ref["estimate"]), ref["standard_error"])
+        err = abs(res[p, 1] - ref["estimate"]) / scale
+        verr = abs(res[p, 2] - ref["sample_variance"]) / ref["sample_variance"]
+        worst = max(worst, err, verr)
+        assert err <= 1e-4 and verr <= 1e-4, (p, res[p], ref)
+    # exact traces from n basis jets of the f64 tape on 256 points
+    k = 256
+    b64 = hip.upload_tape(GpuTapeData.from_tape_f64(mlp256))
+    exact = lcv.hessian_trace_batch(hip, b64, torch.from_numpy(X[:k].astype(np.float64)).cuda())
+    H0 = po.hessian(mlp256, X[0].astype(np.float64))
+    assert abs(exact[0] - np.trace(H0)) <= 1e-12 * np.sum(np.abs(np.diag(H0)))
+    z = (res[:k, 1] - exact) / res[:k, 3]
+    print(f"stde n=256 kind {kind}: worst rel err vs oracle {worst:.3g}; mean |z| {np.mean(np.abs(z)):.3f}, "
+          f"mean z {np.mean(z):.3f}, max |z| {np.max(np.abs(z)):.2f}")
+    assert abs(np.mean(z)) <= 4.0 / np.sqrt(k) and np.mean(np.abs(z)) <= 1.0 and np.max(np.abs(z)) <= 5.0
+
+
+@pytest.fixture(scope="module")
+def dag_suite():
+    specs = wl.random_dag_suite(3, 64, 1.0)
+    tapes = [wl.random_dag(3, k, length, n) for k, (length, n, B) in enumerate(specs)]
+    xs = [wl.points_uniform(300 + k, B, n, -1.0, 1.0) for k, (length, n, B) in enumerate(specs)]
+    return tapes, xs
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_dag_suite_full_one_launch(hip, dag_suite, f64):
+    tapes, xs = dag_suite
+    dt, tdt = (np.float64, torch.float64) if f64 else (np.float32, torch.float32)
+    jobs = []
+    for t, x in zip(tapes, xs):
+        b = hip.upload_tape(GpuTapeData.from_tape_f64(t) if f64 else GpuTapeData.from_tape(t))
+        B, n = x.shape
+        jobs.append((b, 1, torch.from_numpy(x.astype(dt)).cuda(), B,
+                     torch.full((B,), float("nan"), dtype=tdt, device="cuda"),
+                     torch.full((B, n), float("nan"), dtype=tdt, device="cuda")))
+    assert sum(j[3] for j in jobs) == 1343728
+    hip.set_fetch(1)
+    try:
+        hip.multi_batch_dev(jobs, wg=256, lds_budget=24576)
+    finally:
+        hip.set_fetch(0)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for (b, _, dx, B, dy, dg), t, x in zip(jobs, tapes, xs):
+        ry, rg = torch.empty_like(dy), torch.empty_like(dg)
+        hip.gradient_batch_dev(b, dx, B, ry, rg)
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(dy), _bits(ry)) and np.array_equal(_bits(dg), _bits(rg))
+        xr = x[:32].astype(dt).astype(np.float64)
+        y_ref, g_ref = po.gradient(t, xr)
+        mag = po.magnitudes(t, xr)
+        tol = 1e-12 if f64 else 1e-3
+        worst = max(worst, assert_f64(dy.cpu().numpy()[:32], y_ref[:, 0], mag["value"][:, 0], "value", tol),
+                    assert_f64(dg.cpu().numpy()[:32], g_ref, mag["grad"], "gradient", tol))
+    print(f"dag suite {'f64' if f64 else 'f32'}: worst ratio {worst:.3g} of {1e-12 if f64 else 1e-3}")
+
+
+def test_cv_n1024_plain_and_learned(hip):
+    n, H, S, P = 1024, 512, 16, 16384
+    tape = wl.tanh_mlp2(n, H, H, 4)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    X = wl.points_normal(4, P, n).astype(np.float32)
+    dX = torch.from_numpy(X).cuda()
+    plain = lcv.laplacian_batch(hip, bufs, dX, S, lcv.DIR_GAUSSIAN, 5)
+    worst = 0.0
+    for p in (0, P - 1):
+        idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
+        dirs = philox.normal(5, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        ref = po.laplacian_with_stats(tape, X[p].astype(np.float64), dirs)
+        err = abs(plain[p, 1] - ref["estimate"]) / max(abs(ref["estimate"]), ref["standard_error"])
+        worst = max(worst, err)
+        assert err <= 1e-4 and abs(plain[p, 2] - ref["sample_variance"]) <= 1e-4 * ref["sample_variance"]
+    est = lcv.LearnedStde(hip, bufs, rank=16, hidden=256, seed=4, npos=8)
+    est.fit(ntuples=1 << 14, epochs=2, batch=1024, lr=2e-3, val_points=64, val_S=S, out_scale=0.03,
+            schedule="cosine")
+    assert np.all(np.isfinite(est.history)) and est.use_cv == (est.variance_ratio < 1.0)
+    k = 256
+    cvres, pl = lcv.laplacian_batch(hip, bufs, dX[:k], S, lcv.DIR_GAUSSIAN, 5, est.cv, with_plain=True)
+    # the same samples give the same plain statistics (the reductions may
+    # sum in another order)
+    assert np.max(np.abs(pl[:, 1] - plain[:k, 1]) / plain[:k, 3]) <= 1e-9
+    assert np.max(np.abs(pl[:, 2] - plain[:k, 2]) / plain[:k, 2]) <= 1e-12
+    b64 = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    exact = lcv.hessian_trace_batch(hip, b64, dX[:k].double())
+    z = (cvres[:, 1] - exact) / cvres[:, 3]
+    ratio = float(np.mean(cvres[:, 2]) / np.mean(pl[:, 2]))
+    print(f"cv n=1024: plain worst rel err vs oracle {worst:.3g}; trained on 2^14 tuples: validation ratio "
+          f"{est.variance_ratio:.4f}, ratio on {k} points {ratio:.4f}, mean z {np.mean(z):.3f}, "
+          f"max |z| {np.max(np.abs(z)):.2f}")
+    assert abs(np.mean(z)) <= 4.0 / np.sqrt(k) and np.max(np.abs(z)) <= 5.5

@@ -3,9 +3,9 @@
 Bars (stated here, used below):
   * per-opcode table: the reference's own ULP budgets (tests/gpu_cpu_parity.rs),
     f64 GPU vs f64 CPU and f32 GPU vs f64 CPU rounded to f32;
-  * composite tapes, f64: |gpu - cpu| <= 1e-12 * max(|cpu_i|, max_j |cpu_j|)
-    per row (transcendental last-ulp differences only);
-  * composite tapes, f32 vs the f64 oracle: same form with 2e-5 (F32_TOL).
+  * composite tapes, f64: elementwise |gpu - cpu| <= 1e-12 * max(|cpu|, M),
+    M the sum of the absolute terms behind the entry (tests/parity_bar.py);
+  * composite tapes, f32 vs the f64 oracle: the same form with 2e-5 (F32_TOL).
 """
 
 import numpy as np
@@ -14,6 +14,7 @@ import pytest
 import pyoracle as po
 from echidna_amd import workloads as wl
 from echidna_amd.tape import GpuError, GpuTapeData, record
+from parity_bar import assert_f64
 from parity_cases import CASES, ulp_diff_f32, ulp_diff_f64
 
 pytestmark = pytest.mark.gpu
@@ -22,18 +23,14 @@ F64_TOL = 1e-12
 F32_TOL = 2e-5
 
 
-def rowrel(gpu, ref):
-    """Max |gpu - ref| over max(|ref|, row max |ref|, batch RMS of ref).
-
-    The batch-RMS floor makes scalar outputs that cancel to near zero (a sum
-    of 64 terms of size ~0.1) be judged against their typical magnitude.
-    """
-    gpu = np.asarray(gpu, np.float64).reshape(ref.shape)
-    rms = np.sqrt(np.mean(ref[np.isfinite(ref)] ** 2)) if np.isfinite(ref).any() else 1.0
-    scale = np.maximum(np.abs(ref), np.max(np.abs(ref), axis=-1, keepdims=True))
-    scale = np.maximum(scale, rms)
-    scale = np.where(scale == 0, 1.0, scale)
-    return np.max(np.abs(gpu - ref) / scale)
+def check_grad(tape, x, y, g, tol=F64_TOL, what=""):
+    """Value and gradient of every point against the oracle at `tol`."""
+    x = np.asarray(x, np.float64).reshape(-1, tape.num_inputs)
+    y_ref, g_ref = po.gradient(tape, x)
+    mag = po.magnitudes(tape, x)
+    B = x.shape[0]
+    assert_f64(np.asarray(y).reshape(B, -1)[:, 0], y_ref[:, 0], mag["value"][:, 0], what + " value", tol)
+    assert_f64(np.asarray(g).reshape(B, -1), g_ref, mag["grad"], what + " gradient", tol)
 
 
 # ---------------------------------------------------------------------------
@@ -83,9 +80,7 @@ def test_rosenbrock_n16_f64(hip):
     x = wl.points_uniform(0, 1024, 16, -2.0, 2.0)
     bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
     y, g = hip.gradient_batch(bufs, x, 1024)
-    y_ref, g_ref = po.gradient(tape, x)
-    assert rowrel(y.reshape(-1, 1), y_ref) <= F64_TOL
-    assert rowrel(g.reshape(1024, 16), g_ref) <= F64_TOL
+    check_grad(tape, x, y, g)
 
 
 def test_rosenbrock_n16_f32(hip):
@@ -93,9 +88,7 @@ def test_rosenbrock_n16_f32(hip):
     x = wl.points_uniform(0, 1024, 16, -2.0, 2.0)
     bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
     y, g = hip.gradient_batch(bufs, x.astype(np.float32), 1024)
-    y_ref, g_ref = po.gradient(tape, x.astype(np.float32).astype(np.float64))
-    assert rowrel(y.reshape(-1, 1), y_ref) <= F32_TOL
-    assert rowrel(g.reshape(1024, 16), g_ref) <= F32_TOL
+    check_grad(tape, x.astype(np.float32), y, g, F32_TOL)
 
 
 @pytest.fixture(scope="module")
@@ -118,10 +111,8 @@ def test_mlp32_f64(hip, mlp32, fuse, wg, lds):
     finally:
         hip.set_fusion(True)
         hip.set_launch(0, 0)
-    y_ref, g_ref = po.gradient(mlp32, x)
     assert np.array_equal(y, yf)
-    assert rowrel(y.reshape(-1, 1), y_ref) <= F64_TOL
-    assert rowrel(g.reshape(B, 32), g_ref) <= F64_TOL
+    check_grad(mlp32, x, y, g)
 
 
 def test_mlp32_f32(hip, mlp32):
@@ -129,12 +120,11 @@ def test_mlp32_f32(hip, mlp32):
     x = wl.points_normal(1, B, 32).astype(np.float32)
     bufs = hip.upload_tape(GpuTapeData.from_tape(mlp32))
     y, g = hip.gradient_batch(bufs, x, B)
-    y_ref, g_ref = po.gradient(mlp32, x.astype(np.float64))
-    assert rowrel(y.reshape(-1, 1), y_ref) <= F32_TOL
-    assert rowrel(g.reshape(B, 32), g_ref) <= F32_TOL
+    check_grad(mlp32, x, y, g, F32_TOL)
     # same-precision CPU path
-    y32, g32 = po.gradient(mlp32, x, dtype=np.float32)
-    assert rowrel(g.reshape(B, 32), g32.astype(np.float64)) <= F32_TOL
+    _, g32 = po.gradient(mlp32, x, dtype=np.float32)
+    mag = po.magnitudes(mlp32, x.astype(np.float64))
+    assert_f64(g.reshape(B, 32), g32.astype(np.float64), mag["grad"], "f32 oracle", F32_TOL)
 
 
 def test_mlp32_full_batch_f32_device(hip, mlp32):
@@ -150,15 +140,8 @@ def test_mlp32_full_batch_f32_device(hip, mlp32):
     hip.gradient_batch_dev(bufs, dx, B, dy, dg)
     hip_sync = torch.cuda.synchronize
     hip_sync()
-    idx = np.arange(0, B, 997)
-    y_ref, g_ref = po.gradient(mlp32, x[idx].astype(np.float64))
-    g = dg.cpu().numpy()[idx]
-    y = dy.cpu().numpy()[idx]
-    assert rowrel(y.reshape(-1, 1), y_ref) <= F32_TOL
-    assert rowrel(g, g_ref) <= F32_TOL
-    # the last partial workgroup is handled
-    g_last_ref = po.gradient(mlp32, x[-3:].astype(np.float64))[1]
-    assert rowrel(dg.cpu().numpy()[-3:], g_last_ref) <= F32_TOL
+    idx = np.concatenate([np.arange(0, B, 997), [B - 3, B - 2, B - 1]])  # and the last partial workgroup
+    check_grad(mlp32, x[idx], dy.cpu().numpy()[idx], dg.cpu().numpy()[idx], F32_TOL)
 
 
 def test_random_dags(hip):
@@ -167,9 +150,7 @@ def test_random_dags(hip):
         x = wl.points_uniform(10 + k, 257, tape.num_inputs, -1.0, 1.0)
         bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
         y, g = hip.gradient_batch(bufs, x, 257)
-        y_ref, g_ref = po.gradient(tape, x)
-        assert rowrel(y.reshape(-1, 1), y_ref) <= 1e-11, k
-        assert rowrel(g.reshape(257, -1), g_ref) <= 1e-11, k
+        check_grad(tape, x, y, g, what=f"dag {k}")
 
 
 def test_multi_output_forward(hip):
@@ -179,11 +160,10 @@ def test_multi_output_forward(hip):
     bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
     assert hip.num_outputs(bufs) == 3
     y = hip.forward_batch(bufs, x, 3).reshape(3, 3)
-    y_ref = po.forward(tape, x)
-    assert rowrel(y, y_ref) <= F64_TOL
+    assert_f64(y, po.forward(tape, x), po.magnitudes(tape, x)["value"], "outputs")
     # gradient seeds output 0
-    _, g = hip.gradient_batch(bufs, x, 3)
-    assert rowrel(g.reshape(3, 2), po.gradient(tape, x)[1]) <= F64_TOL
+    y0, g = hip.gradient_batch(bufs, x, 3)
+    check_grad(tape, x, y0.reshape(3, 3)[:, 0], g)
 
 
 # ---------------------------------------------------------------------------

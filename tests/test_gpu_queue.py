@@ -12,7 +12,7 @@ import torch
 import pyoracle as po
 from echidna_amd import workloads as wl
 from echidna_amd.tape import GpuError, GpuTapeData
-from test_gpu_parity import F64_TOL, rowrel
+from test_gpu_parity import check_grad
 
 pytestmark = pytest.mark.gpu
 
@@ -73,9 +73,7 @@ def test_queue_matches_oracle_f64(hip):
                      torch.empty(B, n, dtype=torch.float64, device="cuda")))
     hip.multi_batch_dev(jobs)
     for (b, _, dx, B, dy, dg), x, tape in zip(jobs, xs, tapes):
-        y_ref, g_ref = po.gradient(tape, x[:64], dtype=np.float64)
-        assert rowrel(dy.cpu().numpy()[:64].reshape(-1, 1), y_ref.reshape(-1, 1)) <= F64_TOL
-        assert rowrel(dg.cpu().numpy()[:64], g_ref) <= F64_TOL
+        check_grad(tape, x[:64], dy.cpu().numpy()[:64], dg.cpu().numpy()[:64])
 
 
 def test_queue_single_point_and_empty(hip):

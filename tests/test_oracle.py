@@ -196,3 +196,30 @@ def test_taylor_orders_exp_series(K):
     t, _ = record(lambda v: v[0].exp(), [0.0])
     c = po.taylor(t, [0.0], [[1.0]], K=K)[0, 0]
     assert np.allclose(c, [1 / math.factorial(k) for k in range(K)], rtol=1e-15)
+
+
+def test_magnitudes_closed_form_mlp():
+    """The magnitude sweep on f = sum_j w2_j tanh(sum_i W1_ji x_i + b_j):
+    gradient terms |w2_j (1 - t_j^2) W1_ji| summed over j; H v terms
+    |w2_j| |2 t_j (1 - t_j^2)| |sum_i W1_ji v_i|^abs |W1_ji| with the inner
+    sum taken over absolute values; the value bound is at least |f|."""
+    tape = wl.tanh_mlp(5, 4, 3)
+    W1, b, w2 = wl.mlp_weights(5, 4, 3)
+    x = philox.normal(2, 1, np.arange(5))
+    v = philox.normal(2, 2, np.arange(5))
+    m = po.magnitudes(tape, x[None], v[None])
+    a = W1 @ x + b
+    t = np.tanh(a)
+    d1 = 1 - t * t
+    gmag = np.abs(w2 * d1) @ np.abs(W1)
+    assert np.allclose(m["grad"][0], gmag, rtol=1e-12)
+    tv = np.abs(W1) @ np.abs(v)
+    hmag = (np.abs(w2) * np.abs(2 * t * d1) * tv) @ np.abs(W1)
+    assert np.allclose(m["hv"][0], hmag, rtol=1e-12)
+    assert np.isclose(m["c1"][0], np.dot(np.abs(w2 * d1), tv), rtol=1e-12)
+    assert np.isclose(m["c2"][0], 0.5 * np.dot(np.abs(v), hmag), rtol=1e-12)
+    y = po.forward(tape, x[None])[0, 0]
+    assert m["value"][0, 0] >= abs(y)
+    _, g = po.gradient(tape, x[None])
+    _, hv = po.hvp(tape, x[None], v[None])
+    assert np.all(m["grad"][0] >= np.abs(g[0])) and np.all(m["hv"][0] >= np.abs(hv[0]) * (1 - 1e-12))

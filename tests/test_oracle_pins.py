@@ -317,3 +317,17 @@ def test_jet_of_x2_plus_y2():
     """tests/gpu_cuda_tests.rs:327-372: x^2+y^2 at (3,4) along (1,0)."""
     tape, _ = record(lambda v: v[0] * v[0] + v[1] * v[1], [3.0, 4.0])
     assert list(po.taylor(tape, [3.0, 4.0], [[1.0, 0.0]], K=3)[0, 0]) == [25.0, 6.0, 1.0]
+
+
+def test_powf_partial_at_zero_base():
+    """powf's reverse partial at a zero base takes b a^(b-1)
+    (src/opcode.rs:350-381): d/dx x^1.0 = 1 at x = 0, d/dx x^2.0 = 0, and
+    d/db 0^b = 0 for b > 0; the same through the Dual (HVP) path."""
+    t1 = record(lambda v: v[0].powf(C(1.0)), [0.5])[0]
+    t2 = record(lambda v: v[0].powf(C(2.0)), [0.5])[0]
+    tv = record(lambda v: v[0].powf(v[1]), [0.5, 1.0])[0]
+    assert po.gradient(t1, [[0.0]])[1][0, 0] == 1.0
+    assert po.gradient(t2, [[0.0]])[1][0, 0] == 0.0
+    assert list(po.gradient(tv, [[0.0, 1.0]])[1][0]) == [1.0, 0.0]
+    g, hv = po.hvp(t2, [[0.0]], [[1.0]])
+    assert g[0, 0] == 0.0 and hv[0, 0] == 2.0
