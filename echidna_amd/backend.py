@@ -115,6 +115,8 @@ _SIGS = {
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
     "ehip_tape_specialize": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
     "ehip_set_specialize": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
+    "ehip_gradient_batch_dev_noalloc": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp]),
+    "ehip_noalloc_scratch_bytes": (ctypes.c_uint64, []),
     "ehip_spec_error": (ctypes.c_char_p, [_vp, ctypes.c_int32]),
     "ehip_spec_source": (ctypes.c_int32, [ctypes.c_int32, _c_u32p, _c_u32p, _c_u32p, _c_f64p, ctypes.c_uint32,
                                           ctypes.c_uint32, ctypes.c_uint32, _c_u32p, ctypes.c_uint32, ctypes.c_int32,
@@ -473,6 +475,12 @@ class HipContext:
         _check(load_library().ehip_gradient_batch_dev(self.handle, tape.handle, _vp(d_inputs.data_ptr()), batch,
                                                        _vp(d_outputs.data_ptr()), _vp(d_grads.data_ptr()),
                                                        _vp(stream or None)))
+
+    def gradient_batch_dev_noalloc(self, tape, d_inputs, batch: int, d_outputs, d_grads, stream: int = 0) -> None:
+        """``gradient_batch_dev`` that allocates nothing (fixed per-GPU scratch)."""
+        _check(load_library().ehip_gradient_batch_dev_noalloc(self.handle, tape.handle, _vp(d_inputs.data_ptr()),
+                                                               batch, _vp(d_outputs.data_ptr()),
+                                                               _vp(d_grads.data_ptr()), _vp(stream or None)))
 
     def multi_batch_dev(self, jobs, wg: int = 0, lds_budget: int = 0, stream: int = 0) -> None:
         """Run many (tape, inputs, outputs) jobs in one persistent launch.

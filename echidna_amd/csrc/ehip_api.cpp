@@ -27,6 +27,9 @@ int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) try {
     return fail(EHIP_NO_DEVICE, "no suitable GPU device found (device %d is %s; the library targets gfx950)", device,
                 prop.gcnArchName);
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  // the allocation-free entry point's scratch on this device; a failure is
+  // reported by that entry point, the rest of the library does not need it
+  (void)noalloc_setup(device);
   ehip_ctx* c = new ehip_ctx();
   c->device = device;
   c->max_lds = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 65536;

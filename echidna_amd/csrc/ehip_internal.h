@@ -153,6 +153,7 @@ struct SpecKernel {
 constexpr uint32_t SPEC_MAX_ENTRIES = 1u << 16;
 
 namespace ehip_detail {
+int32_t noalloc_setup(int device);  // ehip_noalloc.hip: the fixed scratch's address on `device` (current)
 hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, uint32_t ppl, hipStream_t st);
 void spec_free(SpecKernel& s);
 }  // namespace ehip_detail
@@ -210,12 +211,38 @@ inline int tune_key(int kind, int K, bool grad) {
   return kind == 0 ? (grad ? 1 : 0) : (kind == 1 ? 2 : (kind == 3 ? 0 : 2 + K));
 }
 
+// HBM register-file bytes one lane of an evaluation needs, and the
+// workgroup size it launches with (0 bytes: the tape-specialised kernel, or
+// every register in LDS).  Mirrors the shape `run` picks.
+inline size_t regfile_lane_bytes(const ehip_ctx* c, const ehip_tape* t, int kind, int K, bool grad,
+                                 uint32_t* wg_out = nullptr) {
+  const DevProgram& dp = t->prog[prog_index(kind, grad)];
+  const uint32_t NC = (kind == 0 || kind == 3) ? 1u : (kind == 1 ? 2u : (uint32_t)K);
+  const size_t S = t->f64 ? 8 : 4;
+  const Tuned& tu = t->tuned[tune_key(kind, K, grad)];
+  const uint32_t wg = tu.wg ? tu.wg : c->wg;
+  if (wg_out) *wg_out = wg;
+  const int slot = kind == 3 ? -1 : tune_key(kind, K, grad);
+  const bool has_spec = slot >= 0 && t->spec[slot].fn != nullptr;
+  const uint32_t fm = tu.wg ? tu.fetch : (has_spec ? 2u : c->fetch);
+  if (fm == 2 && slot >= 0) return 0;
+  const size_t per_reg = (size_t)NC * wg * S;
+  const size_t win_lds = window_lds(fm == 1);
+  const size_t budget =
+      std::min<size_t>(tu.wg ? tu.lds_budget : c->lds_budget, c->max_lds > win_lds ? c->max_lds - win_lds : 0);
+  const uint32_t R = dp.host.nregs;
+  const uint32_t R_lds = (uint32_t)std::min<size_t>(R, budget / per_reg);
+  return (size_t)(R - R_lds) * NC * S;
+}
+
 // Launch one interpreter over `L.B` lanes.  kind 0 scalar, 1 dual, 2 jet(K).
 // `ws` (optional): a caller-owned HBM register file used instead of the
 // context's grow-only one (graphs keep their own so a later reallocation
-// cannot pull the memory from under a recorded launch).
+// cannot pull the memory from under a recorded launch).  `fixed`
+// (optional): device memory of `fixed_cap` bytes that holds the register
+// file instead; nothing is allocated and a launch that needs more fails.
 inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, hipStream_t s, const Tuned* force = nullptr,
-                   DevBuf* ws = nullptr) {
+                   DevBuf* ws = nullptr, void* fixed = nullptr, size_t fixed_cap = 0) {
   const DevProgram& dp = t->prog[prog_index(kind, L.g != nullptr)];
   L.prog = dp.d_stream;
   L.nwin = dp.nwin;
@@ -249,9 +276,16 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   uint64_t lanes = grid * wg;
   if (R > R_lds) {
     size_t bytes = (size_t)(R - R_lds) * NC * lanes * S;
-    DevBuf& rf = ws ? *ws : c->regs;
-    HIP_TRY(rf.reserve(bytes), "register file");
-    L.gregs = rf.p;
+    if (fixed) {
+      if (bytes > fixed_cap)
+        return fail(EHIP_OTHER, "GPU error: the register file needs %zu bytes, above the %zu of the fixed scratch", bytes,
+                    fixed_cap);
+      L.gregs = fixed;
+    } else {
+      DevBuf& rf = ws ? *ws : c->regs;
+      HIP_TRY(rf.reserve(bytes), "register file");
+      L.gregs = rf.p;
+    }
   } else {
     L.gregs = nullptr;
   }

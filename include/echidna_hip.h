@@ -235,6 +235,21 @@ int32_t ehip_autotune_dev(ehip_ctx* ctx, ehip_tape* tape, int32_t what, uint32_t
                           uint64_t x_stride, const void* d_dirs, uint64_t batch, void* d_y, void* d_g, void* d_h,
                           uint32_t* out5);
 
+/* Allocation-free ehip_gradient_batch_dev.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  Same arguments,
+ * results and launch shape as ehip_gradient_batch_dev, but the interpreters'
+ * HBM register file lives in a fixed device array of
+ * ehip_noalloc_scratch_bytes() bytes (64 MiB) per GPU, held until the
+ * process ends, and the call allocates nothing: batches whose register file
+ * exceeds it run as several launches of whole workgroups on `stream`.  A
+ * tape whose single workgroup needs more than the array is refused
+ * (EHIP_OTHER).  Every call on one GPU shares that GPU's array (across
+ * contexts too): calls that overlap on one GPU race; serialize them. */
+int32_t ehip_gradient_batch_dev_noalloc(ehip_ctx* ctx, const ehip_tape* tape, const void* d_inputs, uint64_t batch,
+                                        void* d_outputs, void* d_grads, void* stream);
+uint64_t ehip_noalloc_scratch_bytes(void);
+
 /* hipGraph capture of repeated evaluations (no counterpart in
  * the reference).  Records `repeats` launches of evaluation `what` (as for
  * ehip_autotune_dev) over fixed device buffers into one graph with its own
