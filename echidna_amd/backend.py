@@ -300,8 +300,10 @@ class HipTapeBuffers:
 
     def free(self) -> None:
         if self.handle:
-            load_library().ehip_tape_free(self.handle)
+            if self._ctx.handle:  # a tape must not outlive its context
+                load_library().ehip_tape_free(self.handle)
             self.handle = None
+            self._ctx._children.discard(self)
 
     def __del__(self):
         try:
@@ -355,8 +357,14 @@ class HipContext:
 
     def close(self) -> None:
         if self.handle:
-            for child in list(self._children):
-                child.close()
+            # graphs and networks first, then the tapes they use, then the context
+            children = list(self._children)
+            for child in children:
+                if not isinstance(child, HipTapeBuffers):
+                    child.close()
+            for child in children:
+                if isinstance(child, HipTapeBuffers):
+                    child.free()
             load_library().ehip_ctx_destroy(self.handle)
             self.handle = None
 
@@ -393,7 +401,11 @@ class HipContext:
                                   data.num_ops, data.num_inputs, data.num_variables, data.output_index,
                                   outs.ctypes.data_as(_c_u32p), len(outs), ctypes.byref(h))
         _check(rc)
-        return HipTapeBuffers(self, h, f64, int(lib.ehip_num_inputs(h)), int(lib.ehip_num_outputs(h)))
+        return self._adopt(HipTapeBuffers(self, h, f64, int(lib.ehip_num_inputs(h)), int(lib.ehip_num_outputs(h))))
+
+    def _adopt(self, tape: "HipTapeBuffers") -> "HipTapeBuffers":
+        self._children.add(tape)  # freed before the context
+        return tape
 
     upload_tape_f64 = upload_tape
 
@@ -403,7 +415,7 @@ class HipContext:
         lib = load_library()
         h = ctypes.c_void_p()
         _check(lib.ehip_tape_upload_json(self.handle, EHIP_F64 if f64 else EHIP_F32, raw, len(raw), ctypes.byref(h)))
-        return HipTapeBuffers(self, h, f64, int(lib.ehip_num_inputs(h)), int(lib.ehip_num_outputs(h)))
+        return self._adopt(HipTapeBuffers(self, h, f64, int(lib.ehip_num_inputs(h)), int(lib.ehip_num_outputs(h))))
 
     def num_outputs(self, tape: HipTapeBuffers) -> int:
         return int(load_library().ehip_num_outputs(tape.handle))

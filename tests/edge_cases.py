@@ -6,9 +6,11 @@ denominator, hypot with Inf and at large magnitudes, upload with empty
 output_indices, expm1/ln1p at small |a|, powi n=1 HVP at 0, negative-base
 power HVPs), tests/gpu_cuda_kernel_parity.rs:20-163 (atan2 at large
 magnitudes, asinh'/acosh' at 1e20, powf on a negative base, max with a NaN
-operand, fract of a negative input) and tests/gpu_stde.rs:1853-2011 (Taylor
+operand, fract of a negative input), tests/gpu_stde.rs:1853-2011 (Taylor
 fract at a negative input, hypot(t^2, 0), powi with a zero base and an
-exponent above the order, powf at a zero base).
+exponent above the order, powf at a zero base) and
+tests/tape_optimization.rs:560-650 (x/x, x*0, x-x and x+(+-0) recorded at a
+regular point and replayed at a singular one).
 
 Each case: (name, tape builder, kind, x, v, K, check).  `kind` is forward,
 gradient, hvp (one x, one direction) or jet (order K along v).  `check`
@@ -134,6 +136,22 @@ def _jet_powf_zero(c, dt):
     assert all(v == 0.0 for v in c[:3]) and all(np.isinf(v) and v > 0 for v in c[3:]), c
 
 
+def _nan_value_and_grad(r, dt):
+    assert np.isnan(r[0][0]) and np.isnan(r[1][0]), r
+
+
+def _nan(r, dt):
+    assert np.isnan(r[0]), r
+
+
+def _pos_zero(r, dt):
+    assert r[0] == 0.0 and not np.signbit(r[0]), r
+
+
+def _neg_zero(r, dt):
+    assert r[0] == 0.0 and np.signbit(r[0]), r
+
+
 CASES = [
     ("atan_grad_1e20", _t(lambda v: v[0].atan(), [1.0]), "gradient", [1e20], None, 0, _atan_large),
     ("atan_grad_3e9", _t(lambda v: v[0].atan(), [1.0]), "gradient", [3e9], None, 0, _atan_3e9),
@@ -161,6 +179,12 @@ CASES = [
      _jet_hypot_depth2),
     ("jet_powi_zero_base_10", _t(lambda v: v[0].powi(10), [0.0]), "jet", [0.0], [1.0], 5, _jet_powi_zero10),
     ("jet_powf_zero_base_2_5", _t(lambda v: v[0].powf(C(2.5)), [1.0]), "jet", [0.0], [1.0], 5, _jet_powf_zero),
+    # recorded identities replayed at singular points (tests/tape_optimization.rs:560-650)
+    ("div_self_replayed_at_0", _t(lambda v: v[0] / v[0], [2.0]), "gradient", [0.0], None, 0, _nan_value_and_grad),
+    ("mul_zero_replayed_at_inf", _t(lambda v: v[0] * 0.0, [3.0]), "forward", [INF], None, 0, _nan),
+    ("sub_self_replayed_at_inf", _t(lambda v: v[0] - v[0], [3.0]), "forward", [INF], None, 0, _nan),
+    ("add_pos_zero_replayed_at_neg_zero", _t(lambda v: v[0] + 0.0, [3.0]), "forward", [-0.0], None, 0, _pos_zero),
+    ("add_neg_zero_replayed_at_neg_zero", _t(lambda v: v[0] + (-0.0), [3.0]), "forward", [-0.0], None, 0, _neg_zero),
 ]
 
 
