@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--no-spec", action="store_true",
                     help="mlp, hvp: leave the tape-specialised kernel out (interpreters only)")
     ap.add_argument("--no-fuse", action="store_true")
+    ap.add_argument("--no-jet-mma", action="store_true",
+                    help="stde, cv: order-3 f32 jets on the exact per-term DENSE path instead of FP32 MFMA")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle error check")
     ap.add_argument("--headline-only", action="store_true",
                     help="default workload: only the fwd+grad headline (no f64/hvp/stde/cv keys)")
@@ -82,7 +84,23 @@ def parse():
     ap.add_argument("--tuples-log2", type=int, default=20, help="cv: training tuples = 2^k")
     ap.add_argument("--epochs", type=int, default=10, help="cv: training epochs")
     ap.add_argument("--dag-scale", type=float, default=1.0, help="dags: shrink lengths and point counts")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "echidna_amd", "tune_cache.json"),
+                    help="persisted autotuner shapes to restore (keyed by tape, dtype, evaluation, batch)")
+    ap.add_argument("--tune-cache-out", default="", help="write the cache here after the run (default: not written)")
+    ap.add_argument("--no-tune-cache", action="store_true", help="autotune every launch in this run")
     return ap.parse_args()
+
+
+CACHE = None  # echidna_amd.tunecache.TuneCache, set by main
+
+
+def tune(ctx, bufs, data, f64: bool, what: int, batch: int, d_x, order: int = 0, extra: str = "", **kw) -> dict:
+    """The launch shape for one evaluation: restored from the tune cache, or
+    autotuned (and recorded in it)."""
+    if CACHE is None:
+        return dict(ctx.autotune_dev(bufs, what, batch, d_x, order=order, **kw), cached=False)
+    key = CACHE.key(data, f64, what, order, batch) + extra
+    return CACHE.autotune(ctx, bufs, key, what, batch, d_x, order=order, **kw)
 
 
 def log(msg: str) -> None:
@@ -189,7 +207,8 @@ def run_fwdgrad(args, dtype: str, headline: bool) -> dict:
             ctx.set_fetch(args.fetch)
     if args.no_fuse:
         ctx.set_fusion(False)
-    bufs = ctx.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    data = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
+    bufs = ctx.upload_tape(data)
     spec = None
     if not args.no_spec:
         # straight-line kernel for this tape (hipRTC), built before any
@@ -219,7 +238,8 @@ def run_fwdgrad(args, dtype: str, headline: bool) -> dict:
         del src, dst
     tuned = None
     if not (args.wg or args.lds):
-        tuned = ctx.autotune_dev(bufs, TUNE_GRADIENT, B, dx, d_y=dy, d_g=dg)
+        tuned = tune(ctx, bufs, data, f64, TUNE_GRADIENT, B, dx, extra="" if spec and "error" not in spec else ":nospec",
+                     d_y=dy, d_g=dg)
         torch.cuda.synchronize()
     with torch.cuda.stream(stream):
         for _ in range(args.warmup):
@@ -300,7 +320,8 @@ def run_hvp(args, compact: bool = False) -> dict:
     f64 = args.dtype == "f64" and not compact
     tape = wl.tanh_mlp(n, H, 1)
     ctx = HipContext.new(0)
-    bufs = ctx.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    data = GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape)
+    bufs = ctx.upload_tape(data)
     dt = np.float64 if f64 else np.float32
     xh = wl.points_normal(1, B, n).astype(dt)
     vh = philox.normal(1, philox.STREAM_DIRS, np.arange(B * n)).reshape(B, n).astype(dt)
@@ -314,7 +335,8 @@ def run_hvp(args, compact: bool = False) -> dict:
             spec = ctx.specialize(bufs, TUNE_HVP)
         except GpuError as e:
             spec = {"error": str(e)}
-    tuned = ctx.autotune_dev(bufs, TUNE_HVP, B, x, d_g=g, d_h=hv, d_dirs=v)
+    tuned = tune(ctx, bufs, data, f64, TUNE_HVP, B, x, extra="" if spec and "error" not in spec else ":nospec",
+                 d_g=g, d_h=hv, d_dirs=v)
     dt_s, _ = timed(lambda: ctx.hvp_batch_dev(bufs, x, n, v, B, g, hv), max(1, args.steps))
     flop = costmodel.hvp_flops(tape)
     peak = FP64_PEAK if f64 else FP32_PEAK
@@ -375,7 +397,7 @@ def lap_rate(var: np.ndarray, lap: np.ndarray, t_sample: float) -> dict:
             "median_point_laplacians_per_s": float(1.0 / (np.median(sr) * t_sample))}
 
 
-def tune_jets(ctx, bufs, X, lanes: int) -> dict:
+def tune_jets(ctx, bufs, data, X, lanes: int, mma: bool) -> dict:
     """Autotune the order-3 jet launch (the STDE samples) on `lanes` lanes:
     rows of X repeated, Gaussian directions."""
     import torch
@@ -386,7 +408,8 @@ def tune_jets(ctx, bufs, X, lanes: int) -> dict:
     rows = X[torch.arange(lanes, device=X.device) % X.shape[0]].contiguous()
     dirs = torch.randn(lanes, n, dtype=X.dtype, device=X.device)
     coeff = torch.empty(lanes, bufs.num_outputs * 3, dtype=X.dtype, device=X.device)
-    res = ctx.autotune_dev(bufs, TUNE_TAYLOR, lanes, rows, d_y=coeff, d_dirs=dirs, order=3)
+    res = tune(ctx, bufs, data, X.dtype == torch.float64, TUNE_TAYLOR, lanes, rows, order=3,
+               extra=":mma" if mma else "", d_y=coeff, d_dirs=dirs)
     torch.cuda.synchronize()
     return res
 
@@ -453,12 +476,15 @@ def run_stde(args, compact: bool = False) -> dict:
     P = args.points or 65536
     tape = wl.tanh_mlp(n, H, 2)
     ctx = HipContext.new(0)
-    bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
+    ctx.set_jet_mma(not args.no_jet_mma)
+    data = GpuTapeData.from_tape(tape)
+    bufs = ctx.upload_tape(data)
     Xh = wl.points_normal(2, P, n).astype(np.float32)
     X = torch.from_numpy(Xh).cuda()
-    tuned = tune_jets(ctx, bufs, X, 1 << 20)
+    tuned = tune_jets(ctx, bufs, data, X, 1 << 20, not args.no_jet_mma)
     flop = costmodel.jet_flops(tape, 3)
     out = {"value": None, "unit": "Laplacians/s", "dtype": "f32 jets, f64 reduction",
+           "jet_dense": "exact per-term" if args.no_jet_mma else "fp32 mfma",
            "config": {"workload": f"plain GPU STDE, tanh-MLP tape n={n} H={H} seed 2 ({tape.counts()['N']} "
                                   f"computed entries), B={P} points, S={S}", "points": P, "S": S,
                       "launch": tuned}}
@@ -512,12 +538,14 @@ def run_cv(args, compact: bool = False) -> dict:
     tape = wl.tanh_mlp2(n, H, H, 4)
     t_record = time.perf_counter() - t0
     ctx = HipContext.new(0)
+    ctx.set_jet_mma(not args.no_jet_mma)
     t0 = time.perf_counter()
-    bufs = ctx.upload_tape(GpuTapeData.from_tape(tape))
+    data = GpuTapeData.from_tape(tape)
+    bufs = ctx.upload_tape(data)
     t_upload = time.perf_counter() - t0
     Xh = wl.points_normal(4, P, n).astype(np.float32)
     X = torch.from_numpy(Xh).cuda()
-    tuned = tune_jets(ctx, bufs, X, 1 << 16)
+    tuned = tune_jets(ctx, bufs, data, X, 1 << 16, not args.no_jet_mma)
     log(f"recorded {tape.counts()['N']} entries in {t_record:.1f} s, uploaded in {t_upload:.1f} s")
     dt_plain, plain = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, lcv.DIR_GAUSSIAN, 5), 1)
     log(f"plain STDE {dt_plain:.2f} s")
@@ -558,6 +586,7 @@ def run_cv(args, compact: bool = False) -> dict:
     zc = (res_cv[:k, 1] - exact) / np.maximum(res_cv[:k, 3], 1e-300)
     zp = (res_pl[:k, 1] - exact) / np.maximum(res_pl[:k, 3], 1e-300)
     out = {"value": float(lps_cv), "unit": "Laplacians/s", "dtype": "f32 jets, f64 reduction, bf16 MFMA CV",
+           "jet_dense": "exact per-term" if args.no_jet_mma else "fp32 mfma",
            "config": {"workload": f"tanh-MLP tape n={n} 2x{H} seed 4 ({tape.counts()['N']} computed entries), "
                                   f"B={P} points, S={S} Gaussian, CV {n}->256->256->{n}x{r} signed {r // 2}+{r // 2}",
                       "points": P, "S": S, "rank": r, "tuples": T, "epochs": args.epochs, "launch": tuned},
@@ -679,7 +708,20 @@ def run_dags(args) -> dict:
 
 
 def main() -> int:
+    global CACHE
     args = parse()
+    if not args.no_tune_cache:
+        from echidna_amd.tunecache import TuneCache
+
+        CACHE = TuneCache(args.tune_cache)
+    rc = run(args)
+    if CACHE is not None and args.tune_cache_out:
+        CACHE.path = args.tune_cache_out
+        CACHE.save()
+    return rc
+
+
+def run(args) -> int:
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(points shard by contiguous ranges across GPUs with no exchange)"}))

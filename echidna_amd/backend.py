@@ -115,6 +115,7 @@ _SIGS = {
                                            ctypes.c_uint64, _vp, _vp, _vp, _c_u32p]),
     "ehip_tape_specialize": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_uint32, _c_u32p]),
     "ehip_set_specialize": (ctypes.c_int32, [_vp, ctypes.c_uint32, ctypes.c_uint32]),
+    "ehip_set_jet_mma": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "ehip_gradient_batch_dev_noalloc": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp]),
     "ehip_noalloc_scratch_bytes": (ctypes.c_uint64, []),
     "ehip_spec_error": (ctypes.c_char_p, [_vp, ctypes.c_int32]),
@@ -383,6 +384,10 @@ class HipContext:
 
     def set_dense(self, enable: bool) -> None:
         _check(load_library().ehip_set_dense(self.handle, int(enable)))
+
+    def set_jet_mma(self, enable: bool) -> None:
+        """Order-3 f32 jets: DENSE blocks on FP32 MFMA (f32 rounding, not bit-identical)."""
+        _check(load_library().ehip_set_jet_mma(self.handle, int(enable)))
 
     def set_fusion(self, enable: bool) -> None:
         _check(load_library().ehip_set_fusion(self.handle, int(enable)))

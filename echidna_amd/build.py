@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libechidna_hip.so")
 
 SOURCES = ["ehip_api.cpp", "ehip_program.cpp", "ehip_kernels.hip", "ehip_stde.hip", "ehip_cv.hip",
-           "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp", "ehip_spec.cpp", "ehip_noalloc.hip"]
+           "ehip_cv_api.cpp", "ehip_sparse.cpp", "ehip_serde.cpp", "ehip_spec.cpp", "ehip_noalloc.hip", "ehip_jetgemm.hip"]
 # Headers whose text the library carries for hipRTC (tape-specialised kernels):
 # (file, C symbol).
 EMBED = [("ehip_math.h", "ehip_math_h_text"), ("philox.h", "philox_h_text")]

@@ -97,6 +97,15 @@ int32_t ehip_set_dense(ehip_ctx* c, int32_t enable) try {
   return exception_status();
 }
 
+int32_t ehip_set_jet_mma(ehip_ctx* c, int32_t enable) try {
+  if (!c) return fail(EHIP_OTHER, "GPU error: null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->jet_mma = enable != 0;
+  return ok();
+} catch (...) {
+  return exception_status();
+}
+
 int32_t ehip_set_fusion(ehip_ctx* c, int32_t enable) try {
   if (!c) return fail(EHIP_OTHER, "GPU error: null context");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -164,6 +173,15 @@ int32_t ehip_tape_upload(ehip_ctx* c, int32_t dtype, const uint32_t* opcodes, co
       return rc;
     }
   }
+  {
+    int32_t rc = upload_jet_segments(t->jseg, t->host, fuse, dense);
+    if (rc != EHIP_OK) {
+      for (auto& p : t->prog) free_program(p);
+      free_jet_segments(t->jseg);
+      delete t;
+      return rc;
+    }
+  }
   uint32_t smax, smask;
   {
     std::lock_guard<std::mutex> lk(c->mu);
@@ -202,6 +220,7 @@ void ehip_tape_free(ehip_tape* t) try {
   if (!t) return;
   if (t->ctx) (void)hipSetDevice(t->ctx->device);
   for (auto& p : t->prog) free_program(p);
+  free_jet_segments(t->jseg);
   for (auto& k : t->spec) spec_free(k);
   for (auto& k : t->retired) spec_free(k);
   delete t;
@@ -235,7 +254,7 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
                              int32_t mode, int32_t fuse, uint32_t* words, uint64_t words_cap, uint32_t* terms,
                              uint64_t terms_cap, uint32_t* pool, uint64_t pool_cap, uint64_t* out5) try {
   uint64_t* out4 = out5;
-  if (!out4 || mode < 0 || mode > 7 || mode == 4 || mode == 6 || fuse < 0 || fuse > 2 ||
+  if (!out4 || mode < 0 || mode > 15 || (mode & 5) == 4 || (mode & 9) == 9 || fuse < 0 || fuse > 2 ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)
@@ -257,7 +276,8 @@ int32_t ehip_compile_program(int32_t dtype, const uint32_t* opcodes, const uint3
   th.num_inputs = num_inputs;
   th.outputs = outs;
   th.f64 = dtype == EHIP_F64;
-  Program p = compile_program(th, (mode & 1) == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0, fuse == 1, (mode & 4) != 0);
+  Program p = compile_program(th, (mode & 1) == 0 ? PM_FORWARD : PM_GRADIENT, fuse != 0, fuse == 1, (mode & 4) != 0,
+                              (mode & 8) ? SIZE_MAX : 0);
   out4[0] = p.ninstr;
   out4[1] = p.nregs;
   out4[3] = p.nterms;

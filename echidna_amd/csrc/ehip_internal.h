@@ -117,6 +117,7 @@ struct ehip_ctx {
   uint32_t spec_max = 0, spec_mask = 0;  // specialise uploads of at most spec_max entries (ehip_set_specialize)
   bool fuse = true;
   bool dense = true;
+  bool jet_mma = false;  // f32 order-3 jets: DENSE blocks on FP32 MFMA (ehip_set_jet_mma)
   size_t max_lds = 65536;
 };
 
@@ -158,6 +159,20 @@ hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, uint32
 void spec_free(SpecKernel& s);
 }  // namespace ehip_detail
 
+// The f32 order-3 jet program cut for the matrix cores (ehip_set_jet_mma):
+// interpreter streams and batch-GEMM DENSE blocks in program order.
+struct JetSegDev {
+  bool gemm = false;
+  uint32_t* d_stream = nullptr;  // interpreter segment
+  uint32_t nwin = 0;
+  DenseGemm g{};                 // gemm segment (regs and lanes set per launch)
+};
+struct JetSegProgram {
+  Program host;                  // the forward program with whole-layer DENSE groups
+  std::vector<JetSegDev> segs;   // empty: no DENSE the matrix cores can run
+  uint32_t* d_pool = nullptr;
+};
+
 struct ehip_tape {
   ehip_ctx* ctx = nullptr;
   bool f64 = false;
@@ -169,6 +184,7 @@ struct ehip_tape {
   SpecKernel spec[8];  // tape-specialised kernels, once compiled, indexed like `tuned`
   std::vector<SpecKernel> retired;  // kernels replaced by a later build, unloaded with the tape
   std::string spec_error[8];  // why an upload-time specialisation failed (the interpreters run instead)
+  JetSegProgram jseg;         // f32 tapes: order-3 jets with DENSE blocks on the matrix cores
 };
 
 namespace ehip_detail {
@@ -190,6 +206,73 @@ inline void free_program(DevProgram& dp) {
   if (dp.d_pool) (void)hipFree(dp.d_pool);
   dp.d_stream = nullptr;
   dp.d_pool = nullptr;
+}
+
+inline void free_jet_segments(JetSegProgram& js) {
+  for (auto& sg : js.segs)
+    if (sg.d_stream) (void)hipFree(sg.d_stream);
+  if (js.d_pool) (void)hipFree(js.d_pool);
+  js.segs.clear();
+  js.d_pool = nullptr;
+}
+
+// Compile and upload the segmented order-3 jet program of an f32 tape: the
+// forward program with each layer's members in one DENSE, cut at every
+// DENSE the matrix cores can run (finite weights, at least 32 members, at
+// most four immediate terms).  A tape without one keeps js.segs empty.
+inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fuse, bool dense) {
+  js.segs.clear();
+  if (h.f64 || !fuse || !dense) return EHIP_OK;
+  js.host = compile_program(h, PM_FORWARD, fuse, dense, false, SIZE_MAX);
+  std::vector<JetSegment> cut = segment_jet_program(js.host);
+  bool any = false;
+  for (const JetSegment& c : cut) {
+    if (!c.gemm) continue;
+    uint32_t nimm = 0;
+    for (uint32_t p = 0; p < c.T; ++p)
+      if ((js.host.pool[c.off_regs + p] >> P_KIND_SHIFT) == T_IMM) ++nimm;
+    if (nimm <= 4) any = true;
+  }
+  if (!any) return EHIP_OK;
+  const std::vector<uint32_t>& pool = js.host.pool;
+  HIP_TRY(hipMalloc((void**)&js.d_pool, std::max<size_t>(pool.size(), 4) * 4), "program upload");
+  HIP_TRY(hipMemcpy(js.d_pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice), "program upload");
+  uint32_t i0 = 0;  // instructions not yet in a segment (a DENSE the cores cannot take stays interpreted)
+  auto flush = [&](uint32_t i1) -> int32_t {
+    if (i1 <= i0) return EHIP_OK;
+    JetSegDev d;
+    const std::vector<uint32_t> st = encode_stream(js.host, false, i0, i1);
+    d.nwin = (uint32_t)(st.size() / WIN_WORDS);
+    HIP_TRY(hipMalloc((void**)&d.d_stream, st.size() * 4), "program upload");
+    js.segs.push_back(d);
+    HIP_TRY(hipMemcpy(d.d_stream, st.data(), st.size() * 4, hipMemcpyHostToDevice), "program upload");
+    i0 = i1;
+    return EHIP_OK;
+  };
+  for (const JetSegment& c : cut) {
+    if (!c.gemm) continue;
+    DenseGemm g{};
+    g.pool = js.d_pool;
+    g.off_regs = c.off_regs;
+    g.off_w = c.off_w;
+    g.m = c.m;
+    g.T = c.T;
+    g.flags = c.flags;
+    for (uint32_t p = 0; p < c.T && g.nimm <= 4; ++p)
+      if ((pool[c.off_regs + p] >> P_KIND_SHIFT) == T_IMM) {
+        if (g.nimm < 4) g.imm[g.nimm] = p;
+        ++g.nimm;
+      }
+    if (g.nimm > 4) continue;
+    int32_t rc = flush(c.i0);
+    if (rc) return rc;
+    JetSegDev d;
+    d.gemm = true;
+    d.g = g;
+    js.segs.push_back(d);
+    i0 = c.i1;
+  }
+  return flush(js.host.ninstr);
 }
 
 // validate_batch_args (src/gpu/mod.rs:572-582)
@@ -264,6 +347,38 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
     return EHIP_OK;
   }
   const bool sf = fm == 1;
+  if (kind == 2 && K == 3 && !t->f64 && c->jet_mma && !t->jseg.segs.empty()) {
+    // every register in HBM: the launches hand their registers on
+    const uint64_t grid = (L.B + wg - 1) / wg, lanes = grid * wg;
+    const size_t bytes = (size_t)t->jseg.host.nregs * NC * lanes * S;
+    if (fixed) {
+      if (bytes > fixed_cap)
+        return fail(EHIP_OTHER, "GPU error: the register file needs %zu bytes, above the %zu of the fixed scratch", bytes,
+                    fixed_cap);
+      L.gregs = fixed;
+    } else {
+      DevBuf& rf = ws ? *ws : c->regs;
+      HIP_TRY(rf.reserve(bytes), "register file");
+      L.gregs = rf.p;
+    }
+    L.R_lds = 0;
+    L.pool = t->jseg.d_pool;
+    for (const JetSegDev& sg : t->jseg.segs) {
+      hipError_t e;
+      if (sg.gemm) {
+        DenseGemm g = sg.g;
+        g.regs = (float*)L.gregs;
+        g.lanes = lanes;
+        e = launch_dense_gemm(g, s);
+      } else {
+        L.prog = sg.d_stream;
+        L.nwin = sg.nwin;
+        e = launch_jet(false, sf, 3, L, wg, 0, s);
+      }
+      if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    }
+    return EHIP_OK;
+  }
   const size_t per_reg = (size_t)NC * wg * S;
   // LDS fetch: two instruction windows share the workgroup's LDS with the registers
   const size_t win_lds = window_lds(sf);

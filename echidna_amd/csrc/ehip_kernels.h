@@ -29,6 +29,21 @@ struct Launch {
   uint64_t seed;
 };
 
+// One DENSE of the segmented f32 order-3 jet program as a batch GEMM over
+// every lane (ehip_jetgemm.hip).  The register file is [reg][3][lanes]
+// floats, every register in HBM.
+struct DenseGemm {
+  const uint32_t* pool;     // the segmented program's DENSE pool (device)
+  uint32_t off_regs, off_w;  // term words then member registers; weights [T][m]
+  uint32_t m, T, flags;
+  uint32_t nimm;            // immediate terms (at most 4) and their term indices
+  uint32_t imm[4];
+  float* regs;
+  uint64_t lanes;           // lanes per register row (a multiple of 64)
+};
+
+hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s);
+
 // Per-point STDE reduction over S samples (f64 Welford/Chan).
 struct StdeReduce {
   const void* jets;       // [npoints*S][3] interleaved c0,c1,c2 (scalar output)

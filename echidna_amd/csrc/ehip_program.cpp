@@ -283,7 +283,7 @@ std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const u
   return "";
 }
 
-Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense, bool dual) {
+Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense, bool dual, size_t dense_chunk) {
   const uint32_t N = (uint32_t)t.opcodes.size();
   const uint32_t ni = t.num_inputs;
   const bool grad = (mode == PM_GRADIENT);
@@ -721,7 +721,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
     }
   }
 
-  if (fuse && dense) group_dense(prog, (int64_t)N, grad ? SIZE_MAX : DENSE_CHUNK, dual);
+  if (fuse && dense) group_dense(prog, (int64_t)N, grad ? SIZE_MAX : (dense_chunk ? dense_chunk : DENSE_CHUNK), dual);
 
   // ---- live ranges ----
   const size_t NV = 2 * (size_t)N;
@@ -893,6 +893,7 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
         // DENSE_REV: [m][T] (a member's weights for all adjoints together)
         bool prod_nz = true;
         bool in_ops = x.op == I_DENSE;
+        bool finite_w = x.op == I_DENSE;
         for (auto& tm : x.terms)
           if (tm.kind != T_IMM && !(tm.vid >= 0 && tm.vid < (int64_t)t.num_inputs)) in_ops = false;
         for (size_t a = 0; a < (x.op == I_DENSE ? T : m); ++a)
@@ -907,11 +908,11 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
             uint32_t lo = 0, hi = 0;
             put_imm(v, lo, hi);
             if ((t.f64 ? v : (double)(float)v) == 0.0) prod_nz = false;
-            if (!std::isfinite(t.f64 ? v : (double)(float)v)) in_ops = false;
+            if (!std::isfinite(t.f64 ? v : (double)(float)v)) in_ops = finite_w = false;
             out.pool.push_back(lo);
             if (t.f64) out.pool.push_back(hi);
           }
-        w[5] = (prod_nz ? D_ALL_PROD_NZ : 0u) | (in_ops ? D_INPUT_OPERANDS : 0u);
+        w[5] = (prod_nz ? D_ALL_PROD_NZ : 0u) | (in_ops ? D_INPUT_OPERANDS : 0u) | (finite_w ? D_FINITE_WEIGHTS : 0u);
         if (in_ops) {
           // the operands' input indices, for interpreters that read an
           // operand's value straight from the input row
@@ -948,7 +949,23 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
 }
 
 
-std::vector<uint32_t> encode_stream(const Program& p, bool f64) {
+std::vector<uint32_t> encode_stream(const Program& p, bool f64) { return encode_stream(p, f64, 0, p.ninstr); }
+
+std::vector<JetSegment> segment_jet_program(const Program& p) {
+  std::vector<JetSegment> out;
+  uint32_t i0 = 0;
+  for (uint32_t i = 0; i < p.ninstr; ++i) {
+    const uint32_t* w = &p.words[(size_t)i * NW];
+    if ((w[0] & 0xFFu) != I_DENSE || !(w[5] & D_FINITE_WEIGHTS) || w[1] < JET_GEMM_MIN_MEMBERS) continue;
+    if (i > i0) out.push_back(JetSegment{false, i0, i, 0, 0, 0, 0, 0});
+    out.push_back(JetSegment{true, i, i + 1, w[1], w[2], w[3], w[4], w[5]});
+    i0 = i + 1;
+  }
+  if (i0 < p.ninstr || out.empty()) out.push_back(JetSegment{false, i0, p.ninstr, 0, 0, 0, 0, 0});
+  return out;
+}
+
+std::vector<uint32_t> encode_stream(const Program& p, bool f64, uint32_t i_begin, uint32_t i_end) {
   const uint32_t IW = f64 ? 2u : 1u;
   const uint32_t TWW = 2 + IW;                     // words per inline term
   const uint32_t MAXLEN = std::min<uint32_t>(255u, WIN_WORDS - 1);
@@ -971,7 +988,7 @@ std::vector<uint32_t> encode_stream(const Program& p, bool f64) {
     out.insert(out.end(), w, w + len);
     used += len;
   };
-  for (uint32_t i = 0; i < p.ninstr; ++i) {
+  for (uint32_t i = i_begin; i < i_end; ++i) {
     const uint32_t* w = &p.words[(size_t)i * NW];
     const uint32_t op = w[0] & 0xFFu;
     const uint32_t fl = (w[0] >> 8) & 0xFFFFu;

@@ -126,6 +126,10 @@ constexpr uint32_t D_ALL_PROD_NZ = 1u;
 // the operands' coefficients of order >= 2 are +0 there.  w6 then holds the
 // pool offset of the T operands' input indices.
 constexpr uint32_t D_INPUT_OPERANDS = 2u;
+// DENSE w5: every weight (plain register terms: 1) is finite in the
+// program's dtype; the f32 jet interpreter may then run the block on FP32
+// MFMA (ehip_set_jet_mma).
+constexpr uint32_t D_FINITE_WEIGHTS = 4u;
 
 struct Program {
   std::vector<uint32_t> words;  // NW words per instruction
@@ -153,12 +157,30 @@ inline bool is_nonsmooth_op(uint32_t op) { return op == 10 || op == 11 || (op >=
 // split into continuation pieces.  I_WEND closes a window, I_END the program.
 constexpr uint32_t WIN_WORDS = 512;
 std::vector<uint32_t> encode_stream(const Program& p, bool f64);
+// Instructions [i_begin, i_end) only, as a stream of its own.
+std::vector<uint32_t> encode_stream(const Program& p, bool f64, uint32_t i_begin, uint32_t i_end);
+
+// The f32 order-3 jet program cut at every DENSE the matrix cores can run
+// (finite weights, at least JET_GEMM_MIN_MEMBERS members): those run as one
+// batch GEMM over every lane of the launch (ehip_set_jet_mma), the
+// instructions between them on the interpreter, with the whole register
+// file in HBM so that the launches hand their registers on.
+constexpr uint32_t JET_GEMM_MIN_MEMBERS = 32;
+struct JetSegment {
+  bool gemm;
+  uint32_t i0, i1;                         // interpreter: instructions [i0, i1)
+  uint32_t m, T, off_regs, off_w, flags;   // gemm: the DENSE's words 1..5
+};
+std::vector<JetSegment> segment_jet_program(const Program& p);
 
 // Build the program; `fuse` enables DOT fusion, `dense` the grouping of
 // fused dot products into DENSE / DENSE_REV.  Never fails on a validated tape.
 // `dual`: the program runs on the dual (HVP) interpreter, whose DENSE_REV
 // also needs each term's P_DEEP flag.
-Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense = true, bool dual = false);
+// `dense_chunk`: members per forward DENSE (0: the default 16; a whole
+// layer for the segmented jet program).
+Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense = true, bool dual = false,
+                        size_t dense_chunk = 0);
 
 // GpuTapeData::validate (src/gpu/mod.rs:397-564); returns "" when valid.
 std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const uint32_t* arg1,

@@ -175,7 +175,10 @@ int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
  * windows, dot terms inline) into `words`; `terms` is unused.  Every mode
  * writes the DENSE pool (operand lists and weights) into `pool`.  Adding 4
  * to a gradient mode compiles the HVP program of the dual interpreter
- * (DENSE_REV terms carry P_DEEP).  out5 =
+ * (DENSE_REV terms carry P_DEEP).  Adding 8 to a forward mode groups
+ * each layer's members into one DENSE, the program the f32 order-3 jets
+ * run cut into interpreter and batch-GEMM segments (ehip_set_jet_mma).
+ * out5 =
  * {instructions, registers, program or stream words, side-table entries T,
  * pool words}.  Returns EHIP_OTHER if a buffer is too small (out5 still
  * holds the sizes). */
@@ -323,6 +326,15 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
  * products sharing one operand list run as one register-blocked instruction
  * (ehip_program.h); results are identical with it off. */
 int32_t ehip_set_dense(ehip_ctx* ctx, int32_t enable);
+
+/* Order-3 Taylor jets of f32 tapes (the STDE samples): run each DENSE
+ * block whose weights are all finite on the FP32 matrix cores
+ * (v_mfma_f32_32x32x2_f32, a wave's 64 jets as the N dimension) instead of
+ * the exact per-term path (enable != 0; default off).  The sums are the
+ * same products in another order, fused, so results agree with the exact
+ * path to f32 rounding, not bit for bit; a wave whose operands hold an Inf
+ * or NaN takes the exact path.  No counterpart in the reference. */
+int32_t ehip_set_jet_mma(ehip_ctx* ctx, int32_t enable);
 
 /* Dot-product fusion for tapes uploaded after this call (default on).  Off
  * runs every tape entry as its own instruction; results are identical. */
