@@ -60,6 +60,7 @@ void ehip_ctx_destroy(ehip_ctx* c) try {
   for (auto& b : c->tup) b.release();
   c->cvx.release();
   c->cvl.release();
+  c->cvdirs.release();
   for (auto& b : c->io) b.release();
   delete c;
 } catch (...) {

@@ -437,6 +437,14 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
   const size_t Sb = t->f64 ? 8 : 4;
   const uint64_t lanes = npoints * (uint64_t)S;
   HIP_TRY(c->jets.reserve(lanes * 3 * Sb), "jet scratch");
+  // With a control variate the reduction needs every sample's direction
+  // again (u = L^T v): the jets write the directions they draw to a buffer
+  // that the reduction reads.
+  void* dirs = nullptr;
+  if (cv) {
+    HIP_TRY(c->cvdirs.reserve(lanes * t->nin * Sb), "directions");
+    dirs = c->cvdirs.p;
+  }
   Launch L{};
   L.nin = t->nin;
   L.nout = 1;
@@ -445,6 +453,7 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
   L.xstride = t->nin;
   L.S = S;
   L.rng = dir_kind;
+  L.dirs_out = dirs;
   L.seed = seed;
   L.y = c->jets.p;
   int32_t rc = run(c, t, 2, 3, L, s);
@@ -454,7 +463,8 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
   R.npoints = npoints;
   R.S = S;
   R.nin = t->nin;
-  R.rng = dir_kind;
+  R.rng = dirs ? RNG_NONE : dir_kind;
+  R.dirs = dirs;
   R.seed = seed;
   R.out = d_results;
   R.out_plain = d_plain_results;

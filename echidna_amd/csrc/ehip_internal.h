@@ -108,6 +108,7 @@ struct ehip_ctx {
   DevBuf qjobs, qtiles, qcount;  // multi-tape queue tables
   DevBuf tup[4];               // control-variate tuple scratch
   DevBuf cvx, cvl;             // control-variate inputs and factors
+  DevBuf cvdirs;               // control-variate pass: the samples' directions
   uint32_t ncu = 256;
   // untuned launches: 256-lane workgroups and 24 KB of register LDS, the
   // shape the autotuner picks for the MLP fwd+grad and order-3 jet programs

@@ -27,6 +27,7 @@ struct Launch {
   uint32_t S;             // lanes per x row (0: one row per lane)
   uint32_t rng;           // RNG_NONE: read dirs; else generate (philox.h)
   uint64_t seed;
+  void* dirs_out;         // jets drawing directions also write them here [B][nin] (or null)
 };
 
 // One DENSE of the segmented f32 order-3 jet program as a batch GEMM over

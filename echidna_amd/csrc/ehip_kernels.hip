@@ -1121,8 +1121,15 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
         if (act) {
           const uint64_t row = L.S ? p / L.S : p;
           v[0] = X[row * L.xstride + i];
-          if (K > 1)
-            v[K > 1 ? 1 : 0] = L.rng ? (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i) : V[p * L.nin + i];
+          if (K > 1) {
+            if (L.rng) {
+              const F d = (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i);
+              v[1] = d;
+              if (L.dirs_out) reinterpret_cast<F*>(L.dirs_out)[p * L.nin + i] = d;
+            } else {
+              v[1] = V[p * L.nin + i];
+            }
+          }
         }
       });
     } else if (op == I_STORE_OUT) {

@@ -96,7 +96,8 @@ struct Gen {
 // Taylor<F, K>, src/bytecode_tape/tangent.rs:97-147, as taylor_forward_kth
 // does per lane): lane p seeds input i with (x_i, v_i, 0, ..) where x is row
 // p / S of X (S = 0: row p) and v is read from V or drawn on the device
-// (philox.h lane_direction), the interpreters' launch convention.
+// (philox.h lane_direction, also written to DO when that is not null), the
+// interpreters' launch convention.
 std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
   Gen g(h);
   const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
@@ -109,7 +110,7 @@ std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
   g.put("typedef ehip::Jet<F, %d> J;\n", K);
   g.put("extern \"C\" __global__ void __launch_bounds__(256) EHIP_OCC ehip_spec(const F* __restrict__ X, "
         "unsigned long long xs, const F* __restrict__ V, F* __restrict__ Y, unsigned long long B, unsigned S, "
-        "unsigned rng, unsigned long long seed) {\n");
+        "unsigned rng, unsigned long long seed, F* __restrict__ DO) {\n");
   // points per lane: a grid-stride loop over the points (the launch sizes
   // the grid)
   g.put("for (unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; p < B;\n"
@@ -117,7 +118,8 @@ std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
   g.put("const F* __restrict__ x = X + (S ? p / S : p) * xs;\n");
   g.put("auto seed_jet = [&](unsigned i) {\n  J r = ehip::jc<F, %d>(x[i]);\n", K);
   if (K > 1)
-    g.put("  r.c[1] = rng ? (F)ehip::lane_direction(rng, seed, S, p, %uu, i) : V[p * %uull + i];\n", n, n);
+    g.put("  if (rng) {\n    r.c[1] = (F)ehip::lane_direction(rng, seed, S, p, %uu, i);\n"
+          "    if (DO) DO[p * %uull + i] = r.c[1];\n  } else {\n    r.c[1] = V[p * %uull + i];\n  }\n", n, n, n);
   g.put("  return r;\n};\n");
   for (uint32_t i = 0; i < N; ++i) {
     const uint32_t op = h.opcodes[i], a = h.arg0[i], b = h.arg1[i];
@@ -349,9 +351,11 @@ hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, uint32
   unsigned long long B = L.B;
   unsigned S = L.S, rng = L.rng;
   unsigned long long seed = L.seed;
+  void* DO = L.dirs_out;
   void* args_fwd[] = {(void*)&X, (void*)&xs, (void*)&Y, (void*)&G, (void*)&B};
   void* args_hvp[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&G, (void*)&H, (void*)&B};
-  void* args_jet[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&Y, (void*)&B, (void*)&S, (void*)&rng, (void*)&seed};
+  void* args_jet[] = {(void*)&X, (void*)&xs, (void*)&V, (void*)&Y, (void*)&B, (void*)&S, (void*)&rng, (void*)&seed,
+                      (void*)&DO};
   const uint64_t per = (uint64_t)wg * (ppl ? ppl : 1u);  // points per workgroup
   const uint64_t grid = (L.B + per - 1) / per;
   if (grid == 0 || grid > 0xFFFFFFFFull) return hipErrorInvalidConfiguration;

@@ -169,9 +169,113 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
   }
 }
 
+// The learned control variate with the directions in memory: per point,
+// U = V L (S samples x n coordinates times n x r) on the FP64 matrix cores,
+// 16 samples x 16 columns per v_mfma_f64_16x16x4f64 tile.  Lane l supplies
+// A[i = l % 16][k = l / 16] and B[k = l / 16][j = l % 16] and holds
+// D[l / 16 + 4 v][l % 16], v = 0..3, so the 16 lanes of a lane group hold
+// one row's columns and q_s = sum_j +-U[s][j]^2 is a shuffle sum inside the
+// group.  Step t of a 16-coordinate slab gives lane group g coordinate
+// 4 g + t, so a lane's four A operands are consecutive in its sample's row.
+// The products of FP32 (or FP64) operands are formed and summed in binary64,
+// as the lane-per-sample form does, in another order.  One wave per point,
+// four points per workgroup; |L|_F^2 is summed from the same loads.
+template <class F>
+__device__ __forceinline__ void load4(const F* p, uint32_t cnt, bool vec, double a[4]) {
+  if (cnt >= 4 && vec) {
+    if constexpr (sizeof(F) == 4) {
+      const float4 x = *reinterpret_cast<const float4*>(p);
+      a[0] = x.x, a[1] = x.y, a[2] = x.z, a[3] = x.w;
+    } else {
+      const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
+      a[0] = x.x, a[1] = x.y, a[2] = y.x, a[3] = y.y;
+    }
+    return;
+  }
+#pragma unroll
+  for (uint32_t t = 0; t < 4; ++t) a[t] = t < cnt ? (double)p[t] : 0.0;
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <class F>
+__global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
+  const uint32_t l = threadIdx.x & 63u, g = l >> 4, i16 = l & 15u;
+  const uint64_t pt = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (pt >= R.npoints) return;  // whole waves: nothing below synchronises the workgroup
+  const F* J = reinterpret_cast<const F*>(R.jets);
+  const F* D = reinterpret_cast<const F*>(R.dirs);
+  const float* LC = R.lcv + pt * (uint64_t)R.nin * R.rank;
+  const uint32_t n = R.nin, r = R.rank, nrb = (r + 15u) / 16u;
+  const bool vec = (n % 4u) == 0u && (reinterpret_cast<uintptr_t>(D) % 16u) == 0u;
+  double lnorm = 0.0;
+  Wf w{0.0, 0.0, 0.0}, wp{0.0, 0.0, 0.0};
+  for (uint32_t s0 = 0; s0 < R.S; s0 += 16u) {
+    const uint32_t sa = s0 + i16;  // the A row this lane supplies
+    const F* Vrow = D + (pt * R.S + (sa < R.S ? sa : 0u)) * (uint64_t)n;
+    f64x4 acc[2] = {f64x4{0.0, 0.0, 0.0, 0.0}, f64x4{0.0, 0.0, 0.0, 0.0}};
+    for (uint32_t k0 = 0; k0 < n; k0 += 16u) {
+      const uint32_t c0 = k0 + 4u * g;  // this lane group's first coordinate
+      const uint32_t cnt = c0 < n ? min(4u, n - c0) : 0u;
+      double a[4];
+      load4<F>(Vrow + c0, sa < R.S ? cnt : 0u, vec, a);
+#pragma unroll
+      for (uint32_t rb = 0; rb < 2; ++rb) {
+        if (rb >= nrb) break;
+        const uint32_t j = rb * 16u + i16;
+#pragma unroll
+        for (uint32_t t = 0; t < 4; ++t) {
+          const double b = (t < cnt && j < r) ? (double)LC[(uint64_t)(c0 + t) * r + j] : 0.0;
+          if (s0 == 0) lnorm += j < R.npos ? b * b : -(b * b);
+          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, acc[rb], 0, 0, 0);
+        }
+      }
+    }
+    // q for samples s0 + g + 4 v: signed squares summed over the 16 columns of the group
+    double qv[4];
+#pragma unroll
+    for (uint32_t v = 0; v < 4; ++v) {
+      double e = 0.0;
+#pragma unroll
+      for (uint32_t rb = 0; rb < 2; ++rb) {
+        const uint32_t j = rb * 16u + i16;
+        const double u = acc[rb][v];
+        e += (j < R.npos) ? u * u : -(u * u);
+      }
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) e += __shfl_xor(e, off, 64);
+      qv[v] = e;
+    }
+    if (s0 == 0)
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) lnorm += __shfl_xor(lnorm, off, 64);
+    // lane (g, v) of the group's first four lanes takes sample s0 + g + 4 v
+    const uint32_t s = s0 + g + 4u * i16;
+    if (i16 < 4u && s < R.S) {
+      const double q = i16 == 0 ? qv[0] : (i16 == 1 ? qv[1] : (i16 == 2 ? qv[2] : qv[3]));
+      const uint64_t row = pt * R.S + s;
+      double sample = 2.0 * (double)J[row * 3 + 2];
+      if (R.sample_scale != 0.0) sample *= R.sample_scale;
+      welford_add(wp, sample);
+      welford_add(w, sample - (q - lnorm));
+    }
+  }
+  w = wave_merge(w);
+  if (R.out_plain) wp = wave_merge(wp);
+  if (l == 0) {
+    const double value = (double)J[(pt * R.S) * 3 + 0];
+    finalize(w, value, 0.0, R.out + pt * 5);
+    if (R.out_plain) finalize(wp, value, 0.0, R.out_plain + pt * 5);
+  }
+}
+
 hipError_t launch_stde_reduce(bool f64, const StdeReduce& R, hipStream_t s) {
   if (R.npoints == 0) return hipSuccess;
   void* args[] = {const_cast<StdeReduce*>(&R)};
+  if (R.lcv && !R.diag && R.rng == RNG_NONE && R.dirs && R.rank <= 32u) {
+    const void* fn = f64 ? (const void*)k_stde_reduce_lcv<double> : (const void*)k_stde_reduce_lcv<float>;
+    return hipLaunchKernel(fn, dim3((uint32_t)((R.npoints + 3) / 4)), dim3(256), args, 0, s);
+  }
   const void* fn = f64 ? (const void*)k_stde_reduce<double> : (const void*)k_stde_reduce<float>;
   return hipLaunchKernel(fn, dim3((uint32_t)R.npoints), dim3(64), args, 0, s);
 }

@@ -196,6 +196,62 @@ def test_controlled_estimator_is_unbiased(hip):
     assert np.array_equal(res[:, 0], plain[:, 0])
 
 
+def _welford(x):
+    return np.mean(x), np.var(x, ddof=1) if len(x) > 1 else 0.0
+
+
+@pytest.mark.parametrize("f64", [False, True], ids=["f32", "f64"])
+@pytest.mark.parametrize("n,rank,npos,S", [(40, 12, 5, 20), (64, 16, 8, 16), (36, 24, 24, 37), (33, 32, 10, 3)])
+def test_controlled_statistics_match_a_host_restatement(hip, f64, n, rank, npos, S):
+    """Per-point statistics of the controlled samples 2 c2 - (v^T L S L^T v -
+    tr(L S L^T)) against numpy in binary64 from the same pieces: the jets of
+    the same points and Philox directions (one GPU evaluation), L(x) from the
+    network's forward pass, the Welford mean and sample variance.  Shapes
+    straddle the 16-sample x 16-column matrix tile and the 16-coordinate slab
+    (partial tiles, two column blocks, n not a multiple of 4, S below 4).
+    Bar: the two sides sum the same binary64 products in different orders,
+    1e-11 of the samples' magnitude for the mean, 1e-9 relative for the
+    variance, widened for f32 tapes by 1e-6 of the jets' part in case the
+    separate jet evaluation rounds a coefficient differently; a swapped row
+    or column of the tile gives O(1) errors."""
+    tape = wl.tanh_mlp(n, 24, 70 + n)
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
+    net = lcv.LearnedControlVariate(hip, n, rank, 32, seed=n, out_scale=1.0)
+    net.set_signs(npos)
+    P, seed = 10, 13
+    dt = np.float64 if f64 else np.float32
+    tdt = torch.float64 if f64 else torch.float32
+    X = wl.points_normal(31, P, n).astype(dt)
+    dX = torch.from_numpy(X).cuda()
+    res, plain = lcv.laplacian_batch(hip, bufs, dX, S, lcv.DIR_GAUSSIAN, seed, net, with_plain=True)
+    V = philox.normal(seed, philox.STREAM_DIRS, np.arange(P * S * n, dtype=np.uint64)).reshape(P * S, n).astype(dt)
+    J = np.empty((P * S, 3), dt)
+    dJ = torch.empty(P * S, 3, dtype=tdt, device="cuda")
+    hip.taylor_kth_batch_dev(bufs, torch.from_numpy(np.repeat(X, S, axis=0)).cuda(), n,
+                             torch.from_numpy(V).cuda(), P * S, 3, dJ)
+    torch.cuda.synchronize()
+    J[:] = dJ.cpu().numpy()
+    L = net.forward(dX.float()).cpu().numpy().astype(np.float64).reshape(P, n, rank)
+    sign = np.where(np.arange(rank) < npos, 1.0, -1.0)
+    for p in range(P):
+        v = V[p * S:(p + 1) * S].astype(np.float64)
+        u = v @ L[p]
+        q = (u * u) @ sign
+        lnorm = float(np.sum(L[p] * L[p] * sign))
+        base = 2.0 * J[p * S:(p + 1) * S, 2].astype(np.float64)
+        samples = base - (q - lnorm)
+        mag = float(np.mean(np.abs(base) + np.abs(q) + abs(lnorm)))
+        m, var = _welford(samples)
+        pm, pvar = _welford(base)
+        tm = 1e-11 * mag + (0.0 if f64 else 1e-6 * float(np.mean(np.abs(base))))
+        tv = 1e-9 if f64 else 1e-6
+        assert res[p, 4] == S and plain[p, 4] == S
+        assert abs(res[p, 1] - m) <= tm, (p, res[p], m, mag)
+        assert abs(res[p, 2] - var) <= tv * var + 1e-20 * mag * mag, (p, res[p], var)
+        assert abs(plain[p, 1] - pm) <= tm and abs(plain[p, 2] - pvar) <= tv * pvar + 1e-20 * mag * mag
+        assert res[p, 0] == plain[p, 0] == J[p * S, 0]
+
+
 def test_training_removes_variance_on_a_low_rank_hessian(hip):
     """softplus ridge with 4 units: H(x) has rank 4 and is PSD, so a rank-8
     L L^T can absorb it; the trained control variate keeps a small fraction
