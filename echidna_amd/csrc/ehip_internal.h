@@ -370,6 +370,8 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
         DenseGemm g = sg.g;
         g.regs = (float*)L.gregs;
         g.lanes = lanes;
+        // c0 is the primal: equal on every lane that reads the same x row
+        g.share = L.xstride == 0 ? 64u : (L.S >= 2 ? L.S : 0u);
         e = launch_dense_gemm(g, s);
       } else {
         L.prog = sg.d_stream;

@@ -7,13 +7,21 @@
 //     C_q[j][lane] = sum_p W[p][j] X_q[p][lane]
 // (members x terms times terms x lanes), with X_q[p] the row of term p's
 // operand register.  A workgroup owns 64 lanes and 128 members: it stages
-// 32 terms x 64 lanes of every coefficient in LDS, and each of its four
+// 16 terms x 64 lanes of every coefficient in LDS, and each of its four
 // waves multiplies its 32 members' weights into them with
 // v_mfma_f32_32x32x2_f32 (two terms a step, FP32 accumulation; lane l
 // supplies A[j = l % 32][k = l / 32] and B[k = l / 32][n = l % 32], and holds
-// the results of rows 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).
-// Immediate terms add their weight to c0; operands that are tape inputs
-// (D_INPUT_OPERANDS) carry +0 from order 2 on, and so do the results.
+// the results of rows 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).  The
+// next stage's operands (and the term descriptors of the one after) are
+// loaded into registers while the matrix cores work on the current one.  Immediate terms add their weight to c0;
+// operands that are tape inputs (D_INPUT_OPERANDS) carry +0 from order 2 on,
+// and so do the results.
+//
+// c0 is the primal, equal on every lane that reads the same x row.  When
+// the lanes of a 64-lane tile fall into PT such groups (`share` lanes each,
+// aligned to the tile), c0 is a PT-column product on the vector ALUs, fused
+// into the same pass, and is broadcast to the group's lanes; the matrix
+// cores then form only c1 (and c2).
 //
 // These are the products of the interpreter's exact path (0 + x c per term
 // when x is finite) summed in another order and fused, so results agree
@@ -34,77 +42,154 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr uint32_t NC = 3;    // coefficients per register (order 3)
 constexpr uint32_t MT = 128;  // members per workgroup: four waves of 32
 constexpr uint32_t NT = 64;   // lanes per workgroup
-constexpr uint32_t KC = 32;   // terms per LDS stage
+constexpr uint32_t KC = 16;   // terms per stage
+constexpr uint32_t RPT = KC / 4;  // stage rows per wave
 
 __device__ __forceinline__ float* row_of(const DenseGemm& G, uint32_t reg, uint32_t q) {
   return G.regs + ((uint64_t)reg * NC + q) * G.lanes;
 }
 
-template <uint32_t NZ>
-__global__ __launch_bounds__(256) void k_dense_gemm(DenseGemm G) {
-  __shared__ float Bs[NZ][KC][NT];
-  __shared__ uint32_t bad[2];  // tile lanes (0..31, 32..63) with a non-finite operand coefficient
-  const uint32_t tid = threadIdx.x, l = tid & 63u, w = tid >> 6, hi = l >> 5, n = l & 31u;
+// NZ: operand coefficients that can be non-zero (2 when the operands are
+// tape inputs).  PT: c0 groups per tile (0: c0 on the matrix cores too).
+// Three waves per SIMD: accumulators in VGPRs, no spills (four spill).
+template <uint32_t NZ, uint32_t PT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_dense_gemm(DenseGemm G) {
+  constexpr uint32_t Q0 = PT ? 1u : 0u;  // first coefficient on the matrix cores
+  constexpr uint32_t NQ = NZ - Q0;
+  constexpr uint32_t PTS = PT ? PT : 1u;
+  constexpr uint32_t LPP = NT / PTS;  // lanes per c0 group inside the tile
+  __shared__ float Bs[NQ][KC][NT];
+  __shared__ float As[4][KC][32];  // each wave's weights: [term][member]
+  __shared__ float C0s[KC][PTS];   // distinct c0 operands of the stage
+  __shared__ float C0r[MT][PTS];   // c0 results
+  __shared__ uint32_t bad[2];      // tile lanes (0..31, 32..63) with a non-finite operand coefficient
+  const uint32_t tid = threadIdx.x, l = tid & 63u, hi = l >> 5, n = l & 31u;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   const uint64_t lane0 = (uint64_t)blockIdx.x * NT;
   const uint32_t m = G.m, T = G.T;
   const uint32_t* PR = G.pool + G.off_regs;
   const float* PW = reinterpret_cast<const float*>(G.pool + G.off_w);
   const uint32_t jw = blockIdx.y * MT + w * 32u;  // this wave's first member
+  const uint32_t j = jw + n;                      // the member whose weights this lane supplies
   const bool live = jw < m;
   if (tid < 2) bad[tid] = 0u;
-  f32x16 acc[NZ][2];
+  f32x16 acc[NQ][2];
 #pragma unroll
-  for (uint32_t q = 0; q < NZ; ++q) acc[q][0] = acc[q][1] = f32x16{};
-  for (uint32_t k0 = 0; k0 < T; k0 += KC) {
-    __syncthreads();
-    // wave w stages terms w, w + 4, ..: each a row segment of 64 lanes
-    for (uint32_t i = w; i < KC; i += 4) {
-      const uint32_t p = k0 + i;
-      float v[NZ];
+  for (uint32_t q = 0; q < NQ; ++q) acc[q][0] = acc[q][1] = f32x16{};
+  float c0acc[PTS];
 #pragma unroll
-      for (uint32_t q = 0; q < NZ; ++q) v[q] = 0.f;
-      if (p < T) {
-        const uint32_t kr = PR[p];
-        if ((kr >> P_KIND_SHIFT) != T_IMM) {
-          bool fin = true;
+  for (uint32_t t = 0; t < PTS; ++t) c0acc[t] = 0.f;
+
+  // Stage registers.  Wave w stages terms w RPT .. w RPT + RPT - 1, lane l
+  // of each, and its own members' weights for the stage's KC terms.
+  // The descriptors travel one stage ahead of the rows: lane r < RPT of the
+  // wave holds term w RPT + r's word of the stage after the one loading.
+  float sb[RPT][NQ], sc[RPT], sa[KC / 2];
+  uint32_t dsc;
+  auto load_desc = [&](uint32_t k0) {
+    const uint32_t p = k0 + w * RPT + l;
+    dsc = (l < RPT && p < T) ? PR[p] : (T_IMM << P_KIND_SHIFT);
+  };
+  auto load_stage = [&](uint32_t k0) {
 #pragma unroll
-          for (uint32_t q = 0; q < NZ; ++q) {
-            v[q] = row_of(G, kr & P_REG_MASK, q)[lane0 + l];
-            fin = fin && isfinite(v[q]);
-          }
-          if (!fin) atomicOr(&bad[hi], 1u << n);
-        }
+    for (uint32_t r = 0; r < RPT; ++r) {
+#pragma unroll
+      for (uint32_t q = 0; q < NQ; ++q) sb[r][q] = 0.f;
+      sc[r] = 0.f;
+      const uint32_t kr = __builtin_amdgcn_readlane(dsc, r);
+      if ((kr >> P_KIND_SHIFT) != T_IMM) {
+        const uint32_t reg = kr & P_REG_MASK;
+#pragma unroll
+        for (uint32_t q = 0; q < NQ; ++q) sb[r][q] = row_of(G, reg, Q0 + q)[lane0 + l];
+        if (PT && l < PTS) sc[r] = row_of(G, reg, 0)[lane0 + l * LPP];
       }
-#pragma unroll
-      for (uint32_t q = 0; q < NZ; ++q) Bs[q][i][l] = v[q];
     }
-    __syncthreads();
-    if (live) {
-      const uint32_t j = jw + n;
-#pragma unroll 4
-      for (uint32_t kk = 0; kk < KC / 2; ++kk) {
-        const uint32_t k = 2 * kk + hi, p = k0 + k;
-        const float a = (p < T && j < m) ? PW[(uint64_t)p * m + j] : 0.f;
 #pragma unroll
-        for (uint32_t q = 0; q < NZ; ++q) {
+    for (uint32_t kk = 0; kk < KC / 2; ++kk) {
+      const uint32_t p = k0 + 2u * kk + hi;
+      sa[kk] = (live && p < T && j < m) ? PW[(uint64_t)p * m + j] : 0.f;
+    }
+  };
+  auto store_stage = [&]() {
+#pragma unroll
+    for (uint32_t r = 0; r < RPT; ++r) {
+      const uint32_t i = w * RPT + r;
+      bool fin = true;
+#pragma unroll
+      for (uint32_t q = 0; q < NQ; ++q) {
+        Bs[q][i][l] = sb[r][q];
+        fin = fin && isfinite(sb[r][q]);
+      }
+      if (!fin) atomicOr(&bad[hi], 1u << n);
+      if (PT && l < PTS) {
+        C0s[i][l] = sc[r];
+        if (!isfinite(sc[r]))  // the whole group
+          for (uint32_t t = l * LPP; t < (l + 1) * LPP; ++t) atomicOr(&bad[t >> 5], 1u << (t & 31u));
+      }
+    }
+#pragma unroll
+    for (uint32_t kk = 0; kk < KC / 2; ++kk) As[w][2u * kk + hi][n] = sa[kk];
+  };
+
+  load_desc(0);
+  load_stage(0);
+  load_desc(KC);
+  for (uint32_t k0 = 0; k0 < T; k0 += KC) {
+    __syncthreads();  // the previous stage's LDS reads are done
+    store_stage();
+    __syncthreads();
+    if (k0 + KC < T) {  // in flight during the products below
+      load_stage(k0 + KC);
+      load_desc(k0 + 2 * KC);
+    }
+    if (live) {
+#pragma unroll
+      for (uint32_t kk = 0; kk < KC / 2; ++kk) {
+        const uint32_t k = 2u * kk + hi;
+        const float a = As[w][k][n];
+#pragma unroll
+        for (uint32_t q = 0; q < NQ; ++q) {
           acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, Bs[q][k][n], acc[q][0], 0, 0, 0);
           acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, Bs[q][k][32 + n], acc[q][1], 0, 0, 0);
+        }
+        if (PT) {
+#pragma unroll
+          for (uint32_t t = 0; t < PTS; ++t) c0acc[t] = fmaf(a, C0s[k][t], c0acc[t]);
         }
       }
     }
   }
-  __syncthreads();  // bad[] complete
+  if (PT) {
+    // the two term parities meet; lanes 0..31 hold member jw + n's c0 per group
+#pragma unroll
+    for (uint32_t t = 0; t < PTS; ++t) c0acc[t] += __shfl_xor(c0acc[t], 32, 64);
+    if (live && hi == 0) {
+      float imm = 0.f;
+      if (j < m)
+        for (uint32_t i = 0; i < G.nimm; ++i) imm += PW[(uint64_t)G.imm[i] * m + j];
+#pragma unroll
+      for (uint32_t t = 0; t < PTS; ++t) C0r[w * 32u + n][t] = c0acc[t] + imm;
+    }
+  }
+  __syncthreads();  // bad[] and C0r complete
   const uint32_t bad0 = bad[0], bad1 = bad[1];
   if (live) {
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
-      const uint32_t row = jw + 8u * (v >> 2) + 4u * hi + (v & 3);
+      const uint32_t rl = 8u * (v >> 2) + 4u * hi + (v & 3), row = jw + rl;
       if (row >= m) continue;
-      float c0[2] = {acc[0][0][v], acc[0][1][v]};
-      for (uint32_t i = 0; i < G.nimm; ++i) {
-        const float wi = PW[(uint64_t)G.imm[i] * m + row];
-        c0[0] = c0[0] + wi;
-        c0[1] = c0[1] + wi;
+      float c0[2];
+      if (PT) {
+        c0[0] = C0r[w * 32u + rl][n / LPP];
+        c0[1] = C0r[w * 32u + rl][(32u + n) / LPP];
+      } else {
+        c0[0] = acc[0][0][v];
+        c0[1] = acc[0][1][v];
+        for (uint32_t i = 0; i < G.nimm; ++i) {
+          const float wi = PW[(uint64_t)G.imm[i] * m + row];
+          c0[0] = c0[0] + wi;
+          c0[1] = c0[1] + wi;
+        }
       }
       const uint32_t dst = PR[T + row];
 #pragma unroll
@@ -113,7 +198,7 @@ __global__ __launch_bounds__(256) void k_dense_gemm(DenseGemm G) {
         const uint64_t ln = lane0 + t * 32u + n;
         row_of(G, dst, 0)[ln] = c0[t];
 #pragma unroll
-        for (uint32_t q = 1; q < NC; ++q) row_of(G, dst, q)[ln] = q < NZ ? acc[q][t][v] : 0.f;
+        for (uint32_t q = 1; q < NC; ++q) row_of(G, dst, q)[ln] = (q >= Q0 && q - Q0 < NQ) ? acc[q - Q0][t][v] : 0.f;
       }
     }
   }
@@ -124,12 +209,12 @@ __global__ __launch_bounds__(256) void k_dense_gemm(DenseGemm G) {
   const uint32_t jend = min(m, (blockIdx.y + 1) * MT);
   for (uint32_t ln = 0; ln < NT; ++ln) {
     if (!(((ln < 32 ? bad0 : bad1) >> (ln & 31u)) & 1u)) continue;
-    for (uint32_t j = blockIdx.y * MT + tid; j < jend; j += blockDim.x) {
+    for (uint32_t jj = blockIdx.y * MT + tid; jj < jend; jj += blockDim.x) {
       float s[NC] = {0.f, 0.f, 0.f};
       for (uint32_t p = 0; p < T; ++p) {
         const uint32_t kr = PR[p];
         const uint32_t kind = kr >> P_KIND_SHIFT;
-        const float c[NC] = {PW[(uint64_t)p * m + j], 0.f, 0.f};
+        const float c[NC] = {PW[(uint64_t)p * m + jj], 0.f, 0.f};
         float x[NC] = {0.f, 0.f, 0.f}, term[NC];
         if (kind != T_IMM)
 #pragma unroll
@@ -147,10 +232,22 @@ __global__ __launch_bounds__(256) void k_dense_gemm(DenseGemm G) {
 #pragma unroll
         for (uint32_t q = 0; q < NC; ++q) s[q] = (p == 0) ? term[q] : s[q] + term[q];
       }
-      const uint32_t dst = PR[T + j];
+      const uint32_t dst = PR[T + jj];
 #pragma unroll
       for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[lane0 + ln] = s[q];
     }
+  }
+}
+
+template <uint32_t NZ>
+const void* pick(uint32_t pt) {
+  switch (pt) {
+    case 1: return (const void*)k_dense_gemm<NZ, 1>;
+    case 2: return (const void*)k_dense_gemm<NZ, 2>;
+    case 4: return (const void*)k_dense_gemm<NZ, 4>;
+    case 8: return (const void*)k_dense_gemm<NZ, 8>;
+    case 16: return (const void*)k_dense_gemm<NZ, 16>;
+    default: return (const void*)k_dense_gemm<NZ, 0>;
   }
 }
 
@@ -159,9 +256,13 @@ __global__ __launch_bounds__(256) void k_dense_gemm(DenseGemm G) {
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s) {
   if (G.m == 0 || G.lanes == 0) return hipSuccess;
   if (G.lanes % NT != 0) return hipErrorInvalidValue;
+  // c0 groups per 64-lane tile: groups that divide the tile, or tiles inside one group
+  uint32_t pt = 0;
+  if (G.share >= NT && G.share % NT == 0) pt = 1;
+  else if (G.share >= 4 && NT % G.share == 0) pt = NT / G.share;
   const dim3 grid((uint32_t)(G.lanes / NT), (G.m + MT - 1) / MT);
   void* args[] = {const_cast<DenseGemm*>(&G)};
-  const void* fn = (G.flags & D_INPUT_OPERANDS) ? (const void*)k_dense_gemm<2> : (const void*)k_dense_gemm<3>;
+  const void* fn = (G.flags & D_INPUT_OPERANDS) ? pick<2>(pt) : pick<3>(pt);
   return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
 }
 

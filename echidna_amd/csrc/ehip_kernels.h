@@ -41,6 +41,7 @@ struct DenseGemm {
   uint32_t imm[4];
   float* regs;
   uint64_t lanes;           // lanes per register row (a multiple of 64)
+  uint32_t share;           // c0 rows repeat over groups of this many lanes (0: they need not)
 };
 
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s);

@@ -7,9 +7,11 @@ terms behind the coefficient (pyoracle.magnitudes), tol 2e-5 for c0 and c1
 and 1e-4 for c2 (one more rounding chain), elementwise; the exact path is
 held to the same bar for comparison.  Lanes whose operands hold an Inf or a
 NaN are recomputed with full Taylor products and must equal the exact path
-bit for bit; f64 tapes never take the matrix path.  Tapes: one-layer MLPs whose DENSE reads tape inputs (c2 of
-the operands is zero), a two-layer MLP whose second DENSE reads full jets,
-member counts off the 32-member tile, and the n=256 H=256 STDE tape.
+bit for bit; f64 tapes never take the matrix path.  Tapes: one-layer MLPs
+whose DENSE reads tape inputs (c2 of the operands is zero), a two-layer MLP
+whose second DENSE reads full jets, member counts off the 32-member tile,
+and the n=256 H=256 STDE tape; STDE lane groups of 4 to 128 lanes per point,
+whose shared primal is formed once per point.
 """
 
 import numpy as np
@@ -126,6 +128,38 @@ def test_mma_f64_tapes_unaffected(mma):
     exact = _jets(hip, bufs, X, V, 48, torch.float64)
     hip.set_jet_mma(True)
     assert np.array_equal(got.view(np.uint64), exact.view(np.uint64))
+
+
+@pytest.mark.parametrize("S", [4, 8, 16, 32, 24, 128])
+@pytest.mark.parametrize("name,build", TAPES[:2], ids=[t[0] for t in TAPES[:2]])
+def test_mma_stde_shared_primal_groups(mma, name, build, S):
+    """STDE lanes of one point share c0 (the primal): with S dividing the
+    64-lane tile (4..32) or a multiple of it (128), c0 is formed once per
+    point on the vector ALUs and broadcast; S = 24 takes the full matrix
+    path.  Every point's estimate and sample variance against the oracle on
+    the same directions, 1e-4 of max(|estimate|, SE) and of the variance, and
+    the exact path's results within the same bar."""
+    hip = mma
+    tape = build()
+    n = tape.num_inputs
+    P, seed = 7, 3  # 7 S lanes: partial tiles for every S
+    X = wl.points_normal(9, P, n).astype(np.float32)
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    dX = torch.from_numpy(X).cuda()
+    res = lcv.laplacian_batch(hip, bufs, dX, S, lcv.DIR_GAUSSIAN, seed)
+    hip.set_jet_mma(False)
+    exact = lcv.laplacian_batch(hip, bufs, dX, S, lcv.DIR_GAUSSIAN, seed)
+    hip.set_jet_mma(True)
+    for p in range(P):
+        idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
+        dirs = philox.normal(seed, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        ref = po.laplacian_with_stats(tape, X[p].astype(np.float64), dirs)
+        scale = max(abs(ref["estimate"]), ref["standard_error"])
+        for got in (res, exact):
+            assert abs(got[p, 1] - ref["estimate"]) <= 1e-4 * scale, (p, got[p], ref)
+            assert abs(got[p, 2] - ref["sample_variance"]) <= 1e-4 * ref["sample_variance"], (p, got[p], ref)
+            assert abs(got[p, 0] - ref["value"]) <= 2e-5 * max(1.0, abs(ref["value"])) and got[p, 4] == S
+    assert not np.array_equal(res[:, 1], exact[:, 1])  # the matrix path ran
 
 
 @pytest.mark.parametrize("kind", [lcv.DIR_RADEMACHER, lcv.DIR_GAUSSIAN])
