@@ -7,8 +7,8 @@
 //   Taylor recurrences      src/taylor_ops.rs:18-916, src/taylor.rs:49-370
 // Every expression keeps the reference's evaluation order; the library is
 // built with -ffp-contract=off so no a*b+c is fused.  Transcendentals come
-// from the ROCm device math library (ocml), except the f32 scalar tanh and
-// its derivative (s_tanh, s_dtanh); powi uses repeated squaring like
+// from the ROCm device math library (ocml), except the scalar tanh and its
+// derivative (s_tanh, s_dtanh); powi uses repeated squaring like
 // llvm.powi on the host.
 #pragma once
 // Also compiled at run time by hipRTC (tape-specialised kernels), which
@@ -80,8 +80,7 @@ template <class F> EH_DEV bool s_exact_i32(F x, int32_t* out) {
 // extended precision; the n=32 H=64 MLP fwd+grad went from 2.24 to 3.95 G
 // points/s.  Signed zeros, infinities and NaN follow tanh and 1/cosh^2; where
 // cosh^2 overflows (|a| > 44.4) the derivative is the subnormal value
-// instead of 0.  f64 keeps the library's tanh and cosh: there the exp pair
-// raised the specialised kernel's registers (512 against 284) and slowed it.
+// instead of 0.
 // e = exp(y) and m = expm1(y) for y <= 0 (or NaN) from one range reduction:
 // y = k ln2 + r with |r| <= ln2 / 2 (Cody-Waite, two-part ln2), p = expm1(r)
 // by its Taylor series to r^7 (truncation below 2e-8 relative), then
@@ -104,6 +103,64 @@ EH_DEV void exp_pair_f32(float y, float& e, float& m) {
   e = ldexpf(1.0f + p, k);
   m = ldexpf(p, k) + (ldexpf(1.0f, k) - 1.0f);
 }
+// f64: the same pair in binary64, e = exp(-2|a|) and
+// m = expm1(-2|a|) from y = k ln2 + r (Cody-Waite, the two-part ln2 of
+// fdlibm), p = expm1(r) by its Taylor series to r^13 (truncation below
+// 1e-17 relative), e = 2^k (1 + p), m = 2^k p + (2^k - 1).  Both results
+// are quotients by 2 + m = 1 + e in [1, 2]: the hardware estimate r of 1 / d
+// (d = 2 + m rounded) is refined by two Newton steps, and each quotient
+// q0 = n r is corrected by its residual n - (2 + m) q0, in which
+// 2 + m = d + ((2 - d) + m) exactly:
+//   tanh|a| = -m / (2 + m), 1/cosh^2 a = (2 / (2 + m)) (2 e / (2 + m)).
+// Over |a| <= 20 they are within 1.6 and 2.8 ulp of the correctly rounded
+// values (binary64 emulation of these steps against 200-bit mpmath, 4,000
+// arguments; the device test bounds them at 2 and 3 ulp).
+EH_DEV void exp_pair_f64(double y, double& e, double& m) {
+  const double kf = rint(y * 1.4426950408889634);
+  double r = fma(-kf, 6.93147180369123816490e-01, y);
+  r = fma(-kf, 1.90821492927058770002e-10, r);
+  double q = 1.6059043836821614599e-10;  // 1/13!
+  q = fma(r, q, 2.0876756987868098979e-09);
+  q = fma(r, q, 2.5052108385441718775e-08);
+  q = fma(r, q, 2.7557319223985890653e-07);
+  q = fma(r, q, 2.7557319223985890653e-06);
+  q = fma(r, q, 2.4801587301587301566e-05);
+  q = fma(r, q, 1.9841269841269841253e-04);
+  q = fma(r, q, 1.3888888888888889419e-03);
+  q = fma(r, q, 8.3333333333333332177e-03);
+  q = fma(r, q, 4.1666666666666664354e-02);
+  q = fma(r, q, 1.6666666666666665741e-01);
+  q = fma(r, q, 0.5);
+  const double p = fma(r * r, q, r);  // r + r^2 (1/2 + r/6 + ..)
+  const int k = (int)kf;
+  e = ldexp(1.0 + p, k);
+  m = ldexp(p, k) + (ldexp(1.0, k) - 1.0);
+}
+// 1 / d for d in [1, 2]
+EH_DEV double recip_12(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+// tanh a and 1/cosh^2 a in binary64 from one reduction, without branches:
+// |a| is clamped at 360 (tanh rounds to +-1 from 19.1 on), 1/cosh^2 is 0
+// beyond 355.29, where the reference's cosh^2 overflows, and NaN propagates
+// through both.
+EH_DEV void tanh_pair_f64(double a, double& t, double& dt) {
+  const double ax = fabs(a);
+  double e, m;
+  exp_pair_f64(-2.0 * (ax > 360.0 ? 360.0 : ax), e, m);
+  const double d = 2.0 + m, derr = (2.0 - d) + m;  // 2 + m = d + derr exactly
+  double r = recip_12(d);
+  const double t0 = -m * r;
+  t = copysign(fma(r, fma(d, -t0, -m) - derr * t0, t0), a);  // t0 + (-m - (2 + m) t0) / d
+  // 1/cosh^2 = (1 + tanh|a|)(1 - tanh|a|) = (2 / (2 + m)) (2 e / (2 + m)),
+  // each quotient corrected by its residual as above
+  const double A0 = 2.0 * r, B0 = (2.0 * e) * r;
+  const double A = fma(r, fma(d, -A0, 2.0) - derr * A0, A0);
+  const double B = fma(r, fma(d, -B0, 2.0 * e) - derr * B0, B0);
+  dt = ax > 355.29 ? 0.0 : A * B;
+}
 template <class F> EH_DEV F s_tanh(F a) {
   if constexpr (sizeof(F) == 4) {
     // 2 + m lies in [1, 2]: the hardware reciprocal (1 ulp) needs no scaling
@@ -111,7 +168,9 @@ template <class F> EH_DEV F s_tanh(F a) {
     exp_pair_f32(F(-2) * fabs(a), e, m);
     return copysign(-m * __builtin_amdgcn_rcpf(F(2) + m), a);
   } else {
-    return tanh(a);
+    double t, dt;
+    tanh_pair_f64(a, t, dt);
+    return t;
   }
 }
 template <class F> EH_DEV F s_dtanh(F a) {
@@ -121,8 +180,31 @@ template <class F> EH_DEV F s_dtanh(F a) {
     const F r = __builtin_amdgcn_rcpf(F(1) + e);  // 1 + e in [1, 2]
     return (F(4) * e) * (r * r);
   } else {
-    const F c = cosh(a);
-    return F(1) / (c * c);
+    double t, dt;
+    tanh_pair_f64(a, t, dt);
+    return dt;
+  }
+}
+// Both at once (one reduction in binary64), for straight-line code with
+// many tanh: the binary64 pair is a called function there, so its ~30
+// dependent steps are not interleaved across units (which held hundreds of
+// registers and spilled), and the code is not repeated per unit.
+struct TanhPair {
+  double t, dt;
+};
+static __device__ __attribute__((noinline)) TanhPair tanh_pair_f64_call(double a) {
+  TanhPair o;
+  tanh_pair_f64(a, o.t, o.dt);
+  return o;
+}
+template <class F> EH_DEV void s_tanh_dtanh(F a, F& t, F& dt) {
+  if constexpr (sizeof(F) == 4) {
+    t = s_tanh(a);
+    dt = s_dtanh(a);
+  } else {
+    const TanhPair o = tanh_pair_f64_call(a);
+    t = o.t;
+    dt = o.dt;
   }
 }
 template <class F> EH_DEV F s_ln2() { return F(0.69314718055994530942); }

@@ -44,7 +44,8 @@ namespace ehip_detail {
 namespace {
 
 // opcodes (src/opcode.rs:21-130) the generator treats specially
-constexpr uint32_t OP_INPUT = 0, OP_CONST = 1, OP_ADD = 2, OP_SUB = 3, OP_MUL = 4, OP_NEG = 12, OP_POWI = 16;
+constexpr uint32_t OP_INPUT = 0, OP_CONST = 1, OP_ADD = 2, OP_SUB = 3, OP_MUL = 4, OP_NEG = 12, OP_POWI = 16,
+                   OP_TANH = 32;
 constexpr uint32_t UNUSED = 0xFFFFFFFFu;
 
 struct Gen {
@@ -89,6 +90,87 @@ struct Gen {
   // Value of constant entry `i` in the tape's dtype.
   double cval(uint32_t i) const { return f64 ? h.values[i] : (double)(float)h.values[i]; }
 };
+
+// The order the forward sweep's entries are written in: tape order, except
+// that an entry whose operands are all written and which ends the life of at
+// least one of them is written at once, before the entries after it.  The
+// values are the same in any order that respects the operands; this one
+// keeps live ranges short where the tape records a reduction late, as a
+// model's output sum over hidden units computed after every unit (each unit's
+// value then dies as soon as it is made, rather than all of them staying live
+// until the sum).  Constants are literals and hold no register: they are
+// written when an entry that uses them is.
+std::vector<uint32_t> emission_order(const TapeHost& h) {
+  const uint32_t N = (uint32_t)h.opcodes.size();
+  auto operands = [&](uint32_t i, uint32_t* o) {
+    const uint32_t op = h.opcodes[i];
+    uint32_t k = 0;
+    if (op == OP_INPUT || op == OP_CONST) return k;
+    if (h.arg0[i] != UNUSED) o[k++] = h.arg0[i];
+    if (op != OP_POWI && h.arg1[i] != UNUSED) o[k++] = h.arg1[i];
+    return k;
+  };
+  std::vector<uint32_t> remaining(N, 0), first(N + 1, 0), cons;
+  uint32_t o[2];
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t k = operands(i, o);
+    for (uint32_t t = 0; t < k; ++t) ++first[o[t] + 1];
+  }
+  for (uint32_t i = 0; i < N; ++i) first[i + 1] += first[i];
+  cons.resize(first[N]);
+  std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t k = operands(i, o);
+    for (uint32_t t = 0; t < k; ++t) {
+      cons[fill[o[t]]++] = i;
+      ++remaining[o[t]];
+    }
+  }
+  for (uint32_t out : h.outputs) remaining[out] += 1;  // outputs stay live to the end
+  std::vector<uint8_t> done(N, 0);
+  std::vector<uint32_t> order, stack;
+  order.reserve(N);
+  auto is_const = [&](uint32_t i) { return h.opcodes[i] == OP_CONST; };
+  auto emit = [&](uint32_t i) {
+    done[i] = 1;
+    order.push_back(i);
+    const uint32_t k = operands(i, o);
+    for (uint32_t t = 0; t < k; ++t) --remaining[o[t]];
+  };
+  // written early: every operand written, and one of them used for the last time
+  auto ready_and_kills = [&](uint32_t c) {
+    uint32_t oc[2];
+    const uint32_t k = operands(c, oc);
+    bool kills = false;
+    for (uint32_t t = 0; t < k; ++t) {
+      if (!done[oc[t]] && !is_const(oc[t])) return false;
+      const uint32_t uses = (k == 2 && oc[0] == oc[1]) ? 2u : 1u;
+      if (!is_const(oc[t]) && remaining[oc[t]] == uses) kills = true;
+    }
+    return kills;
+  };
+  for (uint32_t i = 0; i < N; ++i) {
+    if (done[i]) continue;
+    emit(i);
+    stack.push_back(i);
+    while (!stack.empty()) {
+      const uint32_t x = stack.back();
+      stack.pop_back();
+      for (uint32_t e = first[x]; e < first[x + 1]; ++e) {
+        const uint32_t c = cons[e];
+        if (!done[c] && ready_and_kills(c)) {
+          uint32_t oc[2];
+          const uint32_t k = operands(c, oc);
+          for (uint32_t t = 0; t < k; ++t)
+            if (!done[oc[t]]) emit(oc[t]);  // a constant recorded later
+          emit(c);
+          stack.push_back(c);
+        }
+      }
+    }
+  }
+  return order;
+}
 
 }  // namespace
 
@@ -171,7 +253,7 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
   g.put("const F* __restrict__ x = X + p * xs;\n");
   if (dual) g.put("const F* __restrict__ v = V + p * %uull;\n", n);
   const char* T = dual ? "D" : "F";
-  for (uint32_t i = 0; i < N; ++i) {
+  for (const uint32_t i : emission_order(h)) {
     const uint32_t op = h.opcodes[i], a = h.arg0[i], b = h.arg1[i];
     if (op == OP_INPUT) {
       if (dual) g.put("const D v%u = {x[%u], v[%u]};\n", i, i, i);
@@ -186,6 +268,13 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
       else g.put("const D v%u = ehip::d_eval<F>(%uu, v%u, D{F(0), F(0)});\n", i, op, a);
     } else {
       switch (op) {
+        case OP_TANH:
+          // binary64: tanh and its derivative from one reduction, the
+          // derivative kept for the reverse sweep (the values s_tanh and
+          // s_dtanh give)
+          if (g.f64 && what == 1) g.put("F v%u, t%u;\nehip::s_tanh_dtanh<F>(v%u, v%u, t%u);\n", i, i, a, i, i);
+          else g.put("const F v%u = ehip::s_eval<F>(%uu, v%u, F(0));\n", i, op, a);
+          break;
         case OP_ADD: g.put("const F v%u = v%u + v%u;\n", i, a, b); break;
         case OP_SUB: g.put("const F v%u = v%u - v%u;\n", i, a, b); break;
         case OP_MUL: g.put("const F v%u = v%u * v%u;\n", i, a, b); break;
@@ -269,6 +358,11 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
       if (op == OP_POWI) {
         g.put("{ const %s da = ehip::%s<F>(%d, v%u, v%u);\n", T, dual ? "d_powi_partial" : "s_powi_partial",
               (int32_t)b, a, ii);
+        g.out += "if (" + live + " && " + nz("da") + ") " + upd(a, "da") + " }\n";
+        continue;
+      }
+      if (op == OP_TANH && g.f64 && !dual) {
+        g.put("{ const F da = t%u;\n", ii);
         g.out += "if (" + live + " && " + nz("da") + ") " + upd(a, "da") + " }\n";
         continue;
       }

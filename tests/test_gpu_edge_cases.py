@@ -112,3 +112,39 @@ def _close32(got, ref):
 def test_gpu_unary_edge_grid_f32(hip, name):
     for what, (got, ref) in zip(("value", "gradient", "jet"), _grid(hip, name, False)):
         assert _close32(got, ref), (name, what, got, ref)
+
+
+@pytest.mark.parametrize("spec", [False, True], ids=["interpreter", "specialised"])
+def test_gpu_tanh_f64_ulp(hip, spec):
+    """The binary64 tanh and 1/cosh^2 (one exp reduction shared by both)
+    against the correctly rounded values (200-bit mpmath): within 2 and 3
+    ulp over 4,096 arguments in [-20, 20], tiny and large arguments; the
+    specialised kernel's called pair gives the interpreter's bits."""
+    mpmath = pytest.importorskip("mpmath")
+    from echidna_amd.backend import TUNE_GRADIENT, HipContext
+
+    tape = unary_tape("tanh")
+    bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
+    rng = np.random.default_rng(11)
+    X = np.concatenate([rng.uniform(-20, 20, 3500), rng.uniform(-1e-3, 1e-3, 300),
+                        np.sign(rng.uniform(-1, 1, 296)) * 10.0 ** rng.uniform(-300, 1.3, 296)])
+    if spec:
+        hip.specialize(bufs, TUNE_GRADIENT)
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 0, fetch=2)
+    y, g = hip.gradient_batch(bufs, X, len(X))
+    y = np.asarray(y, np.float64).ravel()
+    g = np.asarray(g, np.float64).ravel()
+    mpmath.mp.prec = 200
+    worst_t = worst_d = 0.0
+    for a, t, d in zip(X, y, g):
+        T = mpmath.tanh(mpmath.mpf(float(a)))
+        D = 1 / mpmath.cosh(mpmath.mpf(float(a))) ** 2
+        worst_t = max(worst_t, float(abs(mpmath.mpf(float(t)) - T)) / np.spacing(abs(float(T))))
+        worst_d = max(worst_d, float(abs(mpmath.mpf(float(d)) - D)) / np.spacing(abs(float(D))))
+    print(f"tanh f64 ({'specialised' if spec else 'interpreter'}): worst {worst_t:.2f} ulp, 1/cosh^2 {worst_d:.2f} ulp")
+    assert worst_t <= 2.0 and worst_d <= 3.0
+    if spec:
+        HipContext.set_tuned(bufs, TUNE_GRADIENT, 256, 0, fetch=0)
+        y0, g0 = hip.gradient_batch(bufs, X, len(X))
+        assert np.array_equal(np.asarray(y0, np.float64).ravel().view(np.uint64), y.view(np.uint64))
+        assert np.array_equal(np.asarray(g0, np.float64).ravel().view(np.uint64), g.view(np.uint64))

@@ -62,7 +62,12 @@ def test_reverse_sweep_shape():
     tape, _ = record(f, [0.5, 0.25, 1.0])
     src, _ = be.spec_source(_data(tape, True), 1, f64=True)
     rev = src.split("F g", 1)[1]
-    assert rev.count("s_partials") == 2  # tanh and the variable product
+    # the variable product; binary64 tanh reuses the derivative its forward
+    # pair made (s_tanh_dtanh), the f32 source calls s_partials for it too
+    assert rev.count("s_partials") == 1 and "s_tanh_dtanh" in src
+    assert "const F da = t" in rev
+    src32, _ = be.spec_source(_data(tape, False), 1, f64=False)
+    assert src32.split("F g", 1)[1].count("s_partials") == 2 and "s_tanh_dtanh" not in src32
     assert "gp[2] = F(0);" in src  # x2 only feeds the dead exp
     assert "s_partials<F>(17u" not in src  # no reverse for exp
 
