@@ -34,7 +34,10 @@ int32_t ehip_ctx_create(int32_t device, ehip_ctx** out) try {
   c->device = device;
   c->max_lds = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 65536;
   c->ncu = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 256;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  // A blocking stream: work enqueued on it waits for what the caller put on
+  // the legacy default stream before (torch's default stream, for one), so
+  // a device input produced there is complete when a kernel reads it.
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "stream create");

@@ -22,7 +22,11 @@
  * synchronously, like the reference's per-call clone_htod / launch /
  * synchronize / clone_dtoh.  Device-pointer entry points (`*_dev`) take
  * buffers already resident in HBM and enqueue on the caller's HIP stream
- * (NULL = the context's stream) without synchronizing.
+ * (NULL = the context's stream) without synchronizing.  The context's stream
+ * is a blocking stream: its work is ordered after work enqueued before it on
+ * the legacy default (null) stream, and the null stream's later work waits
+ * for it.  Inputs produced on any other stream must be complete (or the
+ * caller's stream passed) before the call.
  *
  * Threading: calls on one context are serialized by the context; the
  * context's scratch registers are reused across calls, so `*_dev` work on

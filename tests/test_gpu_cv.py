@@ -40,6 +40,25 @@ def test_init_matches_philox_formula(hip):
     assert np.array_equal(net.params(), cv_ref.init_params(N, H, R, 11, 0.5))
 
 
+def test_context_stream_waits_for_the_default_stream(hip):
+    """An input written by a kernel still queued on torch's default stream
+    is read complete by a call on the context's stream (a blocking stream,
+    ordered after the legacy default stream), with no synchronisation in
+    between: the f64 -> f32 conversion of a learned-CV caller's points is
+    the case that exposed the race."""
+    net = random_cv(hip)
+    X = torch.from_numpy(wl.points_normal(3, 37, N)).cuda()
+    want = net.forward(X.float())
+    busy = torch.full((2048, 2048), 1e-4, device="cuda")
+    for _ in range(8):
+        busy = busy @ busy  # keeps the default stream busy; entries decay towards 0
+    xf = X.float() + busy[0, :1] * 0.0  # written only after the products
+    L = torch.empty_like(want)
+    net.forward_dev(xf, 37, L)
+    torch.cuda.synchronize()
+    assert torch.equal(L, want)
+
+
 def test_forward_matches_reference(hip):
     net = random_cv(hip)
     X = wl.points_normal(3, 37, N).astype(np.float32)
