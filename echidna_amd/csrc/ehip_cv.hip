@@ -1,14 +1,16 @@
 // Kernels of the learned control variate (ehip_cv.h).
 //
-// GEMM: 64x64 output tile per 256-thread workgroup, four waves in a 2x2
-// grid, each wave a 32x32 block of v_mfma_f32_32x32x16_bf16.  Operand
-// layout of that instruction (checked on the device): lane l holds
-// A[m = l%32][k = 8*(l/32) + i] and B[k = 8*(l/32) + i][n = l%32] for
-// i < 8, and C[m][n] for n = l%32, m = 8*(j/4) + 4*(l/32) + j%4, j < 16.
-// Tiles of A and B go global -> registers -> LDS as BF16 with k contiguous
-// (row pitch 40 BF16 = 80 bytes keeps every 8-element operand read 16-byte
-// aligned); the next K step's global loads are in flight while the current
-// step's MFMAs run.
+// GEMM: a (64 T) x (64 T) output tile per 256-thread workgroup (T = 1 or
+// 2), four waves in a 2x2 grid, each wave T x T blocks of 32x32 from
+// v_mfma_f32_32x32x16_bf16.  Operand layout of that instruction (checked on
+// the device): lane l holds A[m = l%32][k = 8*(l/32) + i] and
+// B[k = 8*(l/32) + i][n = l%32] for i < 8, and C[m][n] for n = l%32,
+// m = 8*(j/4) + 4*(l/32) + j%4, j < 16.  Tiles of A and B go global ->
+// registers -> LDS as BF16 with k contiguous (row pitch 40 BF16 = 80 bytes
+// keeps every 8-element operand read 16-byte aligned); the next K step's
+// global loads are in flight while the current step's MFMAs run.  The
+// 128x128 tile reads half the operand bytes per product of the 64x64 one
+// and is used where it still gives the chip enough workgroups.
 #include <hip/hip_runtime.h>
 
 #include "ehip_cv.h"
@@ -19,7 +21,7 @@ namespace ehip {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 32, PITCH = BK + 8;
+constexpr int BK = 32, PITCH = BK + 8;
 
 // Eight consecutive elements p[0..7] (stride 1) if all lie below `lim`
 // elements from the run start, else the in-range ones and zeros.
@@ -35,44 +37,53 @@ __device__ __forceinline__ void load8(const float* p, int64_t avail, float* o) {
   }
 }
 
-// Stage one operand tile (64 rows of the non-k dimension x 32 k) into
-// registers.  KC: k is the contiguous dimension of this operand.
-template <bool KC>
+// Stage one operand tile (64 T rows of the non-k dimension x 32 k) into
+// registers, eight elements per thread per 64 rows.  KC: k is the
+// contiguous dimension of this operand.
+template <int T, bool KC>
 __device__ __forceinline__ void fetch_tile(const float* base, int64_t s_row, int64_t s_k, int rows, int K, int r0,
                                            int k0, float* o) {
   const int t = threadIdx.x;
-  if (KC) {
-    const int row = t >> 2, kq = (t & 3) * 8;
-    const int gr = r0 + row, gk = k0 + kq;
-    if (gr < rows && gk < K) load8(base + (int64_t)gr * s_row + gk, K - gk, o);
-    else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) o[j] = 0.0f;
-    }
-  } else {
-    const int kr = t >> 3, rq = (t & 7) * 8;
-    const int gk = k0 + kr, gr = r0 + rq;
-    if (gk < K && gr < rows) load8(base + (int64_t)gk * s_k + gr, rows - gr, o);
-    else {
+  for (int q = 0; q < T; ++q) {
+    float* oq = o + 8 * q;
+    if (KC) {
+      const int row = (t >> 2) + 64 * q, kq = (t & 3) * 8;
+      const int gr = r0 + row, gk = k0 + kq;
+      if (gr < rows && gk < K) load8(base + (int64_t)gr * s_row + gk, K - gk, oq);
+      else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) o[j] = 0.0f;
+        for (int j = 0; j < 8; ++j) oq[j] = 0.0f;
+      }
+    } else {
+      const int kr = t >> 3, rq = (t & 7) * 8 + 64 * q;
+      const int gk = k0 + kr, gr = r0 + rq;
+      if (gk < K && gr < rows) load8(base + (int64_t)gk * s_k + gr, rows - gr, oq);
+      else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) oq[j] = 0.0f;
+      }
     }
   }
 }
 
-template <bool KC>
+template <int T, bool KC>
 __device__ __forceinline__ void store_tile(__bf16* lds, const float* o) {
   const int t = threadIdx.x;
-  if (KC) {
-    const int row = t >> 2, kq = (t & 3) * 8;
-    bf16x8 v;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = (__bf16)o[j];
-    *reinterpret_cast<bf16x8*>(lds + row * PITCH + kq) = v;
-  } else {
-    const int kr = t >> 3, rq = (t & 7) * 8;
+  for (int q = 0; q < T; ++q) {
+    const float* oq = o + 8 * q;
+    if (KC) {
+      const int row = (t >> 2) + 64 * q, kq = (t & 3) * 8;
+      bf16x8 v;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) lds[(rq + j) * PITCH + kr] = (__bf16)o[j];
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)oq[j];
+      *reinterpret_cast<bf16x8*>(lds + row * PITCH + kq) = v;
+    } else {
+      const int kr = t >> 3, rq = (t & 7) * 8 + 64 * q;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) lds[(rq + j) * PITCH + kr] = (__bf16)oq[j];
+    }
   }
 }
 
@@ -94,44 +105,112 @@ __device__ __forceinline__ void gemm_store(const Gemm& g, int m, int n, float v)
 
 // blockIdx.z = K slice [kz * kc, (kz + 1) * kc) (kc = K when not split).
 // Split: raw sums to part[kz][m][n]; otherwise the epilogue.
-template <bool AK, bool BKC>
-__global__ __launch_bounds__(256) void k_gemm(Gemm g, int kc) {
-  __shared__ __attribute__((aligned(16))) __bf16 As[BM * PITCH];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * PITCH];
+// After the K loop the operand LDS holds each wave's 32x32 output blocks in
+// turn (row pitch 33 floats), so full tiles leave as 16-byte row pieces
+// instead of single floats; edge tiles and split-K slices store directly.
+constexpr int STAGE_PITCH = 33;
+
+template <int T, bool AK, bool BKC>
+__global__ __launch_bounds__(256, 3) void k_gemm(Gemm g, int kc) {
+  constexpr int BM = 64 * T, BN = 64 * T;
+  constexpr int OPB = (BM + BN) * PITCH * 2, STB = 4 * 32 * STAGE_PITCH * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[OPB > STB ? OPB : STB];
+  __bf16* As = reinterpret_cast<__bf16*>(smem);
+  __bf16* Bs = As + BM * PITCH;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * 32 * T, wn = (wave & 1) * 32 * T;
   const int kb = blockIdx.z * kc, ke = min(g.K, kb + kc);
   // A(m,k): rows = M, row stride sam, k stride sak; B(k,n): rows = N.
-  float ra[8], rb[8];
-  fetch_tile<AK>(g.A, g.sam, g.sak, g.M, ke, m0, kb, ra);
-  fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, ke, n0, kb, rb);
-  f32x16 acc = {};
+  float ra[8 * T], rb[8 * T];
+  fetch_tile<T, AK>(g.A, g.sam, g.sak, g.M, ke, m0, kb, ra);
+  fetch_tile<T, BKC>(g.B, g.sbn, g.sbk, g.N, ke, n0, kb, rb);
+  f32x16 acc[T][T];
+#pragma unroll
+  for (int i = 0; i < T; ++i)
+#pragma unroll
+    for (int j = 0; j < T; ++j) acc[i][j] = f32x16{};
   for (int k0 = kb; k0 < ke; k0 += BK) {
     __syncthreads();
-    store_tile<AK>(As, ra);
-    store_tile<BKC>(Bs, rb);
+    store_tile<T, AK>(As, ra);
+    store_tile<T, BKC>(Bs, rb);
     __syncthreads();
     if (k0 + BK < ke) {
-      fetch_tile<AK>(g.A, g.sam, g.sak, g.M, ke, m0, k0 + BK, ra);
-      fetch_tile<BKC>(g.B, g.sbn, g.sbk, g.N, ke, n0, k0 + BK, rb);
+      fetch_tile<T, AK>(g.A, g.sam, g.sak, g.M, ke, m0, k0 + BK, ra);
+      fetch_tile<T, BKC>(g.B, g.sbn, g.sbk, g.N, ke, n0, k0 + BK, rb);
     }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
       const int ko = kk + 8 * (lane >> 5);
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(As + (wm + (lane & 31)) * PITCH + ko);
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + (wn + (lane & 31)) * PITCH + ko);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      bf16x8 a[T], b[T];
+#pragma unroll
+      for (int i = 0; i < T; ++i) {
+        a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + 32 * i + (lane & 31)) * PITCH + ko);
+        b[i] = *reinterpret_cast<const bf16x8*>(Bs + (wn + 32 * i + (lane & 31)) * PITCH + ko);
+      }
+#pragma unroll
+      for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int j = 0; j < T; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
-  const int n = n0 + wn + (lane & 31);
-  if (n >= g.N) return;
+  const bool full = gridDim.z == 1 && m0 + BM <= g.M && n0 + BN <= g.N && (g.ldc & 3) == 0 &&
+                    (reinterpret_cast<uintptr_t>(g.C) & 15u) == 0;
+  if (full) {
+    __syncthreads();  // every wave is done reading the operand tiles
+    float* stage = reinterpret_cast<float*>(smem) + wave * (32 * STAGE_PITCH);
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int m = m0 + wm + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-    if (m >= g.M) continue;
-    if (gridDim.z > 1) g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = acc[j];
-    else gemm_store(g, m, n, acc[j]);
+    for (int bi = 0; bi < T; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < T; ++bj) {
+        const int nl = lane & 31, nb = n0 + wn + 32 * bj, mb = m0 + wm + 32 * bi;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int ml = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          float v = acc[bi][bj][j];
+          switch (g.epi) {
+            case EPI_BIAS: v = g.alpha * (v + g.bias[nb + nl]); break;
+            case EPI_BIAS_TANH: v = tanhf(v + g.bias[nb + nl]); break;
+            case EPI_DTANH: {
+              const float h = g.aux[(int64_t)(mb + ml) * g.ldaux + nb + nl];
+              v = v * (1.0f - h * h);
+              break;
+            }
+            default: break;
+          }
+          stage[ml * STAGE_PITCH + nl] = v;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // rows of the block as eight 16-byte pieces each, eight rows a pass
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = q * 8 + (lane >> 3), c4 = (lane & 7) * 4;
+          const float* sr = stage + row * STAGE_PITCH + c4;
+          float4 v = make_float4(sr[0], sr[1], sr[2], sr[3]);
+          float4* dst = reinterpret_cast<float4*>(g.C + (int64_t)(mb + row) * g.ldc + nb + c4);
+          if (g.accumulate) {
+            const float4 o = *dst;
+            v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+          }
+          *dst = v;
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    return;
+  }
+#pragma unroll
+  for (int bj = 0; bj < T; ++bj) {
+    const int n = n0 + wn + 32 * bj + (lane & 31);
+    if (n >= g.N) continue;
+#pragma unroll
+    for (int bi = 0; bi < T; ++bi)
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int m = m0 + wm + 32 * bi + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        if (m >= g.M) continue;
+        if (gridDim.z > 1) g.part[((int64_t)blockIdx.z * g.M + m) * g.N + n] = acc[bi][bj][j];
+        else gemm_store(g, m, n, acc[bi][bj][j]);
+      }
   }
 }
 
@@ -145,12 +224,21 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(Gemm g, int nsplit) {
   gemm_store(g, (int)(i / g.N), (int)(i % g.N), v);
 }
 
+template <int T>
+const void* gemm_fn(bool ak, bool bk) {
+  return ak ? (bk ? (const void*)k_gemm<T, true, true> : (const void*)k_gemm<T, true, false>)
+            : (bk ? (const void*)k_gemm<T, false, true> : (const void*)k_gemm<T, false, false>);
+}
+
 hipError_t launch_gemm(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
   const bool ak = g.sak == 1, bk = g.sbk == 1;
   if ((!ak && g.sam != 1) || (!bk && g.sbn != 1)) return hipErrorInvalidValue;
-  const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
-  // a long K over too few tiles to fill the chip: split it (256 CUs)
+  // 128x128 tiles when they still make at least two workgroups per CU (256 CUs)
+  const int64_t t128 = (int64_t)((g.M + 127) / 128) * ((g.N + 127) / 128);
+  const int T = t128 >= 512 ? 2 : 1;
+  const int tm = (g.M + 64 * T - 1) / (64 * T), tn = (g.N + 64 * T - 1) / (64 * T);
+  // a long K over too few tiles to fill the chip: split it
   int split = 1;
   const int64_t MN = (int64_t)g.M * g.N;
   while (split < 16 && (int64_t)tm * tn * split < 512 && g.K / (split * 2) >= 512 &&
@@ -158,8 +246,7 @@ hipError_t launch_gemm(const Gemm& g, hipStream_t s) {
     split *= 2;
   const int kc = split > 1 ? ((g.K + split - 1) / split + BK - 1) / BK * BK : g.K;
   const dim3 grid(tn, tm, split);
-  const void* fn = ak ? (bk ? (const void*)k_gemm<true, true> : (const void*)k_gemm<true, false>)
-                      : (bk ? (const void*)k_gemm<false, true> : (const void*)k_gemm<false, false>);
+  const void* fn = T == 2 ? gemm_fn<2>(ak, bk) : gemm_fn<1>(ak, bk);
   int kcv = kc;
   void* args[] = {const_cast<Gemm*>(&g), &kcv};
   hipError_t e = hipLaunchKernel(fn, grid, dim3(256), args, 0, s);

@@ -25,7 +25,8 @@ struct ehip_cv {
   DevBuf params, grads, mom1, mom2;
   DevBuf act;       // H1, H2, Lf, dLf, dZ2, dZ1, loss for `cap` tuples
   DevBuf ws;        // split-K partial sums of the dZ2 GEMM (16 slices)
-  uint32_t cap = 0;
+  DevBuf inf;       // inference: H1, H2 for `inf_cap` points
+  uint32_t cap = 0, inf_cap = 0;
   uint64_t step = 0;
   float gain = 1.0f;  // L = gain * (W3 h2 + b3)
   uint32_t npos = 0;  // columns with sign +1 (the rest -1); r = L L^T
@@ -33,7 +34,8 @@ struct ehip_cv {
 
 namespace {
 
-constexpr uint32_t CV_CHUNK = 2048;  // tuples per forward/backward pass
+constexpr uint32_t CV_CHUNK = 2048;       // tuples per forward/backward pass
+constexpr uint32_t CV_INF_CHUNK = 16384;  // points per inference pass (hidden activations only)
 
 struct Act {
   float *H1, *H2, *Lf, *dLf, *dZ2, *dZ1, *loss;
@@ -45,6 +47,20 @@ int32_t cv_reserve(ehip_cv* cv, uint32_t T) {
   HIP_TRY(cv->act.reserve(per * T * sizeof(float)), "control variate activations");
   HIP_TRY(cv->ws.reserve(16ull * T * cv->H * sizeof(float)), "control variate workspace");
   cv->cap = T;
+  return EHIP_OK;
+}
+
+// Hidden activations of an inference pass over up to T points (L goes
+// straight to the caller's buffer): larger chunks than training keeps fill
+// the chip in the two narrow hidden-layer products.
+int32_t cv_reserve_inf(ehip_cv* cv, uint32_t T, Act& a) {
+  if (T > cv->inf_cap) {
+    HIP_TRY(cv->inf.reserve(2ull * cv->H * T * sizeof(float)), "control variate activations");
+    cv->inf_cap = T;
+  }
+  a = Act{};
+  a.H1 = (float*)cv->inf.p;
+  a.H2 = a.H1 + (size_t)cv->H * T;
   return EHIP_OK;
 }
 
@@ -238,6 +254,7 @@ void ehip_cv_destroy(ehip_cv* cv) try {
   cv->mom2.release();
   cv->act.release();
   cv->ws.release();
+  cv->inf.release();
   delete cv;
 } catch (...) {
   (void)exception_status();
@@ -307,10 +324,10 @@ int32_t ehip_cv_forward_dev(ehip_cv* cv, const float* d_x, uint64_t batch, float
   std::lock_guard<std::mutex> lk(cv->ctx->mu);
   (void)hipSetDevice(cv->ctx->device);
   hipStream_t s = pick(cv->ctx, stream);
-  const uint32_t chunk = (uint32_t)std::min<uint64_t>(batch, CV_CHUNK);
-  int32_t rc = cv_reserve(cv, chunk);
+  const uint32_t chunk = (uint32_t)std::min<uint64_t>(batch, CV_INF_CHUNK);
+  Act a;
+  int32_t rc = cv_reserve_inf(cv, chunk, a);
   if (rc) return rc;
-  const Act a = cv_act(cv, chunk);
   const size_t n = cv->n, nr = (size_t)cv->n * cv->r;
   for (uint64_t t0 = 0; t0 < batch; t0 += chunk) {
     const uint32_t c = (uint32_t)std::min<uint64_t>(chunk, batch - t0);
@@ -464,7 +481,7 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
   R.S = S;
   R.nin = t->nin;
   R.rng = dirs ? RNG_NONE : dir_kind;
-  R.dirs = dirs;
+  R.dirs_cmajor = dirs;
   R.seed = seed;
   R.out = d_results;
   R.out_plain = d_plain_results;
@@ -479,10 +496,10 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
     }
     HIP_TRY(c->cvl.reserve(npoints * (uint64_t)cv->n * cv->r * sizeof(float)), "cv factors");
     float* Lp = (float*)c->cvl.p;
-    const uint32_t chunk = (uint32_t)std::min<uint64_t>(npoints, CV_CHUNK);
-    rc = cv_reserve(cv, chunk);
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>(npoints, CV_INF_CHUNK);
+    Act a;
+    rc = cv_reserve_inf(cv, chunk, a);
     if (rc) return rc;
-    const Act a = cv_act(cv, chunk);
     for (uint64_t t0 = 0; t0 < npoints; t0 += chunk) {
       const uint32_t cc = (uint32_t)std::min<uint64_t>(chunk, npoints - t0);
       rc = cv_forward(cv, xf + t0 * t->nin, cc, Lp + t0 * (uint64_t)cv->n * cv->r, a, s);

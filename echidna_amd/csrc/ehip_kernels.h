@@ -27,7 +27,9 @@ struct Launch {
   uint32_t S;             // lanes per x row (0: one row per lane)
   uint32_t rng;           // RNG_NONE: read dirs; else generate (philox.h)
   uint64_t seed;
-  void* dirs_out;         // jets drawing directions also write them here [B][nin] (or null)
+  void* dirs_out;         // jets drawing directions also write them here, coordinate-major
+                          // [nin][B] (lane p, coordinate i at i * B + p: a wave's writes are
+                          // contiguous), or null
 };
 
 // One DENSE of the segmented f32 order-3 jet program as a batch GEMM over
@@ -53,7 +55,8 @@ struct StdeReduce {
   uint32_t S, nin;
   uint32_t rng;           // how directions were made (for control variates)
   uint64_t seed;
-  const void* dirs;       // [npoints*S][nin] when rng == RNG_NONE
+  const void* dirs;       // [npoints*S][nin] when rng == RNG_NONE (user directions)
+  const void* dirs_cmajor;  // or the jets' own directions, coordinate-major [nin][npoints*S]
   const void* diag;       // diagonal control variate [.. ][nin] or null
   uint64_t diag_stride;   // 0: one diagonal shared by all points
   double* out;            // [npoints][5]: value, estimate, variance, std error, count

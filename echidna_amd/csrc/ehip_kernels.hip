@@ -1125,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
             if (L.rng) {
               const F d = (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i);
               v[1] = d;
-              if (L.dirs_out) reinterpret_cast<F*>(L.dirs_out)[p * L.nin + i] = d;
+              if (L.dirs_out) reinterpret_cast<F*>(L.dirs_out)[(uint64_t)i * L.B + p] = d;
             } else {
               v[1] = V[p * L.nin + i];
             }

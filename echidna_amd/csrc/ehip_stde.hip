@@ -175,72 +175,108 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
 // A[i = l % 16][k = l / 16] and B[k = l / 16][j = l % 16] and holds
 // D[l / 16 + 4 v][l % 16], v = 0..3, so the 16 lanes of a lane group hold
 // one row's columns and q_s = sum_j +-U[s][j]^2 is a shuffle sum inside the
-// group.  Step t of a 16-coordinate slab gives lane group g coordinate
-// 4 g + t, so a lane's four A operands are consecutive in its sample's row.
-// The products of FP32 (or FP64) operands are formed and summed in binary64,
-// as the lane-per-sample form does, in another order.  One wave per point,
-// four points per workgroup; |L|_F^2 is summed from the same loads.
-template <class F>
-__device__ __forceinline__ void load4(const F* p, uint32_t cnt, bool vec, double a[4]) {
-  if (cnt >= 4 && vec) {
-    if constexpr (sizeof(F) == 4) {
-      const float4 x = *reinterpret_cast<const float4*>(p);
-      a[0] = x.x, a[1] = x.y, a[2] = x.z, a[3] = x.w;
-    } else {
-      const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
-      a[0] = x.x, a[1] = x.y, a[2] = y.x, a[3] = y.y;
-    }
-    return;
-  }
-#pragma unroll
-  for (uint32_t t = 0; t < 4; ++t) a[t] = t < cnt ? (double)p[t] : 0.0;
-}
-
+// group.  The directions are coordinate-major (sample p of the pass at
+// c * npoints * S + p, as the jets wrote them), so the 16 lanes of a group
+// read 16 consecutive samples of one coordinate; step t of a 16-coordinate
+// slab gives lane group g coordinate 4 t + g, so the wave reads four
+// consecutive rows of L.  Four slabs' operands are loaded before their
+// MFMAs run.  The products of FP32 (or FP64) operands are formed and summed
+// in binary64.  One wave per point, four points per workgroup; |L|_F^2 is
+// summed from the same loads.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-template <class F>
+template <class F, uint32_t NRB>
 __global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
+  constexpr uint32_t U = 4;  // slabs of 16 coordinates per batch of loads
   const uint32_t l = threadIdx.x & 63u, g = l >> 4, i16 = l & 15u;
   const uint64_t pt = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
   if (pt >= R.npoints) return;  // whole waves: nothing below synchronises the workgroup
   const F* J = reinterpret_cast<const F*>(R.jets);
-  const F* D = reinterpret_cast<const F*>(R.dirs);
+  const F* D = reinterpret_cast<const F*>(R.dirs_cmajor);
+  const uint64_t NL = R.npoints * (uint64_t)R.S;  // coordinate stride of the directions
   const float* LC = R.lcv + pt * (uint64_t)R.nin * R.rank;
-  const uint32_t n = R.nin, r = R.rank, nrb = (r + 15u) / 16u;
-  const bool vec = (n % 4u) == 0u && (reinterpret_cast<uintptr_t>(D) % 16u) == 0u;
+  const uint32_t n = R.nin, r = R.rank;
   double lnorm = 0.0;
   Wf w{0.0, 0.0, 0.0}, wp{0.0, 0.0, 0.0};
-  for (uint32_t s0 = 0; s0 < R.S; s0 += 16u) {
-    const uint32_t sa = s0 + i16;  // the A row this lane supplies
-    const F* Vrow = D + (pt * R.S + (sa < R.S ? sa : 0u)) * (uint64_t)n;
-    f64x4 acc[2] = {f64x4{0.0, 0.0, 0.0, 0.0}, f64x4{0.0, 0.0, 0.0, 0.0}};
-    for (uint32_t k0 = 0; k0 < n; k0 += 16u) {
-      const uint32_t c0 = k0 + 4u * g;  // this lane group's first coordinate
-      const uint32_t cnt = c0 < n ? min(4u, n - c0) : 0u;
-      double a[4];
-      load4<F>(Vrow + c0, sa < R.S ? cnt : 0u, vec, a);
+  // Lanes past the rank or the sample count read a valid element (column 0,
+  // sample s0) and zero it: no divergent loads in the loop.
+  bool jok[NRB];
+  const float* pb0[NRB];
 #pragma unroll
-      for (uint32_t rb = 0; rb < 2; ++rb) {
-        if (rb >= nrb) break;
-        const uint32_t j = rb * 16u + i16;
+  for (uint32_t rb = 0; rb < NRB; ++rb) {
+    const uint32_t j = rb * 16u + i16;
+    jok[rb] = j < r;
+    pb0[rb] = LC + (uint64_t)g * r + (jok[rb] ? j : 0u);
+  }
+  // one slab (16 coordinates, four MFMA steps) from coordinate k0 + g on
+  auto slab = [&](f64x4* acc, const F*& qa, const float** qb, bool first, bool sok, uint32_t k0) {
+    F a[4];
+    float b[NRB][4];
 #pragma unroll
-        for (uint32_t t = 0; t < 4; ++t) {
-          const double b = (t < cnt && j < r) ? (double)LC[(uint64_t)(c0 + t) * r + j] : 0.0;
-          if (s0 == 0) lnorm += j < R.npos ? b * b : -(b * b);
-          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, acc[rb], 0, 0, 0);
-        }
+    for (uint32_t t = 0; t < 4; ++t) {
+      const bool cok = k0 + 4u * t + g < n;
+      a[t] = cok ? *qa : F(0);
+      qa += 4 * NL;
+#pragma unroll
+      for (uint32_t rb = 0; rb < NRB; ++rb) {
+        b[rb][t] = cok ? *qb[rb] : 0.0f;
+        qb[rb] += 4 * r;
       }
     }
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t)
+#pragma unroll
+      for (uint32_t rb = 0; rb < NRB; ++rb) {
+        const double bd = jok[rb] ? (double)b[rb][t] : 0.0;
+        if (first) lnorm += rb * 16u + i16 < R.npos ? bd * bd : -(bd * bd);
+        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(sok ? (double)a[t] : 0.0, bd, acc[rb], 0, 0, 0);
+      }
+  };
+  for (uint32_t s0 = 0; s0 < R.S; s0 += 16u) {
+    const uint32_t sa = s0 + i16;  // the A row this lane supplies
+    const bool sok = sa < R.S, first = s0 == 0;
+    const F* qa = D + (uint64_t)g * NL + pt * R.S + (sok ? sa : s0);
+    const float* qb[NRB];
+#pragma unroll
+    for (uint32_t rb = 0; rb < NRB; ++rb) qb[rb] = pb0[rb];
+    f64x4 acc[NRB];
+#pragma unroll
+    for (uint32_t rb = 0; rb < NRB; ++rb) acc[rb] = f64x4{0.0, 0.0, 0.0, 0.0};
+    uint32_t k0 = 0;
+    // whole batches of U slabs: every coordinate in range, all loads first
+    for (; k0 + 16u * U <= n; k0 += 16u * U) {
+      F a[U * 4];
+      float b[NRB][U * 4];
+#pragma unroll
+      for (uint32_t q = 0; q < U * 4; ++q) {
+        a[q] = *qa;
+        qa += 4 * NL;
+#pragma unroll
+        for (uint32_t rb = 0; rb < NRB; ++rb) {
+          b[rb][q] = *qb[rb];
+          qb[rb] += 4 * r;
+        }
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < U * 4; ++q)
+#pragma unroll
+        for (uint32_t rb = 0; rb < NRB; ++rb) {
+          const double bd = jok[rb] ? (double)b[rb][q] : 0.0;
+          if (first) lnorm += rb * 16u + i16 < R.npos ? bd * bd : -(bd * bd);
+          acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(sok ? (double)a[q] : 0.0, bd, acc[rb], 0, 0, 0);
+        }
+    }
+    for (; k0 < n; k0 += 16u) slab(acc, qa, qb, first, sok, k0);
     // q for samples s0 + g + 4 v: signed squares summed over the 16 columns of the group
     double qv[4];
 #pragma unroll
     for (uint32_t v = 0; v < 4; ++v) {
       double e = 0.0;
 #pragma unroll
-      for (uint32_t rb = 0; rb < 2; ++rb) {
+      for (uint32_t rb = 0; rb < NRB; ++rb) {
         const uint32_t j = rb * 16u + i16;
-        const double u = acc[rb][v];
-        e += (j < R.npos) ? u * u : -(u * u);
+        const double uu = acc[rb][v];
+        e += (j < R.npos) ? uu * uu : -(uu * uu);
       }
 #pragma unroll
       for (int off = 8; off >= 1; off >>= 1) e += __shfl_xor(e, off, 64);
@@ -272,8 +308,12 @@ __global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
 hipError_t launch_stde_reduce(bool f64, const StdeReduce& R, hipStream_t s) {
   if (R.npoints == 0) return hipSuccess;
   void* args[] = {const_cast<StdeReduce*>(&R)};
-  if (R.lcv && !R.diag && R.rng == RNG_NONE && R.dirs && R.rank <= 32u) {
-    const void* fn = f64 ? (const void*)k_stde_reduce_lcv<double> : (const void*)k_stde_reduce_lcv<float>;
+  if (R.dirs_cmajor) {
+    // the jets' own directions: only the learned control variate reads them
+    if (!R.lcv || R.diag || R.rng != RNG_NONE || R.rank > 32u) return hipErrorInvalidValue;
+    const bool one = R.rank <= 16u;
+    const void* fn = f64 ? (one ? (const void*)k_stde_reduce_lcv<double, 1> : (const void*)k_stde_reduce_lcv<double, 2>)
+                         : (one ? (const void*)k_stde_reduce_lcv<float, 1> : (const void*)k_stde_reduce_lcv<float, 2>);
     return hipLaunchKernel(fn, dim3((uint32_t)((R.npoints + 3) / 4)), dim3(256), args, 0, s);
   }
   const void* fn = f64 ? (const void*)k_stde_reduce<double> : (const void*)k_stde_reduce<float>;
