@@ -9,6 +9,9 @@ resident in HBM; one step = one gradient_batch over the whole batch.  The
 same line carries the rest of the metric, each measured in this run with
 its error against the FP64 oracle and a CPU baseline:
 
+  fanin_f64  the headline evaluation with binary64 adjoint accumulation
+           (FP32 forward and partials, each adjoint's fan-in summed in
+           binary64), its rate and its error against the oracle;
   f64      the same fwd+grad in f64;
   hvp      one Hessian-vector product per point on the same tape, f32;
   stde     plain GPU STDE on the n=256 H=256 MLP (seed 2), B = 65,536,
@@ -152,9 +155,9 @@ def cpu_rates(make_fn, single_units: int, pool_units: int, unit: str, kind_sampl
                       f"threads (the box's CPU share for one GPU, of nproc {info['nproc']})"}
 
 
-def oracle_leg_fwdgrad(tape, x: np.ndarray, y: np.ndarray, g: np.ndarray, f64: bool) -> tuple:
-    """CPU baseline of fwd+grad and the GPU results of a strided sample of
-    points against the f64 oracle."""
+def fwdgrad_error(tape, x: np.ndarray, y: np.ndarray, g: np.ndarray, f64: bool) -> dict:
+    """The GPU values and gradients of a strided sample of points against
+    the f64 oracle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -171,6 +174,16 @@ def oracle_leg_fwdgrad(tape, x: np.ndarray, y: np.ndarray, g: np.ndarray, f64: b
                np.max(np.abs(gy - y_ref[:, 0]) / np.maximum(np.abs(y_ref[:, 0]), mag["value"][:, 0])),
                np.max(np.abs(gg - g_ref) / np.maximum(np.abs(g_ref), mag["grad"]))))}
     err["within_bar"] = bool(err["max_err_over_term_magnitude"] <= err["bar"])
+    return err
+
+
+def oracle_leg_fwdgrad(tape, x: np.ndarray, y: np.ndarray, g: np.ndarray, f64: bool) -> tuple:
+    """CPU baseline of fwd+grad and the GPU results of a strided sample of
+    points against the f64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    err = fwdgrad_error(tape, x, y, g, f64)
     xb = np.ascontiguousarray(x[:1 << 18].astype(np.float64))
 
     def make(threads, units):
@@ -189,7 +202,7 @@ def run_fwdgrad(args, dtype: str, headline: bool) -> dict:
 
     from echidna_amd import costmodel
     from echidna_amd import workloads as wl
-    from echidna_amd.backend import TUNE_GRADIENT, HipContext
+    from echidna_amd.backend import SPEC_GRAD_ACC64, TUNE_GRADIENT, HipContext
     from echidna_amd.tape import GpuError, GpuTapeData
 
     f64 = dtype == "f64"
@@ -301,6 +314,25 @@ def run_fwdgrad(args, dtype: str, headline: bool) -> dict:
                                       dy.cpu().numpy(), dg.cpu().numpy(), f64)
         out["cpu_baseline"] = cpu
         out["error_vs_oracle"] = err
+    if headline and not f64 and spec and "error" not in spec:
+        # the same evaluation with binary64 adjoint accumulation (the FP32
+        # forward sweep and partials, each adjoint's fan-in summed in
+        # binary64), at the tuned launch shape
+        out["fanin_f64"] = {"specialized": ctx.specialize(bufs, SPEC_GRAD_ACC64)}
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                ctx.gradient_batch_dev(bufs, dx, B, dy, dg, stream=sh)
+        torch.cuda.synchronize()
+        for k in range(args.steps):
+            starts[k].record(stream)
+            ctx.gradient_batch_dev(bufs, dx, B, dy, dg, stream=sh)
+            ends[k].record(stream)
+        torch.cuda.synchronize()
+        kms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+        out["fanin_f64"].update({"value": B / (kms * 1e-3), "unit": "points/s", "kernel_ms": kms})
+        if not args.no_cpu:
+            out["fanin_f64"]["error_vs_oracle"] = fwdgrad_error(
+                tape, x.astype(np.float32).astype(np.float64), dy.cpu().numpy(), dg.cpu().numpy(), False)
     ctx.close()
     return out
 
@@ -750,7 +782,7 @@ def run(args) -> int:
         "graph_ms_per_step": head["graph_ms_per_step"],
         "roofline": head["roofline"],
     }
-    for key in ("cpu_baseline", "error_vs_oracle"):
+    for key in ("cpu_baseline", "error_vs_oracle", "fanin_f64"):
         if key in head:
             out[key] = head[key]
     if not args.headline_only:

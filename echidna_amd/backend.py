@@ -149,6 +149,8 @@ class EhipJob(ctypes.Structure):
 
 # Evaluation kinds of the autotuner.
 TUNE_FORWARD, TUNE_GRADIENT, TUNE_HVP, TUNE_TAYLOR = 0, 1, 2, 3
+# specialize(what=SPEC_GRAD_ACC64): f32 gradient kernel with binary64 adjoints
+SPEC_GRAD_ACC64 = 8
 
 _lib = None
 
@@ -541,7 +543,9 @@ class HipContext:
     @staticmethod
     def specialize(tape: "HipTapeBuffers", what: int = 1, waves: int = 0) -> dict:
         """Build the tape-specialised kernel for ``what`` (0 forward, 1 gradient, 2 hvp,
-        3..7 Taylor jets of order 1..5, which the STDE entry points use too).
+        3..7 Taylor jets of order 1..5, which the STDE entry points use too;
+        8: the gradient kernel of an f32 tape with binary64 adjoint
+        accumulation, serving the gradient evaluations in place of 1).
 
         Untuned evaluations of that kind then run it; ``set_tuned`` (fetch 2
         or an interpreter shape) and ``autotune_dev`` choose explicitly.

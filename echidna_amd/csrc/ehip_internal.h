@@ -154,6 +154,11 @@ struct SpecKernel {
 // with the tape).
 constexpr uint32_t SPEC_MAX_ENTRIES = 1u << 16;
 
+// ehip_tape_specialize `what` of the value-and-gradient kernel of an f32
+// tape whose adjoints are accumulated in binary64 (it serves the gradient
+// evaluations, slot 1).
+constexpr int32_t SPEC_GRAD_ACC64 = 8;
+
 namespace ehip_detail {
 int32_t noalloc_setup(int device);  // ehip_noalloc.hip: the fixed scratch's address on `device` (current)
 hipError_t spec_launch(const SpecKernel& s, const Launch& L, uint32_t wg, uint32_t ppl, hipStream_t st);

@@ -227,12 +227,16 @@ std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
 // reverse_tangent_inner, tangent.rs:191-262, as in hvp, tangent.rs:268-310);
 // 3..7 Taylor jets of order what - 2.
 std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
+  // what 8: value and gradient with binary64 adjoints on an f32 tape
+  const bool acc64 = what == SPEC_GRAD_ACC64 && !h.f64;
+  if (what == SPEC_GRAD_ACC64) what = 1;
   if (what >= 3) return spec_source_jets(h, what - 2, waves);
   Gen g(h);
   const bool dual = what == 2;
   const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
   static const char* kWhat[] = {"values", "value and gradient of output 0", "gradient and HVP of output 0"};
-  g.put("// generated: %u entries, %u inputs, %u outputs, %s, %s\n", N, n, m, h.f64 ? "f64" : "f32", kWhat[what]);
+  g.put("// generated: %u entries, %u inputs, %u outputs, %s, %s%s\n", N, n, m, h.f64 ? "f64" : "f32", kWhat[what],
+        acc64 ? ", binary64 adjoints" : "");
   g.put("#include \"ehip_math.h\"\n");
   // occupancy target: the compiler keeps at most 512 / waves registers per
   // lane, spilling the rest
@@ -301,11 +305,19 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
         active[i] = active[h.arg0[i]] || (op != OP_POWI && h.arg1[i] != UNUSED && active[h.arg1[i]]);
     }
     const uint32_t seed = h.outputs[0];
-    g.put(dual ? "D g%u = {F(1), F(0)};\n" : "F g%u = F(1);\n", seed);
+    // Adjoint type: F, or binary64 for an f32 tape (acc64) -- every partial
+    // is still formed in FP32 from the FP32 forward values, and the sums of
+    // an entry's contributions (its fan-in, one per use) are kept in
+    // binary64, so contributions that cancel lose no FP32 bits.
+    const char* A = acc64 ? "double" : "F";
+    const std::string cast = acc64 ? "(double)" : "";
+    if (dual) g.put("D g%u = {F(1), F(0)};\n", seed);
+    else g.put("%s g%u = %s(1);\n", A, seed, A);
     touched[seed] = 1;
     auto declare = [&](uint32_t a) {
       if (!touched[a]) {
-        g.put(dual ? "D g%u = {F(0), F(0)};\n" : "F g%u = F(0);\n", a);
+        if (dual) g.put("D g%u = {F(0), F(0)};\n", a);
+        else g.put("%s g%u = %s(0);\n", A, a, A);
         touched[a] = 1;
       }
     };
@@ -314,7 +326,7 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
       declare(a);
       const std::string ga = "g" + std::to_string(a), gi = "g" + std::to_string(i);
       if (dual) g.out += ga + " = ehip::d_add(" + ga + ", ehip::d_mul(" + P + ", " + gi + "));\n";
-      else g.out += ga + " = " + ga + " + " + P + " * " + gi + ";\n";
+      else g.out += ga + " = " + ga + " + " + cast + P + " * " + gi + ";\n";
     };
     const std::string one = g.lit(1.0), mone = g.lit(-1.0), mzero = g.lit(-0.0);
     for (uint32_t ii = N; ii-- > 0;) {
@@ -349,12 +361,12 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
       if (ua) declare(a);
       if (ub) declare(b);
       const std::string gi = "g" + std::to_string(ii);
-      const std::string live = dual ? "!ehip::d_zero(" + gi + ")" : gi + " != F(0)";
+      const std::string live = dual ? "!ehip::d_zero(" + gi + ")" : gi + " != " + A + "(0)";
       auto nz = [&](const char* d) { return dual ? std::string("!ehip::d_zero(") + d + ")" : std::string(d) + " != F(0)"; };
       auto upd = [&](uint32_t t, const char* d) {
         const std::string gt = "g" + std::to_string(t);
         return dual ? gt + " = ehip::d_add(" + gt + ", ehip::d_mul(" + d + ", " + gi + "));"
-                    : gt + " = " + gt + " + " + d + " * " + gi + ";";
+                    : gt + " = " + gt + " + " + cast + d + " * " + gi + ";";
       };
       if (op == OP_POWI) {
         g.put("{ const %s da = ehip::%s<F>(%d, v%u, v%u);\n", T, dual ? "d_powi_partial" : "s_powi_partial",
@@ -388,7 +400,7 @@ std::string spec_source(const TapeHost& h, int what, uint32_t waves) {
       } else if (dual) {
         g.put("gp[%u] = g%u.re;\nhp[%u] = g%u.eps;\n", k, k, k, k);
       } else {
-        g.put("gp[%u] = g%u;\n", k, k);
+        g.put("gp[%u] = (F)g%u;\n", k, k);
       }
     }
   }
@@ -492,7 +504,7 @@ int32_t spec_build(ehip_ctx* c, const TapeHost& h, int what, uint32_t waves, con
   s.code_bytes = (uint32_t)code.size();
   s.compile_ms = ms;
   s.dual = what == 2;
-  s.order = what >= 3 ? (uint32_t)(what - 2) : 0u;
+  s.order = what >= 3 && what <= 7 ? (uint32_t)(what - 2) : 0u;
   s.waves = waves;
   return EHIP_OK;
 }
@@ -501,10 +513,14 @@ int32_t spec_build(ehip_ctx* c, const TapeHost& h, int what, uint32_t waves, con
 
 int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, uint32_t* out5) try {
   if (!t || !out5) return fail(EHIP_OTHER, "GPU error: null argument");
-  if (what < 0 || what > 7)
+  if (what < 0 || what > SPEC_GRAD_ACC64)
     return fail(EHIP_OTHER,
-                "GPU error: specialised kernels exist for forward (0), gradient (1), hvp (2) and taylor order 1..5 (3..7)");
+                "GPU error: specialised kernels exist for forward (0), gradient (1), hvp (2), taylor order 1..5 (3..7) "
+                "and gradient with binary64 adjoints (8)");
   if (what == 2 && t->nout != 1) return fail(EHIP_OTHER, "GPU error: hvp is defined for scalar-output tapes only");
+  if (what == SPEC_GRAD_ACC64 && t->f64)
+    return fail(EHIP_OTHER, "GPU error: binary64 adjoint accumulation is for f32 tapes (f64 adjoints already are)");
+  const int slot = what == SPEC_GRAD_ACC64 ? 1 : what;  // the evaluation it serves
   if (waves_per_eu > 8) return fail(EHIP_OTHER, "GPU error: waves_per_eu must be 0 (automatic) or 1..8");
   if (t->host.opcodes.size() > SPEC_MAX_ENTRIES)
     return fail(EHIP_OTHER, "GPU error: tape too long to specialise (%zu entries > %u)", t->host.opcodes.size(),
@@ -545,8 +561,8 @@ int32_t ehip_tape_specialize(ehip_tape* t, int32_t what, uint32_t waves_per_eu, 
   (void)hipSetDevice(c->device);
   // a graph captured earlier may still launch the kernel this one
   // replaces: its module stays loaded until the tape is freed
-  if (t->spec[what].mod) t->retired.push_back(t->spec[what]);
-  t->spec[what] = s;
+  if (t->spec[slot].mod) t->retired.push_back(t->spec[slot]);
+  t->spec[slot] = s;
   out5[0] = s.vgprs;
   out5[1] = s.scratch;
   out5[2] = (uint32_t)std::min(s.compile_ms * 1000.0, 4.0e9);
@@ -562,7 +578,8 @@ int32_t ehip_spec_source(int32_t dtype, const uint32_t* opcodes, const uint32_t*
                          const double* constants, uint32_t num_ops, uint32_t num_inputs, uint32_t output_index,
                          const uint32_t* output_indices, uint32_t num_outputs, int32_t what,
                          uint32_t waves_per_eu, const char* arch, char* src, uint64_t src_cap, uint64_t* out3) try {
-  if (!out3 || what < 0 || what > 7 || waves_per_eu > 8 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
+  if (!out3 || what < 0 || what > SPEC_GRAD_ACC64 || waves_per_eu > 8 || (dtype != EHIP_F32 && dtype != EHIP_F64) ||
+      (what == SPEC_GRAD_ACC64 && dtype != EHIP_F32) ||
       (num_ops > 0 && (!opcodes || !arg0 || !arg1 || !constants)))
     return fail(EHIP_OTHER, "GPU error: bad argument");
   for (uint32_t i = 0; i < num_ops; ++i)

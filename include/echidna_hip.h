@@ -300,7 +300,15 @@ int32_t ehip_get_tuned(const ehip_tape* tape, int32_t what, uint32_t order, uint
  * entries are refused.  The compile runs outside the context lock.  Building
  * a kernel again makes the new one serve later calls; the one it replaces
  * stays loaded until the tape is freed, so graphs captured with it replay
- * unchanged. */
+ * unchanged.
+ * what 8 (f32 tapes only) builds the value-and-gradient kernel with binary64
+ * adjoints: the forward sweep and every partial stay FP32, and each entry's
+ * adjoint -- the sum of one contribution per use of the entry -- is kept in
+ * binary64 and rounded to FP32 once, at the gradient store, so cancelling
+ * contributions lose no FP32 bits.  It serves the gradient evaluations in
+ * place of the what 1 kernel (rebuild what 1 to go back).  The reference
+ * accumulates f32 adjoints in f32 (src/gpu/kernels/tape_eval.cu:340-344), so
+ * results differ from it by that rounding; no counterpart in its API. */
 int32_t ehip_tape_specialize(ehip_tape* tape, int32_t what, uint32_t waves_per_eu, uint32_t* out5);
 
 /* Specialise every tape uploaded after this call whose entry count is at

@@ -75,7 +75,9 @@ def test_reverse_sweep_shape():
 def test_source_rejects_bad_input():
     tape, _ = record(lambda v: v[0] * v[1], [1.0, 2.0])
     with pytest.raises(GpuError, match="bad argument"):
-        be.spec_source(_data(tape, False), 8, f64=False)
+        be.spec_source(_data(tape, False), 9, f64=False)
+    with pytest.raises(GpuError, match="bad argument"):  # binary64 adjoints: f32 tapes only
+        be.spec_source(_data(tape, True), be.SPEC_GRAD_ACC64, f64=True)
     d = _data(tape, False)
     d.arg0 = np.array(d.arg0, dtype=np.uint32)
     d.arg0[-1] = 99
@@ -115,3 +117,30 @@ def test_jet_source_compiles_for_every_order(order):
     assert info["code_bytes"] > 0 and f"typedef ehip::Jet<F, {order}> J;" in src
     assert ("ehip::lane_direction" in src) == (order > 1)
     assert src.count("y[") == len(_data(tape, False).output_indices) * order
+
+
+@pytest.mark.parametrize("name", ["kitchen_sink", "mlp", "guards"])
+def test_binary64_adjoint_source(name):
+    """what 8 on an f32 tape: the same FP32 forward sweep and FP32 partials as
+    the gradient kernel, every adjoint declared binary64 with each
+    contribution widened before it is added, one rounding per stored
+    gradient; it compiles for gfx950."""
+    if name == "guards":
+        import dense_cases as dc
+
+        tape = dc.tape()
+    else:
+        tape = TAPES[name]()
+    d = _data(tape, False)
+    src1, _ = be.spec_source(d, 1, f64=False)
+    src8, info = be.spec_source(d, be.SPEC_GRAD_ACC64, f64=False, arch="gfx950")
+    assert info["code_bytes"] > 0 and "binary64 adjoints" in src8
+    fwd1, rev1 = src1.split("F g", 1)
+    fwd8, rev8 = src8.split("double g", 1)
+    assert fwd1.replace(", value and gradient of output 0\n", ", value and gradient of output 0, binary64 adjoints\n") \
+        == fwd8
+    assert not re.search(r"^F g\d+ = ", src8, re.M)
+    # every update widens its contribution; gradients are rounded once
+    updates = re.findall(r"(g\d+) = \1 \+ (.*?) \* g\d+;", rev8)
+    assert updates and all(u[1].startswith("(double)") for u in updates)
+    assert len(re.findall(r"^gp\[\d+\] = \(F\)g\d+;$", src8, re.M)) + src8.count("] = F(0);") == tape.num_inputs
