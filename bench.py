@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--fetch", type=int, choices=[0, 1, 2], default=0,
                     help="with --wg/--lds: program fetch, 0 LDS windows, 1 scalar cache, "
                          "2 tape-specialised kernel (mlp)")
+    ap.add_argument("--ppl", type=int, default=0,
+                    help="with --wg and --fetch 2: points per lane of the tape-specialised kernel")
     ap.add_argument("--no-spec", action="store_true",
                     help="mlp, hvp: leave the tape-specialised kernel out (interpreters only)")
     ap.add_argument("--no-fuse", action="store_true")
@@ -232,7 +234,7 @@ def run_fwdgrad(args, dtype: str, headline: bool) -> dict:
         except GpuError as e:
             spec = {"error": str(e)}
     if args.fetch == 2 and args.wg:
-        ctx.set_tuned(bufs, TUNE_GRADIENT, args.wg, 0, fetch=2)
+        ctx.set_tuned(bufs, TUNE_GRADIENT, args.wg, 0, fetch=2, points_per_lane=args.ppl)
     tdt = torch.float64 if f64 else torch.float32
     dx = torch.from_numpy(x.astype(np.float64 if f64 else np.float32)).cuda()
     dy = torch.empty(B, dtype=tdt, device="cuda")
@@ -285,7 +287,8 @@ def run_fwdgrad(args, dtype: str, headline: bool) -> dict:
     peak = FP64_PEAK if f64 else FP32_PEAK
     achieved = flop * B / (kernel_ms * 1e-3)
     compulsory = costmodel.compulsory_bytes(tape, "gradient", S)
-    launch = tuned or {"wg": args.wg, "lds_budget": args.lds, "fetch": args.fetch}
+    launch = tuned or {"wg": args.wg, "lds_budget": args.lds, "fetch": args.fetch,
+                       "points_per_lane": max(1, args.ppl) if args.fetch == 2 else 1}
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_mlp32_fwdgrad_{dtype}.json")
     if os.path.exists(tpath) and args.batch_log2 == 20:
@@ -293,8 +296,11 @@ def run_fwdgrad(args, dtype: str, headline: bool) -> dict:
             prof = json.load(f)
         # counters hold for the launch shape (and build) they were taken with
         shape = {k: launch.get(k) for k in ("wg", "lds_budget", "fetch")}
+        shape["points_per_lane"] = max(1, launch.get("points_per_lane") or 1)
         for entry in (prof if isinstance(prof, list) else [prof]):
-            if entry.get("launch") == shape:
+            want = dict(entry.get("launch", {}))
+            want.setdefault("points_per_lane", 1)
+            if want == shape:
                 traffic = entry.get("hbm_bytes_per_launch")
     out["roofline"] = {"bound": "fp64 valu" if f64 else "fp32 valu", "achieved": achieved / 1e12,
                        "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
