@@ -225,22 +225,55 @@ inline void free_jet_segments(JetSegProgram& js) {
 // Compile and upload the segmented order-3 jet program of an f32 tape: the
 // forward program with each layer's members in one DENSE, cut at every
 // DENSE the matrix cores can run (finite weights, at least 32 members, at
-// most four immediate terms).  A tape without one keeps js.segs empty.
+// most four immediate terms).  The in-place unary ops the compiler placed
+// right after such a DENSE (its layer's activations) move into the GEMM's
+// epilogue.  A tape without such a DENSE keeps js.segs empty.
 inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fuse, bool dense) {
   js.segs.clear();
   if (h.f64 || !fuse || !dense) return EHIP_OK;
   js.host = compile_program(h, PM_FORWARD, fuse, dense, false, SIZE_MAX);
   std::vector<JetSegment> cut = segment_jet_program(js.host);
-  bool any = false;
-  for (const JetSegment& c : cut) {
-    if (!c.gemm) continue;
+  std::vector<uint32_t>& pool = js.host.pool;
+  auto immediates = [&](const JetSegment& c, uint32_t* idx) {
     uint32_t nimm = 0;
-    for (uint32_t p = 0; p < c.T; ++p)
-      if ((js.host.pool[c.off_regs + p] >> P_KIND_SHIFT) == T_IMM) ++nimm;
-    if (nimm <= 4) any = true;
+    for (uint32_t p = 0; p < c.T && nimm <= 4; ++p)
+      if ((pool[c.off_regs + p] >> P_KIND_SHIFT) == T_IMM) {
+        if (nimm < 4 && idx) idx[nimm] = p;
+        ++nimm;
+      }
+    return nimm;
+  };
+  // per GEMM: the epilogue ops' pool offset and the first instruction after them
+  std::vector<uint32_t> epi_off(cut.size(), ~0u), next(cut.size(), 0);
+  bool any = false;
+  for (size_t ci = 0; ci < cut.size(); ++ci) {
+    const JetSegment& c = cut[ci];
+    if (!c.gemm || immediates(c, nullptr) > 4) continue;
+    any = true;
+    std::vector<uint32_t> member_reg(c.m), ops(c.m, 0);
+    for (uint32_t k = 0; k < c.m; ++k) member_reg[k] = pool[c.off_regs + c.T + k];
+    std::vector<uint32_t> order(c.m);
+    for (uint32_t k = 0; k < c.m; ++k) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return member_reg[x] < member_reg[y]; });
+    uint32_t i = c.i1;
+    bool taken = false;
+    for (; i < js.host.ninstr; ++i) {
+      const uint32_t* w = &js.host.words[(size_t)i * NW];
+      const uint32_t op = w[0] & 0xFFu, fl = w[0] >> 8;
+      if (op >= 64u || !jet_gemm_epilogue_op(op) || (fl & (F_A_IMM | F_HAS_B)) || w[1] != w[2]) break;
+      auto it = std::lower_bound(order.begin(), order.end(), w[2],
+                                 [&](uint32_t k, uint32_t r) { return member_reg[k] < r; });
+      if (it == order.end() || member_reg[*it] != w[2] || ops[*it]) break;
+      ops[*it] = op;
+      taken = true;
+    }
+    next[ci] = i;
+    if (taken) {
+      epi_off[ci] = (uint32_t)pool.size();
+      pool.insert(pool.end(), ops.begin(), ops.end());
+    }
   }
   if (!any) return EHIP_OK;
-  const std::vector<uint32_t>& pool = js.host.pool;
   HIP_TRY(hipMalloc((void**)&js.d_pool, std::max<size_t>(pool.size(), 4) * 4), "program upload");
   HIP_TRY(hipMemcpy(js.d_pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice), "program upload");
   uint32_t i0 = 0;  // instructions not yet in a segment (a DENSE the cores cannot take stays interpreted)
@@ -255,7 +288,8 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     i0 = i1;
     return EHIP_OK;
   };
-  for (const JetSegment& c : cut) {
+  for (size_t ci = 0; ci < cut.size(); ++ci) {
+    const JetSegment& c = cut[ci];
     if (!c.gemm) continue;
     DenseGemm g{};
     g.pool = js.d_pool;
@@ -264,19 +298,16 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     g.m = c.m;
     g.T = c.T;
     g.flags = c.flags;
-    for (uint32_t p = 0; p < c.T && g.nimm <= 4; ++p)
-      if ((pool[c.off_regs + p] >> P_KIND_SHIFT) == T_IMM) {
-        if (g.nimm < 4) g.imm[g.nimm] = p;
-        ++g.nimm;
-      }
+    g.nimm = immediates(c, g.imm);
     if (g.nimm > 4) continue;
+    g.off_epi = epi_off[ci];
     int32_t rc = flush(c.i0);
     if (rc) return rc;
     JetSegDev d;
     d.gemm = true;
     d.g = g;
     js.segs.push_back(d);
-    i0 = c.i1;
+    i0 = next[ci];
   }
   return flush(js.host.ninstr);
 }

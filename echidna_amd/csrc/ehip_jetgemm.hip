@@ -23,6 +23,11 @@
 // into the same pass, and is broadcast to the group's lanes; the matrix
 // cores then form only c1 (and c2).
 //
+// An activation the program computes in place right after the DENSE (the
+// next layer's tanh, say) is applied here, with the interpreter's jet rule,
+// before the member is stored: the member's pre-activation is never written
+// and the interpreter segment after the GEMM skips those instructions.
+//
 // These are the products of the interpreter's exact path (0 + x c per term
 // when x is finite) summed in another order and fused, so results agree
 // with it to f32 rounding.  A lane whose operand coefficients include an Inf
@@ -49,6 +54,34 @@ __device__ __forceinline__ float* row_of(const DenseGemm& G, uint32_t reg, uint3
   return G.regs + ((uint64_t)reg * NC + q) * G.lanes;
 }
 
+// The member's activation when the program computes it in place right
+// after the DENSE (jet_gemm_epilogue_op): the interpreter's own jet rules
+// for these ops (j_eval), applied to the member's jet before it is stored.
+__device__ __forceinline__ void epilogue_op(uint32_t op, float* c) {
+  float r[NC];
+  switch (op) {
+    case OP_TANH: jt::tanh_<float, NC>(c, r); break;
+    case OP_EXP: jt::exp_<float, NC>(c, r); break;
+    case OP_SIN: {
+      float co[NC];
+      jt::sin_cos<float, NC>(c, r, co);
+      break;
+    }
+    case OP_COS: {
+      float sn[NC];
+      jt::sin_cos<float, NC>(c, sn, r);
+      break;
+    }
+    case OP_NEG:
+#pragma unroll
+      for (uint32_t q = 0; q < NC; ++q) r[q] = -c[q];
+      break;
+    default: return;
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < NC; ++q) c[q] = r[q];
+}
+
 // NZ: operand coefficients that can be non-zero (2 when the operands are
 // tape inputs).  PT: c0 groups per tile (0: c0 on the matrix cores too).
 // Three waves per SIMD: accumulators in VGPRs, no spills (four spill).
@@ -69,6 +102,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const uint32_t m = G.m, T = G.T;
   const uint32_t* PR = G.pool + G.off_regs;
   const float* PW = reinterpret_cast<const float*>(G.pool + G.off_w);
+  const uint32_t* PE = G.off_epi != ~0u ? G.pool + G.off_epi : nullptr;  // members' activations
   const uint32_t jw = blockIdx.y * MT + w * 32u;  // this wave's first member
   const uint32_t j = jw + n;                      // the member whose weights this lane supplies
   const bool live = jw < m;
@@ -192,13 +226,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
       }
       const uint32_t dst = PR[T + row];
+      const uint32_t op = PE ? PE[row] : 0u;
 #pragma unroll
       for (uint32_t t = 0; t < 2; ++t) {
         if (((t ? bad1 : bad0) >> n) & 1u) continue;  // recomputed below
         const uint64_t ln = lane0 + t * 32u + n;
-        row_of(G, dst, 0)[ln] = c0[t];
+        float c[NC];
+        c[0] = c0[t];
 #pragma unroll
-        for (uint32_t q = 1; q < NC; ++q) row_of(G, dst, q)[ln] = (q >= Q0 && q - Q0 < NQ) ? acc[q - Q0][t][v] : 0.f;
+        for (uint32_t q = 1; q < NC; ++q) c[q] = (q >= Q0 && q - Q0 < NQ) ? acc[q - Q0][t][v] : 0.f;
+        if (op) epilogue_op(op, c);
+#pragma unroll
+        for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[ln] = c[q];
       }
     }
   }
@@ -233,6 +272,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         for (uint32_t q = 0; q < NC; ++q) s[q] = (p == 0) ? term[q] : s[q] + term[q];
       }
       const uint32_t dst = PR[T + jj];
+      if (PE && PE[jj]) epilogue_op(PE[jj], s);
 #pragma unroll
       for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[lane0 + ln] = s[q];
     }

@@ -44,6 +44,7 @@ struct DenseGemm {
   float* regs;
   uint64_t lanes;           // lanes per register row (a multiple of 64)
   uint32_t share;           // c0 rows repeat over groups of this many lanes (0: they need not)
+  uint32_t off_epi;         // pool offset of m epilogue ops (jet_gemm_epilogue_op, 0: none), or ~0u
 };
 
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s);

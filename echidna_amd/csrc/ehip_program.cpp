@@ -9,6 +9,7 @@
 #include <functional>
 #include <map>
 #include <queue>
+#include <tuple>
 
 namespace ehip {
 
@@ -68,6 +69,7 @@ struct AIns {
   uint32_t kop = 0;              // KINK: opcode of the nonsmooth entry
   std::vector<int64_t> members;  // DENSE: destinations; DENSE_REV: adjoints g_j
   std::vector<double> W;         // DENSE / DENSE_REV weights [member][term]
+  bool inplace = false;          // unary op that takes its operand's register
 };
 
 constexpr size_t DENSE_MIN = 4;     // members worth a DENSE instruction
@@ -281,6 +283,63 @@ std::string validate_tape(const uint32_t* opcodes, const uint32_t* arg0, const u
   for (uint32_t j = 0; j < num_outputs; ++j)
     if (outs[j] >= num_variables) return err("output_indices[%u] (%u) >= num_variables (%u)", j, outs[j], num_variables);
   return "";
+}
+
+// The segmented jet program (whole-layer DENSE groups, forward): a unary
+// op whose operand is a DENSE member that nothing else reads moves to just
+// after its DENSE and computes in place -- the member's register takes the
+// result -- so the batch GEMM that runs the DENSE can apply it in its
+// epilogue (an activation layer).  The moved op reads only that member,
+// which the DENSE defines, and its result is a value of its own, so moving
+// it earlier changes no value.  Read-out DOT pieces of one chain that the
+// move leaves adjacent merge into one DOT: the same terms in the same order,
+// so every rounding is unchanged.
+void hoist_member_unaries(std::vector<AIns>& prog, size_t NV) {
+  std::vector<uint32_t> reads(NV, 0);
+  auto rd = [&](int64_t v) { if (v >= 0) ++reads[(size_t)v]; };
+  for (const AIns& x : prog) {
+    if (x.op == I_LOADN) continue;
+    if (!x.a.imm) rd(x.a.vid);
+    if (!x.b.imm && (x.flags & F_HAS_B)) rd(x.b.vid);
+    if (x.op == I_DOT || x.op == I_DENSE)
+      for (const ATerm& tm : x.terms) rd(tm.vid);
+    if (x.op == I_DOT && (x.flags & F_DOT_CONT)) rd(x.dst);
+  }
+  std::vector<int64_t> dense_of(NV, -1);  // member vid -> position of its DENSE
+  for (size_t p = 0; p < prog.size(); ++p)
+    if (prog[p].op == I_DENSE)
+      for (int64_t v : prog[p].members) dense_of[(size_t)v] = (int64_t)p;
+  std::vector<std::vector<size_t>> moved(prog.size());
+  std::vector<char> is_moved(prog.size(), 0);
+  for (size_t q = 0; q < prog.size(); ++q) {
+    const AIns& x = prog[q];
+    if (x.op >= 64u || x.op == OPC_POWI || (x.flags & (F_HAS_B | F_A_IMM)) || x.a.vid < 0) continue;
+    const int64_t d = dense_of[(size_t)x.a.vid];
+    if (d < 0 || reads[(size_t)x.a.vid] != 1) continue;
+    moved[(size_t)d].push_back(q);
+    is_moved[q] = 1;
+  }
+  std::vector<AIns> out;
+  out.reserve(prog.size());
+  for (size_t p = 0; p < prog.size(); ++p) {
+    if (is_moved[p]) continue;
+    out.push_back(std::move(prog[p]));
+    for (size_t q : moved[p]) {
+      out.push_back(std::move(prog[q]));
+      out.back().inplace = true;
+    }
+  }
+  prog.clear();
+  for (AIns& x : out) {
+    if (!prog.empty() && x.op == I_DOT && (x.flags & F_DOT_CONT) && prog.back().op == I_DOT &&
+        prog.back().dst == x.dst) {
+      AIns& y = prog.back();
+      y.terms.insert(y.terms.end(), x.terms.begin(), x.terms.end());
+      y.full = false;
+      continue;
+    }
+    prog.push_back(std::move(x));
+  }
 }
 
 Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense, bool dual, size_t dense_chunk) {
@@ -722,6 +781,8 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
   }
 
   if (fuse && dense) group_dense(prog, (int64_t)N, grad ? SIZE_MAX : (dense_chunk ? dense_chunk : DENSE_CHUNK), dual);
+  const bool jet_segments = fuse && dense && mode == PM_FORWARD && !dual && dense_chunk == SIZE_MAX;
+  if (jet_segments) hoist_member_unaries(prog, 2 * (size_t)N);
 
   // ---- live ranges ----
   const size_t NV = 2 * (size_t)N;
@@ -784,19 +845,32 @@ Program compile_program(const TapeHost& t, ProgMode mode, bool fuse, bool dense,
   std::sort(ivs.begin(), ivs.end(), [](const Iv& a, const Iv& b) { return a.s < b.s || (a.s == b.s && a.v < b.v); });
   std::vector<uint32_t> reg(NV, 0);
   std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> freeq;
-  typedef std::pair<int64_t, uint32_t> EndReg;
+  // (end, register, owner): an in-place op hands its operand's register to
+  // its result, so a register is freed only when its current owner ends
+  typedef std::tuple<int64_t, uint32_t, size_t> EndReg;
   std::priority_queue<EndReg, std::vector<EndReg>, std::greater<EndReg>> active;
+  std::vector<size_t> owner;
   uint32_t nregs = 0;
   for (const Iv& iv : ivs) {
-    while (!active.empty() && active.top().first < iv.s) {
-      freeq.push(active.top().second);
+    while (!active.empty() && std::get<0>(active.top()) < iv.s) {
+      const uint32_t r0 = std::get<1>(active.top());
+      if (owner[r0] == std::get<2>(active.top())) freeq.push(r0);
       active.pop();
     }
     uint32_t r;
-    if (!freeq.empty()) { r = freeq.top(); freeq.pop(); }
-    else r = nregs++;
+    const AIns* dx = def[iv.v] >= 0 ? &prog[(size_t)def[iv.v]] : nullptr;
+    if (dx && dx->inplace && dx->dst == (int64_t)iv.v && last[(size_t)dx->a.vid] == iv.s) {
+      r = reg[(size_t)dx->a.vid];  // the operand ends here: its register holds the result
+    } else if (!freeq.empty()) {
+      r = freeq.top();
+      freeq.pop();
+    } else {
+      r = nregs++;
+      owner.push_back(SIZE_MAX);
+    }
     reg[iv.v] = r;
-    active.push({iv.e, r});
+    owner[r] = iv.v;
+    active.push(EndReg{iv.e, r, iv.v});
   }
 
   // ---- encode ----

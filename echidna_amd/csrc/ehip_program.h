@@ -166,6 +166,10 @@ std::vector<uint32_t> encode_stream(const Program& p, bool f64, uint32_t i_begin
 // instructions between them on the interpreter, with the whole register
 // file in HBM so that the launches hand their registers on.
 constexpr uint32_t JET_GEMM_MIN_MEMBERS = 32;
+// Unary ops a DENSE's batch GEMM applies to its members in its epilogue
+// when the program computes them in place right after the DENSE (the
+// activation of an MLP layer): neg 12, exp 17, sin 24, cos 25, tanh 32.
+inline bool jet_gemm_epilogue_op(uint32_t op) { return op == 12 || op == 17 || op == 24 || op == 25 || op == 32; }
 struct JetSegment {
   bool gemm;
   uint32_t i0, i1;                         // interpreter: instructions [i0, i1)

@@ -197,3 +197,54 @@ def test_multi_output_program():
     y, g = emu.run(words, terms, x, 2, 3, info["registers"], nterms=info["nterms"], pool=info["pool"])
     y_ref, g_ref = po.gradient(tape, x)
     assert np.allclose(y, y_ref, rtol=1e-14) and np.allclose(g, g_ref, rtol=1e-14)
+
+
+@pytest.mark.parametrize("which", ["mlp1", "mlp2", "mixed"])
+def test_segmented_jet_program_moves_activations_next_to_their_dense(which):
+    """The segmented jet program (whole-layer DENSE groups, mode 8): each
+    activation whose operand is a DENSE member read by nothing else follows
+    its DENSE at once and computes in place (destination = operand = the
+    member's register), which is what lets the layer's batch GEMM apply it in
+    its epilogue; the read-out DOT pieces left adjacent merge into one DOT.
+    Values are unchanged: the emulated program equals the DOT-level one bit
+    for bit.  In `mixed` a member also feeds a second op, so its activation
+    stays where the tape put it."""
+    if which == "mlp1":
+        tape = wl.tanh_mlp(40, 36, 7)
+    elif which == "mlp2":
+        tape = wl.tanh_mlp2(24, 40, 33, 8)
+    else:
+        W = np.random.default_rng(3).normal(size=(6, 5))
+
+        def f(v):
+            hs = [sum(float(W[j, i]) * v[i] for i in range(5)) for j in range(6)]
+            acts = [h.tanh() if j != 2 else h.exp() for j, h in enumerate(hs)]
+            return sum(acts[1:], acts[0]) + hs[4] * 0.5
+
+        tape, _ = record(f, [0.1 * i for i in range(5)])
+    d = GpuTapeData.from_tape(tape)
+    words, terms, info = compile_program(d, mode=8, f64=False)
+    ops = [int(w[0]) & 0xFF for w in words]
+    dense_at = [i for i, o in enumerate(ops) if o == emu.I_DENSE]
+    assert dense_at
+    pool = info["pool"]
+    for i in dense_at:
+        _, members, _ = emu.dense_parts(words[i], pool, False)
+        members = set(int(r) for r in members)
+        k = i + 1
+        moved = []
+        while k < len(ops) and ops[k] < 64 and int(words[k][1]) == int(words[k][2]) and int(words[k][2]) in members:
+            moved.append(int(words[k][2]))
+            k += 1
+        hidden = 5 if which == "mixed" else len(members)
+        assert len(moved) == hidden and len(set(moved)) == hidden, (which, i, moved)
+    if which != "mixed":
+        assert ops.count(emu.I_DOT) == 1  # the read-out, one instruction
+    x = (wl.points_normal(12, 9, tape.num_inputs) * 0.8).astype(np.float32)
+    ref_words, ref_terms, ref_info = compile_program(d, mode=0, f64=False, dense=False)
+    got = emu.run(words, terms, x, tape.num_inputs, 1, info["registers"], f64=False, mode=0, nterms=info["nterms"],
+                  pool=pool)
+    ref = emu.run(ref_words, ref_terms, x, tape.num_inputs, 1, ref_info["registers"], f64=False, mode=0,
+                  nterms=ref_info["nterms"], pool=ref_info["pool"])
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
