@@ -57,19 +57,32 @@ __device__ __forceinline__ float* row_of(const DenseGemm& G, uint32_t reg, uint3
 // The member's activation when the program computes it in place right
 // after the DENSE (jet_gemm_epilogue_op): the interpreter's own jet rules
 // for these ops (j_eval), applied to the member's jet before it is stored.
-__device__ __forceinline__ void epilogue_op(uint32_t op, float* c) {
+// The primal part -- tanh, exp, sin and cos of c0 -- depends only on c0;
+// where c0 is shared by a group of lanes it is formed once per (member,
+// group) and passed in as p0, p1 (sin, cos), the same values the rule would
+// compute per lane.
+__device__ __forceinline__ void epilogue_primal(uint32_t op, float c0, float& p0, float& p1) {
+  switch (op) {
+    case OP_TANH: p0 = tanhf(c0); break;
+    case OP_EXP: p0 = expf(c0); break;
+    case OP_SIN:
+    case OP_COS: p0 = sinf(c0), p1 = cosf(c0); break;
+    default: break;
+  }
+}
+__device__ __forceinline__ void epilogue_tail(uint32_t op, float p0, float p1, float* c) {
   float r[NC];
   switch (op) {
-    case OP_TANH: jt::tanh_<float, NC>(c, r); break;
-    case OP_EXP: jt::exp_<float, NC>(c, r); break;
+    case OP_TANH: jt::tanh_tail<float, NC>(c, p0, r); break;
+    case OP_EXP: jt::exp_tail<float, NC>(c, p0, r); break;
     case OP_SIN: {
       float co[NC];
-      jt::sin_cos<float, NC>(c, r, co);
+      jt::sin_cos_tail<float, NC>(c, p0, p1, r, co);
       break;
     }
     case OP_COS: {
       float sn[NC];
-      jt::sin_cos<float, NC>(c, sn, r);
+      jt::sin_cos_tail<float, NC>(c, p0, p1, sn, r);
       break;
     }
     case OP_NEG:
@@ -80,6 +93,11 @@ __device__ __forceinline__ void epilogue_op(uint32_t op, float* c) {
   }
 #pragma unroll
   for (uint32_t q = 0; q < NC; ++q) c[q] = r[q];
+}
+__device__ __forceinline__ void epilogue_op(uint32_t op, float* c) {
+  float p0 = 0.f, p1 = 0.f;
+  epilogue_primal(op, c[0], p0, p1);
+  epilogue_tail(op, p0, p1, c);
 }
 
 // NZ: operand coefficients that can be non-zero (2 when the operands are
@@ -95,6 +113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ float As[4][KC][32];  // each wave's weights: [term][member]
   __shared__ float C0s[KC][PTS];   // distinct c0 operands of the stage
   __shared__ float C0r[MT][PTS];   // c0 results
+  __shared__ float P0r[PT ? MT : 1][PTS], P1r[PT ? MT : 1][PTS];  // their activation's primal part
   __shared__ uint32_t bad[2];      // tile lanes (0..31, 32..63) with a non-finite operand coefficient
   const uint32_t tid = threadIdx.x, l = tid & 63u, hi = l >> 5, n = l & 31u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -201,8 +220,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       float imm = 0.f;
       if (j < m)
         for (uint32_t i = 0; i < G.nimm; ++i) imm += PW[(uint64_t)G.imm[i] * m + j];
+      const uint32_t op = (PE && j < m) ? PE[j] : 0u;
 #pragma unroll
-      for (uint32_t t = 0; t < PTS; ++t) C0r[w * 32u + n][t] = c0acc[t] + imm;
+      for (uint32_t t = 0; t < PTS; ++t) {
+        const float c0 = c0acc[t] + imm;
+        C0r[w * 32u + n][t] = c0;
+        float p0 = 0.f, p1 = 0.f;
+        if (op) epilogue_primal(op, c0, p0, p1);
+        P0r[w * 32u + n][t] = p0;
+        P1r[w * 32u + n][t] = p1;
+      }
     }
   }
   __syncthreads();  // bad[] and C0r complete
@@ -235,7 +262,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         c[0] = c0[t];
 #pragma unroll
         for (uint32_t q = 1; q < NC; ++q) c[q] = (q >= Q0 && q - Q0 < NQ) ? acc[q - Q0][t][v] : 0.f;
-        if (op) epilogue_op(op, c);
+        if (op) {
+          if (PT) {
+            const uint32_t grp = (t * 32u + n) / LPP;
+            epilogue_tail(op, P0r[w * 32u + rl][grp], P1r[w * 32u + rl][grp], c);
+          } else {
+            epilogue_op(op, c);
+          }
+        }
 #pragma unroll
         for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[ln] = c[q];
       }

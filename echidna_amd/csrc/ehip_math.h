@@ -620,8 +620,9 @@ template <class F, int K> EH_DEV void scale(const F* a, F s, F* c) {
 #pragma unroll
   for (int k = 0; k < K; ++k) c[k] = s * a[k];
 }
-template <class F, int K> EH_DEV void exp_(const F* a, F* c) {
-  c[0] = exp(a[0]);
+// exp_ with exp(a[0]) given (a batch GEMM forms it once per shared primal)
+template <class F, int K> EH_DEV void exp_tail(const F* a, F e0, F* c) {
+  c[0] = e0;
 #pragma unroll
   for (int k = 1; k < K; ++k) {
     F s = F(0);
@@ -630,6 +631,7 @@ template <class F, int K> EH_DEV void exp_(const F* a, F* c) {
     c[k] = s / F(k);
   }
 }
+template <class F, int K> EH_DEV void exp_(const F* a, F* c) { exp_tail<F, K>(a, exp(a[0]), c); }
 template <class F, int K> EH_DEV void nanj(F* c) {
 #pragma unroll
   for (int k = 0; k < K; ++k) c[k] = s_nan<F>();
@@ -664,9 +666,10 @@ template <class F, int K> EH_DEV void sqrt_(const F* a, F* c) {
     c[k] = (a[k] - s) / two_c0;
   }
 }
-template <class F, int K> EH_DEV void sin_cos(const F* a, F* s, F* co) {
-  s[0] = sin(a[0]);
-  co[0] = cos(a[0]);
+// sin_cos with sin(a[0]) and cos(a[0]) given
+template <class F, int K> EH_DEV void sin_cos_tail(const F* a, F s0, F c0, F* s, F* co) {
+  s[0] = s0;
+  co[0] = c0;
 #pragma unroll
   for (int k = 1; k < K; ++k) {
     F inv_k = F(1) / F(k);
@@ -679,6 +682,9 @@ template <class F, int K> EH_DEV void sin_cos(const F* a, F* s, F* co) {
     s[k] = ss * inv_k;
     co[k] = -sc * inv_k;
   }
+}
+template <class F, int K> EH_DEV void sin_cos(const F* a, F* s, F* co) {
+  sin_cos_tail<F, K>(a, sin(a[0]), cos(a[0]), s, co);
 }
 template <class F, int K> EH_DEV void sinh_cosh(const F* a, F* sh, F* ch) {
   sh[0] = sinh(a[0]);
@@ -742,9 +748,10 @@ template <class F, int K> EH_DEV void tan_(const F* a, F* c) {
     sc[k] = sk;
   }
 }
-template <class F, int K> EH_DEV void tanh_(const F* a, F* c) {
+// tanh_ with tanh(a[0]) given
+template <class F, int K> EH_DEV void tanh_tail(const F* a, F t0, F* c) {
   F sc[K];
-  c[0] = tanh(a[0]);
+  c[0] = t0;
   sc[0] = F(1) - c[0] * c[0];
 #pragma unroll
   for (int k = 1; k < K; ++k) {
@@ -758,6 +765,7 @@ template <class F, int K> EH_DEV void tanh_(const F* a, F* c) {
     sc[k] = -sk;
   }
 }
+template <class F, int K> EH_DEV void tanh_(const F* a, F* c) { tanh_tail<F, K>(a, tanh(a[0]), c); }
 template <class F, int K> EH_DEV void asinh_(const F* a, F* c) {
   F s1[K], s2[K];
   c[0] = asinh(a[0]);
