@@ -292,8 +292,13 @@ class HipTapeBuffers:
         self.dtype = np.float64 if f64 else np.float32
 
     def program_info(self, mode: int = 1) -> dict:
+        """Modes 0..2: the forward, gradient and HVP programs; 3: the segmented
+        order-3 jet program of an f32 tape (ehip_set_jet_mma)."""
         out = (ctypes.c_uint64 * 4)()
         _check(load_library().ehip_program_info(self.handle, mode, out))
+        if mode == 3:
+            return {"segments": int(out[0]), "gemms": int(out[1]), "gemm_activations": int(out[2]),
+                    "registers": int(out[3])}
         return {"instructions": int(out[0]), "registers": int(out[1]), "words": int(out[2]),
                 "dot_terms": int(out[3])}
 

@@ -241,7 +241,18 @@ uint32_t ehip_num_inputs(const ehip_tape* t) { return t ? t->nin : 0; }
 int32_t ehip_tape_dtype(const ehip_tape* t) { return t ? (t->f64 ? EHIP_F64 : EHIP_F32) : -1; }
 
 int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) try {
-  if (!t || !out4 || mode < 0 || mode > 2) return fail(EHIP_OTHER, "GPU error: bad argument");
+  if (!t || !out4 || mode < 0 || mode > 3) return fail(EHIP_OTHER, "GPU error: bad argument");
+  if (mode == 3) {
+    const JetSegProgram& js = t->jseg;
+    out4[0] = js.segs.size();
+    out4[1] = out4[2] = 0;
+    for (const JetSegDev& sg : js.segs) {
+      out4[1] += sg.gemm ? 1u : 0u;
+      out4[2] += (sg.gemm && sg.g.off_epi != ~0u) ? 1u : 0u;
+    }
+    out4[3] = js.segs.empty() ? 0u : js.host.nregs;
+    return ok();
+  }
   const Program& p = t->prog[mode].host;
   out4[0] = p.ninstr;
   out4[1] = p.nregs;

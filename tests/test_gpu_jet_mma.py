@@ -23,7 +23,7 @@ from echidna_amd import cv as lcv
 from echidna_amd import philox
 from echidna_amd import workloads as wl
 from echidna_amd.backend import TUNE_TAYLOR, HipContext
-from echidna_amd.tape import GpuTapeData
+from echidna_amd.tape import GpuTapeData, record
 from parity_bar import f64_mismatch
 
 pytestmark = pytest.mark.gpu
@@ -60,9 +60,31 @@ def _check(got, tape, X, V, what):
     return worst
 
 
+def _mlp_skip(n=48, H=40, seed=64):
+    """tanh MLP plus x_0 x_1 / 2: two inputs are read again after the first
+    layer, so the inputs keep all their rows (no direction-only seeding)."""
+    W, b, w2 = wl.mlp_weights(n, H, seed)
+
+    def f(x):
+        acc = x[0] * x[1] * 0.5
+        for j in range(H):
+            s = x[0] * float(W[j, 0])
+            for i in range(1, n):
+                s = s + x[i] * float(W[j, i])
+            acc = acc + (s + float(b[j])).tanh() * float(w2[j])
+        return acc
+
+    tape, _ = record(f, [0.0] * n)
+    return tape
+
+
+# tanh (and softplus's exp) run in the GEMM's epilogue, the ridge's ln1p on
+# the interpreter after it; the skip connection reads two inputs again.
 TAPES = [("mlp_n48_h40", lambda: wl.tanh_mlp(48, 40, 61)),
          ("mlp2_n40_h70_h33", lambda: wl.tanh_mlp2(40, 70, 33, 62)),
-         ("mlp_n256_h256", lambda: wl.tanh_mlp(256, 256, 2))]
+         ("mlp_n256_h256", lambda: wl.tanh_mlp(256, 256, 2)),
+         ("mlp_skip_n48_h40", _mlp_skip),
+         ("ridge_n40_h36", lambda: wl.softplus_ridge(40, 36, 63))]
 
 
 @pytest.mark.parametrize("shape", [(256, 24576, 1), (128, 0, 0), (64, 1 << 16, 1)], ids=["split", "hbm", "lds"])
@@ -192,3 +214,20 @@ def test_mma_stde_n256_full_batch(mma, kind):
     print(f"mma stde kind {kind}: worst rel err vs oracle {worst:.3g}; mean z {np.mean(z):.3f}, "
           f"max |z| {np.max(np.abs(z)):.2f}")
     assert abs(np.mean(z)) <= 4.0 / np.sqrt(k) and np.max(np.abs(z)) <= 5.0
+
+
+@pytest.mark.parametrize("name,want", [("mlp_n48_h40", (3, 1, 1)), ("mlp2_n40_h70_h33", (4, 2, 2)),
+                                       ("mlp_skip_n48_h40", (3, 1, 1)), ("ridge_n40_h36", (3, 1, 1))])
+def test_segment_plan(hip, name, want):
+    """The cut: a LOADN segment, the GEMMs (each applying its layer's tanh
+    or exp in its epilogue), the read-out (the skip connection's product
+    rides in the first segment, the ridge's ln1p after the GEMM); the
+    activations take their members' registers, so the register file holds
+    the inputs and one layer's members."""
+    tape = dict(TAPES)[name]()
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    info = bufs.program_info(3)
+    assert (info["segments"], info["gemms"], info["gemm_activations"]) == want, info
+    n = tape.num_inputs
+    widths = {"mlp_n48_h40": 40, "mlp2_n40_h70_h33": 70, "mlp_skip_n48_h40": 40, "ridge_n40_h36": 36}
+    assert info["registers"] <= n + widths[name] + 2, info
