@@ -298,7 +298,7 @@ class HipTapeBuffers:
         _check(load_library().ehip_program_info(self.handle, mode, out))
         if mode == 3:
             return {"segments": int(out[0]), "gemms": int(out[1]), "gemm_activations": int(out[2]),
-                    "registers": int(out[3])}
+                    "gemm_readouts": int(out[3])}
         return {"instructions": int(out[0]), "registers": int(out[1]), "words": int(out[2]),
                 "dot_terms": int(out[3])}
 

@@ -245,12 +245,12 @@ int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) try 
   if (mode == 3) {
     const JetSegProgram& js = t->jseg;
     out4[0] = js.segs.size();
-    out4[1] = out4[2] = 0;
+    out4[1] = out4[2] = out4[3] = 0;
     for (const JetSegDev& sg : js.segs) {
       out4[1] += sg.gemm ? 1u : 0u;
       out4[2] += (sg.gemm && sg.g.off_epi != ~0u) ? 1u : 0u;
+      out4[3] += (sg.gemm && sg.g.off_ro != ~0u) ? 1u : 0u;
     }
-    out4[3] = js.segs.empty() ? 0u : js.host.nregs;
     return ok();
   }
   const Program& p = t->prog[mode].host;

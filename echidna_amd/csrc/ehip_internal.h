@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -222,6 +223,44 @@ inline void free_jet_segments(JetSegProgram& js) {
   js.d_pool = nullptr;
 }
 
+// Read-out fusion check: true when no instruction from `from` on reads any
+// of the registers `regs` before overwriting it.
+inline bool registers_dead_from(const Program& p, uint32_t from, const std::vector<uint32_t>& regs) {
+  const uint32_t T = p.nterms;
+  std::vector<char> live(p.nregs + 1, 0);
+  for (uint32_t r : regs)
+    if (r < live.size()) live[r] = 1;
+  auto rd = [&](uint32_t r) { return r < live.size() && live[r]; };
+  auto wr = [&](uint32_t r) { if (r < live.size()) live[r] = 0; };
+  for (uint32_t i = from; i < p.ninstr; ++i) {
+    const uint32_t* w = &p.words[(size_t)i * NW];
+    const uint32_t op = w[0] & 0xFFu, fl = w[0] >> 8;
+    if (op < 64u) {
+      if (!(fl & F_A_IMM) && rd(w[2])) return false;
+      if ((fl & F_HAS_B) && !(fl & F_B_IMM) && rd(w[4])) return false;
+      wr(w[1]);
+    } else if (op == I_DOT) {
+      for (uint32_t k = 0; k < w[2]; ++k)
+        if ((p.terms[w[3] + k] & 3u) != T_IMM && rd(p.terms[T + w[3] + k])) return false;
+      if ((fl & F_DOT_CONT) && rd(w[1])) return false;
+      wr(w[1]);
+    } else if (op == I_DENSE) {
+      for (uint32_t k = 0; k < w[2]; ++k) {
+        const uint32_t tw = p.pool[w[3] + k];
+        if ((tw >> P_KIND_SHIFT) != T_IMM && rd(tw & P_REG_MASK)) return false;
+      }
+      for (uint32_t j = 0; j < w[1]; ++j) wr(p.pool[w[3] + w[2] + j]);
+    } else if (op == I_LOADN) {
+      for (uint32_t k = 0; k < w[2]; ++k) wr(p.terms[T + w[3] + k]);
+    } else if (op == I_STORE_OUT) {
+      if (!(fl & F_A_IMM) && rd(w[2])) return false;
+    } else {
+      return false;
+    }
+  }
+  return true;
+}
+
 // Compile and upload the segmented order-3 jet program of an f32 tape: the
 // forward program with each layer's members in one DENSE, cut at every
 // DENSE the matrix cores can run (finite weights, at least 32 members, at
@@ -244,7 +283,7 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     return nimm;
   };
   // per GEMM: the epilogue ops' pool offset and the first instruction after them
-  std::vector<uint32_t> epi_off(cut.size(), ~0u), next(cut.size(), 0);
+  std::vector<uint32_t> epi_off(cut.size(), ~0u), ro_off(cut.size(), ~0u), next(cut.size(), 0);
   bool any = false;
   for (size_t ci = 0; ci < cut.size(); ++ci) {
     const JetSegment& c = cut[ci];
@@ -271,6 +310,60 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     if (taken) {
       epi_off[ci] = (uint32_t)pool.size();
       pool.insert(pool.end(), ops.begin(), ops.end());
+    }
+    // Read-out: the next instruction is a DOT whose products take every
+    // member once (finite constant weights) and nothing later reads a
+    // member.  The GEMM sums each workgroup row's members into the register
+    // of the row's first member, and the DOT keeps its other terms (a
+    // continued chain, immediates, other registers) after those partials.
+    if (i < js.host.ninstr) {
+      uint32_t* w = &js.host.words[(size_t)i * NW];
+      const uint32_t Tn = js.host.nterms, n = w[2], first = w[3];
+      if ((w[0] & 0xFFu) == I_DOT) {
+        std::vector<uint32_t> wt(c.m, 0u), keep_meta, keep_reg, keep_bits;
+        std::vector<char> seen(c.m, 0);
+        uint32_t nprod = 0;
+        bool ok = true;
+        for (uint32_t k = 0; k < n && ok; ++k) {
+          const uint32_t meta = js.host.terms[first + k], kind = meta & 3u;
+          const uint32_t reg = js.host.terms[Tn + first + k], bits = js.host.terms[2 * Tn + first + k];
+          auto it = std::lower_bound(order.begin(), order.end(), reg,
+                                     [&](uint32_t x, uint32_t r) { return member_reg[x] < r; });
+          const bool member = kind != T_IMM && it != order.end() && member_reg[*it] == reg;
+          if (!member) {
+            keep_meta.push_back(meta & ~(0xFFu << M_RUN_SHIFT));
+            keep_reg.push_back(reg);
+            keep_bits.push_back(bits);
+            continue;
+          }
+          float wf;
+          std::memcpy(&wf, &bits, 4);
+          if (kind != T_PROD || seen[*it] || !std::isfinite(wf)) ok = false;
+          else {
+            seen[*it] = 1;
+            wt[*it] = bits;
+            ++nprod;
+          }
+        }
+        const uint32_t tiles = (c.m + JET_GEMM_MT - 1) / JET_GEMM_MT;
+        if (ok && nprod == c.m && tiles + keep_meta.size() <= n &&
+            registers_dead_from(js.host, i + 1, member_reg)) {
+          ro_off[ci] = (uint32_t)pool.size();
+          pool.insert(pool.end(), wt.begin(), wt.end());
+          uint32_t k = 0;
+          for (uint32_t y = 0; y < tiles; ++y, ++k) {
+            js.host.terms[first + k] = T_REG | (1u << M_RUN_SHIFT);
+            js.host.terms[Tn + first + k] = member_reg[y * JET_GEMM_MT];
+            js.host.terms[2 * Tn + first + k] = 0u;
+          }
+          for (size_t q = 0; q < keep_meta.size(); ++q, ++k) {
+            js.host.terms[first + k] = keep_meta[q] | (1u << M_RUN_SHIFT);
+            js.host.terms[Tn + first + k] = keep_reg[q];
+            js.host.terms[2 * Tn + first + k] = keep_bits[q];
+          }
+          w[2] = k;
+        }
+      }
     }
   }
   if (!any) return EHIP_OK;
@@ -301,6 +394,7 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     g.nimm = immediates(c, g.imm);
     if (g.nimm > 4) continue;
     g.off_epi = epi_off[ci];
+    g.off_ro = ro_off[ci];
     int32_t rc = flush(c.i0);
     if (rc) return rc;
     JetSegDev d;

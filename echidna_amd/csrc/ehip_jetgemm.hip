@@ -45,7 +45,7 @@ namespace {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t NC = 3;    // coefficients per register (order 3)
-constexpr uint32_t MT = 128;  // members per workgroup: four waves of 32
+constexpr uint32_t MT = JET_GEMM_MT;  // members per workgroup: four waves of 32
 constexpr uint32_t NT = 64;   // lanes per workgroup
 constexpr uint32_t KC = 16;   // terms per stage
 constexpr uint32_t RPT = KC / 4;  // stage rows per wave
@@ -114,6 +114,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ float C0s[KC][PTS];   // distinct c0 operands of the stage
   __shared__ float C0r[MT][PTS];   // c0 results
   __shared__ float P0r[PT ? MT : 1][PTS], P1r[PT ? MT : 1][PTS];  // their activation's primal part
+  __shared__ float RO[4][NT][NC];  // read-out: each wave's partial sums per lane
+  __shared__ float ROB[MT][NC];    // read-out of a recomputed lane: its members in order
   __shared__ uint32_t bad[2];      // tile lanes (0..31, 32..63) with a non-finite operand coefficient
   const uint32_t tid = threadIdx.x, l = tid & 63u, hi = l >> 5, n = l & 31u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -122,6 +124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const uint32_t* PR = G.pool + G.off_regs;
   const float* PW = reinterpret_cast<const float*>(G.pool + G.off_w);
   const uint32_t* PE = G.off_epi != ~0u ? G.pool + G.off_epi : nullptr;  // members' activations
+  const float* PRO = G.off_ro != ~0u ? reinterpret_cast<const float*>(G.pool + G.off_ro) : nullptr;
   const uint32_t jw = blockIdx.y * MT + w * 32u;  // this wave's first member
   const uint32_t j = jw + n;                      // the member whose weights this lane supplies
   const bool live = jw < m;
@@ -234,6 +237,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
   __syncthreads();  // bad[] and C0r complete
   const uint32_t bad0 = bad[0], bad1 = bad[1];
+  float ro[2][NC];  // read-out partial sums of this lane's two columns
+#pragma unroll
+  for (uint32_t t = 0; t < 2; ++t)
+#pragma unroll
+    for (uint32_t q = 0; q < NC; ++q) ro[t][q] = 0.f;
   if (live) {
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
@@ -270,8 +278,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             epilogue_op(op, c);
           }
         }
+        if (PRO) {
+          const float wr = PRO[row];
 #pragma unroll
-        for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[ln] = c[q];
+          for (uint32_t q = 0; q < NC; ++q) ro[t][q] = ro[t][q] + wr * c[q];
+        } else {
+#pragma unroll
+          for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[ln] = c[q];
+        }
+      }
+    }
+  }
+  if (PRO) {
+    // lanes n and n + 32 hold the other 16 rows of the same columns; then
+    // the four waves' 32-member sums meet in LDS, in wave order
+#pragma unroll
+    for (uint32_t t = 0; t < 2; ++t)
+#pragma unroll
+      for (uint32_t q = 0; q < NC; ++q) ro[t][q] = ro[t][q] + __shfl_xor(ro[t][q], 32, 64);
+    if (hi == 0)
+#pragma unroll
+      for (uint32_t t = 0; t < 2; ++t)
+#pragma unroll
+        for (uint32_t q = 0; q < NC; ++q) RO[w][t * 32u + n][q] = ro[t][q];
+    __syncthreads();
+    if (tid < NT * NC) {
+      const uint32_t col = tid / NC, q = tid % NC;
+      if (!(((col < 32 ? bad0 : bad1) >> (col & 31u)) & 1u)) {  // recomputed lanes: below
+        const float v = ((RO[0][col][q] + RO[1][col][q]) + RO[2][col][q]) + RO[3][col][q];
+        row_of(G, PR[T + blockIdx.y * MT], q)[lane0 + col] = v;
       }
     }
   }
@@ -282,6 +317,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const uint32_t jend = min(m, (blockIdx.y + 1) * MT);
   for (uint32_t ln = 0; ln < NT; ++ln) {
     if (!(((ln < 32 ? bad0 : bad1) >> (ln & 31u)) & 1u)) continue;
+    if (PRO) __syncthreads();  // the previous lane's ROB reads are done
     for (uint32_t jj = blockIdx.y * MT + tid; jj < jend; jj += blockDim.x) {
       float s[NC] = {0.f, 0.f, 0.f};
       for (uint32_t p = 0; p < T; ++p) {
@@ -307,8 +343,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
       const uint32_t dst = PR[T + jj];
       if (PE && PE[jj]) epilogue_op(PE[jj], s);
+      if (PRO) {
 #pragma unroll
-      for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[lane0 + ln] = s[q];
+        for (uint32_t q = 0; q < NC; ++q) ROB[jj - blockIdx.y * MT][q] = s[q];
+      } else {
+#pragma unroll
+        for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[lane0 + ln] = s[q];
+      }
+    }
+    if (PRO) {
+      __syncthreads();
+      if (tid < NC) {  // the row's members in order: the full Taylor product with [w, 0, 0] each
+        float acc = 0.f;
+        for (uint32_t jj = blockIdx.y * MT; jj < jend; ++jj) {
+          const uint32_t k = jj - blockIdx.y * MT;
+          const float cw[NC] = {PRO[jj], 0.f, 0.f};
+          float x[NC], term[NC];
+#pragma unroll
+          for (uint32_t q = 0; q < NC; ++q) x[q] = ROB[k][q];
+          jt::mul<float, NC>(cw, x, term);
+          acc = (jj == blockIdx.y * MT) ? term[tid] : acc + term[tid];
+        }
+        row_of(G, PR[T + blockIdx.y * MT], tid)[lane0 + ln] = acc;
+      }
     }
   }
 }

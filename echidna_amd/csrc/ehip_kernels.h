@@ -45,7 +45,15 @@ struct DenseGemm {
   uint64_t lanes;           // lanes per register row (a multiple of 64)
   uint32_t share;           // c0 rows repeat over groups of this many lanes (0: they need not)
   uint32_t off_epi;         // pool offset of m epilogue ops (jet_gemm_epilogue_op, 0: none), or ~0u
+  // Read-out: the DOT after the DENSE (and its activations) sums w_j x_j over
+  // every member, which nothing else reads.  The GEMM then stores, instead of
+  // its members, one partial sum per workgroup row of members (members
+  // [y JET_GEMM_MT, (y + 1) JET_GEMM_MT)) into the register of that row's
+  // first member, and the program's DOT adds the partials.
+  uint32_t off_ro;          // pool offset of the m read-out weights (f32 bits), or ~0u
 };
+
+constexpr uint32_t JET_GEMM_MT = 128;  // members per workgroup of the jet batch GEMM
 
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s);
 

@@ -169,8 +169,8 @@ int32_t ehip_laplacian_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const voi
  * out[1]=registers, out[2]=program words, out[3]=fused dot terms.  Mode 3,
  * the f32 order-3 jet program cut for the matrix cores (ehip_set_jet_mma):
  * out[0]=segments, out[1]=batch-GEMM segments, out[2]=GEMMs that apply
- * their layer's activation in the epilogue, out[3]=registers (each a row
- * per coefficient and lane in HBM). */
+ * their layer's activation in the epilogue, out[3]=GEMMs that also sum the
+ * read-out dot product over their members (which are then never stored). */
 int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
 
 /* Compile a tape into its register program without a device (inspection
