@@ -119,13 +119,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ uint32_t bad[2];      // tile lanes (0..31, 32..63) with a non-finite operand coefficient
   const uint32_t tid = threadIdx.x, l = tid & 63u, hi = l >> 5, n = l & 31u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-  const uint64_t lane0 = (uint64_t)blockIdx.x * NT;
+  // Workgroup -> (lane tile, member row).  Workgroups go to the eight XCDs
+  // in turn (id mod 8), each XCD with its own L2: ids 8 apart share one, so
+  // the member rows of a lane tile are placed 8 ids apart and read the
+  // tile's operand rows from the same L2 at about the same time.
+  const uint32_t nrow = (G.m + MT - 1) / MT, grp = blockIdx.x / (8u * nrow), rem = blockIdx.x % (8u * nrow);
+  const uint32_t by = rem / 8u;                            // member row
+  const uint64_t tile = (uint64_t)grp * 8u + rem % 8u;     // lane tile
+  if (tile * NT >= G.lanes) return;  // padding of the last group: the whole workgroup leaves
+  const uint64_t lane0 = tile * NT;
   const uint32_t m = G.m, T = G.T;
   const uint32_t* PR = G.pool + G.off_regs;
   const float* PW = reinterpret_cast<const float*>(G.pool + G.off_w);
   const uint32_t* PE = G.off_epi != ~0u ? G.pool + G.off_epi : nullptr;  // members' activations
   const float* PRO = G.off_ro != ~0u ? reinterpret_cast<const float*>(G.pool + G.off_ro) : nullptr;
-  const uint32_t jw = blockIdx.y * MT + w * 32u;  // this wave's first member
+  const uint32_t jw = by * MT + w * 32u;  // this wave's first member
   const uint32_t j = jw + n;                      // the member whose weights this lane supplies
   const bool live = jw < m;
   if (tid < 2) bad[tid] = 0u;
@@ -306,7 +314,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       const uint32_t col = tid / NC, q = tid % NC;
       if (!(((col < 32 ? bad0 : bad1) >> (col & 31u)) & 1u)) {  // recomputed lanes: below
         const float v = ((RO[0][col][q] + RO[1][col][q]) + RO[2][col][q]) + RO[3][col][q];
-        row_of(G, PR[T + blockIdx.y * MT], q)[lane0 + col] = v;
+        row_of(G, PR[T + by * MT], q)[lane0 + col] = v;
       }
     }
   }
@@ -314,11 +322,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // Lanes with a non-finite operand: every member of this workgroup term by
   // term with full Taylor products, in program order, as the interpreter's
   // exact path computes them.
-  const uint32_t jend = min(m, (blockIdx.y + 1) * MT);
+  const uint32_t jend = min(m, (by + 1) * MT);
   for (uint32_t ln = 0; ln < NT; ++ln) {
     if (!(((ln < 32 ? bad0 : bad1) >> (ln & 31u)) & 1u)) continue;
     if (PRO) __syncthreads();  // the previous lane's ROB reads are done
-    for (uint32_t jj = blockIdx.y * MT + tid; jj < jend; jj += blockDim.x) {
+    for (uint32_t jj = by * MT + tid; jj < jend; jj += blockDim.x) {
       float s[NC] = {0.f, 0.f, 0.f};
       for (uint32_t p = 0; p < T; ++p) {
         const uint32_t kr = PR[p];
@@ -345,7 +353,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (PE && PE[jj]) epilogue_op(PE[jj], s);
       if (PRO) {
 #pragma unroll
-        for (uint32_t q = 0; q < NC; ++q) ROB[jj - blockIdx.y * MT][q] = s[q];
+        for (uint32_t q = 0; q < NC; ++q) ROB[jj - by * MT][q] = s[q];
       } else {
 #pragma unroll
         for (uint32_t q = 0; q < NC; ++q) row_of(G, dst, q)[lane0 + ln] = s[q];
@@ -355,16 +363,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       __syncthreads();
       if (tid < NC) {  // the row's members in order: the full Taylor product with [w, 0, 0] each
         float acc = 0.f;
-        for (uint32_t jj = blockIdx.y * MT; jj < jend; ++jj) {
-          const uint32_t k = jj - blockIdx.y * MT;
+        for (uint32_t jj = by * MT; jj < jend; ++jj) {
+          const uint32_t k = jj - by * MT;
           const float cw[NC] = {PRO[jj], 0.f, 0.f};
           float x[NC], term[NC];
 #pragma unroll
           for (uint32_t q = 0; q < NC; ++q) x[q] = ROB[k][q];
           jt::mul<float, NC>(cw, x, term);
-          acc = (jj == blockIdx.y * MT) ? term[tid] : acc + term[tid];
+          acc = (jj == by * MT) ? term[tid] : acc + term[tid];
         }
-        row_of(G, PR[T + blockIdx.y * MT], tid)[lane0 + ln] = acc;
+        row_of(G, PR[T + by * MT], tid)[lane0 + ln] = acc;
       }
     }
   }
@@ -391,7 +399,10 @@ hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s) {
   uint32_t pt = 0;
   if (G.share >= NT && G.share % NT == 0) pt = 1;
   else if (G.share >= 4 && NT % G.share == 0) pt = NT / G.share;
-  const dim3 grid((uint32_t)(G.lanes / NT), (G.m + MT - 1) / MT);
+  const uint64_t tiles = G.lanes / NT, nrow = (G.m + MT - 1) / MT;
+  const uint64_t wgs = (tiles + 7) / 8 * 8 * nrow;  // groups of eight lane tiles
+  if (wgs > 0xFFFFFFFFull) return hipErrorInvalidValue;
+  const dim3 grid((uint32_t)wgs);
   void* args[] = {const_cast<DenseGemm*>(&G)};
   const void* fn = (G.flags & D_INPUT_OPERANDS) ? pick<2>(pt) : pick<3>(pt);
   return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
