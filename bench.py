@@ -478,16 +478,15 @@ def stde_oracle_leg(tape, X: np.ndarray, res: np.ndarray, S: int, seed: int, kin
     from echidna_amd import philox
 
     n = X.shape[1]
-    draw = philox.rademacher if kind == lcv.DIR_RADEMACHER else philox.normal
     worst = 0.0
     for p in points:
         idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
-        dirs = draw(seed, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        dirs = philox.direction(kind, seed, idx).reshape(S, n).astype(np.float32).astype(np.float64)
         ref = pyoracle.laplacian_with_stats(tape, X[p].astype(np.float64), dirs)
         worst = max(worst, abs(res[p, 1] - ref["estimate"]) / max(abs(ref["estimate"]), ref["standard_error"]))
     err = {"points": len(points), "max_err_over_max_estimate_se": float(worst), "bar": 1e-4}
     xs = np.repeat(X[:64].astype(np.float64), S, axis=0)
-    vs = draw(seed, philox.STREAM_DIRS, np.arange(xs.size, dtype=np.uint64)).reshape(-1, n)
+    vs = philox.direction(kind, seed, np.arange(xs.size, dtype=np.uint64)).reshape(-1, n)
 
     def make(threads, units):
         return lambda: pyoracle.taylor(tape, xs[:units], vs[:units], K=3, nthreads=threads)

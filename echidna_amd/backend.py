@@ -293,9 +293,12 @@ class HipTapeBuffers:
 
     def program_info(self, mode: int = 1) -> dict:
         """Modes 0..2: the forward, gradient and HVP programs; 3: the segmented
-        order-3 jet program of an f32 tape (ehip_set_jet_mma)."""
+        order-3 jet program of an f32 tape (ehip_set_jet_mma); 4: its direct
+        plan for device-drawn directions."""
         out = (ctypes.c_uint64 * 4)()
         _check(load_library().ehip_program_info(self.handle, mode, out))
+        if mode == 4:
+            return {"direct_gemms": int(out[0]), "inputs_dropped": int(out[1]), "segments_skipped": int(out[2])}
         if mode == 3:
             return {"segments": int(out[0]), "gemms": int(out[1]), "gemm_activations": int(out[2]),
                     "gemm_readouts": int(out[3])}

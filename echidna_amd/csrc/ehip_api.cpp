@@ -241,7 +241,15 @@ uint32_t ehip_num_inputs(const ehip_tape* t) { return t ? t->nin : 0; }
 int32_t ehip_tape_dtype(const ehip_tape* t) { return t ? (t->f64 ? EHIP_F64 : EHIP_F32) : -1; }
 
 int32_t ehip_program_info(const ehip_tape* t, int32_t mode, uint64_t* out4) try {
-  if (!t || !out4 || mode < 0 || mode > 3) return fail(EHIP_OTHER, "GPU error: bad argument");
+  if (!t || !out4 || mode < 0 || mode > 4) return fail(EHIP_OTHER, "GPU error: bad argument");
+  if (mode == 4) {
+    const JetSegProgram& js = t->jseg;
+    out4[0] = js.direct_gemms;
+    out4[1] = js.direct_dropped;
+    out4[2] = out4[3] = 0;
+    for (const JetSegDev& sg : js.segs) out4[2] += sg.skip_direct ? 1u : 0u;
+    return ok();
+  }
   if (mode == 3) {
     const JetSegProgram& js = t->jseg;
     out4[0] = js.segs.size();

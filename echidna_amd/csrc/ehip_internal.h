@@ -13,11 +13,13 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/echidna_hip.h"
 #include "ehip_kernels.h"
 #include "ehip_program.h"
+#include "philox.h"
 
 using namespace ehip;
 
@@ -172,12 +174,20 @@ struct JetSegDev {
   bool gemm = false;
   uint32_t* d_stream = nullptr;  // interpreter segment
   uint32_t nwin = 0;
+  // The same segment in the direct plan (GEMMs read x and device-drawn
+  // directions themselves, the LOADN loads only inputs others read): its
+  // own stream when it holds that LOADN, or nothing left to run.
+  uint32_t* d_stream_direct = nullptr;
+  uint32_t nwin_direct = 0;
+  bool skip_direct = false;
   DenseGemm g{};                 // gemm segment (regs and lanes set per launch)
 };
 struct JetSegProgram {
   Program host;                  // the forward program with whole-layer DENSE groups
   std::vector<JetSegDev> segs;   // empty: no DENSE the matrix cores can run
   uint32_t* d_pool = nullptr;
+  uint32_t direct_gemms = 0;     // GEMMs with a direct form (0: no direct plan)
+  uint32_t direct_dropped = 0;   // inputs the direct plan's LOADN leaves out
 };
 
 struct ehip_tape {
@@ -216,47 +226,60 @@ inline void free_program(DevProgram& dp) {
 }
 
 inline void free_jet_segments(JetSegProgram& js) {
-  for (auto& sg : js.segs)
+  for (auto& sg : js.segs) {
     if (sg.d_stream) (void)hipFree(sg.d_stream);
+    if (sg.d_stream_direct) (void)hipFree(sg.d_stream_direct);
+  }
   if (js.d_pool) (void)hipFree(js.d_pool);
   js.segs.clear();
   js.d_pool = nullptr;
+  js.direct_gemms = js.direct_dropped = 0;
+}
+
+// The registers instruction i of a forward program reads (rd) and then
+// writes (wr), in that order.  False for an instruction it does not know.
+template <class RD, class WR>
+inline bool visit_registers(const Program& p, uint32_t i, RD rd, WR wr) {
+  const uint32_t T = p.nterms;
+  const uint32_t* w = &p.words[(size_t)i * NW];
+  const uint32_t op = w[0] & 0xFFu, fl = w[0] >> 8;
+  if (op < 64u) {
+    if (!(fl & F_A_IMM)) rd(w[2]);
+    if ((fl & F_HAS_B) && !(fl & F_B_IMM)) rd(w[4]);
+    wr(w[1]);
+  } else if (op == I_DOT) {
+    for (uint32_t k = 0; k < w[2]; ++k)
+      if ((p.terms[w[3] + k] & 3u) != T_IMM) rd(p.terms[T + w[3] + k]);
+    if (fl & F_DOT_CONT) rd(w[1]);
+    wr(w[1]);
+  } else if (op == I_DENSE) {
+    for (uint32_t k = 0; k < w[2]; ++k) {
+      const uint32_t tw = p.pool[w[3] + k];
+      if ((tw >> P_KIND_SHIFT) != T_IMM) rd(tw & P_REG_MASK);
+    }
+    for (uint32_t j = 0; j < w[1]; ++j) wr(p.pool[w[3] + w[2] + j]);
+  } else if (op == I_LOADN) {
+    for (uint32_t k = 0; k < w[2]; ++k) wr(p.terms[T + w[3] + k]);
+  } else if (op == I_STORE_OUT) {
+    if (!(fl & F_A_IMM)) rd(w[2]);
+  } else {
+    return false;
+  }
+  return true;
 }
 
 // Read-out fusion check: true when no instruction from `from` on reads any
 // of the registers `regs` before overwriting it.
 inline bool registers_dead_from(const Program& p, uint32_t from, const std::vector<uint32_t>& regs) {
-  const uint32_t T = p.nterms;
   std::vector<char> live(p.nregs + 1, 0);
   for (uint32_t r : regs)
     if (r < live.size()) live[r] = 1;
-  auto rd = [&](uint32_t r) { return r < live.size() && live[r]; };
-  auto wr = [&](uint32_t r) { if (r < live.size()) live[r] = 0; };
   for (uint32_t i = from; i < p.ninstr; ++i) {
-    const uint32_t* w = &p.words[(size_t)i * NW];
-    const uint32_t op = w[0] & 0xFFu, fl = w[0] >> 8;
-    if (op < 64u) {
-      if (!(fl & F_A_IMM) && rd(w[2])) return false;
-      if ((fl & F_HAS_B) && !(fl & F_B_IMM) && rd(w[4])) return false;
-      wr(w[1]);
-    } else if (op == I_DOT) {
-      for (uint32_t k = 0; k < w[2]; ++k)
-        if ((p.terms[w[3] + k] & 3u) != T_IMM && rd(p.terms[T + w[3] + k])) return false;
-      if ((fl & F_DOT_CONT) && rd(w[1])) return false;
-      wr(w[1]);
-    } else if (op == I_DENSE) {
-      for (uint32_t k = 0; k < w[2]; ++k) {
-        const uint32_t tw = p.pool[w[3] + k];
-        if ((tw >> P_KIND_SHIFT) != T_IMM && rd(tw & P_REG_MASK)) return false;
-      }
-      for (uint32_t j = 0; j < w[1]; ++j) wr(p.pool[w[3] + w[2] + j]);
-    } else if (op == I_LOADN) {
-      for (uint32_t k = 0; k < w[2]; ++k) wr(p.terms[T + w[3] + k]);
-    } else if (op == I_STORE_OUT) {
-      if (!(fl & F_A_IMM) && rd(w[2])) return false;
-    } else {
-      return false;
-    }
+    bool read = false;
+    const bool known = visit_registers(
+        p, i, [&](uint32_t r) { read = read || (r < live.size() && live[r]); },
+        [&](uint32_t r) { if (r < live.size()) live[r] = 0; });
+    if (!known || read) return false;
   }
   return true;
 }
@@ -367,17 +390,93 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     }
   }
   if (!any) return EHIP_OK;
+  // Direct plan.  A GEMM whose operands are all inputs the program's LOADN
+  // loads (and nothing rewrites before it) can read x and device-drawn
+  // directions itself; the LOADN of that plan then loads only the inputs
+  // some other instruction reads.
+  std::vector<uint32_t> in_off(cut.size(), ~0u);
+  Program dprog;  // the direct plan's program: the same, with that LOADN
+  uint32_t li = ~0u, dkept = 0;
+  for (uint32_t i = 0; i < js.host.ninstr && li == ~0u; ++i)
+    if ((js.host.words[(size_t)i * NW] & 0xFFu) == I_LOADN) li = i;
+  js.direct_gemms = js.direct_dropped = 0;
+  if (li != ~0u) {
+    const uint32_t Tn = js.host.nterms, nl = js.host.words[(size_t)li * NW + 2], lo = js.host.words[(size_t)li * NW + 3];
+    std::unordered_map<uint32_t, uint32_t> in_of;  // register -> input index
+    for (uint32_t k = 0; k < nl; ++k) in_of[js.host.terms[Tn + lo + k]] = js.host.terms[2 * Tn + lo + k];
+    std::vector<char> is_direct(js.host.ninstr, 0);
+    for (size_t ci = 0; ci < cut.size(); ++ci) {
+      const JetSegment& c = cut[ci];
+      if (!c.gemm || !(c.flags & D_INPUT_OPERANDS) || immediates(c, nullptr) > 4 || c.i0 <= li) continue;
+      std::vector<uint32_t> idx(c.T, 0u);
+      std::unordered_map<uint32_t, char> ops;
+      bool ok = true;
+      for (uint32_t p = 0; p < c.T && ok; ++p) {
+        const uint32_t tw = pool[c.off_regs + p];
+        if ((tw >> P_KIND_SHIFT) == T_IMM) continue;
+        auto it = in_of.find(tw & P_REG_MASK);
+        if (it == in_of.end()) ok = false;
+        else idx[p] = it->second, ops[tw & P_REG_MASK] = 1;
+      }
+      for (uint32_t i = li + 1; i < c.i0 && ok; ++i)
+        ok = visit_registers(js.host, i, [](uint32_t) {}, [&](uint32_t r) { if (ops.count(r)) ok = false; });
+      if (!ok) continue;
+      in_off[ci] = (uint32_t)pool.size();
+      pool.insert(pool.end(), idx.begin(), idx.end());
+      is_direct[c.i0] = 1;
+      ++js.direct_gemms;
+    }
+    if (js.direct_gemms) {
+      // inputs read by an instruction other than a direct GEMM
+      std::unordered_map<uint32_t, char> live, needed;
+      for (auto& e : in_of) live[e.first] = 1;
+      for (uint32_t i = li + 1; i < js.host.ninstr; ++i) {
+        auto wr = [&](uint32_t r) { live.erase(r); };
+        bool known;
+        if (is_direct[i]) known = visit_registers(js.host, i, [](uint32_t) {}, wr);
+        else known = visit_registers(js.host, i, [&](uint32_t r) { if (live.count(r)) needed[r] = 1; }, wr);
+        if (!known) {
+          for (auto& e : live) needed[e.first] = 1;
+          break;
+        }
+      }
+      dprog = js.host;
+      const uint32_t IW = 1;  // f32 immediates: one word per term
+      for (uint32_t k = 0; k < nl; ++k) {
+        const uint32_t reg = js.host.terms[Tn + lo + k];
+        if (!needed.count(reg)) continue;
+        dprog.terms[lo + dkept] = js.host.terms[lo + k];
+        dprog.terms[Tn + lo + dkept] = reg;
+        dprog.terms[2 * Tn + (size_t)(lo + dkept) * IW] = js.host.terms[2 * Tn + (size_t)(lo + k) * IW];
+        ++dkept;
+      }
+      for (uint32_t k = 0; k < dkept; ++k)  // runs of up to 8 terms, as the compiler sets them for a LOADN
+        dprog.terms[lo + k] = (dprog.terms[lo + k] & ~(0xFFu << M_RUN_SHIFT)) | (std::min(8u, dkept - k) << M_RUN_SHIFT);
+      dprog.words[(size_t)li * NW + 2] = dkept;
+      js.direct_dropped = nl - dkept;
+    }
+  }
   HIP_TRY(hipMalloc((void**)&js.d_pool, std::max<size_t>(pool.size(), 4) * 4), "program upload");
   HIP_TRY(hipMemcpy(js.d_pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice), "program upload");
   uint32_t i0 = 0;  // instructions not yet in a segment (a DENSE the cores cannot take stays interpreted)
   auto flush = [&](uint32_t i1) -> int32_t {
     if (i1 <= i0) return EHIP_OK;
-    JetSegDev d;
+    js.segs.push_back(JetSegDev{});
+    JetSegDev& d = js.segs.back();
     const std::vector<uint32_t> st = encode_stream(js.host, false, i0, i1);
     d.nwin = (uint32_t)(st.size() / WIN_WORDS);
     HIP_TRY(hipMalloc((void**)&d.d_stream, st.size() * 4), "program upload");
-    js.segs.push_back(d);
     HIP_TRY(hipMemcpy(d.d_stream, st.data(), st.size() * 4, hipMemcpyHostToDevice), "program upload");
+    if (js.direct_dropped && i0 <= li && li < i1) {
+      if (dkept == 0 && i1 == i0 + 1) {
+        d.skip_direct = true;  // the segment is that LOADN, with nothing left to load
+      } else {
+        const std::vector<uint32_t> sd = encode_stream(dprog, false, i0, i1);
+        d.nwin_direct = (uint32_t)(sd.size() / WIN_WORDS);
+        HIP_TRY(hipMalloc((void**)&d.d_stream_direct, sd.size() * 4), "program upload");
+        HIP_TRY(hipMemcpy(d.d_stream_direct, sd.data(), sd.size() * 4, hipMemcpyHostToDevice), "program upload");
+      }
+    }
     i0 = i1;
     return EHIP_OK;
   };
@@ -395,6 +494,7 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     if (g.nimm > 4) continue;
     g.off_epi = epi_off[ci];
     g.off_ro = ro_off[ci];
+    g.off_in = in_off[ci];
     int32_t rc = flush(c.i0);
     if (rc) return rc;
     JetSegDev d;
@@ -479,22 +579,37 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
   }
   const bool sf = fm == 1;
   if (kind == 2 && K == 3 && !t->f64 && c->jet_mma && !t->jseg.segs.empty()) {
+    const JetSegProgram& js = t->jseg;
     // every register in HBM: the launches hand their registers on
     const uint64_t grid = (L.B + wg - 1) / wg, lanes = grid * wg;
-    const size_t bytes = (size_t)t->jseg.host.nregs * NC * lanes * S;
+    const size_t bytes = (size_t)js.host.nregs * NC * lanes * S;
+    // Direct plan for device-drawn directions: Rademacher signs are drawn in
+    // the GEMMs; Gaussians once, coordinate-major, into the caller's
+    // direction buffer or after the register file.
+    uint32_t direct = 0;
+    if (js.direct_gemms && L.rng == RNG_RADEMACHER) direct = 1;
+    else if (js.direct_dropped && L.rng == RNG_GAUSSIAN) direct = 2;
+    size_t dbytes = (direct == 2 && !L.dirs_out) ? ((size_t)L.nin * L.B * 4 + 255) / 256 * 256 : 0;
     if (fixed) {
+      if (bytes + dbytes > fixed_cap && dbytes) direct = 0, dbytes = 0;  // no room: the plain plan
       if (bytes > fixed_cap)
         return fail(EHIP_OTHER, "GPU error: the register file needs %zu bytes, above the %zu of the fixed scratch", bytes,
                     fixed_cap);
       L.gregs = fixed;
     } else {
       DevBuf& rf = ws ? *ws : c->regs;
-      HIP_TRY(rf.reserve(bytes), "register file");
+      HIP_TRY(rf.reserve(bytes + dbytes), "register file");
       L.gregs = rf.p;
     }
+    float* dcm = nullptr;
+    if (direct == 2) {
+      dcm = L.dirs_out ? (float*)L.dirs_out : (float*)((char*)L.gregs + bytes);
+      hipError_t e = launch_draw_dirs(L.rng, L.seed, L.S, L.B, L.nin, dcm, s);
+      if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    }
     L.R_lds = 0;
-    L.pool = t->jseg.d_pool;
-    for (const JetSegDev& sg : t->jseg.segs) {
+    L.pool = js.d_pool;
+    for (const JetSegDev& sg : js.segs) {
       hipError_t e;
       if (sg.gemm) {
         DenseGemm g = sg.g;
@@ -502,10 +617,21 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
         g.lanes = lanes;
         // c0 is the primal: equal on every lane that reads the same x row
         g.share = L.xstride == 0 ? 64u : (L.S >= 2 ? L.S : 0u);
+        g.direct = (direct && g.off_in != ~0u) ? direct : 0u;
+        g.x = (const float*)L.x;
+        g.xstride = L.xstride;
+        g.S = L.S;
+        g.nin = L.nin;
+        g.B = L.B;
+        g.seed = L.seed;
+        g.dcm = dcm;
+        g.dirs_out = (float*)L.dirs_out;
         e = launch_dense_gemm(g, s);
       } else {
-        L.prog = sg.d_stream;
-        L.nwin = sg.nwin;
+        if (direct && sg.skip_direct) continue;
+        const bool alt = direct && sg.d_stream_direct;
+        L.prog = alt ? sg.d_stream_direct : sg.d_stream;
+        L.nwin = alt ? sg.nwin_direct : sg.nwin;
         e = launch_jet(false, sf, 3, L, wg, 0, s);
       }
       if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -28,6 +28,15 @@
 // before the member is stored: the member's pre-activation is never written
 // and the interpreter segment after the GEMM skips those instructions.
 //
+// Direct operands (DenseGemm::direct): when every operand is a tape input
+// and the directions are drawn on the device, the operand jets (x_i, v_i, 0)
+// come from their sources instead of the input registers: x from its row,
+// v as the Rademacher sign drawn here (philox.h, the block cache holding a
+// lane's 128 consecutive signs) or from directions drawn coordinate-major
+// beforehand (k_draw_dirs).  The values are the ones the interpreter's
+// LOADN writes, so results do not change; the program then skips writing
+// inputs only GEMMs read.
+//
 // These are the products of the interpreter's exact path (0 + x c per term
 // when x is finite) summed in another order and fused, so results agree
 // with it to f32 rounding.  A lane whose operand coefficients include an Inf
@@ -38,6 +47,7 @@
 #include "ehip_kernels.h"
 #include "ehip_math.h"
 #include "ehip_program.h"
+#include "philox.h"
 
 namespace ehip {
 namespace {
@@ -102,9 +112,11 @@ __device__ __forceinline__ void epilogue_op(uint32_t op, float* c) {
 
 // NZ: operand coefficients that can be non-zero (2 when the operands are
 // tape inputs).  PT: c0 groups per tile (0: c0 on the matrix cores too).
+// DM: operand source (DenseGemm::direct; NZ == 2 only).
 // Three waves per SIMD: accumulators in VGPRs, no spills (four spill).
-template <uint32_t NZ, uint32_t PT>
+template <uint32_t NZ, uint32_t PT, uint32_t DM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_dense_gemm(DenseGemm G) {
+  static_assert(DM == 0 || NZ == 2, "direct operands are tape inputs");
   constexpr uint32_t Q0 = PT ? 1u : 0u;  // first coefficient on the matrix cores
   constexpr uint32_t NQ = NZ - Q0;
   constexpr uint32_t PTS = PT ? PT : 1u;
@@ -133,6 +145,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const float* PW = reinterpret_cast<const float*>(G.pool + G.off_w);
   const uint32_t* PE = G.off_epi != ~0u ? G.pool + G.off_epi : nullptr;  // members' activations
   const float* PRO = G.off_ro != ~0u ? reinterpret_cast<const float*>(G.pool + G.off_ro) : nullptr;
+  const uint32_t* PI = G.pool + (DM ? G.off_in : 0u);  // direct: the terms' input indices
   const uint32_t jw = by * MT + w * 32u;  // this wave's first member
   const uint32_t j = jw + n;                      // the member whose weights this lane supplies
   const bool live = jw < m;
@@ -143,18 +156,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   float c0acc[PTS];
 #pragma unroll
   for (uint32_t t = 0; t < PTS; ++t) c0acc[t] = 0.f;
+  // Direct operands: this lane's x row and draw base, and the x row of the
+  // c0 group it loads (lane l < PT), fixed for the whole product.
+  const uint64_t dlane = lane0 + l;
+  const bool dact = DM && dlane < G.B;
+  const uint64_t dx = dact ? (G.S ? dlane / G.S : dlane) * G.xstride : 0;
+  const uint64_t dbase = DM == 1 ? dlane * G.nin : 0;
+  uint64_t dx0 = 0;
+  bool dact0 = false;
+  if (DM && PT && l < PTS) {
+    const uint64_t ln = lane0 + l * LPP;
+    dact0 = ln < G.B;
+    dx0 = dact0 ? (G.S ? ln / G.S : ln) * G.xstride : 0;
+  }
+  DirBlock db;  // direct 1: the block of this lane's last sign
 
   // Stage registers.  Wave w stages terms w RPT .. w RPT + RPT - 1, lane l
   // of each, and its own members' weights for the stage's KC terms.
   // The descriptors travel one stage ahead of the rows: lane r < RPT of the
   // wave holds term w RPT + r's word of the stage after the one loading.
   float sb[RPT][NQ], sc[RPT], sa[KC / 2];
-  uint32_t dsc;
+  uint32_t dsc, din = 0u;
   auto load_desc = [&](uint32_t k0) {
     const uint32_t p = k0 + w * RPT + l;
     dsc = (l < RPT && p < T) ? PR[p] : (T_IMM << P_KIND_SHIFT);
+    if (DM) din = (l < RPT && p < T) ? PI[p] : 0u;
   };
   auto load_stage = [&](uint32_t k0) {
+    // direct 1: the stage's RPT signs of this lane first (bit r set: term r
+    // draws -1), in a loop that holds one copy of the generator
+    uint32_t sg = 0u;
+    if (DM == 1 && dact) {
+#pragma unroll 1
+      for (uint32_t r = 0; r < RPT; ++r) {
+        const uint32_t kr = __builtin_amdgcn_readlane(dsc, r);
+        if ((kr >> P_KIND_SHIFT) == T_IMM) continue;
+        const uint32_t in = __builtin_amdgcn_readlane(din, r);
+        const bool neg = philox_direction(RNG_RADEMACHER, G.seed, dbase + in, db) < 0.0;
+        sg |= (neg ? 1u : 0u) << r;
+        if (G.dirs_out && by == 0) G.dirs_out[(uint64_t)in * G.B + dlane] = neg ? -1.f : 1.f;
+      }
+    }
 #pragma unroll
     for (uint32_t r = 0; r < RPT; ++r) {
 #pragma unroll
@@ -162,10 +204,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       sc[r] = 0.f;
       const uint32_t kr = __builtin_amdgcn_readlane(dsc, r);
       if ((kr >> P_KIND_SHIFT) != T_IMM) {
-        const uint32_t reg = kr & P_REG_MASK;
+        if (DM) {
+          const uint32_t in = __builtin_amdgcn_readlane(din, r);
+          float d = 0.f;
+          if (dact) d = DM == 1 ? (((sg >> r) & 1u) ? -1.f : 1.f) : G.dcm[(uint64_t)in * G.B + dlane];
+          if (Q0 == 0) {
+            sb[r][0] = dact ? G.x[dx + in] : 0.f;
+            sb[r][NQ - 1] = d;
+          } else {
+            sb[r][0] = d;
+          }
+          if (PT && l < PTS) sc[r] = dact0 ? G.x[dx0 + in] : 0.f;
+        } else {
+          const uint32_t reg = kr & P_REG_MASK;
 #pragma unroll
-        for (uint32_t q = 0; q < NQ; ++q) sb[r][q] = row_of(G, reg, Q0 + q)[lane0 + l];
-        if (PT && l < PTS) sc[r] = row_of(G, reg, 0)[lane0 + l * LPP];
+          for (uint32_t q = 0; q < NQ; ++q) sb[r][q] = row_of(G, reg, Q0 + q)[lane0 + l];
+          if (PT && l < PTS) sc[r] = row_of(G, reg, 0)[lane0 + l * LPP];
+        }
       }
     }
 #pragma unroll
@@ -333,9 +388,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const uint32_t kind = kr >> P_KIND_SHIFT;
         const float c[NC] = {PW[(uint64_t)p * m + jj], 0.f, 0.f};
         float x[NC] = {0.f, 0.f, 0.f}, term[NC];
-        if (kind != T_IMM)
+        if (kind != T_IMM) {
+          if (DM) {
+            const uint64_t lp = lane0 + ln;
+            if (lp < G.B) {
+              x[0] = G.x[(G.S ? lp / G.S : lp) * G.xstride + PI[p]];
+              x[1] = DM == 1 ? (float)philox_direction(RNG_RADEMACHER, G.seed, lp * G.nin + PI[p])
+                             : G.dcm[(uint64_t)PI[p] * G.B + lp];
+            }
+          } else {
 #pragma unroll
-          for (uint32_t q = 0; q < NZ; ++q) x[q] = row_of(G, kr & P_REG_MASK, q)[lane0 + ln];
+            for (uint32_t q = 0; q < NZ; ++q) x[q] = row_of(G, kr & P_REG_MASK, q)[lane0 + ln];
+          }
+        }
         if (kind == T_PROD) {
           if (kr & P_CLEFT) jt::mul<float, NC>(c, x, term);
           else jt::mul<float, NC>(x, c, term);
@@ -378,19 +443,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
-template <uint32_t NZ>
+template <uint32_t NZ, uint32_t DM>
 const void* pick(uint32_t pt) {
   switch (pt) {
-    case 1: return (const void*)k_dense_gemm<NZ, 1>;
-    case 2: return (const void*)k_dense_gemm<NZ, 2>;
-    case 4: return (const void*)k_dense_gemm<NZ, 4>;
-    case 8: return (const void*)k_dense_gemm<NZ, 8>;
-    case 16: return (const void*)k_dense_gemm<NZ, 16>;
-    default: return (const void*)k_dense_gemm<NZ, 0>;
+    case 1: return (const void*)k_dense_gemm<NZ, 1, DM>;
+    case 2: return (const void*)k_dense_gemm<NZ, 2, DM>;
+    case 4: return (const void*)k_dense_gemm<NZ, 4, DM>;
+    case 8: return (const void*)k_dense_gemm<NZ, 8, DM>;
+    case 16: return (const void*)k_dense_gemm<NZ, 16, DM>;
+    default: return (const void*)k_dense_gemm<NZ, 0, DM>;
   }
 }
 
+// Lane p's directions over inputs [8 y, 8 y + 8): consecutive components
+// share their Philox block.
+__global__ __launch_bounds__(256) void k_draw_dirs(uint32_t kind, uint64_t seed, uint32_t S, uint64_t B, uint32_t n,
+                                                   float* out) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= B) return;
+  const uint32_t i0 = blockIdx.y * 8u, i1 = min(n, i0 + 8u);
+  DirBlock db;
+  for (uint32_t i = i0; i < i1; ++i) out[(uint64_t)i * B + p] = (float)lane_direction(kind, seed, S, p, n, i, db);
+}
+
 }  // namespace
+
+hipError_t launch_draw_dirs(uint32_t kind, uint64_t seed, uint32_t S, uint64_t B, uint32_t nin, float* out,
+                            hipStream_t s) {
+  if (B == 0 || nin == 0) return hipSuccess;
+  const uint64_t gx = (B + 255) / 256, gy = (nin + 7) / 8;
+  if (gx > 0x7FFFFFFFull || gy > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_draw_dirs, dim3((uint32_t)gx, (uint32_t)gy), dim3(256), 0, s, kind, seed, S, B, nin, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s) {
   if (G.m == 0 || G.lanes == 0) return hipSuccess;
@@ -404,7 +489,12 @@ hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s) {
   if (wgs > 0xFFFFFFFFull) return hipErrorInvalidValue;
   const dim3 grid((uint32_t)wgs);
   void* args[] = {const_cast<DenseGemm*>(&G)};
-  const void* fn = (G.flags & D_INPUT_OPERANDS) ? pick<2>(pt) : pick<3>(pt);
+  if (G.direct > 2 || (G.direct && (!(G.flags & D_INPUT_OPERANDS) || G.off_in == ~0u || !G.x))) return hipErrorInvalidValue;
+  if (G.direct == 2 && !G.dcm) return hipErrorInvalidValue;
+  const void* fn = !(G.flags & D_INPUT_OPERANDS) ? pick<3, 0>(pt)
+                   : G.direct == 1              ? pick<2, 1>(pt)
+                   : G.direct == 2              ? pick<2, 2>(pt)
+                                                : pick<2, 0>(pt);
   return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
 }
 

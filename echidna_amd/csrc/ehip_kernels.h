@@ -51,11 +51,29 @@ struct DenseGemm {
   // [y JET_GEMM_MT, (y + 1) JET_GEMM_MT)) into the register of that row's
   // first member, and the program's DOT adds the partials.
   uint32_t off_ro;          // pool offset of the m read-out weights (f32 bits), or ~0u
+  // Direct operands: every operand is a tape input (jet (x_i, v_i, 0)), and
+  // the GEMM reads x and the lane's direction itself instead of the input
+  // registers, which the program then need not write.
+  uint32_t off_in;          // pool offset of the T terms' input indices, or ~0u (no direct form)
+  uint32_t direct;          // per launch: 0 operand registers; 1 x, Rademacher signs drawn here;
+                            // 2 x, directions from dcm
+  const float* x;           // primal rows: lane p reads row (S ? p / S : p), stride xstride
+  uint64_t xstride;
+  uint32_t S, nin;
+  uint64_t B;               // lanes in the batch (lanes at or past B read zeros)
+  uint64_t seed;            // direct 1: the directions' Philox seed (philox.h)
+  const float* dcm;         // direct 2: directions, coordinate-major [nin][B]
+  float* dirs_out;          // direct 1: the drawn signs written coordinate-major [nin][B], or null
 };
 
 constexpr uint32_t JET_GEMM_MT = 128;  // members per workgroup of the jet batch GEMM
 
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s);
+
+// Directions of lanes [0, B) drawn on the device (philox.h lane_direction,
+// rounded to f32), written coordinate-major: out[i * B + p].
+hipError_t launch_draw_dirs(uint32_t kind, uint64_t seed, uint32_t S, uint64_t B, uint32_t nin, float* out,
+                            hipStream_t s);
 
 // Per-point STDE reduction over S samples (f64 Welford/Chan).
 struct StdeReduce {

@@ -1022,13 +1022,14 @@ __global__ __launch_bounds__(256) void k_dual(Launch L) {
         wrd(T.reg(), cur);
       }
     } else if (op == I_LOADN) {
+      DirBlock db;  // consecutive inputs share a direction block
       load_inputs<F, 2, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
         v[0] = F(0);
         v[1] = F(0);
         if (act) {
           const uint64_t row = L.S ? p / L.S : p;
           v[0] = X[row * L.xstride + i];
-          v[1] = L.rng ? (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i) : V[p * L.nin + i];
+          v[1] = L.rng ? (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i, db) : V[p * L.nin + i];
         }
       });
     } else if (op == I_SEED) {
@@ -1115,6 +1116,7 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
       dense_fwd<F, K, (K <= 2 ? 16 : (K == 3 ? jet3_block<F>() : 4)), true>(R, L.pool, I.u(1), I.u(2), I.u(3), I.u(4),
                                                                           I.u(5), I.u(6), XR);
     } else if (op == I_LOADN) {
+      DirBlock db;  // consecutive inputs share a direction block
       load_inputs<F, K, LDSONLY>(R, ip, I.u(2), [&](uint32_t i, F* v) {
 #pragma unroll
         for (int q = 0; q < K; ++q) v[q] = F(0);
@@ -1123,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_jet(Launch L) {
           v[0] = X[row * L.xstride + i];
           if (K > 1) {
             if (L.rng) {
-              const F d = (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i);
+              const F d = (F)lane_direction(L.rng, L.seed, L.S, p, L.nin, i, db);
               v[1] = d;
               if (L.dirs_out) reinterpret_cast<F*>(L.dirs_out)[(uint64_t)i * L.B + p] = d;
             } else {

@@ -178,8 +178,8 @@ std::vector<uint32_t> emission_order(const TapeHost& h) {
 // Taylor<F, K>, src/bytecode_tape/tangent.rs:97-147, as taylor_forward_kth
 // does per lane): lane p seeds input i with (x_i, v_i, 0, ..) where x is row
 // p / S of X (S = 0: row p) and v is read from V or drawn on the device
-// (philox.h lane_direction, also written to DO[i * B + p] when that is not
-// null), the interpreters' launch convention.
+// (philox.h lane_direction, through a block cache; also written to
+// DO[i * B + p] when that is not null), the interpreters' launch convention.
 std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
   Gen g(h);
   const uint32_t N = (uint32_t)h.opcodes.size(), n = h.num_inputs, m = (uint32_t)h.outputs.size();
@@ -198,9 +198,10 @@ std::string spec_source_jets(const TapeHost& h, int K, uint32_t waves) {
   g.put("for (unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; p < B;\n"
         "     p += (unsigned long long)gridDim.x * blockDim.x) {\n");
   g.put("const F* __restrict__ x = X + (S ? p / S : p) * xs;\n");
+  if (K > 1) g.put("ehip::DirBlock db;  // consecutive inputs share a direction block\n");
   g.put("auto seed_jet = [&](unsigned i) {\n  J r = ehip::jc<F, %d>(x[i]);\n", K);
   if (K > 1)
-    g.put("  if (rng) {\n    r.c[1] = (F)ehip::lane_direction(rng, seed, S, p, %uu, i);\n"
+    g.put("  if (rng) {\n    r.c[1] = (F)ehip::lane_direction(rng, seed, S, p, %uu, i, db);\n"
           "    if (DO) DO[(unsigned long long)i * B + p] = r.c[1];\n  } else {\n    r.c[1] = V[p * %uull + i];\n"
           "  }\n", n, n);
   g.put("  return r;\n};\n");

@@ -122,9 +122,10 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
       double u[CV_RMAX];
 #pragma unroll
       for (int j = 0; j < CV_RMAX; ++j) u[j] = 0.0;
+      DirBlock db;
       if (has)
         for (uint32_t i = part; i < R.nin; i += G) {
-          const double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, i) : (double)D[row * R.nin + i];
+          const double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, i, db) : (double)D[row * R.nin + i];
           const float* li = LC + (uint64_t)i * R.rank;
 #pragma unroll
           for (int j = 0; j < CV_RMAX; ++j)
@@ -147,8 +148,9 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
     if (DG) {
       const F* d = DG + pt * R.diag_stride;
       double cv = 0.0;
+      DirBlock db;
       for (uint32_t j = 0; j < R.nin; ++j) {
-        double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, j) : (double)D[row * R.nin + j];
+        double v = R.rng ? lane_direction(R.rng, R.seed, R.S, row, R.nin, j, db) : (double)D[row * R.nin + j];
         cv += (double)d[j] * v * v;
       }
       sample -= cv;

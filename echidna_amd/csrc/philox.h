@@ -1,9 +1,13 @@
 // Philox4x32-10 (Salmon et al., SC'11) for device and host, plus the draws the
 // package uses.  Same definition as echidna_amd/philox.py: key = (seed, stream),
-// counter = (index_lo, index_hi, 0, 0), one block per scalar draw; uniforms
-// take 53 bits from words 0,1 with a half-ulp offset; Gaussians are Box-Muller
-// in binary64 (cosine branch) from words (0,1) and (2,3); Rademacher signs
-// come from bit 0 of word 0.
+// counter = (block_lo, block_hi, 0, 0).  Scalar draws (philox_draw) take one
+// block each: uniforms take 53 bits from words 0,1 with a half-ulp offset;
+// Gaussians are Box-Muller in binary64 (cosine branch) from words (0,1) and
+// (2,3); Rademacher signs come from bit 0 of word 0.  Direction components
+// (philox_direction, the STDE and jet directions) share blocks: Rademacher
+// component idx is bit idx mod 128 of block idx / 128 (bit b of word b / 32),
+// Gaussian components idx = 2k and 2k + 1 are the cosine and sine branches of
+// block k's Box-Muller pair.
 #pragma once
 // Also compiled at run time by hipRTC (tape-specialised jet kernels), which
 // supplies the HIP runtime declarations and fixed-width integers itself.
@@ -16,7 +20,7 @@ namespace ehip {
 
 struct Philox4 { uint32_t v[4]; };
 
-__host__ __device__ inline Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+__host__ __device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                                  uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
@@ -33,7 +37,7 @@ __host__ __device__ inline Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint3
   return Philox4{{c0, c1, c2, c3}};
 }
 
-__host__ __device__ inline double philox_u53(uint32_t wa, uint32_t wb) {
+__host__ __device__ __forceinline__ double philox_u53(uint32_t wa, uint32_t wb) {
   uint64_t m = ((uint64_t)(wa >> 6) << 27) | (uint64_t)(wb >> 5);
   return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
@@ -44,7 +48,7 @@ enum : uint32_t { RNG_NONE = 0, RNG_RADEMACHER = 1, RNG_GAUSSIAN = 2, RNG_BASIS 
 constexpr uint32_t STREAM_POINTS = 1, STREAM_DIRS = 7, STREAM_CV = 9;
 
 // Draw `idx` of the (seed, stream) sequence: a Rademacher sign or a Gaussian.
-__host__ __device__ inline double philox_draw(uint32_t kind, uint64_t seed, uint32_t stream, uint64_t idx) {
+__host__ __device__ __forceinline__ double philox_draw(uint32_t kind, uint64_t seed, uint32_t stream, uint64_t idx) {
   Philox4 w = philox4x32_10((uint32_t)idx, (uint32_t)(idx >> 32), 0u, 0u, (uint32_t)seed, stream);
   if (kind == RNG_RADEMACHER) return (w.v[0] & 1u) ? -1.0 : 1.0;
   double u1 = philox_u53(w.v[0], w.v[1]);
@@ -52,17 +56,64 @@ __host__ __device__ inline double philox_draw(uint32_t kind, uint64_t seed, uint
   return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
 }
 
-// Direction component `idx` of the (seed, STREAM_DIRS) stream.
-__host__ __device__ inline double philox_direction(uint32_t kind, uint64_t seed, uint64_t idx) {
-  return philox_draw(kind, seed, STREAM_DIRS, idx);
+// Direction components come in blocks: 128 Rademacher signs or 2 Gaussians.
+__host__ __device__ __forceinline__ uint64_t direction_block(uint32_t kind, uint64_t idx) {
+  return kind == RNG_RADEMACHER ? idx >> 7 : idx >> 1;
+}
+
+// One block's direction values, kept while consecutive components of a lane
+// come from it.  Rademacher: the block's four words; Gaussian: the cosine
+// and sine branch.
+struct DirBlock {  // scalar fields: an array here would be indexed through scratch memory
+  uint64_t blk = ~(uint64_t)0;
+  uint32_t w0 = 0u, w1 = 0u, w2 = 0u, w3 = 0u;
+  double g0 = 0.0, g1 = 0.0;
+};
+
+__host__ __device__ __forceinline__ void direction_fill(uint32_t kind, uint64_t seed, uint64_t blk, DirBlock& d) {
+  Philox4 r = philox4x32_10((uint32_t)blk, (uint32_t)(blk >> 32), 0u, 0u, (uint32_t)seed, STREAM_DIRS);
+  d.blk = blk;
+  if (kind == RNG_RADEMACHER) {
+    d.w0 = r.v[0], d.w1 = r.v[1], d.w2 = r.v[2], d.w3 = r.v[3];
+    return;
+  }
+  const double rad = sqrt(-2.0 * log(philox_u53(r.v[0], r.v[1])));
+  const double th = 2.0 * 3.141592653589793 * philox_u53(r.v[2], r.v[3]);
+  d.g0 = rad * cos(th);
+  d.g1 = rad * sin(th);
+}
+
+// Direction component `idx` of the (seed, STREAM_DIRS) stream, through a
+// block cache `d` (refilled when idx leaves its block).
+__host__ __device__ __forceinline__ double philox_direction(uint32_t kind, uint64_t seed, uint64_t idx, DirBlock& d) {
+  const uint64_t blk = direction_block(kind, idx);
+  if (blk != d.blk) direction_fill(kind, seed, blk, d);
+  if (kind == RNG_RADEMACHER) {
+    const uint32_t b = (uint32_t)idx & 127u;
+    const uint32_t lo = (b & 32u) ? d.w1 : d.w0, hi = (b & 32u) ? d.w3 : d.w2;
+    return ((((b & 64u) ? hi : lo) >> (b & 31u)) & 1u) ? -1.0 : 1.0;
+  }
+  return (idx & 1u) ? d.g1 : d.g0;
+}
+
+__host__ __device__ __forceinline__ double philox_direction(uint32_t kind, uint64_t seed, uint64_t idx) {
+  DirBlock d;
+  return philox_direction(kind, seed, idx, d);
 }
 
 // Component i of the direction of lane p (p = point * S + sample) with n
-// inputs: drawn at index p * n + i, or the basis vector e_(p mod S).
-__host__ __device__ inline double lane_direction(uint32_t kind, uint64_t seed, uint32_t S, uint64_t p, uint32_t n,
-                                                 uint32_t i) {
+// inputs: drawn at index p * n + i, or the basis vector e_(p mod S).  `d`
+// caches the block of the previous component read through it.
+__host__ __device__ __forceinline__ double lane_direction(uint32_t kind, uint64_t seed, uint32_t S, uint64_t p, uint32_t n,
+                                                 uint32_t i, DirBlock& d) {
   if (kind == RNG_BASIS) return (p % S) == i ? 1.0 : 0.0;
-  return philox_direction(kind, seed, p * n + i);
+  return philox_direction(kind, seed, p * n + i, d);
+}
+
+__host__ __device__ __forceinline__ double lane_direction(uint32_t kind, uint64_t seed, uint32_t S, uint64_t p, uint32_t n,
+                                                 uint32_t i) {
+  DirBlock d;
+  return lane_direction(kind, seed, S, p, n, i, d);
 }
 
 }  // namespace ehip

@@ -7,10 +7,16 @@ in ``csrc/philox.h`` implements the same function, so points, weights and
 STDE directions drawn on either side agree bit for bit in their u32 words.
 
 Streams used by this package (key = (seed, stream)):
-  counter = (index_lo, index_hi, 0, 0); one Philox block per scalar draw.
+  counter = (block_lo, block_hi, 0, 0); one Philox block per scalar draw.
 Uniform in (0, 1): 53-bit mantissa from words 0,1 with a half-ulp offset.
 Gaussian: Box-Muller in binary64 from two such uniforms (words 0,1 and 2,3),
 keeping the cosine branch.  Rademacher: sign from bit 0 of word 0.
+
+Direction components (``direction``: the STDE and jet directions drawn on
+the device, stream STREAM_DIRS) share blocks: Rademacher component idx is
+bit idx mod 128 of block idx // 128 (bit b of word b // 32, set = -1);
+Gaussian components 2k and 2k + 1 are the cosine and sine branches of block
+k's Box-Muller pair.
 """
 
 from __future__ import annotations
@@ -98,6 +104,34 @@ def normal(seed: int, stream: int, idx) -> np.ndarray:
 def rademacher(seed: int, stream: int, idx) -> np.ndarray:
     w0, _, _, _ = _words(seed, stream, idx)
     return np.where((w0 & np.uint32(1)) == 0, 1.0, -1.0)
+
+
+DIR_RADEMACHER, DIR_GAUSSIAN = 1, 2
+
+
+def direction(kind: int, seed: int, idx) -> np.ndarray:
+    """Direction components ``idx`` of the (seed, STREAM_DIRS) stream, as the
+    device draws them (``csrc/philox.h`` philox_direction)."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    if kind == DIR_RADEMACHER:
+        w = np.stack(_words(seed, STREAM_DIRS, idx >> np.uint64(7)))
+        b = (idx & np.uint64(127)).astype(np.int64)
+        word = np.take_along_axis(w, (b >> 5)[None, ...], axis=0)[0]
+        bit = (word >> (b & 31).astype(np.uint32)) & np.uint32(1)
+        return np.where(bit == 0, 1.0, -1.0)
+    if kind == DIR_GAUSSIAN:
+        w0, w1, w2, w3 = _words(seed, STREAM_DIRS, idx >> np.uint64(1))
+        rad = np.sqrt(-2.0 * np.log(_u53(w0, w1)))
+        th = 2.0 * np.pi * _u53(w2, w3)
+        return np.where((idx & np.uint64(1)) == 0, rad * np.cos(th), rad * np.sin(th))
+    raise ValueError(f"direction kind {kind}: 1 Rademacher or 2 Gaussian")
+
+
+def lane_directions(kind: int, seed: int, n: int, lanes) -> np.ndarray:
+    """Directions of jet lanes ``lanes`` (lane p = point * S + sample) over
+    ``n`` inputs: rows of components p * n .. p * n + n - 1."""
+    lanes = np.asarray(lanes, dtype=np.uint64).reshape(-1, 1)
+    return direction(kind, seed, lanes * np.uint64(n) + np.arange(n, dtype=np.uint64)[None, :])
 
 
 def normal_matrix(seed: int, stream: int, rows: int, cols: int, std: float = 1.0) -> np.ndarray:

@@ -170,7 +170,11 @@ int32_t ehip_laplacian_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, const voi
  * the f32 order-3 jet program cut for the matrix cores (ehip_set_jet_mma):
  * out[0]=segments, out[1]=batch-GEMM segments, out[2]=GEMMs that apply
  * their layer's activation in the epilogue, out[3]=GEMMs that also sum the
- * read-out dot product over their members (which are then never stored). */
+ * read-out dot product over their members (which are then never stored).
+ * Mode 4, that program's direct plan (used when the directions are drawn on
+ * the device): out[0]=GEMMs that read x and the directions themselves,
+ * out[1]=inputs its LOADN no longer loads, out[2]=interpreter segments it
+ * skips, out[3]=0. */
 int32_t ehip_program_info(const ehip_tape* tape, int32_t mode, uint64_t* out4);
 
 /* Compile a tape into its register program without a device (inspection

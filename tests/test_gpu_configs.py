@@ -104,11 +104,10 @@ def test_stde_n256_full_batch(hip, mlp256, kind):
     dX = torch.from_numpy(X).cuda()
     res = lcv.laplacian_batch(hip, bufs, dX, S, kind, seed)
     assert np.all(res[:, 4] == S)
-    draw = philox.rademacher if kind == lcv.DIR_RADEMACHER else philox.normal
     worst = 0.0
     for p in (0, 31337, P - 1):
         idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
-        dirs = draw(seed, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        dirs = philox.direction(kind, seed, idx).reshape(S, n).astype(np.float32).astype(np.float64)
         ref = po.laplacian_with_stats(mlp256, X[p].astype(np.float64), dirs)
         scale = max(abs(ref["estimate"]), ref["standard_error"])
         err = abs(res[p, 1] - ref["estimate"]) / scale
@@ -178,7 +177,7 @@ def test_cv_n1024_plain_and_learned(hip):
     worst = 0.0
     for p in (0, P - 1):
         idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
-        dirs = philox.normal(5, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        dirs = philox.direction(philox.DIR_GAUSSIAN, 5, idx).reshape(S, n).astype(np.float32).astype(np.float64)
         ref = po.laplacian_with_stats(tape, X[p].astype(np.float64), dirs)
         err = abs(plain[p, 1] - ref["estimate"]) / max(abs(ref["estimate"]), ref["standard_error"])
         worst = max(worst, err)

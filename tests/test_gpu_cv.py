@@ -243,7 +243,7 @@ def test_controlled_statistics_match_a_host_restatement(hip, f64, n, rank, npos,
     X = wl.points_normal(31, P, n).astype(dt)
     dX = torch.from_numpy(X).cuda()
     res, plain = lcv.laplacian_batch(hip, bufs, dX, S, lcv.DIR_GAUSSIAN, seed, net, with_plain=True)
-    V = philox.normal(seed, philox.STREAM_DIRS, np.arange(P * S * n, dtype=np.uint64)).reshape(P * S, n).astype(dt)
+    V = philox.direction(philox.DIR_GAUSSIAN, seed, np.arange(P * S * n, dtype=np.uint64)).reshape(P * S, n).astype(dt)
     J = np.empty((P * S, 3), dt)
     dJ = torch.empty(P * S, 3, dtype=tdt, device="cuda")
     hip.taylor_kth_batch_dev(bufs, torch.from_numpy(np.repeat(X, S, axis=0)).cuda(), n,

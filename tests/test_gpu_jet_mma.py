@@ -21,6 +21,7 @@ import torch
 import pyoracle as po
 from echidna_amd import cv as lcv
 from echidna_amd import philox
+from echidna_amd import stde_gpu as sg
 from echidna_amd import workloads as wl
 from echidna_amd.backend import TUNE_TAYLOR, HipContext
 from echidna_amd.tape import GpuTapeData, record
@@ -174,7 +175,7 @@ def test_mma_stde_shared_primal_groups(mma, name, build, S):
     hip.set_jet_mma(True)
     for p in range(P):
         idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
-        dirs = philox.normal(seed, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        dirs = philox.direction(philox.DIR_GAUSSIAN, seed, idx).reshape(S, n).astype(np.float32).astype(np.float64)
         ref = po.laplacian_with_stats(tape, X[p].astype(np.float64), dirs)
         scale = max(abs(ref["estimate"]), ref["standard_error"])
         for got in (res, exact):
@@ -197,11 +198,10 @@ def test_mma_stde_n256_full_batch(mma, kind):
     bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
     dX = torch.from_numpy(X).cuda()
     res = lcv.laplacian_batch(hip, bufs, dX, S, kind, seed)
-    draw = philox.rademacher if kind == lcv.DIR_RADEMACHER else philox.normal
     worst = 0.0
     for p in (0, 31337, P - 1):
         idx = np.arange(p * S * n, (p + 1) * S * n, dtype=np.uint64)
-        dirs = draw(seed, philox.STREAM_DIRS, idx).reshape(S, n).astype(np.float32).astype(np.float64)
+        dirs = philox.direction(kind, seed, idx).reshape(S, n).astype(np.float32).astype(np.float64)
         ref = po.laplacian_with_stats(tape, X[p].astype(np.float64), dirs)
         err = abs(res[p, 1] - ref["estimate"]) / max(abs(ref["estimate"]), ref["standard_error"])
         verr = abs(res[p, 2] - ref["sample_variance"]) / ref["sample_variance"]
@@ -231,3 +231,64 @@ def test_segment_plan(hip, name, want):
     info = bufs.program_info(3)
     got = (info["segments"], info["gemms"], info["gemm_activations"], info["gemm_readouts"])
     assert got == want, info
+
+
+@pytest.mark.parametrize("kind", [philox.DIR_RADEMACHER, philox.DIR_GAUSSIAN])
+@pytest.mark.parametrize("name,want", [("mlp_n48_h40", (1, 48, 1)), ("mlp2_n40_h70_h33", (1, 40, 1)),
+                                       ("mlp_skip_n48_h40", (1, 46, 0)), ("ridge_n40_h36", (1, 40, 1))])
+def test_direct_plan_equals_supplied_directions(mma, kind, name, want):
+    """Directions drawn on the device take the direct plan: the GEMM over
+    tape inputs reads x and the directions itself (Rademacher signs drawn
+    in the GEMM, Gaussians drawn once coordinate-major), and the LOADN loads
+    only inputs another instruction reads (the skip connection's two), so
+    the interpreter segment of a plain MLP is not launched.  The same
+    directions supplied from the host (philox.lane_directions) take the
+    register plan.  Estimates, variances and values are bit-identical: the
+    GEMM sees the values the LOADN would have stored, and the host mirror
+    gives the device's directions.  S = 2 puts c0 on the matrix cores, 16
+    and 64 share it per point; lane counts leave partial tiles."""
+    hip = mma
+    tape = dict(TAPES)[name]()
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    info = bufs.program_info(4)
+    assert (info["direct_gemms"], info["inputs_dropped"], info["segments_skipped"]) == want, info
+    n = tape.num_inputs
+    for P, S, seed in ((150, 2, 5), (97, 16, 6), (13, 64, 7)):
+        X = wl.points_normal(seed, P, n).astype(np.float32)
+        dX = torch.from_numpy(X).cuda()
+        V = philox.lane_directions(kind, seed, n, np.arange(P * S)).astype(np.float32)
+        dV = torch.from_numpy(V).cuda()
+        got = torch.full((P, 5), float("nan"), dtype=torch.float64, device="cuda")
+        ref = torch.full((P, 5), float("nan"), dtype=torch.float64, device="cuda")
+        sg.laplacian_batch_dev(hip, bufs, dX, P, S, kind, seed, got)
+        sg.laplacian_batch_dev(hip, bufs, dX, P, S, sg.DIR_SUPPLIED, 0, ref, d_dirs=dV)
+        torch.cuda.synchronize()
+        g, r = got.cpu().numpy(), ref.cpu().numpy()
+        assert np.all(np.isfinite(r)), (P, S)
+        assert np.array_equal(g.view(np.uint64), r.view(np.uint64)), (P, S, np.argwhere(g != r)[:4])
+
+
+@pytest.mark.parametrize("kind", [lcv.DIR_RADEMACHER, lcv.DIR_GAUSSIAN])
+def test_direct_plan_writes_the_directions_the_cv_reads(mma, kind):
+    """The learned control variate reads the jets' directions back
+    (coordinate-major) to form u = L(x)^T v.  On the direct plan the GEMM
+    writes the Rademacher signs it draws and the draw kernel the Gaussians;
+    the controlled and plain statistics must agree with the interpreter's
+    (register plan, directions written by its LOADN) to f32 rounding: 1e-4
+    of max(|estimate|, SE) of the plain estimator."""
+    hip = mma
+    n = 48
+    tape = dict(TAPES)["mlp_n48_h40"]()
+    bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
+    net = lcv.LearnedControlVariate(hip, n, 4, 32, seed=3, out_scale=1.0)
+    P, S, seed = 40, 16, 11
+    dX = torch.from_numpy(wl.points_normal(12, P, n).astype(np.float32)).cuda()
+    a, ap = lcv.laplacian_batch(hip, bufs, dX, S, kind, seed, net, with_plain=True)
+    hip.set_jet_mma(False)
+    b, bp = lcv.laplacian_batch(hip, bufs, dX, S, kind, seed, net, with_plain=True)
+    hip.set_jet_mma(True)
+    scale = np.maximum(np.abs(bp[:, 1]), bp[:, 3])
+    assert np.all(np.abs(ap[:, 1] - bp[:, 1]) <= 1e-4 * scale), (ap[:3], bp[:3])
+    assert np.all(np.abs(a[:, 1] - b[:, 1]) <= 1e-4 * scale), (a[:3], b[:3])
+    assert np.all(np.abs(a[:, 2] - b[:, 2]) <= 1e-4 * bp[:, 2])
+    assert not np.allclose(a[:, 1], ap[:, 1])  # the control variate changed the samples

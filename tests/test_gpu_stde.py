@@ -84,8 +84,7 @@ def test_batched_device_directions(hip, kind):
     bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape))
     res = sg.laplacian_batch(hip, bufs, X, S, kind=kind, seed=77)
     idx = np.arange(P * S * n, dtype=np.uint64)
-    draw = philox.rademacher if kind == sg.DIR_RADEMACHER else philox.normal
-    dirs = draw(77, philox.STREAM_DIRS, idx).reshape(P, S, n)
+    dirs = philox.direction(kind, 77, idx).reshape(P, S, n)
     for p in (0, 13, P - 1):
         ref = po.laplacian_with_stats(tape, X[p], dirs[p])
         tol = 1e-10 if kind == sg.DIR_RADEMACHER else 1e-9

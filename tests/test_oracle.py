@@ -35,6 +35,34 @@ def test_philox_distributions():
     assert set(np.unique(r)) == {-1.0, 1.0} and abs(r.mean()) < 0.02
 
 
+def test_direction_blocks():
+    """Device direction components (philox.direction): Rademacher component
+    128 b + k is bit k of block b's 128 bits (word k // 32, -1 when set),
+    from the KAT-pinned generator; Gaussian components 2k, 2k + 1 are one
+    Box-Muller pair of block k: their squares sum to -2 ln u1 and their angle
+    is 2 pi u2 (u from words 0,1 and 2,3).  Then moments: mean 0, variance 1,
+    pairs and neighbouring signs uncorrelated."""
+    seed = 21
+    for b in (0, 1, 7, 2 ** 33 + 5):
+        w = [int(v) for v in philox.philox4x32_10(b & 0xFFFFFFFF, b >> 32, 0, 0, seed, philox.STREAM_DIRS)]
+        got = philox.direction(philox.DIR_RADEMACHER, seed, np.arange(128, dtype=np.uint64) + np.uint64(128 * b))
+        want = [-1.0 if (w[k // 32] >> (k % 32)) & 1 else 1.0 for k in range(128)]
+        assert list(got) == want, b
+    k = np.array([0, 1, 5, 2 ** 34 + 3], dtype=np.uint64)
+    g = philox.direction(philox.DIR_GAUSSIAN, seed, np.stack([2 * k, 2 * k + 1], axis=1).ravel()).reshape(-1, 2)
+    w0, w1, w2, w3 = philox._words(seed, philox.STREAM_DIRS, k)
+    u1 = philox.uniform(seed, philox.STREAM_DIRS, k)
+    u2 = philox._u53(w2, w3)
+    assert np.allclose(g[:, 0] ** 2 + g[:, 1] ** 2, -2 * np.log(u1), rtol=1e-13)
+    assert np.allclose(np.mod(np.arctan2(g[:, 1], g[:, 0]), 2 * np.pi), 2 * np.pi * u2, rtol=1e-12, atol=1e-12)
+    z = philox.direction(philox.DIR_GAUSSIAN, 3, np.arange(400000, dtype=np.uint64))
+    assert abs(z.mean()) < 0.008 and abs(z.var() - 1) < 0.012 and abs(np.mean(z[0::2] * z[1::2])) < 0.012
+    r = philox.direction(philox.DIR_RADEMACHER, 3, np.arange(400000, dtype=np.uint64))
+    assert set(np.unique(r)) == {-1.0, 1.0} and abs(r.mean()) < 0.008 and abs(np.mean(r[1:] * r[:-1])) < 0.008
+    L = philox.lane_directions(philox.DIR_GAUSSIAN, 3, 10, [0, 7])
+    assert np.array_equal(L[1], z[70:80]) and np.array_equal(L[0], z[:10])
+
+
 def test_rosenbrock_gradient_closed_form():
     tape = wl.rosenbrock(2)
     pts = np.array([[-1.2, 1.0], [1.0, 1.0], [0.0, 0.0], [0.3, -0.7]])
