@@ -519,7 +519,10 @@ def run_stde(args, compact: bool = False) -> dict:
     Xh = wl.points_normal(2, P, n).astype(np.float32)
     X = torch.from_numpy(Xh).cuda()
     tuned = tune_jets(ctx, bufs, data, X, 1 << 20, not args.no_jet_mma)
-    flop = costmodel.jet_flops(tape, 3)
+    # jet arithmetic per direction: the per-direction coefficients plus the
+    # point's primal shared by its S directions (costmodel.jet_flops_split)
+    f_point, f_lane = costmodel.jet_flops_split(tape, 3)
+    flop = f_lane + f_point / S
     out = {"value": None, "unit": "Laplacians/s", "dtype": "f32 jets, f64 reduction",
            "jet_dense": "exact per-term" if args.no_jet_mma else "fp32 mfma",
            "config": {"workload": f"plain GPU STDE, tanh-MLP tape n={n} H={H} seed 2 ({tape.counts()['N']} "
@@ -538,7 +541,7 @@ def run_stde(args, compact: bool = False) -> dict:
         results[name] = res
         t_sample = dt / (P * S)
         out[name] = {"seconds": dt, "jets_per_s": P * S / dt, "t_sample_ns": t_sample * 1e9,
-                     "valu_frac": flop * P * S / dt / FP32_PEAK,
+                     "fp32_frac": flop * P * S / dt / FP32_PEAK,
                      **lap_rate(res[:k, 2], exact, t_sample),
                      "mean_rel_se_at_S": float(np.mean(res[:k, 3] / np.abs(exact)))}
     rad = results["rademacher"]
@@ -547,7 +550,8 @@ def run_stde(args, compact: bool = False) -> dict:
                           "mean_abs_z": float(np.mean(np.abs(z))), "frac_within_3se": float(np.mean(np.abs(z) < 3.0)),
                           "mean_rel_err": float(np.mean(np.abs(rad[:k, 1] - exact) / np.abs(exact)))}
     out["value"] = out["rademacher"]["laplacians_per_s"]
-    out["flop_per_jet"] = flop
+    out["flop_per_jet"] = {"per_direction": f_lane, "per_point": f_point, "charged": flop,
+                           "tape_every_coefficient": costmodel.jet_flops(tape, 3)}
     if not args.no_cpu:
         out["cpu_baseline"], out["error_vs_oracle"] = stde_oracle_leg(tape, Xh, rad, S, 2, lcv.DIR_RADEMACHER,
                                                                       (0, P // 2, P - 1))

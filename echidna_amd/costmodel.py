@@ -23,6 +23,12 @@ the instructions any implementation issues):
   multiply of two jets K(K+1)/2 products and K(K-1)/2 adds, unary function
   about 2K^2 (its recurrence), per computed entry.
 
+`jet_flops_split` counts the same jet arithmetic where it has to happen
+when S directions share one point (STDE): the primal coefficient c0 is a
+function of x alone, once per point, and the higher coefficients once per
+direction; coefficients that are zero by construction (an input's jet is
+(x, v, 0, ..), a constant's (c, 0, ..)) and their products are not counted.
+
 `compulsory_bytes` is the HBM traffic no implementation avoids: the inputs
 read once and the results written once.
 """
@@ -143,6 +149,71 @@ def jet_flops(tape, K: int = 3) -> int:
         else:
             total += 2 * K * K
     return total
+
+
+def jet_flops_split(tape, K: int = 3):
+    """(per point, per direction) arithmetic of order-K jets seeded (x, v, 0, ..):
+    per coefficient k that can be non-zero, a multiply by a constant or a
+    negation costs 1, and an add 1 where both of its operands can be
+    non-zero (else it copies); a product of two jets costs 2p - 1 for its p
+    non-zero pairs a_j b_(k-j); an elementary function (and a division or
+    binary function of two jets) costs 2K per coefficient, as `jet_flops`
+    charges 2K^2 per entry.  Coefficient 0 goes to the point, the rest to
+    the direction.  An elementary function's c_k is non-zero when one of its
+    operand's c_1 .. c_k is."""
+    ops, a0, a1, _ = _arrays(tape)
+    full = (1 << K) - 1
+    nz = np.zeros(len(ops), dtype=np.int64)
+    point = lane = 0
+    for i in range(len(ops)):
+        op = int(ops[i])
+        if op == int(O.Input):
+            nz[i] = 0b11 & full
+            continue
+        if op == int(O.Const):
+            nz[i] = 1
+            continue
+        _, operands = _operands(ops, a0, a1, i)
+        ms = [int(nz[a]) for _, a in operands]
+        var = [ops[a] != int(O.Const) for _, a in operands]
+        cost = [0] * K
+        if op in _LINEAR and len(ms) == 2:
+            nz[i] = ms[0] | ms[1]
+            for k in range(K):  # an add where both coefficients can be non-zero, else a copy
+                if (ms[0] >> k) & (ms[1] >> k) & 1:
+                    cost[k] = 1
+        elif op in _LINEAR or op == int(O.Neg) or (op in (int(O.Mul), int(O.Div)) and not all(var)):
+            m = 0
+            for x, v in zip(ms, var):
+                if v:
+                    m |= x
+            nz[i] = m | 1
+            for k in range(K):
+                if (m >> k) & 1:
+                    cost[k] = 1
+        elif op == int(O.Mul):
+            m = 0
+            for k in range(K):
+                pairs = sum(1 for j in range(k + 1) if (ms[0] >> j) & 1 and (ms[1] >> (k - j)) & 1)
+                if pairs:
+                    m |= 1 << k
+                    cost[k] = 2 * pairs - 1
+            nz[i] = m
+        else:
+            m = 1
+            anyv = 0
+            for x in ms:
+                anyv |= x
+            for k in range(1, K):
+                if anyv & (((1 << (k + 1)) - 1) & ~1):
+                    m |= 1 << k
+            nz[i] = m
+            for k in range(K):
+                if (m >> k) & 1:
+                    cost[k] = 2 * K
+        point += cost[0]
+        lane += sum(cost[1:])
+    return point, lane
 
 
 def compulsory_bytes(tape, what: str, scalar_bytes: int) -> int:

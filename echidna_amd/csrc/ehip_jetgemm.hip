@@ -6,12 +6,15 @@
 // A DENSE's members then are, per coefficient q,
 //     C_q[j][lane] = sum_p W[p][j] X_q[p][lane]
 // (members x terms times terms x lanes), with X_q[p] the row of term p's
-// operand register.  A workgroup owns 64 lanes and 128 members: it stages
-// 16 terms x 64 lanes of every coefficient in LDS, and each of its four
+// operand register.  A workgroup owns NT lanes and 128 members: it stages
+// 16 terms x NT lanes of every coefficient in LDS, and each of its four
 // waves multiplies its 32 members' weights into them with
 // v_mfma_f32_32x32x2_f32 (two terms a step, FP32 accumulation; lane l
 // supplies A[j = l % 32][k = l / 32] and B[k = l / 32][n = l % 32], and holds
-// the results of rows 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).  The
+// the results of rows 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32), one
+// product per 32-lane column block.  NT is 64, or 128 when the matrix cores
+// form one coefficient only (c1 of tape inputs, c0 apart): every weight read
+// from LDS then feeds four products, as with two coefficients at 64 lanes.  The
 // next stage's operands (and the term descriptors of the one after) are
 // loaded into registers while the matrix cores work on the current one.  Immediate terms add their weight to c0;
 // operands that are tape inputs (D_INPUT_OPERANDS) carry +0 from order 2 on,
@@ -56,7 +59,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t NC = 3;    // coefficients per register (order 3)
 constexpr uint32_t MT = JET_GEMM_MT;  // members per workgroup: four waves of 32
-constexpr uint32_t NT = 64;   // lanes per workgroup
 constexpr uint32_t KC = 16;   // terms per stage
 constexpr uint32_t RPT = KC / 4;  // stage rows per wave
 
@@ -112,11 +114,15 @@ __device__ __forceinline__ void epilogue_op(uint32_t op, float* c) {
 
 // NZ: operand coefficients that can be non-zero (2 when the operands are
 // tape inputs).  PT: c0 groups per tile (0: c0 on the matrix cores too).
-// DM: operand source (DenseGemm::direct; NZ == 2 only).
+// DM: operand source (DenseGemm::direct; NZ == 2 only).  NT: lanes per
+// workgroup, 64 or 128 (one coefficient on the matrix cores).
 // Three waves per SIMD: accumulators in VGPRs, no spills (four spill).
-template <uint32_t NZ, uint32_t PT, uint32_t DM>
+template <uint32_t NZ, uint32_t PT, uint32_t DM, uint32_t NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_dense_gemm(DenseGemm G) {
   static_assert(DM == 0 || NZ == 2, "direct operands are tape inputs");
+  static_assert(NT == 64 || (NT == 128 && PT != 0 && NZ == 2), "128 lanes: one coefficient on the matrix cores");
+  constexpr uint32_t NB = NT / 64;   // lanes each thread stages per operand row
+  constexpr uint32_t NTC = NT / 32;  // 32-lane column blocks: products per wave and step
   constexpr uint32_t Q0 = PT ? 1u : 0u;  // first coefficient on the matrix cores
   constexpr uint32_t NQ = NZ - Q0;
   constexpr uint32_t PTS = PT ? PT : 1u;
@@ -128,7 +134,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ float P0r[PT ? MT : 1][PTS], P1r[PT ? MT : 1][PTS];  // their activation's primal part
   __shared__ float RO[4][NT][NC];  // read-out: each wave's partial sums per lane
   __shared__ float ROB[MT][NC];    // read-out of a recomputed lane: its members in order
-  __shared__ uint32_t bad[2];      // tile lanes (0..31, 32..63) with a non-finite operand coefficient
+  __shared__ uint32_t bad[NTC];    // tile lanes (32 t .. 32 t + 31) with a non-finite operand coefficient
   const uint32_t tid = threadIdx.x, l = tid & 63u, hi = l >> 5, n = l & 31u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   // Workgroup -> (lane tile, member row).  Workgroups go to the eight XCDs
@@ -149,19 +155,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const uint32_t jw = by * MT + w * 32u;  // this wave's first member
   const uint32_t j = jw + n;                      // the member whose weights this lane supplies
   const bool live = jw < m;
-  if (tid < 2) bad[tid] = 0u;
-  f32x16 acc[NQ][2];
+  if (tid < NTC) bad[tid] = 0u;
+  f32x16 acc[NQ][NTC];
 #pragma unroll
-  for (uint32_t q = 0; q < NQ; ++q) acc[q][0] = acc[q][1] = f32x16{};
+  for (uint32_t q = 0; q < NQ; ++q)
+#pragma unroll
+    for (uint32_t t = 0; t < NTC; ++t) acc[q][t] = f32x16{};
   float c0acc[PTS];
 #pragma unroll
   for (uint32_t t = 0; t < PTS; ++t) c0acc[t] = 0.f;
-  // Direct operands: this lane's x row and draw base, and the x row of the
-  // c0 group it loads (lane l < PT), fixed for the whole product.
-  const uint64_t dlane = lane0 + l;
-  const bool dact = DM && dlane < G.B;
-  const uint64_t dx = dact ? (G.S ? dlane / G.S : dlane) * G.xstride : 0;
-  const uint64_t dbase = DM == 1 ? dlane * G.nin : 0;
+  // Direct operands: the x rows and draw bases of the lanes this thread
+  // stages (l, l + 64), and the x row of the c0 group it loads (lane l <
+  // PT), fixed for the whole product.
+  bool dact[NB];
+  uint64_t dx[NB], dbase[NB];
+#pragma unroll
+  for (uint32_t b = 0; b < NB; ++b) {
+    const uint64_t dlane = lane0 + b * 64u + l;
+    dact[b] = DM && dlane < G.B;
+    dx[b] = dact[b] ? (G.S ? dlane / G.S : dlane) * G.xstride : 0;
+    dbase[b] = DM == 1 ? dlane * G.nin : 0;
+  }
   uint64_t dx0 = 0;
   bool dact0 = false;
   if (DM && PT && l < PTS) {
@@ -169,13 +183,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     dact0 = ln < G.B;
     dx0 = dact0 ? (G.S ? ln / G.S : ln) * G.xstride : 0;
   }
-  DirBlock db;  // direct 1: the block of this lane's last sign
+  DirBlock db[NB];  // direct 1: the blocks of these lanes' last signs
 
   // Stage registers.  Wave w stages terms w RPT .. w RPT + RPT - 1, lane l
   // of each, and its own members' weights for the stage's KC terms.
   // The descriptors travel one stage ahead of the rows: lane r < RPT of the
   // wave holds term w RPT + r's word of the stage after the one loading.
-  float sb[RPT][NQ], sc[RPT], sa[KC / 2];
+  float sb[RPT][NQ][NB], sc[RPT], sa[KC / 2];
   uint32_t dsc, din = 0u;
   auto load_desc = [&](uint32_t k0) {
     const uint32_t p = k0 + w * RPT + l;
@@ -183,42 +197,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (DM) din = (l < RPT && p < T) ? PI[p] : 0u;
   };
   auto load_stage = [&](uint32_t k0) {
-    // direct 1: the stage's RPT signs of this lane first (bit r set: term r
-    // draws -1), in a loop that holds one copy of the generator
-    uint32_t sg = 0u;
-    if (DM == 1 && dact) {
+    // direct 1: the stage's RPT signs of each staged lane first (bit r set:
+    // term r draws -1), in a loop that holds one copy of the generator per
+    // staged lane
+    uint32_t sg[NB];
+#pragma unroll
+    for (uint32_t b = 0; b < NB; ++b) sg[b] = 0u;
+    if (DM == 1) {
+#pragma unroll
+      for (uint32_t b = 0; b < NB; ++b) {
+        if (!dact[b]) continue;
 #pragma unroll 1
-      for (uint32_t r = 0; r < RPT; ++r) {
-        const uint32_t kr = __builtin_amdgcn_readlane(dsc, r);
-        if ((kr >> P_KIND_SHIFT) == T_IMM) continue;
-        const uint32_t in = __builtin_amdgcn_readlane(din, r);
-        const bool neg = philox_direction(RNG_RADEMACHER, G.seed, dbase + in, db) < 0.0;
-        sg |= (neg ? 1u : 0u) << r;
-        if (G.dirs_out && by == 0) G.dirs_out[(uint64_t)in * G.B + dlane] = neg ? -1.f : 1.f;
+        for (uint32_t r = 0; r < RPT; ++r) {
+          const uint32_t kr = __builtin_amdgcn_readlane(dsc, r);
+          if ((kr >> P_KIND_SHIFT) == T_IMM) continue;
+          const uint32_t in = __builtin_amdgcn_readlane(din, r);
+          const bool neg = philox_direction(RNG_RADEMACHER, G.seed, dbase[b] + in, db[b]) < 0.0;
+          sg[b] |= (neg ? 1u : 0u) << r;
+        }
       }
     }
 #pragma unroll
     for (uint32_t r = 0; r < RPT; ++r) {
 #pragma unroll
-      for (uint32_t q = 0; q < NQ; ++q) sb[r][q] = 0.f;
+      for (uint32_t q = 0; q < NQ; ++q)
+#pragma unroll
+        for (uint32_t b = 0; b < NB; ++b) sb[r][q][b] = 0.f;
       sc[r] = 0.f;
       const uint32_t kr = __builtin_amdgcn_readlane(dsc, r);
       if ((kr >> P_KIND_SHIFT) != T_IMM) {
         if (DM) {
           const uint32_t in = __builtin_amdgcn_readlane(din, r);
-          float d = 0.f;
-          if (dact) d = DM == 1 ? (((sg >> r) & 1u) ? -1.f : 1.f) : G.dcm[(uint64_t)in * G.B + dlane];
-          if (Q0 == 0) {
-            sb[r][0] = dact ? G.x[dx + in] : 0.f;
-            sb[r][NQ - 1] = d;
-          } else {
-            sb[r][0] = d;
+#pragma unroll
+          for (uint32_t b = 0; b < NB; ++b) {
+            float d = 0.f;
+            if (dact[b]) {
+              const uint64_t at = (uint64_t)in * G.B + lane0 + b * 64u + l;
+              if (DM == 1) {
+                d = ((sg[b] >> r) & 1u) ? -1.f : 1.f;
+                if (G.dirs_out && by == 0) G.dirs_out[at] = d;
+              } else {
+                d = G.dcm[at];
+              }
+            }
+            if (Q0 == 0) {
+              sb[r][0][b] = dact[b] ? G.x[dx[b] + in] : 0.f;
+              sb[r][NQ - 1][b] = d;
+            } else {
+              sb[r][0][b] = d;
+            }
           }
           if (PT && l < PTS) sc[r] = dact0 ? G.x[dx0 + in] : 0.f;
         } else {
           const uint32_t reg = kr & P_REG_MASK;
 #pragma unroll
-          for (uint32_t q = 0; q < NQ; ++q) sb[r][q] = row_of(G, reg, Q0 + q)[lane0 + l];
+          for (uint32_t q = 0; q < NQ; ++q)
+#pragma unroll
+            for (uint32_t b = 0; b < NB; ++b) sb[r][q][b] = row_of(G, reg, Q0 + q)[lane0 + b * 64u + l];
           if (PT && l < PTS) sc[r] = row_of(G, reg, 0)[lane0 + l * LPP];
         }
       }
@@ -233,13 +268,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
     for (uint32_t r = 0; r < RPT; ++r) {
       const uint32_t i = w * RPT + r;
-      bool fin = true;
 #pragma unroll
-      for (uint32_t q = 0; q < NQ; ++q) {
-        Bs[q][i][l] = sb[r][q];
-        fin = fin && isfinite(sb[r][q]);
+      for (uint32_t b = 0; b < NB; ++b) {
+        bool fin = true;
+#pragma unroll
+        for (uint32_t q = 0; q < NQ; ++q) {
+          Bs[q][i][b * 64u + l] = sb[r][q][b];
+          fin = fin && isfinite(sb[r][q][b]);
+        }
+        if (!fin) atomicOr(&bad[2u * b + hi], 1u << n);
       }
-      if (!fin) atomicOr(&bad[hi], 1u << n);
       if (PT && l < PTS) {
         C0s[i][l] = sc[r];
         if (!isfinite(sc[r]))  // the whole group
@@ -267,10 +305,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const uint32_t k = 2u * kk + hi;
         const float a = As[w][k][n];
 #pragma unroll
-        for (uint32_t q = 0; q < NQ; ++q) {
-          acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, Bs[q][k][n], acc[q][0], 0, 0, 0);
-          acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, Bs[q][k][32 + n], acc[q][1], 0, 0, 0);
-        }
+        for (uint32_t q = 0; q < NQ; ++q)
+#pragma unroll
+          for (uint32_t t = 0; t < NTC; ++t)
+            acc[q][t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, Bs[q][k][t * 32u + n], acc[q][t], 0, 0, 0);
         if (PT) {
 #pragma unroll
           for (uint32_t t = 0; t < PTS; ++t) c0acc[t] = fmaf(a, C0s[k][t], c0acc[t]);
@@ -299,10 +337,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
   }
   __syncthreads();  // bad[] and C0r complete
-  const uint32_t bad0 = bad[0], bad1 = bad[1];
-  float ro[2][NC];  // read-out partial sums of this lane's two columns
+  uint32_t badm[NTC], anybad = 0u;
 #pragma unroll
-  for (uint32_t t = 0; t < 2; ++t)
+  for (uint32_t t = 0; t < NTC; ++t) anybad |= (badm[t] = bad[t]);
+  float ro[NTC][NC];  // read-out partial sums of this lane's columns
+#pragma unroll
+  for (uint32_t t = 0; t < NTC; ++t)
 #pragma unroll
     for (uint32_t q = 0; q < NC; ++q) ro[t][q] = 0.f;
   if (live) {
@@ -310,24 +350,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int v = 0; v < 16; ++v) {
       const uint32_t rl = 8u * (v >> 2) + 4u * hi + (v & 3), row = jw + rl;
       if (row >= m) continue;
-      float c0[2];
+      float c0[NTC];
       if (PT) {
-        c0[0] = C0r[w * 32u + rl][n / LPP];
-        c0[1] = C0r[w * 32u + rl][(32u + n) / LPP];
+#pragma unroll
+        for (uint32_t t = 0; t < NTC; ++t) c0[t] = C0r[w * 32u + rl][(t * 32u + n) / LPP];
       } else {
-        c0[0] = acc[0][0][v];
-        c0[1] = acc[0][1][v];
+#pragma unroll
+        for (uint32_t t = 0; t < NTC; ++t) c0[t] = acc[0][t][v];
         for (uint32_t i = 0; i < G.nimm; ++i) {
           const float wi = PW[(uint64_t)G.imm[i] * m + row];
-          c0[0] = c0[0] + wi;
-          c0[1] = c0[1] + wi;
+#pragma unroll
+          for (uint32_t t = 0; t < NTC; ++t) c0[t] = c0[t] + wi;
         }
       }
       const uint32_t dst = PR[T + row];
       const uint32_t op = PE ? PE[row] : 0u;
 #pragma unroll
-      for (uint32_t t = 0; t < 2; ++t) {
-        if (((t ? bad1 : bad0) >> n) & 1u) continue;  // recomputed below
+      for (uint32_t t = 0; t < NTC; ++t) {
+        if ((badm[t] >> n) & 1u) continue;  // recomputed below
         const uint64_t ln = lane0 + t * 32u + n;
         float c[NC];
         c[0] = c0[t];
@@ -356,30 +396,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // lanes n and n + 32 hold the other 16 rows of the same columns; then
     // the four waves' 32-member sums meet in LDS, in wave order
 #pragma unroll
-    for (uint32_t t = 0; t < 2; ++t)
+    for (uint32_t t = 0; t < NTC; ++t)
 #pragma unroll
       for (uint32_t q = 0; q < NC; ++q) ro[t][q] = ro[t][q] + __shfl_xor(ro[t][q], 32, 64);
     if (hi == 0)
 #pragma unroll
-      for (uint32_t t = 0; t < 2; ++t)
+      for (uint32_t t = 0; t < NTC; ++t)
 #pragma unroll
         for (uint32_t q = 0; q < NC; ++q) RO[w][t * 32u + n][q] = ro[t][q];
     __syncthreads();
-    if (tid < NT * NC) {
-      const uint32_t col = tid / NC, q = tid % NC;
-      if (!(((col < 32 ? bad0 : bad1) >> (col & 31u)) & 1u)) {  // recomputed lanes: below
+    for (uint32_t e = tid; e < NT * NC; e += blockDim.x) {
+      const uint32_t col = e / NC, q = e % NC;
+      if (!((bad[col >> 5] >> (col & 31u)) & 1u)) {  // recomputed lanes: below
         const float v = ((RO[0][col][q] + RO[1][col][q]) + RO[2][col][q]) + RO[3][col][q];
         row_of(G, PR[T + by * MT], q)[lane0 + col] = v;
       }
     }
   }
-  if ((bad0 | bad1) == 0u) return;
+  if (anybad == 0u) return;
   // Lanes with a non-finite operand: every member of this workgroup term by
   // term with full Taylor products, in program order, as the interpreter's
   // exact path computes them.
   const uint32_t jend = min(m, (by + 1) * MT);
   for (uint32_t ln = 0; ln < NT; ++ln) {
-    if (!(((ln < 32 ? bad0 : bad1) >> (ln & 31u)) & 1u)) continue;
+    if (!((bad[ln >> 5] >> (ln & 31u)) & 1u)) continue;
     if (PRO) __syncthreads();  // the previous lane's ROB reads are done
     for (uint32_t jj = by * MT + tid; jj < jend; jj += blockDim.x) {
       float s[NC] = {0.f, 0.f, 0.f};
@@ -444,14 +484,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 template <uint32_t NZ, uint32_t DM>
-const void* pick(uint32_t pt) {
+const void* pick(uint32_t pt, uint32_t nt) {
+  if (NZ == 2 && nt == 128) {
+    switch (pt) {
+      case 1: return (const void*)k_dense_gemm<2, 1, DM, 128>;
+      case 2: return (const void*)k_dense_gemm<2, 2, DM, 128>;
+      case 4: return (const void*)k_dense_gemm<2, 4, DM, 128>;
+      case 8: return (const void*)k_dense_gemm<2, 8, DM, 128>;
+      case 16: return (const void*)k_dense_gemm<2, 16, DM, 128>;
+      default: return nullptr;
+    }
+  }
   switch (pt) {
-    case 1: return (const void*)k_dense_gemm<NZ, 1, DM>;
-    case 2: return (const void*)k_dense_gemm<NZ, 2, DM>;
-    case 4: return (const void*)k_dense_gemm<NZ, 4, DM>;
-    case 8: return (const void*)k_dense_gemm<NZ, 8, DM>;
-    case 16: return (const void*)k_dense_gemm<NZ, 16, DM>;
-    default: return (const void*)k_dense_gemm<NZ, 0, DM>;
+    case 1: return (const void*)k_dense_gemm<NZ, 1, DM, 64>;
+    case 2: return (const void*)k_dense_gemm<NZ, 2, DM, 64>;
+    case 4: return (const void*)k_dense_gemm<NZ, 4, DM, 64>;
+    case 8: return (const void*)k_dense_gemm<NZ, 8, DM, 64>;
+    case 16: return (const void*)k_dense_gemm<NZ, 16, DM, 64>;
+    default: return (const void*)k_dense_gemm<NZ, 0, DM, 64>;
   }
 }
 
@@ -479,22 +529,31 @@ hipError_t launch_draw_dirs(uint32_t kind, uint64_t seed, uint32_t S, uint64_t B
 
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s) {
   if (G.m == 0 || G.lanes == 0) return hipSuccess;
-  if (G.lanes % NT != 0) return hipErrorInvalidValue;
-  // c0 groups per 64-lane tile: groups that divide the tile, or tiles inside one group
-  uint32_t pt = 0;
-  if (G.share >= NT && G.share % NT == 0) pt = 1;
-  else if (G.share >= 4 && NT % G.share == 0) pt = NT / G.share;
-  const uint64_t tiles = G.lanes / NT, nrow = (G.m + MT - 1) / MT;
+  if (G.lanes % 64 != 0) return hipErrorInvalidValue;
+  // c0 groups per lane tile: groups that divide the tile, or tiles inside one group
+  auto groups = [&](uint32_t nt) -> uint32_t {
+    if (G.share >= nt && G.share % nt == 0) return 1;
+    if (G.share >= 4 && nt % G.share == 0) return nt / G.share;
+    return 0;
+  };
+  // 128-lane tiles when the matrix cores form c1 alone (inputs' c0 shared, c2 zero)
+  const bool inputs = (G.flags & D_INPUT_OPERANDS) != 0;
+  uint32_t nt = 64, pt = groups(64);
+  if (inputs && G.lanes % 128 == 0) {
+    const uint32_t p = groups(128);
+    if (p >= 1 && p <= 16) nt = 128, pt = p;
+  }
+  const uint64_t tiles = G.lanes / nt, nrow = (G.m + MT - 1) / MT;
   const uint64_t wgs = (tiles + 7) / 8 * 8 * nrow;  // groups of eight lane tiles
   if (wgs > 0xFFFFFFFFull) return hipErrorInvalidValue;
   const dim3 grid((uint32_t)wgs);
   void* args[] = {const_cast<DenseGemm*>(&G)};
   if (G.direct > 2 || (G.direct && (!(G.flags & D_INPUT_OPERANDS) || G.off_in == ~0u || !G.x))) return hipErrorInvalidValue;
   if (G.direct == 2 && !G.dcm) return hipErrorInvalidValue;
-  const void* fn = !(G.flags & D_INPUT_OPERANDS) ? pick<3, 0>(pt)
-                   : G.direct == 1              ? pick<2, 1>(pt)
-                   : G.direct == 2              ? pick<2, 2>(pt)
-                                                : pick<2, 0>(pt);
+  const void* fn = !inputs            ? pick<3, 0>(pt, nt)
+                   : G.direct == 1   ? pick<2, 1>(pt, nt)
+                   : G.direct == 2   ? pick<2, 2>(pt, nt)
+                                     : pick<2, 0>(pt, nt);
   return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
 }
 

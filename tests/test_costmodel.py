@@ -15,6 +15,22 @@ def test_hand_counted_tape():
     assert cm.fwdgrad_flops(tape) == 8
     # jets, K = 3: a product of two jets 9, sin 18, add 3
     assert cm.jet_flops(tape, 3) == 30
+    # seeded (x, v, 0): x0 x1 has pairs (0,0) | (0,1),(1,0) | (1,1): 1 | 3 | 1;
+    # sin 6 per coefficient; the add 1 per coefficient
+    assert cm.jet_flops_split(tape, 3) == (1 + 6 + 1, 4 + 12 + 2)
+
+
+def test_jet_split_on_the_stde_mlp():
+    """n=256 H=256 tanh MLP: the layer's 256 x 256 products and 255 x 256
+    sums have c0 (per point) and c1 (per direction) only, the bias adds c0
+    only (c1 is a copy); tanh costs 6 per coefficient, c0..c2; the
+    read-out's 256 products and 255 sums take all three coefficients, and
+    its first add (to the constant 0) c0 alone."""
+    tape = wl.tanh_mlp(256, 256, 2)
+    layer = 256 * 256 + 255 * 256
+    point = layer + 256 + 6 * 256 + (256 + 256)
+    lane = layer + 12 * 256 + 2 * (256 + 255)
+    assert cm.jet_flops_split(tape, 3) == (point, lane)
 
 
 def test_mlp_counts():
