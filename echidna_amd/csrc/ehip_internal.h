@@ -456,6 +456,24 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
       js.direct_dropped = nl - dkept;
     }
   }
+  // Packed weights of every GEMM (16-byte aligned), and the sequential
+  // operand count of the direct ones (DenseGemm::nseq).
+  std::vector<uint32_t> wp_off(cut.size(), 0u), nseq(cut.size(), 0u);
+  for (size_t ci = 0; ci < cut.size(); ++ci) {
+    const JetSegment& c = cut[ci];
+    if (!c.gemm || immediates(c, nullptr) > 4) continue;
+    pool.resize((pool.size() + 3) / 4 * 4, 0u);
+    wp_off[ci] = (uint32_t)pool.size();
+    pool.resize(pool.size() + jet_gemm_packed_words(c.T, c.m), 0u);
+    pack_jet_gemm_weights(pool.data() + c.off_w, c.T, c.m, pool.data() + wp_off[ci]);
+    if (in_off[ci] != ~0u && h.num_inputs % 4 == 0) {
+      uint32_t q = 0;
+      while (q < c.T && (pool[c.off_regs + q] >> P_KIND_SHIFT) != T_IMM && pool[in_off[ci] + q] == q) ++q;
+      bool rest_imm = true;
+      for (uint32_t p = q; p < c.T; ++p) rest_imm = rest_imm && (pool[c.off_regs + p] >> P_KIND_SHIFT) == T_IMM;
+      if (rest_imm) nseq[ci] = q;
+    }
+  }
   HIP_TRY(hipMalloc((void**)&js.d_pool, std::max<size_t>(pool.size(), 4) * 4), "program upload");
   HIP_TRY(hipMemcpy(js.d_pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice), "program upload");
   uint32_t i0 = 0;  // instructions not yet in a segment (a DENSE the cores cannot take stays interpreted)
@@ -495,6 +513,8 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
     g.off_epi = epi_off[ci];
     g.off_ro = ro_off[ci];
     g.off_in = in_off[ci];
+    g.off_wp = wp_off[ci];
+    g.nseq = nseq[ci];
     int32_t rc = flush(c.i0);
     if (rc) return rc;
     JetSegDev d;

@@ -59,7 +59,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t NC = 3;    // coefficients per register (order 3)
 constexpr uint32_t MT = JET_GEMM_MT;  // members per workgroup: four waves of 32
-constexpr uint32_t KC = 16;   // terms per stage
+constexpr uint32_t KC = JET_GEMM_KC;  // terms per stage
 constexpr uint32_t RPT = KC / 4;  // stage rows per wave
 
 __device__ __forceinline__ float* row_of(const DenseGemm& G, uint32_t reg, uint32_t q) {
@@ -152,6 +152,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const uint32_t* PE = G.off_epi != ~0u ? G.pool + G.off_epi : nullptr;  // members' activations
   const float* PRO = G.off_ro != ~0u ? reinterpret_cast<const float*>(G.pool + G.off_ro) : nullptr;
   const uint32_t* PI = G.pool + (DM ? G.off_in : 0u);  // direct: the terms' input indices
+  const uint32_t nst = (T + KC - 1) / KC;  // stages
   const uint32_t jw = by * MT + w * 32u;  // this wave's first member
   const uint32_t j = jw + n;                      // the member whose weights this lane supplies
   const bool live = jw < m;
@@ -203,7 +204,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     uint32_t sg[NB];
 #pragma unroll
     for (uint32_t b = 0; b < NB; ++b) sg[b] = 0u;
-    if (DM == 1) {
+    if (DM == 1 && G.nseq) {
+      // terms k0 + w RPT + r are inputs of the same index (below nseq): the
+      // lane's four signs are bits idx0 .. idx0 + 3 of its stream, idx0 a
+      // multiple of 4, so within one 32-bit word of one block
+      const uint32_t p0 = k0 + w * RPT;
+      const uint32_t nv = G.nseq > p0 ? min(G.nseq - p0, RPT) : 0u;
+#pragma unroll
+      for (uint32_t b = 0; b < NB; ++b) {
+        if (!dact[b] || nv == 0u) continue;
+        const uint64_t idx0 = dbase[b] + p0, blk = idx0 >> 7;
+        if (blk != db[b].blk) direction_fill(RNG_RADEMACHER, G.seed, blk, db[b]);
+        const uint32_t bi = (uint32_t)idx0 & 127u;
+        const uint32_t lo = (bi & 32u) ? db[b].w1 : db[b].w0, hi2 = (bi & 32u) ? db[b].w3 : db[b].w2;
+        sg[b] = ((((bi & 64u) ? hi2 : lo) >> (bi & 31u)) & 0xFu) & ((1u << nv) - 1u);
+      }
+    } else if (DM == 1) {
 #pragma unroll
       for (uint32_t b = 0; b < NB; ++b) {
         if (!dact[b]) continue;
@@ -258,10 +274,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
       }
     }
+    // this lane's 8 packed weights of the stage: two 16-byte loads
+    if (live) {
+      const float4* wp = reinterpret_cast<const float4*>(G.pool + G.off_wp) +
+                         (((size_t)(by * (MT / 32u) + w) * nst + k0 / KC) * 64u + l) * 2u;
+      const float4 w0 = wp[0], w1 = wp[1];
+      sa[0] = w0.x, sa[1] = w0.y, sa[2] = w0.z, sa[3] = w0.w;
+      sa[4] = w1.x, sa[5] = w1.y, sa[6] = w1.z, sa[7] = w1.w;
+    } else {
 #pragma unroll
-    for (uint32_t kk = 0; kk < KC / 2; ++kk) {
-      const uint32_t p = k0 + 2u * kk + hi;
-      sa[kk] = (live && p < T && j < m) ? PW[(uint64_t)p * m + j] : 0.f;
+      for (uint32_t kk = 0; kk < KC / 2; ++kk) sa[kk] = 0.f;
     }
   };
   auto store_stage = [&]() {

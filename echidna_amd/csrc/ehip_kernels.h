@@ -64,9 +64,35 @@ struct DenseGemm {
   uint64_t seed;            // direct 1: the directions' Philox seed (philox.h)
   const float* dcm;         // direct 2: directions, coordinate-major [nin][B]
   float* dirs_out;          // direct 1: the drawn signs written coordinate-major [nin][B], or null
+  // Weights packed for the matrix cores (pack_jet_gemm_weights): per block
+  // of 32 members, per stage of JET_GEMM_KC terms, per lane l of a wave the
+  // 8 weights it supplies (member 32 b + l % 32, terms 2 kk + l / 32), zero
+  // past T and m.  Pool offset, a multiple of 4 words.
+  uint32_t off_wp;
+  // Direct 1 with terms [0, nseq) the inputs 0 .. nseq - 1 in order and
+  // the rest immediates, nin a multiple of 4: a lane's four signs of a stage
+  // are one 4-bit field of its Philox block.  0: the general path.
+  uint32_t nseq;
 };
 
 constexpr uint32_t JET_GEMM_MT = 128;  // members per workgroup of the jet batch GEMM
+constexpr uint32_t JET_GEMM_KC = 16;   // terms per stage of the jet batch GEMM
+
+// The packed weights (DenseGemm::off_wp) of a DENSE with T terms and m
+// members whose weights are W[p * m + j] (f32 bits).
+inline void pack_jet_gemm_weights(const uint32_t* W, uint32_t T, uint32_t m, uint32_t* out) {
+  const uint32_t nst = (T + JET_GEMM_KC - 1) / JET_GEMM_KC, nmb = (m + 31) / 32;
+  for (uint32_t jb = 0; jb < nmb; ++jb)
+    for (uint32_t s = 0; s < nst; ++s)
+      for (uint32_t l = 0; l < 64; ++l)
+        for (uint32_t kk = 0; kk < JET_GEMM_KC / 2; ++kk) {
+          const uint32_t p = s * JET_GEMM_KC + 2 * kk + l / 32, j = jb * 32 + l % 32;
+          out[(((size_t)jb * nst + s) * 64 + l) * (JET_GEMM_KC / 2) + kk] = (p < T && j < m) ? W[(size_t)p * m + j] : 0u;
+        }
+}
+inline size_t jet_gemm_packed_words(uint32_t T, uint32_t m) {
+  return (size_t)((m + 31) / 32) * ((T + JET_GEMM_KC - 1) / JET_GEMM_KC) * 64 * (JET_GEMM_KC / 2);
+}
 
 hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s);
 
