@@ -188,6 +188,7 @@ struct JetSegProgram {
   uint32_t* d_pool = nullptr;
   uint32_t direct_gemms = 0;     // GEMMs with a direct form (0: no direct plan)
   uint32_t direct_dropped = 0;   // inputs the direct plan's LOADN leaves out
+  bool direct_seq = false;       // every direct GEMM reads the inputs in order (DenseGemm::nseq)
 };
 
 struct ehip_tape {
@@ -234,6 +235,7 @@ inline void free_jet_segments(JetSegProgram& js) {
   js.segs.clear();
   js.d_pool = nullptr;
   js.direct_gemms = js.direct_dropped = 0;
+  js.direct_seq = false;
 }
 
 // The registers instruction i of a forward program reads (rd) and then
@@ -400,6 +402,7 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
   for (uint32_t i = 0; i < js.host.ninstr && li == ~0u; ++i)
     if ((js.host.words[(size_t)i * NW] & 0xFFu) == I_LOADN) li = i;
   js.direct_gemms = js.direct_dropped = 0;
+  js.direct_seq = false;
   if (li != ~0u) {
     const uint32_t Tn = js.host.nterms, nl = js.host.words[(size_t)li * NW + 2], lo = js.host.words[(size_t)li * NW + 3];
     std::unordered_map<uint32_t, uint32_t> in_of;  // register -> input index
@@ -474,6 +477,9 @@ inline int32_t upload_jet_segments(JetSegProgram& js, const TapeHost& h, bool fu
       if (rest_imm) nseq[ci] = q;
     }
   }
+  js.direct_seq = js.direct_gemms > 0;
+  for (size_t ci = 0; ci < cut.size(); ++ci)
+    if (in_off[ci] != ~0u && nseq[ci] == 0) js.direct_seq = false;
   HIP_TRY(hipMalloc((void**)&js.d_pool, std::max<size_t>(pool.size(), 4) * 4), "program upload");
   HIP_TRY(hipMemcpy(js.d_pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice), "program upload");
   uint32_t i0 = 0;  // instructions not yet in a segment (a DENSE the cores cannot take stays interpreted)
@@ -603,12 +609,13 @@ inline int32_t run(ehip_ctx* c, const ehip_tape* t, int kind, int K, Launch L, h
     // every register in HBM: the launches hand their registers on
     const uint64_t grid = (L.B + wg - 1) / wg, lanes = grid * wg;
     const size_t bytes = (size_t)js.host.nregs * NC * lanes * S;
-    // Direct plan for device-drawn directions: Rademacher signs are drawn in
-    // the GEMMs; Gaussians once, coordinate-major, into the caller's
-    // direction buffer or after the register file.
+    // Direct plan for device-drawn directions: Rademacher signs of inputs
+    // in order are drawn in the GEMMs; other directions once,
+    // coordinate-major, into the caller's direction buffer or after the
+    // register file.
     uint32_t direct = 0;
-    if (js.direct_gemms && L.rng == RNG_RADEMACHER) direct = 1;
-    else if (js.direct_dropped && L.rng == RNG_GAUSSIAN) direct = 2;
+    if (js.direct_gemms && L.rng == RNG_RADEMACHER && js.direct_seq) direct = 1;
+    else if (js.direct_dropped && (L.rng == RNG_RADEMACHER || L.rng == RNG_GAUSSIAN)) direct = 2;
     size_t dbytes = (direct == 2 && !L.dirs_out) ? ((size_t)L.nin * L.B * 4 + 255) / 256 * 256 : 0;
     if (fixed) {
       if (bytes + dbytes > fixed_cap && dbytes) direct = 0, dbytes = 0;  // no room: the plain plan

@@ -34,11 +34,12 @@
 // Direct operands (DenseGemm::direct): when every operand is a tape input
 // and the directions are drawn on the device, the operand jets (x_i, v_i, 0)
 // come from their sources instead of the input registers: x from its row,
-// v as the Rademacher sign drawn here (philox.h, the block cache holding a
-// lane's 128 consecutive signs) or from directions drawn coordinate-major
-// beforehand (k_draw_dirs).  The values are the ones the interpreter's
-// LOADN writes, so results do not change; the program then skips writing
-// inputs only GEMMs read.
+// v as the Rademacher sign drawn here when the operands are the inputs in
+// order (DenseGemm::nseq: a lane's four signs of a stage are one 4-bit
+// field of its Philox block, which a cache keeps for 128 signs) or from
+// directions drawn coordinate-major beforehand (k_draw_dirs).  The values
+// are the ones the interpreter's LOADN writes, so results do not change;
+// the program then skips writing inputs only GEMMs read.
 //
 // These are the products of the interpreter's exact path (0 + x c per term
 // when x is finite) summed in another order and fused, so results agree
@@ -199,12 +200,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   };
   auto load_stage = [&](uint32_t k0) {
     // direct 1: the stage's RPT signs of each staged lane first (bit r set:
-    // term r draws -1), in a loop that holds one copy of the generator per
-    // staged lane
+    // term r draws -1)
     uint32_t sg[NB];
 #pragma unroll
     for (uint32_t b = 0; b < NB; ++b) sg[b] = 0u;
-    if (DM == 1 && G.nseq) {
+    if (DM == 1) {
       // terms k0 + w RPT + r are inputs of the same index (below nseq): the
       // lane's four signs are bits idx0 .. idx0 + 3 of its stream, idx0 a
       // multiple of 4, so within one 32-bit word of one block
@@ -218,19 +218,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const uint32_t bi = (uint32_t)idx0 & 127u;
         const uint32_t lo = (bi & 32u) ? db[b].w1 : db[b].w0, hi2 = (bi & 32u) ? db[b].w3 : db[b].w2;
         sg[b] = ((((bi & 64u) ? hi2 : lo) >> (bi & 31u)) & 0xFu) & ((1u << nv) - 1u);
-      }
-    } else if (DM == 1) {
-#pragma unroll
-      for (uint32_t b = 0; b < NB; ++b) {
-        if (!dact[b]) continue;
-#pragma unroll 1
-        for (uint32_t r = 0; r < RPT; ++r) {
-          const uint32_t kr = __builtin_amdgcn_readlane(dsc, r);
-          if ((kr >> P_KIND_SHIFT) == T_IMM) continue;
-          const uint32_t in = __builtin_amdgcn_readlane(din, r);
-          const bool neg = philox_direction(RNG_RADEMACHER, G.seed, dbase[b] + in, db[b]) < 0.0;
-          sg[b] |= (neg ? 1u : 0u) << r;
-        }
       }
     }
 #pragma unroll
@@ -571,7 +558,7 @@ hipError_t launch_dense_gemm(const DenseGemm& G, hipStream_t s) {
   const dim3 grid((uint32_t)wgs);
   void* args[] = {const_cast<DenseGemm*>(&G)};
   if (G.direct > 2 || (G.direct && (!(G.flags & D_INPUT_OPERANDS) || G.off_in == ~0u || !G.x))) return hipErrorInvalidValue;
-  if (G.direct == 2 && !G.dcm) return hipErrorInvalidValue;
+  if ((G.direct == 2 && !G.dcm) || (G.direct == 1 && (G.nseq == 0 || G.nin % 4 != 0))) return hipErrorInvalidValue;
   const void* fn = !inputs            ? pick<3, 0>(pt, nt)
                    : G.direct == 1   ? pick<2, 1>(pt, nt)
                    : G.direct == 2   ? pick<2, 2>(pt, nt)

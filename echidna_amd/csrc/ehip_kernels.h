@@ -55,8 +55,8 @@ struct DenseGemm {
   // the GEMM reads x and the lane's direction itself instead of the input
   // registers, which the program then need not write.
   uint32_t off_in;          // pool offset of the T terms' input indices, or ~0u (no direct form)
-  uint32_t direct;          // per launch: 0 operand registers; 1 x, Rademacher signs drawn here;
-                            // 2 x, directions from dcm
+  uint32_t direct;          // per launch: 0 operand registers; 1 x, Rademacher signs drawn here
+                            // (needs nseq); 2 x, directions from dcm
   const float* x;           // primal rows: lane p reads row (S ? p / S : p), stride xstride
   uint64_t xstride;
   uint32_t S, nin;
@@ -69,9 +69,9 @@ struct DenseGemm {
   // 8 weights it supplies (member 32 b + l % 32, terms 2 kk + l / 32), zero
   // past T and m.  Pool offset, a multiple of 4 words.
   uint32_t off_wp;
-  // Direct 1 with terms [0, nseq) the inputs 0 .. nseq - 1 in order and
-  // the rest immediates, nin a multiple of 4: a lane's four signs of a stage
-  // are one 4-bit field of its Philox block.  0: the general path.
+  // Terms [0, nseq) are the inputs 0 .. nseq - 1 in order and the rest
+  // immediates, nin a multiple of 4 (0: not so): direct 1 takes a lane's
+  // four signs of a stage as one 4-bit field of its Philox block.
   uint32_t nseq;
 };
 

@@ -235,7 +235,8 @@ def test_segment_plan(hip, name, want):
 
 @pytest.mark.parametrize("kind", [philox.DIR_RADEMACHER, philox.DIR_GAUSSIAN])
 @pytest.mark.parametrize("name,want", [("mlp_n48_h40", (1, 48, 1)), ("mlp2_n40_h70_h33", (1, 40, 1)),
-                                       ("mlp_skip_n48_h40", (1, 46, 0)), ("ridge_n40_h36", (1, 40, 1))])
+                                       ("mlp_skip_n48_h40", (1, 46, 0)), ("ridge_n40_h36", (1, 40, 1)),
+                                       ("mlp_n45_h40", (1, 45, 1))])
 def test_direct_plan_equals_supplied_directions(mma, kind, name, want):
     """Directions drawn on the device take the direct plan: the GEMM over
     tape inputs reads x and the directions itself (Rademacher signs drawn
@@ -246,9 +247,11 @@ def test_direct_plan_equals_supplied_directions(mma, kind, name, want):
     register plan.  Estimates, variances and values are bit-identical: the
     GEMM sees the values the LOADN would have stored, and the host mirror
     gives the device's directions.  S = 2 puts c0 on the matrix cores, 16
-    and 64 share it per point; lane counts leave partial tiles."""
+    and 64 share it per point; lane counts leave partial tiles.  With n = 45
+    (not a multiple of 4) the Rademacher signs are drawn coordinate-major
+    first, as the Gaussians are."""
     hip = mma
-    tape = dict(TAPES)[name]()
+    tape = wl.tanh_mlp(45, 40, 65) if name == "mlp_n45_h40" else dict(TAPES)[name]()
     bufs = hip.upload_tape(GpuTapeData.from_tape(tape))
     info = bufs.program_info(4)
     assert (info["direct_gemms"], info["inputs_dropped"], info["segments_skipped"]) == want, info
