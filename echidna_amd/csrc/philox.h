@@ -7,7 +7,7 @@
 // (philox_direction, the STDE and jet directions) share blocks: Rademacher
 // component idx is bit idx mod 128 of block idx / 128 (bit b of word b / 32),
 // Gaussian components idx = 2k and 2k + 1 are the cosine and sine branches of
-// block k's Box-Muller pair.
+// block k's Box-Muller pair (dir_log, dir_cos_sin: basic operations only).
 #pragma once
 // Also compiled at run time by hipRTC (tape-specialised jet kernels), which
 // supplies the HIP runtime declarations and fixed-width integers itself.
@@ -61,6 +61,77 @@ __host__ __device__ __forceinline__ uint64_t direction_block(uint32_t kind, uint
   return kind == RNG_RADEMACHER ? idx >> 7 : idx >> 1;
 }
 
+// Box-Muller for the direction draws from IEEE binary64 basic operations
+// only (+ - * /, sqrt, exact bit manipulation), so that the host mirror
+// (echidna_amd/philox.py) gives the same bits.
+//
+// ln u for u in (0, 1]: u = 2^e f with f in [sqrt(1/2), sqrt(2)); with
+// s = (f - 1) / (f + 1), ln f = 2 atanh s = 2 (s + s^3/3 + .. + s^19/19)
+// (|s| <= 0.1716: the next term is below 2^-53 of the sum); ln u = e ln2_hi
+// + (ln f + e ln2_lo) with ln 2 split Cody-Waite style (ln2_hi has 21
+// significant bits, so e ln2_hi is exact).
+__host__ __device__ __forceinline__ double dir_log(double u) {
+  uint64_t b;
+  __builtin_memcpy(&b, &u, 8);
+  int e = (int)((b >> 52) & 0x7FFu) - 1023;
+  b = (b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;  // f in [1, 2)
+  double f;
+  __builtin_memcpy(&f, &b, 8);
+  if (f > 1.4142135623730951) {
+    f = f * 0.5;
+    e = e + 1;
+  }
+  const double s = (f - 1.0) / (f + 1.0), t = s * s;
+  double p = 1.0 / 19.0;
+  p = p * t + 1.0 / 17.0;
+  p = p * t + 1.0 / 15.0;
+  p = p * t + 1.0 / 13.0;
+  p = p * t + 1.0 / 11.0;
+  p = p * t + 1.0 / 9.0;
+  p = p * t + 1.0 / 7.0;
+  p = p * t + 1.0 / 5.0;
+  p = p * t + 1.0 / 3.0;
+  const double lnf = 2.0 * s + (2.0 * s) * (t * p);
+  const double de = (double)e;
+  return de * 0.6931467056274414 + (lnf + de * 4.7493250390316726e-07);
+}
+
+// cos and sin of 2 pi u, u in (0, 1) a multiple of 2^-54: 8u = q + r
+// exactly (octant q, r in [0, 1)); the angle within the octant's
+// quadrant is r pi/4 (even q) or (1 - r) pi/4 (odd q, 1 - r exact), whose
+// sine and cosine come from their Taylor series to x^17 and x^16 (|x| <=
+// pi/4: the next terms are below 2^-60), then the quadrant's rotation.
+__host__ __device__ __forceinline__ void dir_cos_sin(double u, double& c, double& sn) {
+  const double t8 = u * 8.0;
+  const int q = (int)t8;
+  const double r = t8 - (double)q;
+  const double x = ((q & 1) ? (1.0 - r) : r) * 0.7853981633974483, x2 = x * x;
+  double ps = -1.0 / 355687428096000.0;  // sin: -1/17!, +1/15!, ...
+  ps = ps * x2 + 1.0 / 1307674368000.0;
+  ps = ps * x2 - 1.0 / 6227020800.0;
+  ps = ps * x2 + 1.0 / 39916800.0;
+  ps = ps * x2 - 1.0 / 362880.0;
+  ps = ps * x2 + 1.0 / 5040.0;
+  ps = ps * x2 - 1.0 / 120.0;
+  ps = ps * x2 + 1.0 / 6.0;
+  const double sx = x - (x * x2) * ps;
+  double pc = 1.0 / 20922789888000.0;  // cos: 1/16!, -1/14!, ...
+  pc = pc * x2 - 1.0 / 87178291200.0;
+  pc = pc * x2 + 1.0 / 479001600.0;
+  pc = pc * x2 - 1.0 / 3628800.0;
+  pc = pc * x2 + 1.0 / 40320.0;
+  pc = pc * x2 - 1.0 / 720.0;
+  pc = pc * x2 + 1.0 / 24.0;
+  const double cx = (1.0 - 0.5 * x2) + (x2 * x2) * pc;
+  const double S = (q & 1) ? cx : sx, C = (q & 1) ? sx : cx;  // sine, cosine within the quadrant
+  switch ((q >> 1) & 3) {
+    case 0: c = C, sn = S; break;
+    case 1: c = -S, sn = C; break;
+    case 2: c = -C, sn = -S; break;
+    default: c = S, sn = -C; break;
+  }
+}
+
 // One block's direction values, kept while consecutive components of a lane
 // come from it.  Rademacher: the block's four words; Gaussian: the cosine
 // and sine branch.
@@ -77,10 +148,11 @@ __host__ __device__ __forceinline__ void direction_fill(uint32_t kind, uint64_t 
     d.w0 = r.v[0], d.w1 = r.v[1], d.w2 = r.v[2], d.w3 = r.v[3];
     return;
   }
-  const double rad = sqrt(-2.0 * log(philox_u53(r.v[0], r.v[1])));
-  const double th = 2.0 * 3.141592653589793 * philox_u53(r.v[2], r.v[3]);
-  d.g0 = rad * cos(th);
-  d.g1 = rad * sin(th);
+  const double rad = sqrt(-2.0 * dir_log(philox_u53(r.v[0], r.v[1])));
+  double c, sn;
+  dir_cos_sin(philox_u53(r.v[2], r.v[3]), c, sn);
+  d.g0 = rad * c;
+  d.g1 = rad * sn;
 }
 
 // Direction component `idx` of the (seed, STREAM_DIRS) stream, through a

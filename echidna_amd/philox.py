@@ -16,7 +16,8 @@ Direction components (``direction``: the STDE and jet directions drawn on
 the device, stream STREAM_DIRS) share blocks: Rademacher component idx is
 bit idx mod 128 of block idx // 128 (bit b of word b // 32, set = -1);
 Gaussian components 2k and 2k + 1 are the cosine and sine branches of block
-k's Box-Muller pair.
+k's Box-Muller pair, computed from binary64 basic operations (``dir_log``,
+``dir_cos_sin``) in the device's order, so both sides give the same bits.
 """
 
 from __future__ import annotations
@@ -109,6 +110,61 @@ def rademacher(seed: int, stream: int, idx) -> np.ndarray:
 DIR_RADEMACHER, DIR_GAUSSIAN = 1, 2
 
 
+def dir_log(u: np.ndarray) -> np.ndarray:
+    """ln u for u in (0, 1], from binary64 basic operations in the order of
+    ``csrc/philox.h`` dir_log: u = 2^e f, f in [sqrt(1/2), sqrt(2)),
+    s = (f - 1)/(f + 1), ln f = 2 atanh s by its series to s^19, and a
+    Cody-Waite split of ln 2."""
+    u = np.asarray(u, dtype=np.float64)
+    b = u.view(np.uint64)
+    e = ((b >> np.uint64(52)) & np.uint64(0x7FF)).astype(np.int64) - 1023
+    f = ((b & np.uint64(0x000FFFFFFFFFFFFF)) | np.uint64(0x3FF0000000000000)).view(np.float64)
+    big = f > 1.4142135623730951
+    f = np.where(big, f * 0.5, f)
+    e = np.where(big, e + 1, e)
+    s = (f - 1.0) / (f + 1.0)
+    t = s * s
+    p = np.full_like(s, 1.0 / 19.0)
+    for k in (17, 15, 13, 11, 9, 7, 5, 3):
+        p = p * t + 1.0 / k
+    lnf = 2.0 * s + (2.0 * s) * (t * p)
+    de = e.astype(np.float64)
+    return de * 0.6931467056274414 + (lnf + de * 4.7493250390316726e-07)
+
+
+_SIN = (-1.0 / 355687428096000.0, 1.0 / 1307674368000.0, -1.0 / 6227020800.0, 1.0 / 39916800.0,
+        -1.0 / 362880.0, 1.0 / 5040.0, -1.0 / 120.0, 1.0 / 6.0)
+_COS = (1.0 / 20922789888000.0, -1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0,
+        -1.0 / 720.0, 1.0 / 24.0)
+
+
+def dir_cos_sin(u: np.ndarray):
+    """cos and sin of 2 pi u in the order of ``csrc/philox.h`` dir_cos_sin:
+    exact octant reduction of 8u, Taylor series within the octant, the
+    quadrant's rotation."""
+    u = np.asarray(u, dtype=np.float64)
+    t8 = u * 8.0
+    q = t8.astype(np.int64)
+    r = t8 - q.astype(np.float64)
+    odd = (q & 1) == 1
+    x = np.where(odd, 1.0 - r, r) * 0.7853981633974483
+    x2 = x * x
+    ps = np.full_like(x, _SIN[0])
+    for c in _SIN[1:]:
+        ps = ps * x2 + c
+    sx = x - (x * x2) * ps
+    pc = np.full_like(x, _COS[0])
+    for c in _COS[1:]:
+        pc = pc * x2 + c
+    cx = (1.0 - 0.5 * x2) + (x2 * x2) * pc
+    S = np.where(odd, cx, sx)
+    C = np.where(odd, sx, cx)
+    k = (q >> 1) & 3
+    cos = np.select([k == 0, k == 1, k == 2], [C, -S, -C], S)
+    sin = np.select([k == 0, k == 1, k == 2], [S, C, -S], -C)
+    return cos, sin
+
+
 def direction(kind: int, seed: int, idx) -> np.ndarray:
     """Direction components ``idx`` of the (seed, STREAM_DIRS) stream, as the
     device draws them (``csrc/philox.h`` philox_direction)."""
@@ -121,9 +177,9 @@ def direction(kind: int, seed: int, idx) -> np.ndarray:
         return np.where(bit == 0, 1.0, -1.0)
     if kind == DIR_GAUSSIAN:
         w0, w1, w2, w3 = _words(seed, STREAM_DIRS, idx >> np.uint64(1))
-        rad = np.sqrt(-2.0 * np.log(_u53(w0, w1)))
-        th = 2.0 * np.pi * _u53(w2, w3)
-        return np.where((idx & np.uint64(1)) == 0, rad * np.cos(th), rad * np.sin(th))
+        rad = np.sqrt(-2.0 * dir_log(_u53(w0, w1)))
+        c, s = dir_cos_sin(_u53(w2, w3))
+        return np.where((idx & np.uint64(1)) == 0, rad * c, rad * s)
     raise ValueError(f"direction kind {kind}: 1 Rademacher or 2 Gaussian")
 
 
