@@ -63,6 +63,30 @@ def test_direction_blocks():
     assert np.array_equal(L[1], z[70:80]) and np.array_equal(L[0], z[:10])
 
 
+def test_direction_box_muller_operations():
+    """The Gaussian directions' ln and (cos, sin) of 2 pi u, computed from
+    basic binary64 operations (philox.dir_log, dir_cos_sin, the device's
+    order): within 1 ulp of libm's log and 1e-15 of its cos and sin over
+    the uniforms the draws produce (multiples of 2^-54 in (0, 1), the ends
+    included), and the f32-rounded Box-Muller values equal to libm's."""
+    rng = np.random.default_rng(11)
+    m = np.concatenate([rng.integers(0, 2 ** 53, 200000, dtype=np.uint64),
+                        np.array([0, 1, 2, 2 ** 52, 2 ** 53 - 1, 2 ** 53 - 2], dtype=np.uint64)])
+    u = (m.astype(np.float64) + 0.5) * 2.0 ** -53
+    lg = philox.dir_log(u)
+    ref = np.log(u)
+    assert np.all(np.abs(lg - ref) <= np.spacing(np.abs(ref)))
+    assert philox.dir_log(np.array([1.0]))[0] == 0.0 and philox.dir_log(np.array([0.5]))[0] == -math.log(2.0)
+    c, sn = philox.dir_cos_sin(u)
+    th = 2.0 * np.pi * u
+    assert np.max(np.abs(c - np.cos(th))) <= 1e-15 and np.max(np.abs(sn - np.sin(th))) <= 1e-15
+    for q in range(8):  # octant edges: 8u = q exactly
+        cq, sq = philox.dir_cos_sin(np.array([q / 8.0]))
+        assert abs(cq[0] - math.cos(q * math.pi / 4)) <= 1e-15 and abs(sq[0] - math.sin(q * math.pi / 4)) <= 1e-15
+    rad = np.sqrt(-2.0 * lg)
+    assert np.array_equal((rad * c).astype(np.float32), (np.sqrt(-2.0 * ref) * np.cos(th)).astype(np.float32))
+
+
 def test_rosenbrock_gradient_closed_form():
     tape = wl.rosenbrock(2)
     pts = np.array([[-1.2, 1.0], [1.0, 1.0], [0.0, 0.0], [0.3, -0.7]])
