@@ -6,8 +6,9 @@
 // (2,3); Rademacher signs come from bit 0 of word 0.  Direction components
 // (philox_direction, the STDE and jet directions) share blocks: Rademacher
 // component idx is bit idx mod 128 of block idx / 128 (bit b of word b / 32),
-// Gaussian components idx = 2k and 2k + 1 are the cosine and sine branches of
-// block k's Box-Muller pair (dir_log, dir_cos_sin: basic operations only).
+// Gaussian components idx = 4k .. 4k + 3 are the cosine and sine branches of
+// block k's two binary32 Box-Muller pairs, (radius word 0, angle word 1) and
+// (word 2, word 3) (dir_log, dir_cos_sin: basic operations only).
 #pragma once
 // Also compiled at run time by hipRTC (tape-specialised jet kernels), which
 // supplies the HIP runtime declarations and fixed-width integers itself.
@@ -56,74 +57,71 @@ __host__ __device__ __forceinline__ double philox_draw(uint32_t kind, uint64_t s
   return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
 }
 
-// Direction components come in blocks: 128 Rademacher signs or 2 Gaussians.
+// Direction components come in blocks: 128 Rademacher signs or 4 Gaussians.
 __host__ __device__ __forceinline__ uint64_t direction_block(uint32_t kind, uint64_t idx) {
-  return kind == RNG_RADEMACHER ? idx >> 7 : idx >> 1;
+  return kind == RNG_RADEMACHER ? idx >> 7 : idx >> 2;
 }
 
-// Box-Muller for the direction draws from IEEE binary64 basic operations
-// only (+ - * /, sqrt, exact bit manipulation), so that the host mirror
-// (echidna_amd/philox.py) gives the same bits.
+// Uniform in (0, 1) from the top 23 bits of a word with a half-ulp offset:
+// (m + 1/2) 2^-23, exact in binary32.
+__host__ __device__ __forceinline__ float philox_u23(uint32_t w) {
+  return ((float)(w >> 9) + 0.5f) * 1.1920928955078125e-07f;
+}
+
+// Box-Muller for the direction draws from IEEE binary32 basic operations
+// only (+ - * /, sqrt, exact bit manipulation; no contraction), so that the
+// host mirror (echidna_amd/philox.py, numpy float32) gives the same bits.
 //
 // ln u for u in (0, 1]: u = 2^e f with f in [sqrt(1/2), sqrt(2)); with
-// s = (f - 1) / (f + 1), ln f = 2 atanh s = 2 (s + s^3/3 + .. + s^19/19)
-// (|s| <= 0.1716: the next term is below 2^-53 of the sum); ln u = e ln2_hi
-// + (ln f + e ln2_lo) with ln 2 split Cody-Waite style (ln2_hi has 21
-// significant bits, so e ln2_hi is exact).
-__host__ __device__ __forceinline__ double dir_log(double u) {
-  uint64_t b;
-  __builtin_memcpy(&b, &u, 8);
-  int e = (int)((b >> 52) & 0x7FFu) - 1023;
-  b = (b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;  // f in [1, 2)
-  double f;
-  __builtin_memcpy(&f, &b, 8);
-  if (f > 1.4142135623730951) {
-    f = f * 0.5;
+// s = (f - 1) / (f + 1), ln f = 2 atanh s = 2 (s + s^3/3 + .. + s^11/11)
+// (|s| <= 0.1716: the next term is below 2^-30 of the sum); ln u = e ln2_hi
+// + (ln f + e ln2_lo) with ln 2 split Cody-Waite style (ln2_hi has 15
+// significant bits, so e ln2_hi is exact for the exponents of u).
+__host__ __device__ __forceinline__ float dir_log(float u) {
+  uint32_t b;
+  __builtin_memcpy(&b, &u, 4);
+  int e = (int)((b >> 23) & 0xFFu) - 127;
+  b = (b & 0x007FFFFFu) | 0x3F800000u;  // f in [1, 2)
+  float f;
+  __builtin_memcpy(&f, &b, 4);
+  if (f > 1.41421354f) {
+    f = f * 0.5f;
     e = e + 1;
   }
-  const double s = (f - 1.0) / (f + 1.0), t = s * s;
-  double p = 1.0 / 19.0;
-  p = p * t + 1.0 / 17.0;
-  p = p * t + 1.0 / 15.0;
-  p = p * t + 1.0 / 13.0;
-  p = p * t + 1.0 / 11.0;
-  p = p * t + 1.0 / 9.0;
-  p = p * t + 1.0 / 7.0;
-  p = p * t + 1.0 / 5.0;
-  p = p * t + 1.0 / 3.0;
-  const double lnf = 2.0 * s + (2.0 * s) * (t * p);
-  const double de = (double)e;
-  return de * 0.6931467056274414 + (lnf + de * 4.7493250390316726e-07);
+  const float s = (f - 1.0f) / (f + 1.0f), t = s * s;
+  float p = 1.0f / 11.0f;
+  p = p * t + 1.0f / 9.0f;
+  p = p * t + 1.0f / 7.0f;
+  p = p * t + 1.0f / 5.0f;
+  p = p * t + 1.0f / 3.0f;
+  const float lnf = 2.0f * s + (2.0f * s) * (t * p);
+  const float de = (float)e;
+  return de * 0.693145751953125f + (lnf + de * 1.42860677e-06f);
 }
 
-// cos and sin of 2 pi u, u in (0, 1) a multiple of 2^-54: 8u = q + r
-// exactly (octant q, r in [0, 1)); the angle within the octant's
-// quadrant is r pi/4 (even q) or (1 - r) pi/4 (odd q, 1 - r exact), whose
-// sine and cosine come from their Taylor series to x^17 and x^16 (|x| <=
-// pi/4: the next terms are below 2^-60), then the quadrant's rotation.
-__host__ __device__ __forceinline__ void dir_cos_sin(double u, double& c, double& sn) {
-  const double t8 = u * 8.0;
+// cos and sin of 2 pi u, u in (0, 1) an odd multiple of 2^-24: 8u = q + r
+// exactly (octant q, r in [0, 1)); the angle within the octant's quadrant is
+// r pi/4 (even q) or (1 - r) pi/4 (odd q, 1 - r exact), whose sine and
+// cosine come from their Taylor series to x^11 and x^12 (|x| <= pi/4: the
+// next terms are below 2^-36), then the quadrant's rotation.
+__host__ __device__ __forceinline__ void dir_cos_sin(float u, float& c, float& sn) {
+  const float t8 = u * 8.0f;
   const int q = (int)t8;
-  const double r = t8 - (double)q;
-  const double x = ((q & 1) ? (1.0 - r) : r) * 0.7853981633974483, x2 = x * x;
-  double ps = -1.0 / 355687428096000.0;  // sin: -1/17!, +1/15!, ...
-  ps = ps * x2 + 1.0 / 1307674368000.0;
-  ps = ps * x2 - 1.0 / 6227020800.0;
-  ps = ps * x2 + 1.0 / 39916800.0;
-  ps = ps * x2 - 1.0 / 362880.0;
-  ps = ps * x2 + 1.0 / 5040.0;
-  ps = ps * x2 - 1.0 / 120.0;
-  ps = ps * x2 + 1.0 / 6.0;
-  const double sx = x - (x * x2) * ps;
-  double pc = 1.0 / 20922789888000.0;  // cos: 1/16!, -1/14!, ...
-  pc = pc * x2 - 1.0 / 87178291200.0;
-  pc = pc * x2 + 1.0 / 479001600.0;
-  pc = pc * x2 - 1.0 / 3628800.0;
-  pc = pc * x2 + 1.0 / 40320.0;
-  pc = pc * x2 - 1.0 / 720.0;
-  pc = pc * x2 + 1.0 / 24.0;
-  const double cx = (1.0 - 0.5 * x2) + (x2 * x2) * pc;
-  const double S = (q & 1) ? cx : sx, C = (q & 1) ? sx : cx;  // sine, cosine within the quadrant
+  const float r = t8 - (float)q;
+  const float x = ((q & 1) ? (1.0f - r) : r) * 0.785398185f, x2 = x * x;
+  float ps = 1.0f / 39916800.0f;  // sin: 1/11!, -1/9!, ...
+  ps = ps * x2 - 1.0f / 362880.0f;
+  ps = ps * x2 + 1.0f / 5040.0f;
+  ps = ps * x2 - 1.0f / 120.0f;
+  ps = ps * x2 + 1.0f / 6.0f;
+  const float sx = x - (x * x2) * ps;
+  float pc = 1.0f / 479001600.0f;  // cos: 1/12!, -1/10!, ...
+  pc = pc * x2 - 1.0f / 3628800.0f;
+  pc = pc * x2 + 1.0f / 40320.0f;
+  pc = pc * x2 - 1.0f / 720.0f;
+  pc = pc * x2 + 1.0f / 24.0f;
+  const float cx = (1.0f - 0.5f * x2) + (x2 * x2) * pc;
+  const float S = (q & 1) ? cx : sx, C = (q & 1) ? sx : cx;  // sine, cosine within the quadrant
   switch ((q >> 1) & 3) {
     case 0: c = C, sn = S; break;
     case 1: c = -S, sn = C; break;
@@ -132,13 +130,22 @@ __host__ __device__ __forceinline__ void dir_cos_sin(double u, double& c, double
   }
 }
 
+// One Box-Muller pair: radius from word wr, angle from word wa.
+__host__ __device__ __forceinline__ void dir_pair(uint32_t wr, uint32_t wa, float& g0, float& g1) {
+  const float rad = sqrtf(-2.0f * dir_log(philox_u23(wr)));
+  float c, sn;
+  dir_cos_sin(philox_u23(wa), c, sn);
+  g0 = rad * c;
+  g1 = rad * sn;
+}
+
 // One block's direction values, kept while consecutive components of a lane
-// come from it.  Rademacher: the block's four words; Gaussian: the cosine
-// and sine branch.
+// come from it.  Rademacher: the block's four words; Gaussian: the two pairs'
+// cosine and sine branches.
 struct DirBlock {  // scalar fields: an array here would be indexed through scratch memory
   uint64_t blk = ~(uint64_t)0;
   uint32_t w0 = 0u, w1 = 0u, w2 = 0u, w3 = 0u;
-  double g0 = 0.0, g1 = 0.0;
+  float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
 };
 
 __host__ __device__ __forceinline__ void direction_fill(uint32_t kind, uint64_t seed, uint64_t blk, DirBlock& d) {
@@ -148,11 +155,8 @@ __host__ __device__ __forceinline__ void direction_fill(uint32_t kind, uint64_t 
     d.w0 = r.v[0], d.w1 = r.v[1], d.w2 = r.v[2], d.w3 = r.v[3];
     return;
   }
-  const double rad = sqrt(-2.0 * dir_log(philox_u53(r.v[0], r.v[1])));
-  double c, sn;
-  dir_cos_sin(philox_u53(r.v[2], r.v[3]), c, sn);
-  d.g0 = rad * c;
-  d.g1 = rad * sn;
+  dir_pair(r.v[0], r.v[1], d.g0, d.g1);
+  dir_pair(r.v[2], r.v[3], d.g2, d.g3);
 }
 
 // Direction component `idx` of the (seed, STREAM_DIRS) stream, through a
@@ -165,7 +169,8 @@ __host__ __device__ __forceinline__ double philox_direction(uint32_t kind, uint6
     const uint32_t lo = (b & 32u) ? d.w1 : d.w0, hi = (b & 32u) ? d.w3 : d.w2;
     return ((((b & 64u) ? hi : lo) >> (b & 31u)) & 1u) ? -1.0 : 1.0;
   }
-  return (idx & 1u) ? d.g1 : d.g0;
+  const uint32_t k = (uint32_t)idx & 3u;
+  return (double)((k & 2u) ? ((k & 1u) ? d.g3 : d.g2) : ((k & 1u) ? d.g1 : d.g0));
 }
 
 __host__ __device__ __forceinline__ double philox_direction(uint32_t kind, uint64_t seed, uint64_t idx) {

@@ -110,64 +110,84 @@ def rademacher(seed: int, stream: int, idx) -> np.ndarray:
 DIR_RADEMACHER, DIR_GAUSSIAN = 1, 2
 
 
+_F = np.float32
+
+
+def _u23(w):
+    """Uniform in (0, 1) from the top 23 bits of a u32 word with a half-ulp
+    offset, exact in binary32 (``csrc/philox.h`` philox_u23)."""
+    return ((w >> np.uint32(9)).astype(np.float32) + _F(0.5)) * _F(1.1920928955078125e-07)
+
+
 def dir_log(u: np.ndarray) -> np.ndarray:
-    """ln u for u in (0, 1], from binary64 basic operations in the order of
+    """ln u for u in (0, 1], from binary32 basic operations in the order of
     ``csrc/philox.h`` dir_log: u = 2^e f, f in [sqrt(1/2), sqrt(2)),
-    s = (f - 1)/(f + 1), ln f = 2 atanh s by its series to s^19, and a
+    s = (f - 1)/(f + 1), ln f = 2 atanh s by its series to s^11, and a
     Cody-Waite split of ln 2."""
-    u = np.asarray(u, dtype=np.float64)
-    b = u.view(np.uint64)
-    e = ((b >> np.uint64(52)) & np.uint64(0x7FF)).astype(np.int64) - 1023
-    f = ((b & np.uint64(0x000FFFFFFFFFFFFF)) | np.uint64(0x3FF0000000000000)).view(np.float64)
-    big = f > 1.4142135623730951
-    f = np.where(big, f * 0.5, f)
+    u = np.asarray(u, dtype=np.float32)
+    b = u.view(np.uint32)
+    e = ((b >> np.uint32(23)) & np.uint32(0xFF)).astype(np.int32) - 127
+    f = ((b & np.uint32(0x007FFFFF)) | np.uint32(0x3F800000)).view(np.float32)
+    big = f > _F(1.41421354)
+    f = np.where(big, f * _F(0.5), f)
     e = np.where(big, e + 1, e)
-    s = (f - 1.0) / (f + 1.0)
+    s = (f - _F(1)) / (f + _F(1))
     t = s * s
-    p = np.full_like(s, 1.0 / 19.0)
-    for k in (17, 15, 13, 11, 9, 7, 5, 3):
-        p = p * t + 1.0 / k
-    lnf = 2.0 * s + (2.0 * s) * (t * p)
-    de = e.astype(np.float64)
-    return de * 0.6931467056274414 + (lnf + de * 4.7493250390316726e-07)
+    p = np.full_like(s, _F(1) / _F(11))
+    for k in (9, 7, 5, 3):
+        p = p * t + _F(1) / _F(k)
+    lnf = _F(2) * s + (_F(2) * s) * (t * p)
+    de = e.astype(np.float32)
+    return de * _F(0.693145751953125) + (lnf + de * _F(1.42860677e-06))
 
 
-_SIN = (-1.0 / 355687428096000.0, 1.0 / 1307674368000.0, -1.0 / 6227020800.0, 1.0 / 39916800.0,
-        -1.0 / 362880.0, 1.0 / 5040.0, -1.0 / 120.0, 1.0 / 6.0)
-_COS = (1.0 / 20922789888000.0, -1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0, 1.0 / 40320.0,
-        -1.0 / 720.0, 1.0 / 24.0)
+# Taylor coefficients, highest order first, each 1/k! rounded to binary32
+_SIN = ((1, 39916800), (-1, 362880), (1, 5040), (-1, 120), (1, 6))
+_COS = ((1, 479001600), (-1, 3628800), (1, 40320), (-1, 720), (1, 24))
 
 
 def dir_cos_sin(u: np.ndarray):
-    """cos and sin of 2 pi u in the order of ``csrc/philox.h`` dir_cos_sin:
-    exact octant reduction of 8u, Taylor series within the octant, the
-    quadrant's rotation."""
-    u = np.asarray(u, dtype=np.float64)
-    t8 = u * 8.0
-    q = t8.astype(np.int64)
-    r = t8 - q.astype(np.float64)
+    """cos and sin of 2 pi u in binary32, in the order of ``csrc/philox.h``
+    dir_cos_sin: exact octant reduction of 8u, Taylor series within the
+    octant, the quadrant's rotation."""
+    u = np.asarray(u, dtype=np.float32)
+    t8 = u * _F(8)
+    q = t8.astype(np.int32)
+    r = t8 - q.astype(np.float32)
     odd = (q & 1) == 1
-    x = np.where(odd, 1.0 - r, r) * 0.7853981633974483
+    x = np.where(odd, _F(1) - r, r) * _F(0.785398185)
     x2 = x * x
-    ps = np.full_like(x, _SIN[0])
-    for c in _SIN[1:]:
-        ps = ps * x2 + c
+
+    def horner(coefs):
+        sg, d = coefs[0]
+        acc = np.full_like(x, _F(1) / _F(d) if sg > 0 else -(_F(1) / _F(d)))
+        for sg, d in coefs[1:]:
+            acc = acc * x2 + _F(1) / _F(d) if sg > 0 else acc * x2 - _F(1) / _F(d)
+        return acc
+
+    ps = horner(_SIN)
     sx = x - (x * x2) * ps
-    pc = np.full_like(x, _COS[0])
-    for c in _COS[1:]:
-        pc = pc * x2 + c
-    cx = (1.0 - 0.5 * x2) + (x2 * x2) * pc
+    pc = horner(_COS)
+    cx = (_F(1) - _F(0.5) * x2) + (x2 * x2) * pc
     S = np.where(odd, cx, sx)
     C = np.where(odd, sx, cx)
     k = (q >> 1) & 3
     cos = np.select([k == 0, k == 1, k == 2], [C, -S, -C], S)
     sin = np.select([k == 0, k == 1, k == 2], [S, C, -S], -C)
-    return cos, sin
+    return cos.astype(np.float32), sin.astype(np.float32)
+
+
+def dir_pair(wr, wa):
+    """One binary32 Box-Muller pair (cosine, sine branch): radius from word
+    ``wr``, angle from word ``wa`` (``csrc/philox.h`` dir_pair)."""
+    rad = np.sqrt(_F(-2) * dir_log(_u23(wr)))
+    c, s = dir_cos_sin(_u23(wa))
+    return rad * c, rad * s
 
 
 def direction(kind: int, seed: int, idx) -> np.ndarray:
     """Direction components ``idx`` of the (seed, STREAM_DIRS) stream, as the
-    device draws them (``csrc/philox.h`` philox_direction)."""
+    device draws them (``csrc/philox.h`` philox_direction), as float64."""
     idx = np.asarray(idx, dtype=np.uint64)
     if kind == DIR_RADEMACHER:
         w = np.stack(_words(seed, STREAM_DIRS, idx >> np.uint64(7)))
@@ -176,10 +196,11 @@ def direction(kind: int, seed: int, idx) -> np.ndarray:
         bit = (word >> (b & 31).astype(np.uint32)) & np.uint32(1)
         return np.where(bit == 0, 1.0, -1.0)
     if kind == DIR_GAUSSIAN:
-        w0, w1, w2, w3 = _words(seed, STREAM_DIRS, idx >> np.uint64(1))
-        rad = np.sqrt(-2.0 * dir_log(_u53(w0, w1)))
-        c, s = dir_cos_sin(_u53(w2, w3))
-        return np.where((idx & np.uint64(1)) == 0, rad * c, rad * s)
+        w0, w1, w2, w3 = _words(seed, STREAM_DIRS, idx >> np.uint64(2))
+        g0, g1 = dir_pair(w0, w1)
+        g2, g3 = dir_pair(w2, w3)
+        k = (idx & np.uint64(3)).astype(np.int64)
+        return np.select([k == 0, k == 1, k == 2], [g0, g1, g2], g3).astype(np.float64)
     raise ValueError(f"direction kind {kind}: 1 Rademacher or 2 Gaussian")
 
 

@@ -156,9 +156,10 @@ int32_t ehip_hessian_diagonal(ehip_ctx* ctx, const ehip_tape* tape, const void* 
  * supplied (dir_kind 0, dirs [npoints*S*n]) or drawn on the device from the
  * Philox stream `seed` (1 Rademacher, 2 Gaussian; component i of direction s
  * of point p is direction component (p*S+s)*n+i: Rademacher component k is
- * bit k mod 128 of Philox block k / 128, Gaussian components 2k and 2k+1
- * are the cosine and sine branches of block k's binary64 Box-Muller pair,
- * rounded to the tape's type).  results [npoints*5]. */
+ * bit k mod 128 of Philox block k / 128, Gaussian components 4k .. 4k+3
+ * are the cosine and sine branches of block k's two binary32 Box-Muller
+ * pairs, radius and angle from words (0, 1) and (2, 3), each uniform
+ * (top 23 bits + 1/2) 2^-23).  results [npoints*5]. */
 int32_t ehip_laplacian_batch(ehip_ctx* ctx, const ehip_tape* tape, const void* xs, uint64_t npoints, uint32_t S,
                              uint32_t dir_kind, uint64_t seed, const void* dirs, double* results);
 /* Device-pointer form; d_diag (optional) is a diagonal control variate with

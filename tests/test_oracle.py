@@ -38,10 +38,11 @@ def test_philox_distributions():
 def test_direction_blocks():
     """Device direction components (philox.direction): Rademacher component
     128 b + k is bit k of block b's 128 bits (word k // 32, -1 when set),
-    from the KAT-pinned generator; Gaussian components 2k, 2k + 1 are one
-    Box-Muller pair of block k: their squares sum to -2 ln u1 and their angle
-    is 2 pi u2 (u from words 0,1 and 2,3).  Then moments: mean 0, variance 1,
-    pairs and neighbouring signs uncorrelated."""
+    from the KAT-pinned generator; Gaussian components 4k .. 4k + 3 are block
+    k's two Box-Muller pairs: each pair's squares sum to -2 ln u_r and its
+    angle is 2 pi u_a, with u = (top 23 bits + 1/2) 2^-23 of words 0, 1 and
+    2, 3.  Then moments: mean 0, variance 1, pairs and neighbouring signs
+    uncorrelated."""
     seed = 21
     for b in (0, 1, 7, 2 ** 33 + 5):
         w = [int(v) for v in philox.philox4x32_10(b & 0xFFFFFFFF, b >> 32, 0, 0, seed, philox.STREAM_DIRS)]
@@ -49,14 +50,19 @@ def test_direction_blocks():
         want = [-1.0 if (w[k // 32] >> (k % 32)) & 1 else 1.0 for k in range(128)]
         assert list(got) == want, b
     k = np.array([0, 1, 5, 2 ** 34 + 3], dtype=np.uint64)
-    g = philox.direction(philox.DIR_GAUSSIAN, seed, np.stack([2 * k, 2 * k + 1], axis=1).ravel()).reshape(-1, 2)
-    w0, w1, w2, w3 = philox._words(seed, philox.STREAM_DIRS, k)
-    u1 = philox.uniform(seed, philox.STREAM_DIRS, k)
-    u2 = philox._u53(w2, w3)
-    assert np.allclose(g[:, 0] ** 2 + g[:, 1] ** 2, -2 * np.log(u1), rtol=1e-13)
-    assert np.allclose(np.mod(np.arctan2(g[:, 1], g[:, 0]), 2 * np.pi), 2 * np.pi * u2, rtol=1e-12, atol=1e-12)
-    z = philox.direction(philox.DIR_GAUSSIAN, 3, np.arange(400000, dtype=np.uint64))
-    assert abs(z.mean()) < 0.008 and abs(z.var() - 1) < 0.012 and abs(np.mean(z[0::2] * z[1::2])) < 0.012
+    g = philox.direction(philox.DIR_GAUSSIAN, seed, (4 * k[:, None] + np.arange(4, dtype=np.uint64)).ravel())
+    g = g.reshape(-1, 4)
+    w = philox._words(seed, philox.STREAM_DIRS, k)
+    for j in (0, 1):
+        ur = ((w[2 * j] >> 9).astype(np.float64) + 0.5) * 2.0 ** -23
+        ua = ((w[2 * j + 1] >> 9).astype(np.float64) + 0.5) * 2.0 ** -23
+        a, c = g[:, 2 * j], g[:, 2 * j + 1]
+        assert np.allclose(a ** 2 + c ** 2, -2 * np.log(ur), rtol=2e-6)
+        assert np.allclose(np.mod(np.arctan2(c, a), 2 * np.pi), 2 * np.pi * ua, rtol=2e-6, atol=2e-6)
+    assert np.all(g.astype(np.float32).astype(np.float64) == g)
+    z = philox.direction(philox.DIR_GAUSSIAN, 3, np.arange(2000000, dtype=np.uint64))
+    assert abs(z.mean()) < 0.004 and abs(z.var() - 1) < 0.005
+    assert abs(np.mean(z[0::2] * z[1::2])) < 0.005 and abs(np.mean(z[0::4] * z[2::4])) < 0.007
     r = philox.direction(philox.DIR_RADEMACHER, 3, np.arange(400000, dtype=np.uint64))
     assert set(np.unique(r)) == {-1.0, 1.0} and abs(r.mean()) < 0.008 and abs(np.mean(r[1:] * r[:-1])) < 0.008
     L = philox.lane_directions(philox.DIR_GAUSSIAN, 3, 10, [0, 7])
@@ -65,26 +71,32 @@ def test_direction_blocks():
 
 def test_direction_box_muller_operations():
     """The Gaussian directions' ln and (cos, sin) of 2 pi u, computed from
-    basic binary64 operations (philox.dir_log, dir_cos_sin, the device's
-    order): within 1 ulp of libm's log and 1e-15 of its cos and sin over
-    the uniforms the draws produce (multiples of 2^-54 in (0, 1), the ends
-    included), and the f32-rounded Box-Muller values equal to libm's."""
-    rng = np.random.default_rng(11)
-    m = np.concatenate([rng.integers(0, 2 ** 53, 200000, dtype=np.uint64),
-                        np.array([0, 1, 2, 2 ** 52, 2 ** 53 - 1, 2 ** 53 - 2], dtype=np.uint64)])
-    u = (m.astype(np.float64) + 0.5) * 2.0 ** -53
-    lg = philox.dir_log(u)
+    basic binary32 operations (philox.dir_log, dir_cos_sin, the device's
+    order), over every uniform the draws produce ((m + 1/2) 2^-23): ln
+    within 2 ulp of libm's (binary64 log rounded), cos and sin within 1.2e-7
+    of libm's; ln 1 = 0 and ln 1/2 = -ln 2 to the last bit; the octant edges;
+    and the grid's moments E[-2 ln u] = 2 and E[cos^2] = 1/2 (the Box-Muller
+    variance) to 1e-7."""
+    m = np.arange(2 ** 23, dtype=np.uint64)
+    u = (m.astype(np.float64) + 0.5) * 2.0 ** -23
+    u32 = u.astype(np.float32)
+    assert np.array_equal(u32.astype(np.float64), u)
+    assert np.array_equal(philox._u23((m << np.uint64(9)).astype(np.uint32) | np.uint32(0x1FF)), u32)
+    lg = philox.dir_log(u32).astype(np.float64)
     ref = np.log(u)
-    assert np.all(np.abs(lg - ref) <= np.spacing(np.abs(ref)))
-    assert philox.dir_log(np.array([1.0]))[0] == 0.0 and philox.dir_log(np.array([0.5]))[0] == -math.log(2.0)
-    c, sn = philox.dir_cos_sin(u)
+    assert np.all(np.abs(lg - ref) <= 2 * np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64))
+    assert philox.dir_log(np.array([1.0], np.float32))[0] == 0.0
+    assert philox.dir_log(np.array([0.5], np.float32))[0] == np.float32(-math.log(2.0))
+    c, sn = philox.dir_cos_sin(u32)
     th = 2.0 * np.pi * u
-    assert np.max(np.abs(c - np.cos(th))) <= 1e-15 and np.max(np.abs(sn - np.sin(th))) <= 1e-15
+    assert np.max(np.abs(c - np.cos(th))) <= 1.2e-7 and np.max(np.abs(sn - np.sin(th))) <= 1.2e-7
     for q in range(8):  # octant edges: 8u = q exactly
-        cq, sq = philox.dir_cos_sin(np.array([q / 8.0]))
-        assert abs(cq[0] - math.cos(q * math.pi / 4)) <= 1e-15 and abs(sq[0] - math.sin(q * math.pi / 4)) <= 1e-15
-    rad = np.sqrt(-2.0 * lg)
-    assert np.array_equal((rad * c).astype(np.float32), (np.sqrt(-2.0 * ref) * np.cos(th)).astype(np.float32))
+        cq, sq = philox.dir_cos_sin(np.array([q / 8.0], np.float32))
+        assert abs(cq[0] - math.cos(q * math.pi / 4)) <= 1.2e-7 and abs(sq[0] - math.sin(q * math.pi / 4)) <= 1.2e-7
+    assert abs(np.mean(-2.0 * lg) - 2.0) < 1e-7 and abs(np.mean(c.astype(np.float64) ** 2) - 0.5) < 1e-7
+    # the radius a planted ln error would move: sqrt(-2 ln u) against libm's
+    rad = np.sqrt(np.float32(-2) * philox.dir_log(u32)).astype(np.float64)
+    assert np.max(np.abs(rad - np.sqrt(-2.0 * ref)) / np.sqrt(-2.0 * ref)) < 3e-7
 
 
 def test_rosenbrock_gradient_closed_form():
