@@ -31,6 +31,10 @@ struct Gemm {
   int64_t ldaux;
   int32_t accumulate;
   float alpha;
+  // Full tiles store C with the non-temporal hint: an output far larger than
+  // L2 that no later step of this kernel reads then does not evict the
+  // operand tiles every workgroup re-reads (set for L of a whole batch).
+  int32_t stream_out;
   // Split-K workspace (optional): with room for ksplit * M * N floats the
   // launcher may split a long K over `ksplit` slices of the grid, write each
   // slice's sums there and add them in slice order before the epilogue.

@@ -20,6 +20,7 @@ namespace ehip {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int BK = 32, PITCH = BK + 8;
 
@@ -192,7 +193,12 @@ __global__ __launch_bounds__(256, 3) void k_gemm(Gemm g, int kc) {
             const float4 o = *dst;
             v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
           }
-          *dst = v;
+          if (g.stream_out) {
+            const f32x4 w = {v.x, v.y, v.z, v.w};
+            __builtin_nontemporal_store(w, reinterpret_cast<f32x4*>(dst));
+          } else {
+            *dst = v;
+          }
         }
         __builtin_amdgcn_wave_barrier();
       }

@@ -104,8 +104,11 @@ Gemm gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk,
   return g;
 }
 
-// Forward pass of T <= cap tuples: H1, H2 and L (into `Lout`, row pitch n*r).
-int32_t cv_forward(ehip_cv* cv, const float* X, uint32_t T, float* Lout, const Act& a, hipStream_t s) {
+// Forward pass of T <= cap tuples: H1, H2 and L (into `Lout`, row pitch n*r);
+// stream_out: L is written past the caches (a whole batch's factors, read
+// once by the reduction).
+int32_t cv_forward(ehip_cv* cv, const float* X, uint32_t T, float* Lout, const Act& a, hipStream_t s,
+                   bool stream_out = false) {
   const int n = (int)cv->n, H = (int)cv->H, nr = (int)(cv->n * cv->r);
   // H1[t][j] = tanh(sum_i X[t][i] W1[j][i] + b1[j])
   HIP_TRY(launch_gemm(gemm(X, n, 1, P(cv, 0), 1, n, a.H1, H, (int)T, H, n, EPI_BIAS_TANH, P(cv, 1)), s), "cv forward");
@@ -113,6 +116,7 @@ int32_t cv_forward(ehip_cv* cv, const float* X, uint32_t T, float* Lout, const A
           "cv forward");
   Gemm g3 = gemm(a.H2, H, 1, P(cv, 4), 1, H, Lout, nr, (int)T, nr, H, EPI_BIAS, P(cv, 5));
   g3.alpha = cv->gain;
+  g3.stream_out = stream_out ? 1 : 0;
   HIP_TRY(launch_gemm(g3, s), "cv forward");
   return EHIP_OK;
 }
@@ -502,7 +506,7 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
     if (rc) return rc;
     for (uint64_t t0 = 0; t0 < npoints; t0 += chunk) {
       const uint32_t cc = (uint32_t)std::min<uint64_t>(chunk, npoints - t0);
-      rc = cv_forward(cv, xf + t0 * t->nin, cc, Lp + t0 * (uint64_t)cv->n * cv->r, a, s);
+      rc = cv_forward(cv, xf + t0 * t->nin, cc, Lp + t0 * (uint64_t)cv->n * cv->r, a, s, true);
       if (rc) return rc;
     }
     R.lcv = Lp;
