@@ -15,9 +15,11 @@ keeping the cosine branch.  Rademacher: sign from bit 0 of word 0.
 Direction components (``direction``: the STDE and jet directions drawn on
 the device, stream STREAM_DIRS) share blocks: Rademacher component idx is
 bit idx mod 128 of block idx // 128 (bit b of word b // 32, set = -1);
-Gaussian components 2k and 2k + 1 are the cosine and sine branches of block
-k's Box-Muller pair, computed from binary64 basic operations (``dir_log``,
-``dir_cos_sin``) in the device's order, so both sides give the same bits.
+Gaussian components 4k .. 4k + 3 are the cosine and sine branches of block
+k's two Box-Muller pairs (radius from word 0 or 2, angle from word 1 or 3,
+each uniform (top 23 bits + 1/2) 2^-23), computed from binary32 basic
+operations (``dir_log``, ``dir_cos_sin``) in the device's order, so both
+sides give the same bits.
 """
 
 from __future__ import annotations
