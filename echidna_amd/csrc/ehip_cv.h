@@ -35,6 +35,9 @@ struct Gemm {
   // L2 that no later step of this kernel reads then does not evict the
   // operand tiles every workgroup re-reads (set for L of a whole batch).
   int32_t stream_out;
+  // C holds bfloat16 (round to nearest even after the epilogue; ldc in
+  // elements); not with accumulate.
+  int32_t out_bf16;
   // Split-K workspace (optional): with room for ksplit * M * N floats the
   // launcher may split a long K over `ksplit` slices of the grid, write each
   // slice's sums there and add them in slice order before the epilogue.

@@ -21,6 +21,16 @@ namespace ehip {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Two floats as one word of round-to-nearest-even bfloat16s (lo first).
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  const __bf16 a = (__bf16)lo, b = (__bf16)hi;
+  uint16_t ua, ub;
+  __builtin_memcpy(&ua, &a, 2);
+  __builtin_memcpy(&ub, &b, 2);
+  return (uint32_t)ua | ((uint32_t)ub << 16);
+}
 
 constexpr int BK = 32, PITCH = BK + 8;
 
@@ -91,6 +101,7 @@ __device__ __forceinline__ void store_tile(__bf16* lds, const float* o) {
 // Epilogue of one output element (shared by the direct and split-K paths).
 __device__ __forceinline__ void gemm_store(const Gemm& g, int m, int n, float v) {
   float* c = g.C + (int64_t)m * g.ldc + n;
+  __bf16* ch = reinterpret_cast<__bf16*>(g.C) + (int64_t)m * g.ldc + n;
   switch (g.epi) {
     case EPI_BIAS: v = g.alpha * (v + g.bias[n]); break;
     case EPI_BIAS_TANH: v = tanhf(v + g.bias[n]); break;
@@ -101,7 +112,8 @@ __device__ __forceinline__ void gemm_store(const Gemm& g, int m, int n, float v)
     }
     default: break;
   }
-  *c = g.accumulate ? *c + v : v;
+  if (g.out_bf16) *ch = (__bf16)v;
+  else *c = g.accumulate ? *c + v : v;
 }
 
 // blockIdx.z = K slice [kz * kc, (kz + 1) * kc) (kc = K when not split).
@@ -188,6 +200,13 @@ __global__ __launch_bounds__(256, 3) void k_gemm(Gemm g, int kc) {
           const int row = q * 8 + (lane >> 3), c4 = (lane & 7) * 4;
           const float* sr = stage + row * STAGE_PITCH + c4;
           float4 v = make_float4(sr[0], sr[1], sr[2], sr[3]);
+          if (g.out_bf16) {
+            const u32x2 w = {pack_bf16(v.x, v.y), pack_bf16(v.z, v.w)};
+            u32x2* dh = reinterpret_cast<u32x2*>(reinterpret_cast<__bf16*>(g.C) + (int64_t)(mb + row) * g.ldc + nb + c4);
+            if (g.stream_out) __builtin_nontemporal_store(w, dh);
+            else *dh = w;
+            continue;
+          }
           float4* dst = reinterpret_cast<float4*>(g.C + (int64_t)(mb + row) * g.ldc + nb + c4);
           if (g.accumulate) {
             const float4 o = *dst;
@@ -238,6 +257,7 @@ const void* gemm_fn(bool ak, bool bk) {
 
 hipError_t launch_gemm(const Gemm& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (g.out_bf16 && g.accumulate) return hipErrorInvalidValue;
   const bool ak = g.sak == 1, bk = g.sbk == 1;
   if ((!ak && g.sam != 1) || (!bk && g.sbn != 1)) return hipErrorInvalidValue;
   // 128x128 tiles when they still make at least two workgroups per CU (256 CUs)

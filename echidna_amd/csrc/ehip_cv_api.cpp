@@ -104,19 +104,21 @@ Gemm gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk,
   return g;
 }
 
-// Forward pass of T <= cap tuples: H1, H2 and L (into `Lout`, row pitch n*r);
-// stream_out: L is written past the caches (a whole batch's factors, read
-// once by the reduction).
-int32_t cv_forward(ehip_cv* cv, const float* X, uint32_t T, float* Lout, const Act& a, hipStream_t s,
-                   bool stream_out = false) {
+// Forward pass of T <= cap tuples: H1, H2 and L (into `Lout`, row pitch n*r,
+// FP32); for_pass: L as bfloat16 written past the caches (a whole batch's
+// factors, read once by the reduction, which forms its samples and |L|^2
+// from the same rounded values).
+int32_t cv_forward(ehip_cv* cv, const float* X, uint32_t T, void* Lout, const Act& a, hipStream_t s,
+                   bool for_pass = false) {
   const int n = (int)cv->n, H = (int)cv->H, nr = (int)(cv->n * cv->r);
   // H1[t][j] = tanh(sum_i X[t][i] W1[j][i] + b1[j])
   HIP_TRY(launch_gemm(gemm(X, n, 1, P(cv, 0), 1, n, a.H1, H, (int)T, H, n, EPI_BIAS_TANH, P(cv, 1)), s), "cv forward");
   HIP_TRY(launch_gemm(gemm(a.H1, H, 1, P(cv, 2), 1, H, a.H2, H, (int)T, H, H, EPI_BIAS_TANH, P(cv, 3)), s),
           "cv forward");
-  Gemm g3 = gemm(a.H2, H, 1, P(cv, 4), 1, H, Lout, nr, (int)T, nr, H, EPI_BIAS, P(cv, 5));
+  Gemm g3 = gemm(a.H2, H, 1, P(cv, 4), 1, H, (float*)Lout, nr, (int)T, nr, H, EPI_BIAS, P(cv, 5));
   g3.alpha = cv->gain;
-  g3.stream_out = stream_out ? 1 : 0;
+  g3.stream_out = for_pass ? 1 : 0;
+  g3.out_bf16 = for_pass ? 1 : 0;
   HIP_TRY(launch_gemm(g3, s), "cv forward");
   return EHIP_OK;
 }
@@ -498,8 +500,8 @@ int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* c, const ehip_tape* t, ehip_cv* cv
       HIP_TRY(launch_to_f32(d_x, true, cnt, (float*)c->cvx.p, s), "cv inputs");
       xf = (const float*)c->cvx.p;
     }
-    HIP_TRY(c->cvl.reserve(npoints * (uint64_t)cv->n * cv->r * sizeof(float)), "cv factors");
-    float* Lp = (float*)c->cvl.p;
+    HIP_TRY(c->cvl.reserve(npoints * (uint64_t)cv->n * cv->r * 2), "cv factors");
+    uint16_t* Lp = (uint16_t*)c->cvl.p;  // bfloat16
     const uint32_t chunk = (uint32_t)std::min<uint64_t>(npoints, CV_INF_CHUNK);
     Act a;
     rc = cv_reserve_inf(cv, chunk, a);

@@ -113,7 +113,8 @@ struct StdeReduce {
   const void* diag;       // diagonal control variate [.. ][nin] or null
   uint64_t diag_stride;   // 0: one diagonal shared by all points
   double* out;            // [npoints][5]: value, estimate, variance, std error, count
-  const float* lcv;       // learned control variate L(x) [npoints][nin*rank] or null
+  const void* lcv;        // learned control variate L(x) [npoints][nin*rank] or null: bfloat16 with
+                          // dirs_cmajor, else float
   uint32_t rank;          // r <= 32
   uint32_t npos;          // columns j < npos enter with +1, the rest with -1
   double* out_plain;      // plain-estimator statistics of the same samples, or null

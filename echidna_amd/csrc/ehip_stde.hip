@@ -90,7 +90,7 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
   const F* J = reinterpret_cast<const F*>(R.jets);
   const F* D = reinterpret_cast<const F*>(R.dirs);
   const F* DG = reinterpret_cast<const F*>(R.diag);
-  const float* LC = R.lcv ? R.lcv + pt * (uint64_t)R.nin * R.rank : nullptr;
+  const float* LC = R.lcv ? reinterpret_cast<const float*>(R.lcv) + pt * (uint64_t)R.nin * R.rank : nullptr;
   // |L|_F^2 of this point, summed over the wave
   double lnorm = 0.0;
   if (LC) {
@@ -182,9 +182,11 @@ __global__ __launch_bounds__(64) void k_stde_reduce(StdeReduce R) {
 // read 16 consecutive samples of one coordinate; step t of a 16-coordinate
 // slab gives lane group g coordinate 4 t + g, so the wave reads four
 // consecutive rows of L.  Four slabs' operands are loaded before their
-// MFMAs run.  The products of FP32 (or FP64) operands are formed and summed
-// in binary64.  One wave per point, four points per workgroup; |L|_F^2 is
-// summed from the same loads.
+// MFMAs run.  L is bfloat16 (half the bytes of the pass's largest read
+// after the directions); the products of its values and the FP32 (or FP64)
+// directions are formed and summed in binary64.  One wave per point, four
+// points per workgroup; |L|_F^2 is summed from the same loads, so the
+// control variate's mean is exactly that of the rounded L.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <class F, uint32_t NRB>
@@ -196,14 +198,14 @@ __global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
   const F* J = reinterpret_cast<const F*>(R.jets);
   const F* D = reinterpret_cast<const F*>(R.dirs_cmajor);
   const uint64_t NL = R.npoints * (uint64_t)R.S;  // coordinate stride of the directions
-  const float* LC = R.lcv + pt * (uint64_t)R.nin * R.rank;
+  const __bf16* LC = reinterpret_cast<const __bf16*>(R.lcv) + pt * (uint64_t)R.nin * R.rank;
   const uint32_t n = R.nin, r = R.rank;
   double lnorm = 0.0;
   Wf w{0.0, 0.0, 0.0}, wp{0.0, 0.0, 0.0};
   // Lanes past the rank or the sample count read a valid element (column 0,
   // sample s0) and zero it: no divergent loads in the loop.
   bool jok[NRB];
-  const float* pb0[NRB];
+  const __bf16* pb0[NRB];
 #pragma unroll
   for (uint32_t rb = 0; rb < NRB; ++rb) {
     const uint32_t j = rb * 16u + i16;
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
     pb0[rb] = LC + (uint64_t)g * r + (jok[rb] ? j : 0u);
   }
   // one slab (16 coordinates, four MFMA steps) from coordinate k0 + g on
-  auto slab = [&](f64x4* acc, const F*& qa, const float** qb, bool first, bool sok, uint32_t k0) {
+  auto slab = [&](f64x4* acc, const F*& qa, const __bf16** qb, bool first, bool sok, uint32_t k0) {
     F a[4];
     float b[NRB][4];
 #pragma unroll
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
       qa += 4 * NL;
 #pragma unroll
       for (uint32_t rb = 0; rb < NRB; ++rb) {
-        b[rb][t] = cok ? *qb[rb] : 0.0f;
+        b[rb][t] = cok ? (float)*qb[rb] : 0.0f;
         qb[rb] += 4 * r;
       }
     }
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
     const uint32_t sa = s0 + i16;  // the A row this lane supplies
     const bool sok = sa < R.S, first = s0 == 0;
     const F* qa = D + (uint64_t)g * NL + pt * R.S + (sok ? sa : s0);
-    const float* qb[NRB];
+    const __bf16* qb[NRB];
 #pragma unroll
     for (uint32_t rb = 0; rb < NRB; ++rb) qb[rb] = pb0[rb];
     f64x4 acc[NRB];
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(256) void k_stde_reduce_lcv(StdeReduce R) {
         qa += 4 * NL;
 #pragma unroll
         for (uint32_t rb = 0; rb < NRB; ++rb) {
-          b[rb][q] = *qb[rb];
+          b[rb][q] = (float)*qb[rb];
           qb[rb] += 4 * r;
         }
       }

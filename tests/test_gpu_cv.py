@@ -219,13 +219,28 @@ def _welford(x):
     return np.mean(x), np.var(x, ddof=1) if len(x) > 1 else 0.0
 
 
+def _bf16(a):
+    """FP32 values rounded to bfloat16, to nearest even (finite inputs)."""
+    b = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = ((b + np.uint64(0x7FFF) + ((b >> np.uint64(16)) & np.uint64(1))) >> np.uint64(16)) << np.uint64(16)
+    return b.astype(np.uint32).view(np.float32)
+
+
+def test_bf16_rounding_helper():
+    """Ties go to the even bfloat16; torch's conversion agrees."""
+    x = np.array([1.0, 1.00390625, 1.01171875, 1.0000001, -3.14159, 1e-30, 65504.0], np.float32)
+    assert _bf16(x)[1] == 1.0 and _bf16(x)[2] == np.float32(1.015625)
+    assert np.array_equal(_bf16(x), torch.from_numpy(x).bfloat16().float().numpy())
+
+
 @pytest.mark.parametrize("f64", [False, True], ids=["f32", "f64"])
 @pytest.mark.parametrize("n,rank,npos,S", [(40, 12, 5, 20), (64, 16, 8, 16), (36, 24, 24, 37), (33, 32, 10, 3)])
 def test_controlled_statistics_match_a_host_restatement(hip, f64, n, rank, npos, S):
     """Per-point statistics of the controlled samples 2 c2 - (v^T L S L^T v -
     tr(L S L^T)) against numpy in binary64 from the same pieces: the jets of
     the same points and Philox directions (one GPU evaluation), L(x) from the
-    network's forward pass, the Welford mean and sample variance.  Shapes
+    network's forward pass rounded to bfloat16 (as the pass stores it), the
+    Welford mean and sample variance.  Shapes
     straddle the 16-sample x 16-column matrix tile and the 16-coordinate slab
     (partial tiles, two column blocks, n not a multiple of 4, S below 4).
     Bar: the two sides sum the same binary64 products in different orders,
@@ -250,7 +265,8 @@ def test_controlled_statistics_match_a_host_restatement(hip, f64, n, rank, npos,
                              torch.from_numpy(V).cuda(), P * S, 3, dJ)
     torch.cuda.synchronize()
     J[:] = dJ.cpu().numpy()
-    L = net.forward(dX.float()).cpu().numpy().astype(np.float64).reshape(P, n, rank)
+    # the pass keeps L(x) in bfloat16 (the network's FP32 output rounded)
+    L = _bf16(net.forward(dX.float()).cpu().numpy()).astype(np.float64).reshape(P, n, rank)
     sign = np.where(np.arange(rank) < npos, 1.0, -1.0)
     for p in range(P):
         v = V[p * S:(p + 1) * S].astype(np.float64)
