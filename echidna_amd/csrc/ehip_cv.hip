@@ -98,6 +98,19 @@ __device__ __forceinline__ void store_tile(__bf16* lds, const float* o) {
   }
 }
 
+// The epilogue's value of output element (m, n) from its sum v.
+__device__ __forceinline__ float epi_value(const Gemm& g, int m, int n, float v) {
+  switch (g.epi) {
+    case EPI_BIAS: return g.alpha * (v + g.bias[n]);
+    case EPI_BIAS_TANH: return tanhf(v + g.bias[n]);
+    case EPI_DTANH: {
+      const float h = g.aux[(int64_t)m * g.ldaux + n];
+      return v * (1.0f - h * h);
+    }
+    default: return v;
+  }
+}
+
 // Epilogue of one output element (shared by the direct and split-K paths).
 __device__ __forceinline__ void gemm_store(const Gemm& g, int m, int n, float v) {
   float* c = g.C + (int64_t)m * g.ldc + n;
@@ -126,7 +139,8 @@ constexpr int STAGE_PITCH = 33;
 template <int T, bool AK, bool BKC>
 __global__ __launch_bounds__(256, 3) void k_gemm(Gemm g, int kc) {
   constexpr int BM = 64 * T, BN = 64 * T;
-  constexpr int OPB = (BM + BN) * PITCH * 2, STB = 4 * 32 * STAGE_PITCH * 4;
+  constexpr int SPB = 32 * T + 1;  // bfloat16 output: a wave's 32 x 32T strip, row pitch SPB floats
+  constexpr int OPB = (BM + BN) * PITCH * 2, STB = 4 * 32 * SPB * 4;
   __shared__ __attribute__((aligned(16))) unsigned char smem[OPB > STB ? OPB : STB];
   __bf16* As = reinterpret_cast<__bf16*>(smem);
   __bf16* Bs = As + BM * PITCH;
@@ -169,6 +183,39 @@ __global__ __launch_bounds__(256, 3) void k_gemm(Gemm g, int kc) {
   }
   const bool full = gridDim.z == 1 && m0 + BM <= g.M && n0 + BN <= g.N && (g.ldc & 3) == 0 &&
                     (reinterpret_cast<uintptr_t>(g.C) & 15u) == 0;
+  if (full && g.out_bf16) {
+    // bfloat16 C: each wave stages the whole 32 x 32T strip of a row block
+    // and stores it as rows of 64T bytes (whole 128-byte lines for T = 2:
+    // half lines written at different times leave HBM masked writes)
+    __syncthreads();  // every wave is done reading the operand tiles
+    float* st = reinterpret_cast<float*>(smem) + wave * (32 * SPB);
+#pragma unroll
+    for (int bi = 0; bi < T; ++bi) {
+      const int mb = m0 + wm + 32 * bi;
+#pragma unroll
+      for (int bj = 0; bj < T; ++bj) {
+        const int nl = lane & 31, nb = n0 + wn + 32 * bj;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int ml = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          st[ml * SPB + 32 * bj + nl] = epi_value(g, mb + ml, nb + nl, acc[bi][bj][j]);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      constexpr int LPR = 8 * T, RPP = 64 / LPR;  // lanes per row, rows per pass
+#pragma unroll
+      for (int q = 0; q < 32 / RPP; ++q) {
+        const int row = q * RPP + lane / LPR, c4 = (lane % LPR) * 4;
+        const float* sr = st + row * SPB + c4;
+        const u32x2 w = {pack_bf16(sr[0], sr[1]), pack_bf16(sr[2], sr[3])};
+        u32x2* dh = reinterpret_cast<u32x2*>(reinterpret_cast<__bf16*>(g.C) + (int64_t)(mb + row) * g.ldc + n0 + wn + c4);
+        if (g.stream_out) __builtin_nontemporal_store(w, dh);
+        else *dh = w;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    return;
+  }
   if (full) {
     __syncthreads();  // every wave is done reading the operand tiles
     float* stage = reinterpret_cast<float*>(smem) + wave * (32 * STAGE_PITCH);
@@ -200,13 +247,6 @@ __global__ __launch_bounds__(256, 3) void k_gemm(Gemm g, int kc) {
           const int row = q * 8 + (lane >> 3), c4 = (lane & 7) * 4;
           const float* sr = stage + row * STAGE_PITCH + c4;
           float4 v = make_float4(sr[0], sr[1], sr[2], sr[3]);
-          if (g.out_bf16) {
-            const u32x2 w = {pack_bf16(v.x, v.y), pack_bf16(v.z, v.w)};
-            u32x2* dh = reinterpret_cast<u32x2*>(reinterpret_cast<__bf16*>(g.C) + (int64_t)(mb + row) * g.ldc + nb + c4);
-            if (g.stream_out) __builtin_nontemporal_store(w, dh);
-            else *dh = w;
-            continue;
-          }
           float4* dst = reinterpret_cast<float4*>(g.C + (int64_t)(mb + row) * g.ldc + nb + c4);
           if (g.accumulate) {
             const float4 o = *dst;
