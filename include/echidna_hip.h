@@ -508,7 +508,9 @@ int32_t ehip_cv_make_tuples_dev(ehip_ctx* ctx, const ehip_tape* tape, uint64_t n
 /* Batched STDE with the learned control variate (cv may be null: plain):
  * S device-drawn directions per point (1 Rademacher, 2 Gaussian), results
  * [npoints][5] as ehip_laplacian_batch_dev; d_plain_results (optional)
- * receives the plain estimator's statistics of the same samples. */
+ * receives the plain estimator's statistics of the same samples.  The pass
+ * holds L(x) in bfloat16 (the network's FP32 output rounded to nearest
+ * even) and forms both v^T L L^T v and tr(L L^T) from those values. */
 int32_t ehip_laplacian_cv_batch_dev(ehip_ctx* ctx, const ehip_tape* tape, ehip_cv* cv, const void* d_x,
                                     uint64_t npoints, uint32_t S, uint32_t dir_kind, uint64_t seed,
                                     double* d_results, double* d_plain_results, void* stream);
