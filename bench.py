@@ -52,7 +52,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12
 FP32_PEAK = 157.3e12
 FP64_PEAK = 78.6e12
-# host threads for the CPU baseline: the box's CPU share for one GPU
+# host threads for the CPU baseline: the host's CPU share for one GPU
 CPU_POOL_MAX = 16
 
 
@@ -157,7 +157,7 @@ def cpu_rates(make_fn, single_units: int, pool_units: int, unit: str, kind_sampl
     return {"value": pool, "unit": unit, "cores": threads, "kind": "port", "single_thread_value": one,
             "statistic": "median of 5 warm runs", **info,
             "sample": f"{kind_sample}: {single_units} per run on 1 thread, {pool_units} per run on {threads} "
-                      f"threads (the box's CPU share for one GPU, of nproc {info['nproc']})"}
+                      f"threads (the host's CPU share for one GPU, of nproc {info['nproc']})"}
 
 
 def fwdgrad_error(tape, x: np.ndarray, y: np.ndarray, g: np.ndarray, f64: bool) -> dict:

@@ -22,6 +22,6 @@ def hip():
 
     ctx = HipContext.new(0)
     if ctx is None:
-        pytest.fail("no gfx950 device visible: GPU tests must run on the MI355X box")
+        pytest.fail("no gfx950 device visible: GPU tests need an MI355X (gfx950)")
     yield ctx
     ctx.close()
