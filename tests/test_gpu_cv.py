@@ -234,15 +234,18 @@ def test_bf16_rounding_helper():
 
 
 @pytest.mark.parametrize("f64", [False, True], ids=["f32", "f64"])
-@pytest.mark.parametrize("n,rank,npos,S", [(40, 12, 5, 20), (64, 16, 8, 16), (36, 24, 24, 37), (33, 32, 10, 3)])
-def test_controlled_statistics_match_a_host_restatement(hip, f64, n, rank, npos, S):
+@pytest.mark.parametrize("n,rank,npos,S,P", [(40, 12, 5, 20, 10), (64, 16, 8, 16, 10), (36, 24, 24, 37, 10),
+                                             (33, 32, 10, 3, 10), (64, 16, 8, 4, 128), (512, 16, 8, 2, 1024)])
+def test_controlled_statistics_match_a_host_restatement(hip, f64, n, rank, npos, S, P):
     """Per-point statistics of the controlled samples 2 c2 - (v^T L S L^T v -
     tr(L S L^T)) against numpy in binary64 from the same pieces: the jets of
     the same points and Philox directions (one GPU evaluation), L(x) from the
     network's forward pass rounded to bfloat16 (as the pass stores it), the
     Welford mean and sample variance.  Shapes
     straddle the 16-sample x 16-column matrix tile and the 16-coordinate slab
-    (partial tiles, two column blocks, n not a multiple of 4, S below 4).
+    (partial tiles, two column blocks, n not a multiple of 4, S below 4);
+    P = 128 (n = 64) and 1024 (n = 512) make L's GEMM store whole 64 x 64
+    and 128 x 128 tiles through its bfloat16 row staging.
     Bar: the two sides sum the same binary64 products in different orders,
     1e-11 of the samples' magnitude for the mean, 1e-9 relative for the
     variance, widened for f32 tapes by 1e-6 of the jets' part in case the
@@ -252,7 +255,7 @@ def test_controlled_statistics_match_a_host_restatement(hip, f64, n, rank, npos,
     bufs = hip.upload_tape(GpuTapeData.from_tape_f64(tape) if f64 else GpuTapeData.from_tape(tape))
     net = lcv.LearnedControlVariate(hip, n, rank, 32, seed=n, out_scale=1.0)
     net.set_signs(npos)
-    P, seed = 10, 13
+    seed = 13
     dt = np.float64 if f64 else np.float32
     tdt = torch.float64 if f64 else torch.float32
     X = wl.points_normal(31, P, n).astype(dt)
