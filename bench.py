@@ -91,6 +91,8 @@ def parse():
     ap.add_argument("--points", type=int, default=0, help="override the workload's point count")
     ap.add_argument("--tuples-log2", type=int, default=20, help="cv: training tuples = 2^k")
     ap.add_argument("--epochs", type=int, default=10, help="cv: training epochs")
+    ap.add_argument("--cv-reps", type=int, default=5,
+                    help="cv: timed passes of each estimator after training (the best is kept for both)")
     ap.add_argument("--dag-scale", type=float, default=1.0, help="dags: shrink lengths and point counts")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "echidna_amd", "tune_cache.json"),
                     help="persisted autotuner shapes to restore (keyed by tape, dtype, evaluation, batch)")
@@ -616,8 +618,13 @@ def run_cv(args, compact: bool = False) -> dict:
             schedule="cosine")
     t_fit = time.perf_counter() - t0 - t_tuples  # fit regenerates the tuples
     log(f"trained in {t_fit:.2f} s, validation variance ratio {est.variance_ratio:.4f}")
+    # both estimators timed after training, in the same clock state, best of
+    # the same number of passes
+    reps = max(1, args.cv_reps)
+    dt_plain, _ = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, lcv.DIR_GAUSSIAN, 5), reps)
     dt_cv, both = timed(lambda: lcv.laplacian_batch(ctx, bufs, X, S, lcv.DIR_GAUSSIAN, 5, est.cv, with_plain=True),
-                        1)
+                        reps)
+    log(f"plain pass {dt_plain * 1e3:.2f} ms, pass with the control variate {dt_cv * 1e3:.2f} ms (best of {reps})")
     res_cv, res_pl = both
     t_s_plain = dt_plain / (P * S)
     t_s_cv = dt_cv / (P * S)
@@ -641,7 +648,8 @@ def run_cv(args, compact: bool = False) -> dict:
            "jet_dense": "exact per-term" if args.no_jet_mma else "fp32 mfma",
            "config": {"workload": f"tanh-MLP tape n={n} 2x{H} seed 4 ({tape.counts()['N']} computed entries), "
                                   f"B={P} points, S={S} Gaussian, CV {n}->256->256->{n}x{r} signed {r // 2}+{r // 2}",
-                      "points": P, "S": S, "rank": r, "tuples": T, "epochs": args.epochs, "launch": tuned},
+                      "points": P, "S": S, "rank": r, "tuples": T, "epochs": args.epochs, "launch": tuned,
+                      "timing": f"best of {max(1, args.cv_reps)} passes each, both after training"},
            "plain": {**rate_plain, "seconds": dt_plain, "t_sample_ns": t_s_plain * 1e9,
                      "jets_per_s": P * S / dt_plain, "mean_var": float(np.mean(res_pl[:, 2]))},
            "learned_cv": {**rate_cv, "seconds": dt_cv, "t_sample_ns": t_s_cv * 1e9,
